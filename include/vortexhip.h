@@ -1,0 +1,149 @@
+/*
+ * vortexhip: the C ABI of the MI355X (gfx950) Galerkin hot path of VortexPanel.jl.
+ *
+ * Plain C: pointers and sizes only.  All indices are 0-based (the Julia binding subtracts 1).  Every call returns
+ * 0 on success or a negative status (VP_E_*); vp_last_error(ctx) gives the message.  One call at a time per context
+ * (not reentrant); a context owns one HIP stream on one device.
+ */
+#ifndef VORTEXHIP_H
+#define VORTEXHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    VP_OK = 0,
+    VP_E_ARGUMENT = -1, /* bad option or input: the reference throws ArgumentError */
+    VP_E_DIMENSION = -2, /* array sizes disagree: the reference throws DimensionMismatch */
+    VP_E_DEVICE = -3,   /* HIP runtime error (allocation, launch) */
+    VP_E_SINGULAR = -4  /* zero pivot in the factorization: the reference's LAPACK getrf reports info > 0 */
+};
+
+typedef struct vp_ctx vp_ctx;
+typedef struct vp_plan vp_plan;
+typedef struct vp_factor vp_factor;
+
+/* AssemblyOptions (src/assembly/assemble.jl:26-41) plus the precision of the path.
+ * test: 0 hat, 1 onering, 2 dual.  precision: 0 plain FP64, 1 mixed (FP32 multipole far field).
+ * `threaded` has no counterpart (the device is always parallel); `external` is not crossable (a Julia closure). */
+typedef struct vp_assembly_options {
+    int test;
+    double far_ratio;
+    int far_points;
+    int near_points;
+    double near_ratio;
+    int near_depth;
+    int touch_levels;
+    double touch_ratio;
+    int touch_order;
+    double corner_angle;
+    double contact_rel;
+    double kernel_far;
+    int precision;
+} vp_assembly_options;
+
+/* The SurfaceGeometry and DOFMapping arrays assemble reads (src/geometry/types.jl:174-199,
+ * src/dofmap/types.jl:73-93): nodes x (nn x 3, row-major), triangles (nt x 3), corner slots (nt x 3), nslot, and the
+ * geometry's touch pairs (ntouch x 2). */
+typedef struct vp_surface {
+    int nn;
+    const double *x;
+    int nt;
+    const int *tris;
+    const int *corner_slot;
+    int nslot;
+    int ntouch;
+    const int *touch;
+} vp_surface;
+
+/* RigidWake (src/wake/rigid.jl:53-62): nodes (nw x 3), triangles (ntri x 3), filaments (nfil x 2), 0-based. */
+typedef struct vp_wake {
+    int nw;
+    const double *x;
+    int ntri;
+    const int *tris;
+    int nfil;
+    const int *filaments;
+} vp_wake;
+
+/* ---- context ---------------------------------------------------------------------------------------------- */
+int vp_create(int device, vp_ctx **out);
+void vp_destroy(vp_ctx *ctx);
+const char *vp_last_error(const vp_ctx *ctx);
+/* Wait for the context's stream. */
+int vp_sync(vp_ctx *ctx);
+
+/* ---- assembly (src/assembly/assemble.jl:377-447) ------------------------------------------------------------ */
+
+/* One-shot `assemble(g, dm, els, sigma, Vinf; opts, wake)`: A is written column-major, nslot x (nslot + nw)
+ * (A[i + j*nslot], row = test slot, column = source slot then wake node), b has nslot entries; pairs/points per
+ * tier in the reference's TIER_NAMES order (self, edge, vertex, contact, near, far).  wake may be NULL. */
+int vp_assemble(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const double *sigma, const double *vinf,
+                const vp_assembly_options *opts, double *A, double *b, long long *pairs, long long *points);
+
+/* The same in steps, with the matrix kept in device memory: a plan holds the geometry, element records, tiles and
+ * rule tables; vp_plan_assemble fills the plan's A and b (on the device) for new sigma and Vinf. */
+int vp_plan_create(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const vp_assembly_options *opts,
+                   vp_plan **out);
+/* flags: VP_PLAN_RECORD_TIERS keeps the tier of every pair of the last assembly (see vp_plan_tiers). */
+enum { VP_PLAN_RECORD_TIERS = 1 };
+int vp_plan_create_ex(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const vp_assembly_options *opts,
+                      int flags, vp_plan **out);
+int vp_plan_assemble(vp_plan *plan, const double *sigma, const double *vinf, long long *pairs, long long *points);
+int vp_plan_read(vp_plan *plan, double *A, double *b);
+/* Tier of every (body test triangle, source triangle) pair (nt x (nt + wake triangles), uint8, TIER_NAMES index)
+ * of the last vp_plan_assemble; the plan must have been created with VP_PLAN_RECORD_TIERS. */
+int vp_plan_tiers(vp_plan *plan, uint8_t *tiers);
+/* Device pointers of the plan's A (column-major, leading dimension nslot) and b, for chaining device calls. */
+int vp_plan_device_A(vp_plan *plan, double **A, double **b, long long *lda);
+void vp_plan_destroy(vp_plan *plan);
+
+/* ---- bordered solve (src/assembly/solve.jl:15-99) ---------------------------------------------------------- */
+
+/* The DOFMapping pieces of the bordered system: C (nslot x nq) and the wake rows W (nw x nq, `wake_rows`,
+ * src/dofmap/system.jl:17) as CSR, and per body the unscaled border vectors e and c (nb x nq each, row-major,
+ * `bordered_rows`, src/dofmap/system.jl:41-50).  W may be NULL when nw == 0. */
+typedef struct vp_constraints {
+    int nslot;
+    int nq;
+    int nw;
+    int nb;
+    const int *C_rowptr;
+    const int *C_col;
+    const double *C_val;
+    const int *W_rowptr;
+    const int *W_col;
+    const double *W_val;
+    const double *e;
+    const double *c;
+} vp_constraints;
+
+/* `factorize_galerkin(dm, A; W)` (solve.jl:81-87): forms M = [Cᵀ A [C; W]  se e; sc cᵀ  0] on the device (A is
+ * column-major nslot x (nslot + nw); on the device when A_on_device, e.g. from vp_plan_device_A) and factors it.
+ * mode 0: plain FP64 (randomized FP64 LU on FP64 MFMA); mode 1: mixed (FP32 LU on FP32 MFMA).  Both refine in FP64
+ * against the FP64 M.  The matrix size is n = nq + nb. */
+int vp_factorize(vp_ctx *ctx, const double *A, long long lda, int A_on_device, const vp_constraints *k, int mode,
+                 vp_factor **out);
+/* Solve M X = R for nrhs right hand sides (`solve_galerkin(F, b)`, solve.jl:95-99): R and X are n x nrhs
+ * row-major host arrays (R[i*nrhs + r]; the caller forms [Cᵀ b; 0]).  iters: refinement steps taken; resid: the
+ * largest |R - M X|_inf / (|M|_inf |X|_inf). */
+int vp_factor_solve(vp_factor *f, const double *R, int nrhs, double *X, int *iters, double *resid);
+/* 1-norm condition number estimate of M (1 / gecon's rcond, solve.jl:85). */
+int vp_factor_cond(vp_factor *f, double *cond);
+/* The border column scales se (nb values, solve.jl:33) and n. */
+int vp_factor_info(vp_factor *f, int *n, double *se);
+void vp_factor_destroy(vp_factor *f);
+
+/* Test entry points (host arrays, row-major): C = A B (sub = 0) or C -= A B (sub = 1) with the library's MFMA GEMM
+ * (prec 0 FP64, 1 FP32; A M x K, B K x N, C M x N, all doubles on the host); and x = M^-1 r for a dense n x n
+ * matrix through the randomized LU (mode 0 FP64, 1 FP32 factors) with FP64 refinement. */
+int vp_gemm(vp_ctx *ctx, int prec, int sub, int M, int N, int K, const double *A, const double *B, double *C);
+int vp_dense_solve(vp_ctx *ctx, int n, const double *M, int mode, const double *r, double *x, int *iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

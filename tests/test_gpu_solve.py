@@ -1,0 +1,102 @@
+"""GPU bordered solve (randomized LU on MFMA + FP64 refinement) against the oracle's LAPACK solve
+(src/assembly/solve.jl:15-99), and the MFMA GEMM against numpy."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vphip import api, dofmap, mesh, solve, wake as wk
+
+pytestmark = pytest.mark.gpu
+
+VG = np.array([1.0, 0.3, -0.2]) / np.linalg.norm([1.0, 0.3, -0.2])
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("shape", [(16, 16, 4), (64, 64, 64), (130, 70, 33), (300, 1, 257), (17, 200, 64)])
+def test_gemm(prec, shape):
+    M, N, K = shape
+    print(f"gemm prec={prec} M={M} N={N} K={K}", flush=True)
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    A, B, Cm = rng.standard_normal((M, K)), rng.standard_normal((K, N)), rng.standard_normal((M, N))
+    if prec == 1:
+        A, B, Cm = (x.astype(np.float32).astype(np.float64) for x in (A, B, Cm))
+    ctx = api.default_context()
+    for sub in (0, 1):
+        out = np.ascontiguousarray(Cm.copy())
+        rc = ctx.lib.vp_gemm(ctx.h, prec, sub, M, N, K, A.ctypes.data_as(C.POINTER(C.c_double)),
+                             B.ctypes.data_as(C.POINTER(C.c_double)), out.ctypes.data_as(C.POINTER(C.c_double)))
+        assert rc == 0
+        ref = (Cm - A @ B) if sub else A @ B
+        err = np.abs(out - ref).max() / np.abs(ref).max()
+        assert err < (1e-13 if prec == 0 else 2e-6), (sub, err)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("n", [5, 64, 300, 700])
+def test_dense_solve(mode, n):
+    print(f"dense solve mode={mode} n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    M = rng.standard_normal((n, n)) + n ** 0.5 * np.eye(n)
+    r = rng.standard_normal(n)
+    x = np.zeros(n)
+    it = C.c_int()
+    ctx = api.default_context()
+    rc = ctx.lib.vp_dense_solve(ctx.h, n, np.ascontiguousarray(M).ctypes.data_as(C.POINTER(C.c_double)), mode,
+                                r.ctypes.data_as(C.POINTER(C.c_double)), x.ctypes.data_as(C.POINTER(C.c_double)),
+                                C.byref(it))
+    assert rc == 0
+    xs = np.linalg.solve(M, r)
+    assert np.abs(x - xs).max() / np.abs(xs).max() < 1e-12, (np.abs(x - xs).max(), it.value)
+
+
+def _case(g, dm, V, w=None):
+    els = api.build_elements(g)
+    sig = api.source_strengths(els, V)
+    A, b, _ = api.assemble(g, dm, els, sig, V, wake=w)
+    W = None if w is None else dofmap.wake_rows(dm, w.te)
+    return A, b, W
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_sphere_solve(mode):
+    g = mesh.icosphere(3)
+    dm = dofmap.build_dofmap(g)
+    A, b, _ = _case(g, dm, VG)
+    print(f"sphere solve mode={mode}", flush=True)
+    sol = solve.solve_galerkin(dm, A, b, mode=mode)
+    ref = O.solve_galerkin(dm, A, b)
+    e = np.max(np.abs(sol["mu"] - ref["mu"])) / np.max(np.abs(ref["mu"]))
+    assert e < 1e-11, e
+    assert abs(sol["lam"][0] - ref["lam"][0]) < 1e-14
+    assert abs(sol["cond"] / ref["cond"] - 1) < 0.05, (sol["cond"], ref["cond"])
+    # lambda takes up an incompatible right hand side (test/assembly/test_sphere.jl:50-51)
+    sh = solve.solve_galerkin(dm, A, b + 1e-3, mode=mode, cond=False)
+    assert abs(sh["lam"][0] - 1e-3) < 1e-9
+
+
+def test_wing_solve_and_batch():
+    g = mesh.naca_wing(mesh.WingSpec(code="0012", n_around=24, n_span=8, aspect=6.0))
+    dm = dofmap.build_dofmap(g)
+    V = np.array([1.0, 0.0, 0.0])
+    w = wk.build_wake(g, dm, V, wk.WakeOptions(length=2.0))
+    A, b, W = _case(g, dm, V, w)
+    print("wing solve", flush=True)
+    for mode in ("fp64", "mixed"):
+        F = solve.factorize_galerkin(dm, A, W=W, mode=mode)
+        # batch of angles: b is linear in Vinf with a fixed (body-fixed) wake
+        els = api.build_elements(g)
+        angs = np.radians(np.linspace(-4, 12, 9))
+        Bs = []
+        for a in angs:
+            Va = np.array([np.cos(a), 0, np.sin(a)])
+            _, ba, _ = api.assemble(g, dm, els, api.source_strengths(els, Va), Va, wake=w)
+            Bs.append(ba)
+        Bm = np.stack(Bs, axis=1)
+        out = solve.solve_galerkin(F, Bm)
+        for j in range(len(angs)):
+            ref = O.solve_galerkin(dm, A, Bm[:, j], W)
+            e = np.max(np.abs(out["mu"][:, j] - ref["mu"])) / np.max(np.abs(ref["mu"]))
+            assert e < 1e-10, (mode, j, e)
+        F.close()

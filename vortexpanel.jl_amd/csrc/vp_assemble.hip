@@ -1,0 +1,1149 @@
+// Galerkin pair assembly on gfx950: A_ij = int psi_i n.u[phi_j] dA and b_i = -int psi_i n.(Vinf + u_sigma) dA.
+//
+// Reference: src/assembly/assemble.jl:245-447 (pair_block, pair_rule, wake_pair_rule, segment_rule, segment_block,
+// add_wake!, assemble), src/assembly/assemble.jl:69-238 (near, vertex and contact rules).
+//
+// Kernels (one stream per plan):
+//   elem_setup     FP64 element records (centroid, diameter, reach ... in the reference's operation order)
+//   test_setup     the far rule's points and test weights on every body test triangle
+//   far_pairs<T>   tiled test x source pairs: lane = source element, wave = one test triangle at a time; every pair
+//                  is classified in FP64 (bit-exact with the oracle); far pairs evaluate their 3-point block in the
+//                  compute type T and reduce into an LDS tile R[test slot][source column], flushed to A with one
+//                  atomic per nonzero tile entry; the other pairs are appended to the near list
+//   seg_pairs      body test triangle x tip filament, far ones evaluated here, the others appended
+//   near_pairs     one wavefront per near/touching pair: the wave builds the pair's rule pieces on a small LDS
+//                  stack (self, edge, vertex with wide-corner split, contact fan splits, recursive near split) and
+//                  evaluates every piece's points in FP64 across its 64 lanes
+//   freestream     b_i -= (Vinf.n) A int psi_i / A
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "vp_capi_internal.hpp"
+#include "vp_device.hpp"
+#include "vp_rules.hpp"
+
+namespace vp {
+
+// ------------------------------------------------------------------------------------------ device constants -------
+
+constexpr int WAVE = 64;
+constexpr int TT_MAX = 128;  // test triangles per far tile
+constexpr int TS = 64;       // source elements per far tile (one per lane)
+constexpr int VMAX = 96;     // distinct test slots per far tile
+constexpr int UMAX = 64;     // distinct source columns per far tile
+constexpr int STACK = 48;    // rule-piece stack entries per wave in near_pairs
+
+enum Tier { T_SELF = 0, T_EDGE, T_VERTEX, T_CONTACT, T_NEAR, T_FAR };
+
+struct TestFar {  // one per body test triangle: the far rule's points, test weights x W, normal, centroid, diam
+    double x[3][3];
+    double wpsi[3][3];  // [q][a]
+    double n[3];
+    double c[3];
+    double diam, area;
+};
+
+struct AsmParams {
+    int nt, nsrc, nseg, nslot, ncol;  // body triangles, triangle sources (body + wake), filaments, rows, columns
+    int nn;                           // body nodes
+    int test;                         // 0 hat, 1 onering, 2 dual
+    double far_ratio, near_ratio, corner_angle, contact_rel, kernel_far;
+    int near_depth;
+    long long lda;
+};
+
+// ------------------------------------------------------------------------------------------ setup kernels ----------
+
+// FlatTriangle + SourceTriangle, src/elements/flat_triangle.jl:48-55 and src/kernels/source_triangle.jl:21-28,
+// with reach (flat_triangle.jl:56).  Contraction off: centroid, diameter and reach feed the bit-exact tiers.
+__global__ void elem_setup(const double *__restrict__ x, const int *__restrict__ tris, int ne, Elem *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    V3<double> a = ld3(x + 3 * tris[3 * e]), b = ld3(x + 3 * tris[3 * e + 1]), c = ld3(x + 3 * tris[3 * e + 2]);
+    V3<double> nA = cross(b - a, c - a);
+    double A2 = cls_norm(nA);
+    V3<double> n = mk3(nA.x / A2, nA.y / A2, nA.z / A2);
+    double len[3] = {cls_norm(b - a), cls_norm(c - b), cls_norm(a - c)};
+    V3<double> ed[3] = {b - a, c - b, a - c};
+    Elem E;
+    V3<double> v[3] = {a, b, c};
+    for (int k = 0; k < 3; k++) {
+        E.v[k][0] = v[k].x;
+        E.v[k][1] = v[k].y;
+        E.v[k][2] = v[k].z;
+        V3<double> nu = cross(ed[k], n);
+        E.nu[k][0] = nu.x / len[k];
+        E.nu[k][1] = nu.y / len[k];
+        E.nu[k][2] = nu.z / len[k];
+        E.len[k] = len[k];
+    }
+    E.n[0] = n.x;
+    E.n[1] = n.y;
+    E.n[2] = n.z;
+    V3<double> s = (a + b) + c;
+    V3<double> cc = mk3(s.x / 3.0, s.y / 3.0, s.z / 3.0);
+    E.c[0] = cc.x;
+    E.c[1] = cc.y;
+    E.c[2] = cc.z;
+    E.area = A2 / 2;
+    E.diam = fmax(fmax(len[0], len[1]), len[2]);
+    E.reach = fmax(fmax(cls_norm(a - cc), cls_norm(b - cc)), cls_norm(c - cc));
+    V3<double> eo[3] = {c - b, a - c, b - a};  // the edge opposite each corner
+    for (int k = 0; k < 3; k++) {
+        E.g[k][0] = -eo[k].x / A2;
+        E.g[k][1] = -eo[k].y / A2;
+        E.g[k][2] = -eo[k].z / A2;
+    }
+    E.pad0 = A2 > 0 ? 0.0 : 1.0;  // degenerate flag, checked by the host
+    out[e] = E;
+}
+
+// test_weights, src/assembly/assemble.jl:52-58
+__device__ __forceinline__ void test_weights(int test, const double *l, double *psi)
+{
+    if (test == 0) {
+        psi[0] = l[0];
+        psi[1] = l[1];
+        psi[2] = l[2];
+    } else if (test == 1) {
+        psi[0] = psi[1] = psi[2] = 1.0;
+    } else {
+        double m = fmax(fmax(l[0], l[1]), l[2]);
+        int is[3], c = 0;
+        for (int k = 0; k < 3; k++) {
+            is[k] = fabs(l[k] - m) < 1e-12;
+            c += is[k];
+        }
+        for (int k = 0; k < 3; k++) psi[k] = (double)is[k] / c;
+    }
+}
+
+__global__ void test_setup(const Elem *__restrict__ el, int nt, RuleTab rt, int test, TestFar *__restrict__ out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const Elem &E = el[t];
+    V3<double> X[3] = {ld3(E.v[0]), ld3(E.v[1]), ld3(E.v[2])};
+    TestFar F;
+    for (int q = 0; q < 3; q++) {
+        const double *l = rt.l + 3 * (rt.off[R_FAR] + q);
+        V3<double> p = cls_point(X, l);
+        F.x[q][0] = p.x;
+        F.x[q][1] = p.y;
+        F.x[q][2] = p.z;
+        double psi[3];
+        test_weights(test, l, psi);
+        double W = rt.w[rt.off[R_FAR] + q] * E.area;
+        for (int a = 0; a < 3; a++) F.wpsi[q][a] = W * psi[a];
+    }
+    for (int k = 0; k < 3; k++) {
+        F.n[k] = E.n[k];
+        F.c[k] = E.c[k];
+    }
+    F.diam = E.diam;
+    F.area = E.area;
+    out[t] = F;
+}
+
+// ------------------------------------------------------------------------------------------ far pairs --------------
+
+// Wave-aggregated append of (t, s) to the near list.
+__device__ __forceinline__ void append_pair(bool want, int t, int s, int2 *list, unsigned int *count, unsigned int cap)
+{
+    unsigned long long mask = __ballot(want);
+    if (!mask) return;
+    int lane = threadIdx.x & (WAVE - 1);
+    unsigned int base = 0;
+    int leader = __ffsll((long long)mask) - 1;
+    if (lane == leader) base = atomicAdd(count, (unsigned int)__popcll(mask));
+    base = __shfl(base, leader);
+    if (want) {
+        unsigned int pos = base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+        if (pos < cap) list[pos] = make_int2(t, s);
+    }
+}
+
+template <typename T> struct FarSrc {  // per-lane source element data for the far tier
+    V3<double> c;                      // centroid (FP64: the classification and the relative offsets)
+    double diam;
+    SrcT<T> s;                         // vertex offsets from the centroid, normal, edges ... in the compute type
+    V3<T> g[3];
+    T sigma;
+};
+
+template <typename T> __device__ __forceinline__ FarSrc<T> load_far_src(const Elem &E, double sigma)
+{
+    FarSrc<T> f;
+    f.c = ld3(E.c);
+    f.diam = E.diam;
+    for (int k = 0; k < 3; k++) {
+        f.s.v[k] = mk3(T(E.v[k][0] - E.c[0]), T(E.v[k][1] - E.c[1]), T(E.v[k][2] - E.c[2]));
+        f.s.nu[k] = mk3(T(E.nu[k][0]), T(E.nu[k][1]), T(E.nu[k][2]));
+        f.s.len[k] = T(E.len[k]);
+        f.g[k] = mk3(T(E.g[k][0]), T(E.g[k][1]), T(E.g[k][2]));
+    }
+    f.s.n = mk3(T(E.n[0]), T(E.n[1]), T(E.n[2]));
+    f.s.diam = T(E.diam);
+    f.s.area = T(E.area);
+    f.sigma = T(sigma);
+    return f;
+}
+
+// One kernel evaluation of source_velocity (src/kernels/source_triangle.jl:134-138) at x (FP64) for the far tier:
+// the multipole beyond kernel_far diameters in T, the closed form in FP64 otherwise (its edge-log sum cancels
+// like r/diam, which single precision cannot carry).
+template <typename T>
+__device__ __forceinline__ V3<double> far_eval(const FarSrc<T> &f, const Elem *__restrict__ E, V3<double> x, double kfar2)
+{
+    V3<double> r = x - f.c;
+    if (cls_dot(r, r) > kfar2) {
+        V3<T> w = sv_far_rel(f.s, mk3(T(r.x), T(r.y), T(r.z)));
+        return mk3((double)w.x, (double)w.y, (double)w.z);
+    }
+    SrcT<double> s;
+    for (int k = 0; k < 3; k++) {
+        s.v[k] = ld3(E->v[k]) - f.c;
+        s.nu[k] = ld3(E->nu[k]);
+        s.len[k] = E->len[k];
+    }
+    s.n = ld3(E->n);
+    s.diam = E->diam;
+    s.area = E->area;
+    V3<double> ra = x - ld3(E->v[0]), rb = x - ld3(E->v[1]), rc = x - ld3(E->v[2]);
+    return sv_exact_abs(s, ra, rb, rc, dot(r, s.n), 0, nullptr, x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__restrict__ tf, const Elem *__restrict__ el,
+                                                 const double *__restrict__ sigma, const int *__restrict__ tt_elems,
+                                                 const uint8_t *__restrict__ tt_loc, const int *__restrict__ tt_rows,
+                                                 const int *__restrict__ tt_nrow, const int *__restrict__ ts_elems,
+                                                 const uint8_t *__restrict__ ts_loc, const int *__restrict__ ts_cols,
+                                                 const int *__restrict__ ts_ncol, double *__restrict__ A,
+                                                 double *__restrict__ bvec, int2 *__restrict__ near_list,
+                                                 unsigned int *__restrict__ near_count, unsigned int near_cap,
+                                                 unsigned long long *__restrict__ far_count, uint8_t *__restrict__ tiers)
+{
+    __shared__ double R[VMAX * UMAX];
+    __shared__ double rbacc[VMAX];
+    __shared__ unsigned int nfar;
+    const int tt = blockIdx.x, st = blockIdx.y;
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < VMAX * UMAX; i += blockDim.x) R[i] = 0.0;
+    if (threadIdx.x < VMAX) rbacc[threadIdx.x] = 0.0;
+    if (threadIdx.x == 0) nfar = 0;
+    const int s = ts_elems[st * TS + lane];
+    const bool valid = s >= 0;
+    FarSrc<T> f;
+    int uloc[3] = {0, 0, 0};
+    if (valid) {
+        f = load_far_src<T>(el[s], s < P.nt ? sigma[s] : 0.0);
+        for (int k = 0; k < 3; k++) uloc[k] = ts_loc[(st * TS + lane) * 3 + k];
+    }
+    const double kfar2 = (P.kernel_far * (valid ? f.diam : 0.0)) * (P.kernel_far * (valid ? f.diam : 0.0));
+    __syncthreads();
+    unsigned int myfar = 0;
+    for (int i = wave; i < TT_MAX; i += 4) {
+        const int t = tt_elems[tt * TT_MAX + i];
+        if (t < 0) break;
+        const TestFar &F = tf[t];
+        const int v0 = tt_loc[(tt * TT_MAX + i) * 3], v1 = tt_loc[(tt * TT_MAX + i) * 3 + 1], v2 = tt_loc[(tt * TT_MAX + i) * 3 + 2];
+        bool far = false;
+        if (valid && s != t) {
+            // pair_rule's far test, src/assembly/assemble.jl:269 (bit-exact: FP64, no contraction)
+            V3<double> d = f.c - ld3(F.c);
+            far = cls_norm(d) > P.far_ratio * fmax(F.diam, f.diam);
+        }
+        append_pair(valid && !far, t, s, near_list, near_count, near_cap);
+        if (tiers && valid && far) tiers[(size_t)t * P.nsrc + s] = T_FAR;
+        if (far) {
+            myfar++;
+            V3<double> n = ld3(F.n);
+            double blk[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, rb[3] = {0, 0, 0};
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                V3<double> w = far_eval<T>(f, el + s, ld3(F.x[q]), kfar2);
+                V3<double> vv = cross(w, n);  // n.(gamma x w) = gamma.(w x n), assemble.jl:251
+                double c0 = dot(vv, mk3((double)f.g[0].x, (double)f.g[0].y, (double)f.g[0].z));
+                double c1 = dot(vv, mk3((double)f.g[1].x, (double)f.g[1].y, (double)f.g[1].z));
+                double c2 = dot(vv, mk3((double)f.g[2].x, (double)f.g[2].y, (double)f.g[2].z));
+                double wn = (double)f.sigma * dot(w, n);
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    double psi = F.wpsi[q][a];
+                    blk[a][0] += psi * c0;
+                    blk[a][1] += psi * c1;
+                    blk[a][2] += psi * c2;
+                    rb[a] -= wn * psi;
+                }
+            }
+            const int vl[3] = {v0, v1, v2};
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) atomicAdd(&R[vl[a] * UMAX + uloc[b]], blk[a][b]);
+#pragma unroll
+            for (int a = 0; a < 3; a++) atomicAdd(&rbacc[vl[a]], rb[a]);
+        }
+    }
+    if (myfar) atomicAdd(&nfar, myfar);
+    __syncthreads();
+    const int nrow = tt_nrow[tt], ncol = ts_ncol[st];
+    for (int i = threadIdx.x; i < nrow * UMAX; i += blockDim.x) {
+        int v = i / UMAX, u = i - v * UMAX;
+        double val = R[v * UMAX + u];
+        if (u < ncol && val != 0.0) unsafeAtomicAdd(&A[(long long)ts_cols[st * UMAX + u] * P.lda + tt_rows[tt * VMAX + v]], val);
+    }
+    for (int v = threadIdx.x; v < nrow; v += blockDim.x)
+        if (rbacc[v] != 0.0) unsafeAtomicAdd(&bvec[tt_rows[tt * VMAX + v]], rbacc[v]);
+    if (threadIdx.x == 0 && nfar) atomicAdd(far_count, (unsigned long long)nfar);
+}
+
+// ------------------------------------------------------------------------------------------ filaments --------------
+
+// segment_rule's far test (src/assembly/assemble.jl:299-301) and the far segment_block (:312-321)
+__global__ void seg_pairs(AsmParams P, const TestFar *__restrict__ tf, const int *__restrict__ corner_slot,
+                          const double *__restrict__ seg, const int *__restrict__ segcol, double *__restrict__ A,
+                          int2 *__restrict__ near_list, unsigned int *__restrict__ near_count, unsigned int near_cap,
+                          unsigned long long *__restrict__ far_count)
+{
+#pragma clang fp contract(off)
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool in = idx < (long long)P.nt * P.nseg;
+    int t = in ? (int)(idx / P.nseg) : 0, f = in ? (int)(idx % P.nseg) : 0;
+    bool far = false;
+    V3<double> fa, fb;
+    if (in) {
+        const TestFar &F = tf[t];
+        fa = ld3(seg + 6 * f);
+        fb = ld3(seg + 6 * f + 3);
+        double L = cls_norm(fb - fa);
+        V3<double> mid = fa + fb;
+        mid = mk3(mid.x / 2, mid.y / 2, mid.z / 2);
+        far = cls_norm(mid - ld3(F.c)) > P.far_ratio * fmax(F.diam, L);
+    }
+    append_pair(in && !far, t, P.nsrc + f, near_list, near_count, near_cap);
+    unsigned long long m = __ballot(far);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(far_count, (unsigned long long)__popcll(m));
+    if (!far) return;
+    const TestFar &F = tf[t];
+    V3<double> n = ld3(F.n);
+    double ca[3] = {0, 0, 0}, cb[3] = {0, 0, 0};
+    for (int q = 0; q < 3; q++) {
+        V3<double> x = ld3(F.x[q]), va, vb;
+        seg_vel(x - fa, x - fb, va, vb);
+        double da = dot(va, n), db = dot(vb, n);
+        for (int a = 0; a < 3; a++) {
+            ca[a] += da * F.wpsi[q][a];
+            cb[a] += db * F.wpsi[q][a];
+        }
+    }
+    const int *ct = corner_slot + 3 * t;
+    int ca_col = segcol[2 * f], cb_col = segcol[2 * f + 1];
+    for (int a = 0; a < 3; a++) {
+        unsafeAtomicAdd(&A[(long long)ca_col * P.lda + ct[a]], ca[a]);
+        unsafeAtomicAdd(&A[(long long)cb_col * P.lda + ct[a]], cb[a]);
+    }
+}
+
+// ------------------------------------------------------------------------------------------ near pairs -------------
+
+struct Piece {  // a piece of the test triangle in barycentric corners, with its area fraction
+    double b[3][3];
+    double frac;
+    int kind;   // 0 near (recursive split), 1 contact (fan split)
+    int level, depth, pad;
+};
+
+struct NearCtx {
+    const Elem *T, *S;          // test element, source element (S null for a filament)
+    V3<double> X[3];            // test triangle's corners
+    V3<double> SX[3];           // source triangle's corners
+    V3<double> fa, fb;          // filament ends
+    bool seg;
+    double tol;                 // contact tolerance
+    int side, test;
+    double sigma;
+    RuleTab rt;
+    const AsmParams *P;
+    double acc[12];             // blk (9, row-major [a][b]) + rb (3); filaments: ca (3) + cb (3)
+    long long npts;
+};
+
+__device__ __forceinline__ double src_distance(const NearCtx &c, V3<double> p)
+{
+    return c.seg ? cls_seg_projection(p, c.fa, c.fb, nullptr) : cls_tri_distance(c.SX, p);
+}
+
+// Evaluate base rule `rid` mapped onto the piece b (append_piece!, src/assembly/assemble.jl:85-91) across the wave.
+__device__ void eval_leaf(NearCtx &c, int rid, const double (*b)[3], double frac)
+{
+    const int n = c.rt.n[rid], off = c.rt.off[rid];
+    const int lane = threadIdx.x & (WAVE - 1);
+    c.npts += n;
+    V3<double> tn = ld3(c.T->n);
+    for (int p = lane; p < n; p += WAVE) {
+        const double *m = c.rt.l + 3 * (off + p);
+        double l[3];
+        {
+#pragma clang fp contract(off)
+            for (int j = 0; j < 3; j++) l[j] = (m[0] * b[0][j] + m[1] * b[1][j]) + m[2] * b[2][j];
+        }
+        double W = c.rt.w[off + p] * frac * c.T->area;
+        double psi[3];
+        test_weights(c.test, l, psi);
+        V3<double> x = cls_point(c.X, l);
+        if (c.seg) {
+            V3<double> va, vb;
+            seg_vel(x - c.fa, x - c.fb, va, vb);
+            double da = dot(va, tn), db = dot(vb, tn);
+            for (int a = 0; a < 3; a++) {
+                c.acc[a] += da * (W * psi[a]);
+                c.acc[3 + a] += db * (W * psi[a]);
+            }
+        } else {
+            const Elem &S = *c.S;
+            V3<double> r = x - ld3(S.c);
+            V3<double> w;
+            double fd = c.P->kernel_far * S.diam;
+            if (cls_dot(r, r) > fd * fd) {
+                SrcT<double> s;
+                for (int k = 0; k < 3; k++) s.v[k] = ld3(S.v[k]) - ld3(S.c);
+                s.area = S.area;
+                w = sv_far_rel(s, r);
+            } else {
+                SrcT<double> s;
+                for (int k = 0; k < 3; k++) {
+                    s.nu[k] = ld3(S.nu[k]);
+                    s.len[k] = S.len[k];
+                }
+                s.n = ld3(S.n);
+                s.diam = S.diam;
+                double z;
+                {
+#pragma clang fp contract(off)
+                    z = cls_dot(r, s.n);
+                }
+                w = sv_exact_abs(s, x - ld3(S.v[0]), x - ld3(S.v[1]), x - ld3(S.v[2]), z, c.side, &S, x);
+            }
+            V3<double> v = cross(w, tn);
+            double cc[3] = {dot(v, ld3(S.g[0])), dot(v, ld3(S.g[1])), dot(v, ld3(S.g[2]))};
+            double wn = c.sigma * dot(w, tn);
+            for (int a = 0; a < 3; a++) {
+                double ps = W * psi[a];
+                for (int bb = 0; bb < 3; bb++) c.acc[3 * a + bb] += ps * cc[bb];
+                c.acc[9 + a] -= wn * ps;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void set_corners(double (*b)[3]) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) b[i][j] = i == j ? 1.0 : 0.0;
+}
+
+// vertex_pieces!, src/assembly/assemble.jl:102-116 (k 0-based, p the piece's physical corners)
+__device__ void vertex_pieces(NearCtx &c, const V3<double> *p, int k, const double (*b)[3], double frac)
+{
+#pragma clang fp contract(off)
+    int i = (k + 1) % 3, j = (k + 2) % 3;
+    V3<double> u = p[i] - p[k], v = p[j] - p[k];
+    double nu = cls_norm(u), nv = cls_norm(v);
+    double cs = cls_dot(u, v) / (nu * nv);
+    cs = cs < -1.0 ? -1.0 : cs > 1.0 ? 1.0 : cs;
+    double th = acos(cs);
+    double bb[3][3];
+    if (th <= kPi / 2) {
+        for (int q = 0; q < 3; q++) {
+            bb[0][q] = b[k][q];
+            bb[1][q] = b[i][q];
+            bb[2][q] = b[j][q];
+        }
+        eval_leaf(c, R_VERT0, bb, frac);
+        return;
+    }
+    int m = (int)ceil(th * (180 / kPi) / c.P->corner_angle);
+    double t0 = 0.0;
+    for (int q = 1; q <= m; q++) {
+        double t1 = 1.0;
+        if (q < m) {
+            double ph = th * q / m;
+            t1 = nu * sin(ph) / (nu * sin(ph) + nv * sin(th - ph));
+        }
+        for (int z = 0; z < 3; z++) {
+            bb[0][z] = b[k][z];
+            bb[1][z] = (1 - t0) * b[i][z] + t0 * b[j][z];
+            bb[2][z] = (1 - t1) * b[i][z] + t1 * b[j][z];
+        }
+        eval_leaf(c, R_VERT0, bb, frac * (t1 - t0));
+        t0 = t1;
+    }
+}
+
+// The splits of the piece's edge k at the source triangle's corners (edge_splits, assemble.jl:160-171)
+__device__ int edge_splits_k(const NearCtx &c, const V3<double> *p, int k, double *ts)
+{
+#pragma clang fp contract(off)
+    V3<double> a = p[k], e = p[(k + 1) % 3];
+    double L = cls_norm(e - a);
+    int n = 0;
+    for (int q = 0; q < 3; q++) {
+        double t;
+        double d = cls_seg_projection(c.SX[q], a, e, &t);
+        if (d <= c.tol && t * L > c.tol && (1 - t) * L > c.tol) ts[n++] = t;
+    }
+    for (int i = 1; i < n; i++)
+        for (int j = i; j > 0 && ts[j] < ts[j - 1]; j--) {
+            double x = ts[j];
+            ts[j] = ts[j - 1];
+            ts[j - 1] = x;
+        }
+    return n;
+}
+
+// Walks the rule pieces of one pair (near_pieces!, assemble.jl:131-142; contact_pieces!, :209-238) depth first on
+// the wave's LDS stack, evaluating every leaf as it is produced.  All lanes run the same control flow.
+__device__ void walk(NearCtx &c, Piece *stk, int sp)
+{
+#pragma clang fp contract(off)
+    while (sp > 0) {
+        Piece pc = stk[--sp];
+        V3<double> p[3] = {cls_point(c.X, pc.b[0]), cls_point(c.X, pc.b[1]), cls_point(c.X, pc.b[2])};
+        if (pc.kind == 0) {
+            double size = fmax(fmax(cls_norm(p[1] - p[0]), cls_norm(p[2] - p[1])), cls_norm(p[0] - p[2]));
+            V3<double> cen = (p[0] + p[1]) + p[2];
+            cen = mk3(cen.x / 3.0, cen.y / 3.0, cen.z / 3.0);
+            if (pc.level < pc.depth && src_distance(c, cen) < c.P->near_ratio * size && sp + 4 <= STACK) {
+                double m12[3], m23[3], m31[3];
+                for (int j = 0; j < 3; j++) {
+                    m12[j] = (pc.b[0][j] + pc.b[1][j]) / 2;
+                    m23[j] = (pc.b[1][j] + pc.b[2][j]) / 2;
+                    m31[j] = (pc.b[2][j] + pc.b[0][j]) / 2;
+                }
+                const double *kids[4][3] = {{pc.b[0], m12, m31}, {m12, pc.b[1], m23}, {m31, m23, pc.b[2]}, {m12, m23, m31}};
+                for (int q = 3; q >= 0; q--) {  // pushed in reverse: popped in the reference's order
+                    Piece ch;
+                    for (int i = 0; i < 3; i++)
+                        for (int j = 0; j < 3; j++) ch.b[i][j] = kids[q][i][j];
+                    ch.frac = pc.frac / 4;
+                    ch.kind = 0;
+                    ch.level = pc.level + 1;
+                    ch.depth = pc.depth;
+                    stk[sp++] = ch;
+                }
+            } else {
+                eval_leaf(c, R_NEAR, pc.b, pc.frac);
+            }
+            continue;
+        }
+        // contact piece
+        if (pc.level < 4) {
+            int k = 0, ns = 0;
+            double sp3[3];
+            for (k = 0; k < 3; k++) {
+                ns = edge_splits_k(c, p, k, sp3);
+                if (ns) break;
+            }
+            if (ns && sp + ns + 1 <= STACK) {
+                double ts[5];
+                ts[0] = 0.0;
+                for (int i = 0; i < ns; i++) ts[i + 1] = sp3[i];
+                ts[ns + 1] = 1.0;
+                const double *bk = pc.b[k], *bk1 = pc.b[(k + 1) % 3], *opp = pc.b[(k + 2) % 3];
+                for (int i = ns; i >= 0; i--) {
+                    Piece ch;
+                    for (int z = 0; z < 3; z++) {
+                        ch.b[0][z] = (1 - ts[i]) * bk[z] + ts[i] * bk1[z];
+                        ch.b[1][z] = (1 - ts[i + 1]) * bk[z] + ts[i + 1] * bk1[z];
+                        ch.b[2][z] = opp[z];
+                    }
+                    ch.frac = pc.frac * (ts[i + 1] - ts[i]);
+                    ch.kind = 1;
+                    ch.level = pc.level + 1;
+                    ch.depth = 0;
+                    stk[sp++] = ch;
+                }
+                continue;
+            }
+        }
+        int on[3], nc = 0;
+        for (int k = 0; k < 3; k++) {
+            on[k] = cls_tri_distance(c.SX, p[k]) <= c.tol;
+            nc += on[k];
+        }
+        if (nc == 2) {
+            int k = !on[2] ? 0 : !on[0] ? 1 : 2;
+            V3<double> mid = p[k] + p[(k + 1) % 3];
+            mid = mk3(mid.x / 2, mid.y / 2, mid.z / 2);
+            if (cls_tri_distance(c.SX, mid) <= c.tol) {
+                eval_leaf(c, R_EDGE0 + k, pc.b, pc.frac);
+                continue;
+            }
+        }
+        if (nc == 1) {
+            vertex_pieces(c, p, on[0] ? 0 : on[1] ? 1 : 2, pc.b, pc.frac);
+        } else if (nc == 0) {
+            Piece ch = pc;
+            ch.kind = 0;
+            ch.level = 0;
+            ch.depth = c.P->near_depth;
+            stk[sp++] = ch;
+        } else {
+            eval_leaf(c, R_SELF, pc.b, pc.frac);
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void near_pairs(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+                                                  const int *__restrict__ elem_nodes, const double *__restrict__ sigma,
+                                                  const int *__restrict__ corner_slot, const int *__restrict__ src_cols,
+                                                  const double *__restrict__ seg, const int *__restrict__ segcol,
+                                                  const int2 *__restrict__ list, unsigned int npairs,
+                                                  const int2 *__restrict__ touch, int ntouch, double *__restrict__ A,
+                                                  double *__restrict__ bvec, unsigned long long *__restrict__ stats,
+                                                  uint8_t *__restrict__ tiers)
+{
+    __shared__ Piece stack[4][STACK];
+    __shared__ unsigned long long wstat[12];
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+    if (threadIdx.x < 12) wstat[threadIdx.x] = 0;
+    __syncthreads();
+    Piece *stk = stack[wave];
+    for (unsigned int pi = blockIdx.x * 4 + wave; pi < npairs; pi += gridDim.x * 4) {
+        const int2 pr = list[pi];
+        const int t = pr.x, s = pr.y;
+        NearCtx c;
+        c.P = &P;
+        c.rt = rt;
+        c.T = el + t;
+        c.test = P.test;
+        c.npts = 0;
+        for (int i = 0; i < 12; i++) c.acc[i] = 0.0;
+        for (int k = 0; k < 3; k++) c.X[k] = ld3(c.T->v[k]);
+        c.seg = s >= P.nsrc;
+        int tier = T_NEAR;
+        int sp = 0;
+        double b0[3][3];
+        set_corners(b0);
+        if (c.seg) {
+            // segment_rule, src/assembly/assemble.jl:299-306 (the far ones were taken by seg_pairs)
+            const int f = s - P.nsrc;
+            c.S = nullptr;
+            c.fa = ld3(seg + 6 * f);
+            c.fb = ld3(seg + 6 * f + 3);
+            c.side = 0;
+            c.sigma = 0.0;
+            int kv = -1;
+            {
+#pragma clang fp contract(off)
+                for (int k = 0; k < 3 && kv < 0; k++)
+                    if (fmin(cls_norm(c.X[k] - c.fa), cls_norm(c.X[k] - c.fb)) <= 1e-12 * c.T->diam) kv = k;
+            }
+            if (kv >= 0) {
+                tier = T_VERTEX;
+                V3<double> u = c.X[(kv + 1) % 3] - c.X[kv], v = c.X[(kv + 2) % 3] - c.X[kv];
+                if (cls_dot(u, v) >= 0)
+                    eval_leaf(c, R_VERT0 + kv, b0, 1.0);
+                else
+                    vertex_pieces(c, c.X, kv, b0, 1.0);
+            } else {
+                Piece r0;
+                set_corners(r0.b);
+                r0.frac = 1.0;
+                r0.kind = 0;
+                r0.level = 0;
+                r0.depth = P.near_depth;
+                stk[sp++] = r0;
+                walk(c, stk, sp);
+            }
+            double ca[3], cb[3];
+            for (int a = 0; a < 3; a++) {
+                ca[a] = wave_sum(c.acc[a]);
+                cb[a] = wave_sum(c.acc[3 + a]);
+            }
+            if (lane == 0) {
+                const int *ct = corner_slot + 3 * t;
+                for (int a = 0; a < 3; a++) {
+                    unsafeAtomicAdd(&A[(long long)segcol[2 * f] * P.lda + ct[a]], ca[a]);
+                    unsafeAtomicAdd(&A[(long long)segcol[2 * f + 1] * P.lda + ct[a]], cb[a]);
+                }
+            }
+        } else {
+            c.S = el + s;
+            for (int k = 0; k < 3; k++) c.SX[k] = ld3(c.S->v[k]);
+            c.side = s == t ? 1 : 0;
+            c.sigma = s < P.nt ? sigma[s] : 0.0;
+            if (s == t) {
+                tier = T_SELF;
+                eval_leaf(c, R_SELF, b0, 1.0);
+            } else {
+                // pair_rule after the far test, src/assembly/assemble.jl:270-281
+                int shared[3];
+                for (int k = 0; k < 3; k++) {
+                    int v = elem_nodes[3 * t + k];
+                    shared[k] = v == elem_nodes[3 * s] || v == elem_nodes[3 * s + 1] || v == elem_nodes[3 * s + 2];
+                }
+                bool contact = false;
+                {
+#pragma clang fp contract(off)
+                    // contact_rule, src/assembly/assemble.jl:199-207
+                    c.tol = P.contact_rel * fmin(c.T->diam, c.S->diam);
+                    if (!(cls_norm(ld3(c.S->c) - ld3(c.T->c)) > (c.T->reach + c.S->reach) + c.tol)) {
+                        bool extra = false;
+                        for (int k = 0; k < 3 && !extra; k++) extra = !shared[k] && cls_tri_distance(c.SX, c.X[k]) <= c.tol;
+                        for (int k = 0; k < 3 && !extra; k++) {
+                            double ts3[3];
+                            extra = edge_splits_k(c, c.X, k, ts3) > 0;
+                        }
+                        contact = extra;
+                    }
+                }
+                int ns = shared[0] + shared[1] + shared[2];
+                if (contact) {
+                    tier = T_CONTACT;
+                    Piece r0;
+                    set_corners(r0.b);
+                    r0.frac = 1.0;
+                    r0.kind = 1;
+                    r0.level = 0;
+                    r0.depth = 0;
+                    stk[sp++] = r0;
+                    walk(c, stk, sp);
+                } else if (ns == 2) {
+                    tier = T_EDGE;
+                    eval_leaf(c, R_EDGE0 + (shared[0] && shared[1] ? 0 : shared[1] && shared[2] ? 1 : 2), b0, 1.0);
+                } else if (ns == 1) {
+                    tier = T_VERTEX;
+                    int kv = shared[0] ? 0 : shared[1] ? 1 : 2;
+                    V3<double> u = c.X[(kv + 1) % 3] - c.X[kv], v = c.X[(kv + 2) % 3] - c.X[kv];
+                    if (cls_dot(u, v) >= 0)
+                        eval_leaf(c, R_VERT0 + kv, b0, 1.0);
+                    else
+                        vertex_pieces(c, c.X, kv, b0, 1.0);
+                } else {
+                    bool tp = false;
+                    if (s < P.nt && ntouch) {  // touch pairs get a deeper near rule (assemble.jl:280)
+                        int lo = 0, hi = ntouch;
+                        while (lo < hi) {
+                            int mid = (lo + hi) / 2;
+                            int2 q = touch[mid];
+                            if (q.x < t || (q.x == t && q.y < s)) lo = mid + 1; else hi = mid;
+                        }
+                        tp = lo < ntouch && touch[lo].x == t && touch[lo].y == s;
+                    }
+                    Piece r0;
+                    set_corners(r0.b);
+                    r0.frac = 1.0;
+                    r0.kind = 0;
+                    r0.level = 0;
+                    r0.depth = P.near_depth + (tp ? 4 : 0);
+                    stk[sp++] = r0;
+                    walk(c, stk, sp);
+                }
+            }
+            double tot[12];
+            for (int i = 0; i < 12; i++) tot[i] = wave_sum(c.acc[i]);
+            if (lane == 0) {
+                const int *ct = corner_slot + 3 * t;
+                const int *cs = src_cols + 3 * s;
+                for (int a = 0; a < 3; a++)
+                    for (int bb = 0; bb < 3; bb++) unsafeAtomicAdd(&A[(long long)cs[bb] * P.lda + ct[a]], tot[3 * a + bb]);
+                if (s < P.nt)
+                    for (int a = 0; a < 3; a++) unsafeAtomicAdd(&bvec[ct[a]], tot[9 + a]);
+            }
+        }
+        if (lane == 0) {
+            atomicAdd(&wstat[tier], 1ull);
+            atomicAdd(&wstat[6 + tier], (unsigned long long)c.npts);
+            if (tiers && !c.seg) tiers[(size_t)t * P.nsrc + s] = (uint8_t)tier;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 12 && wstat[threadIdx.x]) atomicAdd(&stats[threadIdx.x], wstat[threadIdx.x]);
+}
+
+// freestream part of b (exact), src/assembly/assemble.jl:422
+__global__ void freestream(const Elem *__restrict__ el, const int *__restrict__ corner_slot, int nt, int test,
+                           double vx, double vy, double vz, double *__restrict__ bvec)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const Elem &E = el[t];
+    double vn;
+    {
+#pragma clang fp contract(off)
+        vn = vx * E.n[0] + vy * E.n[1] + vz * E.n[2];
+    }
+    double ti = test == 1 ? 1.0 : 1.0 / 3;
+    for (int a = 0; a < 3; a++) unsafeAtomicAdd(&bvec[corner_slot[3 * t + a]], -(vn * E.area * ti));
+}
+
+// ------------------------------------------------------------------------------------------ host side --------------
+
+namespace {
+
+// Morton code of a point in the unit cube (10 bits per axis)
+uint32_t morton3(double x, double y, double z)
+{
+    auto spread = [](uint32_t v) {
+        v &= 0x3ff;
+        v = (v | (v << 16)) & 0x030000FF;
+        v = (v | (v << 8)) & 0x0300F00F;
+        v = (v | (v << 4)) & 0x030C30C3;
+        v = (v | (v << 2)) & 0x09249249;
+        return v;
+    };
+    auto q = [](double u) { return (uint32_t)std::min(1023.0, std::max(0.0, u * 1023.0)); };
+    return spread(q(x)) | (spread(q(y)) << 1) | (spread(q(z)) << 2);
+}
+
+// Greedy tiles along a space-filling order: up to `cap` elements and `lmax` distinct columns per tile.
+void build_tiles(const std::vector<int> &order, const std::vector<int> &cols3, int cap, int lmax, std::vector<int> &elems,
+                 std::vector<uint8_t> &loc, std::vector<int> &ucols, std::vector<int> &nu, int ucap)
+{
+    size_t i = 0;
+    while (i < order.size()) {
+        std::vector<int> uc;
+        std::vector<int> tile;
+        std::vector<uint8_t> tl;
+        while (i < order.size() && (int)tile.size() < cap) {
+            int e = order[i];
+            int add = 0;
+            int idx[3];
+            for (int k = 0; k < 3; k++) {
+                int col = cols3[3 * e + k];
+                auto it = std::find(uc.begin(), uc.end(), col);
+                if (it == uc.end()) {
+                    bool dup = false;
+                    for (int j = 0; j < k; j++) dup |= cols3[3 * e + j] == col;
+                    if (!dup) add++;
+                }
+            }
+            if ((int)uc.size() + add > lmax) break;
+            for (int k = 0; k < 3; k++) {
+                int col = cols3[3 * e + k];
+                auto it = std::find(uc.begin(), uc.end(), col);
+                if (it == uc.end()) {
+                    uc.push_back(col);
+                    idx[k] = (int)uc.size() - 1;
+                } else {
+                    idx[k] = (int)(it - uc.begin());
+                }
+            }
+            tile.push_back(e);
+            for (int k = 0; k < 3; k++) tl.push_back((uint8_t)idx[k]);
+            i++;
+        }
+        for (size_t k = 0; k < (size_t)cap; k++) {
+            elems.push_back(k < tile.size() ? tile[k] : -1);
+            for (int j = 0; j < 3; j++) loc.push_back(k < tile.size() ? tl[3 * k + j] : 0);
+        }
+        for (int k = 0; k < ucap; k++) ucols.push_back(k < (int)uc.size() ? uc[k] : -1);
+        nu.push_back((int)uc.size());
+    }
+}
+
+std::vector<int> space_order(const std::vector<double> &cent, int ne)
+{
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int e = 0; e < ne; e++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = std::min(lo[k], cent[3 * e + k]);
+            hi[k] = std::max(hi[k], cent[3 * e + k]);
+        }
+    double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+    std::vector<std::pair<uint32_t, int>> key(ne);
+    for (int e = 0; e < ne; e++)
+        key[e] = {morton3((cent[3 * e] - lo[0]) / span, (cent[3 * e + 1] - lo[1]) / span, (cent[3 * e + 2] - lo[2]) / span), e};
+    std::stable_sort(key.begin(), key.end());
+    std::vector<int> order(ne);
+    for (int e = 0; e < ne; e++) order[e] = key[e].second;
+    return order;
+}
+
+}  // namespace
+
+#define HIPCHECK(x)                                                                                                    \
+    do {                                                                                                               \
+        hipError_t e_ = (x);                                                                                           \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_));                \
+    } while (0)
+
+template <typename X> X *dalloc(size_t n)
+{
+    X *p = nullptr;
+    if (n == 0) n = 1;
+    HIPCHECK(hipMalloc((void **)&p, n * sizeof(X)));
+    return p;
+}
+
+template <typename X> X *dupload(const std::vector<X> &v, hipStream_t s)
+{
+    X *p = dalloc<X>(v.size());
+    if (!v.empty()) HIPCHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice, s));
+    return p;
+}
+
+Plan::~Plan()
+{
+    for (void *p : owned) (void)hipFree(p);
+    if (A && ownsA) (void)hipFree(A);
+}
+
+void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_assembly_options &o)
+{
+    ctx = ctx_;
+    hipStream_t st = ctx->stream;
+    opts = o;
+    if (o.test < 0 || o.test > 2) throw std::invalid_argument("test must be hat (0), onering (1) or dual (2)");
+    if (o.precision != 0 && o.precision != 1) throw std::invalid_argument("precision must be 0 (fp64) or 1 (mixed)");
+    if (g.nt <= 0 || g.nn <= 0 || g.nslot <= 0) throw std::invalid_argument("empty surface");
+    nt = g.nt;
+    nn = g.nn;
+    nslot = g.nslot;
+    nw = wk ? wk->nw : 0;
+    nwt = wk ? wk->ntri : 0;
+    nseg = wk ? wk->nfil : 0;
+    nsrc = nt + nwt;
+    ncol = nslot + nw;
+    lda = nslot;
+    for (int t = 0; t < nt; t++)
+        for (int k = 0; k < 3; k++) {
+            int v = g.tris[3 * t + k], cs = g.corner_slot[3 * t + k];
+            if (v < 0 || v >= nn) throw std::out_of_range("triangle node index out of range");
+            if (cs < 0 || cs >= nslot) throw std::out_of_range("corner slot out of range");
+        }
+    for (int t = 0; t < nwt; t++)
+        for (int k = 0; k < 3; k++)
+            if (wk->tris[3 * t + k] < 0 || wk->tris[3 * t + k] >= nw) throw std::out_of_range("wake triangle index out of range");
+    for (int f = 0; f < nseg; f++)
+        for (int k = 0; k < 2; k++)
+            if (wk->filaments[2 * f + k] < 0 || wk->filaments[2 * f + k] >= nw) throw std::out_of_range("filament index out of range");
+
+    // rules (TierRules, src/elements/rules.jl:161-174)
+    std::vector<HostRule> rules(R_COUNT);
+    rules[R_FAR] = triangle_rule(o.far_points);
+    rules[R_NEAR] = triangle_rule(o.near_points);
+    if (rules[R_FAR].n() != 3) throw std::invalid_argument("the far tier kernel takes the 3-point rule (far_points = 3)");
+    rules[R_SELF] = self_rule(o.touch_levels, o.touch_ratio, o.touch_order);
+    HostRule e = edge_graded_rule(o.touch_levels, o.touch_ratio, o.touch_order);
+    HostRule v = vertex_graded_rule(o.touch_levels, o.touch_ratio, o.touch_order);
+    for (int k = 0; k < 3; k++) {
+        rules[R_EDGE0 + k] = rotate_rule(e, k + 1);
+        rules[R_VERT0 + k] = rotate_rule(v, k + 1);
+    }
+    std::vector<double> rl, rw;
+    for (int r = 0; r < R_COUNT; r++) {
+        rt.off[r] = (int)rw.size();
+        rt.n[r] = rules[r].n();
+        rl.insert(rl.end(), rules[r].l.begin(), rules[r].l.end());
+        rw.insert(rw.end(), rules[r].w.begin(), rules[r].w.end());
+    }
+    double *dl = dupload(rl, st), *dw = dupload(rw, st);
+    owned.push_back(dl);
+    owned.push_back(dw);
+    rt.l = dl;
+    rt.w = dw;
+
+    // elements: body then wake; node ids (wake offset by nn so no wake corner is shared with the body)
+    std::vector<double> xall((size_t)3 * (nn + nw));
+    std::memcpy(xall.data(), g.x, sizeof(double) * 3 * nn);
+    if (nw) std::memcpy(xall.data() + 3 * nn, wk->x, sizeof(double) * 3 * nw);
+    std::vector<int> en((size_t)3 * nsrc), cols((size_t)3 * nsrc);
+    for (int t = 0; t < nt; t++)
+        for (int k = 0; k < 3; k++) {
+            en[3 * t + k] = g.tris[3 * t + k];
+            cols[3 * t + k] = g.corner_slot[3 * t + k];
+        }
+    for (int t = 0; t < nwt; t++)
+        for (int k = 0; k < 3; k++) {
+            en[3 * (nt + t) + k] = nn + wk->tris[3 * t + k];
+            cols[3 * (nt + t) + k] = nslot + wk->tris[3 * t + k];
+        }
+    double *dx = dupload(xall, st);
+    d_nodes = dupload(en, st);
+    d_cols = dupload(cols, st);
+    d_cs = dupload(std::vector<int>(g.corner_slot, g.corner_slot + 3 * nt), st);
+    owned.insert(owned.end(), {dx, d_nodes, d_cols, d_cs});
+    d_el = dalloc<Elem>(nsrc);
+    owned.push_back(d_el);
+    elem_setup<<<(nsrc + 255) / 256, 256, 0, st>>>(dx, d_nodes, nsrc, d_el);
+    HIPCHECK(hipGetLastError());
+    std::vector<Elem> hel(nsrc);
+    HIPCHECK(hipMemcpyAsync(hel.data(), d_el, sizeof(Elem) * nsrc, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    for (int e2 = 0; e2 < nsrc; e2++)
+        if (hel[e2].pad0 != 0.0) throw std::invalid_argument("degenerate triangle " + std::to_string(e2));
+    d_tf = dalloc<TestFar>(nt);
+    owned.push_back(d_tf);
+    test_setup<<<(nt + 255) / 256, 256, 0, st>>>(d_el, nt, rt, o.test, d_tf);
+    HIPCHECK(hipGetLastError());
+
+    // filaments
+    std::vector<double> segx((size_t)6 * std::max(nseg, 1));
+    std::vector<int> segc((size_t)2 * std::max(nseg, 1));
+    for (int f = 0; f < nseg; f++) {
+        int a = wk->filaments[2 * f], b = wk->filaments[2 * f + 1];
+        for (int k = 0; k < 3; k++) {
+            segx[6 * f + k] = wk->x[3 * a + k];
+            segx[6 * f + 3 + k] = wk->x[3 * b + k];
+        }
+        segc[2 * f] = nslot + a;
+        segc[2 * f + 1] = nslot + b;
+    }
+    d_seg = dupload(segx, st);
+    d_segcol = dupload(segc, st);
+    owned.insert(owned.end(), {d_seg, d_segcol});
+
+    // touch pairs, sorted with both orders (assemble.jl:383-386)
+    std::vector<int2> tp;
+    for (int i = 0; i < g.ntouch; i++) {
+        tp.push_back(make_int2(g.touch[2 * i], g.touch[2 * i + 1]));
+        tp.push_back(make_int2(g.touch[2 * i + 1], g.touch[2 * i]));
+    }
+    std::sort(tp.begin(), tp.end(), [](int2 a, int2 b) { return a.x < b.x || (a.x == b.x && a.y < b.y); });
+    ntouch = (int)tp.size();
+    d_touch = dupload(tp, st);
+    owned.push_back(d_touch);
+
+    // tiles over a space-filling order of the centroids
+    std::vector<double> cent((size_t)3 * nsrc);
+    for (int e2 = 0; e2 < nsrc; e2++)
+        for (int k = 0; k < 3; k++) cent[3 * e2 + k] = hel[e2].c[k];
+    std::vector<double> tcent(cent.begin(), cent.begin() + 3 * nt);
+    std::vector<int> torder = space_order(tcent, nt);
+    std::vector<int> sorder_b = space_order(tcent, nt), sorder;
+    sorder = sorder_b;
+    if (nwt) {
+        std::vector<double> wc(cent.begin() + 3 * nt, cent.end());
+        std::vector<int> wo = space_order(wc, nwt);
+        for (int x : wo) sorder.push_back(nt + x);
+    }
+    std::vector<int> tcols(g.corner_slot, g.corner_slot + 3 * nt);
+    std::vector<int> tte, ttr, ttn, tse, tsc, tsn;
+    std::vector<uint8_t> ttl, tsl;
+    build_tiles(torder, tcols, TT_MAX, VMAX, tte, ttl, ttr, ttn, VMAX);
+    build_tiles(sorder, cols, TS, UMAX, tse, tsl, tsc, tsn, UMAX);
+    ntt = (int)ttn.size();
+    nts = (int)tsn.size();
+    d_tte = dupload(tte, st);
+    d_ttl = dupload(ttl, st);
+    d_ttr = dupload(ttr, st);
+    d_ttn = dupload(ttn, st);
+    d_tse = dupload(tse, st);
+    d_tsl = dupload(tsl, st);
+    d_tsc = dupload(tsc, st);
+    d_tsn = dupload(tsn, st);
+    owned.insert(owned.end(), {d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc, d_tsn});
+
+    // outputs and work lists
+    A = dalloc<double>((size_t)lda * ncol);
+    ownsA = true;
+    d_b = dalloc<double>(nslot);
+    d_sigma = dalloc<double>(nt);
+    d_stats = dalloc<unsigned long long>(16);
+    d_count = dalloc<unsigned int>(2);
+    owned.insert(owned.end(), {d_b, d_sigma, d_stats, d_count});
+    near_cap = (unsigned int)std::min<long long>((long long)nt * 160 + 1024, 1ll << 31);
+    d_near = dalloc<int2>(near_cap);
+    owned.push_back(d_near);
+    if (record_tiers) {
+        d_tiers = dalloc<uint8_t>((size_t)nt * nsrc);
+        owned.push_back(d_tiers);
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+}
+
+void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
+{
+    hipStream_t st = ctx->stream;
+    HIPCHECK(hipMemcpyAsync(d_sigma, sigma, sizeof(double) * nt, host_sigma ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, st));
+    for (int attempt = 0; attempt < 2; attempt++) {
+        HIPCHECK(hipMemsetAsync(A, 0, sizeof(double) * (size_t)lda * ncol, st));
+        HIPCHECK(hipMemsetAsync(d_b, 0, sizeof(double) * nslot, st));
+        HIPCHECK(hipMemsetAsync(d_stats, 0, sizeof(unsigned long long) * 16, st));
+        HIPCHECK(hipMemsetAsync(d_count, 0, sizeof(unsigned int) * 2, st));
+        if (d_tiers) HIPCHECK(hipMemsetAsync(d_tiers, 255, (size_t)nt * nsrc, st));
+        AsmParams P;
+        P.nt = nt;
+        P.nsrc = nsrc;
+        P.nseg = nseg;
+        P.nslot = nslot;
+        P.ncol = ncol;
+        P.nn = nn;
+        P.test = opts.test;
+        P.far_ratio = opts.far_ratio;
+        P.near_ratio = opts.near_ratio;
+        P.corner_angle = opts.corner_angle;
+        P.contact_rel = opts.contact_rel;
+        P.kernel_far = opts.kernel_far;
+        P.near_depth = opts.near_depth;
+        P.lda = lda;
+        dim3 grid(ntt, nts);
+        if (opts.precision == 1)
+            far_pairs<float><<<grid, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc, d_tsn,
+                                                   A, d_b, d_near, d_count, near_cap, d_stats + 5, d_tiers);
+        else
+            far_pairs<double><<<grid, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc,
+                                                    d_tsn, A, d_b, d_near, d_count, near_cap, d_stats + 5, d_tiers);
+        HIPCHECK(hipGetLastError());
+        if (nseg) {
+            long long np = (long long)nt * nseg;
+            seg_pairs<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(P, d_tf, d_cs, d_seg, d_segcol, A, d_near, d_count,
+                                                                    near_cap, d_stats + 5);
+            HIPCHECK(hipGetLastError());
+        }
+        unsigned int cnt = 0;
+        HIPCHECK(hipMemcpyAsync(&cnt, d_count, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        if (cnt > near_cap) {  // rare: more near pairs than budgeted; grow and redo
+            (void)hipFree(d_near);
+            owned.erase(std::find(owned.begin(), owned.end(), (void *)d_near));
+            near_cap = cnt + 1024;
+            d_near = dalloc<int2>(near_cap);
+            owned.push_back(d_near);
+            continue;
+        }
+        nnear = cnt;
+        if (cnt) {
+            unsigned int blocks = std::min<unsigned int>((cnt + 3) / 4, 65535u);
+            near_pairs<<<blocks, 256, 0, st>>>(P, rt, d_el, d_nodes, d_sigma, d_cs, d_cols, d_seg, d_segcol, d_near, cnt,
+                                               d_touch, ntouch, A, d_b, d_stats, d_tiers);
+            HIPCHECK(hipGetLastError());
+        }
+        freestream<<<(nt + 255) / 256, 256, 0, st>>>(d_el, d_cs, nt, opts.test, vinf[0], vinf[1], vinf[2], d_b);
+        HIPCHECK(hipGetLastError());
+        break;
+    }
+}
+
+void Plan::stats(long long *pairs, long long *points)
+{
+    unsigned long long h[16];
+    HIPCHECK(hipMemcpyAsync(h, d_stats, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < 6; k++) {
+        pairs[k] = (long long)h[k];
+        points[k] = (long long)h[6 + k];
+    }
+    // far pairs were counted in slot 5 by far_pairs and seg_pairs (their points are 3 each)
+    points[5] = 3 * pairs[5] + points[5];
+}
+
+}  // namespace vp

@@ -1,0 +1,265 @@
+// The C ABI (include/vortexhip.h): argument checks, error strings and the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <exception>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "vp_capi_internal.hpp"
+
+namespace {
+
+int fail(vp::Ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+template <typename F> int guard(vp::Ctx *c, F &&f)
+{
+    try {
+        f();
+        if (c) c->err.clear();
+        return VP_OK;
+    } catch (const std::invalid_argument &e) {
+        return fail(c, VP_E_ARGUMENT, e.what());
+    } catch (const std::out_of_range &e) {
+        return fail(c, VP_E_ARGUMENT, e.what());
+    } catch (const std::length_error &e) {
+        return fail(c, VP_E_DIMENSION, e.what());
+    } catch (const std::domain_error &e) {
+        return fail(c, VP_E_SINGULAR, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(c, VP_E_DEVICE, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(c, VP_E_DEVICE, e.what());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int vp_create(int device, vp_ctx **out)
+{
+    if (!out) return VP_E_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return VP_E_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return VP_E_DEVICE;
+    vp_ctx *c = new (std::nothrow) vp_ctx;
+    if (!c) return VP_E_DEVICE;
+    c->c.device = device;
+    if (hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return VP_E_DEVICE;
+    }
+    *out = c;
+    return VP_OK;
+}
+
+void vp_destroy(vp_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    (void)hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+}
+
+const char *vp_last_error(const vp_ctx *ctx) { return ctx ? ctx->c.err.c_str() : "no context"; }
+
+int vp_sync(vp_ctx *ctx)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    return guard(&ctx->c, [&] {
+        hipError_t e = hipStreamSynchronize(ctx->c.stream);
+        if (e != hipSuccess) throw std::runtime_error(hipGetErrorString(e));
+    });
+}
+
+int vp_plan_create(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const vp_assembly_options *opts, vp_plan **out)
+{
+    return vp_plan_create_ex(ctx, g, wake, opts, 0, out);
+}
+
+int vp_plan_create_ex(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const vp_assembly_options *opts, int flags,
+                      vp_plan **out)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if (!g || !opts || !out) return fail(&ctx->c, VP_E_ARGUMENT, "null argument");
+    *out = nullptr;
+    vp_plan *p = new (std::nothrow) vp_plan;
+    if (!p) return fail(&ctx->c, VP_E_DEVICE, "host allocation failed");
+    p->p.record_tiers = (flags & VP_PLAN_RECORD_TIERS) != 0;
+    int rc = guard(&ctx->c, [&] {
+        (void)hipSetDevice(ctx->c.device);
+        p->p.build(&ctx->c, *g, wake && (wake->nw > 0) ? wake : nullptr, *opts);
+    });
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return VP_OK;
+}
+
+int vp_plan_assemble(vp_plan *plan, const double *sigma, const double *vinf, long long *pairs, long long *points)
+{
+    if (!plan || !plan->p.ctx) return VP_E_ARGUMENT;
+    vp::Ctx *c = plan->p.ctx;
+    if (!sigma || !vinf) return fail(c, VP_E_ARGUMENT, "null argument");
+    return guard(c, [&] {
+        plan->p.assemble(sigma, vinf, true);
+        if (pairs && points) plan->p.stats(pairs, points);
+    });
+}
+
+int vp_plan_read(vp_plan *plan, double *A, double *b)
+{
+    if (!plan || !plan->p.ctx) return VP_E_ARGUMENT;
+    vp::Ctx *c = plan->p.ctx;
+    return guard(c, [&] {
+        vp::Plan &P = plan->p;
+        if (A && hipMemcpyAsync(A, P.A, sizeof(double) * (size_t)P.lda * P.ncol, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            throw std::runtime_error("copy of A failed");
+        if (b && hipMemcpyAsync(b, P.d_b, sizeof(double) * P.nslot, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+            throw std::runtime_error("copy of b failed");
+        if (hipStreamSynchronize(c->stream) != hipSuccess) throw std::runtime_error("stream synchronization failed");
+    });
+}
+
+int vp_plan_tiers(vp_plan *plan, uint8_t *tiers)
+{
+    if (!plan || !plan->p.ctx || !tiers) return VP_E_ARGUMENT;
+    vp::Ctx *c = plan->p.ctx;
+    if (!plan->p.d_tiers) return fail(c, VP_E_ARGUMENT, "the plan does not record tiers (VP_PLAN_RECORD_TIERS)");
+    return guard(c, [&] {
+        vp::Plan &P = plan->p;
+        if (hipMemcpyAsync(tiers, P.d_tiers, (size_t)P.nt * P.nsrc, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            throw std::runtime_error("copy of the tiers failed");
+    });
+}
+
+int vp_plan_device_A(vp_plan *plan, double **A, double **b, long long *lda)
+{
+    if (!plan || !plan->p.A) return VP_E_ARGUMENT;
+    if (A) *A = plan->p.A;
+    if (b) *b = plan->p.d_b;
+    if (lda) *lda = plan->p.lda;
+    return VP_OK;
+}
+
+void vp_plan_destroy(vp_plan *plan)
+{
+    if (!plan) return;
+    if (plan->p.ctx) (void)hipStreamSynchronize(plan->p.ctx->stream);
+    delete plan;
+}
+
+int vp_factorize(vp_ctx *ctx, const double *A, long long lda, int A_on_device, const vp_constraints *k, int mode,
+                 vp_factor **out)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if (!A || !k || !out || !k->C_rowptr || !k->C_col || !k->C_val || (k->nw > 0 && (!k->W_rowptr || !k->W_col || !k->W_val)) ||
+        (k->nb > 0 && (!k->e || !k->c)))
+        return fail(&ctx->c, VP_E_ARGUMENT, "null argument");
+    if (lda < k->nslot) return fail(&ctx->c, VP_E_DIMENSION, "lda is smaller than nslot");
+    *out = nullptr;
+    vp_factor *f = new (std::nothrow) vp_factor;
+    if (!f) return fail(&ctx->c, VP_E_DEVICE, "host allocation failed");
+    double *tmp = nullptr;
+    int rc = guard(&ctx->c, [&] {
+        (void)hipSetDevice(ctx->c.device);
+        const double *Ad = A;
+        if (!A_on_device) {
+            size_t bytes = sizeof(double) * (size_t)lda * (size_t)(k->nslot + k->nw);
+            if (hipMalloc((void **)&tmp, bytes) != hipSuccess) throw std::runtime_error("device allocation of A failed");
+            if (hipMemcpyAsync(tmp, A, bytes, hipMemcpyHostToDevice, ctx->c.stream) != hipSuccess)
+                throw std::runtime_error("copy of A failed");
+            Ad = tmp;
+        }
+        f->f.build(&ctx->c, Ad, lda, *k, mode);
+    });
+    if (tmp) {
+        (void)hipStreamSynchronize(ctx->c.stream);
+        (void)hipFree(tmp);
+    }
+    if (rc) {
+        delete f;
+        return rc;
+    }
+    *out = f;
+    return VP_OK;
+}
+
+int vp_factor_solve(vp_factor *f, const double *R, int nrhs, double *X, int *iters, double *resid)
+{
+    if (!f || !f->f.ctx) return VP_E_ARGUMENT;
+    vp::Ctx *c = f->f.ctx;
+    if (!R || !X || nrhs <= 0) return fail(c, VP_E_ARGUMENT, "null argument or nrhs < 1");
+    return guard(c, [&] {
+        int it = f->f.solve(R, nrhs, X, false, 30, resid);
+        if (iters) *iters = it;
+    });
+}
+
+int vp_factor_cond(vp_factor *f, double *cond)
+{
+    if (!f || !f->f.ctx || !cond) return VP_E_ARGUMENT;
+    return guard(f->f.ctx, [&] {
+        if (f->f.cond < 0) f->f.cond = f->f.anorm1 * f->f.cond_estimate();
+        *cond = f->f.cond;
+    });
+}
+
+int vp_factor_info(vp_factor *f, int *n, double *se)
+{
+    if (!f) return VP_E_ARGUMENT;
+    if (n) *n = f->f.n;
+    if (se)
+        for (size_t i = 0; i < f->f.se.size(); i++) se[i] = f->f.se[i];
+    return VP_OK;
+}
+
+void vp_factor_destroy(vp_factor *f)
+{
+    if (!f) return;
+    if (f->f.ctx) (void)hipStreamSynchronize(f->f.ctx->stream);
+    delete f;
+}
+
+int vp_gemm(vp_ctx *ctx, int prec, int sub, int M, int N, int K, const double *A, const double *B, double *C)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if (!A || !B || !C || (prec != 0 && prec != 1) || M <= 0 || N <= 0 || K <= 0)
+        return fail(&ctx->c, VP_E_ARGUMENT, "bad argument");
+    return guard(&ctx->c, [&] { vp::gemm_host(ctx->c.stream, prec, sub, M, N, K, A, B, C); });
+}
+
+int vp_dense_solve(vp_ctx *ctx, int n, const double *M, int mode, const double *r, double *x, int *iters)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if (!M || !r || !x || n <= 0 || (mode != 0 && mode != 1)) return fail(&ctx->c, VP_E_ARGUMENT, "bad argument");
+    return guard(&ctx->c, [&] {
+        int it = vp::dense_solve_host(&ctx->c, n, M, mode, r, x);
+        if (iters) *iters = it;
+    });
+}
+
+int vp_assemble(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const double *sigma, const double *vinf,
+                const vp_assembly_options *opts, double *A, double *b, long long *pairs, long long *points)
+{
+    vp_plan *p = nullptr;
+    int rc = vp_plan_create(ctx, g, wake, opts, &p);
+    if (rc) return rc;
+    rc = vp_plan_assemble(p, sigma, vinf, pairs, points);
+    if (!rc) rc = vp_plan_read(p, A, b);
+    vp_plan_destroy(p);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,102 @@
+// Internal types behind the C ABI: the context (device + stream + last error) and the assembly plan.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/vortexhip.h"
+
+namespace vp {
+
+struct Elem;
+struct TestFar;
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+};
+
+struct RuleTab {  // all rules' points (3 barycentric coordinates each) and weights, concatenated
+    const double *l;
+    const double *w;
+    int off[9];
+    int n[9];
+};
+
+struct Plan {
+    Ctx *ctx = nullptr;
+    vp_assembly_options opts{};
+    int nt = 0, nn = 0, nslot = 0, nw = 0, nwt = 0, nseg = 0, nsrc = 0, ncol = 0, ntouch = 0;
+    long long lda = 0;
+    int ntt = 0, nts = 0;
+    unsigned int near_cap = 0, nnear = 0;
+    std::vector<void *> owned;
+    // device buffers
+    Elem *d_el = nullptr;
+    TestFar *d_tf = nullptr;
+    int *d_nodes = nullptr, *d_cols = nullptr, *d_cs = nullptr;
+    double *d_seg = nullptr;
+    int *d_segcol = nullptr;
+    int2 *d_touch = nullptr;
+    int2 *d_near = nullptr;
+    int *d_tte = nullptr, *d_ttr = nullptr, *d_ttn = nullptr, *d_tse = nullptr, *d_tsc = nullptr, *d_tsn = nullptr;
+    uint8_t *d_ttl = nullptr, *d_tsl = nullptr;
+    uint8_t *d_tiers = nullptr;
+    double *A = nullptr, *d_b = nullptr, *d_sigma = nullptr;
+    bool ownsA = false;
+    unsigned long long *d_stats = nullptr;
+    unsigned int *d_count = nullptr;
+    RuleTab rt{};
+    bool record_tiers = false;
+
+    ~Plan();
+    void build(Ctx *ctx, const vp_surface &g, const vp_wake *wake, const vp_assembly_options &o);
+    void assemble(const double *sigma, const double *vinf, bool host_sigma);
+    void stats(long long *pairs, long long *points);
+};
+
+// The factored bordered system (solve.jl:65-87): the original FP64 bordered matrix M (row-major, npad x npad, the
+// identity on the padding), its randomized LU in FP64 (mode 0) or FP32 (mode 1), the butterflies, and workspaces.
+struct Factor {
+    Ctx *ctx = nullptr;
+    int mode = 0, nslot = 0, nq = 0, nw = 0, nb = 0, n = 0, npad = 0;
+    long long ld = 0;
+    double *M = nullptr, *Mt = nullptr, *d_ru = nullptr, *d_rv = nullptr;
+    double *G64 = nullptr, *linv64 = nullptr, *uinv64 = nullptr;
+    float *G32 = nullptr, *linv32 = nullptr, *uinv32 = nullptr;
+    double *X = nullptr, *R = nullptr, *Y = nullptr, *B = nullptr, *nrm = nullptr;
+    float *Xf = nullptr;
+    int ws_nrhs = 0, last_iters = 0;
+    double anorm1 = 0.0, anorm_inf = -1.0, cond = -1.0;
+    std::vector<double> se;
+    std::vector<void *> owned;
+    Factor *tfac = nullptr;
+
+    ~Factor();
+    void build(Ctx *c, const double *A, long long lda, const vp_constraints &k, int mode);
+    void init_from_dense(const double *Md, long long ldd, int n, int npad, int mode, double anorm);
+    void ensure_ws(int nrhs);
+    void apply_inverse(int nrhs);
+    int solve(const double *B, int nrhs, double *X, bool device_io, int max_iter, double *resid);
+    double cond_estimate();
+    void solve_transposed(const double *z, double *x);
+    void transpose_into(const double *in, double *out);
+};
+
+void gemm_host(hipStream_t s, int prec, int sub, int M, int N, int K, const double *A, const double *B, double *C);
+int dense_solve_host(Ctx *c, int n, const double *M, int mode, const double *r, double *x);
+
+}  // namespace vp
+
+struct vp_factor {
+    vp::Factor f;
+};
+
+struct vp_ctx {
+    vp::Ctx c;
+};
+struct vp_plan {
+    vp::Plan p;
+};

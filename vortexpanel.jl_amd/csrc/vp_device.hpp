@@ -1,0 +1,244 @@
+// Device-side kernels of the Galerkin hot path for gfx950: the unit source triangle, the linear-strength filament,
+// the Rosenhead-Moore blob, and the geometric predicates of the quadrature tiers.
+//
+// Reference: src/kernels/source_triangle.jl:8-138, src/kernels/segment.jl:15-25, src/kernels/blob.jl:21-25,
+// src/elements/flat_triangle.jl:59-78 (distance_to), src/assembly/assemble.jl:145-184.
+//
+// Precision: everything is templated on the compute type T (double for the FP64 path, float for the far field of
+// the mixed path).  Functions named cls_* decide quadrature tiers and rule pieces; they are compiled with FP
+// contraction off and in the reference's operation order, so that tiers and rule sizes match the FP64 oracle bit for
+// bit (hipcc contracts a*b+c into FMA by default, Julia never does).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vp {
+
+constexpr double kPi = 3.14159265358979323846;
+
+template <typename T> struct V3 {
+    T x, y, z;
+};
+
+template <typename T> __device__ __forceinline__ V3<T> mk3(T a, T b, T c) { return V3<T>{a, b, c}; }
+template <typename T> __device__ __forceinline__ V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <typename T> __device__ __forceinline__ V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename T> __device__ __forceinline__ V3<T> operator*(T s, V3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <typename T> __device__ __forceinline__ V3<T> operator*(V3<T> a, T s) { return {s * a.x, s * a.y, s * a.z}; }
+template <typename T> __device__ __forceinline__ T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename T> __device__ __forceinline__ V3<T> cross(V3<T> a, V3<T> b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double dlog(double x) { return log(x); }
+__device__ __forceinline__ float dlog(float x) { return __logf(x); }
+__device__ __forceinline__ double datan2(double y, double x) { return atan2(y, x); }
+__device__ __forceinline__ float datan2(float y, float x) { return atan2f(y, x); }
+__device__ __forceinline__ double drcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float drcp(float x) { return __frcp_rn(x); }
+
+// Element data on the device: one record per flat triangle (body, then wake), 40 doubles (320 bytes).
+// v: vertices, n: unit normal, c: centroid, nu: in-plane outward edge normals, len: edge lengths, g: gamma_b
+// (the sheet of a unit doublet at corner b, -e_b/(2A)), reach: largest centroid-to-corner distance.
+struct Elem {
+    double v[3][3];
+    double n[3];
+    double c[3];
+    double area, diam, reach, pad0;
+    double nu[3][3];
+    double len[3];
+    double g[3][3];
+};
+static_assert(sizeof(Elem) == 40 * 8, "Elem layout");
+
+// Source triangle in registers, in the compute type, with coordinates relative to `origin` (the element's centroid
+// for the far field, so FP32 keeps its digits where the cancellation happens).
+template <typename T> struct SrcT {
+    V3<T> v[3], n, nu[3];
+    T len[3], diam, area;
+};
+
+// edge_log, src/kernels/source_triangle.jl:39-44
+template <typename T> __device__ __forceinline__ T edge_log(V3<T> ri, V3<T> rj, T d)
+{
+    T a = dsqrt(dot(ri, ri)), b = dsqrt(dot(rj, rj));
+    T p = dot(ri, rj);
+    V3<T> c = cross(ri, rj);
+    T s2d2 = p < T(0) ? T(2) * dot(c, c) / (a * b - p) : T(2) * (a * b + p);
+    T t = a + b + d;
+    return dlog(t * t / s2d2);
+}
+
+// inplane_fraction, src/kernels/source_triangle.jl:52-64 (FP64 only: the self pair's in-plane points)
+__device__ inline double inplane_fraction(const Elem &e, V3<double> x, double tol)
+{
+#pragma clang fp contract(off)
+    V3<double> a = mk3(e.v[0][0], e.v[0][1], e.v[0][2]), b = mk3(e.v[1][0], e.v[1][1], e.v[1][2]),
+               c = mk3(e.v[2][0], e.v[2][1], e.v[2][2]), n = mk3(e.n[0], e.n[1], e.n[2]);
+    double A2 = 2 * e.area;
+    double l[3] = {dot(cross(b - x, c - x), n) / A2, dot(cross(c - x, a - x), n) / A2, dot(cross(a - x, b - x), n) / A2};
+    for (int k = 0; k < 3; k++)
+        if (l[k] < -tol) return 0.0;
+    int nz = 0;
+    for (int k = 0; k < 3; k++) nz += fabs(l[k]) <= tol;
+    if (nz == 0) return 1.0;
+    if (nz == 1) return 0.5;
+    int k = 0;
+    while (k < 2 && !(fabs(l[k]) > tol)) k++;
+    V3<double> vk = mk3(e.v[k][0], e.v[k][1], e.v[k][2]);
+    V3<double> e1 = mk3(e.v[(k + 1) % 3][0], e.v[(k + 1) % 3][1], e.v[(k + 1) % 3][2]) - vk;
+    V3<double> e2 = mk3(e.v[(k + 2) % 3][0], e.v[(k + 2) % 3][1], e.v[(k + 2) % 3][2]) - vk;
+    double ca = dot(e1, e2) / (dsqrt(dot(e1, e1)) * dsqrt(dot(e2, e2)));
+    ca = ca < -1 ? -1 : ca > 1 ? 1 : ca;
+    return acos(ca) / (2 * kPi);
+}
+
+// source_velocity_exact, src/kernels/source_triangle.jl:75-89, from the offsets ra, rb, rc of the point from the
+// vertices and its height z = (x - centroid).n over the plane.  The in-plane branch (side) needs the element in
+// FP64 and is only reached by the touching tiers (ein non-null).
+template <typename T>
+__device__ __forceinline__ V3<T> sv_exact_abs(const SrcT<T> &s, V3<T> ra, V3<T> rb, V3<T> rc, T z, int side,
+                                              const Elem *ein, V3<double> xabs)
+{
+    V3<T> wt = edge_log(ra, rb, s.len[0]) * s.nu[0] + edge_log(rb, rc, s.len[1]) * s.nu[1] + edge_log(rc, ra, s.len[2]) * s.nu[2];
+    T Om;
+    if (fabs(z) <= T(1e-12) * s.diam) {
+        Om = ein ? T(side * 2 * kPi * inplane_fraction(*ein, xabs, 1e-12)) : T(0);
+    } else {
+        T la = dsqrt(dot(ra, ra)), lb = dsqrt(dot(rb, rb)), lc = dsqrt(dot(rc, rc));
+        T N = dot(ra, cross(rb, rc));
+        T D = la * lb * lc + dot(ra, rb) * lc + dot(ra, rc) * lb + dot(rb, rc) * la;
+        Om = T(2) * datan2(N, D);
+    }
+    return T(1.0 / (4 * kPi)) * (wt + Om * s.n);
+}
+
+// source_velocity_far, src/kernels/source_triangle.jl:100-119 (vertex offsets from the centroid are s.v)
+template <typename T> __device__ __forceinline__ V3<T> sv_far_rel(const SrcT<T> &s, V3<T> r)
+{
+    T R2 = dot(r, r);
+    T iR = T(1) / dsqrt(R2);
+    T iR2 = iR * iR, iR3 = iR2 * iR, iR5 = iR3 * iR2, iR7 = iR5 * iR2, iR9 = iR7 * iR2;
+    // sum over the vertices of quad/24 - oct/180, collected as coefficients of r and of each d
+    T cr = T(0);
+    V3<T> w = mk3(T(0), T(0), T(0));
+    for (int k = 0; k < 3; k++) {
+        V3<T> d = s.v[k];
+        T dr = dot(d, r), dd = dot(d, d);
+        // r terms: (15 dr^2 iR7 - 3 dd iR5)/24 - (-105 dr^3 iR9 + 45 dd dr iR7)/180
+        cr += (T(15) * dr * dr * iR7 - T(3) * dd * iR5) * T(1.0 / 24) - (T(45) * dd * dr * iR7 - T(105) * dr * dr * dr * iR9) * T(1.0 / 180);
+        // d terms: (-6 dr iR5)/24 - (45 dr^2 iR7 - 9 dd iR5)/180
+        T cd = T(-6) * dr * iR5 * T(1.0 / 24) - (T(45) * dr * dr * iR7 - T(9) * dd * iR5) * T(1.0 / 180);
+        w = w + cd * d;
+    }
+    w = w + (iR3 + cr) * r;
+    return (s.area * T(1.0 / (4 * kPi))) * w;
+}
+
+// segment_velocity, src/kernels/segment.jl:15-25
+template <typename T> __device__ __forceinline__ void seg_vel(V3<T> ra, V3<T> rb, V3<T> &va, V3<T> &vb)
+{
+    T Ra = dsqrt(dot(ra, ra)), Rb = dsqrt(dot(rb, rb));
+    V3<T> c = cross(ra, rb);
+    T p = dot(ra, rb);
+    T den = p >= T(0) ? Ra * Rb + p : dot(c, c) / (Ra * Rb - p);
+    if (den == T(0) || Ra == T(0) || Rb == T(0)) {
+        va = vb = mk3(T(0), T(0), T(0));
+        return;
+    }
+    V3<T> f = (T(1) / (T(4 * kPi) * den)) * c;
+    va = (T(1) / Ra) * f;
+    vb = (T(1) / Rb) * f;
+}
+
+// blob_velocity, src/kernels/blob.jl:21-25, r = x - xp
+template <typename T> __device__ __forceinline__ V3<T> blob_vel(V3<T> r, V3<T> al, T d2core)
+{
+    T d2 = dot(r, r) + d2core;
+    T s = dsqrt(d2);
+    return (T(1) / (T(4 * kPi) * d2 * s)) * cross(al, r);
+}
+
+// ---------------------------------------------------------------- classification (contraction off, FP64) ----------
+
+__device__ inline double cls_norm(V3<double> a)
+{
+#pragma clang fp contract(off)
+    return __builtin_sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+}
+
+__device__ inline double cls_dot(V3<double> a, V3<double> b)
+{
+#pragma clang fp contract(off)
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+
+__device__ inline V3<double> cls_axpy(double s, V3<double> a, V3<double> b)  // s*a + b
+{
+#pragma clang fp contract(off)
+    return mk3(s * a.x + b.x, s * a.y + b.y, s * a.z + b.z);
+}
+
+__device__ inline V3<double> cls_point(const V3<double> *x, const double *l)  // point, flat_triangle.jl:36
+{
+#pragma clang fp contract(off)
+    return mk3((l[0] * x[0].x + l[1] * x[1].x) + l[2] * x[2].x, (l[0] * x[0].y + l[1] * x[1].y) + l[2] * x[2].y,
+               (l[0] * x[0].z + l[1] * x[1].z) + l[2] * x[2].z);
+}
+
+// Vector helpers for the classification code: every floating-point operation of the tier decisions is written in a
+// contraction-off scope (clang only fuses a multiply and an add that are both allowed to contract).
+__device__ inline V3<double> cls_sub(V3<double> a, V3<double> b)
+{
+#pragma clang fp contract(off)
+    return mk3(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+
+__device__ inline V3<double> cls_add_scaled(V3<double> a, double s, V3<double> b)  // a + s*b
+{
+#pragma clang fp contract(off)
+    return mk3(a.x + s * b.x, a.y + s * b.y, a.z + s * b.z);
+}
+
+// distance_to (Ericson 5.1.5), src/elements/flat_triangle.jl:59-78
+__device__ inline double cls_tri_distance(const V3<double> *x, V3<double> p)
+{
+#pragma clang fp contract(off)
+    V3<double> a = x[0], b = x[1], c = x[2];
+    V3<double> ab = cls_sub(b, a), ac = cls_sub(c, a), ap = cls_sub(p, a);
+    double d1 = cls_dot(ab, ap), d2 = cls_dot(ac, ap);
+    if (d1 <= 0 && d2 <= 0) return cls_norm(ap);
+    V3<double> bp = cls_sub(p, b);
+    double d3 = cls_dot(ab, bp), d4 = cls_dot(ac, bp);
+    if (d3 >= 0 && d4 <= d3) return cls_norm(bp);
+    double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0 && d1 >= 0 && d3 <= 0) return cls_norm(cls_sub(p, cls_add_scaled(a, d1 / (d1 - d3), ab)));
+    V3<double> cp = cls_sub(p, c);
+    double d5 = cls_dot(ab, cp), d6 = cls_dot(ac, cp);
+    if (d6 >= 0 && d5 <= d6) return cls_norm(cp);
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0 && d2 >= 0 && d6 <= 0) return cls_norm(cls_sub(p, cls_add_scaled(a, d2 / (d2 - d6), ac)));
+    double va = d3 * d6 - d5 * d4;
+    if (va <= 0 && d4 - d3 >= 0 && d5 - d6 >= 0)
+        return cls_norm(cls_sub(p, cls_add_scaled(b, (d4 - d3) / ((d4 - d3) + (d5 - d6)), cls_sub(c, b))));
+    double denom = 1 / (va + vb + vc);
+    return cls_norm(cls_sub(p, cls_add_scaled(cls_add_scaled(a, vb * denom, ab), vc * denom, ac)));
+}
+
+// segment_projection, src/assembly/assemble.jl:145-149
+__device__ inline double cls_seg_projection(V3<double> p, V3<double> a, V3<double> c, double *tout)
+{
+#pragma clang fp contract(off)
+    V3<double> e = cls_sub(c, a);
+    double t = cls_dot(cls_sub(p, a), e) / cls_dot(e, e);
+    t = t < 0.0 ? 0.0 : t > 1.0 ? 1.0 : t;
+    if (tout) *tout = t;
+    return cls_norm(cls_sub(p, cls_add_scaled(a, t, e)));
+}
+
+__device__ __forceinline__ V3<double> ld3(const double *p) { return mk3(p[0], p[1], p[2]); }
+
+}  // namespace vp

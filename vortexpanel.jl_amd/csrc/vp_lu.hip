@@ -1,0 +1,849 @@
+// The bordered dense solve on gfx950: build the bordered matrix on the device, randomize it with a depth-2 recursive
+// butterfly transform, factor it by a blocked LU without pivoting whose trailing updates run on MFMA (FP64 MFMA for
+// the plain FP64 path, FP32 MFMA for the mixed path), and solve with FP64 iterative refinement on the original
+// FP64 matrix.
+//
+// Reference: src/assembly/solve.jl:15-99 (bordered_system, bordered_matrix, solve_galerkin, factorize_galerkin) and
+// src/dofmap/system.jl:26-50 (reduce_system, bordered_rows).  The reference factors with LAPACK getrf (partial
+// pivoting).  Partial pivoting needs n sequential pivot searches across the whole column; here the random butterfly
+// transform (Parker 1995; Baboulin, Dongarra et al. 2013, MAGMA's gesv_rbt) makes pivoting unnecessary with high
+// probability, and refinement in FP64 (as LAPACK's dsgesv) brings the solution to FP64 accuracy.  The condition
+// estimate is Hager-Higham's 1-norm estimator (LAPACK dlacn2, the algorithm behind the reference's gecon call).
+//
+// Layouts: all device matrices are row-major with a leading dimension; right hand sides are n x nrhs row-major.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "vp_capi_internal.hpp"
+
+namespace vp {
+
+#define HIPCHECK2(x)                                                                                                   \
+    do {                                                                                                               \
+        hipError_t e_ = (x);                                                                                           \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_));                \
+    } while (0)
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+// Both 16x16x4 MFMAs take A[m][k] from lane m + 16k and B[k][n] from lane n + 16k.  Their 16x16 results differ:
+// the FP32 one holds D[4(l/16) + v][l%16] in lane l, register v; the FP64 one D[(l/16) + 4v][l%16] (read on gfx950).
+template <> struct Mfma<double> {
+    typedef d4 acc;
+    static __device__ __forceinline__ acc op(double a, double b, acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <> struct Mfma<float> {
+    typedef f4 acc;
+    static __device__ __forceinline__ acc op(float a, float b, acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+// ------------------------------------------------------------------------------------------ GEMM (MFMA) ------------
+
+// C = A B (SET) or C -= A B (SUB), row-major, arbitrary M, N, K.  Workgroup tile BM x BN, four waves in a 2 x 2
+// grid each owning a (BM/2) x (BN/2) sub-tile of 16 x 16 MFMA blocks.  A is staged in LDS transposed (As[k][m]) and B
+// as is (Bs[k][n]), so each lane of an MFMA reads A[m = l%16][k = l/16] and B[k = l/16][n = l%16].  In place
+// (C aliasing B) is allowed when one workgroup row covers all of M (the whole of B's tile is read before C is written),
+// and (C aliasing A) when one workgroup column covers all of N.
+template <typename T, int BM, int BN, int KB, bool SUB>
+__global__ __launch_bounds__(256) void gemm(int M, int N, int K, const T *A, long long lda, const T *B, long long ldb,
+                                            T *C, long long ldc)
+{
+    constexpr int WM = BM / 2, WN = BN / 2, TI = WM / 16, TJ = WN / 16;
+    __shared__ T As[KB][BM + 1];
+    __shared__ T Bs[KB][BN + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const long long m0 = (long long)blockIdx.y * BM, n0 = (long long)blockIdx.x * BN;
+    typename Mfma<T>::acc acc[TI][TJ];
+    for (int i = 0; i < TI; i++)
+        for (int j = 0; j < TJ; j++)
+            for (int v = 0; v < 4; v++) acc[i][j][v] = T(0);
+    for (int k0 = 0; k0 < K; k0 += KB) {
+        for (int idx = tid; idx < BM * KB; idx += 256) {
+            int r = idx / KB, k = idx - r * KB;
+            long long gr = m0 + r;
+            int gk = k0 + k;
+            As[k][r] = (gr < M && gk < K) ? A[gr * lda + gk] : T(0);
+        }
+        for (int idx = tid; idx < KB * BN; idx += 256) {
+            int k = idx / BN, c = idx - k * BN;
+            int gk = k0 + k;
+            long long gc = n0 + c;
+            Bs[k][c] = (gk < K && gc < N) ? B[(long long)gk * ldb + gc] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KB; kk += 4) {
+            T a[TI], b[TJ];
+#pragma unroll
+            for (int i = 0; i < TI; i++) a[i] = As[kk + (lane >> 4)][wr * WM + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < TJ; j++) b[j] = Bs[kk + (lane >> 4)][wc * WN + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < TI; i++)
+#pragma unroll
+                for (int j = 0; j < TJ; j++) acc[i][j] = Mfma<T>::op(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < TI; i++)
+#pragma unroll
+        for (int j = 0; j < TJ; j++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                long long r = m0 + wr * WM + i * 16 + Mfma<T>::row(lane, v);
+                long long c = n0 + wc * WN + j * 16 + (lane & 15);
+                if (r < M && c < N) {
+                    if (SUB)
+                        C[r * ldc + c] -= acc[i][j][v];
+                    else
+                        C[r * ldc + c] = acc[i][j][v];
+                }
+            }
+}
+
+template <typename T> struct Tile;
+template <> struct Tile<double> {
+    static constexpr int BM = 64, BN = 64, KB = 16;
+};
+template <> struct Tile<float> {
+    static constexpr int BM = 128, BN = 128, KB = 16;
+};
+
+template <typename T, bool SUB>
+void launch_gemm(hipStream_t s, int M, int N, int K, const T *A, long long lda, const T *B, long long ldb, T *C, long long ldc)
+{
+    if (M <= 0 || N <= 0 || K <= 0) return;
+    constexpr int BM = Tile<T>::BM, BN = Tile<T>::BN, KB = Tile<T>::KB;
+    dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
+    gemm<T, BM, BN, KB, SUB><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc);
+    HIPCHECK2(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------------ LU pieces --------------
+
+constexpr int NB = 64;  // panel width
+
+// Factor the NB x NB diagonal block at (k0, k0) in place without pivoting (D = L U, L unit lower), and write the
+// inverses of L and U (NB x NB, row-major) for the panel's triangular solves.  A zero or non-finite pivot sets *bad.
+template <typename T>
+__global__ __launch_bounds__(256) void diag_lu(T *G, long long ld, int k0, int nb, T *Linv, T *Uinv, int *bad)
+{
+    __shared__ T D[NB][NB + 1];
+    __shared__ T X[NB][NB + 1];
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < NB * NB; idx += 256) {
+        int i = idx / NB, j = idx % NB;
+        D[i][j] = (i < nb && j < nb) ? G[(long long)(k0 + i) * ld + k0 + j] : T(i == j ? 1 : 0);
+    }
+    __syncthreads();
+    for (int k = 0; k < nb; k++) {
+        T p = D[k][k];
+        if (tid == 0 && (!(p != T(0)) || !isfinite((double)p))) atomicExch(bad, 1);
+        for (int i = k + 1 + tid; i < nb; i += 256) D[i][k] /= p;
+        __syncthreads();
+        int m = nb - k - 1;
+        for (int idx = tid; idx < m * m; idx += 256) {
+            int i = k + 1 + idx / m, j = k + 1 + idx % m;
+            D[i][j] -= D[i][k] * D[k][j];
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < nb * nb; idx += 256) {
+        int i = idx / nb, j = idx % nb;
+        G[(long long)(k0 + i) * ld + k0 + j] = D[i][j];
+    }
+    // inverse of the unit lower L: column j by forward substitution, one thread per column
+    for (int j = tid; j < NB; j += 256) {
+        for (int i = 0; i < NB; i++) {
+            T s = T(i == j ? 1 : 0);
+            for (int k = j; k < i; k++) s -= D[i][k] * X[k][j];
+            X[i][j] = i < j ? T(0) : s;
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < NB * NB; idx += 256) Linv[idx] = X[idx / NB][idx % NB];
+    __syncthreads();
+    // inverse of the upper U: column j by back substitution
+    for (int j = tid; j < NB; j += 256) {
+        for (int i = NB - 1; i >= 0; i--) {
+            if (i > j) {
+                X[i][j] = T(0);
+                continue;
+            }
+            T s = T(i == j ? 1 : 0);
+            for (int k = i + 1; k <= j; k++) s -= D[i][k] * X[k][j];
+            X[i][j] = s / D[i][i];
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < NB * NB; idx += 256) Uinv[idx] = X[idx / NB][idx % NB];
+}
+
+// ------------------------------------------------------------------------------------------ bordered matrix --------
+
+// M (row-major, n x n, ld) rows/cols < nq: M[p][q] = sum_i sum_j C[i,p] Cf[j,q] A[i + j lda]  (Cᵀ A [C; W]),
+// with C given by column (CSC: cp/ci/cv over nq columns) and Cf = [C; W] by column (fp/fi/fv).
+// Threads run along p (coalesced reads of A's columns); a 64 x 64 LDS tile transposes to row-major writes.
+__global__ __launch_bounds__(256) void reduce_tile(const double *__restrict__ A, long long lda, int nq,
+                                                   const int *__restrict__ cp, const int *__restrict__ ci,
+                                                   const double *__restrict__ cv, const int *__restrict__ fp,
+                                                   const int *__restrict__ fi, const double *__restrict__ fv,
+                                                   double *__restrict__ M, long long ld)
+{
+    __shared__ double t[64][65];
+    const int p0 = blockIdx.x * 64, q0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int p = p0 + tx;
+    for (int qq = ty; qq < 64; qq += 4) {
+        int q = q0 + qq;
+        double s = 0.0;
+        if (p < nq && q < nq) {
+            for (int a = fp[q]; a < fp[q + 1]; a++) {
+                const double *col = A + (long long)fi[a] * lda;
+                double cs = 0.0;
+                for (int b = cp[p]; b < cp[p + 1]; b++) cs += cv[b] * col[ci[b]];
+                s += fv[a] * cs;
+            }
+        }
+        t[qq][tx] = s;
+    }
+    __syncthreads();
+    for (int pp = ty; pp < 64; pp += 4) {
+        int pr = p0 + pp, qc = q0 + tx;
+        if (pr < nq && qc < nq) M[(long long)pr * ld + qc] = t[tx][pp];
+    }
+}
+
+__global__ void diag_abs_sum(const double *M, long long ld, int nq, double *out)
+{
+    __shared__ double s[256];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nq; i += 256) v += fabs(M[(long long)i * ld + i]);
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s[0];
+}
+
+// border rows/columns (solve.jl:30-35) and the identity on the padding
+__global__ void border_fill(double *M, long long ld, int nq, int nb, int npad, const double *e, const double *c,
+                            const double *scale_e, const double *scale_c)
+{
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // part 1: columns nq..nq+nb-1 for rows < nq, and rows nq..nq+nb-1 for columns < nq
+    if (idx < (long long)nq * nb) {
+        int k = (int)(idx / nq), p = (int)(idx % nq);
+        M[(long long)p * ld + nq + k] = scale_e[k] * e[(long long)k * nq + p];
+        M[(long long)(nq + k) * ld + p] = scale_c[k] * c[(long long)k * nq + p];
+    }
+    // part 2: the (nb x nb) corner is zero; padding rows/columns: zero with 1 on the diagonal
+    long long j = idx;
+    int n = nq + nb;
+    if (j < (long long)nb * nb) M[(long long)(nq + j / nb) * ld + nq + j % nb] = 0.0;
+    if (j < (long long)(npad - n) * npad) {
+        int r = n + (int)(j / npad), cc = (int)(j % npad);
+        M[(long long)r * ld + cc] = r == cc ? 1.0 : 0.0;
+        M[(long long)cc * ld + r] = r == cc ? 1.0 : 0.0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------ butterflies -----------
+
+// Depth-2 recursive butterfly on both sides, G' = Uᵀ M V (one pass: every thread owns a 4 x 4 group of entries at
+// rows {i, i+h, i+2h, i+3h} and columns {j, j+h, j+2h, j+3h}, h = npad/4).  ru/rv hold, per side, the level-1
+// diagonals (R, S of size npad/2) then the level-2 ones (Ra, Sa, Rb, Sb of size npad/4).
+__device__ __forceinline__ void bfly4(double *v, const double *r, int i, int h)
+{
+    // level 2 within halves, then level 1:  (B^T x)_top = R (x_top + x_bot)/sqrt2, (B^T x)_bot = S (x_top - x_bot)/sqrt2
+    const double s2 = 0.70710678118654752440;
+    const double *R1 = r, *S1 = r + 2 * h, *Ra = r + 4 * h, *Sa = r + 5 * h, *Rb = r + 6 * h, *Sb = r + 7 * h;
+    double u0 = Ra[i] * (v[0] + v[1]) * s2, u1 = Sa[i] * (v[0] - v[1]) * s2;
+    double u2 = Rb[i] * (v[2] + v[3]) * s2, u3 = Sb[i] * (v[2] - v[3]) * s2;
+    v[0] = R1[i] * (u0 + u2) * s2;
+    v[1] = R1[i + h] * (u1 + u3) * s2;
+    v[2] = S1[i] * (u0 - u2) * s2;
+    v[3] = S1[i + h] * (u1 - u3) * s2;
+}
+
+template <typename T>
+__global__ void rbt_both(const double *__restrict__ M, long long ld, int npad, const double *__restrict__ ru,
+                         const double *__restrict__ rv, T *__restrict__ G, long long ldg)
+{
+    const int h = npad / 4;
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)h * h) return;
+    int i = (int)(idx / h), j = (int)(idx % h);
+    double m[4][4];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) m[a][b] = M[(long long)(i + a * h) * ld + j + b * h];
+    for (int b = 0; b < 4; b++) {  // left: columns of the group
+        double v[4] = {m[0][b], m[1][b], m[2][b], m[3][b]};
+        bfly4(v, ru, i, h);
+        for (int a = 0; a < 4; a++) m[a][b] = v[a];
+    }
+    for (int a = 0; a < 4; a++) bfly4(m[a], rv, j, h);  // right: rows of the group (M V acts like B^T on rows)
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) G[(long long)(i + a * h) * ldg + j + b * h] = (T)m[a][b];
+}
+
+// Left butterfly on a block of right hand sides (npad x nrhs, row-major): out = Uᵀ in
+__global__ void rbt_rows(const double *__restrict__ in, int npad, int nrhs, const double *__restrict__ ru, double *out)
+{
+    const int h = npad / 4;
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)h * nrhs) return;
+    int i = (int)(idx / nrhs), c = (int)(idx % nrhs);
+    double v[4];
+    for (int a = 0; a < 4; a++) v[a] = in[(long long)(i + a * h) * nrhs + c];
+    bfly4(v, ru, i, h);
+    for (int a = 0; a < 4; a++) out[(long long)(i + a * h) * nrhs + c] = v[a];
+}
+
+// x = V y: V = diag(C2a, C2b) C1; (C1 y)_top = (R y_top + S y_bot)/sqrt2, (C1 y)_bot = (R y_top - S y_bot)/sqrt2
+__global__ void rbt_apply_v(const double *__restrict__ y, int npad, int nrhs, const double *__restrict__ rv, double *x)
+{
+    const int h = npad / 4;
+    const double s2 = 0.70710678118654752440;
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)h * nrhs) return;
+    int i = (int)(idx / nrhs), c = (int)(idx % nrhs);
+    const double *R1 = rv, *S1 = rv + 2 * h, *Ra = rv + 4 * h, *Sa = rv + 5 * h, *Rb = rv + 6 * h, *Sb = rv + 7 * h;
+    double v[4];
+    for (int a = 0; a < 4; a++) v[a] = y[(long long)(i + a * h) * nrhs + c];
+    // level 1 on index pairs (i, i+2h) and (i+h, i+3h)
+    double w0 = (R1[i] * v[0] + S1[i] * v[2]) * s2, w2 = (R1[i] * v[0] - S1[i] * v[2]) * s2;
+    double w1 = (R1[i + h] * v[1] + S1[i + h] * v[3]) * s2, w3 = (R1[i + h] * v[1] - S1[i + h] * v[3]) * s2;
+    // level 2 within halves: pairs (i, i+h) and (i+2h, i+3h)
+    x[(long long)i * nrhs + c] = (Ra[i] * w0 + Sa[i] * w1) * s2;
+    x[(long long)(i + h) * nrhs + c] = (Ra[i] * w0 - Sa[i] * w1) * s2;
+    x[(long long)(i + 2 * h) * nrhs + c] = (Rb[i] * w2 + Sb[i] * w3) * s2;
+    x[(long long)(i + 3 * h) * nrhs + c] = (Rb[i] * w2 - Sb[i] * w3) * s2;
+}
+
+template <typename S, typename D> __global__ void convert(const S *in, D *out, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (D)in[i];
+}
+
+__global__ void axpy_inplace(double *x, const double *dx, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += dx[i];
+}
+
+// per right hand side: max |r|, max |x| (columns of n x nrhs row-major), into out[2*c], out[2*c+1]
+__global__ void colmax2(const double *r, const double *x, int n, int nrhs, double *out)
+{
+    __shared__ double s1[256], s2[256];
+    int c = blockIdx.x;
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        a = fmax(a, fabs(r[(long long)i * nrhs + c]));
+        b = fmax(b, fabs(x[(long long)i * nrhs + c]));
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + o]);
+            s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * c] = s1[0];
+        out[2 * c + 1] = s2[0];
+    }
+}
+
+// infinity norm of rows (max over rows of the sum of |M[i][:]|): partial per block
+__global__ void row_abs_max(const double *M, long long ld, int n, double *out)
+{
+    __shared__ double s[256];
+    double best = 0.0;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        double v = 0.0;
+        for (int j = threadIdx.x; j < n; j += 256) v += fabs(M[(long long)i * ld + j]);
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) best = fmax(best, s[0]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = best;
+}
+
+// column sums of |M| (the 1-norm's columns), one thread per column
+__global__ void col_abs_sum(const double *M, long long ld, int n, double *out)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double v = 0.0;
+    for (int i = 0; i < n; i++) v += fabs(M[(long long)i * ld + j]);
+    out[j] = v;
+}
+
+// ------------------------------------------------------------------------------------------ host side --------------
+
+template <typename T> T *dmalloc(size_t n)
+{
+    T *p = nullptr;
+    HIPCHECK2(hipMalloc((void **)&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return p;
+}
+
+Factor::~Factor()
+{
+    delete tfac;
+    for (void *p : owned) (void)hipFree(p);
+}
+
+static unsigned blocks(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+template <typename T> void lu_factor(hipStream_t s, T *G, long long ld, int npad, T *linv, T *uinv, int *bad)
+{
+    for (int k0 = 0; k0 < npad; k0 += NB) {
+        int nb = std::min(NB, npad - k0);
+        diag_lu<T><<<1, 256, 0, s>>>(G, ld, k0, nb, linv + (size_t)(k0 / NB) * NB * NB, uinv + (size_t)(k0 / NB) * NB * NB, bad);
+        HIPCHECK2(hipGetLastError());
+        int m = npad - k0 - nb;
+        if (m <= 0) break;
+        T *D = G + (long long)k0 * ld + k0;
+        // L21 = A21 U11^-1 (in place: one workgroup column covers the panel's NB columns)
+        launch_gemm<T, false>(s, m, nb, nb, D + (long long)nb * ld, ld, uinv + (size_t)(k0 / NB) * NB * NB, NB,
+                              D + (long long)nb * ld, ld);
+        // U12 = L11^-1 A12 (in place: one workgroup row covers the panel's NB rows)
+        launch_gemm<T, false>(s, nb, m, nb, linv + (size_t)(k0 / NB) * NB * NB, NB, D + nb, ld, D + nb, ld);
+        // A22 -= L21 U12
+        launch_gemm<T, true>(s, m, m, nb, D + (long long)nb * ld, ld, D + nb, ld, D + (long long)nb * ld + nb, ld);
+    }
+}
+
+// X (npad x nrhs, row-major, in place) <- U^-1 L^-1 X
+template <typename T> void lu_solve(hipStream_t s, const T *G, long long ld, int npad, const T *linv, const T *uinv, T *X, int nrhs)
+{
+    for (int k0 = 0; k0 < npad; k0 += NB) {
+        int nb = std::min(NB, npad - k0);
+        T *xk = X + (long long)k0 * nrhs;
+        launch_gemm<T, false>(s, nb, nrhs, nb, linv + (size_t)(k0 / NB) * NB * NB, NB, xk, nrhs, xk, nrhs);
+        int m = npad - k0 - nb;
+        if (m > 0) launch_gemm<T, true>(s, m, nrhs, nb, G + (long long)(k0 + nb) * ld + k0, ld, xk, nrhs, xk + (long long)nb * nrhs, nrhs);
+    }
+    int last = ((npad - 1) / NB) * NB;
+    for (int k0 = last; k0 >= 0; k0 -= NB) {
+        int nb = std::min(NB, npad - k0);
+        T *xk = X + (long long)k0 * nrhs;
+        launch_gemm<T, false>(s, nb, nrhs, nb, uinv + (size_t)(k0 / NB) * NB * NB, NB, xk, nrhs, xk, nrhs);
+        if (k0 > 0) launch_gemm<T, true>(s, k0, nrhs, nb, G + k0, ld, xk, nrhs, X, nrhs);
+    }
+}
+
+void Factor::build(Ctx *c, const double *A, long long lda, const vp_constraints &k, int mode_)
+{
+    ctx = c;
+    mode = mode_;
+    hipStream_t s = c->stream;
+    if (mode != 0 && mode != 1) throw std::invalid_argument("mode must be 0 (fp64) or 1 (mixed)");
+    nslot = k.nslot;
+    nq = k.nq;
+    nw = k.nw;
+    nb = k.nb;
+    if (nq <= 0 || nslot <= 0 || nb < 0) throw std::invalid_argument("empty system");
+    n = nq + nb;
+    npad = ((n + 255) / 256) * 256;
+    ld = npad;
+    // CSC of C (nslot x nq) and of Cf = [C; W] ((nslot + nw) x nq) from the CSR inputs
+    std::vector<std::vector<std::pair<int, double>>> ccol(nq), fcol(nq);
+    for (int i = 0; i < nslot; i++)
+        for (int a = k.C_rowptr[i]; a < k.C_rowptr[i + 1]; a++) {
+            int q = k.C_col[a];
+            if (q < 0 || q >= nq) throw std::out_of_range("C column index out of range");
+            ccol[q].push_back({i, k.C_val[a]});
+            fcol[q].push_back({i, k.C_val[a]});
+        }
+    for (int i = 0; i < nw; i++)
+        for (int a = k.W_rowptr[i]; a < k.W_rowptr[i + 1]; a++) {
+            int q = k.W_col[a];
+            if (q < 0 || q >= nq) throw std::out_of_range("W column index out of range");
+            fcol[q].push_back({nslot + i, k.W_val[a]});
+        }
+    auto csc = [](const std::vector<std::vector<std::pair<int, double>>> &cols, std::vector<int> &p, std::vector<int> &ix,
+                  std::vector<double> &v) {
+        p.assign(1, 0);
+        for (auto &col : cols) {
+            for (auto &e : col) {
+                ix.push_back(e.first);
+                v.push_back(e.second);
+            }
+            p.push_back((int)ix.size());
+        }
+    };
+    std::vector<int> cp, ci, fp, fi;
+    std::vector<double> cv, fv;
+    csc(ccol, cp, ci, cv);
+    csc(fcol, fp, fi, fv);
+    auto up = [&](auto &vec) {
+        using E = typename std::decay<decltype(vec)>::type::value_type;
+        E *d = dmalloc<E>(vec.size());
+        if (!vec.empty()) HIPCHECK2(hipMemcpyAsync(d, vec.data(), vec.size() * sizeof(E), hipMemcpyHostToDevice, s));
+        owned.push_back(d);
+        return d;
+    };
+    int *dcp = up(cp), *dci = up(ci), *dfp = up(fp), *dfi = up(fi);
+    double *dcv = up(cv), *dfv = up(fv);
+    M = dmalloc<double>((size_t)ld * npad);
+    owned.push_back(M);
+    HIPCHECK2(hipMemsetAsync(M, 0, sizeof(double) * (size_t)ld * npad, s));
+    dim3 g((nq + 63) / 64, (nq + 63) / 64);
+    reduce_tile<<<g, 256, 0, s>>>(A, lda, nq, dcp, dci, dcv, dfp, dfi, dfv, M, ld);
+    HIPCHECK2(hipGetLastError());
+    // border scaling: scale = sum |diag(Ar)| / nq; e and c scaled to 2-norm = scale (solve.jl:30-35)
+    double *dsum = dmalloc<double>(1);
+    owned.push_back(dsum);
+    diag_abs_sum<<<1, 256, 0, s>>>(M, ld, nq, dsum);
+    double hsum = 0.0;
+    HIPCHECK2(hipMemcpyAsync(&hsum, dsum, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK2(hipStreamSynchronize(s));
+    double scale = hsum / nq;
+    se.assign(nb, 0.0);
+    std::vector<double> sc(nb);
+    for (int b = 0; b < nb; b++) {
+        double ne = 0.0, nc = 0.0;
+        for (int p = 0; p < nq; p++) {
+            ne += k.e[(size_t)b * nq + p] * k.e[(size_t)b * nq + p];
+            nc += k.c[(size_t)b * nq + p] * k.c[(size_t)b * nq + p];
+        }
+        se[b] = scale / std::sqrt(ne);
+        sc[b] = scale / std::sqrt(nc);
+    }
+    std::vector<double> ev(k.e, k.e + (size_t)nb * nq), cvv(k.c, k.c + (size_t)nb * nq);
+    double *de = up(ev), *dc = up(cvv), *dse = up(se), *dsc = up(sc);
+    long long work = std::max<long long>((long long)nq * nb, (long long)(npad - n) * npad);
+    work = std::max<long long>(work, (long long)nb * nb);
+    if (work > 0) border_fill<<<blocks(work), 256, 0, s>>>(M, ld, nq, nb, npad, de, dc, dse, dsc);
+    HIPCHECK2(hipGetLastError());
+    // the random butterflies (fixed seed: the factorization is deterministic)
+    std::mt19937_64 rng(20261203ull);
+    std::uniform_real_distribution<double> U(-0.5, 0.5);
+    std::vector<double> ru(2 * npad), rv(2 * npad);
+    for (auto &x : ru) x = std::exp(U(rng) / 10);
+    for (auto &x : rv) x = std::exp(U(rng) / 10);
+    d_ru = up(ru);
+    d_rv = up(rv);
+    // 1-norm of M (for the condition estimate): column sums
+    double *dcol = dmalloc<double>(npad);
+    owned.push_back(dcol);
+    col_abs_sum<<<blocks(n), 256, 0, s>>>(M, ld, n, dcol);
+    std::vector<double> hcol(n);
+    HIPCHECK2(hipMemcpyAsync(hcol.data(), dcol, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIPCHECK2(hipStreamSynchronize(s));
+    anorm1 = *std::max_element(hcol.begin(), hcol.end());
+    // randomized copy in the factor type, then LU
+    int *dbad = dmalloc<int>(1);
+    owned.push_back(dbad);
+    HIPCHECK2(hipMemsetAsync(dbad, 0, sizeof(int), s));
+    size_t nblk = (size_t)(npad / NB) * NB * NB;
+    long long h = npad / 4;
+    if (mode == 0) {
+        G64 = dmalloc<double>((size_t)ld * npad);
+        linv64 = dmalloc<double>(nblk);
+        uinv64 = dmalloc<double>(nblk);
+        owned.insert(owned.end(), {G64, linv64, uinv64});
+        rbt_both<double><<<blocks(h * h), 256, 0, s>>>(M, ld, npad, d_ru, d_rv, G64, ld);
+        HIPCHECK2(hipGetLastError());
+        lu_factor<double>(s, G64, ld, npad, linv64, uinv64, dbad);
+    } else {
+        G32 = dmalloc<float>((size_t)ld * npad);
+        linv32 = dmalloc<float>(nblk);
+        uinv32 = dmalloc<float>(nblk);
+        owned.insert(owned.end(), {G32, linv32, uinv32});
+        rbt_both<float><<<blocks(h * h), 256, 0, s>>>(M, ld, npad, d_ru, d_rv, G32, ld);
+        HIPCHECK2(hipGetLastError());
+        lu_factor<float>(s, G32, ld, npad, linv32, uinv32, dbad);
+    }
+    int hbad = 0;
+    HIPCHECK2(hipMemcpyAsync(&hbad, dbad, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK2(hipStreamSynchronize(s));
+    if (hbad) throw std::domain_error("zero or non-finite pivot in the randomized LU (singular bordered matrix)");
+    ws_nrhs = 0;
+}
+
+void Factor::ensure_ws(int nrhs)
+{
+    if (nrhs <= ws_nrhs) return;
+    for (void *p : {(void *)X, (void *)R, (void *)Y, (void *)B, (void *)Xf, (void *)nrm}) {
+        if (!p) continue;
+        owned.erase(std::find(owned.begin(), owned.end(), p));
+        (void)hipFree(p);
+    }
+    size_t sz = (size_t)npad * nrhs;
+    X = dmalloc<double>(sz);
+    R = dmalloc<double>(sz);
+    Y = dmalloc<double>(sz);
+    B = dmalloc<double>(sz);
+    Xf = dmalloc<float>(sz);
+    nrm = dmalloc<double>(2 * (size_t)nrhs);
+    owned.insert(owned.end(), {X, R, Y, B, Xf, nrm});
+    ws_nrhs = nrhs;
+}
+
+// y = M^-1 r for the current workspace block (in: Y holds r, out: Y holds M^-1 r), npad x nrhs
+void Factor::apply_inverse(int nrhs)
+{
+    hipStream_t s = ctx->stream;
+    long long h = npad / 4;
+    rbt_rows<<<blocks(h * nrhs), 256, 0, s>>>(Y, npad, nrhs, d_ru, R);
+    HIPCHECK2(hipGetLastError());
+    if (mode == 0) {
+        lu_solve<double>(s, G64, ld, npad, linv64, uinv64, R, nrhs);
+    } else {
+        convert<double, float><<<blocks((long long)npad * nrhs), 256, 0, s>>>(R, Xf, (long long)npad * nrhs);
+        lu_solve<float>(s, G32, ld, npad, linv32, uinv32, Xf, nrhs);
+        convert<float, double><<<blocks((long long)npad * nrhs), 256, 0, s>>>(Xf, R, (long long)npad * nrhs);
+    }
+    rbt_apply_v<<<blocks(h * nrhs), 256, 0, s>>>(R, npad, nrhs, d_rv, Y);
+    HIPCHECK2(hipGetLastError());
+}
+
+// Solve M X = Bh (n x nrhs row-major host or device) with FP64 iterative refinement (LAPACK dsgesv's stopping test:
+// |r|_inf <= |x|_inf |M|_inf eps sqrt(n), at most max_iter refinements).
+int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int max_iter, double *resid_out)
+{
+    hipStream_t s = ctx->stream;
+    ensure_ws(nrhs);
+    size_t nn = (size_t)n * nrhs, sz = (size_t)npad * nrhs;
+    HIPCHECK2(hipMemsetAsync(B, 0, sizeof(double) * sz, s));
+    HIPCHECK2(hipMemcpyAsync(B, Bin, sizeof(double) * nn, device_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    HIPCHECK2(hipMemcpyAsync(Y, B, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
+    apply_inverse(nrhs);
+    HIPCHECK2(hipMemcpyAsync(X, Y, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
+    if (anorm_inf < 0) {
+        double *dpart = dmalloc<double>(1024);
+        row_abs_max<<<1024, 256, 0, s>>>(M, ld, n, dpart);
+        std::vector<double> part(1024);
+        HIPCHECK2(hipMemcpyAsync(part.data(), dpart, sizeof(double) * 1024, hipMemcpyDeviceToHost, s));
+        HIPCHECK2(hipStreamSynchronize(s));
+        (void)hipFree(dpart);
+        anorm_inf = *std::max_element(part.begin(), part.end());
+    }
+    const double eps = 1.1102230246251565e-16, tolf = anorm_inf * eps * std::sqrt((double)n);
+    std::vector<double> hn(2 * (size_t)nrhs);
+    int it = 0;
+    double worst = 0.0;
+    for (;; it++) {
+        // R = B - M X (FP64, MFMA)
+        HIPCHECK2(hipMemcpyAsync(Y, B, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
+        launch_gemm<double, true>(s, npad, nrhs, npad, M, ld, X, nrhs, Y, nrhs);
+        colmax2<<<nrhs, 256, 0, s>>>(Y, X, n, nrhs, nrm);
+        HIPCHECK2(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, s));
+        HIPCHECK2(hipStreamSynchronize(s));
+        bool done = true;
+        worst = 0.0;
+        for (int c = 0; c < nrhs; c++) {
+            if (hn[2 * c] > hn[2 * c + 1] * tolf) done = false;
+            worst = std::max(worst, hn[2 * c] / std::max(hn[2 * c + 1] * anorm_inf, 1e-300));
+        }
+        if (done || it >= max_iter) break;
+        apply_inverse(nrhs);
+        axpy_inplace<<<blocks((long long)sz), 256, 0, s>>>(X, Y, (long long)sz);
+        HIPCHECK2(hipGetLastError());
+    }
+    if (resid_out) *resid_out = worst;
+    HIPCHECK2(hipMemcpyAsync(Xout, X, sizeof(double) * nn, device_io ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    HIPCHECK2(hipStreamSynchronize(s));
+    last_iters = it;
+    return it;
+}
+
+// Hager-Higham estimate of |M^-1|_1 (the iteration of LAPACK dlacn2, which gecon runs on the reference's factors).
+// It needs products with M^-1 and M^-T; M^-T comes from a second randomized factorization, of Mᵀ, made on first use.
+double Factor::cond_estimate()
+{
+    hipStream_t s = ctx->stream;
+    if (!Mt) {
+        Mt = dmalloc<double>((size_t)ld * npad);
+        owned.push_back(Mt);
+        transpose_into(M, Mt);
+    }
+    std::vector<double> x(n, 1.0 / n), y(n), z(n);
+    double est = 0.0;
+    int jlast = -1;
+    for (int iter = 0; iter < 5; iter++) {
+        solve(x.data(), 1, y.data(), false, 30, nullptr);  // y = M^-1 x
+        double ny = 0.0;
+        for (double v : y) ny += std::fabs(v);
+        if (iter > 0 && ny <= est) {
+            est = std::max(est, ny);
+            break;
+        }
+        est = ny;
+        for (int i = 0; i < n; i++) z[i] = y[i] >= 0 ? 1.0 : -1.0;
+        solve_transposed(z.data(), x.data());  // x = M^-T z
+        int j = 0;
+        for (int i = 1; i < n; i++)
+            if (std::fabs(x[i]) > std::fabs(x[j])) j = i;
+        if (j == jlast) break;
+        jlast = j;
+        std::fill(x.begin(), x.end(), 0.0);
+        x[j] = 1.0;
+    }
+    // the alternative estimate of dlacn2 (altsgn)
+    std::vector<double> alt(n);
+    for (int i = 0; i < n; i++) alt[i] = (i % 2 ? -1.0 : 1.0) * (1.0 + (double)i / (n > 1 ? n - 1 : 1));
+    solve(alt.data(), 1, y.data(), false, 30, nullptr);
+    double t = 0.0;
+    for (double v : y) t += std::fabs(v);
+    t = 2 * t / (3.0 * n);
+    est = std::max(est, t);
+    (void)s;
+    return est;
+}
+
+__global__ void transpose_kernel(const double *in, double *out, long long ld, int npad)
+{
+    __shared__ double t[32][33];
+    int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) t[r][tx] = in[(long long)(by + r) * ld + bx + tx];
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) out[(long long)(bx + r) * ld + by + tx] = t[tx][r];
+}
+
+void Factor::transpose_into(const double *in, double *out)
+{
+    dim3 g(npad / 32, npad / 32);
+    transpose_kernel<<<g, 256, 0, ctx->stream>>>(in, out, ld, npad);
+    HIPCHECK2(hipGetLastError());
+}
+
+// x = M^-T z with the factorization of Mᵀ (same butterflies-and-LU scheme, FP64 refinement against Mᵀ)
+void Factor::solve_transposed(const double *z, double *x)
+{
+    if (!tfac) {
+        tfac = new Factor();
+        tfac->ctx = ctx;
+        tfac->init_from_dense(Mt, ld, n, npad, mode, anorm1);
+    }
+    tfac->solve(z, 1, x, false, 30, nullptr);
+}
+
+// Factor a dense bordered matrix already on the device (n x n in an npad x npad row-major buffer).
+void Factor::init_from_dense(const double *Md, long long ldd, int n_, int npad_, int mode_, double anorm)
+{
+    hipStream_t s = ctx->stream;
+    mode = mode_;
+    n = n_;
+    npad = npad_;
+    ld = ldd;
+    nq = n;
+    nb = 0;
+    anorm1 = anorm;
+    M = const_cast<double *>(Md);
+    std::mt19937_64 rng(20261204ull);
+    std::uniform_real_distribution<double> U(-0.5, 0.5);
+    std::vector<double> ru(2 * npad), rv(2 * npad);
+    for (auto &v : ru) v = std::exp(U(rng) / 10);
+    for (auto &v : rv) v = std::exp(U(rng) / 10);
+    d_ru = dmalloc<double>(ru.size());
+    d_rv = dmalloc<double>(rv.size());
+    owned.insert(owned.end(), {d_ru, d_rv});
+    HIPCHECK2(hipMemcpyAsync(d_ru, ru.data(), sizeof(double) * ru.size(), hipMemcpyHostToDevice, s));
+    HIPCHECK2(hipMemcpyAsync(d_rv, rv.data(), sizeof(double) * rv.size(), hipMemcpyHostToDevice, s));
+    int *dbad = dmalloc<int>(1);
+    owned.push_back(dbad);
+    HIPCHECK2(hipMemsetAsync(dbad, 0, sizeof(int), s));
+    size_t nblk = (size_t)(npad / NB) * NB * NB;
+    long long h = npad / 4;
+    if (mode == 0) {
+        G64 = dmalloc<double>((size_t)ld * npad);
+        linv64 = dmalloc<double>(nblk);
+        uinv64 = dmalloc<double>(nblk);
+        owned.insert(owned.end(), {G64, linv64, uinv64});
+        rbt_both<double><<<blocks(h * h), 256, 0, s>>>(M, ld, npad, d_ru, d_rv, G64, ld);
+        lu_factor<double>(s, G64, ld, npad, linv64, uinv64, dbad);
+    } else {
+        G32 = dmalloc<float>((size_t)ld * npad);
+        linv32 = dmalloc<float>(nblk);
+        uinv32 = dmalloc<float>(nblk);
+        owned.insert(owned.end(), {G32, linv32, uinv32});
+        rbt_both<float><<<blocks(h * h), 256, 0, s>>>(M, ld, npad, d_ru, d_rv, G32, ld);
+        lu_factor<float>(s, G32, ld, npad, linv32, uinv32, dbad);
+    }
+    HIPCHECK2(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------------ test hooks ------------
+
+void gemm_host(hipStream_t s, int prec, int sub, int M_, int N_, int K_, const double *A, const double *B, double *C)
+{
+    size_t na = (size_t)M_ * K_, nbb = (size_t)K_ * N_, nc = (size_t)M_ * N_;
+    double *dA = dmalloc<double>(na), *dB = dmalloc<double>(nbb), *dC = dmalloc<double>(nc);
+    HIPCHECK2(hipMemcpyAsync(dA, A, na * 8, hipMemcpyHostToDevice, s));
+    HIPCHECK2(hipMemcpyAsync(dB, B, nbb * 8, hipMemcpyHostToDevice, s));
+    HIPCHECK2(hipMemcpyAsync(dC, C, nc * 8, hipMemcpyHostToDevice, s));
+    if (prec == 0) {
+        if (sub)
+            launch_gemm<double, true>(s, M_, N_, K_, dA, K_, dB, N_, dC, N_);
+        else
+            launch_gemm<double, false>(s, M_, N_, K_, dA, K_, dB, N_, dC, N_);
+    } else {
+        float *fA = dmalloc<float>(na), *fB = dmalloc<float>(nbb), *fC = dmalloc<float>(nc);
+        convert<double, float><<<blocks(na), 256, 0, s>>>(dA, fA, na);
+        convert<double, float><<<blocks(nbb), 256, 0, s>>>(dB, fB, nbb);
+        convert<double, float><<<blocks(nc), 256, 0, s>>>(dC, fC, nc);
+        if (sub)
+            launch_gemm<float, true>(s, M_, N_, K_, fA, K_, fB, N_, fC, N_);
+        else
+            launch_gemm<float, false>(s, M_, N_, K_, fA, K_, fB, N_, fC, N_);
+        convert<float, double><<<blocks(nc), 256, 0, s>>>(fC, dC, nc);
+        HIPCHECK2(hipStreamSynchronize(s));
+        (void)hipFree(fA);
+        (void)hipFree(fB);
+        (void)hipFree(fC);
+    }
+    HIPCHECK2(hipMemcpyAsync(C, dC, nc * 8, hipMemcpyDeviceToHost, s));
+    HIPCHECK2(hipStreamSynchronize(s));
+    (void)hipFree(dA);
+    (void)hipFree(dB);
+    (void)hipFree(dC);
+}
+
+int dense_solve_host(Ctx *c, int n, const double *Mh, int mode, const double *r, double *x)
+{
+    hipStream_t s = c->stream;
+    int npad = ((n + 255) / 256) * 256;
+    double *dM = dmalloc<double>((size_t)npad * npad);
+    std::vector<double> hM((size_t)npad * npad, 0.0);
+    for (int i = 0; i < npad; i++)
+        for (int j = 0; j < npad; j++) hM[(size_t)i * npad + j] = (i < n && j < n) ? Mh[(size_t)i * n + j] : (i == j ? 1.0 : 0.0);
+    HIPCHECK2(hipMemcpyAsync(dM, hM.data(), hM.size() * 8, hipMemcpyHostToDevice, s));
+    Factor f;
+    f.ctx = c;
+    f.owned.push_back(dM);
+    f.init_from_dense(dM, npad, n, npad, mode, 0.0);
+    int it = f.solve(r, 1, x, false, 30, nullptr);
+    return it;
+}
+}  // namespace vp

@@ -1,0 +1,66 @@
+"""ctypes binding of libvortexhip.so (include/vortexhip.h).  The library must be built (vphip.build); there is no
+fallback: without it every entry point raises."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBPATH = os.path.join(PKG, "lib", "libvortexhip.so")
+
+VP_OK, VP_E_ARGUMENT, VP_E_DIMENSION, VP_E_DEVICE, VP_E_SINGULAR = 0, -1, -2, -3, -4
+
+
+class AssemblyOptionsC(C.Structure):
+    _fields_ = [("test", C.c_int), ("far_ratio", C.c_double), ("far_points", C.c_int), ("near_points", C.c_int),
+                ("near_ratio", C.c_double), ("near_depth", C.c_int), ("touch_levels", C.c_int),
+                ("touch_ratio", C.c_double), ("touch_order", C.c_int), ("corner_angle", C.c_double),
+                ("contact_rel", C.c_double), ("kernel_far", C.c_double), ("precision", C.c_int)]
+
+
+class SurfaceC(C.Structure):
+    _fields_ = [("nn", C.c_int), ("x", C.POINTER(C.c_double)), ("nt", C.c_int), ("tris", C.POINTER(C.c_int)),
+                ("corner_slot", C.POINTER(C.c_int)), ("nslot", C.c_int), ("ntouch", C.c_int),
+                ("touch", C.POINTER(C.c_int))]
+
+
+class WakeC(C.Structure):
+    _fields_ = [("nw", C.c_int), ("x", C.POINTER(C.c_double)), ("ntri", C.c_int), ("tris", C.POINTER(C.c_int)),
+                ("nfil", C.c_int), ("filaments", C.POINTER(C.c_int))]
+
+
+_lib = None
+
+# every symbol include/vortexhip.h declares
+EXPORTS = ["vp_create", "vp_destroy", "vp_last_error", "vp_sync", "vp_assemble", "vp_plan_create", "vp_plan_assemble",
+           "vp_plan_read", "vp_plan_create_ex", "vp_plan_tiers", "vp_plan_device_A", "vp_plan_destroy", "vp_factorize",
+           "vp_factor_solve", "vp_factor_cond", "vp_factor_info", "vp_factor_destroy", "vp_gemm", "vp_dense_solve"]
+
+
+def load():
+    """Load the built library (raises when it has not been built: the GPU path has no CPU fallback)"""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        lib = C.CDLL(LIBPATH)
+        lib.vp_last_error.restype = C.c_char_p
+        lib.vp_last_error.argtypes = [C.c_void_p]
+        lib.vp_destroy.restype = None
+        lib.vp_plan_destroy.restype = None
+        _lib = lib
+    return _lib
+
+
+def ptr(a, ct=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ct)) if a is not None else None
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)

@@ -285,12 +285,12 @@ def naca_wing(spec: WingSpec) -> Surface:
     for j in range(spec.n_span):
         for k in range(N):
             a, bq, cq, d = nid(j, k), nid(j, k + 1), nid(j + 1, k + 1), nid(j + 1, k)
-            # outward: the ring runs clockwise seen from +y (lower surface first), so (a, d, c) and (a, c, b)
+            # outward: (a, b, c, d) runs counterclockwise about the outward normal
             L13, L24 = np.linalg.norm(X[cq] - X[a]), np.linalg.norm(X[d] - X[bq])
             if L13 <= L24 * (1 + 1e-12):
-                t1, t2 = [a, cq, bq], [a, d, cq]
+                t1, t2 = [a, bq, cq], [a, cq, d]
             else:
-                t1, t2 = [a, d, bq], [bq, d, cq]
+                t1, t2 = [a, bq, d], [bq, cq, d]
             surf = TE_LOWER if k < nh else TE_UPPER
             for t in (t1, t2):
                 tris.append(t)
