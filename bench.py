@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark of the Galerkin hot path on one MI355X: pair integrals per second of `assemble` on the abused
+ellipsoid (semi-axes 2, 1, 0.5; uv mesh 71 x 142: 9,942 nodes, 19,880 triangles; clearance-bounded jitter 0.25 and
+5 % sliver triangles; Vinf at 5 degrees), plain FP64.  A step is one whole assembly of A (9,942 x 9,942) and b with
+the geometry resident in device memory.  Also reported: the mixed path (FP32 multipole far field), the bordered
+factorization + solve in both modes with their errors against the exact ellipsoid flow, the far-tier kernel's roofline
+(algorithmic FLOP/s from HIP events on the assembly stream), and the FP64 CPU oracle timed on the host on a bounded
+sample of the same assembly (rows of test triangles).
+
+python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "vortexpanel.jl_amd"), ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+# algorithmic FP64 flops per far-tier evaluation (add/sub/mul = 1, FMA = 2, sqrt/div/log/atan2 = 1): the closed form
+# (src/kernels/source_triangle.jl:75-89) 132, the octupole multipole (:100-119) 258, and the pair_block work per point
+# (src/assembly/assemble.jl:248-255) 73
+FLOPS_EXACT, FLOPS_MULTIPOLE, FLOPS_BLOCK = 132, 258, 73
+PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload():
+    from vphip import dofmap, mesh
+    g = mesh.abused_ellipsoid(nth=71, nph=142, axes=(2.0, 1.0, 0.5), jitter=0.25, sliver_frac=0.05, sliver_ratio=0.1,
+                              seed=20261203)
+    dm = dofmap.build_dofmap(g)
+    a = np.radians(5.0)
+    V = np.array([np.cos(a), 0.0, np.sin(a)])
+    return g, dm, V
+
+
+def cpu_baseline(g, dm, V, target_s=12.0):
+    """The FP64 oracle (C, OpenMP) on the host's cores over a bounded sample of test-triangle rows"""
+    from oracle import oracle as O
+    threads = int(os.environ.get("VP_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    el = O.elements(g.x, g.tris)
+    sig = O.source_strengths(el, V)
+    pilot = max(threads * 4, 64)
+    t0 = time.perf_counter()
+    O.assemble_rows(g, dm, sig, V, 0, pilot, nthreads=threads)
+    tp = time.perf_counter() - t0
+    rows = int(min(g.nt, max(pilot, pilot * target_s / max(tp, 1e-3))))
+    t0 = time.perf_counter()
+    npairs = O.assemble_rows(g, dm, sig, V, 0, rows, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=npairs / dt, unit="pair-integrals/s", cores=threads, kind="port",
+                sample=f"FP64 C oracle assemble over test triangles 0..{rows - 1} of {g.nt} ({npairs:.3e} pair "
+                       f"integrals, {dt:.1f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one MI355X, and the path runs on one device "
+              "(how it would shard is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    from vphip import api, dofmap, solve
+    from vphip.exact import EllipsoidFlow
+
+    g, dm, V = workload()
+    log(f"workload: nt={g.nt} nn={g.nn} nslot={dm.nslot} mesh sha256 {g.digest()[:16]}")
+    ctx = api.default_context()
+    els = api.build_elements(g)
+    sig = api.source_strengths(els, V)
+    plans = {m: api.AssemblyPlan(g, dm, None, api.AssemblyOptions(precision=m), ctx=ctx, timing=True)
+             for m in ("fp64", "mixed")}
+    for m, p in plans.items():
+        for _ in range(max(args.warmup, 1)):
+            p.assemble(sig, V)
+    ctx.sync()
+    # timed region: K plain-FP64 assemblies
+    p = plans["fp64"]
+    far_ms = []
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        p.assemble(sig, V)
+        far_ms.append(p.timings()["far"])
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    ms_step = 1e3 * dt / args.steps
+    pairs = sum(p.pairs.values())
+    tm = p.timings()
+    value = pairs / (dt / args.steps)
+    log(f"fp64 assemble {ms_step:.2f} ms/step, {value:.3e} pair-integrals/s, kernels {tm}")
+    # roofline of the dominant kernel (far tier)
+    far_pairs_n = p.pairs["far"]
+    ex = tm["far_exact_points"]
+    flops = ex * (FLOPS_EXACT + FLOPS_BLOCK) + (3 * far_pairs_n - ex) * (FLOPS_MULTIPOLE + FLOPS_BLOCK)
+    far_avg = float(np.mean(far_ms)) / 1e3
+    achieved = flops / far_avg
+    # mixed path
+    pm = plans["mixed"]
+    tms = []
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        pm.assemble(sig, V)
+        ctx.sync()
+        tms.append(time.perf_counter() - t1)
+    mixed_ms = 1e3 * float(np.median(tms))
+    log(f"mixed assemble {mixed_ms:.2f} ms/step, kernels {pm.timings()}")
+    A64, b64 = p.read()
+    Amx, bmx = pm.read()
+    entry_err = float(np.max(np.abs(Amx - A64)) / np.max(np.abs(A64)))
+    # bordered factorization + solve in both modes, errors against the exact flow
+    ef = EllipsoidFlow((2.0, 1.0, 0.5), V)
+    mu_ex = ef.doublet(g.x[dm.slot_node])
+    scale = ef.doublet_scale()
+    solves = {}
+    for m, plan in (("fp64", p), ("mixed", pm)):
+        ctx.sync()
+        t1 = time.perf_counter()
+        F = solve.factorize_galerkin(dm, plan, mode=m)
+        ctx.sync()
+        tf = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        out = solve.solve_galerkin(F, b64 if m == "fp64" else bmx)
+        ts = time.perf_counter() - t1
+        err = out["mu"] - mu_ex
+        err -= np.sum(dm.slot_area * err) / np.sum(dm.slot_area)
+        solves[m] = dict(factor_s=tf, solve_s=ts, refinement_iters=out["iters"],
+                         mu_L2=float(np.sqrt(np.sum(dm.slot_area * err ** 2) / np.sum(dm.slot_area)) / scale),
+                         mu_max=float(np.max(np.abs(err)) / scale), mu=out["mu"])
+        F.close()
+    dmu = float(np.max(np.abs(solves["mixed"]["mu"] - solves["fp64"]["mu"])) / np.max(np.abs(solves["fp64"]["mu"])))
+    for m in solves:
+        solves[m].pop("mu")
+        log(f"{m} solve {solves[m]}")
+    cpu = None if args.no_cpu else cpu_baseline(g, dm, V)
+    if cpu:
+        log(f"cpu baseline {cpu}")
+    line = {
+        "metric": "pair-integrals/s",
+        "value": value,
+        "unit": "pair-integrals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded abused ellipsoid mesh generated in-process)",
+        "config": {"workload": "assemble, abused ellipsoid 2:1:0.5 uv 71x142 (9,942 nodes, 19,880 triangles, jitter "
+                               "0.25, 5% slivers), Vinf 5 deg, default TierRules, plain FP64",
+                   "nt": g.nt, "nslot": dm.nslot, "pairs_per_step": pairs, "tier_pairs": p.pairs,
+                   "tier_points": p.points},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64, "traffic": None,
+                     "kernel": "far_pairs<double>", "kernel_ms": far_avg * 1e3,
+                     "algorithmic_flops": flops, "far_exact_points": ex, "far_points": 3 * far_pairs_n},
+        "cpu_baseline": cpu,
+        "mixed": {"assemble_ms": mixed_ms, "pair_integrals_per_s": pairs / (mixed_ms / 1e3),
+                  "max_entry_rel_diff_vs_fp64": entry_err, "max_mu_rel_diff_vs_fp64": dmu},
+        "solve": solves,
+        "kernels_ms": {k: v for k, v in tm.items() if k in ("far", "filaments", "near", "freestream")},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -88,8 +88,9 @@ int vp_assemble(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const dou
  * rule tables; vp_plan_assemble fills the plan's A and b (on the device) for new sigma and Vinf. */
 int vp_plan_create(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const vp_assembly_options *opts,
                    vp_plan **out);
-/* flags: VP_PLAN_RECORD_TIERS keeps the tier of every pair of the last assembly (see vp_plan_tiers). */
-enum { VP_PLAN_RECORD_TIERS = 1 };
+/* flags: VP_PLAN_RECORD_TIERS keeps the tier of every pair of the last assembly (see vp_plan_tiers);
+ * VP_PLAN_TIMING records per-kernel HIP events (see vp_plan_timings). */
+enum { VP_PLAN_RECORD_TIERS = 1, VP_PLAN_TIMING = 2 };
 int vp_plan_create_ex(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const vp_assembly_options *opts,
                       int flags, vp_plan **out);
 int vp_plan_assemble(vp_plan *plan, const double *sigma, const double *vinf, long long *pairs, long long *points);
@@ -97,6 +98,10 @@ int vp_plan_read(vp_plan *plan, double *A, double *b);
 /* Tier of every (body test triangle, source triangle) pair (nt x (nt + wake triangles), uint8, TIER_NAMES index)
  * of the last vp_plan_assemble; the plan must have been created with VP_PLAN_RECORD_TIERS. */
 int vp_plan_tiers(vp_plan *plan, uint8_t *tiers);
+/* Per-kernel device times of the last vp_plan_assemble, from HIP events on the context's stream (the plan must be
+ * created with VP_PLAN_TIMING): ms[0] far pairs, ms[1] filaments, ms[2] near/touching pairs, ms[3] freestream, and
+ * ms[4] the number of near-list pairs, ms[5] the far tier's closed-form (non-multipole) point evaluations. */
+int vp_plan_timings(vp_plan *plan, double *ms);
 /* Device pointers of the plan's A (column-major, leading dimension nslot) and b, for chaining device calls. */
 int vp_plan_device_A(vp_plan *plan, double **A, double **b, long long *lda);
 void vp_plan_destroy(vp_plan *plan);

@@ -821,11 +821,11 @@ static int is_touch(const int *tp, int ntp, int t, int s)
  * tiers (optional, nt x (nt + nwt) bytes, row-major by test triangle) receives each body/wake pair's tier.
  * Returns 0, -1 degenerate triangle, -2 bad index, -3 bad option.
  */
-int vpo_assemble(int nn, const double *x, int nt, const int *tris, const int *corner_slot, int nslot,
-                 const double *sigma, const double *vinf, int ntouch, const int *touch,
-                 int nwx, const double *wx, int nwt, const int *wtris, int nwf, const int *wfil,
-                 const vpo_opts *o, int nthreads, double *A, double *bvec, long long *pairs, long long *points,
-                 unsigned char *tiers)
+int vpo_assemble_rows(int nn, const double *x, int nt, const int *tris, const int *corner_slot, int nslot,
+                      const double *sigma, const double *vinf, int ntouch, const int *touch,
+                      int nwx, const double *wx, int nwt, const int *wtris, int nwf, const int *wfil,
+                      const vpo_opts *o, int nthreads, double *A, double *bvec, long long *pairs, long long *points,
+                      unsigned char *tiers, int t_begin, int t_end)
 {
     if (o->test < 0 || o->test > 2) return -3;
     tier_rules_t R;
@@ -861,7 +861,7 @@ int vpo_assemble(int nn, const double *x, int nt, const int *tris, const int *co
         rule_t tmp;
         rule_init(&tmp);
 #pragma omp for schedule(dynamic, 4)
-        for (int t = 0; t < nt; t++) {
+        for (int t = t_begin; t < t_end; t++) {
             const el_t *T = els + t;
             const int *tri = tris + 3 * t;
             memset(buf, 0, sizeof(double) * 3 * (size_t)ncol);
@@ -929,6 +929,18 @@ int vpo_assemble(int nn, const double *x, int nt, const int *tris, const int *co
     free(tp);
     tier_rules_free(&R);
     return 0;
+}
+
+/* The whole assembly (all test triangles); vpo_assemble_rows restricts the rows to test triangles
+ * [t_begin, t_end) -- the bounded sample the CPU baseline times. */
+int vpo_assemble(int nn, const double *x, int nt, const int *tris, const int *corner_slot, int nslot,
+                 const double *sigma, const double *vinf, int ntouch, const int *touch,
+                 int nwx, const double *wx, int nwt, const int *wtris, int nwf, const int *wfil,
+                 const vpo_opts *o, int nthreads, double *A, double *bvec, long long *pairs, long long *points,
+                 unsigned char *tiers)
+{
+    return vpo_assemble_rows(nn, x, nt, tris, corner_slot, nslot, sigma, vinf, ntouch, touch, nwx, wx, nwt, wtris,
+                             nwf, wfil, o, nthreads, A, bvec, pairs, points, tiers, 0, nt);
 }
 
 /* One pair block, for the per-pair parity tests: tier, block and RHS of test triangle t and source triangle s

@@ -202,7 +202,8 @@ template <typename T> __device__ __forceinline__ FarSrc<T> load_far_src(const El
 // the multipole beyond kernel_far diameters in T, the closed form in FP64 otherwise (its edge-log sum cancels
 // like r/diam, which single precision cannot carry).
 template <typename T>
-__device__ __forceinline__ V3<double> far_eval(const FarSrc<T> &f, const Elem *__restrict__ E, V3<double> x, double kfar2)
+__device__ __forceinline__ V3<double> far_eval(const FarSrc<T> &f, const Elem *__restrict__ E, V3<double> x, double kfar2,
+                                               unsigned int &nexact)
 {
     V3<double> r = x - f.c;
     if (cls_dot(r, r) > kfar2) {
@@ -219,6 +220,7 @@ __device__ __forceinline__ V3<double> far_eval(const FarSrc<T> &f, const Elem *_
     s.diam = E->diam;
     s.area = E->area;
     V3<double> ra = x - ld3(E->v[0]), rb = x - ld3(E->v[1]), rc = x - ld3(E->v[2]);
+    nexact++;
     return sv_exact_abs(s, ra, rb, rc, dot(r, s.n), 0, nullptr, x);
 }
 
@@ -235,12 +237,12 @@ __global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__r
 {
     __shared__ double R[VMAX * UMAX];
     __shared__ double rbacc[VMAX];
-    __shared__ unsigned int nfar;
+    __shared__ unsigned int nfar, nexw;
     const int tt = blockIdx.x, st = blockIdx.y;
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < VMAX * UMAX; i += blockDim.x) R[i] = 0.0;
     if (threadIdx.x < VMAX) rbacc[threadIdx.x] = 0.0;
-    if (threadIdx.x == 0) nfar = 0;
+    if (threadIdx.x == 0) nfar = nexw = 0;
     const int s = ts_elems[st * TS + lane];
     const bool valid = s >= 0;
     FarSrc<T> f;
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__r
     }
     const double kfar2 = (P.kernel_far * (valid ? f.diam : 0.0)) * (P.kernel_far * (valid ? f.diam : 0.0));
     __syncthreads();
-    unsigned int myfar = 0;
+    unsigned int myfar = 0, myex = 0;
     for (int i = wave; i < TT_MAX; i += 4) {
         const int t = tt_elems[tt * TT_MAX + i];
         if (t < 0) break;
@@ -271,7 +273,7 @@ __global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__r
             double blk[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, rb[3] = {0, 0, 0};
 #pragma unroll
             for (int q = 0; q < 3; q++) {
-                V3<double> w = far_eval<T>(f, el + s, ld3(F.x[q]), kfar2);
+                V3<double> w = far_eval<T>(f, el + s, ld3(F.x[q]), kfar2, myex);
                 V3<double> vv = cross(w, n);  // n.(gamma x w) = gamma.(w x n), assemble.jl:251
                 double c0 = dot(vv, mk3((double)f.g[0].x, (double)f.g[0].y, (double)f.g[0].z));
                 double c1 = dot(vv, mk3((double)f.g[1].x, (double)f.g[1].y, (double)f.g[1].z));
@@ -296,6 +298,7 @@ __global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__r
         }
     }
     if (myfar) atomicAdd(&nfar, myfar);
+    if (myex) atomicAdd(&nexw, myex);
     __syncthreads();
     const int nrow = tt_nrow[tt], ncol = ts_ncol[st];
     for (int i = threadIdx.x; i < nrow * UMAX; i += blockDim.x) {
@@ -306,6 +309,7 @@ __global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__r
     for (int v = threadIdx.x; v < nrow; v += blockDim.x)
         if (rbacc[v] != 0.0) unsafeAtomicAdd(&bvec[tt_rows[tt * VMAX + v]], rbacc[v]);
     if (threadIdx.x == 0 && nfar) atomicAdd(far_count, (unsigned long long)nfar);
+    if (threadIdx.x == 0 && nexw) atomicAdd(far_count + 7, (unsigned long long)nexw);  // stats[12]: exact far points
 }
 
 // ------------------------------------------------------------------------------------------ filaments --------------
@@ -902,6 +906,8 @@ template <typename X> X *dupload(const std::vector<X> &v, hipStream_t s)
 
 Plan::~Plan()
 {
+    for (int k = 0; k < 5; k++)
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
     for (void *p : owned) (void)hipFree(p);
     if (A && ownsA) (void)hipFree(A);
 }
@@ -1063,6 +1069,7 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
     near_cap = (unsigned int)std::min<long long>((long long)nt * 160 + 1024, 1ll << 31);
     d_near = dalloc<int2>(near_cap);
     owned.push_back(d_near);
+    for (int k = 0; k < 5; k++) HIPCHECK(hipEventCreate(&ev[k]));
     if (record_tiers) {
         d_tiers = dalloc<uint8_t>((size_t)nt * nsrc);
         owned.push_back(d_tiers);
@@ -1096,6 +1103,7 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
         P.near_depth = opts.near_depth;
         P.lda = lda;
         dim3 grid(ntt, nts);
+        if (timing) HIPCHECK(hipEventRecord(ev[0], st));
         if (opts.precision == 1)
             far_pairs<float><<<grid, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc, d_tsn,
                                                    A, d_b, d_near, d_count, near_cap, d_stats + 5, d_tiers);
@@ -1103,6 +1111,7 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
             far_pairs<double><<<grid, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc,
                                                     d_tsn, A, d_b, d_near, d_count, near_cap, d_stats + 5, d_tiers);
         HIPCHECK(hipGetLastError());
+        if (timing) HIPCHECK(hipEventRecord(ev[1], st));
         if (nseg) {
             long long np = (long long)nt * nseg;
             seg_pairs<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(P, d_tf, d_cs, d_seg, d_segcol, A, d_near, d_count,
@@ -1121,16 +1130,33 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
             continue;
         }
         nnear = cnt;
+        if (timing) HIPCHECK(hipEventRecord(ev[2], st));
         if (cnt) {
             unsigned int blocks = std::min<unsigned int>((cnt + 3) / 4, 65535u);
             near_pairs<<<blocks, 256, 0, st>>>(P, rt, d_el, d_nodes, d_sigma, d_cs, d_cols, d_seg, d_segcol, d_near, cnt,
                                                d_touch, ntouch, A, d_b, d_stats, d_tiers);
             HIPCHECK(hipGetLastError());
         }
+        if (timing) HIPCHECK(hipEventRecord(ev[3], st));
         freestream<<<(nt + 255) / 256, 256, 0, st>>>(d_el, d_cs, nt, opts.test, vinf[0], vinf[1], vinf[2], d_b);
         HIPCHECK(hipGetLastError());
+        if (timing) HIPCHECK(hipEventRecord(ev[4], st));
         break;
     }
+}
+
+void Plan::kernel_ms(double *ms)
+{
+    HIPCHECK(hipEventSynchronize(ev[4]));
+    for (int k = 0; k < 4; k++) {
+        float t = 0;
+        HIPCHECK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+        ms[k] = t;
+    }
+    ms[4] = (double)nnear;
+    unsigned long long h[16];
+    HIPCHECK(hipMemcpy(h, d_stats, sizeof(h), hipMemcpyDeviceToHost));
+    ms[5] = (double)h[12];
 }
 
 void Plan::stats(long long *pairs, long long *points)

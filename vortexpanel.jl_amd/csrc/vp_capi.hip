@@ -94,6 +94,7 @@ int vp_plan_create_ex(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, con
     vp_plan *p = new (std::nothrow) vp_plan;
     if (!p) return fail(&ctx->c, VP_E_DEVICE, "host allocation failed");
     p->p.record_tiers = (flags & VP_PLAN_RECORD_TIERS) != 0;
+    p->p.timing = (flags & VP_PLAN_TIMING) != 0;
     int rc = guard(&ctx->c, [&] {
         (void)hipSetDevice(ctx->c.device);
         p->p.build(&ctx->c, *g, wake && (wake->nw > 0) ? wake : nullptr, *opts);
@@ -142,6 +143,13 @@ int vp_plan_tiers(vp_plan *plan, uint8_t *tiers)
             hipStreamSynchronize(c->stream) != hipSuccess)
             throw std::runtime_error("copy of the tiers failed");
     });
+}
+
+int vp_plan_timings(vp_plan *plan, double *ms)
+{
+    if (!plan || !plan->p.ctx || !ms) return VP_E_ARGUMENT;
+    if (!plan->p.timing) return fail(plan->p.ctx, VP_E_ARGUMENT, "the plan was not created with VP_PLAN_TIMING");
+    return guard(plan->p.ctx, [&] { plan->p.kernel_ms(ms); });
 }
 
 int vp_plan_device_A(vp_plan *plan, double **A, double **b, long long *lda)

@@ -50,11 +50,14 @@ struct Plan {
     unsigned int *d_count = nullptr;
     RuleTab rt{};
     bool record_tiers = false;
+    bool timing = false;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 
     ~Plan();
     void build(Ctx *ctx, const vp_surface &g, const vp_wake *wake, const vp_assembly_options &o);
     void assemble(const double *sigma, const double *vinf, bool host_sigma);
     void stats(long long *pairs, long long *points);
+    void kernel_ms(double *ms);
 };
 
 // The factored bordered system (solve.jl:65-87): the original FP64 bordered matrix M (row-major, npad x npad, the

@@ -158,7 +158,7 @@ class AssemblyPlan:
     """vp_plan_*: geometry, element records, tiles and rules resident on the device; A and b stay on the device"""
 
     def __init__(self, g, dm, wake=None, opts: AssemblyOptions | None = None, ctx: Context | None = None,
-                 record_tiers: bool = False):
+                 record_tiers: bool = False, timing: bool = False):
         self.ctx = ctx or default_context()
         self.opts = opts or AssemblyOptions()
         oc = self.opts.to_c()
@@ -170,7 +170,7 @@ class AssemblyPlan:
         self.nwt = 0 if wake is None else len(wake.tris)
         h = C.c_void_p()
         rc = lib.vp_plan_create_ex(self.ctx.h, C.byref(s), C.byref(w) if w is not None else None, C.byref(oc),
-                                   1 if record_tiers else 0, C.byref(h))
+                                   (1 if record_tiers else 0) | (2 if timing else 0), C.byref(h))
         _check(self.ctx.h, rc)
         self.h = h
         self.record_tiers = record_tiers
@@ -188,6 +188,13 @@ class AssemblyPlan:
         self.pairs = dict(zip(TIER_NAMES, pairs.tolist()))
         self.points = dict(zip(TIER_NAMES, points.tolist()))
         return self
+
+    def timings(self):
+        """Device times (ms) of the last assembly's kernels, from HIP events on the plan's stream"""
+        ms = np.zeros(6)
+        _check(self.ctx.h, self.ctx.lib.vp_plan_timings(self.h, L.ptr(ms)))
+        return dict(far=ms[0], filaments=ms[1], near=ms[2], freestream=ms[3], near_pairs=int(ms[4]),
+                    far_exact_points=int(ms[5]))
 
     def tiers(self):
         """Tier index of every (body test triangle, source triangle) pair of the last assembly"""
