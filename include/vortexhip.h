@@ -99,8 +99,10 @@ int vp_plan_read(vp_plan *plan, double *A, double *b);
  * of the last vp_plan_assemble; the plan must have been created with VP_PLAN_RECORD_TIERS. */
 int vp_plan_tiers(vp_plan *plan, uint8_t *tiers);
 /* Per-kernel device times of the last vp_plan_assemble, from HIP events on the context's stream (the plan must be
- * created with VP_PLAN_TIMING): ms[0] far pairs, ms[1] filaments, ms[2] near/touching pairs, ms[3] freestream, and
- * ms[4] the number of near-list pairs, ms[5] the far tier's closed-form (non-multipole) point evaluations. */
+ * created with VP_PLAN_TIMING): ms[0] far pairs and far filaments, ms[1] routing of the non-far pairs and the
+ * touching pairs' rule pieces, ms[2] plain near pairs, ms[3] touching pairs' points, ms[4] freestream; then ms[5]
+ * the number of single-piece near pairs, ms[6] of touching rule pieces, ms[7] the far tier's closed-form
+ * (non-multipole) point evaluations (8 values). */
 int vp_plan_timings(vp_plan *plan, double *ms);
 /* Device pointers of the plan's A (column-major, leading dimension nslot) and b, for chaining device calls. */
 int vp_plan_device_A(vp_plan *plan, double **A, double **b, long long *lda);

@@ -11,9 +11,16 @@
 //                  compute type T and reduce into an LDS tile R[test slot][source column], flushed to A with one
 //                  atomic per nonzero tile entry; the other pairs are appended to the near list
 //   seg_pairs      body test triangle x tip filament, far ones evaluated here, the others appended
-//   near_pairs     one wavefront per near/touching pair: the wave builds the pair's rule pieces on a small LDS
-//                  stack (self, edge, vertex with wide-corner split, contact fan splits, recursive near split) and
-//                  evaluates every piece's points in FP64 across its 64 lanes
+//   triage         one thread per non-far pair routes it: touching (filaments, self, shared corners, contact,
+//                  geometric touch pairs), plain near with one piece, plain near with several pieces
+//   touch_simple   one wavefront per self / shared-edge / shared-corner pair (no contact): the stored rule or the
+//                  wide corner's wedges, points flattened over the 64 lanes
+//   touch_gen      one thread per remaining touching pair (contact, touch pairs, filaments): its exact tier and rule
+//                  pieces (contact fan splits, vertex wedges, near splits) into a leaf list
+//   group_eval     one wavefront per such pair: the points of all its pieces flattened over the 64 lanes
+//   near_thread    one thread per plain near pair whose near rule is the single root piece
+//   near_group     one wavefront per plain near pair with several pieces: all split tests of the quadtree in
+//                  parallel over the lanes, leaves by ballot, their points flattened over the lanes
 //   freestream     b_i -= (Vinf.n) A int psi_i / A
 #include <hip/hip_runtime.h>
 
@@ -38,7 +45,6 @@ constexpr int TT_MAX = 128;  // test triangles per far tile
 constexpr int TS = 64;       // source elements per far tile (one per lane)
 constexpr int VMAX = 96;     // distinct test slots per far tile
 constexpr int UMAX = 64;     // distinct source columns per far tile
-constexpr int STACK = 48;    // rule-piece stack entries per wave in near_pairs
 
 enum Tier { T_SELF = 0, T_EDGE, T_VERTEX, T_CONTACT, T_NEAR, T_FAR };
 
@@ -47,7 +53,7 @@ struct TestFar {  // one per body test triangle: the far rule's points, test wei
     double wpsi[3][3];  // [q][a]
     double n[3];
     double c[3];
-    double diam, area;
+    double diam, area, reach, pad;
 };
 
 struct AsmParams {
@@ -151,10 +157,22 @@ __global__ void test_setup(const Elem *__restrict__ el, int nt, RuleTab rt, int 
     }
     F.diam = E.diam;
     F.area = E.area;
+    F.reach = E.reach;
+    F.pad = 0.0;
     out[t] = F;
 }
 
 // ------------------------------------------------------------------------------------------ far pairs --------------
+
+// contact_rule's early out, src/assembly/assemble.jl:200-201: a pair can only be a contact pair when its centroids
+// are within reach(T) + reach(S) + contact_rel min(diam_T, diam_S)
+__device__ inline bool cls_contact_possible(V3<double> cs, V3<double> ct, double rs, double rt, double ds, double dt,
+                                            double contact_rel)
+{
+#pragma clang fp contract(off)
+    double tol = contact_rel * fmin(dt, ds);
+    return !(cls_norm(cls_sub(cs, ct)) > (rt + rs) + tol);
+}
 
 // Wave-aggregated append of (t, s) to the near list.
 __device__ __forceinline__ void append_pair(bool want, int t, int s, int2 *list, unsigned int *count, unsigned int cap)
@@ -174,7 +192,7 @@ __device__ __forceinline__ void append_pair(bool want, int t, int s, int2 *list,
 
 template <typename T> struct FarSrc {  // per-lane source element data for the far tier
     V3<double> c;                      // centroid (FP64: the classification and the relative offsets)
-    double diam;
+    double diam, reach;
     SrcT<T> s;                         // vertex offsets from the centroid, normal, edges ... in the compute type
     V3<T> g[3];
     T sigma;
@@ -185,14 +203,11 @@ template <typename T> __device__ __forceinline__ FarSrc<T> load_far_src(const El
     FarSrc<T> f;
     f.c = ld3(E.c);
     f.diam = E.diam;
-    for (int k = 0; k < 3; k++) {
+    f.reach = E.reach;
+    for (int k = 0; k < 3; k++) {  // the multipole needs the vertex offsets and the area only
         f.s.v[k] = mk3(T(E.v[k][0] - E.c[0]), T(E.v[k][1] - E.c[1]), T(E.v[k][2] - E.c[2]));
-        f.s.nu[k] = mk3(T(E.nu[k][0]), T(E.nu[k][1]), T(E.nu[k][2]));
-        f.s.len[k] = T(E.len[k]);
         f.g[k] = mk3(T(E.g[k][0]), T(E.g[k][1]), T(E.g[k][2]));
     }
-    f.s.n = mk3(T(E.n[0]), T(E.n[1]), T(E.n[2]));
-    f.s.diam = T(E.diam);
     f.s.area = T(E.area);
     f.sigma = T(sigma);
     return f;
@@ -271,7 +286,7 @@ __global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__r
             myfar++;
             V3<double> n = ld3(F.n);
             double blk[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, rb[3] = {0, 0, 0};
-#pragma unroll
+#pragma unroll 1
             for (int q = 0; q < 3; q++) {
                 V3<double> w = far_eval<T>(f, el + s, ld3(F.x[q]), kfar2, myex);
                 V3<double> vv = cross(w, n);  // n.(gamma x w) = gamma.(w x n), assemble.jl:251
@@ -361,107 +376,70 @@ __global__ void seg_pairs(AsmParams P, const TestFar *__restrict__ tf, const int
 
 // ------------------------------------------------------------------------------------------ near pairs -------------
 
+constexpr int PSTACK = 32;  // rule-piece stack entries per thread in triage
+
 struct Piece {  // a piece of the test triangle in barycentric corners, with its area fraction
     double b[3][3];
     double frac;
-    int kind;   // 0 near (recursive split), 1 contact (fan split)
+    int kind;  // 0 near (recursive split), 1 contact (fan split)
     int level, depth, pad;
 };
 
-struct NearCtx {
-    const Elem *T, *S;          // test element, source element (S null for a filament)
-    V3<double> X[3];            // test triangle's corners
-    V3<double> SX[3];           // source triangle's corners
-    V3<double> fa, fb;          // filament ends
-    bool seg;
-    double tol;                 // contact tolerance
-    int side, test;
-    double sigma;
-    RuleTab rt;
-    const AsmParams *P;
-    double acc[12];             // blk (9, row-major [a][b]) + rb (3); filaments: ca (3) + cb (3)
-    long long npts;
+struct Leaf {  // a rule piece to evaluate: the pair, the base rule, the piece's barycentric corners, its area fraction
+    double b[3][3];
+    double frac;
+    int t, s, rid, npt;
 };
 
-__device__ __forceinline__ double src_distance(const NearCtx &c, V3<double> p)
-{
-    return c.seg ? cls_seg_projection(p, c.fa, c.fb, nullptr) : cls_tri_distance(c.SX, p);
-}
+// Leaf generation for one pair on one thread (triage): the rule pieces of pair_rule's touching tiers and of the near
+// rule of touch pairs and filaments.  Run twice: once counting (out null), once writing at the reserved base.
+struct Gen {
+    V3<double> X[3], SX[3], fa, fb;
+    bool seg;
+    double tol;
+    double near_ratio, corner_angle;
+    int near_depth;
+    int t, s;
+    const int *rn;  // rule sizes
+    Leaf *out;
+    unsigned int base, cap, n;
+    long long npts;
+    bool overflow;
+};
 
-// Evaluate base rule `rid` mapped onto the piece b (append_piece!, src/assembly/assemble.jl:85-91) across the wave.
-__device__ void eval_leaf(NearCtx &c, int rid, const double (*b)[3], double frac)
+__device__ void gen_emit(Gen &g, int rid, const double (*b)[3], double frac)
 {
-    const int n = c.rt.n[rid], off = c.rt.off[rid];
-    const int lane = threadIdx.x & (WAVE - 1);
-    c.npts += n;
-    V3<double> tn = ld3(c.T->n);
-    for (int p = lane; p < n; p += WAVE) {
-        const double *m = c.rt.l + 3 * (off + p);
-        double l[3];
-        {
-#pragma clang fp contract(off)
-            for (int j = 0; j < 3; j++) l[j] = (m[0] * b[0][j] + m[1] * b[1][j]) + m[2] * b[2][j];
-        }
-        double W = c.rt.w[off + p] * frac * c.T->area;
-        double psi[3];
-        test_weights(c.test, l, psi);
-        V3<double> x = cls_point(c.X, l);
-        if (c.seg) {
-            V3<double> va, vb;
-            seg_vel(x - c.fa, x - c.fb, va, vb);
-            double da = dot(va, tn), db = dot(vb, tn);
-            for (int a = 0; a < 3; a++) {
-                c.acc[a] += da * (W * psi[a]);
-                c.acc[3 + a] += db * (W * psi[a]);
-            }
-        } else {
-            const Elem &S = *c.S;
-            V3<double> r = x - ld3(S.c);
-            V3<double> w;
-            double fd = c.P->kernel_far * S.diam;
-            if (cls_dot(r, r) > fd * fd) {
-                SrcT<double> s;
-                for (int k = 0; k < 3; k++) s.v[k] = ld3(S.v[k]) - ld3(S.c);
-                s.area = S.area;
-                w = sv_far_rel(s, r);
-            } else {
-                SrcT<double> s;
-                for (int k = 0; k < 3; k++) {
-                    s.nu[k] = ld3(S.nu[k]);
-                    s.len[k] = S.len[k];
-                }
-                s.n = ld3(S.n);
-                s.diam = S.diam;
-                double z;
-                {
-#pragma clang fp contract(off)
-                    z = cls_dot(r, s.n);
-                }
-                w = sv_exact_abs(s, x - ld3(S.v[0]), x - ld3(S.v[1]), x - ld3(S.v[2]), z, c.side, &S, x);
-            }
-            V3<double> v = cross(w, tn);
-            double cc[3] = {dot(v, ld3(S.g[0])), dot(v, ld3(S.g[1])), dot(v, ld3(S.g[2]))};
-            double wn = c.sigma * dot(w, tn);
-            for (int a = 0; a < 3; a++) {
-                double ps = W * psi[a];
-                for (int bb = 0; bb < 3; bb++) c.acc[3 * a + bb] += ps * cc[bb];
-                c.acc[9 + a] -= wn * ps;
-            }
-        }
+    if (g.out && g.base + g.n < g.cap) {
+        Leaf &L = g.out[g.base + g.n];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) L.b[i][j] = b[i][j];
+        L.frac = frac;
+        L.t = g.t;
+        L.s = g.s;
+        L.rid = rid;
+        L.npt = g.rn[rid];
     }
+    g.n++;
+    g.npts += g.rn[rid];
 }
 
-__device__ __forceinline__ void set_corners(double (*b)[3]) {
+__device__ __forceinline__ double src_distance(const Gen &g, V3<double> p)
+{
+    return g.seg ? cls_seg_projection(p, g.fa, g.fb, nullptr) : cls_tri_distance(g.SX, p);
+}
+
+__device__ __forceinline__ void set_corners(double (*b)[3])
+{
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) b[i][j] = i == j ? 1.0 : 0.0;
 }
 
 // vertex_pieces!, src/assembly/assemble.jl:102-116 (k 0-based, p the piece's physical corners)
-__device__ void vertex_pieces(NearCtx &c, const V3<double> *p, int k, const double (*b)[3], double frac)
+__device__ void vertex_pieces(Gen &g, const V3<double> *p, int k, const double (*b)[3], double frac)
 {
 #pragma clang fp contract(off)
     int i = (k + 1) % 3, j = (k + 2) % 3;
-    V3<double> u = p[i] - p[k], v = p[j] - p[k];
+    V3<double> u = cls_sub(p[i], p[k]), v = cls_sub(p[j], p[k]);
     double nu = cls_norm(u), nv = cls_norm(v);
     double cs = cls_dot(u, v) / (nu * nv);
     cs = cs < -1.0 ? -1.0 : cs > 1.0 ? 1.0 : cs;
@@ -473,10 +451,10 @@ __device__ void vertex_pieces(NearCtx &c, const V3<double> *p, int k, const doub
             bb[1][q] = b[i][q];
             bb[2][q] = b[j][q];
         }
-        eval_leaf(c, R_VERT0, bb, frac);
+        gen_emit(g, R_VERT0, bb, frac);
         return;
     }
-    int m = (int)ceil(th * (180 / kPi) / c.P->corner_angle);
+    int m = (int)ceil(th * (180 / kPi) / g.corner_angle);
     double t0 = 0.0;
     for (int q = 1; q <= m; q++) {
         double t1 = 1.0;
@@ -489,22 +467,35 @@ __device__ void vertex_pieces(NearCtx &c, const V3<double> *p, int k, const doub
             bb[1][z] = (1 - t0) * b[i][z] + t0 * b[j][z];
             bb[2][z] = (1 - t1) * b[i][z] + t1 * b[j][z];
         }
-        eval_leaf(c, R_VERT0, bb, frac * (t1 - t0));
+        gen_emit(g, R_VERT0, bb, frac * (t1 - t0));
         t0 = t1;
     }
 }
 
+// vertex_rule, src/assembly/assemble.jl:119-125
+__device__ void vertex_rule(Gen &g, int k)
+{
+#pragma clang fp contract(off)
+    double b0[3][3];
+    set_corners(b0);
+    V3<double> u = cls_sub(g.X[(k + 1) % 3], g.X[k]), v = cls_sub(g.X[(k + 2) % 3], g.X[k]);
+    if (cls_dot(u, v) >= 0)
+        gen_emit(g, R_VERT0 + k, b0, 1.0);
+    else
+        vertex_pieces(g, g.X, k, b0, 1.0);
+}
+
 // The splits of the piece's edge k at the source triangle's corners (edge_splits, assemble.jl:160-171)
-__device__ int edge_splits_k(const NearCtx &c, const V3<double> *p, int k, double *ts)
+__device__ int edge_splits_k(const Gen &g, const V3<double> *p, int k, double *ts)
 {
 #pragma clang fp contract(off)
     V3<double> a = p[k], e = p[(k + 1) % 3];
-    double L = cls_norm(e - a);
+    double L = cls_norm(cls_sub(e, a));
     int n = 0;
     for (int q = 0; q < 3; q++) {
         double t;
-        double d = cls_seg_projection(c.SX[q], a, e, &t);
-        if (d <= c.tol && t * L > c.tol && (1 - t) * L > c.tol) ts[n++] = t;
+        double d = cls_seg_projection(g.SX[q], a, e, &t);
+        if (d <= g.tol && t * L > g.tol && (1 - t) * L > g.tol) ts[n++] = t;
     }
     for (int i = 1; i < n; i++)
         for (int j = i; j > 0 && ts[j] < ts[j - 1]; j--) {
@@ -515,19 +506,26 @@ __device__ int edge_splits_k(const NearCtx &c, const V3<double> *p, int k, doubl
     return n;
 }
 
-// Walks the rule pieces of one pair (near_pieces!, assemble.jl:131-142; contact_pieces!, :209-238) depth first on
-// the wave's LDS stack, evaluating every leaf as it is produced.  All lanes run the same control flow.
-__device__ void walk(NearCtx &c, Piece *stk, int sp)
+// The rule pieces of near_pieces! (assemble.jl:131-142) and contact_pieces! (:209-238), depth first on an explicit
+// stack, children pushed in reverse so they come out in the reference's order.
+__device__ void walk(Gen &g, Piece root)
 {
 #pragma clang fp contract(off)
+    Piece stk[PSTACK];
+    int sp = 0;
+    stk[sp++] = root;
     while (sp > 0) {
         Piece pc = stk[--sp];
-        V3<double> p[3] = {cls_point(c.X, pc.b[0]), cls_point(c.X, pc.b[1]), cls_point(c.X, pc.b[2])};
+        V3<double> p[3] = {cls_point(g.X, pc.b[0]), cls_point(g.X, pc.b[1]), cls_point(g.X, pc.b[2])};
         if (pc.kind == 0) {
-            double size = fmax(fmax(cls_norm(p[1] - p[0]), cls_norm(p[2] - p[1])), cls_norm(p[0] - p[2]));
+            double size = fmax(fmax(cls_norm(cls_sub(p[1], p[0])), cls_norm(cls_sub(p[2], p[1]))), cls_norm(cls_sub(p[0], p[2])));
             V3<double> cen = (p[0] + p[1]) + p[2];
             cen = mk3(cen.x / 3.0, cen.y / 3.0, cen.z / 3.0);
-            if (pc.level < pc.depth && src_distance(c, cen) < c.P->near_ratio * size && sp + 4 <= STACK) {
+            if (pc.level < pc.depth && src_distance(g, cen) < g.near_ratio * size) {
+                if (sp + 4 > PSTACK) {
+                    g.overflow = true;
+                    continue;
+                }
                 double m12[3], m23[3], m31[3];
                 for (int j = 0; j < 3; j++) {
                     m12[j] = (pc.b[0][j] + pc.b[1][j]) / 2;
@@ -535,7 +533,7 @@ __device__ void walk(NearCtx &c, Piece *stk, int sp)
                     m31[j] = (pc.b[2][j] + pc.b[0][j]) / 2;
                 }
                 const double *kids[4][3] = {{pc.b[0], m12, m31}, {m12, pc.b[1], m23}, {m31, m23, pc.b[2]}, {m12, m23, m31}};
-                for (int q = 3; q >= 0; q--) {  // pushed in reverse: popped in the reference's order
+                for (int q = 3; q >= 0; q--) {
                     Piece ch;
                     for (int i = 0; i < 3; i++)
                         for (int j = 0; j < 3; j++) ch.b[i][j] = kids[q][i][j];
@@ -546,19 +544,22 @@ __device__ void walk(NearCtx &c, Piece *stk, int sp)
                     stk[sp++] = ch;
                 }
             } else {
-                eval_leaf(c, R_NEAR, pc.b, pc.frac);
+                gen_emit(g, R_NEAR, pc.b, pc.frac);
             }
             continue;
         }
-        // contact piece
         if (pc.level < 4) {
             int k = 0, ns = 0;
             double sp3[3];
             for (k = 0; k < 3; k++) {
-                ns = edge_splits_k(c, p, k, sp3);
+                ns = edge_splits_k(g, p, k, sp3);
                 if (ns) break;
             }
-            if (ns && sp + ns + 1 <= STACK) {
+            if (ns) {
+                if (sp + ns + 1 > PSTACK) {
+                    g.overflow = true;
+                    continue;
+                }
                 double ts[5];
                 ts[0] = 0.0;
                 for (int i = 0; i < ns; i++) ts[i + 1] = sp3[i];
@@ -582,30 +583,273 @@ __device__ void walk(NearCtx &c, Piece *stk, int sp)
         }
         int on[3], nc = 0;
         for (int k = 0; k < 3; k++) {
-            on[k] = cls_tri_distance(c.SX, p[k]) <= c.tol;
+            on[k] = cls_tri_distance(g.SX, p[k]) <= g.tol;
             nc += on[k];
         }
         if (nc == 2) {
             int k = !on[2] ? 0 : !on[0] ? 1 : 2;
             V3<double> mid = p[k] + p[(k + 1) % 3];
             mid = mk3(mid.x / 2, mid.y / 2, mid.z / 2);
-            if (cls_tri_distance(c.SX, mid) <= c.tol) {
-                eval_leaf(c, R_EDGE0 + k, pc.b, pc.frac);
+            if (cls_tri_distance(g.SX, mid) <= g.tol) {
+                gen_emit(g, R_EDGE0 + k, pc.b, pc.frac);
                 continue;
             }
         }
         if (nc == 1) {
-            vertex_pieces(c, p, on[0] ? 0 : on[1] ? 1 : 2, pc.b, pc.frac);
+            vertex_pieces(g, p, on[0] ? 0 : on[1] ? 1 : 2, pc.b, pc.frac);
         } else if (nc == 0) {
             Piece ch = pc;
             ch.kind = 0;
             ch.level = 0;
-            ch.depth = c.P->near_depth;
+            ch.depth = g.near_depth;
             stk[sp++] = ch;
         } else {
-            eval_leaf(c, R_SELF, pc.b, pc.frac);
+            gen_emit(g, R_SELF, pc.b, pc.frac);
         }
     }
+}
+
+// Classify one non-far pair exactly as pair_rule / segment_rule do (src/assembly/assemble.jl:266-306, after the far
+// test) and produce its rule pieces; returns the tier, or -1 for a plain near pair (near tier, no touch pair).
+__device__ int classify_and_generate(Gen &g, const AsmParams &P, const Elem *el, const int *elem_nodes,
+                                     const int2 *touch, int ntouch, bool force_walk)
+{
+#pragma clang fp contract(off)
+    double b0[3][3];
+    set_corners(b0);
+    const Elem &T = el[g.t];
+    if (g.seg) {
+        // segment_rule: the vertex rule where the filament starts at a corner of T, else the near rule
+        int kv = -1;
+        for (int k = 0; k < 3 && kv < 0; k++)
+            if (fmin(cls_norm(cls_sub(g.X[k], g.fa)), cls_norm(cls_sub(g.X[k], g.fb))) <= 1e-12 * T.diam) kv = k;
+        if (kv >= 0) {
+            vertex_rule(g, kv);
+            return T_VERTEX;
+        }
+        Piece r0;
+        set_corners(r0.b);
+        r0.frac = 1.0;
+        r0.kind = 0;
+        r0.level = 0;
+        r0.depth = g.near_depth;
+        walk(g, r0);
+        return T_NEAR;
+    }
+    if (g.s == g.t) {
+        gen_emit(g, R_SELF, b0, 1.0);
+        return T_SELF;
+    }
+    const Elem &S = el[g.s];
+    int shared[3];
+    for (int k = 0; k < 3; k++) {
+        int v = elem_nodes[3 * g.t + k];
+        shared[k] = v == elem_nodes[3 * g.s] || v == elem_nodes[3 * g.s + 1] || v == elem_nodes[3 * g.s + 2];
+    }
+    // contact_rule, src/assembly/assemble.jl:199-207
+    g.tol = P.contact_rel * fmin(T.diam, S.diam);
+    if (cls_contact_possible(ld3(S.c), ld3(T.c), S.reach, T.reach, S.diam, T.diam, P.contact_rel)) {
+        bool extra = false;
+        for (int k = 0; k < 3 && !extra; k++) extra = !shared[k] && cls_tri_distance(g.SX, g.X[k]) <= g.tol;
+        for (int k = 0; k < 3 && !extra; k++) {
+            double ts3[3];
+            extra = edge_splits_k(g, g.X, k, ts3) > 0;
+        }
+        if (extra) {
+            Piece r0;
+            set_corners(r0.b);
+            r0.frac = 1.0;
+            r0.kind = 1;
+            r0.level = 0;
+            r0.depth = 0;
+            walk(g, r0);
+            return T_CONTACT;
+        }
+    }
+    int ns = shared[0] + shared[1] + shared[2];
+    if (ns == 2) {
+        gen_emit(g, R_EDGE0 + (shared[0] && shared[1] ? 0 : shared[1] && shared[2] ? 1 : 2), b0, 1.0);
+        return T_EDGE;
+    }
+    if (ns == 1) {
+        vertex_rule(g, shared[0] ? 0 : shared[1] ? 1 : 2);
+        return T_VERTEX;
+    }
+    bool tp = false;
+    if (g.s < P.nt && ntouch) {  // geometric touch pairs get a deeper near rule (assemble.jl:280)
+        int lo = 0, hi = ntouch;
+        while (lo < hi) {
+            int mid = (lo + hi) / 2;
+            int2 q = touch[mid];
+            if (q.x < g.t || (q.x == g.t && q.y < g.s)) lo = mid + 1; else hi = mid;
+        }
+        tp = lo < ntouch && touch[lo].x == g.t && touch[lo].y == g.s;
+    }
+    if (!tp && !force_walk) return -1;  // a plain near pair (near_thread / near_group take it)
+    Piece r0;
+    set_corners(r0.b);
+    r0.frac = 1.0;
+    r0.kind = 0;
+    r0.level = 0;
+    r0.depth = g.near_depth + (tp ? 4 : 0);
+    walk(g, r0);
+    return T_NEAR;
+}
+
+constexpr int NG_MAXDEPTH = 5;  // deepest near rule the parallel-tree kernel (near_group) takes
+
+// One thread per non-far pair, a router that decides exactly as pair_rule / segment_rule do (assemble.jl:266-306,
+// after the far test) but builds no rule: pairs whose rule needs the piece recursion (contact per contact_rule's
+// test, geometric touch pairs, filaments, near rules deeper than near_group takes) go to the general touching list
+// (touch_gen); the self pair and shared-edge / shared-corner pairs to the simple touching list as (t, s, rule code,
+// tier); plain near pairs to near_thread when the root piece is not split (near_pieces!' first test, assemble.jl:134,
+// on the whole triangle, whose corners, size and centroid are the element's own) and to near_group otherwise.
+__global__ __launch_bounds__(256) void triage(AsmParams P, const Elem *__restrict__ el, const int *__restrict__ elem_nodes,
+                                              const int2 *__restrict__ list, unsigned int npairs,
+                                              const int2 *__restrict__ touch, int ntouch, int2 *__restrict__ touch_out,
+                                              int2 *__restrict__ single_out, int2 *__restrict__ multi_out,
+                                              int4 *__restrict__ simple_out, unsigned int *__restrict__ counts,
+                                              unsigned int cap)
+{
+#pragma clang fp contract(off)
+    for (unsigned int pi = blockIdx.x * blockDim.x + threadIdx.x; pi < npairs; pi += gridDim.x * blockDim.x) {
+        const int2 pr = list[pi];
+        const int t = pr.x, s = pr.y;
+        int route = 0;  // 0 general touching, 1 single near piece, 2 several near pieces, 3 simple touching
+        int code = 0, tier = T_SELF;
+        if (s == t) {
+            route = 3;
+            code = R_SELF;
+            tier = T_SELF;
+        } else if (s < P.nsrc) {
+            const Elem &T = el[t], &S = el[s];
+            int sh[3];
+            for (int a = 0; a < 3; a++) {
+                int va = elem_nodes[3 * t + a];
+                sh[a] = va == elem_nodes[3 * s] || va == elem_nodes[3 * s + 1] || va == elem_nodes[3 * s + 2];
+            }
+            const int ns = sh[0] + sh[1] + sh[2];
+            bool contact = false;
+            const V3<double> SX[3] = {ld3(S.v[0]), ld3(S.v[1]), ld3(S.v[2])};
+            if (cls_contact_possible(ld3(S.c), ld3(T.c), S.reach, T.reach, S.diam, T.diam, P.contact_rel)) {
+                const double tol = P.contact_rel * fmin(T.diam, S.diam);
+                const V3<double> X[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
+                for (int k = 0; k < 3 && !contact; k++) contact = !sh[k] && cls_tri_distance(SX, X[k]) <= tol;
+                for (int k = 0; k < 3 && !contact; k++) {  // splits_an_edge, assemble.jl:174-184
+                    const V3<double> a = X[k], e = X[(k + 1) % 3];
+                    const double L = cls_norm(cls_sub(e, a));
+                    for (int q = 0; q < 3 && !contact; q++) {
+                        double tq;
+                        const double d = cls_seg_projection(SX[q], a, e, &tq);
+                        contact = d <= tol && tq * L > tol && (1 - tq) * L > tol;
+                    }
+                }
+            }
+            if (contact) {
+                route = 0;
+            } else if (ns == 2) {
+                route = 3;
+                code = R_EDGE0 + (sh[0] && sh[1] ? 0 : sh[1] && sh[2] ? 1 : 2);
+                tier = T_EDGE;
+            } else if (ns == 1) {
+                route = 3;
+                code = R_VERT0 + (sh[0] ? 0 : sh[1] ? 1 : 2);
+                tier = T_VERTEX;
+            } else {
+                bool tp = false;
+                if (ntouch && s < P.nt) {
+                    int lo = 0, hi = ntouch;
+                    while (lo < hi) {
+                        int mid = (lo + hi) / 2;
+                        int2 q = touch[mid];
+                        if (q.x < t || (q.x == t && q.y < s)) lo = mid + 1; else hi = mid;
+                    }
+                    tp = lo < ntouch && touch[lo].x == t && touch[lo].y == s;
+                }
+                if (!tp && P.near_depth <= NG_MAXDEPTH) {
+                    const bool split = P.near_depth > 0 && cls_tri_distance(SX, ld3(T.c)) < P.near_ratio * T.diam;
+                    route = split ? 2 : 1;
+                }
+            }
+        }
+        // one counter update per wavefront and route (the routes' counters are shared by the whole grid)
+        const int lane = threadIdx.x & (WAVE - 1);
+        unsigned int pos = 0;
+        for (int r = 0; r < 4; r++) {
+            const unsigned long long mask = __ballot(route == r);
+            if (!mask) continue;
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned int base = 0;
+            if (lane == leader) base = atomicAdd(&counts[r], (unsigned int)__popcll(mask));
+            base = __shfl(base, leader);
+            if (route == r) pos = base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+        }
+        if (pos < cap) {
+            if (route == 3)
+                simple_out[pos] = make_int4(t, s, code, tier);
+            else
+                (route == 0 ? touch_out : route == 1 ? single_out : multi_out)[pos] = pr;
+        }
+    }
+}
+
+// One thread per touching pair: its exact tier (pair_rule / segment_rule after the far test) and its rule pieces,
+// counted first, then written contiguously into the leaf list at a base reserved with one atomic, with one group entry
+// (first piece, pieces, points, tier).
+__global__ __launch_bounds__(256) void touch_gen(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+                                                 const int *__restrict__ elem_nodes, const double *__restrict__ seg,
+                                                 const int2 *__restrict__ list, unsigned int npairs,
+                                                 const int2 *__restrict__ touch, int ntouch, Leaf *__restrict__ leaves,
+                                                 unsigned int leaf_cap, int4 *__restrict__ groups, unsigned int group_cap,
+                                                 unsigned int *__restrict__ counts, unsigned long long *__restrict__ stats,
+                                                 uint8_t *__restrict__ tiers)
+{
+    __shared__ unsigned long long bst[13];
+    if (threadIdx.x < 13) bst[threadIdx.x] = 0;
+    __syncthreads();
+    for (unsigned int pi = blockIdx.x * blockDim.x + threadIdx.x; pi < npairs; pi += gridDim.x * blockDim.x) {
+        const int2 pr = list[pi];
+        Gen g;
+        g.t = pr.x;
+        g.s = pr.y;
+        g.seg = pr.y >= P.nsrc;
+        g.near_ratio = P.near_ratio;
+        g.corner_angle = P.corner_angle;
+        g.near_depth = P.near_depth;
+        g.rn = rt.n;
+        g.tol = 0.0;
+        const Elem &T = el[g.t];
+        for (int k = 0; k < 3; k++) g.X[k] = ld3(T.v[k]);
+        if (g.seg) {
+            const int f = g.s - P.nsrc;
+            g.fa = ld3(seg + 6 * f);
+            g.fb = ld3(seg + 6 * f + 3);
+        } else {
+            for (int k = 0; k < 3; k++) g.SX[k] = ld3(el[g.s].v[k]);
+        }
+        g.out = nullptr;
+        g.n = 0;
+        g.npts = 0;
+        g.overflow = false;
+        const int tier = classify_and_generate(g, P, el, elem_nodes, touch, ntouch, true);
+        const unsigned int n = g.n;
+        const long long npts = g.npts;
+        g.base = atomicAdd(&counts[0], n);
+        const unsigned int gi = atomicAdd(&counts[1], 1u);
+        g.cap = leaf_cap;
+        g.out = leaves;
+        g.n = 0;
+        g.npts = 0;
+        (void)classify_and_generate(g, P, el, elem_nodes, touch, ntouch, true);
+        if (gi < group_cap) groups[gi] = make_int4((int)g.base, (int)n, (int)npts, tier);
+        atomicAdd(&bst[tier], 1ull);
+        atomicAdd(&bst[6 + tier], (unsigned long long)npts);
+        if (g.overflow) atomicAdd(&bst[12], 1ull);
+        if (tiers && !g.seg) tiers[(size_t)g.t * P.nsrc + g.s] = (uint8_t)tier;
+    }
+    __syncthreads();
+    if (threadIdx.x < 12 && bst[threadIdx.x]) atomicAdd(&stats[threadIdx.x], bst[threadIdx.x]);
+    if (threadIdx.x == 12 && bst[12]) atomicAdd(&stats[13], bst[12]);
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -614,157 +858,96 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-__global__ __launch_bounds__(256) void near_pairs(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
-                                                  const int *__restrict__ elem_nodes, const double *__restrict__ sigma,
-                                                  const int *__restrict__ corner_slot, const int *__restrict__ src_cols,
-                                                  const double *__restrict__ seg, const int *__restrict__ segcol,
-                                                  const int2 *__restrict__ list, unsigned int npairs,
-                                                  const int2 *__restrict__ touch, int ntouch, double *__restrict__ A,
-                                                  double *__restrict__ bvec, unsigned long long *__restrict__ stats,
-                                                  uint8_t *__restrict__ tiers)
+// One wavefront per pair group: the points of all the pair's rule pieces flattened over the 64 lanes (a lane walks
+// the pieces with a cursor), in FP64 (pair_block, assemble.jl:245-258, or segment_block, :312-321), reduced over the
+// wave and added to the pair's block of A and its rows of b.
+__global__ __launch_bounds__(256) void group_eval(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+                                                  const double *__restrict__ sigma, const int *__restrict__ corner_slot,
+                                                  const int *__restrict__ src_cols, const double *__restrict__ seg,
+                                                  const int *__restrict__ segcol, const Leaf *__restrict__ leaves,
+                                                  const int4 *__restrict__ groups, unsigned int ngroups,
+                                                  double *__restrict__ A, double *__restrict__ bvec)
 {
-    __shared__ Piece stack[4][STACK];
-    __shared__ unsigned long long wstat[12];
-    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
-    if (threadIdx.x < 12) wstat[threadIdx.x] = 0;
-    __syncthreads();
-    Piece *stk = stack[wave];
-    for (unsigned int pi = blockIdx.x * 4 + wave; pi < npairs; pi += gridDim.x * 4) {
-        const int2 pr = list[pi];
-        const int t = pr.x, s = pr.y;
-        NearCtx c;
-        c.P = &P;
-        c.rt = rt;
-        c.T = el + t;
-        c.test = P.test;
-        c.npts = 0;
-        for (int i = 0; i < 12; i++) c.acc[i] = 0.0;
-        for (int k = 0; k < 3; k++) c.X[k] = ld3(c.T->v[k]);
-        c.seg = s >= P.nsrc;
-        int tier = T_NEAR;
-        int sp = 0;
-        double b0[3][3];
-        set_corners(b0);
-        if (c.seg) {
-            // segment_rule, src/assembly/assemble.jl:299-306 (the far ones were taken by seg_pairs)
-            const int f = s - P.nsrc;
-            c.S = nullptr;
-            c.fa = ld3(seg + 6 * f);
-            c.fb = ld3(seg + 6 * f + 3);
-            c.side = 0;
-            c.sigma = 0.0;
-            int kv = -1;
-            {
-#pragma clang fp contract(off)
-                for (int k = 0; k < 3 && kv < 0; k++)
-                    if (fmin(cls_norm(c.X[k] - c.fa), cls_norm(c.X[k] - c.fb)) <= 1e-12 * c.T->diam) kv = k;
+    const int lane = threadIdx.x & (WAVE - 1), wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (unsigned int gi = blockIdx.x * 4 + wave; gi < ngroups; gi += gridDim.x * 4) {
+        const int4 grp = groups[gi];
+        const int t = leaves[grp.x].t, s = leaves[grp.x].s, ntot = grp.z;
+        const bool isseg = s >= P.nsrc;
+        const Elem &T = el[t];
+        const V3<double> X[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
+        const V3<double> tn = ld3(T.n);
+        double acc[12];
+        for (int i = 0; i < 12; i++) acc[i] = 0.0;
+        int li = grp.x, lbase = 0;
+#pragma unroll 1
+        for (int pp = lane; pp < ntot; pp += WAVE) {
+            while (pp >= lbase + leaves[li].npt) {
+                lbase += leaves[li].npt;
+                li++;
             }
-            if (kv >= 0) {
-                tier = T_VERTEX;
-                V3<double> u = c.X[(kv + 1) % 3] - c.X[kv], v = c.X[(kv + 2) % 3] - c.X[kv];
-                if (cls_dot(u, v) >= 0)
-                    eval_leaf(c, R_VERT0 + kv, b0, 1.0);
-                else
-                    vertex_pieces(c, c.X, kv, b0, 1.0);
-            } else {
-                Piece r0;
-                set_corners(r0.b);
-                r0.frac = 1.0;
-                r0.kind = 0;
-                r0.level = 0;
-                r0.depth = P.near_depth;
-                stk[sp++] = r0;
-                walk(c, stk, sp);
-            }
-            double ca[3], cb[3];
-            for (int a = 0; a < 3; a++) {
-                ca[a] = wave_sum(c.acc[a]);
-                cb[a] = wave_sum(c.acc[3 + a]);
-            }
-            if (lane == 0) {
-                const int *ct = corner_slot + 3 * t;
+            const Leaf &lf = leaves[li];
+            const int p = pp - lbase, off = rt.off[lf.rid];
+            const double *m = rt.l + 3 * (off + p);
+            double l[3];
+            for (int j = 0; j < 3; j++) l[j] = (m[0] * lf.b[0][j] + m[1] * lf.b[1][j]) + m[2] * lf.b[2][j];
+            const double W = rt.w[off + p] * lf.frac * T.area;
+            double psi[3];
+            test_weights(P.test, l, psi);
+            const V3<double> x = cls_point(X, l);
+            if (isseg) {
+                const int f = s - P.nsrc;
+                V3<double> va, vb;
+                seg_vel(x - ld3(seg + 6 * f), x - ld3(seg + 6 * f + 3), va, vb);
+                const double da = dot(va, tn), db = dot(vb, tn);
                 for (int a = 0; a < 3; a++) {
-                    unsafeAtomicAdd(&A[(long long)segcol[2 * f] * P.lda + ct[a]], ca[a]);
-                    unsafeAtomicAdd(&A[(long long)segcol[2 * f + 1] * P.lda + ct[a]], cb[a]);
+                    acc[a] += da * (W * psi[a]);
+                    acc[3 + a] += db * (W * psi[a]);
                 }
+                continue;
             }
-        } else {
-            c.S = el + s;
-            for (int k = 0; k < 3; k++) c.SX[k] = ld3(c.S->v[k]);
-            c.side = s == t ? 1 : 0;
-            c.sigma = s < P.nt ? sigma[s] : 0.0;
-            if (s == t) {
-                tier = T_SELF;
-                eval_leaf(c, R_SELF, b0, 1.0);
+            const Elem &S = el[s];
+            const V3<double> sc = ld3(S.c);
+            const V3<double> r = x - sc;
+            const double fd = P.kernel_far * S.diam;
+            V3<double> w;
+            if (cls_dot(r, r) > fd * fd) {
+                SrcT<double> sr;
+                for (int k = 0; k < 3; k++) sr.v[k] = ld3(S.v[k]) - sc;
+                sr.area = S.area;
+                w = sv_far_rel(sr, r);
             } else {
-                // pair_rule after the far test, src/assembly/assemble.jl:270-281
-                int shared[3];
+                SrcT<double> sr;
                 for (int k = 0; k < 3; k++) {
-                    int v = elem_nodes[3 * t + k];
-                    shared[k] = v == elem_nodes[3 * s] || v == elem_nodes[3 * s + 1] || v == elem_nodes[3 * s + 2];
+                    sr.nu[k] = ld3(S.nu[k]);
+                    sr.len[k] = S.len[k];
                 }
-                bool contact = false;
-                {
-#pragma clang fp contract(off)
-                    // contact_rule, src/assembly/assemble.jl:199-207
-                    c.tol = P.contact_rel * fmin(c.T->diam, c.S->diam);
-                    if (!(cls_norm(ld3(c.S->c) - ld3(c.T->c)) > (c.T->reach + c.S->reach) + c.tol)) {
-                        bool extra = false;
-                        for (int k = 0; k < 3 && !extra; k++) extra = !shared[k] && cls_tri_distance(c.SX, c.X[k]) <= c.tol;
-                        for (int k = 0; k < 3 && !extra; k++) {
-                            double ts3[3];
-                            extra = edge_splits_k(c, c.X, k, ts3) > 0;
-                        }
-                        contact = extra;
-                    }
-                }
-                int ns = shared[0] + shared[1] + shared[2];
-                if (contact) {
-                    tier = T_CONTACT;
-                    Piece r0;
-                    set_corners(r0.b);
-                    r0.frac = 1.0;
-                    r0.kind = 1;
-                    r0.level = 0;
-                    r0.depth = 0;
-                    stk[sp++] = r0;
-                    walk(c, stk, sp);
-                } else if (ns == 2) {
-                    tier = T_EDGE;
-                    eval_leaf(c, R_EDGE0 + (shared[0] && shared[1] ? 0 : shared[1] && shared[2] ? 1 : 2), b0, 1.0);
-                } else if (ns == 1) {
-                    tier = T_VERTEX;
-                    int kv = shared[0] ? 0 : shared[1] ? 1 : 2;
-                    V3<double> u = c.X[(kv + 1) % 3] - c.X[kv], v = c.X[(kv + 2) % 3] - c.X[kv];
-                    if (cls_dot(u, v) >= 0)
-                        eval_leaf(c, R_VERT0 + kv, b0, 1.0);
-                    else
-                        vertex_pieces(c, c.X, kv, b0, 1.0);
-                } else {
-                    bool tp = false;
-                    if (s < P.nt && ntouch) {  // touch pairs get a deeper near rule (assemble.jl:280)
-                        int lo = 0, hi = ntouch;
-                        while (lo < hi) {
-                            int mid = (lo + hi) / 2;
-                            int2 q = touch[mid];
-                            if (q.x < t || (q.x == t && q.y < s)) lo = mid + 1; else hi = mid;
-                        }
-                        tp = lo < ntouch && touch[lo].x == t && touch[lo].y == s;
-                    }
-                    Piece r0;
-                    set_corners(r0.b);
-                    r0.frac = 1.0;
-                    r0.kind = 0;
-                    r0.level = 0;
-                    r0.depth = P.near_depth + (tp ? 4 : 0);
-                    stk[sp++] = r0;
-                    walk(c, stk, sp);
-                }
+                sr.n = ld3(S.n);
+                sr.diam = S.diam;
+                w = sv_exact_abs(sr, x - ld3(S.v[0]), x - ld3(S.v[1]), x - ld3(S.v[2]), cls_dot(r, sr.n), s == t ? 1 : 0,
+                                 &S, x);
             }
-            double tot[12];
-            for (int i = 0; i < 12; i++) tot[i] = wave_sum(c.acc[i]);
-            if (lane == 0) {
-                const int *ct = corner_slot + 3 * t;
+            const V3<double> v = cross(w, tn);
+            const double c0 = dot(v, ld3(S.g[0])), c1 = dot(v, ld3(S.g[1])), c2 = dot(v, ld3(S.g[2]));
+            const double wn = (s < P.nt ? sigma[s] : 0.0) * dot(w, tn);
+            for (int a = 0; a < 3; a++) {
+                const double ps = W * psi[a];
+                acc[3 * a] += ps * c0;
+                acc[3 * a + 1] += ps * c1;
+                acc[3 * a + 2] += ps * c2;
+                acc[9 + a] -= wn * ps;
+            }
+        }
+        double tot[12];
+        for (int i = 0; i < 12; i++) tot[i] = wave_sum(acc[i]);
+        if (lane == 0) {
+            const int *ct = corner_slot + 3 * t;
+            if (isseg) {
+                const int f = s - P.nsrc;
+                for (int a = 0; a < 3; a++) {
+                    unsafeAtomicAdd(&A[(long long)segcol[2 * f] * P.lda + ct[a]], tot[a]);
+                    unsafeAtomicAdd(&A[(long long)segcol[2 * f + 1] * P.lda + ct[a]], tot[3 + a]);
+                }
+            } else {
                 const int *cs = src_cols + 3 * s;
                 for (int a = 0; a < 3; a++)
                     for (int bb = 0; bb < 3; bb++) unsafeAtomicAdd(&A[(long long)cs[bb] * P.lda + ct[a]], tot[3 * a + bb]);
@@ -772,14 +955,379 @@ __global__ __launch_bounds__(256) void near_pairs(AsmParams P, RuleTab rt, const
                     for (int a = 0; a < 3; a++) unsafeAtomicAdd(&bvec[ct[a]], tot[9 + a]);
             }
         }
+    }
+}
+
+// The self pair and shared-edge / shared-corner pairs (no contact): one wavefront per pair.  Its pieces are the
+// stored rule on the whole triangle (self, edge k, vertex k when the corner is at most 90 degrees) or vertex_pieces!'
+// equal-angle wedges of a wide corner (assemble.jl:102-125), kept in LDS; their points are flattened over the lanes.
+constexpr int MAXWEDGE = 16;
+
+__device__ __forceinline__ double wave_sum2(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void touch_simple(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+                                                    const double *__restrict__ sigma, const int *__restrict__ corner_slot,
+                                                    const int *__restrict__ src_cols, const int4 *__restrict__ list,
+                                                    unsigned int npairs, double *__restrict__ A, double *__restrict__ bvec,
+                                                    unsigned long long *__restrict__ stats, uint8_t *__restrict__ tiers)
+{
+    __shared__ double wedge[4][MAXWEDGE][10];
+    __shared__ unsigned long long wst[12];
+    const int lane = threadIdx.x & (WAVE - 1), wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (threadIdx.x < 12) wst[threadIdx.x] = 0;
+    __syncthreads();
+    for (unsigned int pi = blockIdx.x * 4 + wave; pi < npairs; pi += gridDim.x * 4) {
+        const int4 e = list[pi];
+        const int t = e.x, s = e.y, tier = e.w;
+        int rid = e.z, npc = 1;
+        const Elem &T = el[t], &S = el[s];
+        const V3<double> X[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) wedge[wave][0][3 * i + j] = i == j ? 1.0 : 0.0;
+        wedge[wave][0][9] = 1.0;
+        if (tier == T_VERTEX) {
+#pragma clang fp contract(off)
+            const int k = rid - R_VERT0, i = (k + 1) % 3, j = (k + 2) % 3;
+            const V3<double> u = cls_sub(X[i], X[k]), v = cls_sub(X[j], X[k]);
+            if (cls_dot(u, v) < 0) {  // vertex_rule's wide corner: vertex_pieces! with the base rule
+                rid = R_VERT0;
+                const double nu = cls_norm(u), nv = cls_norm(v);
+                double cs = cls_dot(u, v) / (nu * nv);
+                cs = cs < -1.0 ? -1.0 : cs > 1.0 ? 1.0 : cs;
+                const double th = acos(cs);
+                double b[3][3];
+                for (int a = 0; a < 3; a++)
+                    for (int c = 0; c < 3; c++) b[a][c] = a == c ? 1.0 : 0.0;
+                if (th <= kPi / 2) {
+                    for (int c = 0; c < 3; c++) {
+                        wedge[wave][0][c] = b[k][c];
+                        wedge[wave][0][3 + c] = b[i][c];
+                        wedge[wave][0][6 + c] = b[j][c];
+                    }
+                } else {
+                    int m = (int)ceil(th * (180 / kPi) / P.corner_angle);
+                    m = m < MAXWEDGE ? m : MAXWEDGE;
+                    double t0 = 0.0;
+                    for (int q = 1; q <= m; q++) {
+                        double t1 = 1.0;
+                        if (q < m) {
+                            const double ph = th * q / m;
+                            t1 = nu * sin(ph) / (nu * sin(ph) + nv * sin(th - ph));
+                        }
+                        for (int c = 0; c < 3; c++) {
+                            wedge[wave][q - 1][c] = b[k][c];
+                            wedge[wave][q - 1][3 + c] = (1 - t0) * b[i][c] + t0 * b[j][c];
+                            wedge[wave][q - 1][6 + c] = (1 - t1) * b[i][c] + t1 * b[j][c];
+                        }
+                        wedge[wave][q - 1][9] = t1 - t0;
+                        t0 = t1;
+                    }
+                    npc = m;
+                }
+            }
+        }
+        const int nr = rt.n[rid], off = rt.off[rid], ntot = npc * nr;
+        const int side = s == t ? 1 : 0;
+        const V3<double> tn = ld3(T.n), sc = ld3(S.c);
+        const double fd = P.kernel_far * S.diam;
+        double acc[12];
+        for (int q = 0; q < 12; q++) acc[q] = 0.0;
+#pragma unroll 1
+        for (int pp = lane; pp < ntot; pp += WAVE) {
+            const int pc = pp / nr, q = pp - pc * nr;
+            const double *bl = wedge[wave][pc];
+            const double *m = rt.l + 3 * (off + q);
+            double l[3];
+            for (int j = 0; j < 3; j++) l[j] = (m[0] * bl[j] + m[1] * bl[3 + j]) + m[2] * bl[6 + j];
+            const double W = rt.w[off + q] * bl[9] * T.area;
+            double psi[3];
+            test_weights(P.test, l, psi);
+            const V3<double> x = cls_point(X, l);
+            const V3<double> r = x - sc;
+            V3<double> w;
+            if (cls_dot(r, r) > fd * fd) {
+                SrcT<double> sr;
+                for (int k = 0; k < 3; k++) sr.v[k] = ld3(S.v[k]) - sc;
+                sr.area = S.area;
+                w = sv_far_rel(sr, r);
+            } else {
+                SrcT<double> sr;
+                for (int k = 0; k < 3; k++) {
+                    sr.nu[k] = ld3(S.nu[k]);
+                    sr.len[k] = S.len[k];
+                }
+                sr.n = ld3(S.n);
+                sr.diam = S.diam;
+                w = sv_exact_abs(sr, x - ld3(S.v[0]), x - ld3(S.v[1]), x - ld3(S.v[2]), cls_dot(r, sr.n), side,
+                                 side ? &S : nullptr, x);
+            }
+            const V3<double> v = cross(w, tn);
+            const double wn = dot(w, tn);
+            for (int a = 0; a < 3; a++) {
+                const double ps = W * psi[a];
+                acc[3 * a] += ps * v.x;
+                acc[3 * a + 1] += ps * v.y;
+                acc[3 * a + 2] += ps * v.z;
+                acc[9 + a] += ps * wn;
+            }
+        }
+        double tot[12];
+        for (int q = 0; q < 12; q++) tot[q] = wave_sum2(acc[q]);
         if (lane == 0) {
-            atomicAdd(&wstat[tier], 1ull);
-            atomicAdd(&wstat[6 + tier], (unsigned long long)c.npts);
-            if (tiers && !c.seg) tiers[(size_t)t * P.nsrc + s] = (uint8_t)tier;
+            const int *ct = corner_slot + 3 * t;
+            const int *cs = src_cols + 3 * s;
+            const double sig = s < P.nt ? sigma[s] : 0.0;
+            for (int a = 0; a < 3; a++) {
+                const V3<double> va = mk3(tot[3 * a], tot[3 * a + 1], tot[3 * a + 2]);
+                for (int bb = 0; bb < 3; bb++) unsafeAtomicAdd(&A[(long long)cs[bb] * P.lda + ct[a]], dot(va, ld3(S.g[bb])));
+                if (s < P.nt) unsafeAtomicAdd(&bvec[ct[a]], -sig * tot[9 + a]);
+            }
+            if (tiers) tiers[(size_t)t * P.nsrc + s] = (uint8_t)tier;
+            atomicAdd(&wst[tier], 1ull);
+            atomicAdd(&wst[6 + tier], (unsigned long long)ntot);
         }
     }
     __syncthreads();
-    if (threadIdx.x < 12 && wstat[threadIdx.x]) atomicAdd(&stats[threadIdx.x], wstat[threadIdx.x]);
+    if (threadIdx.x < 12 && wst[threadIdx.x]) atomicAdd(&stats[threadIdx.x], wst[threadIdx.x]);
+}
+
+// One point of a plain near pair's rule: accumulates W psi_a (w x n_T) (acc[3a..3a+2]) and W psi_a (w . n_T)
+// (acc[9+a]); the sheets gamma_b and the source strength are applied once per pair (pair_block, assemble.jl:245-258).
+__device__ __forceinline__ void near_point(const V3<double> *X, const V3<double> *SX, const Elem &S, V3<double> tn,
+                                           V3<double> sc, double fd2, double W, const double *l, int test, double *acc)
+{
+    double psi[3];
+    test_weights(test, l, psi);
+    const V3<double> x = cls_point(X, l);
+    const V3<double> r = x - sc;
+    V3<double> w;
+    if (cls_dot(r, r) > fd2) {
+        SrcT<double> sr;
+        for (int k = 0; k < 3; k++) sr.v[k] = SX[k] - sc;
+        sr.area = S.area;
+        w = sv_far_rel(sr, r);
+    } else {
+        SrcT<double> sr;
+        for (int k = 0; k < 3; k++) {
+            sr.nu[k] = ld3(S.nu[k]);
+            sr.len[k] = S.len[k];
+        }
+        sr.n = ld3(S.n);
+        sr.diam = S.diam;
+        w = sv_exact_abs(sr, x - SX[0], x - SX[1], x - SX[2], cls_dot(r, sr.n), 0, nullptr, x);
+    }
+    const V3<double> v = cross(w, tn);
+    const double wn = dot(w, tn);
+    for (int a = 0; a < 3; a++) {
+        const double ps = W * psi[a];
+        acc[3 * a] += ps * v.x;
+        acc[3 * a + 1] += ps * v.y;
+        acc[3 * a + 2] += ps * v.z;
+        acc[9 + a] += ps * wn;
+    }
+}
+
+// The pair's block from the accumulated sums: A[ct_a, cs_b] += (sum W psi_a w x n) . gamma_b, b[ct_a] -= sigma sum ...
+__device__ __forceinline__ void near_scatter(const AsmParams &P, int t, int s, const Elem &S, const double *tot,
+                                             const double *sigma, const int *corner_slot, const int *src_cols, double *A,
+                                             double *bvec)
+{
+    const int *ct = corner_slot + 3 * t;
+    const int *cs = src_cols + 3 * s;
+    const double sig = s < P.nt ? sigma[s] : 0.0;
+    for (int a = 0; a < 3; a++) {
+        const V3<double> va = mk3(tot[3 * a], tot[3 * a + 1], tot[3 * a + 2]);
+        for (int bb = 0; bb < 3; bb++) unsafeAtomicAdd(&A[(long long)cs[bb] * P.lda + ct[a]], dot(va, ld3(S.g[bb])));
+        if (s < P.nt) unsafeAtomicAdd(&bvec[ct[a]], -sig * tot[9 + a]);
+    }
+}
+
+// Plain near pairs whose near rule is the single root piece (near_rule, assemble.jl:69-73): one thread per pair,
+// the 7-point rule on the whole test triangle.
+__global__ __launch_bounds__(256) void near_thread(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+                                                   const double *__restrict__ sigma, const int *__restrict__ corner_slot,
+                                                   const int *__restrict__ src_cols, const int2 *__restrict__ list,
+                                                   unsigned int npairs, double *__restrict__ A, double *__restrict__ bvec,
+                                                   unsigned long long *__restrict__ stats, uint8_t *__restrict__ tiers)
+{
+    __shared__ unsigned long long wst;
+    if (threadIdx.x == 0) wst = 0;
+    __syncthreads();
+    const int nr = rt.n[R_NEAR], off = rt.off[R_NEAR];
+    unsigned long long mypairs = 0;
+    for (unsigned int pi = blockIdx.x * blockDim.x + threadIdx.x; pi < npairs; pi += gridDim.x * blockDim.x) {
+        const int2 pr = list[pi];
+        const int t = pr.x, s = pr.y;
+        const Elem &T = el[t], &S = el[s];
+        const V3<double> X[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
+        const V3<double> SX[3] = {ld3(S.v[0]), ld3(S.v[1]), ld3(S.v[2])};
+        const double fd = P.kernel_far * S.diam;
+        double acc[12];
+        for (int i = 0; i < 12; i++) acc[i] = 0.0;
+#pragma unroll 1
+        for (int q = 0; q < nr; q++)
+            near_point(X, SX, S, ld3(T.n), ld3(S.c), fd * fd, rt.w[off + q] * T.area, rt.l + 3 * (off + q), P.test, acc);
+        near_scatter(P, t, s, S, acc, sigma, corner_slot, src_cols, A, bvec);
+        if (tiers) tiers[(size_t)t * P.nsrc + s] = (uint8_t)T_NEAR;
+        mypairs++;
+    }
+    if (mypairs) atomicAdd(&wst, mypairs);
+    __syncthreads();
+    if (threadIdx.x == 0 && wst) {
+        atomicAdd(&stats[T_NEAR], wst);
+        atomicAdd(&stats[6 + T_NEAR], wst * (unsigned long long)nr);
+    }
+}
+
+// Corners of quadtree node (level L, index idx: its base-4 digits, most significant first, are the children taken
+// from the root), by the same quarters in the same order as near_pieces! (assemble.jl:79-82, 131-142).
+__device__ __forceinline__ void node_corners(int L, int idx, double (*b)[3])
+{
+#pragma clang fp contract(off)
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) b[i][j] = i == j ? 1.0 : 0.0;
+    for (int i = 0; i < L; i++) {
+        const int c = (idx >> (2 * (L - 1 - i))) & 3;
+        double m12[3], m23[3], m31[3];
+        for (int j = 0; j < 3; j++) {
+            m12[j] = (b[0][j] + b[1][j]) / 2;
+            m23[j] = (b[1][j] + b[2][j]) / 2;
+            m31[j] = (b[2][j] + b[0][j]) / 2;
+        }
+        for (int j = 0; j < 3; j++) {
+            const double b0 = b[0][j], b1 = b[1][j], b2 = b[2][j];
+            b[0][j] = c == 0 ? b0 : c == 1 ? m12[j] : c == 2 ? m31[j] : m12[j];
+            b[1][j] = c == 0 ? m12[j] : c == 1 ? b1 : c == 2 ? m23[j] : m23[j];
+            b[2][j] = c == 0 ? m31[j] : c == 1 ? m23[j] : c == 2 ? b2 : m31[j];
+        }
+    }
+}
+
+__device__ __forceinline__ void node_of(int nid, int &L, int &idx)  // level-order id -> (level, index)
+{
+    L = 0;
+    int off = 0, cnt = 1;
+    while (nid >= off + cnt) {
+        off += cnt;
+        cnt *= 4;
+        L++;
+    }
+    idx = nid - off;
+}
+
+// Plain near pairs whose root piece splits: one wavefront per pair evaluates the split test of every node of the
+// full quadtree at once across its lanes (each test depends on that node's own corners only, so this is the
+// recursive near_pieces! exactly), collects the leaves -- nodes whose ancestors all split and that do not split or
+// reach the depth -- by ballot, and evaluates their 7-point rules flattened over the lanes.
+__global__ __launch_bounds__(256) void near_group(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+                                                  const double *__restrict__ sigma, const int *__restrict__ corner_slot,
+                                                  const int *__restrict__ src_cols, const int2 *__restrict__ list,
+                                                  unsigned int npairs, double *__restrict__ A, double *__restrict__ bvec,
+                                                  unsigned long long *__restrict__ stats, uint8_t *__restrict__ tiers)
+{
+    constexpr int MAXINT = (1 << (2 * NG_MAXDEPTH)) / 3 + 1;  // internal nodes of a depth-5 tree (341)
+    constexpr int MAXLEAF = 1 << (2 * NG_MAXDEPTH);           // 1024
+    __shared__ unsigned long long splitw[4][(MAXINT + 63) / 64];
+    __shared__ unsigned short leaf[4][MAXLEAF];
+    __shared__ unsigned long long wst[2];
+    const int lane = threadIdx.x & (WAVE - 1), wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (threadIdx.x < 2) wst[threadIdx.x] = 0;
+    __syncthreads();
+    const int depth = P.near_depth;
+    int nint = 0, ncand = 1;
+    for (int Lv = 0, c = 1; Lv <= depth; Lv++, c *= 4) {
+        if (Lv < depth) nint += c;
+        if (Lv > 0) ncand += c;
+    }
+    const int nr = rt.n[R_NEAR], off = rt.off[R_NEAR];
+    unsigned long long mypairs = 0, mypts = 0;
+    for (unsigned int pi = blockIdx.x * 4 + wave; pi < npairs; pi += gridDim.x * 4) {
+        const int2 pr = list[pi];
+        const int t = pr.x, s = pr.y;
+        const Elem &T = el[t], &S = el[s];
+        const V3<double> X[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
+        const V3<double> SX[3] = {ld3(S.v[0]), ld3(S.v[1]), ld3(S.v[2])};
+        // 1. split tests of all internal nodes
+        for (int base = 0; base < nint; base += WAVE) {
+            const int nid = base + lane;
+            bool split = false;
+            if (nid < nint) {
+#pragma clang fp contract(off)
+                int L, idx;
+                node_of(nid, L, idx);
+                double b[3][3];
+                node_corners(L, idx, b);
+                const V3<double> p1 = cls_point(X, b[0]), p2 = cls_point(X, b[1]), p3 = cls_point(X, b[2]);
+                const double size = fmax(fmax(cls_norm(cls_sub(p2, p1)), cls_norm(cls_sub(p3, p2))), cls_norm(cls_sub(p1, p3)));
+                V3<double> cen = (p1 + p2) + p3;
+                cen = mk3(cen.x / 3.0, cen.y / 3.0, cen.z / 3.0);
+                split = cls_tri_distance(SX, cen) < P.near_ratio * size;
+            }
+            const unsigned long long m = __ballot(split);
+            if (lane == 0) splitw[wave][base / WAVE] = m;
+        }
+        // 2. leaves: every ancestor splits, the node itself does not (or is at the depth)
+        int nl = 0;
+        for (int base = 0; base < ncand; base += WAVE) {
+            const int nid = base + lane;
+            bool isleaf = false;
+            if (nid < ncand) {
+                int L, idx;
+                node_of(nid, L, idx);
+                bool reach = true;
+                for (int Lu = L - 1, iu = idx >> 2, offu = 0; Lu >= 0 && reach; Lu--, iu >>= 2) {
+                    offu = 0;
+                    for (int k = 0, c = 1; k < Lu; k++, c *= 4) offu += c;
+                    const int aid = offu + iu;
+                    reach = (splitw[wave][aid >> 6] >> (aid & 63)) & 1ull;
+                }
+                const bool own = nid < nint ? ((splitw[wave][nid >> 6] >> (nid & 63)) & 1ull) != 0 : false;
+                isleaf = reach && !own;
+            }
+            const unsigned long long m = __ballot(isleaf);
+            if (isleaf) leaf[wave][nl + __popcll(m & ((1ull << lane) - 1ull))] = (unsigned short)nid;
+            nl += __popcll(m);
+        }
+        // 3. the leaves' points
+        const double fd = P.kernel_far * S.diam;
+        double acc[12];
+        for (int i = 0; i < 12; i++) acc[i] = 0.0;
+#pragma unroll 1
+        for (int pp = lane; pp < nl * nr; pp += WAVE) {
+            const int li = pp / nr, q = pp - li * nr;
+            int L, idx;
+            node_of(leaf[wave][li], L, idx);
+            double b[3][3];
+            node_corners(L, idx, b);
+            const double *m = rt.l + 3 * (off + q);
+            double l[3];
+            for (int j = 0; j < 3; j++) l[j] = (m[0] * b[0][j] + m[1] * b[1][j]) + m[2] * b[2][j];
+            double frac = 1.0;
+            for (int i = 0; i < L; i++) frac /= 4;
+            near_point(X, SX, S, ld3(T.n), ld3(S.c), fd * fd, rt.w[off + q] * frac * T.area, l, P.test, acc);
+        }
+        double tot[12];
+        for (int i = 0; i < 12; i++) tot[i] = wave_sum(acc[i]);
+        if (lane == 0) {
+            near_scatter(P, t, s, S, tot, sigma, corner_slot, src_cols, A, bvec);
+            if (tiers) tiers[(size_t)t * P.nsrc + s] = (uint8_t)T_NEAR;
+        }
+        mypairs++;
+        mypts += (unsigned long long)nl * nr;
+    }
+    if (lane == 0 && mypairs) {
+        atomicAdd(&wst[0], mypairs);
+        atomicAdd(&wst[1], mypts);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && wst[0]) {
+        atomicAdd(&stats[T_NEAR], wst[0]);
+        atomicAdd(&stats[6 + T_NEAR], wst[1]);
+    }
 }
 
 // freestream part of b (exact), src/assembly/assemble.jl:422
@@ -906,7 +1454,7 @@ template <typename X> X *dupload(const std::vector<X> &v, hipStream_t s)
 
 Plan::~Plan()
 {
-    for (int k = 0; k < 5; k++)
+    for (int k = 0; k < 6; k++)
         if (ev[k]) (void)hipEventDestroy(ev[k]);
     for (void *p : owned) (void)hipFree(p);
     if (A && ownsA) (void)hipFree(A);
@@ -1064,12 +1612,20 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
     d_b = dalloc<double>(nslot);
     d_sigma = dalloc<double>(nt);
     d_stats = dalloc<unsigned long long>(16);
-    d_count = dalloc<unsigned int>(2);
+    d_count = dalloc<unsigned int>(7);
     owned.insert(owned.end(), {d_b, d_sigma, d_stats, d_count});
-    near_cap = (unsigned int)std::min<long long>((long long)nt * 160 + 1024, 1ll << 31);
-    d_near = dalloc<int2>(near_cap);
-    owned.push_back(d_near);
-    for (int k = 0; k < 5; k++) HIPCHECK(hipEventCreate(&ev[k]));
+    nonfar_cap = (unsigned int)std::min<long long>((long long)nt * 160 + 1024, 1ll << 31);
+    leaf_cap = (unsigned int)std::min<long long>((long long)nt * 24 + 1024, 1ll << 31);
+    group_cap = (unsigned int)std::min<long long>((long long)nt * 24 + 1024, 1ll << 31);
+    d_nonfar = dalloc<int2>(nonfar_cap);
+    d_touchp = dalloc<int2>(nonfar_cap);
+    d_near = dalloc<int2>(nonfar_cap);
+    d_nearm = dalloc<int2>(nonfar_cap);
+    d_simple = dalloc<int4>(nonfar_cap);
+    d_leaves = dalloc<Leaf>(leaf_cap);
+    d_groups = dalloc<int4>(group_cap);
+    owned.insert(owned.end(), {d_nonfar, d_touchp, d_near, d_nearm, d_simple, d_leaves, d_groups});
+    for (int k = 0; k < 6; k++) HIPCHECK(hipEventCreate(&ev[k]));
     if (record_tiers) {
         d_tiers = dalloc<uint8_t>((size_t)nt * nsrc);
         owned.push_back(d_tiers);
@@ -1081,11 +1637,14 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
 {
     hipStream_t st = ctx->stream;
     HIPCHECK(hipMemcpyAsync(d_sigma, sigma, sizeof(double) * nt, host_sigma ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, st));
-    for (int attempt = 0; attempt < 2; attempt++) {
+    // The work lists grow when a pass finds more pairs than budgeted, and the assembly is redone; each grows at
+    // most once per list, so three redos cover every case.
+    for (int attempt = 0;; attempt++) {
+        if (attempt == 4) throw std::runtime_error("work lists did not fit after regrowing");
         HIPCHECK(hipMemsetAsync(A, 0, sizeof(double) * (size_t)lda * ncol, st));
         HIPCHECK(hipMemsetAsync(d_b, 0, sizeof(double) * nslot, st));
         HIPCHECK(hipMemsetAsync(d_stats, 0, sizeof(unsigned long long) * 16, st));
-        HIPCHECK(hipMemsetAsync(d_count, 0, sizeof(unsigned int) * 2, st));
+        HIPCHECK(hipMemsetAsync(d_count, 0, sizeof(unsigned int) * 7, st));
         if (d_tiers) HIPCHECK(hipMemsetAsync(d_tiers, 255, (size_t)nt * nsrc, st));
         AsmParams P;
         P.nt = nt;
@@ -1106,57 +1665,115 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
         if (timing) HIPCHECK(hipEventRecord(ev[0], st));
         if (opts.precision == 1)
             far_pairs<float><<<grid, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc, d_tsn,
-                                                   A, d_b, d_near, d_count, near_cap, d_stats + 5, d_tiers);
+                                                   A, d_b, d_nonfar, d_count, nonfar_cap, d_stats + 5, d_tiers);
         else
             far_pairs<double><<<grid, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc,
-                                                    d_tsn, A, d_b, d_near, d_count, near_cap, d_stats + 5, d_tiers);
+                                                    d_tsn, A, d_b, d_nonfar, d_count, nonfar_cap, d_stats + 5, d_tiers);
         HIPCHECK(hipGetLastError());
-        if (timing) HIPCHECK(hipEventRecord(ev[1], st));
         if (nseg) {
             long long np = (long long)nt * nseg;
-            seg_pairs<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(P, d_tf, d_cs, d_seg, d_segcol, A, d_near, d_count,
-                                                                    near_cap, d_stats + 5);
+            seg_pairs<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(P, d_tf, d_cs, d_seg, d_segcol, A, d_nonfar, d_count,
+                                                                    nonfar_cap, d_stats + 5);
             HIPCHECK(hipGetLastError());
         }
-        unsigned int cnt = 0;
-        HIPCHECK(hipMemcpyAsync(&cnt, d_count, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        if (timing) HIPCHECK(hipEventRecord(ev[1], st));
+        unsigned int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+        HIPCHECK(hipMemcpyAsync(cnt, d_count, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
-        if (cnt > near_cap) {  // rare: more near pairs than budgeted; grow and redo
-            (void)hipFree(d_near);
-            owned.erase(std::find(owned.begin(), owned.end(), (void *)d_near));
-            near_cap = cnt + 1024;
-            d_near = dalloc<int2>(near_cap);
-            owned.push_back(d_near);
+        if (cnt[0] > nonfar_cap) {  // more non-far pairs than budgeted: grow every pair list and redo
+            for (int2 **pp : {&d_nonfar, &d_touchp, &d_near, &d_nearm}) {
+                owned.erase(std::find(owned.begin(), owned.end(), (void *)*pp));
+                (void)hipFree(*pp);
+            }
+            owned.erase(std::find(owned.begin(), owned.end(), (void *)d_simple));
+            (void)hipFree(d_simple);
+            nonfar_cap = cnt[0] + 1024;
+            d_nonfar = dalloc<int2>(nonfar_cap);
+            d_touchp = dalloc<int2>(nonfar_cap);
+            d_near = dalloc<int2>(nonfar_cap);
+            d_nearm = dalloc<int2>(nonfar_cap);
+            d_simple = dalloc<int4>(nonfar_cap);
+            owned.insert(owned.end(), {d_nonfar, d_touchp, d_near, d_nearm, d_simple});
             continue;
         }
-        nnear = cnt;
+        nnonfar = cnt[0];
+        if (cnt[0]) {
+            triage<<<std::min<unsigned int>((cnt[0] + 255) / 256, 65535u), 256, 0, st>>>(
+                P, d_el, d_nodes, d_nonfar, cnt[0], d_touch, ntouch, d_touchp, d_near, d_nearm, d_simple, d_count + 1,
+                nonfar_cap);
+            HIPCHECK(hipGetLastError());
+        }
+        HIPCHECK(hipMemcpyAsync(cnt, d_count, 5 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        nsimple = cnt[4];
+        ntouchp = cnt[1];
+        nnear = cnt[2];
+        nnearm = cnt[3];
+        if (cnt[1]) {
+            touch_gen<<<std::min<unsigned int>((cnt[1] + 255) / 256, 65535u), 256, 0, st>>>(
+                P, rt, d_el, d_nodes, d_seg, d_touchp, cnt[1], d_touch, ntouch, d_leaves, leaf_cap, d_groups, group_cap,
+                d_count + 5, d_stats, d_tiers);
+            HIPCHECK(hipGetLastError());
+        }
         if (timing) HIPCHECK(hipEventRecord(ev[2], st));
-        if (cnt) {
-            unsigned int blocks = std::min<unsigned int>((cnt + 3) / 4, 65535u);
-            near_pairs<<<blocks, 256, 0, st>>>(P, rt, d_el, d_nodes, d_sigma, d_cs, d_cols, d_seg, d_segcol, d_near, cnt,
-                                               d_touch, ntouch, A, d_b, d_stats, d_tiers);
+        if (nsimple) {
+            touch_simple<<<std::min<unsigned int>((nsimple + 3) / 4, 65535u), 256, 0, st>>>(
+                P, rt, d_el, d_sigma, d_cs, d_cols, d_simple, nsimple, A, d_b, d_stats, d_tiers);
+            HIPCHECK(hipGetLastError());
+        }
+        if (cnt[2]) {
+            near_thread<<<std::min<unsigned int>((cnt[2] + 255) / 256, 65535u), 256, 0, st>>>(
+                P, rt, d_el, d_sigma, d_cs, d_cols, d_near, cnt[2], A, d_b, d_stats, d_tiers);
+            HIPCHECK(hipGetLastError());
+        }
+        if (cnt[3]) {
+            near_group<<<std::min<unsigned int>((cnt[3] + 3) / 4, 65535u), 256, 0, st>>>(
+                P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, cnt[3], A, d_b, d_stats, d_tiers);
             HIPCHECK(hipGetLastError());
         }
         if (timing) HIPCHECK(hipEventRecord(ev[3], st));
+        HIPCHECK(hipMemcpyAsync(cnt + 5, d_count + 5, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        if (cnt[5] > leaf_cap || cnt[6] > group_cap) {  // more rule pieces than budgeted: grow and redo
+            owned.erase(std::find(owned.begin(), owned.end(), (void *)d_leaves));
+            owned.erase(std::find(owned.begin(), owned.end(), (void *)d_groups));
+            (void)hipFree(d_leaves);
+            (void)hipFree(d_groups);
+            leaf_cap = std::max(leaf_cap, cnt[5] + 1024);
+            group_cap = std::max(group_cap, cnt[6] + 1024);
+            d_leaves = dalloc<Leaf>(leaf_cap);
+            d_groups = dalloc<int4>(group_cap);
+            owned.insert(owned.end(), {d_leaves, d_groups});
+            continue;
+        }
+        nleaves = cnt[5];
+        ngroups = cnt[6];
+        if (cnt[6]) {
+            group_eval<<<std::min<unsigned int>((cnt[6] + 3) / 4, 65535u), 256, 0, st>>>(
+                P, rt, d_el, d_sigma, d_cs, d_cols, d_seg, d_segcol, d_leaves, d_groups, cnt[6], A, d_b);
+            HIPCHECK(hipGetLastError());
+        }
+        if (timing) HIPCHECK(hipEventRecord(ev[4], st));
         freestream<<<(nt + 255) / 256, 256, 0, st>>>(d_el, d_cs, nt, opts.test, vinf[0], vinf[1], vinf[2], d_b);
         HIPCHECK(hipGetLastError());
-        if (timing) HIPCHECK(hipEventRecord(ev[4], st));
+        if (timing) HIPCHECK(hipEventRecord(ev[5], st));
         break;
     }
 }
 
 void Plan::kernel_ms(double *ms)
 {
-    HIPCHECK(hipEventSynchronize(ev[4]));
-    for (int k = 0; k < 4; k++) {
+    HIPCHECK(hipEventSynchronize(ev[5]));
+    for (int k = 0; k < 5; k++) {
         float t = 0;
         HIPCHECK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
         ms[k] = t;
     }
-    ms[4] = (double)nnear;
+    ms[5] = (double)nnear;
+    ms[6] = (double)nleaves;
     unsigned long long h[16];
     HIPCHECK(hipMemcpy(h, d_stats, sizeof(h), hipMemcpyDeviceToHost));
-    ms[5] = (double)h[12];
+    ms[7] = (double)h[12];
 }
 
 void Plan::stats(long long *pairs, long long *points)

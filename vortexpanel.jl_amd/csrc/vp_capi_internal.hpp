@@ -51,7 +51,14 @@ struct Plan {
     RuleTab rt{};
     bool record_tiers = false;
     bool timing = false;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    unsigned int nonfar_cap = 0, leaf_cap = 0, nnonfar = 0, nleaves = 0, group_cap = 0, ngroups = 0;
+    int4 *d_groups = nullptr;
+    int2 *d_nearm = nullptr, *d_touchp = nullptr;
+    unsigned int nnearm = 0, ntouchp = 0, nsimple = 0;
+    int4 *d_simple = nullptr;
+    int2 *d_nonfar = nullptr;
+    struct Leaf *d_leaves = nullptr;
 
     ~Plan();
     void build(Ctx *ctx, const vp_surface &g, const vp_wake *wake, const vp_assembly_options &o);

@@ -191,10 +191,11 @@ class AssemblyPlan:
 
     def timings(self):
         """Device times (ms) of the last assembly's kernels, from HIP events on the plan's stream"""
-        ms = np.zeros(6)
+        ms = np.zeros(8)
         _check(self.ctx.h, self.ctx.lib.vp_plan_timings(self.h, L.ptr(ms)))
-        return dict(far=ms[0], filaments=ms[1], near=ms[2], freestream=ms[3], near_pairs=int(ms[4]),
-                    far_exact_points=int(ms[5]))
+        return dict(far=float(ms[0]), classify=float(ms[1]), near=float(ms[2]), touching=float(ms[3]),
+                    freestream=float(ms[4]), near_single_pairs=int(ms[5]), touching_leaves=int(ms[6]),
+                    far_exact_points=int(ms[7]))
 
     def tiers(self):
         """Tier index of every (body test triangle, source triangle) pair of the last assembly"""
