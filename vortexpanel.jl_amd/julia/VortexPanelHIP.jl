@@ -1,0 +1,304 @@
+"""
+    VortexPanelHIP
+
+Julia binding of libvortexhip (the MI355X Galerkin hot path) for VortexPanel.jl: `assemble`, `factorize_galerkin` and
+`solve_galerkin` with the reference's signatures (src/assembly/assemble.jl:377-447, src/assembly/solve.jl:51-99),
+calling the C ABI of include/vortexhip.h through `ccall`. Indices cross the boundary 0-based; host arrays are
+allocated here and kept alive with `GC.@preserve` for the duration of each call; the library owns every device buffer
+behind its opaque handles.
+
+Load it next to VortexPanel:
+
+    using VortexPanel
+    include(joinpath(VORTEXHIP_ROOT, "vortexpanel.jl_amd", "julia", "VortexPanelHIP.jl"))
+    A, b, stats = VortexPanelHIP.assemble(g, dm, els, σ, V∞; opts=AssemblyOptions(), wake=wake)
+    F = VortexPanelHIP.factorize_galerkin(dm, A; W=wake_rows(dm, wake), precision=:mixed)
+    sol = VortexPanelHIP.solve_galerkin(F, b)
+
+The library path comes from `ENV["VORTEXHIP_LIB"]`, else the in-tree build `vortexpanel.jl_amd/lib/libvortexhip.so`.
+"""
+module VortexPanelHIP
+
+using LinearAlgebra
+using SparseArrays
+using StaticArrays
+import VortexPanel
+import VortexPanel: AssemblyOptions, DOFMapping, RigidWake, SurfaceGeometry, bordered_rows
+
+const libvortexhip = get(ENV, "VORTEXHIP_LIB", normpath(joinpath(@__DIR__, "..", "lib", "libvortexhip.so")))
+
+const VP_OK = Cint(0)
+const VP_E_ARGUMENT = Cint(-1)
+const VP_E_DIMENSION = Cint(-2)
+const VP_E_DEVICE = Cint(-3)
+const VP_E_SINGULAR = Cint(-4)
+const TIER_NAMES = (:self, :edge, :vertex, :contact, :near, :far)   # src/assembly/assemble.jl:260
+
+# ---- C structs (field order and types of include/vortexhip.h) ---------------------------------------------------
+
+struct VPAssemblyOptions      # vp_assembly_options
+    test::Cint
+    far_ratio::Cdouble
+    far_points::Cint
+    near_points::Cint
+    near_ratio::Cdouble
+    near_depth::Cint
+    touch_levels::Cint
+    touch_ratio::Cdouble
+    touch_order::Cint
+    corner_angle::Cdouble
+    contact_rel::Cdouble
+    kernel_far::Cdouble
+    precision::Cint
+end
+
+struct VPSurface              # vp_surface
+    nn::Cint
+    x::Ptr{Cdouble}
+    nt::Cint
+    tris::Ptr{Cint}
+    corner_slot::Ptr{Cint}
+    nslot::Cint
+    ntouch::Cint
+    touch::Ptr{Cint}
+end
+
+struct VPWake                 # vp_wake
+    nw::Cint
+    x::Ptr{Cdouble}
+    ntri::Cint
+    tris::Ptr{Cint}
+    nfil::Cint
+    filaments::Ptr{Cint}
+end
+
+struct VPConstraints          # vp_constraints
+    nslot::Cint
+    nq::Cint
+    nw::Cint
+    nb::Cint
+    C_rowptr::Ptr{Cint}
+    C_col::Ptr{Cint}
+    C_val::Ptr{Cdouble}
+    W_rowptr::Ptr{Cint}
+    W_col::Ptr{Cint}
+    W_val::Ptr{Cdouble}
+    e::Ptr{Cdouble}
+    c::Ptr{Cdouble}
+end
+
+# ---- context and errors -----------------------------------------------------------------------------------------
+
+mutable struct Context
+    ptr::Ptr{Cvoid}
+    device::Int
+end
+
+const DEFAULT = Ref{Union{Nothing,Context}}(nothing)
+
+"""The library context on `device` (one HIP stream; calls on a context are serialised by the caller)."""
+function Context(device::Integer=0)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    rc = ccall((:vp_create, libvortexhip), Cint, (Cint, Ref{Ptr{Cvoid}}), device, out)
+    rc == VP_OK || error("vortexhip: no usable HIP device $device (status $rc)")
+    ctx = Context(out[], device)
+    finalizer(c -> c.ptr == C_NULL || ccall((:vp_destroy, libvortexhip), Cvoid, (Ptr{Cvoid},), c.ptr), ctx)
+    return ctx
+end
+
+default_context() = something(DEFAULT[], (DEFAULT[] = Context(0)))
+
+last_error(ctx::Context) = unsafe_string(ccall((:vp_last_error, libvortexhip), Cstring, (Ptr{Cvoid},), ctx.ptr))
+
+"""Rethrow a status as the exception the reference raises for the same condition."""
+function check(ctx::Context, rc::Cint)
+    rc == VP_OK && return nothing
+    msg = last_error(ctx)
+    rc == VP_E_ARGUMENT && throw(ArgumentError(msg))          # check_options, assemble.jl:43-46
+    rc == VP_E_DIMENSION && throw(DimensionMismatch(msg))     # reduce_system, src/dofmap/system.jl:28
+    rc == VP_E_SINGULAR && throw(SingularException(0))         # getrf info > 0
+    error("vortexhip: $msg")
+end
+
+# ---- flat arrays (0-based) --------------------------------------------------------------------------------------
+
+flat3(v::AbstractVector{<:SVector{3}}, ::Type{T}) where {T} = T[p[k] for p in v for k in 1:3]
+
+test_code(s::Symbol) = s === :hat ? 0 : s === :onering ? 1 : s === :dual ? 2 :
+    throw(ArgumentError("test must be :hat, :onering or :dual, not $s"))
+
+function c_options(o::AssemblyOptions, precision::Symbol)
+    precision in (:fp64, :mixed) || throw(ArgumentError("precision must be :fp64 or :mixed, not $precision"))
+    return VPAssemblyOptions(test_code(o.test), o.far_ratio, o.far_points, o.near_points, o.near_ratio, o.near_depth,
+        o.touch_levels, o.touch_ratio, o.touch_order, o.corner_angle, o.contact_rel, o.kernel_far,
+        precision === :mixed ? 1 : 0)
+end
+
+struct SurfaceArrays
+    x::Vector{Cdouble}
+    tris::Vector{Cint}
+    corner_slot::Vector{Cint}
+    touch::Vector{Cint}
+    nslot::Int
+end
+
+function SurfaceArrays(g::SurfaceGeometry, dm::DOFMapping)
+    x = flat3(g.x, Cdouble)
+    tris = Cint[t[k] - 1 for t in g.tris for k in 1:3]
+    cs = Cint[c[k] - 1 for c in dm.corner_slot for k in 1:3]
+    touch = Cint[v - 1 for p in g.touch_pairs for v in (p[1], p[2])]
+    return SurfaceArrays(x, tris, cs, touch, dm.nslot)
+end
+
+c_surface(s::SurfaceArrays) = VPSurface(length(s.x) ÷ 3, pointer(s.x), length(s.tris) ÷ 3, pointer(s.tris),
+    pointer(s.corner_slot), s.nslot, length(s.touch) ÷ 2, isempty(s.touch) ? C_NULL : pointer(s.touch))
+
+struct WakeArrays
+    x::Vector{Cdouble}
+    tris::Vector{Cint}
+    filaments::Vector{Cint}
+end
+
+WakeArrays(w::RigidWake) = WakeArrays(flat3(w.x, Cdouble), Cint[t[k] - 1 for t in w.tris for k in 1:3],
+    Cint[v - 1 for f in w.filaments for v in (f[1], f[2])])
+
+c_wake(w::WakeArrays) = VPWake(length(w.x) ÷ 3, pointer(w.x), length(w.tris) ÷ 3, pointer(w.tris),
+    length(w.filaments) ÷ 2, isempty(w.filaments) ? C_NULL : pointer(w.filaments))
+
+# ---- assemble (src/assembly/assemble.jl:377-447) ------------------------------------------------------------------
+
+"""
+    assemble(g, dm, els, σ, V∞; opts=AssemblyOptions(), external=nothing, wake=nothing, precision=:fp64, ctx)
+
+The reference's `assemble` on the GPU: returns `(A, b, stats)` with A nslot × (nslot + nwake) (row = test slot,
+column = source slot then wake node), b and `stats = (pairs, points, time)` keyed by tier. `els` is accepted for
+signature compatibility; the library builds its element records from `g.x`/`g.tris` (the same construction as
+`build_elements`). `external` (an arbitrary closure) cannot cross the C boundary: pass `nothing`.
+"""
+function assemble(g::SurfaceGeometry, dm::DOFMapping, els, σ::AbstractVector, V∞::SVector{3};
+                  opts::AssemblyOptions=AssemblyOptions(), external=nothing, wake::Union{Nothing,RigidWake}=nothing,
+                  precision::Symbol=:fp64, ctx::Context=default_context())
+    external === nothing || throw(ArgumentError("external velocity closures are not supported by the GPU path"))
+    length(σ) == length(g.tris) || throw(DimensionMismatch("σ has $(length(σ)) entries for $(length(g.tris)) triangles"))
+    o = c_options(opts, precision)
+    s = SurfaceArrays(g, dm)
+    w = wake === nothing ? nothing : WakeArrays(wake)
+    nw = wake === nothing ? 0 : length(wake.x)
+    A = Matrix{Cdouble}(undef, dm.nslot, dm.nslot + nw)     # column-major, exactly the library's layout
+    b = Vector{Cdouble}(undef, dm.nslot)
+    sig = Vector{Cdouble}(σ)
+    vinf = Cdouble[V∞[1], V∞[2], V∞[3]]
+    pairs = zeros(Clonglong, 6)
+    points = zeros(Clonglong, 6)
+    t = @elapsed GC.@preserve s w A b sig vinf pairs points begin
+        sc = Ref(c_surface(s))
+        wref = Ref(w === nothing ? VPWake(0, C_NULL, 0, C_NULL, 0, C_NULL) : c_wake(w))
+        wc = w === nothing ? Ptr{VPWake}(C_NULL) : Base.unsafe_convert(Ptr{VPWake}, wref)
+        rc = GC.@preserve wref ccall((:vp_assemble, libvortexhip), Cint,
+            (Ptr{Cvoid}, Ref{VPSurface}, Ptr{VPWake}, Ptr{Cdouble}, Ptr{Cdouble}, Ref{VPAssemblyOptions},
+             Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Clonglong}, Ptr{Clonglong}),
+            ctx.ptr, sc, wc, sig, vinf, Ref(o), A, b, pairs, points)
+        check(ctx, rc)
+    end
+    stats = (pairs=Dict(zip(TIER_NAMES, Int.(pairs))), points=Dict(zip(TIER_NAMES, Int.(points))), time=t)
+    return A, b, stats
+end
+
+# ---- bordered solve (src/assembly/solve.jl:15-99) -----------------------------------------------------------------
+
+"""
+The factored bordered system on the device, with the fields `solve_galerkin(F, b)` reads in the reference's
+`GalerkinFactorization` (solve.jl:65-72): `se`, `C`, `nq`, `nb`, `cond`.
+"""
+mutable struct GalerkinFactorizationHIP{T,M}
+    ptr::Ptr{Cvoid}
+    ctx::Context
+    se::Vector{T}
+    C::M
+    nq::Int
+    nb::Int
+    cond::T
+end
+
+csr(S::SparseMatrixCSC) = (St = sparse(transpose(S)); (Cint.(St.colptr .- 1), Cint.(St.rowval .- 1), Vector{Cdouble}(St.nzval)))
+
+"""
+    factorize_galerkin(dm, A; W=nothing, precision=:fp64, ctx)
+
+Forms M = [CᵀA[C;W] se·e; sc·cᵀ 0] on the device and factors it (`precision = :fp64`: FP64 LU on FP64 MFMA;
+`:mixed`: FP32 LU on FP32 MFMA); solves refine in FP64 against the FP64 M.
+"""
+function factorize_galerkin(dm::DOFMapping, A::AbstractMatrix; W=nothing, precision::Symbol=:fp64,
+                            ctx::Context=default_context())
+    precision in (:fp64, :mixed) || throw(ArgumentError("precision must be :fp64 or :mixed, not $precision"))
+    nw = W === nothing ? 0 : size(W, 1)
+    size(A) == (dm.nslot, dm.nslot + nw) || throw(DimensionMismatch("A is $(size(A)), expected ($(dm.nslot), $(dm.nslot + nw))"))
+    rows = bordered_rows(dm)
+    nb = length(rows)
+    Cr, Cc, Cv = csr(dm.C)
+    Wr, Wc, Wv = W === nothing ? (Cint[0], Cint[], Cdouble[]) : csr(sparse(W))
+    e = Cdouble[r.e[i] for r in rows for i in 1:dm.nq]
+    c = Cdouble[r.c[i] for r in rows for i in 1:dm.nq]
+    Ad = Matrix{Cdouble}(A)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve Cr Cc Cv Wr Wc Wv e c Ad begin
+        k = VPConstraints(dm.nslot, dm.nq, nw, nb, pointer(Cr), pointer(Cc), pointer(Cv),
+            nw == 0 ? C_NULL : pointer(Wr), nw == 0 ? C_NULL : pointer(Wc), nw == 0 ? C_NULL : pointer(Wv),
+            pointer(e), pointer(c))
+        rc = ccall((:vp_factorize, libvortexhip), Cint,
+            (Ptr{Cvoid}, Ptr{Cdouble}, Clonglong, Cint, Ref{VPConstraints}, Cint, Ref{Ptr{Cvoid}}),
+            ctx.ptr, Ad, dm.nslot, 0, Ref(k), precision === :mixed ? 1 : 0, out)
+        check(ctx, rc)
+    end
+    n = Ref{Cint}(0)
+    se = zeros(Cdouble, nb)
+    ccall((:vp_factor_info, libvortexhip), Cint, (Ptr{Cvoid}, Ref{Cint}, Ptr{Cdouble}), out[], n, se)
+    cnd = Ref{Cdouble}(0)
+    check(ctx, ccall((:vp_factor_cond, libvortexhip), Cint, (Ptr{Cvoid}, Ref{Cdouble}), out[], cnd))
+    F = GalerkinFactorizationHIP(out[], ctx, se, dm.C, dm.nq, nb, cnd[])
+    finalizer(f -> f.ptr == C_NULL || ccall((:vp_factor_destroy, libvortexhip), Cvoid, (Ptr{Cvoid},), f.ptr), F)
+    return F
+end
+
+"""
+    solve_galerkin(F::GalerkinFactorizationHIP, b)
+
+`solve_galerkin(F, b)` of solve.jl:95-99: `(q, μ, λ, cond)`. `b` may also be a matrix whose columns are right hand
+sides (an angle-of-attack sweep solved as one batch); q, μ, λ then have one column per right hand side.
+"""
+function solve_galerkin(F::GalerkinFactorizationHIP, b::AbstractVecOrMat)
+    B = reshape(Matrix{Cdouble}(reshape(b, size(b, 1), :)), size(b, 1), :)
+    nrhs = size(B, 2)
+    n = F.nq + F.nb
+    R = vcat(Matrix{Cdouble}(F.C' * B), zeros(Cdouble, F.nb, nrhs))
+    Rr = Matrix{Cdouble}(transpose(R))          # row-major n × nrhs for the library
+    Xr = similar(Rr)
+    iters = Ref{Cint}(0)
+    resid = Ref{Cdouble}(0)
+    GC.@preserve Rr Xr begin
+        rc = ccall((:vp_factor_solve, libvortexhip), Cint,
+            (Ptr{Cvoid}, Ptr{Cdouble}, Cint, Ptr{Cdouble}, Ref{Cint}, Ref{Cdouble}), F.ptr, Rr, nrhs, Xr, iters, resid)
+        check(F.ctx, rc)
+    end
+    X = Matrix(transpose(Xr))
+    q = X[1:F.nq, :]
+    λ = F.se .* X[F.nq+1:n, :]
+    b isa AbstractVector && return (q=vec(q), μ=F.C * vec(q), λ=vec(λ), cond=F.cond)
+    return (q=q, μ=F.C * q, λ=λ, cond=F.cond)
+end
+
+"""
+    solve_galerkin(dm, A, b; W=nothing, precision=:fp64)
+
+The one-shot form of solve.jl:51-59: `(q, μ, λ, cond, residual)`.
+"""
+function solve_galerkin(dm::DOFMapping, A::AbstractMatrix, b::AbstractVector; W=nothing, precision::Symbol=:fp64,
+                        ctx::Context=default_context())
+    F = factorize_galerkin(dm, A; W=W, precision=precision, ctx=ctx)
+    s = solve_galerkin(F, b)
+    M, rhs, _ = VortexPanel.bordered_system(dm, A, b; W=W)
+    x = [s.q; s.λ ./ F.se]
+    return (q=s.q, μ=s.μ, λ=s.λ, cond=s.cond, residual=norm(M * x - rhs) / norm(rhs))
+end
+
+end # module
