@@ -144,6 +144,23 @@ int vp_factor_cond(vp_factor *f, double *cond);
 int vp_factor_info(vp_factor *f, int *n, double *se);
 void vp_factor_destroy(vp_factor *f);
 
+/* ---- N-body velocity sums (src/post/Post.jl:32-112, src/freewake/step.jl:176-187) -------------------------- */
+
+/* `surface_velocity(els, dm, sigma, mu, Vinf; wake, mu_wake, kernel_far)` (Post.jl:32-64): the velocity at every
+ * body triangle's centroid (u, nt x 3): Vinf + sum_s sigma_s w_s + gamma_s x w_s (the own element on its normal's
+ * side), gamma from the slot values mu (nslot) through corner_slot; with a wake, its sheets from mu_wake (one value
+ * per wake node) and its tip filaments.  precision 0: FP64; 1: FP32 kernels with FP64 sums (own element in FP64). */
+int vp_surface_velocity(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const double *sigma, const double *mu,
+                        const double *mu_wake, const double *vinf, double kernel_far, int precision, double *u);
+/* `body_velocity(els, sigma, gamma, xs; kernel_far)` (Post.jl:92-112): the body's sources and sheets (gamma, nt x 3)
+ * at np off-body points xs (np x 3), without the freestream. */
+int vp_body_velocity(vp_ctx *ctx, const vp_surface *g, const double *sigma, const double *gamma, int np,
+                     const double *xs, double kernel_far, int precision, double *u);
+/* `blob_velocities(blobs, X)` (step.jl:176-187) with blob_velocity (src/kernels/blob.jl:21-25): nb Rosenhead-Moore
+ * particles (positions bx and strengths alpha nb x 3, core radii delta) at np points. */
+int vp_blob_velocities(vp_ctx *ctx, int nb, const double *bx, const double *alpha, const double *delta, int np,
+                       const double *xs, int precision, double *u);
+
 /* Test entry points (host arrays, row-major): C = A B (sub = 0) or C -= A B (sub = 1) with the library's MFMA GEMM
  * (prec 0 FP64, 1 FP32; A M x K, B K x N, C M x N, all doubles on the host); and x = M^-1 r for a dense n x n
  * matrix through the randomized LU (mode 0 FP64, 1 FP32 factors) with FP64 refinement. */

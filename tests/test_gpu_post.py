@@ -1,0 +1,90 @@
+"""GPU N-body sums (vp_surface_velocity, vp_body_velocity, vp_blob_velocities) against the FP64 oracle
+(src/post/Post.jl:32-112, src/freewake/step.jl:176-187).  FP64: within 1e-12 of the largest velocity; FP32 kernels
+with FP64 sums: within 2e-5 (source triangles) and 5e-5 (blobs) of it."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from vphip import api, dofmap, mesh, post, wake as wk
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-12, "mixed": 2e-5}
+
+
+def _err(u, ref):
+    """largest deviation relative to the largest reference velocity (absolute where the reference is all zero)"""
+    return np.abs(u - ref).max() / max(np.abs(ref).max(), 1e-300)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("level", [1, 3])
+def test_surface_velocity_sphere(level, precision):
+    print(f"surface_velocity icosphere {level} {precision}", flush=True)
+    g = mesh.icosphere(level)
+    dm = dofmap.build_dofmap(g)
+    V = np.array([1.0, 0.3, -0.2])
+    sig = api.source_strengths(api.build_elements(g), V)
+    mu = np.random.default_rng(level).standard_normal(dm.nslot)
+    u = post.surface_velocity(g, dm, sig, mu, V, precision=precision)
+    ref = O.surface_velocity(g, dm, sig, mu, V)
+    assert _err(u, ref) < TOL[precision], _err(u, ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_surface_velocity_wing_with_wake(precision):
+    print(f"surface_velocity wing+wake {precision}", flush=True)
+    g = mesh.naca_wing(mesh.WingSpec(code="0012", n_around=24, n_span=8, aspect=4.0))
+    dm = dofmap.build_dofmap(g)
+    V = np.array([np.cos(np.radians(5)), 0, np.sin(np.radians(5))])
+    w = wk.build_wake(g, dm, V, wk.WakeOptions(length=2.0))
+    rng = np.random.default_rng(5)
+    sig = api.source_strengths(api.build_elements(g), V)
+    mu = rng.standard_normal(dm.nslot)
+    mu_wake = wk.wake_values(w, rng.standard_normal(dm.wake_te.n))
+    u = post.surface_velocity(g, dm, sig, mu, V, wake=w, mu_wake=mu_wake, precision=precision)
+    ref = O.surface_velocity(g, dm, sig, mu, V, wake=w, mu_wake=mu_wake)
+    assert _err(u, ref) < TOL[precision], _err(u, ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_body_velocity_off_body(precision):
+    print(f"body_velocity {precision}", flush=True)
+    g = mesh.abused_ellipsoid(nth=12, nph=24, axes=(2.0, 1.0, 0.5), jitter=0.25, sliver_frac=0.05,
+                              sliver_ratio=0.1, seed=9)
+    rng = np.random.default_rng(7)
+    sig = rng.standard_normal(g.nt)
+    gam = rng.standard_normal((g.nt, 3))
+    d = rng.standard_normal((777, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    xs = d * rng.uniform(0.6, 6.0, 777)[:, None] * np.array([2.0, 1.0, 0.5])   # inside and outside, off the surface
+    u = post.body_velocity(g, sig, gam, xs, precision=precision)
+    ref = O.body_velocity(g, sig, gam, xs)
+    assert _err(u, ref) < TOL[precision], _err(u, ref)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("nb,npts", [(1, 1), (300, 257), (2049, 1000)])
+def test_blob_velocities(nb, npts, precision):
+    print(f"blob_velocities nb={nb} np={npts} {precision}", flush=True)
+    rng = np.random.default_rng(nb + npts)
+    bx = rng.standard_normal((nb, 3)) + np.array([5.0, 0, 0])
+    ba = rng.standard_normal((nb, 3))
+    bd = rng.uniform(0.02, 0.2, nb)
+    xs = np.vstack([rng.standard_normal((npts - 1, 3)) + np.array([5.0, 0, 0]), bx[:1]])  # one target on a blob
+    u = post.blob_velocities(bx, ba, bd, xs, precision=precision)
+    ref = O.blob_velocities(bx, ba, bd, xs)
+    assert _err(u, ref) < (1e-12 if precision == "fp64" else 5e-5), _err(u, ref)
+
+
+def test_empty_and_bad_inputs():
+    print("field edge cases", flush=True)
+    u = post.blob_velocities(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0), np.ones((4, 3)))
+    assert np.all(u == 0)
+    assert post.blob_velocities(np.ones((2, 3)), np.ones((2, 3)), np.ones(2), np.zeros((0, 3))).shape == (0, 3)
+    with pytest.raises(ValueError):
+        post.blob_velocities(np.ones((2, 3)), np.ones((2, 3)), np.ones(2), np.ones((3, 3)), precision="fp16")
+    g = mesh.icosphere(1)
+    dm = dofmap.build_dofmap(g)
+    with pytest.raises(ValueError):
+        post.surface_velocity(g, dm, np.zeros(g.nt), np.zeros(dm.nslot + 1), [1.0, 0, 0])

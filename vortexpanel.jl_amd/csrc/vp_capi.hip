@@ -258,6 +258,53 @@ int vp_dense_solve(vp_ctx *ctx, int n, const double *M, int mode, const double *
     });
 }
 
+int vp_surface_velocity(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const double *sigma, const double *mu,
+                        const double *mu_wake, const double *vinf, double kernel_far, int precision, double *u)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if (!g || !g->x || !g->tris || !g->corner_slot || !sigma || !mu || !vinf || !u)
+        return fail(&ctx->c, VP_E_ARGUMENT, "null argument");
+    if (precision != 0 && precision != 1) return fail(&ctx->c, VP_E_ARGUMENT, "precision must be 0 (fp64) or 1 (mixed)");
+    if (g->nt <= 0 || g->nn <= 0 || g->nslot <= 0) return fail(&ctx->c, VP_E_DIMENSION, "empty surface");
+    const vp_wake *w = wake && wake->nw > 0 ? wake : nullptr;
+    if (w && (!w->x || !mu_wake || (w->ntri > 0 && !w->tris) || (w->nfil > 0 && !w->filaments)))
+        return fail(&ctx->c, VP_E_ARGUMENT, "null wake argument");
+    return guard(&ctx->c, [&] {
+        (void)hipSetDevice(ctx->c.device);
+        vp::surface_velocity_host(&ctx->c, *g, w, sigma, mu, mu_wake, vinf, kernel_far, precision, u);
+    });
+}
+
+int vp_body_velocity(vp_ctx *ctx, const vp_surface *g, const double *sigma, const double *gamma, int np,
+                     const double *xs, double kernel_far, int precision, double *u)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if (!g || !g->x || !g->tris || !sigma || !gamma || (np > 0 && (!xs || !u)))
+        return fail(&ctx->c, VP_E_ARGUMENT, "null argument");
+    if (precision != 0 && precision != 1) return fail(&ctx->c, VP_E_ARGUMENT, "precision must be 0 (fp64) or 1 (mixed)");
+    if (np < 0 || g->nt <= 0 || g->nn <= 0) return fail(&ctx->c, VP_E_DIMENSION, "bad sizes");
+    if (np == 0) return VP_OK;
+    return guard(&ctx->c, [&] {
+        (void)hipSetDevice(ctx->c.device);
+        vp::body_velocity_host(&ctx->c, *g, sigma, gamma, np, xs, kernel_far, precision, u);
+    });
+}
+
+int vp_blob_velocities(vp_ctx *ctx, int nb, const double *bx, const double *alpha, const double *delta, int np,
+                       const double *xs, int precision, double *u)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if ((nb > 0 && (!bx || !alpha || !delta)) || (np > 0 && (!xs || !u)))
+        return fail(&ctx->c, VP_E_ARGUMENT, "null argument");
+    if (precision != 0 && precision != 1) return fail(&ctx->c, VP_E_ARGUMENT, "precision must be 0 (fp64) or 1 (mixed)");
+    if (nb < 0 || np < 0) return fail(&ctx->c, VP_E_DIMENSION, "negative size");
+    if (np == 0) return VP_OK;
+    return guard(&ctx->c, [&] {
+        (void)hipSetDevice(ctx->c.device);
+        vp::blob_velocities_host(&ctx->c, nb, bx, alpha, delta, np, xs, precision, u);
+    });
+}
+
 int vp_assemble(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const double *sigma, const double *vinf,
                 const vp_assembly_options *opts, double *A, double *b, long long *pairs, long long *points)
 {

@@ -96,6 +96,12 @@ struct Factor {
 };
 
 void gemm_host(hipStream_t s, int prec, int sub, int M, int N, int K, const double *A, const double *B, double *C);
+void surface_velocity_host(Ctx *c, const vp_surface &g, const vp_wake *w, const double *sigma, const double *mu,
+                           const double *mu_wake, const double *vinf, double kernel_far, int precision, double *u);
+void body_velocity_host(Ctx *c, const vp_surface &g, const double *sigma, const double *gam, int np, const double *xs,
+                        double kernel_far, int precision, double *u);
+void blob_velocities_host(Ctx *c, int nb, const double *bx, const double *ba, const double *bd, int np,
+                          const double *xs, int precision, double *u);
 int dense_solve_host(Ctx *c, int n, const double *M, int mode, const double *r, double *x);
 
 }  // namespace vp

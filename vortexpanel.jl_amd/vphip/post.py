@@ -1,0 +1,160 @@
+"""Post-processing of a Galerkin solution: the N-body velocity sums on the GPU (surface_velocity, body_velocity, blob
+sums through libvortexhip) and the per-element and force formulas on the host.
+
+Mirrors src/post/Post.jl and src/post/forces.jl: same names, same argument meaning.  The velocity sums have no CPU
+fallback (they raise when the library is missing); the host formulas are O(nt) or O(nte^2) array arithmetic.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .api import Context, _check, _surface_c, _wake_c, default_context
+
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+
+def _prec(p):
+    if p not in PRECISIONS:
+        raise ValueError(f"precision must be one of {tuple(PRECISIONS)}, not {p!r}")
+    return PRECISIONS[p]
+
+
+def element_values(dm, mu):
+    """Corner doublet values of every element, src/post/Post.jl:19"""
+    return np.asarray(mu, dtype=np.float64)[dm.corner_slot]
+
+
+def sheet_strengths(els, dm, mu):
+    """Vortex sheet gamma = n x grad(mu) of every element, Post.jl:22-23 (els: api.Elements)"""
+    return np.einsum("tk,tkj->tj", element_values(dm, mu), els.gamma)
+
+
+def surface_velocity(g, dm, sigma, mu, vinf, wake=None, mu_wake=None, kernel_far=10.0, precision="fp64",
+                     ctx: Context | None = None):
+    """Velocity at every body triangle's centroid from the singularities, Post.jl:32-64 (GPU N-body sum):
+    V∞ + sum_s sigma_s w_s + gamma_s x w_s, the own element on its normal's side, plus the wake's sheets and tip
+    filaments from the wake node doublet values mu_wake (`wake_values`)."""
+    ctx = ctx or default_context()
+    s, ks = _surface_c(g, dm)
+    w, kw = _wake_c(wake)
+    if wake is not None and mu_wake is None:
+        raise ValueError("a wake needs mu_wake")
+    sig, m = L.f64(sigma), L.f64(mu)
+    if sig.shape != (len(g.tris),) or m.shape != (dm.nslot,):
+        raise ValueError("sigma must have one value per triangle and mu one per slot")
+    mw = L.f64(mu_wake) if wake is not None else np.zeros(1)
+    u = np.zeros((len(g.tris), 3))
+    rc = ctx.lib.vp_surface_velocity(ctx.h, C.byref(s), C.byref(w) if w is not None else None, L.ptr(sig), L.ptr(m),
+                                     L.ptr(mw), L.ptr(L.f64(vinf)), C.c_double(kernel_far), _prec(precision), L.ptr(u))
+    _check(ctx.h, rc)
+    return u
+
+
+def body_velocity(g, sigma, gamma, xs, kernel_far=10.0, precision="fp64", ctx: Context | None = None):
+    """Velocity of the body's sources and sheets at off-body points, without the freestream, Post.jl:92-112"""
+    ctx = ctx or default_context()
+    keep = dict(x=L.f64(g.x), tris=L.i32(g.tris))
+    s = L.SurfaceC(len(g.x), L.ptr(keep["x"]), len(g.tris), L.ptr(keep["tris"], C.c_int), None, 0, 0, None)
+    X = L.f64(xs).reshape(-1, 3)
+    gam = L.f64(gamma).reshape(-1, 3)
+    sig = L.f64(sigma)
+    if sig.shape != (len(g.tris),) or gam.shape != (len(g.tris), 3):
+        raise ValueError("sigma and gamma must have one entry per triangle")
+    u = np.zeros_like(X)
+    rc = ctx.lib.vp_body_velocity(ctx.h, C.byref(s), L.ptr(sig), L.ptr(gam), len(X), L.ptr(X),
+                                  C.c_double(kernel_far), _prec(precision), L.ptr(u))
+    _check(ctx.h, rc)
+    return u
+
+
+def blob_velocities(bx, alpha, delta, xs, precision="fp64", ctx: Context | None = None):
+    """Velocity of Rosenhead-Moore particles (positions bx, strengths alpha, cores delta) at the points xs by direct
+    summation, src/freewake/step.jl:176-187"""
+    ctx = ctx or default_context()
+    bx, ba = L.f64(bx).reshape(-1, 3), L.f64(alpha).reshape(-1, 3)
+    bd = L.f64(delta).reshape(-1)
+    if not (len(bx) == len(ba) == len(bd)):
+        raise ValueError("blob arrays disagree in length")
+    X = L.f64(xs).reshape(-1, 3)
+    u = np.zeros_like(X)
+    rc = ctx.lib.vp_blob_velocities(ctx.h, len(bd), L.ptr(bx), L.ptr(ba), L.ptr(bd), len(X), L.ptr(X),
+                                    _prec(precision), L.ptr(u))
+    _check(ctx.h, rc)
+    return u
+
+
+def gradient_velocity(els, dm, mu, vinf):
+    """V∞ - (V∞.n)n + grad(mu) per element, Post.jl:72-73"""
+    V = np.asarray(vinf, dtype=np.float64)
+    g = np.einsum("tk,tkj->tj", element_values(dm, mu), els.gradl)
+    return V - (els.n @ V)[:, None] * els.n + g
+
+
+def pressure_coefficient(u, vinf, dmudt=None):
+    """1 - |u|^2/|V∞|^2 (- 2 dmu/dt/|V∞|^2), Post.jl:82-84"""
+    V = np.asarray(vinf, dtype=np.float64)
+    uu = np.einsum("ij,ij->i", u, u)
+    return 1 - (uu if dmudt is None else uu + 2 * np.asarray(dmudt)) / V.dot(V)
+
+
+def pressure_force(els, cp, origin=(0.0, 0.0, 0.0)):
+    """F = -sum Cp n A and its moment about origin, in units of q∞, src/post/forces.jl:11-19"""
+    f = -(np.asarray(cp) * els.area)[:, None] * els.n
+    return f.sum(0), np.cross(els.centroid - np.asarray(origin, dtype=np.float64), f).sum(0)
+
+
+def te_segments(wte):
+    """Segments of the trailing edge lines along tau, forces.jl:22-23"""
+    out = []
+    for line, closed in zip(wte.lines, wte.closed):
+        m = len(line)
+        out += [(line[i], line[(i + 1) % m]) for i in range(m if closed else m - 1)]
+    return out
+
+
+def kutta_joukowski_force(wte, mu_w, vinf):
+    """F = (2/|V∞|^2) V∞ x integral of mu_w dl along the trailing edge lines, forces.jl:34-39"""
+    V = np.asarray(vinf, dtype=np.float64)
+    x = np.asarray(wte.x, dtype=np.float64)
+    G = np.zeros(3)
+    for a, b in te_segments(wte):
+        G += (mu_w[a] + mu_w[b]) / 2 * (x[b] - x[a])
+    return 2 / V.dot(V) * np.cross(V, G)
+
+
+def trefftz_drag(wte, mu_w, vinf):
+    """Induced drag in the Trefftz plane from the strips' mean circulation and the trailing vortices at the trailing
+    edge nodes (midpoint normalwash on the lifting line), in units of q∞, forces.jl:53-81.  Vectorised over strips."""
+    V = np.asarray(vinf, dtype=np.float64)
+    d = V / np.linalg.norm(V)
+    x = np.asarray(wte.x, dtype=np.float64)
+    P = x - np.outer(x @ d, d)
+    mu_w = np.asarray(mu_w, dtype=np.float64)
+    vp, vg, sa, sb, sg = [], [], [], [], []
+    for line, closed in zip(wte.lines, wte.closed):
+        line = list(line)
+        m = len(line)
+        nseg = m if closed else m - 1
+        G = [(mu_w[line[i]] + mu_w[line[(i + 1) % m]]) / 2 for i in range(nseg)]
+        for i in range(m):
+            before = G[i - 1] if i > 0 else (G[nseg - 1] if closed else 0.0)
+            after = G[i] if i < nseg else 0.0
+            vp.append(P[line[i]])
+            vg.append(after - before)
+        for i in range(nseg):
+            sa.append(P[line[i]])
+            sb.append(P[line[(i + 1) % m]])
+            sg.append(G[i])
+    if not sg:
+        return 0.0
+    vp, vg, sa, sb, sg = (np.asarray(v) for v in (vp, vg, sa, sb, sg))
+    c = (sa + sb) / 2
+    ell = np.linalg.norm(sb - sa, axis=1)
+    n = np.cross(d, (sb - sa) / ell[:, None])
+    r = c[:, None, :] - vp[None, :, :]
+    v = np.einsum("k,skj->sj", vg / (4 * np.pi), np.cross(d, r) / np.sum(r * r, axis=2)[:, :, None])
+    D = np.sum(sg * np.einsum("sj,sj->s", n, v) * ell)
+    return 2 * D / V.dot(V)
