@@ -130,3 +130,22 @@ def test_host_post_formulas_match_oracle():
     ug = post.gradient_velocity(els, dm, mu, VG)
     assert np.abs(ug - O.gradient_velocity(el, dm, mu, VG)).max() < 1e-12
     assert np.abs(post.pressure_coefficient(ug, VG) - O.pressure_coefficient(ug, VG)).max() < 1e-14
+
+
+def test_wing_loads_host_matches_oracle():
+    """vphip.sweep.wing_loads (pressure CL, Trefftz CDi, KJ) gives the oracle's numbers for the same solution"""
+    from vphip import sweep
+    g = mesh.naca_wing(mesh.WingSpec(code="0012", n_around=16, n_span=6, aspect=4.0))
+    dm = dofmap.build_dofmap(g)
+    rng = np.random.default_rng(4)
+    q = rng.standard_normal(dm.nq)
+    mu = dm.C @ q
+    V, eL = sweep.wing_freestream(6.0)
+    got = sweep.wing_loads(api.build_elements(g), dm, mu, q, V, eL, 4.0)
+    el = O.elements(g.x, g.tris)
+    cp = O.pressure_coefficient(O.gradient_velocity(el, dm, mu, V), V)
+    Fp, _ = O.pressure_force(el, cp)
+    mu_w = dm.K @ q
+    assert abs(got["CL"] - Fp @ eL / 4.0) < 1e-13 * max(1.0, abs(got["CL"]))
+    assert abs(got["CDi_T"] - O.trefftz_drag(dm.wake_te, mu_w, V) / 4.0) < 1e-12 * max(1.0, abs(got["CDi_T"]))
+    assert np.allclose(got["CL_KJ"], O.kutta_joukowski_force(dm.wake_te, mu_w, V) @ eL / 4.0, rtol=1e-13, atol=1e-14)

@@ -203,6 +203,12 @@ class AssemblyPlan:
         _check(self.ctx.h, self.ctx.lib.vp_plan_tiers(self.h, L.ptr(out, C.c_uint8)))
         return out
 
+    def read_b(self):
+        """b of the last assembly (A stays on the device)"""
+        b = np.zeros(self.nslot)
+        _check(self.ctx.h, self.ctx.lib.vp_plan_read(self.h, None, L.ptr(b)))
+        return b
+
     def read(self):
         A = np.zeros((self.nslot + self.nw, self.nslot))  # column-major nslot x ncol
         b = np.zeros(self.nslot)
