@@ -104,6 +104,12 @@ int vp_plan_tiers(vp_plan *plan, uint8_t *tiers);
  * the number of single-piece near pairs, ms[6] of touching rule pieces, ms[7] the far tier's closed-form
  * (non-multipole) point evaluations (8 values). */
 int vp_plan_timings(vp_plan *plan, double *ms);
+/* Backward assembly (no counterpart in the reference, which has no derivatives; it serves the adjoint dJ/dx of a
+ * Galerkin solution): the gradient of lambdaᵀ (b(x) - A(x) mu) with respect to the node coordinates, for adjoint
+ * weights lambda (nslot) and doublet values mu (nslot + nw: slots, then wake nodes), with the rules of the last
+ * vp_plan_assemble and sigma = -Vinf.n (source_strengths, assemble.jl:364).  grad: (nn + nw) x 3, body nodes then
+ * wake nodes. */
+int vp_plan_adjoint(vp_plan *plan, const double *lambda, const double *mu, const double *vinf, double *grad);
 /* Device pointers of the plan's A (column-major, leading dimension nslot) and b, for chaining device calls. */
 int vp_plan_device_A(vp_plan *plan, double **A, double **b, long long *lda);
 void vp_plan_destroy(vp_plan *plan);
@@ -138,6 +144,9 @@ int vp_factorize(vp_ctx *ctx, const double *A, long long lda, int A_on_device, c
  * row-major host arrays (R[i*nrhs + r]; the caller forms [Cᵀ b; 0]).  iters: refinement steps taken; resid: the
  * largest |R - M X|_inf / (|M|_inf |X|_inf). */
 int vp_factor_solve(vp_factor *f, const double *R, int nrhs, double *X, int *iters, double *resid);
+/* Solve Mᵀ X = R (the adjoint system; no counterpart in the reference): same layout as vp_factor_solve, through a
+ * factorization of Mᵀ made on first use and shared with vp_factor_cond, FP64 refinement against Mᵀ. */
+int vp_factor_solve_transposed(vp_factor *f, const double *R, int nrhs, double *X, int *iters, double *resid);
 /* 1-norm condition number estimate of M (1 / gecon's rcond, solve.jl:85). */
 int vp_factor_cond(vp_factor *f, double *cond);
 /* The border column scales se (nb values, solve.jl:33) and n. */

@@ -1457,6 +1457,8 @@ Plan::~Plan()
     for (int k = 0; k < 6; k++)
         if (ev[k]) (void)hipEventDestroy(ev[k]);
     for (void *p : owned) (void)hipFree(p);
+    if (d_aleaves) (void)hipFree(d_aleaves);
+    if (d_agroups) (void)hipFree(d_agroups);
     if (A && ownsA) (void)hipFree(A);
 }
 
@@ -1633,6 +1635,26 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
     HIPCHECK(hipStreamSynchronize(st));
 }
 
+AsmParams Plan::params() const
+{
+    AsmParams P;
+    P.nt = nt;
+    P.nsrc = nsrc;
+    P.nseg = nseg;
+    P.nslot = nslot;
+    P.ncol = ncol;
+    P.nn = nn;
+    P.test = opts.test;
+    P.far_ratio = opts.far_ratio;
+    P.near_ratio = opts.near_ratio;
+    P.corner_angle = opts.corner_angle;
+    P.contact_rel = opts.contact_rel;
+    P.kernel_far = opts.kernel_far;
+    P.near_depth = opts.near_depth;
+    P.lda = lda;
+    return P;
+}
+
 void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
 {
     hipStream_t st = ctx->stream;
@@ -1646,21 +1668,7 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
         HIPCHECK(hipMemsetAsync(d_stats, 0, sizeof(unsigned long long) * 16, st));
         HIPCHECK(hipMemsetAsync(d_count, 0, sizeof(unsigned int) * 7, st));
         if (d_tiers) HIPCHECK(hipMemsetAsync(d_tiers, 255, (size_t)nt * nsrc, st));
-        AsmParams P;
-        P.nt = nt;
-        P.nsrc = nsrc;
-        P.nseg = nseg;
-        P.nslot = nslot;
-        P.ncol = ncol;
-        P.nn = nn;
-        P.test = opts.test;
-        P.far_ratio = opts.far_ratio;
-        P.near_ratio = opts.near_ratio;
-        P.corner_angle = opts.corner_angle;
-        P.contact_rel = opts.contact_rel;
-        P.kernel_far = opts.kernel_far;
-        P.near_depth = opts.near_depth;
-        P.lda = lda;
+        const AsmParams P = params();
         dim3 grid(ntt, nts);
         if (timing) HIPCHECK(hipEventRecord(ev[0], st));
         if (opts.precision == 1)
@@ -1759,6 +1767,7 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
         if (timing) HIPCHECK(hipEventRecord(ev[5], st));
         break;
     }
+    assembled = true;
 }
 
 void Plan::kernel_ms(double *ms)
@@ -1790,3 +1799,5 @@ void Plan::stats(long long *pairs, long long *points)
 }
 
 }  // namespace vp
+
+#include "vp_adjoint.hpp"

@@ -161,6 +161,17 @@ int vp_plan_device_A(vp_plan *plan, double **A, double **b, long long *lda)
     return VP_OK;
 }
 
+int vp_plan_adjoint(vp_plan *plan, const double *lambda, const double *mu, const double *vinf, double *grad)
+{
+    if (!plan || !plan->p.ctx) return VP_E_ARGUMENT;
+    vp::Ctx *c = plan->p.ctx;
+    if (!lambda || !mu || !vinf || !grad) return fail(c, VP_E_ARGUMENT, "null argument");
+    return guard(c, [&] {
+        (void)hipSetDevice(c->device);
+        plan->p.adjoint(lambda, mu, vinf, grad);
+    });
+}
+
 void vp_plan_destroy(vp_plan *plan)
 {
     if (!plan) return;
@@ -211,6 +222,17 @@ int vp_factor_solve(vp_factor *f, const double *R, int nrhs, double *X, int *ite
     if (!R || !X || nrhs <= 0) return fail(c, VP_E_ARGUMENT, "null argument or nrhs < 1");
     return guard(c, [&] {
         int it = f->f.solve(R, nrhs, X, false, 30, resid);
+        if (iters) *iters = it;
+    });
+}
+
+int vp_factor_solve_transposed(vp_factor *f, const double *R, int nrhs, double *X, int *iters, double *resid)
+{
+    if (!f || !f->f.ctx) return VP_E_ARGUMENT;
+    vp::Ctx *c = f->f.ctx;
+    if (!R || !X || nrhs <= 0) return fail(c, VP_E_ARGUMENT, "null argument or nrhs < 1");
+    return guard(c, [&] {
+        int it = f->f.solve_t(R, nrhs, X, resid);
         if (iters) *iters = it;
     });
 }

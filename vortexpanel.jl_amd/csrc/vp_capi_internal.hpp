@@ -11,6 +11,7 @@ namespace vp {
 
 struct Elem;
 struct TestFar;
+struct AsmParams;
 
 struct Ctx {
     int device = 0;
@@ -57,6 +58,14 @@ struct Plan {
     int2 *d_nearm = nullptr, *d_touchp = nullptr;
     unsigned int nnearm = 0, ntouchp = 0, nsimple = 0;
     int4 *d_simple = nullptr;
+    bool assembled = false;
+    // backward assembly (adjoint) buffers, made on first use
+    double *d_lam = nullptr, *d_mu = nullptr, *d_G = nullptr;
+    int *d_slotnode = nullptr;
+    unsigned long long *d_astats = nullptr;
+    unsigned int *d_acount = nullptr, aleaf_cap = 0, agroup_cap = 0;
+    struct Leaf *d_aleaves = nullptr;
+    int4 *d_agroups = nullptr;
     int2 *d_nonfar = nullptr;
     struct Leaf *d_leaves = nullptr;
 
@@ -65,6 +74,8 @@ struct Plan {
     void assemble(const double *sigma, const double *vinf, bool host_sigma);
     void stats(long long *pairs, long long *points);
     void kernel_ms(double *ms);
+    AsmParams params() const;
+    void adjoint(const double *lam, const double *mu, const double *vinf, double *G);
 };
 
 // The factored bordered system (solve.jl:65-87): the original FP64 bordered matrix M (row-major, npad x npad, the
@@ -92,6 +103,7 @@ struct Factor {
     int solve(const double *B, int nrhs, double *X, bool device_io, int max_iter, double *resid);
     double cond_estimate();
     void solve_transposed(const double *z, double *x);
+    int solve_t(const double *B, int nrhs, double *X, double *resid);
     void transpose_into(const double *in, double *out);
 };
 

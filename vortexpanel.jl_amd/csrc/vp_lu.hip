@@ -739,14 +739,22 @@ void Factor::transpose_into(const double *in, double *out)
 }
 
 // x = M^-T z with the factorization of Mᵀ (same butterflies-and-LU scheme, FP64 refinement against Mᵀ)
-void Factor::solve_transposed(const double *z, double *x)
+void Factor::solve_transposed(const double *z, double *x) { solve_t(z, 1, x, nullptr); }
+
+// Solve Mᵀ X = B (n x nrhs row-major host arrays) with the factorization of Mᵀ and FP64 refinement against Mᵀ
+int Factor::solve_t(const double *Bin, int nrhs, double *Xout, double *resid)
 {
+    if (!Mt) {
+        Mt = dmalloc<double>((size_t)ld * npad);
+        owned.push_back(Mt);
+        transpose_into(M, Mt);
+    }
     if (!tfac) {
         tfac = new Factor();
         tfac->ctx = ctx;
         tfac->init_from_dense(Mt, ld, n, npad, mode, anorm1);
     }
-    tfac->solve(z, 1, x, false, 30, nullptr);
+    return tfac->solve(Bin, nrhs, Xout, false, 30, resid);
 }
 
 // Factor a dense bordered matrix already on the device (n x n in an npad x npad row-major buffer).

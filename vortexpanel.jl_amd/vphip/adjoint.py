@@ -1,0 +1,224 @@
+"""The discrete adjoint of a wing solution: dCL/dx and dCDi/dx for every node coordinate, and dCL/dalpha.
+
+No counterpart in the reference (it has no derivatives); the objectives are the reference's own loads
+(validation/common/wing_case.jl:53-67: CL from the pressure force of the gradient-route Cp, CDi from the Trefftz plane,
+src/post/forces.jl:11-81).  For an objective J(x, q) of the bordered solution M(x) [q; l] = [Cᵀ b(x); 0]:
+
+    dJ/dx = dJ/dx|_q + d/dx [ lamᵀ (b(x) - A(x) [C; W] q) ],   lam = C psi_q,   Mᵀ psi = [dJ/dq; 0]
+
+(the border rows drop out: J does not see a uniform doublet, so psi has no border part).  The adjoint solve runs on
+the device (vp_factor_solve_transposed), the bracket is the backward assembly (vp_plan_adjoint, which recomputes
+the pair integrals instead of storing dA/dx), and the explicit terms dJ/dx|_q, dJ/dq, dJ/dalpha are per-element
+host formulas differentiated by complex steps (exact to rounding).
+
+What is held fixed: the quadrature rules of the forward assembly (tiers and piece positions in barycentric
+coordinates), the wake's offsets from its trailing edge nodes (its nodes move with them) and its direction, the DOFMap.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .api import AssemblyOptions, AssemblyPlan, Context, _check, build_elements, default_context, source_strengths
+from .dofmap import wake_rows
+from .solve import factorize_galerkin, solve_factored
+from .sweep import wing_freestream
+
+H = 1e-30  # complex step
+
+
+def _cross(a, b):
+    return np.stack([a[..., 1] * b[..., 2] - a[..., 2] * b[..., 1], a[..., 2] * b[..., 0] - a[..., 0] * b[..., 2],
+                     a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], axis=-1)
+
+
+def _dot(a, b):
+    return np.sum(a * b, axis=-1)
+
+
+def cl_terms(X, m, V, eL, area_ref):
+    """Per-triangle contributions to CL = F.eL/S with F = -sum Cp A n (pressure_force, forces.jl:11-19) and the
+    gradient-route Cp (Post.jl:72-84), as an analytic function of the vertices X (nt, 3, 3) and corner doublets m
+    (nt, 3): complex arguments give complex-step derivatives."""
+    a, b, c = X[:, 0], X[:, 1], X[:, 2]
+    N = _cross(b - a, c - a)
+    A2 = np.sqrt(_dot(N, N))
+    n = N / A2[:, None]
+    e = np.stack([c - b, a - c, b - a], axis=1)
+    gradl = _cross(n[:, None, :], e) / A2[:, None, None]
+    u = V - _dot(n, V)[:, None] * n + np.einsum("tk,tkj->tj", m, gradl)
+    cp = 1 - _dot(u, u) / _dot(V, V)
+    return -cp * (A2 / 2) * _dot(n, eL) / area_ref, u, gradl, A2, n
+
+
+def cl_explicit(g, dm, x, mu, alpha_deg, area_ref):
+    """CL and its partial derivatives at fixed doublets: d/dmu (nslot), d/dx (nn, 3), d/dalpha (per degree)"""
+    V, eL = wing_freestream(alpha_deg)
+    tris = g.tris
+    X = x[tris]
+    m = np.asarray(mu)[dm.corner_slot]
+    f, u, gradl, A2, n = cl_terms(X, m, V, eL, area_ref)
+    CL = float(np.sum(f))
+    # d/du of each term, then d/dmu through grad(lambda_k)
+    dfdu = (2 * (A2 / 2) * _dot(n, eL) / (area_ref * _dot(V, V)))[:, None] * u
+    dmu = np.zeros(dm.nslot)
+    np.add.at(dmu, dm.corner_slot, np.einsum("tj,tkj->tk", dfdu, gradl))
+    dx = np.zeros_like(x)
+    for k in range(3):
+        for d in range(3):
+            Xc = X.astype(complex)
+            Xc[:, k, d] += 1j * H
+            fk, *_ = cl_terms(Xc, m, V, eL, area_ref)
+            np.add.at(dx[:, d], tris[:, k], fk.imag / H)
+    a = np.radians(alpha_deg) + 1j * H
+    fa, *_ = cl_terms(X, m, np.array([np.cos(a), 0, np.sin(a)]), np.array([-np.sin(a), 0, np.cos(a)]), area_ref)
+    dalpha = float(np.sum(fa).imag / H) * np.pi / 180
+    return CL, dmu, dx, dalpha
+
+
+def trefftz_of(P, lines, closed, mu_w, d):
+    """trefftz_drag (forces.jl:53-81) as an analytic function of the trailing edge node positions P and mu_w (complex
+    steps welcome), D in units of q∞ before the 2/|V∞|^2 factor"""
+    Pp = P - _dot(P, d)[:, None] * d
+    vp, vg, sa, sb, sg = [], [], [], [], []
+    for line, cl in zip(lines, closed):
+        line = list(line)
+        m = len(line)
+        nseg = m if cl else m - 1
+        G = [(mu_w[line[i]] + mu_w[line[(i + 1) % m]]) / 2 for i in range(nseg)]
+        for i in range(m):
+            before = G[i - 1] if i > 0 else (G[nseg - 1] if cl else 0.0)
+            after = G[i] if i < nseg else 0.0
+            vp.append(Pp[line[i]])
+            vg.append(after - before)
+        for i in range(nseg):
+            sa.append(Pp[line[i]])
+            sb.append(Pp[line[(i + 1) % m]])
+            sg.append(G[i])
+    vp, vg, sa, sb, sg = (np.asarray(v) for v in (vp, vg, sa, sb, sg))
+    c = (sa + sb) / 2
+    ell = np.sqrt(_dot(sb - sa, sb - sa))
+    n = _cross(d, (sb - sa) / ell[:, None])
+    r = c[:, None, :] - vp[None, :, :]
+    v = np.einsum("k,skj->sj", vg / (4 * np.pi), _cross(d, r) / _dot(r, r)[:, :, None])
+    return np.sum(sg * _dot(n, v) * ell)
+
+
+def cdi_explicit(dm, x, mu_w, alpha_deg, area_ref):
+    """CDi (Trefftz) and its partial derivatives: d/dmu_w (nte), d/dx (nn, 3) through the trailing edge nodes,
+    d/dalpha (per degree)"""
+    wte = dm.wake_te
+    V, _ = wing_freestream(alpha_deg)
+    scale = 2 / np.dot(V, V) / area_ref
+    d = V / np.sqrt(np.dot(V, V))
+    P = np.asarray(x)[wte.node]
+    mw = np.asarray(mu_w, dtype=float)
+    CD = float(trefftz_of(P, wte.lines, wte.closed, mw, d)) * scale
+    dmw = np.zeros(len(mw))
+    for j in range(len(mw)):
+        e = mw.astype(complex)
+        e[j] += 1j * H
+        dmw[j] = trefftz_of(P, wte.lines, wte.closed, e, d).imag / H * scale
+    dx = np.zeros_like(x)
+    for j in range(len(P)):
+        for k in range(3):
+            Pc = P.astype(complex)
+            Pc[j, k] += 1j * H
+            dx[wte.node[j], k] += trefftz_of(Pc, wte.lines, wte.closed, mw, d).imag / H * scale
+    a = np.radians(alpha_deg) + 1j * H
+    Vc = np.array([np.cos(a), 0, np.sin(a)])
+    da = trefftz_of(P, wte.lines, wte.closed, mw, Vc / np.sqrt(np.sum(Vc * Vc))) * 2 / np.sum(Vc * Vc) / area_ref
+    return CD, dmw, dx, float(da.imag / H) * np.pi / 180
+
+
+class WingAdjoint:
+    """Forward solve of a wing with a rigid wake, then the adjoint gradients of CL and CDi.
+
+    precision: 'fp64' (FP64 assembly and LU) or 'mixed' (FP32 multipole far field, FP32 LU on MFMA; FP64
+    refinement of the forward and the adjoint solves)."""
+
+    def __init__(self, g, dm, wake, alpha_deg, area_ref, precision="fp64", opts: AssemblyOptions | None = None,
+                 ctx: Context | None = None):
+        self.g, self.dm, self.wake, self.alpha, self.S = g, dm, wake, float(alpha_deg), float(area_ref)
+        self.ctx = ctx or default_context()
+        self.precision = precision
+        self.timings = {}
+        V, _ = wing_freestream(alpha_deg)
+        self.V = V
+        t0 = time.perf_counter()
+        els = build_elements(g)
+        self.plan = AssemblyPlan(g, dm, wake, opts or AssemblyOptions(precision=precision), ctx=self.ctx)
+        self.plan.assemble(source_strengths(els, V), V)
+        self.b = self.plan.read_b()
+        self.W = wake_rows(dm, wake.te)
+        self.F = factorize_galerkin(dm, self.plan, W=self.W, mode=precision)
+        sol = solve_factored(self.F, self.b)
+        self.ctx.sync()
+        self.timings["forward_s"] = time.perf_counter() - t0
+        self.q, self.mu = sol["q"], sol["mu"]
+        self.mu_w = dm.K @ self.q                   # trailing edge doublet jumps
+        self.mu_wn = np.asarray(self.W @ self.q)    # wake node values (columns after the slots)
+
+    def loads(self):
+        CL, *_ = cl_explicit(self.g, self.dm, self.g.x, self.mu, self.alpha, self.S)
+        CD, *_ = cdi_explicit(self.dm, self.g.x, self.mu_w, self.alpha, self.S)
+        return dict(CL=CL, CDi=CD)
+
+    def _adjoint_weights(self, dJdq):
+        F = self.F
+        R = np.zeros(F.n)
+        R[:F.nq] = dJdq
+        X = np.zeros(F.n)
+        it, res = C.c_int(), C.c_double()
+        _check(self.ctx.h, self.ctx.lib.vp_factor_solve_transposed(F.h, L.ptr(L.f64(R)), 1, L.ptr(X), C.byref(it),
+                                                                   C.byref(res)))
+        return F.C @ X[:F.nq]
+
+    def _backward(self, lam):
+        """d/dx [lamᵀ (b - A [mu; mu_wake])] on the device, wake node gradients moved to their trailing edge nodes"""
+        nn = len(self.g.x)
+        mu_full = L.f64(np.concatenate([self.mu, self.mu_wn]))
+        G = np.zeros((nn + len(self.wake.x), 3))
+        _check(self.ctx.h, self.ctx.lib.vp_plan_adjoint(self.plan.h, L.ptr(L.f64(lam)), L.ptr(mu_full),
+                                                        L.ptr(L.f64(self.V)), L.ptr(G)))
+        gx = G[:nn].copy()
+        np.add.at(gx, self.dm.wake_te.node[self.wake.te], G[nn:])
+        return gx
+
+    def _db_dalpha(self):
+        """db/dalpha at fixed geometry and wake (per radian): b is linear in V∞ (sources and freestream term)"""
+        a = np.radians(self.alpha)
+        Vd = np.array([-np.sin(a), 0.0, np.cos(a)])
+        self.plan.assemble(source_strengths(build_elements(self.g), Vd), Vd)
+        bd = self.plan.read_b()
+        self.plan.assemble(source_strengths(build_elements(self.g), self.V), self.V)  # restore the forward state
+        return bd
+
+    def gradient(self, objective="CL"):
+        """dJ/dx (nn, 3) and dJ/dalpha (per degree) of J = CL or CDi"""
+        t0 = time.perf_counter()
+        if objective == "CL":
+            J, dmu, dx, da = cl_explicit(self.g, self.dm, self.g.x, self.mu, self.alpha, self.S)
+            dJdq = self.F.C.T @ dmu
+        elif objective == "CDi":
+            J, dmw, dx, da = cdi_explicit(self.dm, self.g.x, self.mu_w, self.alpha, self.S)
+            dJdq = self.dm.K.T @ dmw
+        else:
+            raise ValueError("objective must be CL or CDi")
+        t1 = time.perf_counter()
+        lam = self._adjoint_weights(dJdq)
+        t2 = time.perf_counter()
+        gx = dx + self._backward(lam)
+        self.ctx.sync()
+        t3 = time.perf_counter()
+        dalpha = da + float(lam @ self._db_dalpha()) * np.pi / 180
+        self.timings.update({f"{objective}_explicit_s": t1 - t0, f"{objective}_adjoint_solve_s": t2 - t1,
+                             f"{objective}_backward_s": t3 - t2})
+        return J, gx, dalpha
+
+    def close(self):
+        self.F.close()
+        self.plan.close()
