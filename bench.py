@@ -63,12 +63,97 @@ def cpu_baseline(g, dm, V, target_s=12.0):
                        f"integrals, {dt:.1f} s)")
 
 
+def wing_adjoint_bench():
+    """Swept tapered NACA 2412 wing (taper 0.5, 25 deg quarter-chord sweep, AR 10, 240 x 120 quads, rigid wake),
+    alpha = 4 deg: forward solve plus the adjoint gradients dCL/dx, dCDi/dx (all node coordinates) and dCL/dalpha,
+    plain FP64 against mixed (FP32 multipole far field and FP32 LU on MFMA, FP64 refinement)"""
+    from vphip import adjoint as AD, dofmap, mesh, sweep, wake as wk
+    spec = mesh.WingSpec(code="2412", n_around=240, n_span=120, aspect=10.0, taper=0.5, sweep_c4_deg=25.0)
+    g = mesh.naca_wing(spec)
+    dm = dofmap.build_dofmap(g)
+    V, _ = sweep.wing_freestream(4.0)
+    w = wk.build_wake(g, dm, V)
+    S = spec.aspect * spec.mean_chord ** 2
+    out = {"nt": g.nt, "nslot": dm.nslot, "wake_nodes": len(w.x)}
+    grads = {}
+    for p in ("fp64", "mixed"):
+        t0 = time.perf_counter()
+        adj = AD.WingAdjoint(g, dm, w, 4.0, S, precision=p)
+        t1 = time.perf_counter()
+        CL, gcl, dcl = adj.gradient("CL")
+        CD, gcd, _ = adj.gradient("CDi")
+        t2 = time.perf_counter()
+        grads[p] = (CL, CD, gcl, gcd, dcl)
+        out[p] = dict(total_s=t2 - t0, forward_s=t1 - t0, adjoint_s=t2 - t1, CL=CL, CDi=CD, dCL_dalpha_per_deg=dcl,
+                      pairs=adj.plan.pairs, **{k: v for k, v in adj.timings.items() if k != "forward_s"})
+        adj.close()
+        log(f"wing adjoint {p}: {out[p]}")
+    a, b = grads["fp64"], grads["mixed"]
+    out["abs_dCL_mixed_vs_fp64"] = abs(a[0] - b[0])
+    out["abs_dCDi_mixed_vs_fp64"] = abs(a[1] - b[1])
+    out["max_rel_dCLdx_mixed_vs_fp64"] = float(np.abs(a[2] - b[2]).max() / np.abs(a[2]).max())
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["total_s"] / out["mixed"]["total_s"]
+    return out
+
+
+def alpha_sweep_bench():
+    """Rectangular NACA 0012 wing, AR 8, 160 x 60 quads, body-fixed rigid wake to 20 chords, 64 angles from -4 to
+    12 deg as one multi-RHS solve (mixed precision), CL by pressure and CDi by Trefftz per angle"""
+    from vphip import dofmap, mesh, sweep, wake as wk
+    spec = mesh.WingSpec(code="0012", n_around=160, n_span=60, aspect=8.0)
+    g = mesh.naca_wing(spec)
+    dm = dofmap.build_dofmap(g)
+    w = wk.build_wake(g, dm, np.array([1.0, 0.0, 0.0]), wk.WakeOptions(length=2.5))
+    t0 = time.perf_counter()
+    sw = sweep.AlphaSweep(g, dm, w, spec.aspect * spec.mean_chord ** 2, precision="mixed")
+    alphas = np.linspace(-4.0, 12.0, 64)
+    r = sw.solve(alphas)
+    t1 = time.perf_counter()
+    out = dict(nt=g.nt, nslot=dm.nslot, n_alpha=64, total_s=t1 - t0, **sw.timings,
+               CL_at_0_4_8_12=[float(np.interp(a, alphas, r["CL"])) for a in (0.0, 4.0, 8.0, 12.0)],
+               CDi_at_4=float(np.interp(4.0, alphas, r["CDi_T"])),
+               dCL_dalpha_per_deg=float(np.polyfit(alphas[16:40], r["CL"][16:40], 1)[0]))
+    sw.close()
+    log(f"alpha sweep: {out}")
+    return out
+
+
+def nbody_bench(g, dm, V, mu):
+    """surface_velocity on the ellipsoid solution and blob sums at the free wake's end size (84,000 particles)"""
+    from vphip import api, post
+    out = {}
+    sig = api.source_strengths(api.build_elements(g), V)
+    for p in ("fp64", "mixed"):
+        post.surface_velocity(g, dm, sig, mu, V, precision=p)
+        t0 = time.perf_counter()
+        post.surface_velocity(g, dm, sig, mu, V, precision=p)
+        dt = time.perf_counter() - t0
+        out[f"surface_velocity_{p}_s"] = dt
+        out[f"surface_velocity_{p}_evals_per_s"] = g.nt * g.nt / dt
+    rng = np.random.default_rng(0)
+    nb = 84000
+    bx = rng.standard_normal((nb, 3)) * np.array([3.0, 3.0, 0.3])
+    ba = rng.standard_normal((nb, 3)) * 1e-3
+    bd = np.full(nb, 0.01)
+    for p in ("fp64", "mixed"):
+        post.blob_velocities(bx[:1000], ba[:1000], bd[:1000], bx[:1000], precision=p)
+        t0 = time.perf_counter()
+        post.blob_velocities(bx, ba, bd, bx, precision=p)
+        dt = time.perf_counter() - t0
+        out[f"blob_{p}_s"] = dt
+        out[f"blob_{p}_interactions_per_s"] = nb * nb / dt
+        out[f"blob_{p}_tflops"] = nb * nb * 28 / dt / 1e12
+    log(f"nbody: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="only the assembly metric (no wing, sweep, N-body)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X, and the path runs on one device "
@@ -143,9 +228,17 @@ def main():
                          mu_max=float(np.max(np.abs(err)) / scale), mu=out["mu"])
         F.close()
     dmu = float(np.max(np.abs(solves["mixed"]["mu"] - solves["fp64"]["mu"])) / np.max(np.abs(solves["fp64"]["mu"])))
+    extras = {}
+    if not args.no_extras:
+        extras["nbody"] = nbody_bench(g, dm, V, solves["fp64"]["mu"])
     for m in solves:
         solves[m].pop("mu")
         log(f"{m} solve {solves[m]}")
+    for p_ in plans.values():
+        p_.close()
+    if not args.no_extras:
+        extras["alpha_sweep"] = alpha_sweep_bench()
+        extras["wing_adjoint"] = wing_adjoint_bench()
     cpu = None if args.no_cpu else cpu_baseline(g, dm, V)
     if cpu:
         log(f"cpu baseline {cpu}")
@@ -175,6 +268,7 @@ def main():
                   "max_entry_rel_diff_vs_fp64": entry_err, "max_mu_rel_diff_vs_fp64": dmu},
         "solve": solves,
         "kernels_ms": tm,
+        **extras,
     }
     print(json.dumps(line))
 

@@ -8,234 +8,21 @@
 // Galerkin solution (the discrete adjoint); there is no counterpart in the reference, which has no derivatives.
 //
 // Nothing of dA/dx is stored.  Every pair's integrals are recomputed with the rule pieces the forward assembly chose
-// (its far test and its non-far pair list; pieces at fixed barycentric positions), and per pair
+// (its far tiles and its routed non-far lists, evaluated by the forward's kernels in adjoint mode; pieces at fixed
+// barycentric positions), and per pair
 //
 //     P_ts = -A_T sum_q w_q psi_lambda(q) Q.w(x_q; S),   Q = sigma_S n_T + n_T x Gamma_S,
 //
 // (pair_block, assemble.jl:245-258, contracted with lambda and mu) is differentiated: the kernel w by forward-mode dual
 // numbers with respect to the three vertices of S (three passes of three tangents; the derivative with respect to the
-// point is minus their sum, by translation invariance), and the test triangle's area and normal, the source's normal
+// point is minus their sum, by translation invariance) or, for the multipole, its reverse derivative by hand; the test
+// triangle's area and normal, the source's normal
 // (sigma) and sheet (Gamma) by their closed-form adjoints.  Filaments (segment_block, :312-321) and the freestream
 // term (:422) likewise.  Gradients are added per node: body nodes, then wake nodes (offset by the body's node count).
 //
 // This file is included by vp_assemble.hip (it uses the plan's tiles, lists and rule generator).
 
 namespace vp {
-
-// ------------------------------------------------------------------------------------------ dual numbers -----------
-
-struct D3 {  // a value and its derivatives along three directions
-    double v, d[3];
-    D3() = default;
-    __device__ constexpr D3(double x) : v(x), d{0.0, 0.0, 0.0} {}
-    __device__ constexpr D3(double x, double a, double b, double c) : v(x), d{a, b, c} {}
-};
-
-__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return D3(a.v + b.v, a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2]); }
-__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return D3(a.v - b.v, a.d[0] - b.d[0], a.d[1] - b.d[1], a.d[2] - b.d[2]); }
-__device__ __forceinline__ D3 operator-(D3 a) { return D3(-a.v, -a.d[0], -a.d[1], -a.d[2]); }
-__device__ __forceinline__ D3 operator*(D3 a, D3 b)
-{
-    return D3(a.v * b.v, a.d[0] * b.v + a.v * b.d[0], a.d[1] * b.v + a.v * b.d[1], a.d[2] * b.v + a.v * b.d[2]);
-}
-__device__ __forceinline__ D3 operator*(D3 a, double s) { return D3(a.v * s, a.d[0] * s, a.d[1] * s, a.d[2] * s); }
-__device__ __forceinline__ D3 operator*(double s, D3 a) { return a * s; }
-__device__ __forceinline__ D3 operator/(D3 a, D3 b)
-{
-    const double q = a.v / b.v, ib = 1.0 / b.v;
-    return D3(q, (a.d[0] - q * b.d[0]) * ib, (a.d[1] - q * b.d[1]) * ib, (a.d[2] - q * b.d[2]) * ib);
-}
-__device__ __forceinline__ D3 &operator+=(D3 &a, D3 b) { return a = a + b; }
-__device__ __forceinline__ D3 &operator-=(D3 &a, D3 b) { return a = a - b; }
-__device__ __forceinline__ bool operator<(D3 a, D3 b) { return a.v < b.v; }
-__device__ __forceinline__ bool operator>(D3 a, D3 b) { return a.v > b.v; }
-__device__ __forceinline__ bool operator<=(D3 a, D3 b) { return a.v <= b.v; }
-__device__ __forceinline__ bool operator>=(D3 a, D3 b) { return a.v >= b.v; }
-__device__ __forceinline__ bool operator==(D3 a, D3 b) { return a.v == b.v; }
-__device__ __forceinline__ D3 dsqrt(D3 a)
-{
-    const double s = __builtin_sqrt(a.v), h = 0.5 / s;
-    return D3(s, a.d[0] * h, a.d[1] * h, a.d[2] * h);
-}
-__device__ __forceinline__ D3 dlog(D3 a)
-{
-    const double i = 1.0 / a.v;
-    return D3(log(a.v), a.d[0] * i, a.d[1] * i, a.d[2] * i);
-}
-__device__ __forceinline__ D3 datan2(D3 y, D3 x)
-{
-    const double i = 1.0 / (x.v * x.v + y.v * y.v);
-    return D3(atan2(y.v, x.v), (x.v * y.d[0] - y.v * x.d[0]) * i, (x.v * y.d[1] - y.v * x.d[1]) * i,
-              (x.v * y.d[2] - y.v * x.d[2]) * i);
-}
-
-__device__ __forceinline__ V3<D3> dconst(V3<double> x) { return mk3(D3(x.x), D3(x.y), D3(x.z)); }
-__device__ __forceinline__ V3<D3> dseed(V3<double> x, double on)
-{
-    return mk3(D3(x.x, on, 0.0, 0.0), D3(x.y, 0.0, on, 0.0), D3(x.z, 0.0, 0.0, on));
-}
-
-// source_velocity_exact (src/kernels/source_triangle.jl:75-89) with the triangle's data formed from its vertices
-// (the constructor, :18-27), in dual numbers.  Off the own element the solid angle is always the atan2 form (its
-// derivative across the plane is the one the closed form has; the in-plane branch's zero has none).
-__device__ __forceinline__ V3<D3> sv_exact_vertices(const V3<D3> *v, V3<double> x, int side, const Elem *self,
-                                                    double diam)
-{
-    const V3<D3> e0 = v[1] - v[0], e1 = v[2] - v[1], e2 = v[0] - v[2];
-    const V3<D3> nA = cross(e0, v[2] - v[0]);
-    const D3 A2 = dsqrt(dot(nA, nA));
-    const V3<D3> n = (D3(1.0) / A2) * nA;
-    const D3 l0 = dsqrt(dot(e0, e0)), l1 = dsqrt(dot(e1, e1)), l2 = dsqrt(dot(e2, e2));
-    const V3<D3> xd = dconst(x);
-    const V3<D3> ra = xd - v[0], rb = xd - v[1], rc = xd - v[2];
-    const V3<D3> nu0 = (D3(1.0) / l0) * cross(e0, n), nu1 = (D3(1.0) / l1) * cross(e1, n),
-                 nu2 = (D3(1.0) / l2) * cross(e2, n);
-    const V3<D3> wt = edge_log(ra, rb, l0) * nu0 + edge_log(rb, rc, l1) * nu1 + edge_log(rc, ra, l2) * nu2;
-    D3 Om;
-    const V3<D3> c = D3(1.0 / 3) * ((v[0] + v[1]) + v[2]);
-    const D3 z = dot(xd - c, n);
-    if (side != 0 && fabs(z.v) <= 1e-12 * diam) {
-        Om = D3(side * 2 * kPi * inplane_fraction(*self, x, 1e-12));
-    } else {
-        const D3 la = dsqrt(dot(ra, ra)), lb = dsqrt(dot(rb, rb)), lc = dsqrt(dot(rc, rc));
-        const D3 N = dot(ra, cross(rb, rc));
-        const D3 D = la * lb * lc + dot(ra, rb) * lc + dot(ra, rc) * lb + dot(rb, rc) * la;
-        Om = D3(2.0) * datan2(N, D);
-    }
-    return D3(1.0 / (4 * kPi)) * (wt + Om * n);
-}
-
-// source_velocity_far (source_triangle.jl:100-119) from the vertices, in dual numbers
-__device__ __forceinline__ V3<D3> sv_far_vertices(const V3<D3> *v, V3<double> x)
-{
-    const V3<D3> c = D3(1.0 / 3) * ((v[0] + v[1]) + v[2]);
-    const V3<D3> nA = cross(v[1] - v[0], v[2] - v[0]);
-    SrcT<D3> s;
-    for (int k = 0; k < 3; k++) s.v[k] = v[k] - c;
-    s.area = D3(0.5) * dsqrt(dot(nA, nA));
-    return sv_far_rel(s, dconst(x) - c);
-}
-
-// phi = Q.w(x; S) (w = source_velocity, source_triangle.jl:134-138, with the forward's switch) and d phi/d(vertex k of
-// S) for k = 0, 1, 2; the value w(x) in wv.
-__device__ void tri_point_grad(const V3<double> *XS, const Elem &S, V3<double> x, V3<double> Q, int side,
-                               double kernel_far, V3<double> &wv, V3<double> *g)
-{
-    const V3<double> r0 = cls_sub(x, ld3(S.c));
-    const double fd = kernel_far * S.diam;
-    const bool far = cls_dot(r0, r0) > fd * fd;
-#pragma unroll 1
-    for (int k = 0; k < 3; k++) {
-        V3<D3> v[3];
-        for (int j = 0; j < 3; j++) v[j] = dseed(XS[j], j == k ? 1.0 : 0.0);
-        const V3<D3> w = far ? sv_far_vertices(v, x) : sv_exact_vertices(v, x, side, &S, S.diam);
-        const D3 phi = w.x * Q.x + w.y * Q.y + w.z * Q.z;
-        g[k] = mk3(phi.d[0], phi.d[1], phi.d[2]);
-        if (k == 0) wv = mk3(w.x.v, w.y.v, w.z.v);
-    }
-}
-
-// phi = n.(mu_a va + mu_b vb) of the filament from fa to fb (segment_velocity, src/kernels/segment.jl:15-25) and its
-// derivatives with respect to fa and fb; the value mu_a va + mu_b vb in wv.
-__device__ void seg_point_grad(V3<double> fa, V3<double> fb, V3<double> x, V3<double> n, double mua, double mub,
-                               V3<double> &wv, V3<double> *g)
-{
-#pragma unroll 1
-    for (int k = 0; k < 2; k++) {
-        const V3<D3> a = dseed(fa, k == 0 ? 1.0 : 0.0), b = dseed(fb, k == 1 ? 1.0 : 0.0), xd = dconst(x);
-        V3<D3> va, vb;
-        seg_vel(xd - a, xd - b, va, vb);
-        const V3<D3> w = D3(mua) * va + D3(mub) * vb;
-        const D3 phi = w.x * n.x + w.y * n.y + w.z * n.z;
-        g[k] = mk3(phi.d[0], phi.d[1], phi.d[2]);
-        if (k == 0) wv = mk3(w.x.v, w.y.v, w.z.v);
-    }
-}
-
-// ------------------------------------------------------------------------------------------ triangle adjoints ------
-
-// g[k] += gradient of p.N (N = (X1 - X0) x (X2 - X0), twice the area times the normal) with respect to vertex k
-__device__ __forceinline__ void cross_adj(const V3<double> *X, V3<double> p, V3<double> *g)
-{
-    g[0] = g[0] + cross(p, X[2] - X[1]);
-    g[1] = g[1] + cross(X[2] - X[0], p);
-    g[2] = g[2] + cross(p, X[1] - X[0]);
-}
-
-// g[k] += gradient of c.n (n the unit normal) with respect to vertex k
-__device__ __forceinline__ void normal_adj(const V3<double> *X, V3<double> c, V3<double> *g)
-{
-    const V3<double> N = cross(X[1] - X[0], X[2] - X[0]);
-    const double A2 = dsqrt(dot(N, N));
-    const V3<double> n = (1.0 / A2) * N;
-    cross_adj(X, (1.0 / A2) * (c - dot(c, n) * n), g);
-}
-
-// g[k] += s times the gradient of |N| (twice the area) with respect to vertex k
-__device__ __forceinline__ void area2_adj(const V3<double> *X, double s, V3<double> *g)
-{
-    const V3<double> N = cross(X[1] - X[0], X[2] - X[0]);
-    cross_adj(X, (s / dsqrt(dot(N, N))) * N, g);
-}
-
-// The sheet Gamma = sum_b mu_b gamma_b of a triangle (gamma_b = -e_b/(2A), src/elements/flat_triangle.jl:23-31)
-__device__ __forceinline__ V3<double> sheet_of(const V3<double> *X, const double *mu)
-{
-    const V3<double> N = cross(X[1] - X[0], X[2] - X[0]);
-    const V3<double> E = mu[0] * (X[2] - X[1]) + mu[1] * (X[0] - X[2]) + mu[2] * (X[1] - X[0]);
-    return (-1.0 / dsqrt(dot(N, N))) * E;
-}
-
-// g[k] += gradient of d.Gamma with respect to vertex k
-__device__ __forceinline__ void sheet_adj(const V3<double> *X, const double *mu, V3<double> d, V3<double> *g)
-{
-    const V3<double> N = cross(X[1] - X[0], X[2] - X[0]);
-    const double A2 = dsqrt(dot(N, N));
-    const V3<double> E = mu[0] * (X[2] - X[1]) + mu[1] * (X[0] - X[2]) + mu[2] * (X[1] - X[0]);
-    g[0] = g[0] - ((mu[1] - mu[2]) / A2) * d;
-    g[1] = g[1] - ((mu[2] - mu[0]) / A2) * d;
-    g[2] = g[2] - ((mu[0] - mu[1]) / A2) * d;
-    area2_adj(X, dot(d, E) / (A2 * A2), g);
-}
-
-// A pair's accumulated sums -> the gradient of P_ts with respect to T's and S's vertices (see the file head).
-struct PairSums {
-    V3<double> Sv;     // sum_q a_q w_q
-    V3<double> GS[3];  // sum_q a_q d(Q.w)/d(vertex k of S)
-    V3<double> GX[3];  // sum_q a_q l_qk d(Q.w)/dx
-};
-
-__device__ void finish_tri_pair(const V3<double> *XT, V3<double> nT, double AT, const V3<double> *XS, const double *muS,
-                                double sigS, V3<double> Gam, bool body, V3<double> Vinf, V3<double> Q,
-                                const PairSums &p, V3<double> *gT, V3<double> *gS)
-{
-    for (int k = 0; k < 3; k++) {
-        gT[k] = (-AT) * p.GX[k];
-        gS[k] = (-AT) * p.GS[k];
-    }
-    area2_adj(XT, -0.5 * dot(Q, p.Sv), gT);
-    normal_adj(XT, (-AT) * (sigS * p.Sv + cross(Gam, p.Sv)), gT);
-    if (body) normal_adj(XS, (AT * dot(nT, p.Sv)) * Vinf, gS);  // sigma_S = -V∞.n_S
-    sheet_adj(XS, muS, (-AT) * cross(p.Sv, nT), gS);
-}
-
-__device__ __forceinline__ void gadd(double *G, int node, V3<double> v)
-{
-    unsafeAtomicAdd(&G[3 * node], v.x);
-    unsafeAtomicAdd(&G[3 * node + 1], v.y);
-    unsafeAtomicAdd(&G[3 * node + 2], v.z);
-}
-
-__device__ __forceinline__ int col_node(int col, int nslot, int nn, const int *slot_node)
-{
-    return col < nslot ? slot_node[col] : nn + (col - nslot);
-}
-
-__device__ __forceinline__ double wsum(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
 
 // ------------------------------------------------------------------------------------------ kernels ---------------
 
@@ -378,8 +165,8 @@ __global__ void adj_seg_far(AsmParams P, RuleTab rt, const TestFar *__restrict__
     gadd(G, P.nn + segcol[2 * f + 1] - P.nslot, (-F.area) * GF[1]);
 }
 
-// Non-far pairs: one wavefront per pair, the points of all its rule pieces (from the forward's rule generator) over
-// the lanes, as group_eval does.
+// The pairs of the general touching list (contact, geometric touch pairs, filaments): one wavefront per pair, the
+// points of all its rule pieces (the forward's leaf list) over the lanes, as group_eval does.
 __global__ __launch_bounds__(256) void adj_group_eval(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                       const int *__restrict__ corner_slot, const int *__restrict__ nodes,
                                                       const int *__restrict__ src_cols, const double *__restrict__ seg,
@@ -500,9 +287,7 @@ void Plan::adjoint(const double *lam_h, const double *mu_h, const double *vinf, 
         d_mu = dalloc<double>(ncol);
         d_G = dalloc<double>((size_t)3 * (nn + nw));
         d_slotnode = dalloc<int>(nslot);
-        d_astats = dalloc<unsigned long long>(16);
-        d_acount = dalloc<unsigned int>(2);
-        owned.insert(owned.end(), {d_lam, d_mu, d_G, d_slotnode, d_astats, d_acount});
+        owned.insert(owned.end(), {d_lam, d_mu, d_G, d_slotnode});
         slot_nodes<<<(nt + 255) / 256, 256, 0, st>>>(d_cs, d_nodes, nt, d_slotnode);
         HIPCHECK(hipGetLastError());
     }
@@ -520,38 +305,27 @@ void Plan::adjoint(const double *lam_h, const double *mu_h, const double *vinf, 
                                                                    d_mu, d_G);
         HIPCHECK(hipGetLastError());
     }
-    // rule pieces of every non-far pair of the last assembly (the forward's generator, all tiers)
-    if (nnonfar) {
-        for (int attempt = 0;; attempt++) {
-            if (attempt == 3) throw std::runtime_error("adjoint rule pieces did not fit after regrowing");
-            if (!d_aleaves) {
-                aleaf_cap = std::max<unsigned int>(aleaf_cap, nnonfar * 4 + 1024);
-                agroup_cap = std::max<unsigned int>(agroup_cap, nnonfar + 1024);
-                d_aleaves = dalloc<Leaf>(aleaf_cap);
-                d_agroups = dalloc<int4>(agroup_cap);
-            }
-            HIPCHECK(hipMemsetAsync(d_acount, 0, sizeof(unsigned int) * 2, st));
-            touch_gen<<<std::min<unsigned int>((nnonfar + 255) / 256, 65535u), 256, 0, st>>>(
-                P, rt, d_el, d_nodes, d_seg, d_nonfar, nnonfar, d_touch, ntouch, d_aleaves, aleaf_cap, d_agroups,
-                agroup_cap, d_acount, d_astats, nullptr);
-            HIPCHECK(hipGetLastError());
-            unsigned int c[2];
-            HIPCHECK(hipMemcpyAsync(c, d_acount, sizeof(c), hipMemcpyDeviceToHost, st));
-            HIPCHECK(hipStreamSynchronize(st));
-            if (c[0] <= aleaf_cap && c[1] <= agroup_cap) {
-                adj_group_eval<<<std::min<unsigned int>((c[1] + 3) / 4, 65535u), 256, 0, st>>>(
-                    P, rt, d_el, d_cs, d_nodes, d_cols, d_seg, d_segcol, d_aleaves, d_agroups, c[1], d_lam, d_mu, vx, vy,
-                    vz, d_G);
-                HIPCHECK(hipGetLastError());
-                break;
-            }
-            (void)hipFree(d_aleaves);
-            (void)hipFree(d_agroups);
-            d_aleaves = nullptr;
-            d_agroups = nullptr;
-            aleaf_cap = std::max(aleaf_cap, c[0] + 1024);
-            agroup_cap = std::max(agroup_cap, c[1] + 1024);
-        }
+    // the non-far pairs of the last assembly, through the forward's own routing and kernels in adjoint mode
+    AdjArgs aa{d_lam, d_mu, d_G, d_nodes, vx, vy, vz};
+    if (nsimple) {
+        touch_simple<true><<<std::min<unsigned int>((nsimple + 3) / 4, 65535u), 256, 0, st>>>(
+            P, rt, d_el, d_sigma, d_cs, d_cols, d_simple, nsimple, nullptr, nullptr, nullptr, nullptr, aa);
+        HIPCHECK(hipGetLastError());
+    }
+    if (nnear) {
+        near_thread<true><<<std::min<unsigned int>((nnear + 255) / 256, 65535u), 256, 0, st>>>(
+            P, rt, d_el, d_sigma, d_cs, d_cols, d_near, nnear, nullptr, nullptr, nullptr, nullptr, aa);
+        HIPCHECK(hipGetLastError());
+    }
+    if (nnearm) {
+        near_group<true><<<std::min<unsigned int>((nnearm + 3) / 4, 65535u), 256, 0, st>>>(
+            P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, nnearm, nullptr, nullptr, nullptr, nullptr, aa);
+        HIPCHECK(hipGetLastError());
+    }
+    if (ngroups) {
+        adj_group_eval<<<std::min<unsigned int>((ngroups + 3) / 4, 65535u), 256, 0, st>>>(
+            P, rt, d_el, d_cs, d_nodes, d_cols, d_seg, d_segcol, d_leaves, d_groups, ngroups, d_lam, d_mu, vx, vy, vz, d_G);
+        HIPCHECK(hipGetLastError());
     }
     adj_freestream<<<(nt + 255) / 256, 256, 0, st>>>(d_el, d_cs, d_nodes, nt, opts.test, d_lam, vx, vy, vz, d_G);
     HIPCHECK(hipGetLastError());

@@ -1,0 +1,341 @@
+// Device helpers of the backward assembly (vp_adjoint.hpp has its kernels and the host side): dual numbers, the
+// source kernel from its vertices in dual numbers, the octupole multipole's reverse derivative, the closed-form
+// adjoints of a triangle's normal, area and sheet, and the per-pair sums.  Included by vp_assemble.hip ahead of the
+// evaluation kernels, which take an adjoint mode.
+#pragma once
+
+namespace vp {
+
+// ------------------------------------------------------------------------------------------ dual numbers -----------
+
+struct D3 {  // a value and its derivatives along three directions
+    double v, d[3];
+    D3() = default;
+    __device__ constexpr D3(double x) : v(x), d{0.0, 0.0, 0.0} {}
+    __device__ constexpr D3(double x, double a, double b, double c) : v(x), d{a, b, c} {}
+};
+
+__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return D3(a.v + b.v, a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2]); }
+__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return D3(a.v - b.v, a.d[0] - b.d[0], a.d[1] - b.d[1], a.d[2] - b.d[2]); }
+__device__ __forceinline__ D3 operator-(D3 a) { return D3(-a.v, -a.d[0], -a.d[1], -a.d[2]); }
+__device__ __forceinline__ D3 operator*(D3 a, D3 b)
+{
+    return D3(a.v * b.v, a.d[0] * b.v + a.v * b.d[0], a.d[1] * b.v + a.v * b.d[1], a.d[2] * b.v + a.v * b.d[2]);
+}
+__device__ __forceinline__ D3 operator*(D3 a, double s) { return D3(a.v * s, a.d[0] * s, a.d[1] * s, a.d[2] * s); }
+__device__ __forceinline__ D3 operator*(double s, D3 a) { return a * s; }
+__device__ __forceinline__ D3 operator/(D3 a, D3 b)
+{
+    const double q = a.v / b.v, ib = 1.0 / b.v;
+    return D3(q, (a.d[0] - q * b.d[0]) * ib, (a.d[1] - q * b.d[1]) * ib, (a.d[2] - q * b.d[2]) * ib);
+}
+__device__ __forceinline__ D3 &operator+=(D3 &a, D3 b) { return a = a + b; }
+__device__ __forceinline__ D3 &operator-=(D3 &a, D3 b) { return a = a - b; }
+__device__ __forceinline__ bool operator<(D3 a, D3 b) { return a.v < b.v; }
+__device__ __forceinline__ bool operator>(D3 a, D3 b) { return a.v > b.v; }
+__device__ __forceinline__ bool operator<=(D3 a, D3 b) { return a.v <= b.v; }
+__device__ __forceinline__ bool operator>=(D3 a, D3 b) { return a.v >= b.v; }
+__device__ __forceinline__ bool operator==(D3 a, D3 b) { return a.v == b.v; }
+__device__ __forceinline__ D3 dsqrt(D3 a)
+{
+    const double s = __builtin_sqrt(a.v), h = 0.5 / s;
+    return D3(s, a.d[0] * h, a.d[1] * h, a.d[2] * h);
+}
+__device__ __forceinline__ D3 dlog(D3 a)
+{
+    const double i = 1.0 / a.v;
+    return D3(log(a.v), a.d[0] * i, a.d[1] * i, a.d[2] * i);
+}
+__device__ __forceinline__ D3 datan2(D3 y, D3 x)
+{
+    const double i = 1.0 / (x.v * x.v + y.v * y.v);
+    return D3(atan2(y.v, x.v), (x.v * y.d[0] - y.v * x.d[0]) * i, (x.v * y.d[1] - y.v * x.d[1]) * i,
+              (x.v * y.d[2] - y.v * x.d[2]) * i);
+}
+
+__device__ __forceinline__ V3<D3> dconst(V3<double> x) { return mk3(D3(x.x), D3(x.y), D3(x.z)); }
+__device__ __forceinline__ V3<D3> dseed(V3<double> x, double on)
+{
+    return mk3(D3(x.x, on, 0.0, 0.0), D3(x.y, 0.0, on, 0.0), D3(x.z, 0.0, 0.0, on));
+}
+
+// source_velocity_exact (src/kernels/source_triangle.jl:75-89) with the triangle's data formed from its vertices
+// (the constructor, :18-27), in dual numbers.  Off the own element the solid angle is always the atan2 form (its
+// derivative across the plane is the one the closed form has; the in-plane branch's zero has none).
+__device__ __forceinline__ V3<D3> sv_exact_vertices(const V3<D3> *v, V3<double> x, int side, const Elem *self,
+                                                    double diam)
+{
+    const V3<D3> e0 = v[1] - v[0], e1 = v[2] - v[1], e2 = v[0] - v[2];
+    const V3<D3> nA = cross(e0, v[2] - v[0]);
+    const D3 A2 = dsqrt(dot(nA, nA));
+    const V3<D3> n = (D3(1.0) / A2) * nA;
+    const D3 l0 = dsqrt(dot(e0, e0)), l1 = dsqrt(dot(e1, e1)), l2 = dsqrt(dot(e2, e2));
+    const V3<D3> xd = dconst(x);
+    const V3<D3> ra = xd - v[0], rb = xd - v[1], rc = xd - v[2];
+    const V3<D3> nu0 = (D3(1.0) / l0) * cross(e0, n), nu1 = (D3(1.0) / l1) * cross(e1, n),
+                 nu2 = (D3(1.0) / l2) * cross(e2, n);
+    const V3<D3> wt = edge_log(ra, rb, l0) * nu0 + edge_log(rb, rc, l1) * nu1 + edge_log(rc, ra, l2) * nu2;
+    D3 Om;
+    const V3<D3> c = D3(1.0 / 3) * ((v[0] + v[1]) + v[2]);
+    const D3 z = dot(xd - c, n);
+    if (side != 0 && fabs(z.v) <= 1e-12 * diam) {
+        Om = D3(side * 2 * kPi * inplane_fraction(*self, x, 1e-12));
+    } else {
+        const D3 la = dsqrt(dot(ra, ra)), lb = dsqrt(dot(rb, rb)), lc = dsqrt(dot(rc, rc));
+        const D3 N = dot(ra, cross(rb, rc));
+        const D3 D = la * lb * lc + dot(ra, rb) * lc + dot(ra, rc) * lb + dot(rb, rc) * la;
+        Om = D3(2.0) * datan2(N, D);
+    }
+    return D3(1.0 / (4 * kPi)) * (wt + Om * n);
+}
+
+// source_velocity_far (source_triangle.jl:100-119) from the vertices, in dual numbers
+__device__ __forceinline__ V3<D3> sv_far_vertices(const V3<D3> *v, V3<double> x)
+{
+    const V3<D3> c = D3(1.0 / 3) * ((v[0] + v[1]) + v[2]);
+    const V3<D3> nA = cross(v[1] - v[0], v[2] - v[0]);
+    SrcT<D3> s;
+    for (int k = 0; k < 3; k++) s.v[k] = v[k] - c;
+    s.area = D3(0.5) * dsqrt(dot(nA, nA));
+    return sv_far_rel(s, dconst(x) - c);
+}
+
+// The octupole multipole (source_velocity_far, source_triangle.jl:100-119) and the reverse derivative of
+// phi = Q.w with respect to the source's vertices, by hand: phi depends on r = x - c, the vertex offsets d_k = v_k - c
+// and the area A; with dr_k = d_k.r, dd_k = d_k.d_k and iRn = |r|^-n,
+//   d phi/d v_j = d phi/d d_j - (1/3) sum_k d phi/d d_k - (1/3) d phi/d r + (phi/A) dA/d v_j.
+// About 200 flops per point against ~3000 for dual numbers.
+__device__ __forceinline__ void far_point_grad(const V3<double> *XS, const Elem &S, V3<double> x, V3<double> Q,
+                                               V3<double> &wv, V3<double> *g)
+{
+    const V3<double> c = ld3(S.c), n = ld3(S.n);
+    const double A = S.area, s = A * (1.0 / (4 * kPi));
+    const V3<double> r = x - c;
+    const double R2 = dot(r, r), iR = 1.0 / dsqrt(R2);
+    const double iR2 = iR * iR, iR3 = iR2 * iR, iR5 = iR3 * iR2, iR7 = iR5 * iR2, iR9 = iR7 * iR2, iR11 = iR9 * iR2;
+    const double qr = dot(Q, r);
+    double cr = 0.0, rr = -3.0 * qr * iR5;
+    V3<double> w = mk3(0.0, 0.0, 0.0), gr = w, gd[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const V3<double> d = XS[k] - c;
+        const double dr = dot(d, r), dd = dot(d, d), qd = dot(Q, d);
+        cr += (15 * dr * dr * iR7 - 3 * dd * iR5) * (1.0 / 24) - (45 * dd * dr * iR7 - 105 * dr * dr * dr * iR9) * (1.0 / 180);
+        const double cd = -6 * dr * iR5 * (1.0 / 24) - (45 * dr * dr * iR7 - 9 * dd * iR5) * (1.0 / 180);
+        w = w + cd * d;
+        const double pdr = qr * (30 * dr * iR7 * (1.0 / 24) - (45 * dd * iR7 - 315 * dr * dr * iR9) * (1.0 / 180)) +
+                           qd * (-6 * iR5 * (1.0 / 24) - 90 * dr * iR7 * (1.0 / 180));
+        const double pdd = qr * (-3 * iR5 * (1.0 / 24) - 45 * dr * iR7 * (1.0 / 180)) + qd * (9 * iR5 * (1.0 / 180));
+        gr = gr + pdr * d;
+        rr += qr * (-105 * dr * dr * iR9 * (1.0 / 24) + 15 * dd * iR7 * (1.0 / 24) + 315 * dd * dr * iR9 * (1.0 / 180) -
+                    945 * dr * dr * dr * iR11 * (1.0 / 180)) +
+              qd * (30 * dr * iR7 * (1.0 / 24) + 315 * dr * dr * iR9 * (1.0 / 180) - 45 * dd * iR7 * (1.0 / 180));
+        gd[k] = s * (cd * Q + pdr * r + (2 * pdd) * d);
+    }
+    w = w + (iR3 + cr) * r;
+    wv = s * w;
+    const double phi = dot(Q, wv);
+    const V3<double> g_r = s * ((iR3 + cr) * Q + gr + rr * r);
+    const V3<double> base = (-1.0 / 3) * ((gd[0] + gd[1]) + gd[2] + g_r);
+    const double pa = 0.5 * phi / A;
+    g[0] = gd[0] + base + pa * cross(n, XS[2] - XS[1]);
+    g[1] = gd[1] + base + pa * cross(XS[2] - XS[0], n);
+    g[2] = gd[2] + base + pa * cross(n, XS[1] - XS[0]);
+}
+
+// phi = Q.w(x; S) (w = source_velocity, source_triangle.jl:134-138, with the forward's switch) and d phi/d(vertex k of
+// S) for k = 0, 1, 2; the value w(x) in wv.  The multipole by its reverse derivative, the closed form by dual numbers.
+__device__ void tri_point_grad(const V3<double> *XS, const Elem &S, V3<double> x, V3<double> Q, int side,
+                               double kernel_far, V3<double> &wv, V3<double> *g)
+{
+    const V3<double> r0 = cls_sub(x, ld3(S.c));
+    const double fd = kernel_far * S.diam;
+    if (cls_dot(r0, r0) > fd * fd) {
+        far_point_grad(XS, S, x, Q, wv, g);
+        return;
+    }
+#pragma unroll 1
+    for (int k = 0; k < 3; k++) {
+        V3<D3> v[3];
+        for (int j = 0; j < 3; j++) v[j] = dseed(XS[j], j == k ? 1.0 : 0.0);
+        const V3<D3> w = sv_exact_vertices(v, x, side, &S, S.diam);
+        const D3 phi = w.x * Q.x + w.y * Q.y + w.z * Q.z;
+        g[k] = mk3(phi.d[0], phi.d[1], phi.d[2]);
+        if (k == 0) wv = mk3(w.x.v, w.y.v, w.z.v);
+    }
+}
+
+// phi = n.(mu_a va + mu_b vb) of the filament from fa to fb (segment_velocity, src/kernels/segment.jl:15-25) and its
+// derivatives with respect to fa and fb; the value mu_a va + mu_b vb in wv.
+__device__ void seg_point_grad(V3<double> fa, V3<double> fb, V3<double> x, V3<double> n, double mua, double mub,
+                               V3<double> &wv, V3<double> *g)
+{
+#pragma unroll 1
+    for (int k = 0; k < 2; k++) {
+        const V3<D3> a = dseed(fa, k == 0 ? 1.0 : 0.0), b = dseed(fb, k == 1 ? 1.0 : 0.0), xd = dconst(x);
+        V3<D3> va, vb;
+        seg_vel(xd - a, xd - b, va, vb);
+        const V3<D3> w = D3(mua) * va + D3(mub) * vb;
+        const D3 phi = w.x * n.x + w.y * n.y + w.z * n.z;
+        g[k] = mk3(phi.d[0], phi.d[1], phi.d[2]);
+        if (k == 0) wv = mk3(w.x.v, w.y.v, w.z.v);
+    }
+}
+
+// ------------------------------------------------------------------------------------------ triangle adjoints ------
+
+// g[k] += gradient of p.N (N = (X1 - X0) x (X2 - X0), twice the area times the normal) with respect to vertex k
+__device__ __forceinline__ void cross_adj(const V3<double> *X, V3<double> p, V3<double> *g)
+{
+    g[0] = g[0] + cross(p, X[2] - X[1]);
+    g[1] = g[1] + cross(X[2] - X[0], p);
+    g[2] = g[2] + cross(p, X[1] - X[0]);
+}
+
+// g[k] += gradient of c.n (n the unit normal) with respect to vertex k
+__device__ __forceinline__ void normal_adj(const V3<double> *X, V3<double> c, V3<double> *g)
+{
+    const V3<double> N = cross(X[1] - X[0], X[2] - X[0]);
+    const double A2 = dsqrt(dot(N, N));
+    const V3<double> n = (1.0 / A2) * N;
+    cross_adj(X, (1.0 / A2) * (c - dot(c, n) * n), g);
+}
+
+// g[k] += s times the gradient of |N| (twice the area) with respect to vertex k
+__device__ __forceinline__ void area2_adj(const V3<double> *X, double s, V3<double> *g)
+{
+    const V3<double> N = cross(X[1] - X[0], X[2] - X[0]);
+    cross_adj(X, (s / dsqrt(dot(N, N))) * N, g);
+}
+
+// The sheet Gamma = sum_b mu_b gamma_b of a triangle (gamma_b = -e_b/(2A), src/elements/flat_triangle.jl:23-31)
+__device__ __forceinline__ V3<double> sheet_of(const V3<double> *X, const double *mu)
+{
+    const V3<double> N = cross(X[1] - X[0], X[2] - X[0]);
+    const V3<double> E = mu[0] * (X[2] - X[1]) + mu[1] * (X[0] - X[2]) + mu[2] * (X[1] - X[0]);
+    return (-1.0 / dsqrt(dot(N, N))) * E;
+}
+
+// g[k] += gradient of d.Gamma with respect to vertex k
+__device__ __forceinline__ void sheet_adj(const V3<double> *X, const double *mu, V3<double> d, V3<double> *g)
+{
+    const V3<double> N = cross(X[1] - X[0], X[2] - X[0]);
+    const double A2 = dsqrt(dot(N, N));
+    const V3<double> E = mu[0] * (X[2] - X[1]) + mu[1] * (X[0] - X[2]) + mu[2] * (X[1] - X[0]);
+    g[0] = g[0] - ((mu[1] - mu[2]) / A2) * d;
+    g[1] = g[1] - ((mu[2] - mu[0]) / A2) * d;
+    g[2] = g[2] - ((mu[0] - mu[1]) / A2) * d;
+    area2_adj(X, dot(d, E) / (A2 * A2), g);
+}
+
+// A pair's accumulated sums -> the gradient of P_ts with respect to T's and S's vertices (see the file head).
+struct PairSums {
+    V3<double> Sv;     // sum_q a_q w_q
+    V3<double> GS[3];  // sum_q a_q d(Q.w)/d(vertex k of S)
+    V3<double> GX[3];  // sum_q a_q l_qk d(Q.w)/dx
+};
+
+__device__ void finish_tri_pair(const V3<double> *XT, V3<double> nT, double AT, const V3<double> *XS, const double *muS,
+                                double sigS, V3<double> Gam, bool body, V3<double> Vinf, V3<double> Q,
+                                const PairSums &p, V3<double> *gT, V3<double> *gS)
+{
+    for (int k = 0; k < 3; k++) {
+        gT[k] = (-AT) * p.GX[k];
+        gS[k] = (-AT) * p.GS[k];
+    }
+    area2_adj(XT, -0.5 * dot(Q, p.Sv), gT);
+    normal_adj(XT, (-AT) * (sigS * p.Sv + cross(Gam, p.Sv)), gT);
+    if (body) normal_adj(XS, (AT * dot(nT, p.Sv)) * Vinf, gS);  // sigma_S = -V∞.n_S
+    sheet_adj(XS, muS, (-AT) * cross(p.Sv, nT), gS);
+}
+
+__device__ __forceinline__ void gadd(double *G, int node, V3<double> v)
+{
+    unsafeAtomicAdd(&G[3 * node], v.x);
+    unsafeAtomicAdd(&G[3 * node + 1], v.y);
+    unsafeAtomicAdd(&G[3 * node + 2], v.z);
+}
+
+__device__ __forceinline__ int col_node(int col, int nslot, int nn, const int *slot_node)
+{
+    return col < nslot ? slot_node[col] : nn + (col - nslot);
+}
+
+__device__ __forceinline__ double wsum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// What the evaluation kernels need in adjoint mode (null lam in the forward assembly)
+struct AdjArgs {
+    const double *lam;  // adjoint weights per slot
+    const double *mu;   // doublet values per column (slots, then wake nodes)
+    double *G;          // gradient per node (body nodes, then wake nodes), 3 each
+    const int *nodes;   // gradient index of every element corner (body nodes, wake nodes after them)
+    double vx, vy, vz;  // V∞ (sigma = -V∞.n)
+};
+
+// A (test triangle, source triangle) pair in adjoint mode: the contraction Q, the weights and the running sums
+struct AdjPair {
+    V3<double> XS[3], Gam, Q;
+    double la[3], muS[3], sigS;
+    PairSums p;
+};
+
+__device__ __forceinline__ void adj_pair_begin(const AdjArgs &aa, int nt, const Elem &S, int s, const int *ct,
+                                               const int *cs, V3<double> nT, AdjPair &ap)
+{
+    for (int k = 0; k < 3; k++) {
+        ap.XS[k] = ld3(S.v[k]);
+        ap.muS[k] = aa.mu[cs[k]];
+        ap.la[k] = aa.lam[ct[k]];
+    }
+    ap.Gam = sheet_of(ap.XS, ap.muS);
+    ap.sigS = s < nt ? -dot(mk3(aa.vx, aa.vy, aa.vz), ld3(S.n)) : 0.0;
+    ap.Q = ap.sigS * nT + cross(nT, ap.Gam);
+    ap.p.Sv = mk3(0.0, 0.0, 0.0);
+    for (int k = 0; k < 3; k++) ap.p.GS[k] = ap.p.GX[k] = ap.p.Sv;
+}
+
+// One rule point (barycentric l on the test triangle, weight W0 = rule weight x piece area fraction)
+__device__ __forceinline__ void adj_point(AdjPair &ap, const V3<double> *XT, const Elem &S, int side, double kernel_far,
+                                          double W0, const double *l, int test)
+{
+    double psi[3];
+    test_weights(test, l, psi);
+    const double a = W0 * ((ap.la[0] * psi[0] + ap.la[1] * psi[1]) + ap.la[2] * psi[2]);
+    if (a == 0.0) return;
+    V3<double> wv, g[3];
+    tri_point_grad(ap.XS, S, cls_point(XT, l), ap.Q, side, kernel_far, wv, g);
+    const V3<double> gx = -1.0 * ((g[0] + g[1]) + g[2]);
+    ap.p.Sv = ap.p.Sv + a * wv;
+    for (int k = 0; k < 3; k++) {
+        ap.p.GS[k] = ap.p.GS[k] + a * g[k];
+        ap.p.GX[k] = ap.p.GX[k] + (a * l[k]) * gx;
+    }
+}
+
+__device__ __forceinline__ void adj_wave_reduce(PairSums &p)
+{
+    p.Sv = mk3(wsum(p.Sv.x), wsum(p.Sv.y), wsum(p.Sv.z));
+    for (int k = 0; k < 3; k++) {
+        p.GS[k] = mk3(wsum(p.GS[k].x), wsum(p.GS[k].y), wsum(p.GS[k].z));
+        p.GX[k] = mk3(wsum(p.GX[k].x), wsum(p.GX[k].y), wsum(p.GX[k].z));
+    }
+}
+
+__device__ __forceinline__ void adj_pair_end(const AdjArgs &aa, int nt, const Elem &T, const V3<double> *XT, int t, int s,
+                                             const int *nodes, const AdjPair &ap)
+{
+    V3<double> gT[3], gS[3];
+    finish_tri_pair(XT, ld3(T.n), T.area, ap.XS, ap.muS, ap.sigS, ap.Gam, s < nt, mk3(aa.vx, aa.vy, aa.vz), ap.Q, ap.p,
+                    gT, gS);
+    for (int k = 0; k < 3; k++) {
+        gadd(aa.G, nodes[3 * t + k], gT[k]);
+        gadd(aa.G, nodes[3 * s + k], gS[k]);
+    }
+}
+
+
+}  // namespace vp

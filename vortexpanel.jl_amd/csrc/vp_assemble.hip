@@ -162,6 +162,12 @@ __global__ void test_setup(const Elem *__restrict__ el, int nt, RuleTab rt, int 
     out[t] = F;
 }
 
+}  // namespace vp
+
+#include "vp_adjoint_dev.hpp"
+
+namespace vp {
+
 // ------------------------------------------------------------------------------------------ far pairs --------------
 
 // contact_rule's early out, src/assembly/assemble.jl:200-201: a pair can only be a contact pair when its centroids
@@ -969,11 +975,13 @@ __device__ __forceinline__ double wave_sum2(double v)
     return v;
 }
 
+template <bool ADJ>
 __global__ __launch_bounds__(256) void touch_simple(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                     const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                     const int *__restrict__ src_cols, const int4 *__restrict__ list,
                                                     unsigned int npairs, double *__restrict__ A, double *__restrict__ bvec,
-                                                    unsigned long long *__restrict__ stats, uint8_t *__restrict__ tiers)
+                                                    unsigned long long *__restrict__ stats, uint8_t *__restrict__ tiers,
+                                                    AdjArgs aa)
 {
     __shared__ double wedge[4][MAXWEDGE][10];
     __shared__ unsigned long long wst[12];
@@ -1034,6 +1042,22 @@ __global__ __launch_bounds__(256) void touch_simple(AsmParams P, RuleTab rt, con
         const int side = s == t ? 1 : 0;
         const V3<double> tn = ld3(T.n), sc = ld3(S.c);
         const double fd = P.kernel_far * S.diam;
+        if constexpr (ADJ) {  // backward assembly: the same pieces, the pair's gradient sums
+            AdjPair ap;
+            adj_pair_begin(aa, P.nt, S, s, corner_slot + 3 * t, src_cols + 3 * s, tn, ap);
+#pragma unroll 1
+            for (int pp = lane; pp < ntot; pp += WAVE) {
+                const int pc = pp / nr, q = pp - pc * nr;
+                const double *bl = wedge[wave][pc];
+                const double *m = rt.l + 3 * (off + q);
+                double l[3];
+                for (int j = 0; j < 3; j++) l[j] = (m[0] * bl[j] + m[1] * bl[3 + j]) + m[2] * bl[6 + j];
+                adj_point(ap, X, S, side, P.kernel_far, rt.w[off + q] * bl[9], l, P.test);
+            }
+            adj_wave_reduce(ap.p);
+            if (lane == 0) adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap);
+            continue;
+        }
         double acc[12];
         for (int q = 0; q < 12; q++) acc[q] = 0.0;
 #pragma unroll 1
@@ -1092,7 +1116,7 @@ __global__ __launch_bounds__(256) void touch_simple(AsmParams P, RuleTab rt, con
         }
     }
     __syncthreads();
-    if (threadIdx.x < 12 && wst[threadIdx.x]) atomicAdd(&stats[threadIdx.x], wst[threadIdx.x]);
+    if (!ADJ && threadIdx.x < 12 && wst[threadIdx.x]) atomicAdd(&stats[threadIdx.x], wst[threadIdx.x]);
 }
 
 // One point of a plain near pair's rule: accumulates W psi_a (w x n_T) (acc[3a..3a+2]) and W psi_a (w . n_T)
@@ -1148,11 +1172,13 @@ __device__ __forceinline__ void near_scatter(const AsmParams &P, int t, int s, c
 
 // Plain near pairs whose near rule is the single root piece (near_rule, assemble.jl:69-73): one thread per pair,
 // the 7-point rule on the whole test triangle.
+template <bool ADJ>
 __global__ __launch_bounds__(256) void near_thread(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                    const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                    const int *__restrict__ src_cols, const int2 *__restrict__ list,
                                                    unsigned int npairs, double *__restrict__ A, double *__restrict__ bvec,
-                                                   unsigned long long *__restrict__ stats, uint8_t *__restrict__ tiers)
+                                                   unsigned long long *__restrict__ stats, uint8_t *__restrict__ tiers,
+                                                   AdjArgs aa)
 {
     __shared__ unsigned long long wst;
     if (threadIdx.x == 0) wst = 0;
@@ -1166,6 +1192,14 @@ __global__ __launch_bounds__(256) void near_thread(AsmParams P, RuleTab rt, cons
         const V3<double> X[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
         const V3<double> SX[3] = {ld3(S.v[0]), ld3(S.v[1]), ld3(S.v[2])};
         const double fd = P.kernel_far * S.diam;
+        if constexpr (ADJ) {
+            AdjPair ap;
+            adj_pair_begin(aa, P.nt, S, s, corner_slot + 3 * t, src_cols + 3 * s, ld3(T.n), ap);
+#pragma unroll 1
+            for (int q = 0; q < nr; q++) adj_point(ap, X, S, 0, P.kernel_far, rt.w[off + q], rt.l + 3 * (off + q), P.test);
+            adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap);
+            continue;
+        }
         double acc[12];
         for (int i = 0; i < 12; i++) acc[i] = 0.0;
 #pragma unroll 1
@@ -1177,7 +1211,7 @@ __global__ __launch_bounds__(256) void near_thread(AsmParams P, RuleTab rt, cons
     }
     if (mypairs) atomicAdd(&wst, mypairs);
     __syncthreads();
-    if (threadIdx.x == 0 && wst) {
+    if (!ADJ && threadIdx.x == 0 && wst) {
         atomicAdd(&stats[T_NEAR], wst);
         atomicAdd(&stats[6 + T_NEAR], wst * (unsigned long long)nr);
     }
@@ -1223,11 +1257,13 @@ __device__ __forceinline__ void node_of(int nid, int &L, int &idx)  // level-ord
 // full quadtree at once across its lanes (each test depends on that node's own corners only, so this is the
 // recursive near_pieces! exactly), collects the leaves -- nodes whose ancestors all split and that do not split or
 // reach the depth -- by ballot, and evaluates their 7-point rules flattened over the lanes.
+template <bool ADJ>
 __global__ __launch_bounds__(256) void near_group(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                   const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                   const int *__restrict__ src_cols, const int2 *__restrict__ list,
                                                   unsigned int npairs, double *__restrict__ A, double *__restrict__ bvec,
-                                                  unsigned long long *__restrict__ stats, uint8_t *__restrict__ tiers)
+                                                  unsigned long long *__restrict__ stats, uint8_t *__restrict__ tiers,
+                                                  AdjArgs aa)
 {
     constexpr int MAXINT = (1 << (2 * NG_MAXDEPTH)) / 3 + 1;  // internal nodes of a depth-5 tree (341)
     constexpr int MAXLEAF = 1 << (2 * NG_MAXDEPTH);           // 1024
@@ -1294,6 +1330,27 @@ __global__ __launch_bounds__(256) void near_group(AsmParams P, RuleTab rt, const
         }
         // 3. the leaves' points
         const double fd = P.kernel_far * S.diam;
+        if constexpr (ADJ) {
+            AdjPair ap;
+            adj_pair_begin(aa, P.nt, S, s, corner_slot + 3 * t, src_cols + 3 * s, ld3(T.n), ap);
+#pragma unroll 1
+            for (int pp = lane; pp < nl * nr; pp += WAVE) {
+                const int li = pp / nr, q = pp - li * nr;
+                int L, idx;
+                node_of(leaf[wave][li], L, idx);
+                double b[3][3];
+                node_corners(L, idx, b);
+                const double *m = rt.l + 3 * (off + q);
+                double l[3];
+                for (int j = 0; j < 3; j++) l[j] = (m[0] * b[0][j] + m[1] * b[1][j]) + m[2] * b[2][j];
+                double frac = 1.0;
+                for (int i = 0; i < L; i++) frac /= 4;
+                adj_point(ap, X, S, 0, P.kernel_far, rt.w[off + q] * frac, l, P.test);
+            }
+            adj_wave_reduce(ap.p);
+            if (lane == 0) adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap);
+            continue;
+        }
         double acc[12];
         for (int i = 0; i < 12; i++) acc[i] = 0.0;
 #pragma unroll 1
@@ -1324,7 +1381,7 @@ __global__ __launch_bounds__(256) void near_group(AsmParams P, RuleTab rt, const
         atomicAdd(&wst[1], mypts);
     }
     __syncthreads();
-    if (threadIdx.x == 0 && wst[0]) {
+    if (!ADJ && threadIdx.x == 0 && wst[0]) {
         atomicAdd(&stats[T_NEAR], wst[0]);
         atomicAdd(&stats[6 + T_NEAR], wst[1]);
     }
@@ -1457,8 +1514,6 @@ Plan::~Plan()
     for (int k = 0; k < 6; k++)
         if (ev[k]) (void)hipEventDestroy(ev[k]);
     for (void *p : owned) (void)hipFree(p);
-    if (d_aleaves) (void)hipFree(d_aleaves);
-    if (d_agroups) (void)hipFree(d_agroups);
     if (A && ownsA) (void)hipFree(A);
 }
 
@@ -1725,18 +1780,18 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
         }
         if (timing) HIPCHECK(hipEventRecord(ev[2], st));
         if (nsimple) {
-            touch_simple<<<std::min<unsigned int>((nsimple + 3) / 4, 65535u), 256, 0, st>>>(
-                P, rt, d_el, d_sigma, d_cs, d_cols, d_simple, nsimple, A, d_b, d_stats, d_tiers);
+            touch_simple<false><<<std::min<unsigned int>((nsimple + 3) / 4, 65535u), 256, 0, st>>>(
+                P, rt, d_el, d_sigma, d_cs, d_cols, d_simple, nsimple, A, d_b, d_stats, d_tiers, AdjArgs{});
             HIPCHECK(hipGetLastError());
         }
         if (cnt[2]) {
-            near_thread<<<std::min<unsigned int>((cnt[2] + 255) / 256, 65535u), 256, 0, st>>>(
-                P, rt, d_el, d_sigma, d_cs, d_cols, d_near, cnt[2], A, d_b, d_stats, d_tiers);
+            near_thread<false><<<std::min<unsigned int>((cnt[2] + 255) / 256, 65535u), 256, 0, st>>>(
+                P, rt, d_el, d_sigma, d_cs, d_cols, d_near, cnt[2], A, d_b, d_stats, d_tiers, AdjArgs{});
             HIPCHECK(hipGetLastError());
         }
         if (cnt[3]) {
-            near_group<<<std::min<unsigned int>((cnt[3] + 3) / 4, 65535u), 256, 0, st>>>(
-                P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, cnt[3], A, d_b, d_stats, d_tiers);
+            near_group<false><<<std::min<unsigned int>((cnt[3] + 3) / 4, 65535u), 256, 0, st>>>(
+                P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, cnt[3], A, d_b, d_stats, d_tiers, AdjArgs{});
             HIPCHECK(hipGetLastError());
         }
         if (timing) HIPCHECK(hipEventRecord(ev[3], st));

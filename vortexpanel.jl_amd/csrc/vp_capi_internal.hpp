@@ -62,10 +62,6 @@ struct Plan {
     // backward assembly (adjoint) buffers, made on first use
     double *d_lam = nullptr, *d_mu = nullptr, *d_G = nullptr;
     int *d_slotnode = nullptr;
-    unsigned long long *d_astats = nullptr;
-    unsigned int *d_acount = nullptr, aleaf_cap = 0, agroup_cap = 0;
-    struct Leaf *d_aleaves = nullptr;
-    int4 *d_agroups = nullptr;
     int2 *d_nonfar = nullptr;
     struct Leaf *d_leaves = nullptr;
 

@@ -159,6 +159,7 @@ class WingAdjoint:
         self.ctx.sync()
         self.timings["forward_s"] = time.perf_counter() - t0
         self.q, self.mu = sol["q"], sol["mu"]
+        self._bd = None
         self.mu_w = dm.K @ self.q                   # trailing edge doublet jumps
         self.mu_wn = np.asarray(self.W @ self.q)    # wake node values (columns after the slots)
 
@@ -189,13 +190,15 @@ class WingAdjoint:
         return gx
 
     def _db_dalpha(self):
-        """db/dalpha at fixed geometry and wake (per radian): b is linear in V∞ (sources and freestream term)"""
-        a = np.radians(self.alpha)
-        Vd = np.array([-np.sin(a), 0.0, np.cos(a)])
-        self.plan.assemble(source_strengths(build_elements(self.g), Vd), Vd)
-        bd = self.plan.read_b()
-        self.plan.assemble(source_strengths(build_elements(self.g), self.V), self.V)  # restore the forward state
-        return bd
+        """db/dalpha at fixed geometry and wake (per radian), made once: b is linear in V∞ (sources and freestream
+        term), so it is the b of dV∞/dalpha.  The re-assembly leaves the plan's rule lists as they were (they do not
+        depend on V∞), which the backward assembly reads."""
+        if self._bd is None:
+            a = np.radians(self.alpha)
+            Vd = np.array([-np.sin(a), 0.0, np.cos(a)])
+            self.plan.assemble(source_strengths(build_elements(self.g), Vd), Vd)
+            self._bd = self.plan.read_b()
+        return self._bd
 
     def gradient(self, objective="CL"):
         """dJ/dx (nn, 3) and dJ/dalpha (per degree) of J = CL or CDi"""
