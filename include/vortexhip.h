@@ -89,8 +89,10 @@ int vp_assemble(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const dou
 int vp_plan_create(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const vp_assembly_options *opts,
                    vp_plan **out);
 /* flags: VP_PLAN_RECORD_TIERS keeps the tier of every pair of the last assembly (see vp_plan_tiers);
- * VP_PLAN_TIMING records per-kernel HIP events (see vp_plan_timings). */
-enum { VP_PLAN_RECORD_TIERS = 1, VP_PLAN_TIMING = 2 };
+ * VP_PLAN_TIMING records per-kernel HIP events (see vp_plan_timings); VP_PLAN_WAKE_ONLY assembles only the pairs of
+ * body test triangles with the wake's sources (the wake columns of A, no body columns, no freestream in b), for the
+ * known vortex elements of explicit_rhs. */
+enum { VP_PLAN_RECORD_TIERS = 1, VP_PLAN_TIMING = 2, VP_PLAN_WAKE_ONLY = 4 };
 int vp_plan_create_ex(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const vp_assembly_options *opts,
                       int flags, vp_plan **out);
 int vp_plan_assemble(vp_plan *plan, const double *sigma, const double *vinf, long long *pairs, long long *points);
@@ -110,6 +112,13 @@ int vp_plan_timings(vp_plan *plan, double *ms);
  * vp_plan_assemble and sigma = -Vinf.n (source_strengths, assemble.jl:364).  grad: (nn + nw) x 3, body nodes then
  * wake nodes. */
 int vp_plan_adjoint(vp_plan *plan, const double *lambda, const double *mu, const double *vinf, double *grad);
+/* `explicit_rhs` (src/assembly/explicit.jl:31-86) of known sheets and filaments: out = A_wake mu_wake (nslot values;
+ * the right hand side is its negative), the wake columns of the last vp_plan_assemble times the doublets of the wake's
+ * nodes (a wake-only plan whose wake holds the known sheets and filaments, one node per corner or filament end). */
+int vp_plan_wake_product(vp_plan *plan, const double *mu_wake, double *out);
+/* `explicit_rhs` of nb vortex particles (positions bx and strengths alpha nb x 3, cores delta): their right hand side
+ * (nslot values, added to b) on the plan's body: the far rule beyond far_ratio max(diam, core), else near_rule. */
+int vp_plan_blob_rhs(vp_plan *plan, int nb, const double *bx, const double *alpha, const double *delta, double *out);
 /* Device pointers of the plan's A (column-major, leading dimension nslot) and b, for chaining device calls. */
 int vp_plan_device_A(vp_plan *plan, double **A, double **b, long long *lda);
 void vp_plan_destroy(vp_plan *plan);

@@ -1,0 +1,72 @@
+"""The free wake on the GPU against the same host logic on the FP64 oracle: explicit_rhs of sheets, filaments and
+particles (near and far) within 1e-12 of the largest entry, and the simulation step by step (trailing edge mu_w,
+Kutta-Joukowski force, particle positions) within 1e-10 in FP64 over steps with conversions; the mixed path's mu_w
+within 1e-6."""
+import numpy as np
+import pytest
+
+from freewake_oracle import OracleFreeWakeSim, oracle_explicit_rhs
+from vphip import api, dofmap, freewake as FW, mesh, sweep, wake as wk
+
+pytestmark = pytest.mark.gpu
+
+V, LIFT = sweep.wing_freestream(8.0)
+
+
+@pytest.fixture(scope="module")
+def wing():
+    g = mesh.naca_wing(mesh.WingSpec(code="0012", n_around=16, n_span=8, aspect=6.0))
+    return g, dofmap.build_dofmap(g)
+
+
+def test_explicit_rhs(wing):
+    print("explicit_rhs sheets+filaments+particles", flush=True)
+    g, dm = wing
+    rng = np.random.default_rng(4)
+    w = wk.build_wake(g, dm, V, wk.WakeOptions(length=0.5))
+    e = FW.rigid_elements(w, rng.standard_normal(dm.wake_te.n))
+    # particles: a cloud behind the wing and a few right next to it (near rules)
+    nb = 700
+    bx = np.vstack([np.array([1.5, 0, 0]) + rng.standard_normal((nb - 20, 3)) * [0.5, 2.0, 0.3],
+                    g.x[rng.integers(0, len(g.x), 20)] + rng.standard_normal((20, 3)) * 0.02])
+    blobs = FW.Blobs(bx, rng.standard_normal((nb, 3)) * 0.01, rng.uniform(0.02, 0.08, nb))
+    got = FW.explicit_rhs(g, dm, e, blobs)
+    ref = oracle_explicit_rhs(g, dm, e, blobs, api.AssemblyOptions())
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max(), np.abs(got - ref).max() / np.abs(ref).max()
+    only = FW.explicit_rhs(g, dm, None, blobs)
+    assert np.abs(only - oracle_explicit_rhs(g, dm, None, blobs, api.AssemblyOptions())).max() < 1e-12 * np.abs(only).max()
+
+
+def test_free_wake_steps_match_oracle(wing):
+    print("free wake 8 steps fp64 vs oracle", flush=True)
+    g, dm = wing
+    dt = wk.te_panel_size(g) * 2
+    o = FW.FreeWakeOptions(dt=dt, max_rows=3)
+    sim = FW.FreeWakeSim(g, dm, V, o)
+    ref = OracleFreeWakeSim(g, dm, V, o)
+    for k in range(8):
+        a, b = sim.step(), ref.step()
+        sc = np.abs(b["mu_w"]).max()
+        assert np.abs(a["mu_w"] - b["mu_w"]).max() < 1e-10 * sc, (k, np.abs(a["mu_w"] - b["mu_w"]).max() / sc)
+        assert np.abs(a["F_kj"] - b["F_kj"]).max() < 1e-10 * np.abs(b["F_kj"]).max()
+        assert a["particles"] == b["particles"] and a["rows"] == b["rows"]
+    assert sim.state.conversions > 0
+    dx = np.abs(sim.state.particles.x - ref.state.particles.x).max()
+    assert dx < 1e-10 * np.abs(ref.state.particles.x).max(), dx
+    for r1, r2 in zip(sim.state.x, ref.state.x):
+        assert np.abs(r1 - r2).max() < 1e-10 * np.abs(r2).max()
+    sim.close()
+
+
+def test_free_wake_mixed(wing):
+    print("free wake 6 steps mixed vs fp64", flush=True)
+    g, dm = wing
+    dt = wk.te_panel_size(g) * 2
+    o = FW.FreeWakeOptions(dt=dt, max_rows=3)
+    runs = {}
+    for p in ("fp64", "mixed"):
+        s = FW.FreeWakeSim(g, dm, V, o, precision=p)
+        runs[p] = s.run(6)
+        s.close()
+    for a, b in zip(runs["mixed"], runs["fp64"]):
+        assert np.abs(a["mu_w"] - b["mu_w"]).max() < 1e-6 * np.abs(b["mu_w"]).max()

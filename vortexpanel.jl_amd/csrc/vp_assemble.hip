@@ -254,12 +254,13 @@ __global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__r
                                                  const int *__restrict__ ts_ncol, double *__restrict__ A,
                                                  double *__restrict__ bvec, int2 *__restrict__ near_list,
                                                  unsigned int *__restrict__ near_count, unsigned int near_cap,
-                                                 unsigned long long *__restrict__ far_count, uint8_t *__restrict__ tiers)
+                                                 unsigned long long *__restrict__ far_count, uint8_t *__restrict__ tiers,
+                                                 int st0)
 {
     __shared__ double R[VMAX * UMAX];
     __shared__ double rbacc[VMAX];
     __shared__ unsigned int nfar, nexw;
-    const int tt = blockIdx.x, st = blockIdx.y;
+    const int tt = blockIdx.x, st = blockIdx.y + st0;
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < VMAX * UMAX; i += blockDim.x) R[i] = 0.0;
     if (threadIdx.x < VMAX) rbacc[threadIdx.x] = 0.0;
@@ -1514,6 +1515,7 @@ Plan::~Plan()
     for (int k = 0; k < 6; k++)
         if (ev[k]) (void)hipEventDestroy(ev[k]);
     for (void *p : owned) (void)hipFree(p);
+    if (d_bnear) (void)hipFree(d_bnear);
     if (A && ownsA) (void)hipFree(A);
 }
 
@@ -1639,7 +1641,7 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
         for (int k = 0; k < 3; k++) cent[3 * e2 + k] = hel[e2].c[k];
     std::vector<double> tcent(cent.begin(), cent.begin() + 3 * nt);
     std::vector<int> torder = space_order(tcent, nt);
-    std::vector<int> sorder_b = space_order(tcent, nt), sorder;
+    std::vector<int> sorder_b = torder, sorder;
     sorder = sorder_b;
     if (nwt) {
         std::vector<double> wc(cent.begin() + 3 * nt, cent.end());
@@ -1650,7 +1652,13 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
     std::vector<int> tte, ttr, ttn, tse, tsc, tsn;
     std::vector<uint8_t> ttl, tsl;
     build_tiles(torder, tcols, TT_MAX, VMAX, tte, ttl, ttr, ttn, VMAX);
-    build_tiles(sorder, cols, TS, UMAX, tse, tsl, tsc, tsn, UMAX);
+    // source tiles: the body's, then the wake's (a wake-only plan runs the far tier over the wake's tiles alone)
+    build_tiles(sorder_b, cols, TS, UMAX, tse, tsl, tsc, tsn, UMAX);
+    nts_body = (int)tsn.size();
+    if (nwt) {
+        std::vector<int> wo(sorder.begin() + nt, sorder.end());
+        build_tiles(wo, cols, TS, UMAX, tse, tsl, tsc, tsn, UMAX);
+    }
     ntt = (int)ttn.size();
     nts = (int)tsn.size();
     d_tte = dupload(tte, st);
@@ -1724,15 +1732,19 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
         HIPCHECK(hipMemsetAsync(d_count, 0, sizeof(unsigned int) * 7, st));
         if (d_tiers) HIPCHECK(hipMemsetAsync(d_tiers, 255, (size_t)nt * nsrc, st));
         const AsmParams P = params();
-        dim3 grid(ntt, nts);
+        const int st0 = wake_only ? nts_body : 0;
+        dim3 grid(ntt, nts - st0);
         if (timing) HIPCHECK(hipEventRecord(ev[0], st));
-        if (opts.precision == 1)
-            far_pairs<float><<<grid, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc, d_tsn,
-                                                   A, d_b, d_nonfar, d_count, nonfar_cap, d_stats + 5, d_tiers);
-        else
-            far_pairs<double><<<grid, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc,
-                                                    d_tsn, A, d_b, d_nonfar, d_count, nonfar_cap, d_stats + 5, d_tiers);
-        HIPCHECK(hipGetLastError());
+        if (nts - st0 > 0) {
+            if (opts.precision == 1)
+                far_pairs<float><<<grid, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc,
+                                                       d_tsn, A, d_b, d_nonfar, d_count, nonfar_cap, d_stats + 5, d_tiers, st0);
+            else
+                far_pairs<double><<<grid, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl,
+                                                        d_tsc, d_tsn, A, d_b, d_nonfar, d_count, nonfar_cap, d_stats + 5,
+                                                        d_tiers, st0);
+            HIPCHECK(hipGetLastError());
+        }
         if (nseg) {
             long long np = (long long)nt * nseg;
             seg_pairs<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(P, d_tf, d_cs, d_seg, d_segcol, A, d_nonfar, d_count,
@@ -1817,8 +1829,10 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
             HIPCHECK(hipGetLastError());
         }
         if (timing) HIPCHECK(hipEventRecord(ev[4], st));
-        freestream<<<(nt + 255) / 256, 256, 0, st>>>(d_el, d_cs, nt, opts.test, vinf[0], vinf[1], vinf[2], d_b);
-        HIPCHECK(hipGetLastError());
+        if (!wake_only) {
+            freestream<<<(nt + 255) / 256, 256, 0, st>>>(d_el, d_cs, nt, opts.test, vinf[0], vinf[1], vinf[2], d_b);
+            HIPCHECK(hipGetLastError());
+        }
         if (timing) HIPCHECK(hipEventRecord(ev[5], st));
         break;
     }
@@ -1856,3 +1870,4 @@ void Plan::stats(long long *pairs, long long *points)
 }  // namespace vp
 
 #include "vp_adjoint.hpp"
+#include "vp_explicit.hpp"

@@ -95,6 +95,7 @@ int vp_plan_create_ex(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, con
     if (!p) return fail(&ctx->c, VP_E_DEVICE, "host allocation failed");
     p->p.record_tiers = (flags & VP_PLAN_RECORD_TIERS) != 0;
     p->p.timing = (flags & VP_PLAN_TIMING) != 0;
+    p->p.wake_only = (flags & VP_PLAN_WAKE_ONLY) != 0;
     int rc = guard(&ctx->c, [&] {
         (void)hipSetDevice(ctx->c.device);
         p->p.build(&ctx->c, *g, wake && (wake->nw > 0) ? wake : nullptr, *opts);
@@ -169,6 +170,25 @@ int vp_plan_adjoint(vp_plan *plan, const double *lambda, const double *mu, const
     return guard(c, [&] {
         (void)hipSetDevice(c->device);
         plan->p.adjoint(lambda, mu, vinf, grad);
+    });
+}
+
+int vp_plan_wake_product(vp_plan *plan, const double *mu_wake, double *out)
+{
+    if (!plan || !plan->p.ctx) return VP_E_ARGUMENT;
+    vp::Ctx *c = plan->p.ctx;
+    if (!out || (plan->p.nw > 0 && !mu_wake)) return fail(c, VP_E_ARGUMENT, "null argument");
+    return guard(c, [&] { plan->p.wake_product(mu_wake, out); });
+}
+
+int vp_plan_blob_rhs(vp_plan *plan, int nb, const double *bx, const double *alpha, const double *delta, double *out)
+{
+    if (!plan || !plan->p.ctx) return VP_E_ARGUMENT;
+    vp::Ctx *c = plan->p.ctx;
+    if (!out || nb < 0 || (nb > 0 && (!bx || !alpha || !delta))) return fail(c, VP_E_ARGUMENT, "null argument");
+    return guard(c, [&] {
+        (void)hipSetDevice(c->device);
+        plan->p.blob_rhs(nb, bx, alpha, delta, out);
     });
 }
 

@@ -59,6 +59,10 @@ struct Plan {
     unsigned int nnearm = 0, ntouchp = 0, nsimple = 0;
     int4 *d_simple = nullptr;
     bool assembled = false;
+    bool wake_only = false;  // assemble only the body test triangles' pairs with the wake's sources
+    int nts_body = 0;        // source tiles of the body (the wake's follow)
+    int2 *d_bnear = nullptr; // near (test triangle, particle) pairs of blob_rhs
+    unsigned int bnear_cap = 0;
     // backward assembly (adjoint) buffers, made on first use
     double *d_lam = nullptr, *d_mu = nullptr, *d_G = nullptr;
     int *d_slotnode = nullptr;
@@ -71,6 +75,8 @@ struct Plan {
     void stats(long long *pairs, long long *points);
     void kernel_ms(double *ms);
     AsmParams params() const;
+    void wake_product(const double *mu_nodes, double *out);
+    void blob_rhs(int nb, const double *bx, const double *ba, const double *bd, double *out);
     void adjoint(const double *lam, const double *mu, const double *vinf, double *G);
 };
 

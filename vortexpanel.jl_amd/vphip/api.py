@@ -158,7 +158,7 @@ class AssemblyPlan:
     """vp_plan_*: geometry, element records, tiles and rules resident on the device; A and b stay on the device"""
 
     def __init__(self, g, dm, wake=None, opts: AssemblyOptions | None = None, ctx: Context | None = None,
-                 record_tiers: bool = False, timing: bool = False):
+                 record_tiers: bool = False, timing: bool = False, wake_only: bool = False):
         self.ctx = ctx or default_context()
         self.opts = opts or AssemblyOptions()
         oc = self.opts.to_c()
@@ -170,7 +170,8 @@ class AssemblyPlan:
         self.nwt = 0 if wake is None else len(wake.tris)
         h = C.c_void_p()
         rc = lib.vp_plan_create_ex(self.ctx.h, C.byref(s), C.byref(w) if w is not None else None, C.byref(oc),
-                                   (1 if record_tiers else 0) | (2 if timing else 0), C.byref(h))
+                                   (1 if record_tiers else 0) | (2 if timing else 0) | (4 if wake_only else 0),
+                                   C.byref(h))
         _check(self.ctx.h, rc)
         self.h = h
         self.record_tiers = record_tiers
@@ -201,6 +202,23 @@ class AssemblyPlan:
         """Tier index of every (body test triangle, source triangle) pair of the last assembly"""
         out = np.zeros((self.nt, self.nt + self.nwt), np.uint8)
         _check(self.ctx.h, self.ctx.lib.vp_plan_tiers(self.h, L.ptr(out, C.c_uint8)))
+        return out
+
+    def wake_product(self, mu_wake):
+        """A_wake mu_wake (nslot): the wake columns of the last assembly times the wake nodes' doublets"""
+        out = np.zeros(self.nslot)
+        mw = L.f64(mu_wake)
+        if mw.shape != (self.nw,):
+            raise ValueError(f"mu_wake has {mw.shape}, expected ({self.nw},)")
+        _check(self.ctx.h, self.ctx.lib.vp_plan_wake_product(self.h, L.ptr(mw) if self.nw else None, L.ptr(out)))
+        return out
+
+    def blob_rhs(self, bx, alpha, delta):
+        """explicit_rhs of vortex particles on the plan's body (nslot values), src/assembly/explicit.jl:60-79"""
+        bx, ba = L.f64(bx).reshape(-1, 3), L.f64(alpha).reshape(-1, 3)
+        bd = L.f64(delta).reshape(-1)
+        out = np.zeros(self.nslot)
+        _check(self.ctx.h, self.ctx.lib.vp_plan_blob_rhs(self.h, len(bd), L.ptr(bx), L.ptr(ba), L.ptr(bd), L.ptr(out)))
         return out
 
     def read_b(self):

@@ -1,0 +1,582 @@
+"""Free wake: the implicit trailing edge panel factored once with the body, free panel rows shed every step, their
+conversion into vortex particles, forward-Euler advection and the loads of every step.
+
+Mirrors src/freewake/state.jl and src/freewake/step.jl (same names, same argument meaning, MESA's step order).  The
+host keeps the wake's bookkeeping (rows, conversions, merging: small and branchy); the device does the work that
+scales: the right hand side of the known sheets, filaments and particles (`explicit_rhs`, a wake-only assembly plan and
+the particle kernels), the solve with the factored matrix, and the velocities at every free node and particle (the body
+N-body sum and the blob sum).  Positions, strengths and doublets stay FP64; with precision 'mixed' the velocity sums use
+FP32 kernels with FP64 sums (the circulation bookkeeping stays FP64).
+"""
+from __future__ import annotations
+
+import math
+import time
+import warnings
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import post
+from .api import AssemblyOptions, AssemblyPlan, Context, build_elements, default_context, source_strengths
+from .dofmap import wake_rows
+from .solve import factorize_galerkin, solve_factored
+from .wake import te_panel
+
+
+# ------------------------------------------------------------------------------------------ options and cores --------
+
+@dataclass
+class FreeWakeOptions:
+    """src/freewake/state.jl:27-39"""
+    dt: float
+    te_len: float | None = None
+    te_direction: str = "freestream"
+    max_rows: int | str = "auto"
+    merge_rows: int | str = 1
+    core: str | float = "auto"
+    overlap: float = 1.5
+    core_factor: float = 1.5
+    core_floor: float | None = None
+    advection: str = "free"
+    delete_beyond: float = math.inf
+
+    def check(self):
+        """check_options, state.jl:41-51"""
+        if not self.dt > 0:
+            raise ValueError("dt must be positive")
+        if not (self.max_rows == "auto" or (isinstance(self.max_rows, int) and self.max_rows >= 0)):
+            raise ValueError("max_rows must be auto or at least 0")
+        if not (self.merge_rows == "auto" or (isinstance(self.merge_rows, int) and self.merge_rows >= 1)):
+            raise ValueError("merge_rows must be auto or at least 1")
+        if self.advection not in ("free", "freestream"):
+            raise ValueError(f"advection must be free or freestream, not {self.advection}")
+        if self.te_direction not in ("freestream", "bisector"):
+            raise ValueError("te_direction must be freestream or bisector")
+        if not (self.core in ("auto", "scaled") or (isinstance(self.core, (int, float)) and self.core > 0)):
+            raise ValueError(f"core must be auto, scaled or a positive number, not {self.core}")
+        if not (self.overlap > 0 and self.core_factor > 0):
+            raise ValueError("overlap and core_factor must be positive")
+        return self
+
+
+def _norm(v):
+    return math.sqrt(float(np.dot(v, v)))
+
+
+class CoreRule:
+    """Core radius of particles and of the blobs that stand for wake panels and filaments, state.jl:68-92"""
+
+    def __init__(self, o: FreeWakeOptions, vinf):
+        self.floor = float(o.core_floor) if o.core_floor is not None else _norm(vinf) * o.dt
+        self.overlap, self.factor = float(o.overlap), float(o.core_factor)
+        if isinstance(o.core, (int, float)) and not isinstance(o.core, bool):
+            self.mode, self.value = "fixed", float(o.core)
+        else:
+            self.mode, self.value = o.core, 0.0
+
+    def tri(self, p, q, r):
+        if self.mode == "fixed":
+            return self.value
+        if self.mode == "auto":
+            return self.overlap * max(_norm(q - p), _norm(r - q), _norm(p - r)) / 2
+        return max(self.factor * math.sqrt(_norm(np.cross(q - p, r - p))), self.floor)
+
+    def seg(self, a, b):
+        if self.mode == "fixed":
+            return self.value
+        if self.mode == "auto":
+            return self.overlap * _norm(b - a) / 2
+        return max(self.factor * _norm(b - a) / 2, self.floor)
+
+
+@dataclass
+class Topology:
+    """Strips of the free wake (state.jl:99-121): neighbouring wake trailing edge nodes along tau, and the end nodes of
+    open lines (the wake's tips)"""
+    nte: int
+    segments: list
+    first_nodes: list
+    last_nodes: list
+
+    @staticmethod
+    def of(dm):
+        wte = dm.wake_te
+        segs, firsts, lasts = [], [], []
+        for line, closed in zip(wte.lines, wte.closed):
+            line = [int(v) for v in line]
+            m = len(line)
+            segs += [(line[i], line[i + 1]) for i in range(m - 1)]
+            if closed:
+                segs.append((line[m - 1], line[0]))
+            else:
+                firsts.append(line[0])
+                lasts.append(line[m - 1])
+        return Topology(wte.n, segs, firsts, lasts)
+
+
+# ------------------------------------------------------------------------------------------ state and elements ------
+
+@dataclass
+class Blobs:
+    """Vortex particles (src/kernels/blob.jl:7-11): positions, strengths, cores"""
+    x: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    alpha: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    delta: np.ndarray = field(default_factory=lambda: np.zeros(0))
+
+    def __len__(self):
+        return len(self.delta)
+
+    @staticmethod
+    def cat(parts):
+        parts = [p for p in parts if len(p)]
+        if not parts:
+            return Blobs()
+        return Blobs(np.vstack([p.x for p in parts]), np.vstack([p.alpha for p in parts]),
+                     np.concatenate([p.delta for p in parts]))
+
+    def take(self, keep):
+        return Blobs(self.x[keep], self.alpha[keep], self.delta[keep])
+
+
+@dataclass
+class FreeWakeState:
+    """state.jl:123-143: node rows newest (R0) to oldest, their frozen doublets, mu_line, particles and merge data"""
+    x: list
+    mu: list
+    mu_line: np.ndarray
+    particles: Blobs
+    conversions: int
+    tag: np.ndarray
+    group: np.ndarray
+    batch_rows: np.ndarray
+    batch_length: np.ndarray
+    batch_width: np.ndarray
+
+    @staticmethod
+    def empty(nte, nstrips=0):
+        return FreeWakeState([], [], np.zeros(nte), Blobs(), 0, np.zeros(0, np.int64), np.zeros(0, np.int8),
+                             np.zeros(nstrips, np.int64), np.zeros(nstrips), np.zeros(nstrips))
+
+    def keep_particles(self, keep):
+        """keep_particles!, state.jl:146-151"""
+        self.particles = self.particles.take(keep)
+        self.tag = self.tag[keep]
+        self.group = self.group[keep]
+
+
+@dataclass
+class WakeElements:
+    """Known vortex elements (state.jl:157-165): triangles with corner doublets (those with an area also as sheets) and
+    filament segments with their end circulations"""
+    corners: list = field(default_factory=list)
+    corner_mu: list = field(default_factory=list)
+    sheets: list = field(default_factory=list)
+    sheet_mu: list = field(default_factory=list)
+    segments: list = field(default_factory=list)
+    segment_gamma: list = field(default_factory=list)
+
+
+def nondegenerate(p, q, r):
+    """state.jl:171"""
+    return _norm(np.cross(q - p, r - p)) > 1e-12 * max(np.dot(q - p, q - p), np.dot(r - p, r - p), np.dot(r - q, r - q))
+
+
+def add_sheet(e, p, q, r, mp, mq, mr):
+    """add_sheet!, state.jl:174-183"""
+    mu = np.array([mp, mq, mr], dtype=np.float64)
+    e.corners.append((p, q, r))
+    e.corner_mu.append(mu)
+    if nondegenerate(p, q, r):
+        e.sheets.append((p, q, r))
+        e.sheet_mu.append(mu)
+
+
+def add_segment(e, a, b, ga, gb):
+    e.segments.append((a, b))
+    e.segment_gamma.append((float(ga), float(gb)))
+
+
+def panel_row(e, st, topo, i):
+    """panel_row!, state.jl:188-201 (i 0-based: the row between node rows i and i + 1)"""
+    x0, x1, m0, m1 = st.x[i], st.x[i + 1], st.mu[i], st.mu[i + 1]
+    for A, B in topo.segments:
+        add_sheet(e, x0[A], x1[A], x0[B], m0[A], m1[A], m0[B])
+        add_sheet(e, x1[A], x1[B], x0[B], m1[A], m1[B], m0[B])
+    for k in topo.first_nodes:
+        add_segment(e, x1[k], x0[k], m1[k], m0[k])
+    for k in topo.last_nodes:
+        add_segment(e, x0[k], x1[k], m0[k], m1[k])
+
+
+def far_end(e, st, topo, i):
+    """far_end!, state.jl:204-212"""
+    x, mu = st.x[i], st.mu[i]
+    for A, B in topo.segments:
+        gA, gB = mu[A] - st.mu_line[A], mu[B] - st.mu_line[B]
+        if gA == 0 and gB == 0:
+            continue
+        add_segment(e, x[B], x[A], gB, gA)
+
+
+def vortex_elements(st, topo):
+    """vortex_elements(st, topo), state.jl:222-237"""
+    e = WakeElements()
+    nr = len(st.x)
+    for i in range(nr - 1):
+        panel_row(e, st, topo, i)
+    if nr >= 1:
+        for A, B in topo.segments:
+            add_segment(e, st.x[0][A], st.x[0][B], st.mu[0][A], st.mu[0][B])
+        far_end(e, st, topo, nr - 1)
+    return e
+
+
+def rigid_elements(w, mu_w):
+    """vortex_elements(w::RigidWake, mu_w), state.jl:240-250"""
+    e = WakeElements()
+    mn = np.asarray(mu_w)[w.te]
+    for t in w.tris:
+        add_sheet(e, w.x[t[0]], w.x[t[1]], w.x[t[2]], mn[t[0]], mn[t[1]], mn[t[2]])
+    for a, b in w.filaments:
+        add_segment(e, w.x[a], w.x[b], mn[a], mn[b])
+    return e
+
+
+def sheet_vorticity(p, q, r, mu):
+    """state.jl:258: the vorticity of a triangle's sheet without its area"""
+    return ((mu[0] + mu[1]) * (q - p) + (mu[1] + mu[2]) * (r - q) + (mu[2] + mu[0]) * (p - r)) / 2
+
+
+def segment_blobs(a, b, ga, gb, delta):
+    """src/kernels/blob.jl:33-38: two 2-point Gauss blobs carrying a filament's strength and first moment"""
+    s = 1 / (2 * math.sqrt(3))
+    xs, al = [], []
+    for xi in (0.5 - s, 0.5 + s):
+        xs.append(a + xi * (b - a))
+        al.append((((1 - xi) * ga + xi * gb) / 2) * (b - a))
+    return xs, al, [delta, delta]
+
+
+def free_wake_blobs(e, core):
+    """state.jl:268-278: a particle at each triangle's centroid with its sheet's vorticity, two per filament"""
+    xs, al, dl = [], [], []
+    for (p, q, r), mu in zip(e.corners, e.corner_mu):
+        xs.append((p + q + r) / 3)
+        al.append(sheet_vorticity(p, q, r, mu))
+        dl.append(core.tri(p, q, r))
+    for (a, b), (ga, gb) in zip(e.segments, e.segment_gamma):
+        x2, a2, d2 = segment_blobs(a, b, ga, gb, core.seg(a, b))
+        xs += x2
+        al += a2
+        dl += d2
+    if not dl:
+        return Blobs()
+    return Blobs(np.array(xs, dtype=np.float64).reshape(-1, 3), np.array(al, dtype=np.float64).reshape(-1, 3),
+                 np.array(dl, dtype=np.float64))
+
+
+def convert_oldest(st, topo, core, merge=None):
+    """convert_oldest!, state.jl:292-328: the oldest panel row into particles; returns how many"""
+    nr = len(st.x)
+    if nr < 2:
+        return 0
+    e = WakeElements()
+    panel_row(e, st, topo, nr - 2)
+    far_end(e, st, topo, nr - 1)
+    blobs = free_wake_blobs(e, core)
+    tag, group = np.zeros(len(blobs), np.int64), np.zeros(len(blobs), np.int8)
+    if merge is not None:
+        up, down = st.x[nr - 2], st.x[nr - 1]
+        for s, (A, B) in enumerate(topo.segments):
+            k = merge[s]
+            if k <= 1:
+                continue
+            m = k - st.batch_rows[s]
+            for h, (u, v) in enumerate(((1 / 3, 1 / 3), (2 / 3, 2 / 3))):
+                tag[2 * s + h] = s + 1
+                group[2 * s + h] = 1 if u + (m - 1 + v) / k < 1 else 2
+            st.batch_rows[s] += 1
+            st.batch_length[s] += (_norm(down[A] - up[A]) + _norm(down[B] - up[B])) / 2
+            st.batch_width[s] = _norm(up[B] - up[A])
+    st.particles = Blobs.cat([st.particles, blobs])
+    st.tag = np.concatenate([st.tag, tag])
+    st.group = np.concatenate([st.group, group])
+    st.mu_line = st.mu[nr - 2].copy()
+    st.x.pop()
+    st.mu.pop()
+    st.conversions += 1
+    if merge is not None:
+        for s in range(len(topo.segments)):
+            if merge[s] > 1 and st.batch_rows[s] >= merge[s]:
+                merge_strip(st, s, core)
+    return len(blobs)
+
+
+def merge_strip(st, s, core):
+    """merge_strip!, state.jl:337-357 (strip s 0-based; tags hold s + 1, 0 when final)"""
+    idx = np.nonzero(st.tag == s + 1)[0]
+    o = np.zeros(3)
+    d = core.tri(o, o + np.array([st.batch_width[s], 0, 0]), o + np.array([st.batch_width[s], st.batch_length[s], 0]))
+    mx, ma = [], []
+    for gi in (1, 2):
+        sel = [i for i in idx if st.group[i] == gi]
+        if not sel:
+            continue
+        al = np.sum(st.particles.alpha[sel], axis=0)
+        w = np.array([_norm(st.particles.alpha[i]) for i in sel])
+        if w.sum() > 0:
+            x = np.sum(w[:, None] * st.particles.x[sel], axis=0) / w.sum()
+        else:
+            x = np.sum(st.particles.x[sel], axis=0) / len(sel)
+        mx.append(x)
+        ma.append(al)
+    st.keep_particles(st.tag != s + 1)
+    merged = Blobs(np.array(mx).reshape(-1, 3), np.array(ma).reshape(-1, 3), np.full(len(mx), d))
+    st.particles = Blobs.cat([st.particles, merged])
+    st.tag = np.concatenate([st.tag, np.zeros(len(mx), np.int64)])
+    st.group = np.concatenate([st.group, np.zeros(len(mx), np.int8)])
+    st.batch_rows[s] = 0
+    st.batch_length[s] = 0.0
+
+
+def vorticity_moments(b):
+    """state.jl:365-369: total vorticity and impulse of a set of particles"""
+    if not len(b):
+        return np.zeros(3), np.zeros(3)
+    return b.alpha.sum(0), np.cross(b.x, b.alpha).sum(0) / 2
+
+
+# ------------------------------------------------------------------------------------------ explicit rhs -------------
+
+def _known_wake(e):
+    """The known sheets and filaments as a wake: one node per corner and per filament end, with their doublets"""
+    from .wake import RigidWake, WakeOptions
+    ns, nf = len(e.sheets), len(e.segments)
+    x = np.zeros((3 * ns + 2 * nf, 3))
+    mu = np.zeros(3 * ns + 2 * nf)
+    for i, ((p, q, r), m) in enumerate(zip(e.sheets, e.sheet_mu)):
+        x[3 * i], x[3 * i + 1], x[3 * i + 2] = p, q, r
+        mu[3 * i:3 * i + 3] = m
+    for j, ((a, b), (ga, gb)) in enumerate(zip(e.segments, e.segment_gamma)):
+        x[3 * ns + 2 * j], x[3 * ns + 2 * j + 1] = a, b
+        mu[3 * ns + 2 * j], mu[3 * ns + 2 * j + 1] = ga, gb
+    tris = np.arange(3 * ns, dtype=np.int32).reshape(-1, 3)
+    fil = (3 * ns + np.arange(2 * nf, dtype=np.int32)).reshape(-1, 2)
+    w = RigidWake(x=x, te=np.zeros(len(x), np.int64), row=np.zeros(len(x), np.int64), tris=tris, filaments=fil,
+                  rows=np.zeros(1), span=1.0, options=WakeOptions())
+    return w, mu
+
+
+def explicit_rhs(g, dm, e: WakeElements | None, blobs: Blobs | None, opts: AssemblyOptions | None = None,
+                 ctx: Context | None = None, body_plan: AssemblyPlan | None = None):
+    """-int psi_i n.u_src dA for every slot (src/assembly/explicit.jl:31-86) on the device.  The sheets and filaments
+    take the assembly's tiers (a wake-only plan: exactly their assembled columns times their doublets); the particles
+    the far rule or near_rule (the plan's particle kernels).  body_plan: a wake-only plan of the body alone, reused for
+    the particles."""
+    ctx = ctx or default_context()
+    opts = opts or AssemblyOptions()
+    b = np.zeros(dm.nslot)
+    zero = np.zeros(len(g.tris))
+    if e is not None and (e.sheets or e.segments):
+        w, mu = _known_wake(e)
+        plan = AssemblyPlan(g, dm, w, opts, ctx=ctx, wake_only=True)
+        plan.assemble(zero, np.zeros(3))
+        b -= plan.wake_product(mu)
+        plan.close()
+    if blobs is not None and len(blobs):
+        plan = body_plan
+        if plan is None:
+            plan = AssemblyPlan(g, dm, None, opts, ctx=ctx, wake_only=True)
+            plan.assemble(zero, np.zeros(3))
+        b += plan.blob_rhs(blobs.x, blobs.alpha, blobs.delta)
+        if body_plan is None:
+            plan.close()
+    return b
+
+
+# ------------------------------------------------------------------------------------------ the simulation ----------
+
+def first_core(core, edge, topo, row):
+    """step.jl:65-70"""
+    w = max(_norm(edge[B] - edge[A]) for A, B in topo.segments)
+    o = np.zeros(3)
+    return core.tri(o, o + np.array([w, 0, 0]), o + np.array([w, row, 0]))
+
+
+def conversion_distance(max_rows, core, edge, topo, te_len, row):
+    """step.jl:72-84"""
+    d = first_core(core, edge, topo, row)
+    return (te_len + (max_rows + 0.5) * row) / d, d
+
+
+def rows_for_distance(cores, core, edge, topo, te_len, row):
+    """step.jl:86-88"""
+    return max(0, math.ceil((cores * first_core(core, edge, topo, row) - te_len) / row - 0.5))
+
+
+def check_conversion_distance(max_rows, core, edge, topo, te_len, row):
+    """step.jl:90-98"""
+    d, dl = conversion_distance(max_rows, core, edge, topo, te_len, row)
+    if d < 2:
+        need = rows_for_distance(2, core, edge, topo, te_len, row)
+        warnings.warn(f"Particles would be created {d:.2f} cores behind the trailing edge (core {dl:.3g}), which biases "
+                      f"the lift; max_rows = {need} keeps them two cores away")
+    return d
+
+
+class FreeWakeSim:
+    """FreeWakeSim (step.jl:8-58): the body at rest in V∞, started impulsively; the trailing edge panel is assembled
+    with the body and the bordered matrix factored once; `step()` changes only the right hand side.
+
+    precision: 'fp64' or 'mixed' (the factorization in FP32 with FP64 refinement, FP32 velocity kernels with FP64 sums).
+    The device work is in `_explicit_rhs`, `_solve` and `_velocities`."""
+
+    def __init__(self, g, dm, vinf, opts: FreeWakeOptions, assembly: AssemblyOptions | None = None,
+                 precision: str = "fp64", ctx: Context | None = None):
+        opts.check()
+        self.g, self.dm, self.opts = g, dm, opts
+        self.V = np.asarray(vinf, dtype=np.float64)
+        self.assembly = assembly or AssemblyOptions()
+        self.precision = precision
+        self.ctx = ctx
+        t0 = time.perf_counter()
+        self.els = build_elements(g)
+        self.sigma = source_strengths(self.els, self.V)
+        te_len = opts.te_len if opts.te_len is not None else 0.3 * _norm(self.V) * opts.dt
+        self.te = te_panel(g, dm, self.V, length=te_len, direction=opts.te_direction)
+        self._setup()
+        self.t_setup = time.perf_counter() - t0
+        self.topo = Topology.of(dm)
+        nte = self.topo.nte
+        self.edge = self.te.x[nte:2 * nte].copy()  # the trailing edge panel's downstream edge (its row 1)
+        self.core = CoreRule(opts, self.V)
+        row = _norm(self.V) * opts.dt
+        if opts.max_rows == "auto":
+            self.keep_rows = rows_for_distance(3, self.core, self.edge, self.topo, self.te.rows[1], row)
+        else:
+            self.keep_rows = int(opts.max_rows)
+            check_conversion_distance(self.keep_rows, self.core, self.edge, self.topo, self.te.rows[1], row)
+        widths = [_norm(self.edge[B] - self.edge[A]) for A, B in self.topo.segments]
+        if opts.merge_rows == "auto":
+            self.merge = [max(1, int(round(w / row))) for w in widths]
+        else:
+            self.merge = None if opts.merge_rows == 1 else [int(opts.merge_rows)] * len(widths)
+        self.state = FreeWakeState.empty(nte, len(self.topo.segments))
+        self.n, self.t = 0, 0.0
+        self.mu = np.zeros(dm.nslot)
+        self.mu_w = np.zeros(nte)
+        self.cp = np.zeros(len(g.tris))
+        self.I_prev = np.zeros(3)
+        self.history = []
+
+    # --- device work (overridable) ---
+    def _setup(self):
+        """assemble the body with the trailing edge panel and factor the bordered matrix once (step.jl:38-46)"""
+        ctx = self.ctx or default_context()
+        self.ctx = ctx
+        o = AssemblyOptions(**{**self.assembly.__dict__, "precision": self.precision})
+        plan = AssemblyPlan(self.g, self.dm, self.te, o, ctx=ctx)
+        plan.assemble(self.sigma, self.V)
+        self.b0 = plan.read_b()
+        self.F = factorize_galerkin(self.dm, plan, W=wake_rows(self.dm, self.te.te), mode=self.precision)
+        plan.close()
+        self.body_plan = AssemblyPlan(self.g, self.dm, None, self.assembly, ctx=ctx, wake_only=True)
+        self.body_plan.assemble(np.zeros(len(self.g.tris)), np.zeros(3))
+
+    def _explicit_rhs(self, we, particles):
+        return explicit_rhs(self.g, self.dm, we, particles, self.assembly, ctx=self.ctx, body_plan=self.body_plan)
+
+    def _solve(self, rhs):
+        out = solve_factored(self.F, rhs)
+        return out["q"], out["mu"], out["lam"]
+
+    def _velocities(self, gam, blobs, X):
+        ub = post.body_velocity(self.g, self.sigma, gam, X, kernel_far=self.assembly.kernel_far,
+                                precision=self.precision, ctx=self.ctx)
+        uw = post.blob_velocities(blobs.x, blobs.alpha, blobs.delta, X, precision=self.precision, ctx=self.ctx)
+        return ub + uw
+
+    # --- the step ---
+    def step(self):
+        """step!, step.jl:113-146"""
+        st, o, dm = self.state, self.opts, self.dm
+        we = vortex_elements(st, self.topo)
+        t0 = time.perf_counter()
+        rhs = self._explicit_rhs(we, st.particles)
+        t_rhs = time.perf_counter() - t0
+        q, mu, lam = self._solve(self.b0 + rhs)
+        mu_w = dm.K @ q
+        loads = self.step_loads(q, mu, mu_w, we)
+        if not st.x:
+            st.x.append(self.edge.copy())
+            st.mu.append(mu_w.copy())
+        nodes = np.vstack(st.x)
+        X = np.vstack([nodes, st.particles.x])
+        t0 = time.perf_counter()
+        U = self.wake_velocities(mu, mu_w, we, X)
+        t_vel = time.perf_counter() - t0
+        nn = len(nodes)
+        k = 0
+        for i, row in enumerate(st.x):
+            st.x[i] = row + o.dt * U[k:k + len(row)]
+            k += len(row)
+        st.particles.x = st.particles.x + o.dt * U[nn:]
+        st.x.insert(0, self.edge.copy())
+        st.mu.insert(0, mu_w.copy())
+        while len(st.x) - 1 > self.keep_rows:
+            convert_oldest(st, self.topo, self.core, self.merge)
+        if math.isfinite(o.delete_beyond):
+            self.delete_far()
+        rec = dict(n=self.n, t=self.t, **loads, rows=len(st.x) - 1, particles=len(st.particles), lam=float(lam[0]),
+                   umax=float(np.max(np.linalg.norm(U, axis=1))) if len(U) else 0.0, t_rhs=t_rhs, t_vel=t_vel)
+        self.history.append(rec)
+        self.mu, self.mu_w = mu, mu_w
+        self.n += 1
+        self.t += o.dt
+        return rec
+
+    def step_loads(self, q, mu, mu_w, we):
+        """step.jl:149-163"""
+        dm, els, V = self.dm, self.els, self.V
+        u = post.gradient_velocity(els, dm, mu, V)
+        dmu = np.zeros(dm.nslot) if self.n == 0 else (mu - self.mu) / self.opts.dt
+        cp = post.pressure_coefficient(u, V, post.element_values(dm, dmu).sum(1) / 3)
+        self.cp = cp
+        Fp, Mp = post.pressure_force(els, cp)
+        Fkj = post.kutta_joukowski_force(dm.wake_te, mu_w, V)
+        gam = post.sheet_strengths(els, dm, mu)
+        body = Blobs(els.centroid.copy(), gam * els.area[:, None], np.zeros(len(els)))
+        blobs = Blobs.cat([body, free_wake_blobs(rigid_elements(self.te, mu_w), self.core),
+                           free_wake_blobs(we, self.core), self.state.particles])
+        Om, I = vorticity_moments(blobs)
+        Fimp = np.full(3, np.nan) if self.n == 0 else -(2 / V.dot(V)) * (I - self.I_prev) / self.opts.dt
+        self.I_prev = I
+        return dict(mu_w=mu_w, F_pressure=Fp, M_pressure=Mp, F_kj=Fkj, F_impulse=Fimp, Omega=Om, I=I)
+
+    def wake_velocities(self, mu, mu_w, we, X):
+        """step.jl:166-173: V∞ + the body's exact kernels + every wake element as blobs"""
+        if self.opts.advection == "freestream":
+            return np.tile(self.V, (len(X), 1))
+        gam = post.sheet_strengths(self.els, self.dm, mu)
+        blobs = Blobs.cat([free_wake_blobs(rigid_elements(self.te, mu_w), self.core), free_wake_blobs(we, self.core),
+                           self.state.particles])
+        return self.V + self._velocities(gam, blobs, X)
+
+    def delete_far(self):
+        """delete_far!, step.jl:190-195"""
+        d = self.V / _norm(self.V)
+        limit = max(float(x @ d) for x in self.edge) + self.opts.delete_beyond * self.te.span
+        self.state.keep_particles(self.state.particles.x @ d <= limit)
+
+    def run(self, nsteps, progress=None):
+        """run!, step.jl:203-216 (without the VTK series)"""
+        for _ in range(nsteps):
+            rec = self.step()
+            if progress is not None:
+                progress(self, rec)
+        return self.history
+
+    def close(self):
+        self.F.close()
+        self.body_plan.close()
