@@ -118,6 +118,33 @@ def alpha_sweep_bench():
     return out
 
 
+def free_wake_bench(nsteps=150):
+    """Impulsively started rectangular NACA 0012 wing, AR 6, 120 x 40 quads, alpha 8 deg, dt = 0.02 chord/V∞, fixed
+    particle core 0.1 x the mean panel edge, mixed precision (FP32 velocity kernels with FP64 sums and bookkeeping):
+    the first nsteps steps (the full run is 1000 steps to ~84,000 particles)"""
+    from vphip import dofmap, freewake as FW, mesh, sweep
+    spec = mesh.WingSpec(code="0012", n_around=120, n_span=40, aspect=6.0)
+    g = mesh.naca_wing(spec)
+    dm = dofmap.build_dofmap(g)
+    V, L = sweep.wing_freestream(8.0)
+    edges = np.concatenate([np.linalg.norm(g.x[g.tris[:, (k + 1) % 3]] - g.x[g.tris[:, k]], axis=1) for k in range(3)])
+    t0 = time.perf_counter()
+    sim = FW.FreeWakeSim(g, dm, V, FW.FreeWakeOptions(dt=0.02, core=float(0.1 * edges.mean())), precision="mixed")
+    t1 = time.perf_counter()
+    steps = []
+    for _ in range(nsteps):
+        ts = time.perf_counter()
+        r = sim.step()
+        steps.append(time.perf_counter() - ts)
+    sim.close()
+    out = dict(nt=g.nt, nslot=dm.nslot, setup_s=t1 - t0, steps=nsteps, mean_step_s=float(np.mean(steps)),
+               last10_step_s=float(np.mean(steps[-10:])), particles=r["particles"], rows=r["rows"],
+               CL_kj=float(r["F_kj"] @ L / spec.aspect), CL_pressure=float(r["F_pressure"] @ L / spec.aspect),
+               t_rhs_last=r["t_rhs"], t_vel_last=r["t_vel"])
+    log(f"free wake: {out}")
+    return out
+
+
 def nbody_bench(g, dm, V, mu):
     """surface_velocity on the ellipsoid solution and blob sums at the free wake's end size (84,000 particles)"""
     from vphip import api, post
@@ -239,6 +266,7 @@ def main():
     if not args.no_extras:
         extras["alpha_sweep"] = alpha_sweep_bench()
         extras["wing_adjoint"] = wing_adjoint_bench()
+        extras["free_wake"] = free_wake_bench()
     cpu = None if args.no_cpu else cpu_baseline(g, dm, V)
     if cpu:
         log(f"cpu baseline {cpu}")

@@ -301,4 +301,89 @@ function solve_galerkin(dm::DOFMapping, A::AbstractMatrix, b::AbstractVector; W=
     return (q=s.q, μ=s.μ, λ=s.λ, cond=s.cond, residual=norm(M * x - rhs) / norm(rhs))
 end
 
+# ---- N-body velocity sums (src/post/Post.jl:32-112, src/freewake/step.jl:176-187) --------------------------------
+
+"""
+    surface_velocity(g, dm, σ, μ, V∞; wake=nothing, μ_wake=nothing, kernel_far=10, precision=:fp64, ctx)
+
+Post.surface_velocity on the GPU: the velocity at every centroid (`Vector{SVector{3}}`). `μ_wake` is one value per wake
+node (`wake_values`).
+"""
+function surface_velocity(g::SurfaceGeometry, dm::DOFMapping, σ::AbstractVector, μ::AbstractVector, V∞::SVector{3};
+                          wake::Union{Nothing,RigidWake}=nothing, μ_wake=nothing, kernel_far::Real=10.0,
+                          precision::Symbol=:fp64, ctx::Context=default_context())
+    s = SurfaceArrays(g, dm)
+    w = wake === nothing ? nothing : WakeArrays(wake)
+    sig, m = Vector{Cdouble}(σ), Vector{Cdouble}(μ)
+    mw = wake === nothing ? Cdouble[0] : Vector{Cdouble}(μ_wake)
+    vinf = Cdouble[V∞[1], V∞[2], V∞[3]]
+    u = zeros(Cdouble, 3, length(g.tris))
+    GC.@preserve s w sig m mw vinf u begin
+        wref = Ref(w === nothing ? VPWake(0, C_NULL, 0, C_NULL, 0, C_NULL) : c_wake(w))
+        wp = w === nothing ? Ptr{VPWake}(C_NULL) : Base.unsafe_convert(Ptr{VPWake}, wref)
+        rc = GC.@preserve wref ccall((:vp_surface_velocity, libvortexhip), Cint,
+            (Ptr{Cvoid}, Ref{VPSurface}, Ptr{VPWake}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Cdouble,
+             Cint, Ptr{Cdouble}),
+            ctx.ptr, Ref(c_surface(s)), wp, sig, m, mw, vinf, kernel_far, precision === :mixed ? 1 : 0, u)
+        check(ctx, rc)
+    end
+    return [SVector(u[1, i], u[2, i], u[3, i]) for i in axes(u, 2)]
+end
+
+"""
+    body_velocity(g, σ, γ, xs; kernel_far=10, precision=:fp64, ctx)
+
+Post.body_velocity on the GPU (γ: the elements' sheets, `sheet_strengths`).
+"""
+function body_velocity(g::SurfaceGeometry, σ::AbstractVector, γ::AbstractVector, xs::AbstractVector{<:SVector{3}};
+                       kernel_far::Real=10.0, precision::Symbol=:fp64, ctx::Context=default_context())
+    x = flat3(g.x, Cdouble)
+    tris = Cint[t[k] - 1 for t in g.tris for k in 1:3]
+    sig, gam, X = Vector{Cdouble}(σ), flat3(γ, Cdouble), flat3(xs, Cdouble)
+    u = zeros(Cdouble, 3, length(xs))
+    GC.@preserve x tris sig gam X u begin
+        sf = VPSurface(length(g.x), pointer(x), length(g.tris), pointer(tris), C_NULL, 0, 0, C_NULL)
+        check(ctx, ccall((:vp_body_velocity, libvortexhip), Cint,
+            (Ptr{Cvoid}, Ref{VPSurface}, Ptr{Cdouble}, Ptr{Cdouble}, Cint, Ptr{Cdouble}, Cdouble, Cint, Ptr{Cdouble}),
+            ctx.ptr, Ref(sf), sig, gam, length(xs), X, kernel_far, precision === :mixed ? 1 : 0, u))
+    end
+    return [SVector(u[1, i], u[2, i], u[3, i]) for i in axes(u, 2)]
+end
+
+"""
+    blob_velocities(blobs, X; precision=:fp64, ctx)
+
+FreeWake.blob_velocities on the GPU: the velocity of Rosenhead–Moore particles (`Blob`: x, α, δ) at the points X.
+"""
+function blob_velocities(blobs::AbstractVector, X::AbstractVector{<:SVector{3}}; precision::Symbol=:fp64,
+                         ctx::Context=default_context())
+    bx = flat3([b.x for b in blobs], Cdouble)
+    ba = flat3([b.α for b in blobs], Cdouble)
+    bd = Cdouble[b.δ for b in blobs]
+    Xf = flat3(X, Cdouble)
+    u = zeros(Cdouble, 3, length(X))
+    GC.@preserve bx ba bd Xf u begin
+        check(ctx, ccall((:vp_blob_velocities, libvortexhip), Cint,
+            (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Cint, Ptr{Cdouble}, Cint, Ptr{Cdouble}),
+            ctx.ptr, length(blobs), bx, ba, bd, length(X), Xf, precision === :mixed ? 1 : 0, u))
+    end
+    return [SVector(u[1, i], u[2, i], u[3, i]) for i in axes(u, 2)]
+end
+
+# ---- the adjoint's device pieces -----------------------------------------------------------------------------------
+
+"""
+    solve_transposed(F, r)
+
+Mᵀ x = r with the factorization (the adjoint system of a Galerkin objective; no counterpart in the reference).
+"""
+function solve_transposed(F::GalerkinFactorizationHIP, r::AbstractVector)
+    R = Vector{Cdouble}(r)
+    X = similar(R)
+    iters, resid = Ref{Cint}(0), Ref{Cdouble}(0)
+    GC.@preserve R X check(F.ctx, ccall((:vp_factor_solve_transposed, libvortexhip), Cint,
+        (Ptr{Cvoid}, Ptr{Cdouble}, Cint, Ptr{Cdouble}, Ref{Cint}, Ref{Cdouble}), F.ptr, R, 1, X, iters, resid))
+    return X
+end
+
 end # module
