@@ -181,7 +181,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the assembly metric (no wing, sweep, N-body)")
+    ap.add_argument("--no-solve", action="store_true", help="no factorizations or solves (with --no-extras: the "
+                    "assemblies alone, the program the far kernel's counter passes profile)")
+    ap.add_argument("--copy-calibration", action="store_true", help="only copy 1 GiB device to device three times "
+                    "(the known-size reference for the HBM byte counters); loads no library")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 3 << 30}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X, and the path runs on one device "
               "(how it would shard is described in DESIGN.md)", file=sys.stderr)
@@ -221,6 +234,10 @@ def main():
     flops = ex * (FLOPS_EXACT + FLOPS_BLOCK) + (3 * far_pairs_n - ex) * (FLOPS_MULTIPOLE + FLOPS_BLOCK)
     far_avg = float(np.mean(far_ms)) / 1e3
     achieved = flops / far_avg
+    traffic = None  # HBM bytes per far-kernel launch, measured by the counter passes of the profile summary
+    tpath = os.path.join(ROOT, "profiles", "far_pairs_traffic.json")
+    if os.path.exists(tpath):
+        traffic = float(json.load(open(tpath))["traffic_bytes_per_launch"])
     # mixed path
     pm = plans["mixed"]
     tms = []
@@ -235,6 +252,11 @@ def main():
     Amx, bmx = pm.read()
     entry_err = float(np.max(np.abs(Amx - A64)) / np.max(np.abs(A64)))
     # bordered factorization + solve in both modes, errors against the exact flow
+    if args.no_solve:
+        print(json.dumps({"metric": "pair-integrals/s", "value": value, "unit": "pair-integrals/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+                          "higher_is_better": True, "note": "assemblies only (--no-solve)"}))
+        return
     ef = EllipsoidFlow((2.0, 1.0, 0.5), V)
     mu_ex = ef.doublet(g.x[dm.slot_node])
     scale = ef.doublet_scale()
@@ -288,7 +310,9 @@ def main():
                    "nt": g.nt, "nslot": dm.nslot, "pairs_per_step": pairs, "tier_pairs": p.pairs,
                    "tier_points": p.points},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64, "traffic": None,
+                     "frac": achieved / PEAK_FP64, "traffic": traffic,
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE passes, copy-calibrated, "
+                                       "profiles/far_pairs_abused_ellipsoid_pmc.md" if traffic else None,
                      "kernel": "far_pairs<double>", "kernel_ms": far_avg * 1e3,
                      "algorithmic_flops": flops, "far_exact_points": ex, "far_points": 3 * far_pairs_n},
         "cpu_baseline": cpu,
