@@ -43,7 +43,8 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
                                                      const int *__restrict__ ts_ncol, const int *__restrict__ src_cols,
                                                      const int *__restrict__ slot_node, const double *__restrict__ lam,
                                                      const double *__restrict__ mu, double vx, double vy, double vz,
-                                                     double *__restrict__ G)
+                                                     double *__restrict__ G, int2 *__restrict__ exact_list,
+                                                     unsigned int *__restrict__ exact_count, unsigned int exact_cap)
 {
     __shared__ double GT[VMAX][3];
     __shared__ double GU[UMAX][3];
@@ -77,7 +78,17 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
         const TestFar &F = tf[t];
         bool far = false;
         if (valid && s != t) far = cls_norm(sc - ld3(F.c)) > P.far_ratio * fmax(F.diam, sdiam);
-        if (!far) continue;
+        // pairs with a closed-form point go to adj_far_exact (dual numbers); here only all-multipole pairs
+        bool multi = far;
+        if (far) {
+            const double fd = P.kernel_far * sdiam;
+            for (int q = 0; q < 3; q++) {
+                const V3<double> r0 = cls_sub(ld3(F.x[q]), sc);
+                multi = multi && cls_dot(r0, r0) > fd * fd;
+            }
+        }
+        append_pair(far && !multi, t, s, exact_list, exact_count, exact_cap);
+        if (!multi) continue;
         const Elem &T = el[t];
         const V3<double> XT[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
         const V3<double> nT = ld3(F.n);
@@ -91,7 +102,7 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
         for (int q = 0; q < 3; q++) {
             const double a = (la[0] * F.wpsi[q][0] + la[1] * F.wpsi[q][1] + la[2] * F.wpsi[q][2]) / F.area;
             V3<double> wv, g[3];
-            tri_point_grad(XS, el[s], ld3(F.x[q]), Q, 0, P.kernel_far, wv, g);
+            far_point_grad(XS, el[s], ld3(F.x[q]), Q, wv, g);
             const V3<double> gx = -1.0 * ((g[0] + g[1]) + g[2]);
             const double *l = rt.l + 3 * (off + q);
             p.Sv = p.Sv + a * wv;
@@ -121,6 +132,30 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
         const double v = GU[i / 3][i % 3];
         if (v != 0.0) unsafeAtomicAdd(&G[3 * col_node(ts_cols[st * UMAX + i / 3], P.nslot, P.nn, slot_node) + i % 3], v);
     }
+}
+
+// Far pairs with a closed-form point (within kernel_far diameters of the source): one thread per pair, the three far
+// points by tri_point_grad (dual numbers for the closed form).
+__global__ __launch_bounds__(256) void adj_far_exact(AsmParams P, RuleTab rt, const TestFar *__restrict__ tf,
+                                                     const Elem *__restrict__ el, const int *__restrict__ src_cols,
+                                                     const int *__restrict__ nodes, const int2 *__restrict__ list,
+                                                     unsigned int npairs, const double *__restrict__ lam,
+                                                     const double *__restrict__ mu, const int *__restrict__ corner_slot,
+                                                     double vx, double vy, double vz, double *__restrict__ G)
+{
+    const unsigned int pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= npairs) return;
+    const int t = list[pi].x, s = list[pi].y;
+    const TestFar &F = tf[t];
+    const Elem &T = el[t], &S = el[s];
+    AdjArgs aa{lam, mu, G, nodes, vx, vy, vz};
+    AdjPair ap;
+    adj_pair_begin(aa, P.nt, S, s, corner_slot + 3 * t, src_cols + 3 * s, ld3(F.n), ap);
+    const V3<double> XT[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
+    const int off = rt.off[R_FAR];
+#pragma unroll 1
+    for (int q = 0; q < 3; q++) adj_point(ap, XT, S, 0, P.kernel_far, rt.w[off + q], rt.l + 3 * (off + q), P.test);
+    adj_pair_end(aa, P.nt, T, XT, t, s, nodes, ap);
 }
 
 // Far filament pairs (seg_pairs' far test and far segment_block): one thread per (test triangle, filament).
@@ -296,9 +331,34 @@ void Plan::adjoint(const double *lam_h, const double *mu_h, const double *vinf, 
     HIPCHECK(hipMemsetAsync(d_G, 0, sizeof(double) * 3 * (nn + nw), st));
     const AsmParams P = params();
     const double vx = vinf[0], vy = vinf[1], vz = vinf[2];
+    // far pairs with a closed-form point: at most the forward's count of closed-form far points
+    unsigned long long hst[16];
+    HIPCHECK(hipMemcpyAsync(hst, d_stats, sizeof(hst), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    const unsigned int ecap = (unsigned int)std::min<unsigned long long>(hst[12] + 1024, 1ull << 31);
+    int2 *d_exact = dalloc<int2>(ecap);
+    unsigned int *d_ecount = dalloc<unsigned int>(1);
+    HIPCHECK(hipMemsetAsync(d_ecount, 0, sizeof(unsigned int), st));
     adj_far_pairs<<<dim3(ntt, nts), 256, 0, st>>>(P, rt, d_tf, d_el, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc, d_tsn,
-                                                  d_cols, d_slotnode, d_lam, d_mu, vx, vy, vz, d_G);
+                                                  d_cols, d_slotnode, d_lam, d_mu, vx, vy, vz, d_G, d_exact, d_ecount,
+                                                  ecap);
     HIPCHECK(hipGetLastError());
+    unsigned int nexact = 0;
+    HIPCHECK(hipMemcpyAsync(&nexact, d_ecount, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (nexact > ecap) {
+        (void)hipFree(d_exact);
+        (void)hipFree(d_ecount);
+        throw std::runtime_error("adjoint: more closed-form far pairs than the forward's closed-form far points");
+    }
+    if (nexact) {
+        adj_far_exact<<<(nexact + 255) / 256, 256, 0, st>>>(P, rt, d_tf, d_el, d_cols, d_nodes, d_exact, nexact, d_lam,
+                                                            d_mu, d_cs, vx, vy, vz, d_G);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    (void)hipFree(d_exact);
+    (void)hipFree(d_ecount);
     if (nseg) {
         const long long np = (long long)nt * nseg;
         adj_seg_far<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(P, rt, d_tf, d_el, d_cs, d_nodes, d_seg, d_segcol, d_lam,

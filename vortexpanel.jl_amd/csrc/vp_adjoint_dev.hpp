@@ -100,88 +100,6 @@ __device__ __forceinline__ V3<D3> sv_far_vertices(const V3<D3> *v, V3<double> x)
     return sv_far_rel(s, dconst(x) - c);
 }
 
-// The octupole multipole (source_velocity_far, source_triangle.jl:100-119) and the reverse derivative of
-// phi = Q.w with respect to the source's vertices, by hand: phi depends on r = x - c, the vertex offsets d_k = v_k - c
-// and the area A; with dr_k = d_k.r, dd_k = d_k.d_k and iRn = |r|^-n,
-//   d phi/d v_j = d phi/d d_j - (1/3) sum_k d phi/d d_k - (1/3) d phi/d r + (phi/A) dA/d v_j.
-// About 200 flops per point against ~3000 for dual numbers.
-__device__ __forceinline__ void far_point_grad(const V3<double> *XS, const Elem &S, V3<double> x, V3<double> Q,
-                                               V3<double> &wv, V3<double> *g)
-{
-    const V3<double> c = ld3(S.c), n = ld3(S.n);
-    const double A = S.area, s = A * (1.0 / (4 * kPi));
-    const V3<double> r = x - c;
-    const double R2 = dot(r, r), iR = 1.0 / dsqrt(R2);
-    const double iR2 = iR * iR, iR3 = iR2 * iR, iR5 = iR3 * iR2, iR7 = iR5 * iR2, iR9 = iR7 * iR2, iR11 = iR9 * iR2;
-    const double qr = dot(Q, r);
-    double cr = 0.0, rr = -3.0 * qr * iR5;
-    V3<double> w = mk3(0.0, 0.0, 0.0), gr = w, gd[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const V3<double> d = XS[k] - c;
-        const double dr = dot(d, r), dd = dot(d, d), qd = dot(Q, d);
-        cr += (15 * dr * dr * iR7 - 3 * dd * iR5) * (1.0 / 24) - (45 * dd * dr * iR7 - 105 * dr * dr * dr * iR9) * (1.0 / 180);
-        const double cd = -6 * dr * iR5 * (1.0 / 24) - (45 * dr * dr * iR7 - 9 * dd * iR5) * (1.0 / 180);
-        w = w + cd * d;
-        const double pdr = qr * (30 * dr * iR7 * (1.0 / 24) - (45 * dd * iR7 - 315 * dr * dr * iR9) * (1.0 / 180)) +
-                           qd * (-6 * iR5 * (1.0 / 24) - 90 * dr * iR7 * (1.0 / 180));
-        const double pdd = qr * (-3 * iR5 * (1.0 / 24) - 45 * dr * iR7 * (1.0 / 180)) + qd * (9 * iR5 * (1.0 / 180));
-        gr = gr + pdr * d;
-        rr += qr * (-105 * dr * dr * iR9 * (1.0 / 24) + 15 * dd * iR7 * (1.0 / 24) + 315 * dd * dr * iR9 * (1.0 / 180) -
-                    945 * dr * dr * dr * iR11 * (1.0 / 180)) +
-              qd * (30 * dr * iR7 * (1.0 / 24) + 315 * dr * dr * iR9 * (1.0 / 180) - 45 * dd * iR7 * (1.0 / 180));
-        gd[k] = s * (cd * Q + pdr * r + (2 * pdd) * d);
-    }
-    w = w + (iR3 + cr) * r;
-    wv = s * w;
-    const double phi = dot(Q, wv);
-    const V3<double> g_r = s * ((iR3 + cr) * Q + gr + rr * r);
-    const V3<double> base = (-1.0 / 3) * ((gd[0] + gd[1]) + gd[2] + g_r);
-    const double pa = 0.5 * phi / A;
-    g[0] = gd[0] + base + pa * cross(n, XS[2] - XS[1]);
-    g[1] = gd[1] + base + pa * cross(XS[2] - XS[0], n);
-    g[2] = gd[2] + base + pa * cross(n, XS[1] - XS[0]);
-}
-
-// phi = Q.w(x; S) (w = source_velocity, source_triangle.jl:134-138, with the forward's switch) and d phi/d(vertex k of
-// S) for k = 0, 1, 2; the value w(x) in wv.  The multipole by its reverse derivative, the closed form by dual numbers.
-__device__ void tri_point_grad(const V3<double> *XS, const Elem &S, V3<double> x, V3<double> Q, int side,
-                               double kernel_far, V3<double> &wv, V3<double> *g)
-{
-    const V3<double> r0 = cls_sub(x, ld3(S.c));
-    const double fd = kernel_far * S.diam;
-    if (cls_dot(r0, r0) > fd * fd) {
-        far_point_grad(XS, S, x, Q, wv, g);
-        return;
-    }
-#pragma unroll 1
-    for (int k = 0; k < 3; k++) {
-        V3<D3> v[3];
-        for (int j = 0; j < 3; j++) v[j] = dseed(XS[j], j == k ? 1.0 : 0.0);
-        const V3<D3> w = sv_exact_vertices(v, x, side, &S, S.diam);
-        const D3 phi = w.x * Q.x + w.y * Q.y + w.z * Q.z;
-        g[k] = mk3(phi.d[0], phi.d[1], phi.d[2]);
-        if (k == 0) wv = mk3(w.x.v, w.y.v, w.z.v);
-    }
-}
-
-// phi = n.(mu_a va + mu_b vb) of the filament from fa to fb (segment_velocity, src/kernels/segment.jl:15-25) and its
-// derivatives with respect to fa and fb; the value mu_a va + mu_b vb in wv.
-__device__ void seg_point_grad(V3<double> fa, V3<double> fb, V3<double> x, V3<double> n, double mua, double mub,
-                               V3<double> &wv, V3<double> *g)
-{
-#pragma unroll 1
-    for (int k = 0; k < 2; k++) {
-        const V3<D3> a = dseed(fa, k == 0 ? 1.0 : 0.0), b = dseed(fb, k == 1 ? 1.0 : 0.0), xd = dconst(x);
-        V3<D3> va, vb;
-        seg_vel(xd - a, xd - b, va, vb);
-        const V3<D3> w = D3(mua) * va + D3(mub) * vb;
-        const D3 phi = w.x * n.x + w.y * n.y + w.z * n.z;
-        g[k] = mk3(phi.d[0], phi.d[1], phi.d[2]);
-        if (k == 0) wv = mk3(w.x.v, w.y.v, w.z.v);
-    }
-}
-
 // ------------------------------------------------------------------------------------------ triangle adjoints ------
 
 // g[k] += gradient of p.N (N = (X1 - X0) x (X2 - X0), twice the area times the normal) with respect to vertex k
@@ -226,6 +144,138 @@ __device__ __forceinline__ void sheet_adj(const V3<double> *X, const double *mu,
     g[1] = g[1] - ((mu[2] - mu[0]) / A2) * d;
     g[2] = g[2] - ((mu[0] - mu[1]) / A2) * d;
     area2_adj(X, dot(d, E) / (A2 * A2), g);
+}
+
+// The octupole multipole (source_velocity_far, source_triangle.jl:100-119) and the reverse derivative of
+// phi = Q.w with respect to the source's vertices, by hand: phi depends on r = x - c, the vertex offsets d_k = v_k - c
+// and the area A; with dr_k = d_k.r, dd_k = d_k.d_k and iRn = |r|^-n,
+//   d phi/d v_j = d phi/d d_j - (1/3) sum_k d phi/d d_k - (1/3) d phi/d r + (phi/A) dA/d v_j.
+// About 200 flops per point against ~3000 for dual numbers.
+__device__ __forceinline__ void far_point_grad(const V3<double> *XS, const Elem &S, V3<double> x, V3<double> Q,
+                                               V3<double> &wv, V3<double> *g)
+{
+    const V3<double> c = ld3(S.c), n = ld3(S.n);
+    const double A = S.area, s = A * (1.0 / (4 * kPi));
+    const V3<double> r = x - c;
+    const double R2 = dot(r, r), iR = 1.0 / dsqrt(R2);
+    const double iR2 = iR * iR, iR3 = iR2 * iR, iR5 = iR3 * iR2, iR7 = iR5 * iR2, iR9 = iR7 * iR2, iR11 = iR9 * iR2;
+    const double qr = dot(Q, r);
+    double cr = 0.0, rr = -3.0 * qr * iR5;
+    V3<double> w = mk3(0.0, 0.0, 0.0), gr = w, gd[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const V3<double> d = XS[k] - c;
+        const double dr = dot(d, r), dd = dot(d, d), qd = dot(Q, d);
+        cr += (15 * dr * dr * iR7 - 3 * dd * iR5) * (1.0 / 24) - (45 * dd * dr * iR7 - 105 * dr * dr * dr * iR9) * (1.0 / 180);
+        const double cd = -6 * dr * iR5 * (1.0 / 24) - (45 * dr * dr * iR7 - 9 * dd * iR5) * (1.0 / 180);
+        w = w + cd * d;
+        const double pdr = qr * (30 * dr * iR7 * (1.0 / 24) - (45 * dd * iR7 - 315 * dr * dr * iR9) * (1.0 / 180)) +
+                           qd * (-6 * iR5 * (1.0 / 24) - 90 * dr * iR7 * (1.0 / 180));
+        const double pdd = qr * (-3 * iR5 * (1.0 / 24) - 45 * dr * iR7 * (1.0 / 180)) + qd * (9 * iR5 * (1.0 / 180));
+        gr = gr + pdr * d;
+        rr += qr * (-105 * dr * dr * iR9 * (1.0 / 24) + 15 * dd * iR7 * (1.0 / 24) + 315 * dd * dr * iR9 * (1.0 / 180) -
+                    945 * dr * dr * dr * iR11 * (1.0 / 180)) +
+              qd * (30 * dr * iR7 * (1.0 / 24) + 315 * dr * dr * iR9 * (1.0 / 180) - 45 * dd * iR7 * (1.0 / 180));
+        gd[k] = s * (cd * Q + pdr * r + (2 * pdd) * d);
+    }
+    w = w + (iR3 + cr) * r;
+    wv = s * w;
+    const double phi = dot(Q, wv);
+    const V3<double> g_r = s * ((iR3 + cr) * Q + gr + rr * r);
+    const V3<double> base = (-1.0 / 3) * ((gd[0] + gd[1]) + gd[2] + g_r);
+    const double pa = 0.5 * phi / A;
+    g[0] = gd[0] + base + pa * cross(n, XS[2] - XS[1]);
+    g[1] = gd[1] + base + pa * cross(XS[2] - XS[0], n);
+    g[2] = gd[2] + base + pa * cross(n, XS[1] - XS[0]);
+}
+
+// The closed form (source_velocity_exact, source_triangle.jl:75-89) and the reverse derivative of phi = Q.w with
+// respect to the source's vertices (x fixed), by hand: w = (1/4pi)[sum_e nu_e L_e + Omega n] with
+// L_e = log((a + b + d)/(a + b - d)) (dL/da = dL/db = -2d/s2d2, dL/dd = 2(a + b)/s2d2, s2d2 the cancellation-free
+// (a + b)^2 - d^2), nu_e = e x n / d and Omega = 2 atan2(N, D); the normal's dependence is gathered into one vector
+// and passed through its closed-form adjoint.  In the own element's plane (side != 0) Omega is the constant one-sided
+// limit; off it the atan2 form is differentiated (also where the forward takes the in-plane zero of side = 0).
+__device__ __forceinline__ void exact_point_grad(const V3<double> *XS, const Elem &S, V3<double> x, V3<double> Q,
+                                                 int side, V3<double> &wv, V3<double> *g)
+{
+    const V3<double> n = ld3(S.n);
+    const V3<double> r[3] = {x - XS[0], x - XS[1], x - XS[2]};
+    const double l[3] = {dsqrt(dot(r[0], r[0])), dsqrt(dot(r[1], r[1])), dsqrt(dot(r[2], r[2]))};
+    const V3<double> nQ = cross(n, Q);
+    V3<double> cn = mk3(0.0, 0.0, 0.0), wt = cn;
+    g[0] = g[1] = g[2] = cn;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int i = k, j = (k + 1) % 3;
+        const V3<double> e = XS[j] - XS[i];
+        const double d = S.len[k], a = l[i], b = l[j];
+        const double p = dot(r[i], r[j]);
+        const V3<double> c = cross(r[i], r[j]);
+        const double s2d2 = p < 0.0 ? 2.0 * dot(c, c) / (a * b - p) : 2.0 * (a * b + p);
+        const double t = a + b + d;
+        const double Le = dlog(t * t / s2d2);
+        const V3<double> nu = ld3(S.nu[k]);
+        wt = wt + Le * nu;
+        const double f = dot(Q, nu);
+        const double dLa = -2.0 * d / s2d2, dLd = 2.0 * (a + b) / s2d2;
+        const V3<double> ge = Le * ((1.0 / d) * nQ - (f / (d * d)) * e);
+        g[i] = g[i] + (f * (-dLa / a)) * r[i] - (f * dLd / d) * e - ge;
+        g[j] = g[j] + (f * (-dLa / b)) * r[j] + (f * dLd / d) * e + ge;
+        cn = cn + (Le / d) * cross(Q, e);
+    }
+    const double qn = dot(Q, n);
+    double Om;
+    const double z = cls_dot(cls_sub(x, ld3(S.c)), n);
+    if (side != 0 && fabs(z) <= 1e-12 * S.diam) {
+        Om = side * 2 * kPi * inplane_fraction(S, x, 1e-12);
+    } else {
+        const V3<double> &ra = r[0], &rb = r[1], &rc = r[2];
+        const double la = l[0], lb = l[1], lc = l[2];
+        const double N = dot(ra, cross(rb, rc));
+        const double D = la * lb * lc + dot(ra, rb) * lc + dot(ra, rc) * lb + dot(rb, rc) * la;
+        Om = 2.0 * datan2(N, D);
+        const double h = 2.0 / (N * N + D * D), dN = h * D * qn, dD = -h * N * qn;
+        g[0] = g[0] - dN * cross(rb, rc) - dD * ((lb * lc + dot(rb, rc)) / la * ra + lc * rb + lb * rc);
+        g[1] = g[1] - dN * cross(rc, ra) - dD * ((la * lc + dot(ra, rc)) / lb * rb + lc * ra + la * rc);
+        g[2] = g[2] - dN * cross(ra, rb) - dD * ((la * lb + dot(ra, rb)) / lc * rc + lb * ra + la * rb);
+    }
+    cn = cn + Om * Q;
+    normal_adj(XS, cn, g);
+    const double s = 1.0 / (4 * kPi);
+    wv = s * (wt + Om * n);
+    g[0] = s * g[0];
+    g[1] = s * g[1];
+    g[2] = s * g[2];
+}
+
+// phi = Q.w(x; S) (w = source_velocity, source_triangle.jl:134-138, with the forward's switch) and d phi/d(vertex k of
+// S) for k = 0, 1, 2; the value w(x) in wv.  Both branches by their reverse derivatives.
+__device__ __forceinline__ void tri_point_grad(const V3<double> *XS, const Elem &S, V3<double> x, V3<double> Q, int side,
+                                               double kernel_far, V3<double> &wv, V3<double> *g)
+{
+    const V3<double> r0 = cls_sub(x, ld3(S.c));
+    const double fd = kernel_far * S.diam;
+    if (cls_dot(r0, r0) > fd * fd)
+        far_point_grad(XS, S, x, Q, wv, g);
+    else
+        exact_point_grad(XS, S, x, Q, side, wv, g);
+}
+
+// phi = n.(mu_a va + mu_b vb) of the filament from fa to fb (segment_velocity, src/kernels/segment.jl:15-25) and its
+// derivatives with respect to fa and fb; the value mu_a va + mu_b vb in wv.
+__device__ void seg_point_grad(V3<double> fa, V3<double> fb, V3<double> x, V3<double> n, double mua, double mub,
+                               V3<double> &wv, V3<double> *g)
+{
+#pragma unroll 1
+    for (int k = 0; k < 2; k++) {
+        const V3<D3> a = dseed(fa, k == 0 ? 1.0 : 0.0), b = dseed(fb, k == 1 ? 1.0 : 0.0), xd = dconst(x);
+        V3<D3> va, vb;
+        seg_vel(xd - a, xd - b, va, vb);
+        const V3<D3> w = D3(mua) * va + D3(mub) * vb;
+        const D3 phi = w.x * n.x + w.y * n.y + w.z * n.z;
+        g[k] = mk3(phi.d[0], phi.d[1], phi.d[2]);
+        if (k == 0) wv = mk3(w.x.v, w.y.v, w.z.v);
+    }
 }
 
 // A pair's accumulated sums -> the gradient of P_ts with respect to T's and S's vertices (see the file head).
