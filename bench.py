@@ -319,6 +319,7 @@ def main():
         "mixed": {"assemble_ms": mixed_ms, "pair_integrals_per_s": pairs / (mixed_ms / 1e3),
                   "max_entry_rel_diff_vs_fp64": entry_err, "max_mu_rel_diff_vs_fp64": dmu},
         "solve": solves,
+        "gemm_tuning": json.loads(ctx.lib.vp_gemm_tuning().decode()),
         "kernels_ms": tm,
         **extras,
     }

@@ -153,14 +153,19 @@ int vp_factorize(vp_ctx *ctx, const double *A, long long lda, int A_on_device, c
  * row-major host arrays (R[i*nrhs + r]; the caller forms [Cᵀ b; 0]).  iters: refinement steps taken; resid: the
  * largest |R - M X|_inf / (|M|_inf |X|_inf). */
 int vp_factor_solve(vp_factor *f, const double *R, int nrhs, double *X, int *iters, double *resid);
-/* Solve Mᵀ X = R (the adjoint system; no counterpart in the reference): same layout as vp_factor_solve, through a
- * factorization of Mᵀ made on first use and shared with vp_factor_cond, FP64 refinement against Mᵀ. */
+/* Solve Mᵀ X = R (the adjoint system; no counterpart in the reference): same layout as vp_factor_solve, through the
+ * same factors read transposed (no second factorization), FP64 refinement against Mᵀ. */
 int vp_factor_solve_transposed(vp_factor *f, const double *R, int nrhs, double *X, int *iters, double *resid);
 /* 1-norm condition number estimate of M (1 / gecon's rcond, solve.jl:85). */
 int vp_factor_cond(vp_factor *f, double *cond);
 /* The border column scales se (nb values, solve.jl:33) and n. */
 int vp_factor_info(vp_factor *f, int *n, double *se);
 void vp_factor_destroy(vp_factor *f);
+/* The LU trailing update's GEMM tile shapes chosen by the autotuner (no counterpart in the reference), as JSON:
+ * per precision the chosen BMxBNxBK (null before the first factorization large enough to time), the update size timed
+ * and each candidate's milliseconds.  The tuner runs inside the first vp_factorize per precision; the environment
+ * variable VP_GEMM_TILE=<index> forces a candidate instead.  The string lives until the next call. */
+const char *vp_gemm_tuning(void);
 
 /* ---- N-body velocity sums (src/post/Post.jl:32-112, src/freewake/step.jl:176-187) -------------------------- */
 
@@ -179,10 +184,12 @@ int vp_body_velocity(vp_ctx *ctx, const vp_surface *g, const double *sigma, cons
 int vp_blob_velocities(vp_ctx *ctx, int nb, const double *bx, const double *alpha, const double *delta, int np,
                        const double *xs, int precision, double *u);
 
-/* Test entry points (host arrays, row-major): C = A B (sub = 0) or C -= A B (sub = 1) with the library's MFMA GEMM
- * (prec 0 FP64, 1 FP32; A M x K, B K x N, C M x N, all doubles on the host); and x = M^-1 r for a dense n x n
- * matrix through the randomized LU (mode 0 FP64, 1 FP32 factors) with FP64 refinement. */
-int vp_gemm(vp_ctx *ctx, int prec, int sub, int M, int N, int K, const double *A, const double *B, double *C);
+/* Test entry points (host arrays, row-major): C = op(A) B or C -= op(A) B with the library's MFMA GEMM (prec 0 FP64,
+ * 1 FP32; B K x N, C M x N, all doubles on the host).  flags: bit 0 subtract; bit 1 transposed A (A stored K x M,
+ * op(A) = Aᵀ), else A is M x K; bits 4-7 the tile shape's index among the autotuner's five candidates (0 is the
+ * default).  And x = M^-1 r for a dense n x n matrix through the randomized LU (mode 0 FP64, 1 FP32 factors) with
+ * FP64 refinement. */
+int vp_gemm(vp_ctx *ctx, int prec, int flags, int M, int N, int K, const double *A, const double *B, double *C);
 int vp_dense_solve(vp_ctx *ctx, int n, const double *M, int mode, const double *r, double *x, int *iters);
 
 #ifdef __cplusplus

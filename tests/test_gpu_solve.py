@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from vphip import api, dofmap, mesh, solve, wake as wk
+from vphip import _lib as L, api, dofmap, mesh, solve, wake as wk
 
 pytestmark = pytest.mark.gpu
 
@@ -14,23 +14,38 @@ VG = np.array([1.0, 0.3, -0.2]) / np.linalg.norm([1.0, 0.3, -0.2])
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-@pytest.mark.parametrize("shape", [(16, 16, 4), (64, 64, 64), (130, 70, 33), (300, 1, 257), (17, 200, 64)])
+@pytest.mark.parametrize("shape", [(16, 16, 4), (64, 64, 64), (130, 70, 33), (300, 1, 257), (17, 200, 64),
+                                   (260, 300, 64)])
 def test_gemm(prec, shape):
+    """Every autotuner tile candidate, plain and transposed A, set and subtract, against numpy."""
     M, N, K = shape
     print(f"gemm prec={prec} M={M} N={N} K={K}", flush=True)
     rng = np.random.default_rng(M * 7 + N * 3 + K)
     A, B, Cm = rng.standard_normal((M, K)), rng.standard_normal((K, N)), rng.standard_normal((M, N))
     if prec == 1:
         A, B, Cm = (x.astype(np.float32).astype(np.float64) for x in (A, B, Cm))
+    At = np.ascontiguousarray(A.T)
     ctx = api.default_context()
-    for sub in (0, 1):
-        out = np.ascontiguousarray(Cm.copy())
-        rc = ctx.lib.vp_gemm(ctx.h, prec, sub, M, N, K, A.ctypes.data_as(C.POINTER(C.c_double)),
-                             B.ctypes.data_as(C.POINTER(C.c_double)), out.ctypes.data_as(C.POINTER(C.c_double)))
-        assert rc == 0
-        ref = (Cm - A @ B) if sub else A @ B
-        err = np.abs(out - ref).max() / np.abs(ref).max()
-        assert err < (1e-13 if prec == 0 else 2e-6), (sub, err)
+    for tile in range(5):
+        for ta in (0, 1):
+            for sub in (0, 1):
+                out = np.ascontiguousarray(Cm.copy())
+                src = At if ta else A
+                rc = ctx.lib.vp_gemm(ctx.h, prec, sub | (ta << 1) | (tile << 4), M, N, K,
+                                     src.ctypes.data_as(C.POINTER(C.c_double)), B.ctypes.data_as(C.POINTER(C.c_double)),
+                                     out.ctypes.data_as(C.POINTER(C.c_double)))
+                assert rc == 0
+                ref = (Cm - A @ B) if sub else A @ B
+                err = np.abs(out - ref).max() / np.abs(ref).max()
+                assert err < (1e-13 if prec == 0 else 2e-6), (tile, ta, sub, err)
+
+
+def test_gemm_rejects_bad_flags():
+    ctx = api.default_context()
+    A = np.zeros(4)
+    for flags in (4, 5 << 4, 1 << 8):
+        rc = ctx.lib.vp_gemm(ctx.h, 0, flags, 2, 2, 2, *(A.ctypes.data_as(C.POINTER(C.c_double)) for _ in range(3)))
+        assert rc == L.VP_E_ARGUMENT
 
 
 @pytest.mark.parametrize("mode", [0, 1])
@@ -100,3 +115,29 @@ def test_wing_solve_and_batch():
             e = np.max(np.abs(out["mu"][:, j] - ref["mu"])) / np.max(np.abs(ref["mu"]))
             assert e < 1e-10, (mode, j, e)
         F.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_wing_transposed_solve(mode):
+    """Mᵀ X = R through the forward factors (the adjoint system) against numpy's solve of the oracle's bordered
+    matrix (src/assembly/solve.jl:15-42), and the duality zᵀ (M^-1 r) = (M^-T z)ᵀ r."""
+    g = mesh.naca_wing(mesh.WingSpec(code="2412", n_around=20, n_span=6, aspect=5.0))
+    dm = dofmap.build_dofmap(g)
+    V = np.array([np.cos(0.07), 0.0, np.sin(0.07)])
+    w = wk.build_wake(g, dm, V, wk.WakeOptions(length=2.0))
+    A, b, W = _case(g, dm, V, w)
+    print(f"wing transposed solve mode={mode}", flush=True)
+    M, _, _ = O.bordered_system(dm, A, b, W)
+    F = solve.factorize_galerkin(dm, A, W=W, mode=mode)
+    rng = np.random.default_rng(5)
+    R = rng.standard_normal((F.n, 3))
+    X = F.solve_bordered_t(R)
+    ref = np.linalg.solve(M.T, R)
+    assert np.abs(X - ref).max() / np.abs(ref).max() < 1e-11
+    assert F.last_resid < 1e-14
+    r, z = rng.standard_normal(F.n), rng.standard_normal(F.n)
+    lhs, rhs = z @ F.solve_bordered(r), F.solve_bordered_t(z) @ r
+    assert abs(lhs - rhs) < 1e-11 * (abs(lhs) + 1)
+    ref_cond = np.linalg.cond(M, 1)
+    assert 0.3 < F.cond / ref_cond <= 1.0 + 1e-6, (F.cond, ref_cond)
+    F.close()

@@ -252,7 +252,7 @@ int vp_factor_solve_transposed(vp_factor *f, const double *R, int nrhs, double *
     vp::Ctx *c = f->f.ctx;
     if (!R || !X || nrhs <= 0) return fail(c, VP_E_ARGUMENT, "null argument or nrhs < 1");
     return guard(c, [&] {
-        int it = f->f.solve_t(R, nrhs, X, resid);
+        int it = f->f.solve(R, nrhs, X, false, 30, resid, true);
         if (iters) *iters = it;
     });
 }
@@ -264,6 +264,13 @@ int vp_factor_cond(vp_factor *f, double *cond)
         if (f->f.cond < 0) f->f.cond = f->f.anorm1 * f->f.cond_estimate();
         *cond = f->f.cond;
     });
+}
+
+const char *vp_gemm_tuning(void)
+{
+    static thread_local std::string report;
+    report = vp::tuning_report();
+    return report.c_str();
 }
 
 int vp_factor_info(vp_factor *f, int *n, double *se)
@@ -282,12 +289,13 @@ void vp_factor_destroy(vp_factor *f)
     delete f;
 }
 
-int vp_gemm(vp_ctx *ctx, int prec, int sub, int M, int N, int K, const double *A, const double *B, double *C)
+int vp_gemm(vp_ctx *ctx, int prec, int flags, int M, int N, int K, const double *A, const double *B, double *C)
 {
     if (!ctx) return VP_E_ARGUMENT;
-    if (!A || !B || !C || (prec != 0 && prec != 1) || M <= 0 || N <= 0 || K <= 0)
+    if (!A || !B || !C || (prec != 0 && prec != 1) || M <= 0 || N <= 0 || K <= 0 || (flags & ~0xF3) ||
+        ((flags >> 4) & 15) >= 5)
         return fail(&ctx->c, VP_E_ARGUMENT, "bad argument");
-    return guard(&ctx->c, [&] { vp::gemm_host(ctx->c.stream, prec, sub, M, N, K, A, B, C); });
+    return guard(&ctx->c, [&] { vp::gemm_host(ctx->c.stream, prec, flags, M, N, K, A, B, C); });
 }
 
 int vp_dense_solve(vp_ctx *ctx, int n, const double *M, int mode, const double *r, double *x, int *iters)

@@ -86,7 +86,7 @@ struct Factor {
     Ctx *ctx = nullptr;
     int mode = 0, nslot = 0, nq = 0, nw = 0, nb = 0, n = 0, npad = 0;
     long long ld = 0;
-    double *M = nullptr, *Mt = nullptr, *d_ru = nullptr, *d_rv = nullptr;
+    double *M = nullptr, *d_ru = nullptr, *d_rv = nullptr;
     double *G64 = nullptr, *linv64 = nullptr, *uinv64 = nullptr;
     float *G32 = nullptr, *linv32 = nullptr, *uinv32 = nullptr;
     double *X = nullptr, *R = nullptr, *Y = nullptr, *B = nullptr, *nrm = nullptr;
@@ -95,21 +95,18 @@ struct Factor {
     double anorm1 = 0.0, anorm_inf = -1.0, cond = -1.0;
     std::vector<double> se;
     std::vector<void *> owned;
-    Factor *tfac = nullptr;
 
     ~Factor();
     void build(Ctx *c, const double *A, long long lda, const vp_constraints &k, int mode);
     void init_from_dense(const double *Md, long long ldd, int n, int npad, int mode, double anorm);
     void ensure_ws(int nrhs);
-    void apply_inverse(int nrhs);
-    int solve(const double *B, int nrhs, double *X, bool device_io, int max_iter, double *resid);
+    void apply_inverse(int nrhs, bool trans);
+    int solve(const double *B, int nrhs, double *X, bool device_io, int max_iter, double *resid, bool trans = false);
     double cond_estimate();
-    void solve_transposed(const double *z, double *x);
-    int solve_t(const double *B, int nrhs, double *X, double *resid);
-    void transpose_into(const double *in, double *out);
 };
 
-void gemm_host(hipStream_t s, int prec, int sub, int M, int N, int K, const double *A, const double *B, double *C);
+std::string tuning_report();
+void gemm_host(hipStream_t s, int prec, int flags, int M, int N, int K, const double *A, const double *B, double *C);
 void surface_velocity_host(Ctx *c, const vp_surface &g, const vp_wake *w, const double *sigma, const double *mu,
                            const double *mu_wake, const double *vinf, double kernel_far, int precision, double *u);
 void body_velocity_host(Ctx *c, const vp_surface &g, const double *sigma, const double *gam, int np, const double *xs,

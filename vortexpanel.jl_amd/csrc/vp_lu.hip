@@ -13,6 +13,9 @@
 // Layouts: all device matrices are row-major with a leading dimension; right hand sides are n x nrhs row-major.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <mutex>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -51,18 +54,20 @@ template <> struct Mfma<float> {
 
 // ------------------------------------------------------------------------------------------ GEMM (MFMA) ------------
 
-// C = A B (SET) or C -= A B (SUB), row-major, arbitrary M, N, K.  Workgroup tile BM x BN, four waves in a 2 x 2
-// grid each owning a (BM/2) x (BN/2) sub-tile of 16 x 16 MFMA blocks.  A is staged in LDS transposed (As[k][m]) and B
-// as is (Bs[k][n]), so each lane of an MFMA reads A[m = l%16][k = l/16] and B[k = l/16][n = l%16].  In place
-// (C aliasing B) is allowed when one workgroup row covers all of M (the whole of B's tile is read before C is written),
-// and (C aliasing A) when one workgroup column covers all of N.
-template <typename T, int BM, int BN, int KB, bool SUB>
+// C = A B (SET) or C -= A B (SUB), row-major, arbitrary M, N, K; with TA, A is read transposed (element (m, k) at
+// A[k lda + m]).  Workgroup tile BM x BN, four waves in a 2 x 2 grid each owning a (BM/2) x (BN/2) sub-tile of 16 x 16
+// MFMA blocks.  A is staged in LDS as As[k][m] and B as Bs[k][n], so each lane of an MFMA reads A[m = l%16][k = l/16]
+// and B[k = l/16][n = l%16].  The next k-tile is loaded into registers while the current one is multiplied.  In place
+// (C aliasing B) is allowed when one workgroup row covers all of M (the whole of B's tile is read before C is
+// written), and (C aliasing A) when one workgroup column covers all of N.
+template <typename T, int BM, int BN, int BK, bool SUB, bool TA>
 __global__ __launch_bounds__(256) void gemm(int M, int N, int K, const T *A, long long lda, const T *B, long long ldb,
                                             T *C, long long ldc)
 {
     constexpr int WM = BM / 2, WN = BN / 2, TI = WM / 16, TJ = WN / 16;
-    __shared__ T As[KB][BM + 1];
-    __shared__ T Bs[KB][BN + 1];
+    constexpr int LA = BM * BK / 256, LB = BK * BN / 256;  // elements each thread stages per k-tile
+    __shared__ T As[BK][BM + 1];
+    __shared__ T Bs[BK][BN + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const long long m0 = (long long)blockIdx.y * BM, n0 = (long long)blockIdx.x * BN;
@@ -70,22 +75,58 @@ __global__ __launch_bounds__(256) void gemm(int M, int N, int K, const T *A, lon
     for (int i = 0; i < TI; i++)
         for (int j = 0; j < TJ; j++)
             for (int v = 0; v < 4; v++) acc[i][j][v] = T(0);
-    for (int k0 = 0; k0 < K; k0 += KB) {
-        for (int idx = tid; idx < BM * KB; idx += 256) {
-            int r = idx / KB, k = idx - r * KB;
-            long long gr = m0 + r;
-            int gk = k0 + k;
-            As[k][r] = (gr < M && gk < K) ? A[gr * lda + gk] : T(0);
-        }
-        for (int idx = tid; idx < KB * BN; idx += 256) {
-            int k = idx / BN, c = idx - k * BN;
-            int gk = k0 + k;
-            long long gc = n0 + c;
-            Bs[k][c] = (gk < K && gc < N) ? B[(long long)gk * ldb + gc] : T(0);
-        }
-        __syncthreads();
+    T ra[LA], rb[LB];
+    auto load = [&](int k0) {
 #pragma unroll
-        for (int kk = 0; kk < KB; kk += 4) {
+        for (int u = 0; u < LA; u++) {
+            const int idx = tid + 256 * u;
+            int r, k;
+            if (TA) {
+                k = idx / BM;
+                r = idx - k * BM;
+            } else {
+                r = idx / BK;
+                k = idx - r * BK;
+            }
+            const long long gr = m0 + r;
+            const int gk = k0 + k;
+            ra[u] = (gr < M && gk < K) ? (TA ? A[(long long)gk * lda + gr] : A[gr * lda + gk]) : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < LB; u++) {
+            const int idx = tid + 256 * u;
+            const int k = idx / BN, c = idx - k * BN;
+            const int gk = k0 + k;
+            const long long gc = n0 + c;
+            rb[u] = (gk < K && gc < N) ? B[(long long)gk * ldb + gc] : T(0);
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int u = 0; u < LA; u++) {
+            const int idx = tid + 256 * u;
+            if (TA) {
+                const int k = idx / BM;
+                As[k][idx - k * BM] = ra[u];
+            } else {
+                const int r = idx / BK;
+                As[idx - r * BK][r] = ra[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < LB; u++) {
+            const int idx = tid + 256 * u;
+            const int k = idx / BN;
+            Bs[k][idx - k * BN] = rb[u];
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        store();
+        __syncthreads();
+        if (k0 + BK < K) load(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
             T a[TI], b[TJ];
 #pragma unroll
             for (int i = 0; i < TI; i++) a[i] = As[kk + (lane >> 4)][wr * WM + i * 16 + (lane & 15)];
@@ -115,22 +156,120 @@ __global__ __launch_bounds__(256) void gemm(int M, int N, int K, const T *A, lon
             }
 }
 
-template <typename T> struct Tile;
-template <> struct Tile<double> {
-    static constexpr int BM = 64, BN = 64, KB = 16;
+// Tile shapes (BM, BN, BK) the autotuner chooses among, per precision; every one covers the 64-wide panels, which the
+// in-place panel solves need.
+struct TileShape {
+    int bm, bn, bk;
 };
-template <> struct Tile<float> {
-    static constexpr int BM = 128, BN = 128, KB = 16;
+constexpr int kNumTiles = 5;
+constexpr TileShape kTiles64[kNumTiles] = {{64, 64, 16}, {128, 64, 16}, {64, 128, 16}, {128, 128, 16}, {128, 128, 8}};
+constexpr TileShape kTiles32[kNumTiles] = {{128, 128, 16}, {64, 64, 16}, {128, 64, 16}, {128, 128, 32}, {256, 128, 16}};
+
+template <typename T, int BM, int BN, int BK, bool SUB, bool TA>
+void launch_tile(hipStream_t s, int M, int N, int K, const T *A, long long lda, const T *B, long long ldb, T *C,
+                 long long ldc)
+{
+    dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
+    gemm<T, BM, BN, BK, SUB, TA><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc);
+    HIPCHECK2(hipGetLastError());
+}
+
+template <typename T, bool SUB, bool TA>
+void launch_gemm_cfg(int cfg, hipStream_t s, int M, int N, int K, const T *A, long long lda, const T *B, long long ldb,
+                     T *C, long long ldc)
+{
+    if (M <= 0 || N <= 0 || K <= 0) return;
+    if constexpr (sizeof(T) == 8) {
+        switch (cfg) {
+        case 1: return launch_tile<T, 128, 64, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        case 2: return launch_tile<T, 64, 128, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        case 3: return launch_tile<T, 128, 128, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        case 4: return launch_tile<T, 128, 128, 8, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        default: return launch_tile<T, 64, 64, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        }
+    } else {
+        switch (cfg) {
+        case 1: return launch_tile<T, 64, 64, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        case 2: return launch_tile<T, 128, 64, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        case 3: return launch_tile<T, 128, 128, 32, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        case 4: return launch_tile<T, 256, 128, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        default: return launch_tile<T, 128, 128, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        }
+    }
+}
+
+// The autotuner: the trailing update's tile shape per precision, chosen by timing every candidate (HIP events, three
+// runs after a warm-up) on the first factorization's own update shape (C -= A B with K = 64, M = N = the matrix size
+// less one panel, at most 8192) in its factor buffer before the matrix is written there; kept for the process.
+// VP_GEMM_TILE=<index> forces a candidate.
+struct GemmTuning {
+    int cfg = -1;
+    double ms[kNumTiles] = {0, 0, 0, 0, 0};
+    int size = 0;
 };
+static GemmTuning g_tune[2];  // [0] FP64, [1] FP32
+static std::mutex g_tune_mutex;
+
+template <typename T> int tuned_cfg(hipStream_t s, T *scratch, long long ld, int npad)
+{
+    const int p = sizeof(T) == 8 ? 0 : 1;
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    GemmTuning &tu = g_tune[p];
+    if (tu.cfg >= 0) return tu.cfg;
+    if (const char *env = std::getenv("VP_GEMM_TILE")) {
+        const int c = std::atoi(env);
+        tu.cfg = (c >= 0 && c < kNumTiles) ? c : 0;
+        return tu.cfg;
+    }
+    const int m = std::min(npad - 64, 8192);
+    if (m < 1024) return 0;  // too small to time well: the default shape, tuned later on a larger matrix
+    HIPCHECK2(hipMemsetAsync(scratch, 0, sizeof(T) * (size_t)ld * (m + 64), s));
+    hipEvent_t e0, e1;
+    HIPCHECK2(hipEventCreate(&e0));
+    HIPCHECK2(hipEventCreate(&e1));
+    int best = 0;
+    for (int c = 0; c < kNumTiles; c++) {
+        T *A = scratch + 64 * ld, *B = scratch + 64, *C = scratch + 64 * ld + 64;
+        launch_gemm_cfg<T, true, false>(c, s, m, m, 64, A, ld, B, ld, C, ld);
+        HIPCHECK2(hipEventRecord(e0, s));
+        for (int r = 0; r < 3; r++) launch_gemm_cfg<T, true, false>(c, s, m, m, 64, A, ld, B, ld, C, ld);
+        HIPCHECK2(hipEventRecord(e1, s));
+        HIPCHECK2(hipEventSynchronize(e1));
+        float t = 0;
+        HIPCHECK2(hipEventElapsedTime(&t, e0, e1));
+        tu.ms[c] = t / 3;
+        if (tu.ms[c] < tu.ms[best]) best = c;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tu.cfg = best;
+    tu.size = m;
+    return best;
+}
+
+std::string tuning_report()
+{
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    std::string out = "{";
+    for (int p = 0; p < 2; p++) {
+        const GemmTuning &tu = g_tune[p];
+        const TileShape *sh = p == 0 ? kTiles64 : kTiles32;
+        auto name = [&](int c) {
+            return std::to_string(sh[c].bm) + "x" + std::to_string(sh[c].bn) + "x" + std::to_string(sh[c].bk);
+        };
+        out += std::string(p ? ", " : "") + (p == 0 ? "\"fp64\": {" : "\"fp32\": {");
+        out += "\"chosen\": " + (tu.cfg < 0 ? std::string("null") : "\"" + name(tu.cfg) + "\"");
+        out += ", \"tuned_on\": " + std::to_string(tu.size) + ", \"ms\": {";
+        for (int c = 0; c < kNumTiles; c++) out += std::string(c ? ", " : "") + "\"" + name(c) + "\": " + std::to_string(tu.ms[c]);
+        out += "}}";
+    }
+    return out + "}";
+}
 
 template <typename T, bool SUB>
 void launch_gemm(hipStream_t s, int M, int N, int K, const T *A, long long lda, const T *B, long long ldb, T *C, long long ldc)
 {
-    if (M <= 0 || N <= 0 || K <= 0) return;
-    constexpr int BM = Tile<T>::BM, BN = Tile<T>::BN, KB = Tile<T>::KB;
-    dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
-    gemm<T, BM, BN, KB, SUB><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc);
-    HIPCHECK2(hipGetLastError());
+    launch_gemm_cfg<T, SUB, false>(0, s, M, N, K, A, lda, B, ldb, C, ldc);
 }
 
 // ------------------------------------------------------------------------------------------ LU pieces --------------
@@ -261,6 +400,29 @@ __global__ void border_fill(double *M, long long ld, int nq, int nb, int npad, c
         int r = n + (int)(j / npad), cc = (int)(j % npad);
         M[(long long)r * ld + cc] = r == cc ? 1.0 : 0.0;
         M[(long long)cc * ld + r] = r == cc ? 1.0 : 0.0;
+    }
+}
+
+// X (npad x nrhs, row-major, in place) <- L^-T U^-T X, the transposed solve with the same factors: forward through
+// Uᵀ (its diagonal blocks' inverses are uinvᵀ, its block (j, k) below the diagonal is U(k, j)ᵀ), then backward through
+// Lᵀ; the transposes are the GEMM's transposed-A reads, so nothing is copied.
+template <typename T> void lu_solve_t(hipStream_t s, const T *G, long long ld, int npad, const T *linv, const T *uinv, T *X, int nrhs)
+{
+    for (int k0 = 0; k0 < npad; k0 += NB) {
+        int nb = std::min(NB, npad - k0);
+        T *xk = X + (long long)k0 * nrhs;
+        launch_gemm_cfg<T, false, true>(0, s, nb, nrhs, nb, uinv + (size_t)(k0 / NB) * NB * NB, NB, xk, nrhs, xk, nrhs);
+        int m = npad - k0 - nb;
+        if (m > 0)
+            launch_gemm_cfg<T, true, true>(0, s, m, nrhs, nb, G + (long long)k0 * ld + k0 + nb, ld, xk, nrhs,
+                                           xk + (long long)nb * nrhs, nrhs);
+    }
+    int last = ((npad - 1) / NB) * NB;
+    for (int k0 = last; k0 >= 0; k0 -= NB) {
+        int nb = std::min(NB, npad - k0);
+        T *xk = X + (long long)k0 * nrhs;
+        launch_gemm_cfg<T, false, true>(0, s, nb, nrhs, nb, linv + (size_t)(k0 / NB) * NB * NB, NB, xk, nrhs, xk, nrhs);
+        if (k0 > 0) launch_gemm_cfg<T, true, true>(0, s, k0, nrhs, nb, G + (long long)k0 * ld, ld, xk, nrhs, X, nrhs);
     }
 }
 
@@ -416,13 +578,12 @@ template <typename T> T *dmalloc(size_t n)
 
 Factor::~Factor()
 {
-    delete tfac;
     for (void *p : owned) (void)hipFree(p);
 }
 
 static unsigned blocks(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
-template <typename T> void lu_factor(hipStream_t s, T *G, long long ld, int npad, T *linv, T *uinv, int *bad)
+template <typename T> void lu_factor(hipStream_t s, T *G, long long ld, int npad, T *linv, T *uinv, int *bad, int cfg)
 {
     for (int k0 = 0; k0 < npad; k0 += NB) {
         int nb = std::min(NB, npad - k0);
@@ -436,8 +597,8 @@ template <typename T> void lu_factor(hipStream_t s, T *G, long long ld, int npad
                               D + (long long)nb * ld, ld);
         // U12 = L11^-1 A12 (in place: one workgroup row covers the panel's NB rows)
         launch_gemm<T, false>(s, nb, m, nb, linv + (size_t)(k0 / NB) * NB * NB, NB, D + nb, ld, D + nb, ld);
-        // A22 -= L21 U12
-        launch_gemm<T, true>(s, m, m, nb, D + (long long)nb * ld, ld, D + nb, ld, D + (long long)nb * ld + nb, ld);
+        // A22 -= L21 U12 (the autotuned tile shape)
+        launch_gemm_cfg<T, true, false>(cfg, s, m, m, nb, D + (long long)nb * ld, ld, D + nb, ld, D + (long long)nb * ld + nb, ld);
     }
 }
 
@@ -571,17 +732,19 @@ void Factor::build(Ctx *c, const double *A, long long lda, const vp_constraints 
         linv64 = dmalloc<double>(nblk);
         uinv64 = dmalloc<double>(nblk);
         owned.insert(owned.end(), {G64, linv64, uinv64});
+        const int cfg = tuned_cfg<double>(s, G64, ld, npad);
         rbt_both<double><<<blocks(h * h), 256, 0, s>>>(M, ld, npad, d_ru, d_rv, G64, ld);
         HIPCHECK2(hipGetLastError());
-        lu_factor<double>(s, G64, ld, npad, linv64, uinv64, dbad);
+        lu_factor<double>(s, G64, ld, npad, linv64, uinv64, dbad, cfg);
     } else {
         G32 = dmalloc<float>((size_t)ld * npad);
         linv32 = dmalloc<float>(nblk);
         uinv32 = dmalloc<float>(nblk);
         owned.insert(owned.end(), {G32, linv32, uinv32});
+        const int cfg = tuned_cfg<float>(s, G32, ld, npad);
         rbt_both<float><<<blocks(h * h), 256, 0, s>>>(M, ld, npad, d_ru, d_rv, G32, ld);
         HIPCHECK2(hipGetLastError());
-        lu_factor<float>(s, G32, ld, npad, linv32, uinv32, dbad);
+        lu_factor<float>(s, G32, ld, npad, linv32, uinv32, dbad, cfg);
     }
     int hbad = 0;
     HIPCHECK2(hipMemcpyAsync(&hbad, dbad, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -609,27 +772,35 @@ void Factor::ensure_ws(int nrhs)
     ws_nrhs = nrhs;
 }
 
-// y = M^-1 r for the current workspace block (in: Y holds r, out: Y holds M^-1 r), npad x nrhs
-void Factor::apply_inverse(int nrhs)
+// y = M^-1 r, or M^-T r with trans, for the current workspace block (in: Y holds r, out: Y holds the product),
+// npad x nrhs.  M^-1 = B_v G^-1 B_uᵀ for G = B_uᵀ M B_v = L U, so M^-T = B_u G^-T B_vᵀ: the same kernels with the
+// two butterflies' roles swapped and the transposed triangular solves.
+void Factor::apply_inverse(int nrhs, bool trans)
 {
     hipStream_t s = ctx->stream;
     long long h = npad / 4;
-    rbt_rows<<<blocks(h * nrhs), 256, 0, s>>>(Y, npad, nrhs, d_ru, R);
+    rbt_rows<<<blocks(h * nrhs), 256, 0, s>>>(Y, npad, nrhs, trans ? d_rv : d_ru, R);
     HIPCHECK2(hipGetLastError());
     if (mode == 0) {
-        lu_solve<double>(s, G64, ld, npad, linv64, uinv64, R, nrhs);
+        if (trans)
+            lu_solve_t<double>(s, G64, ld, npad, linv64, uinv64, R, nrhs);
+        else
+            lu_solve<double>(s, G64, ld, npad, linv64, uinv64, R, nrhs);
     } else {
         convert<double, float><<<blocks((long long)npad * nrhs), 256, 0, s>>>(R, Xf, (long long)npad * nrhs);
-        lu_solve<float>(s, G32, ld, npad, linv32, uinv32, Xf, nrhs);
+        if (trans)
+            lu_solve_t<float>(s, G32, ld, npad, linv32, uinv32, Xf, nrhs);
+        else
+            lu_solve<float>(s, G32, ld, npad, linv32, uinv32, Xf, nrhs);
         convert<float, double><<<blocks((long long)npad * nrhs), 256, 0, s>>>(Xf, R, (long long)npad * nrhs);
     }
-    rbt_apply_v<<<blocks(h * nrhs), 256, 0, s>>>(R, npad, nrhs, d_rv, Y);
+    rbt_apply_v<<<blocks(h * nrhs), 256, 0, s>>>(R, npad, nrhs, trans ? d_ru : d_rv, Y);
     HIPCHECK2(hipGetLastError());
 }
 
-// Solve M X = Bh (n x nrhs row-major host or device) with FP64 iterative refinement (LAPACK dsgesv's stopping test:
-// |r|_inf <= |x|_inf |M|_inf eps sqrt(n), at most max_iter refinements).
-int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int max_iter, double *resid_out)
+// Solve M X = Bh, or Mᵀ X = Bh with trans (n x nrhs row-major host or device), with FP64 iterative refinement (LAPACK
+// dsgesv's stopping test: |r|_inf <= |x|_inf |op(M)|_inf eps sqrt(n), at most max_iter refinements).
+int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int max_iter, double *resid_out, bool trans)
 {
     hipStream_t s = ctx->stream;
     ensure_ws(nrhs);
@@ -637,9 +808,18 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
     HIPCHECK2(hipMemsetAsync(B, 0, sizeof(double) * sz, s));
     HIPCHECK2(hipMemcpyAsync(B, Bin, sizeof(double) * nn, device_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     HIPCHECK2(hipMemcpyAsync(Y, B, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
-    apply_inverse(nrhs);
+    apply_inverse(nrhs, trans);
     HIPCHECK2(hipMemcpyAsync(X, Y, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
-    if (anorm_inf < 0) {
+    if (trans && anorm1 <= 0) {
+        double *dcol = dmalloc<double>(npad);
+        col_abs_sum<<<blocks(n), 256, 0, s>>>(M, ld, n, dcol);
+        std::vector<double> hcol(n);
+        HIPCHECK2(hipMemcpyAsync(hcol.data(), dcol, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+        HIPCHECK2(hipStreamSynchronize(s));
+        (void)hipFree(dcol);
+        anorm1 = *std::max_element(hcol.begin(), hcol.end());
+    }
+    if (!trans && anorm_inf < 0) {
         double *dpart = dmalloc<double>(1024);
         row_abs_max<<<1024, 256, 0, s>>>(M, ld, n, dpart);
         std::vector<double> part(1024);
@@ -648,14 +828,18 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
         (void)hipFree(dpart);
         anorm_inf = *std::max_element(part.begin(), part.end());
     }
-    const double eps = 1.1102230246251565e-16, tolf = anorm_inf * eps * std::sqrt((double)n);
+    const double opnorm = trans ? anorm1 : anorm_inf;  // |Mᵀ|_inf = |M|_1
+    const double eps = 1.1102230246251565e-16, tolf = opnorm * eps * std::sqrt((double)n);
     std::vector<double> hn(2 * (size_t)nrhs);
     int it = 0;
     double worst = 0.0;
     for (;; it++) {
         // R = B - M X (FP64, MFMA)
         HIPCHECK2(hipMemcpyAsync(Y, B, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
-        launch_gemm<double, true>(s, npad, nrhs, npad, M, ld, X, nrhs, Y, nrhs);
+        if (trans)
+            launch_gemm_cfg<double, true, true>(0, s, npad, nrhs, npad, M, ld, X, nrhs, Y, nrhs);
+        else
+            launch_gemm<double, true>(s, npad, nrhs, npad, M, ld, X, nrhs, Y, nrhs);
         colmax2<<<nrhs, 256, 0, s>>>(Y, X, n, nrhs, nrm);
         HIPCHECK2(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, s));
         HIPCHECK2(hipStreamSynchronize(s));
@@ -663,10 +847,10 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
         worst = 0.0;
         for (int c = 0; c < nrhs; c++) {
             if (hn[2 * c] > hn[2 * c + 1] * tolf) done = false;
-            worst = std::max(worst, hn[2 * c] / std::max(hn[2 * c + 1] * anorm_inf, 1e-300));
+            worst = std::max(worst, hn[2 * c] / std::max(hn[2 * c + 1] * opnorm, 1e-300));
         }
         if (done || it >= max_iter) break;
-        apply_inverse(nrhs);
+        apply_inverse(nrhs, trans);
         axpy_inplace<<<blocks((long long)sz), 256, 0, s>>>(X, Y, (long long)sz);
         HIPCHECK2(hipGetLastError());
     }
@@ -678,15 +862,9 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
 }
 
 // Hager-Higham estimate of |M^-1|_1 (the iteration of LAPACK dlacn2, which gecon runs on the reference's factors).
-// It needs products with M^-1 and M^-T; M^-T comes from a second randomized factorization, of Mᵀ, made on first use.
+// It needs products with M^-1 and M^-T, both from the one factorization.
 double Factor::cond_estimate()
 {
-    hipStream_t s = ctx->stream;
-    if (!Mt) {
-        Mt = dmalloc<double>((size_t)ld * npad);
-        owned.push_back(Mt);
-        transpose_into(M, Mt);
-    }
     std::vector<double> x(n, 1.0 / n), y(n), z(n);
     double est = 0.0;
     int jlast = -1;
@@ -700,7 +878,7 @@ double Factor::cond_estimate()
         }
         est = ny;
         for (int i = 0; i < n; i++) z[i] = y[i] >= 0 ? 1.0 : -1.0;
-        solve_transposed(z.data(), x.data());  // x = M^-T z
+        solve(z.data(), 1, x.data(), false, 30, nullptr, true);  // x = M^-T z
         int j = 0;
         for (int i = 1; i < n; i++)
             if (std::fabs(x[i]) > std::fabs(x[j])) j = i;
@@ -717,44 +895,7 @@ double Factor::cond_estimate()
     for (double v : y) t += std::fabs(v);
     t = 2 * t / (3.0 * n);
     est = std::max(est, t);
-    (void)s;
     return est;
-}
-
-__global__ void transpose_kernel(const double *in, double *out, long long ld, int npad)
-{
-    __shared__ double t[32][33];
-    int bx = blockIdx.x * 32, by = blockIdx.y * 32;
-    int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    for (int r = ty; r < 32; r += 8) t[r][tx] = in[(long long)(by + r) * ld + bx + tx];
-    __syncthreads();
-    for (int r = ty; r < 32; r += 8) out[(long long)(bx + r) * ld + by + tx] = t[tx][r];
-}
-
-void Factor::transpose_into(const double *in, double *out)
-{
-    dim3 g(npad / 32, npad / 32);
-    transpose_kernel<<<g, 256, 0, ctx->stream>>>(in, out, ld, npad);
-    HIPCHECK2(hipGetLastError());
-}
-
-// x = M^-T z with the factorization of Mᵀ (same butterflies-and-LU scheme, FP64 refinement against Mᵀ)
-void Factor::solve_transposed(const double *z, double *x) { solve_t(z, 1, x, nullptr); }
-
-// Solve Mᵀ X = B (n x nrhs row-major host arrays) with the factorization of Mᵀ and FP64 refinement against Mᵀ
-int Factor::solve_t(const double *Bin, int nrhs, double *Xout, double *resid)
-{
-    if (!Mt) {
-        Mt = dmalloc<double>((size_t)ld * npad);
-        owned.push_back(Mt);
-        transpose_into(M, Mt);
-    }
-    if (!tfac) {
-        tfac = new Factor();
-        tfac->ctx = ctx;
-        tfac->init_from_dense(Mt, ld, n, npad, mode, anorm1);
-    }
-    return tfac->solve(Bin, nrhs, Xout, false, 30, resid);
 }
 
 // Factor a dense bordered matrix already on the device (n x n in an npad x npad row-major buffer).
@@ -789,22 +930,39 @@ void Factor::init_from_dense(const double *Md, long long ldd, int n_, int npad_,
         linv64 = dmalloc<double>(nblk);
         uinv64 = dmalloc<double>(nblk);
         owned.insert(owned.end(), {G64, linv64, uinv64});
+        const int cfg = tuned_cfg<double>(s, G64, ld, npad);
         rbt_both<double><<<blocks(h * h), 256, 0, s>>>(M, ld, npad, d_ru, d_rv, G64, ld);
-        lu_factor<double>(s, G64, ld, npad, linv64, uinv64, dbad);
+        lu_factor<double>(s, G64, ld, npad, linv64, uinv64, dbad, cfg);
     } else {
         G32 = dmalloc<float>((size_t)ld * npad);
         linv32 = dmalloc<float>(nblk);
         uinv32 = dmalloc<float>(nblk);
         owned.insert(owned.end(), {G32, linv32, uinv32});
+        const int cfg = tuned_cfg<float>(s, G32, ld, npad);
         rbt_both<float><<<blocks(h * h), 256, 0, s>>>(M, ld, npad, d_ru, d_rv, G32, ld);
-        lu_factor<float>(s, G32, ld, npad, linv32, uinv32, dbad);
+        lu_factor<float>(s, G32, ld, npad, linv32, uinv32, dbad, cfg);
     }
     HIPCHECK2(hipGetLastError());
 }
 
 // ------------------------------------------------------------------------------------------ test hooks ------------
 
-void gemm_host(hipStream_t s, int prec, int sub, int M_, int N_, int K_, const double *A, const double *B, double *C)
+template <typename T> void gemm_flags(hipStream_t s, int flags, int M_, int N_, int K_, const T *A, const T *B, T *C)
+{
+    const bool sub = flags & 1, ta = flags & 2;
+    const int tile = (flags >> 4) & 15;
+    const long long lda = ta ? M_ : K_;
+    if (sub && ta)
+        launch_gemm_cfg<T, true, true>(tile, s, M_, N_, K_, A, lda, B, N_, C, N_);
+    else if (sub)
+        launch_gemm_cfg<T, true, false>(tile, s, M_, N_, K_, A, lda, B, N_, C, N_);
+    else if (ta)
+        launch_gemm_cfg<T, false, true>(tile, s, M_, N_, K_, A, lda, B, N_, C, N_);
+    else
+        launch_gemm_cfg<T, false, false>(tile, s, M_, N_, K_, A, lda, B, N_, C, N_);
+}
+
+void gemm_host(hipStream_t s, int prec, int flags, int M_, int N_, int K_, const double *A, const double *B, double *C)
 {
     size_t na = (size_t)M_ * K_, nbb = (size_t)K_ * N_, nc = (size_t)M_ * N_;
     double *dA = dmalloc<double>(na), *dB = dmalloc<double>(nbb), *dC = dmalloc<double>(nc);
@@ -812,19 +970,13 @@ void gemm_host(hipStream_t s, int prec, int sub, int M_, int N_, int K_, const d
     HIPCHECK2(hipMemcpyAsync(dB, B, nbb * 8, hipMemcpyHostToDevice, s));
     HIPCHECK2(hipMemcpyAsync(dC, C, nc * 8, hipMemcpyHostToDevice, s));
     if (prec == 0) {
-        if (sub)
-            launch_gemm<double, true>(s, M_, N_, K_, dA, K_, dB, N_, dC, N_);
-        else
-            launch_gemm<double, false>(s, M_, N_, K_, dA, K_, dB, N_, dC, N_);
+        gemm_flags<double>(s, flags, M_, N_, K_, dA, dB, dC);
     } else {
         float *fA = dmalloc<float>(na), *fB = dmalloc<float>(nbb), *fC = dmalloc<float>(nc);
         convert<double, float><<<blocks(na), 256, 0, s>>>(dA, fA, na);
         convert<double, float><<<blocks(nbb), 256, 0, s>>>(dB, fB, nbb);
         convert<double, float><<<blocks(nc), 256, 0, s>>>(dC, fC, nc);
-        if (sub)
-            launch_gemm<float, true>(s, M_, N_, K_, fA, K_, fB, N_, fC, N_);
-        else
-            launch_gemm<float, false>(s, M_, N_, K_, fA, K_, fB, N_, fC, N_);
+        gemm_flags<float>(s, flags, M_, N_, K_, fA, fB, fC);
         convert<float, double><<<blocks(nc), 256, 0, s>>>(fC, dC, nc);
         HIPCHECK2(hipStreamSynchronize(s));
         (void)hipFree(fA);

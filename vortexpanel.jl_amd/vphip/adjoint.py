@@ -172,10 +172,7 @@ class WingAdjoint:
         F = self.F
         R = np.zeros(F.n)
         R[:F.nq] = dJdq
-        X = np.zeros(F.n)
-        it, res = C.c_int(), C.c_double()
-        _check(self.ctx.h, self.ctx.lib.vp_factor_solve_transposed(F.h, L.ptr(L.f64(R)), 1, L.ptr(X), C.byref(it),
-                                                                   C.byref(res)))
+        X = F.solve_bordered_t(R)
         return F.C @ X[:F.nq]
 
     def _backward(self, lam):

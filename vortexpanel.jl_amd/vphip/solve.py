@@ -81,6 +81,18 @@ class GalerkinFactorization:
         self.last_iters, self.last_resid = it.value, res.value
         return X[:, 0] if one else X
 
+    def solve_bordered_t(self, R):
+        """X = M^-T R for R (n,) or (n, nrhs), through the same factors read transposed"""
+        R = np.asarray(R, dtype=np.float64)
+        one = R.ndim == 1
+        R2 = L.f64(R.reshape(self.n, -1))
+        X = np.zeros_like(R2)
+        it, res = C.c_int(), C.c_double()
+        _check(self.ctx.h, self.ctx.lib.vp_factor_solve_transposed(self.h, L.ptr(R2), R2.shape[1], L.ptr(X),
+                                                                   C.byref(it), C.byref(res)))
+        self.last_iters, self.last_resid = it.value, res.value
+        return X[:, 0] if one else X
+
     def close(self):
         if self.h:
             self.ctx.lib.vp_factor_destroy(self.h)
