@@ -33,8 +33,26 @@ __global__ void slot_nodes(const int *__restrict__ corner_slot, const int *__res
         for (int k = 0; k < 3; k++) out[corner_slot[3 * t + k]] = nodes[3 * t + k];
 }
 
+// A far tile's LDS gradient sums (per local test slot and local source column) added to the node gradients
+__device__ __forceinline__ void far_tile_flush(const AsmParams &P, double (*GT)[3], double (*GU)[3], int tt, int st,
+                                               const int *tt_rows, const int *tt_nrow, const int *ts_cols,
+                                               const int *ts_ncol, const int *slot_node, double *G)
+{
+    const int nrow = tt_nrow[tt], ncol = ts_ncol[st];
+    for (int i = threadIdx.x; i < nrow * 3; i += blockDim.x) {
+        const double v = GT[i / 3][i % 3];
+        if (v != 0.0) unsafeAtomicAdd(&G[3 * slot_node[tt_rows[tt * VMAX + i / 3]] + i % 3], v);
+    }
+    for (int i = threadIdx.x; i < ncol * 3; i += blockDim.x) {
+        const double v = GU[i / 3][i % 3];
+        if (v != 0.0) unsafeAtomicAdd(&G[3 * col_node(ts_cols[st * UMAX + i / 3], P.nslot, P.nn, slot_node) + i % 3], v);
+    }
+}
+
 // The far tier (the forward's tiles and far test, far_pairs): lane = source element, wave = test triangle; the
-// gradients are summed per local test slot and local source column in LDS and flushed per tile.
+// gradients are summed per local test slot and local source column in LDS and flushed per tile.  Pairs with a
+// closed-form point are queued in LDS as (test index in the tile) x TS + lane and written out as one contiguous segment
+// per tile (exact_segs: tile, first entry, count), for adj_far_exact, which keeps the tile's LDS sums.
 __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, const TestFar *__restrict__ tf,
                                                      const Elem *__restrict__ el, const int *__restrict__ tt_elems,
                                                      const uint8_t *__restrict__ tt_loc, const int *__restrict__ tt_rows,
@@ -43,15 +61,19 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
                                                      const int *__restrict__ ts_ncol, const int *__restrict__ src_cols,
                                                      const int *__restrict__ slot_node, const double *__restrict__ lam,
                                                      const double *__restrict__ mu, double vx, double vy, double vz,
-                                                     double *__restrict__ G, int2 *__restrict__ exact_list,
-                                                     unsigned int *__restrict__ exact_count, unsigned int exact_cap)
+                                                     double *__restrict__ G, unsigned short *__restrict__ exact_list,
+                                                     int4 *__restrict__ exact_segs, unsigned int *__restrict__ exact_count,
+                                                     unsigned int exact_cap)
 {
     __shared__ double GT[VMAX][3];
     __shared__ double GU[UMAX][3];
+    __shared__ unsigned short exq[TT_MAX * TS];
+    __shared__ unsigned int nexq, exbase;
     const int tt = blockIdx.x, st = blockIdx.y;
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < VMAX * 3; i += blockDim.x) GT[i / 3][i % 3] = 0.0;
     for (int i = threadIdx.x; i < UMAX * 3; i += blockDim.x) GU[i / 3][i % 3] = 0.0;
+    if (threadIdx.x == 0) nexq = 0;
     const V3<double> Vinf = mk3(vx, vy, vz);
     const int s = ts_elems[st * TS + lane];
     const bool valid = s >= 0;
@@ -87,7 +109,14 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
                 multi = multi && cls_dot(r0, r0) > fd * fd;
             }
         }
-        append_pair(far && !multi, t, s, exact_list, exact_count, exact_cap);
+        const bool ex = far && !multi;
+        const unsigned long long em = __ballot(ex);
+        if (em) {
+            unsigned int b0 = 0;
+            if (lane == 0) b0 = atomicAdd(&nexq, (unsigned int)__popcll(em));
+            b0 = __shfl(b0, 0);
+            if (ex) exq[b0 + __popcll(em & ((1ull << lane) - 1ull))] = (unsigned short)(i * TS + lane);
+        }
         if (!multi) continue;
         const Elem &T = el[t];
         const V3<double> XT[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
@@ -123,39 +152,91 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
         }
     }
     __syncthreads();
-    const int nrow = tt_nrow[tt], ncol = ts_ncol[st];
-    for (int i = threadIdx.x; i < nrow * 3; i += blockDim.x) {
-        const double v = GT[i / 3][i % 3];
-        if (v != 0.0) unsafeAtomicAdd(&G[3 * slot_node[tt_rows[tt * VMAX + i / 3]] + i % 3], v);
+    if (threadIdx.x == 0 && nexq) {
+        exbase = atomicAdd(exact_count, nexq);
+        const unsigned int sg = atomicAdd(exact_count + 1, 1u);
+        exact_segs[sg] = make_int4(tt, st, (int)exbase, (int)nexq);  // at most one segment per tile: in bounds
     }
-    for (int i = threadIdx.x; i < ncol * 3; i += blockDim.x) {
-        const double v = GU[i / 3][i % 3];
-        if (v != 0.0) unsafeAtomicAdd(&G[3 * col_node(ts_cols[st * UMAX + i / 3], P.nslot, P.nn, slot_node) + i % 3], v);
-    }
+    __syncthreads();
+    for (unsigned int j = threadIdx.x; j < nexq; j += blockDim.x)
+        if (exbase + j < exact_cap) exact_list[exbase + j] = exq[j];
+    far_tile_flush(P, GT, GU, tt, st, tt_rows, tt_nrow, ts_cols, ts_ncol, slot_node, G);
 }
 
-// Far pairs with a closed-form point (within kernel_far diameters of the source): one thread per pair, the three far
-// points by tri_point_grad (dual numbers for the closed form).
+// Far pairs with a closed-form point (within kernel_far diameters of the source), one workgroup per tile segment
+// queued by adj_far_pairs: thread per pair, the three far points by tri_point_grad, the gradients summed in LDS per
+// local test slot and source column as in adj_far_pairs and flushed once.
 __global__ __launch_bounds__(256) void adj_far_exact(AsmParams P, RuleTab rt, const TestFar *__restrict__ tf,
-                                                     const Elem *__restrict__ el, const int *__restrict__ src_cols,
-                                                     const int *__restrict__ nodes, const int2 *__restrict__ list,
-                                                     unsigned int npairs, const double *__restrict__ lam,
-                                                     const double *__restrict__ mu, const int *__restrict__ corner_slot,
-                                                     double vx, double vy, double vz, double *__restrict__ G)
+                                                     const Elem *__restrict__ el, const int *__restrict__ tt_elems,
+                                                     const uint8_t *__restrict__ tt_loc, const int *__restrict__ tt_rows,
+                                                     const int *__restrict__ tt_nrow, const int *__restrict__ ts_elems,
+                                                     const uint8_t *__restrict__ ts_loc, const int *__restrict__ ts_cols,
+                                                     const int *__restrict__ ts_ncol, const int *__restrict__ src_cols,
+                                                     const int *__restrict__ slot_node, const double *__restrict__ lam,
+                                                     const double *__restrict__ mu, double vx, double vy, double vz,
+                                                     double *__restrict__ G, const unsigned short *__restrict__ list,
+                                                     const int4 *__restrict__ segs)
 {
-    const unsigned int pi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pi >= npairs) return;
-    const int t = list[pi].x, s = list[pi].y;
-    const TestFar &F = tf[t];
-    const Elem &T = el[t], &S = el[s];
-    AdjArgs aa{lam, mu, G, nodes, vx, vy, vz};
-    AdjPair ap;
-    adj_pair_begin(aa, P.nt, S, s, corner_slot + 3 * t, src_cols + 3 * s, ld3(F.n), ap);
-    const V3<double> XT[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
+    __shared__ double GT[VMAX][3];
+    __shared__ double GU[UMAX][3];
+    const int4 sg = segs[blockIdx.x];
+    const int tt = sg.x, st = sg.y;
+    for (int i = threadIdx.x; i < VMAX * 3; i += blockDim.x) GT[i / 3][i % 3] = 0.0;
+    for (int i = threadIdx.x; i < UMAX * 3; i += blockDim.x) GU[i / 3][i % 3] = 0.0;
+    __syncthreads();
+    const V3<double> Vinf = mk3(vx, vy, vz);
     const int off = rt.off[R_FAR];
 #pragma unroll 1
-    for (int q = 0; q < 3; q++) adj_point(ap, XT, S, 0, P.kernel_far, rt.w[off + q], rt.l + 3 * (off + q), P.test);
-    adj_pair_end(aa, P.nt, T, XT, t, s, nodes, ap);
+    for (int j = threadIdx.x; j < sg.w; j += blockDim.x) {
+        const int code = list[sg.z + j], i = code / TS, ls = code - i * TS;
+        const int t = tt_elems[tt * TT_MAX + i], s = ts_elems[st * TS + ls];
+        const TestFar &F = tf[t];
+        const Elem &S = el[s], &T = el[t];
+        V3<double> XS[3];
+        double muS[3];
+        int uloc[3], vl[3];
+        double la[3];
+        for (int k = 0; k < 3; k++) {
+            XS[k] = ld3(S.v[k]);
+            muS[k] = mu[src_cols[3 * s + k]];
+            uloc[k] = ts_loc[(st * TS + ls) * 3 + k];
+            vl[k] = tt_loc[(tt * TT_MAX + i) * 3 + k];
+            la[k] = lam[tt_rows[tt * VMAX + vl[k]]];
+        }
+        const V3<double> Gam = sheet_of(XS, muS);
+        const double sigS = s < P.nt ? -dot(Vinf, ld3(S.n)) : 0.0;
+        const V3<double> nT = ld3(F.n);
+        const V3<double> Q = sigS * nT + cross(nT, Gam);
+        PairSums p;
+        p.Sv = mk3(0.0, 0.0, 0.0);
+        for (int k = 0; k < 3; k++) p.GS[k] = p.GX[k] = mk3(0.0, 0.0, 0.0);
+#pragma unroll 1
+        for (int q = 0; q < 3; q++) {
+            const double a = (la[0] * F.wpsi[q][0] + la[1] * F.wpsi[q][1] + la[2] * F.wpsi[q][2]) / F.area;
+            V3<double> wv, g[3];
+            tri_point_grad(XS, S, ld3(F.x[q]), Q, 0, P.kernel_far, wv, g);
+            const V3<double> gx = -1.0 * ((g[0] + g[1]) + g[2]);
+            const double *l = rt.l + 3 * (off + q);
+            p.Sv = p.Sv + a * wv;
+            for (int k = 0; k < 3; k++) {
+                p.GS[k] = p.GS[k] + a * g[k];
+                p.GX[k] = p.GX[k] + (a * l[k]) * gx;
+            }
+        }
+        const V3<double> XT[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
+        V3<double> gT[3], gS[3];
+        finish_tri_pair(XT, nT, F.area, XS, muS, sigS, Gam, s < P.nt, Vinf, Q, p, gT, gS);
+        for (int k = 0; k < 3; k++) {
+            atomicAdd(&GT[vl[k]][0], gT[k].x);
+            atomicAdd(&GT[vl[k]][1], gT[k].y);
+            atomicAdd(&GT[vl[k]][2], gT[k].z);
+            atomicAdd(&GU[uloc[k]][0], gS[k].x);
+            atomicAdd(&GU[uloc[k]][1], gS[k].y);
+            atomicAdd(&GU[uloc[k]][2], gS[k].z);
+        }
+    }
+    __syncthreads();
+    far_tile_flush(P, GT, GU, tt, st, tt_rows, tt_nrow, ts_cols, ts_ncol, slot_node, G);
 }
 
 // Far filament pairs (seg_pairs' far test and far segment_block): one thread per (test triangle, filament).
@@ -336,29 +417,33 @@ void Plan::adjoint(const double *lam_h, const double *mu_h, const double *vinf, 
     HIPCHECK(hipMemcpyAsync(hst, d_stats, sizeof(hst), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     const unsigned int ecap = (unsigned int)std::min<unsigned long long>(hst[12] + 1024, 1ull << 31);
-    int2 *d_exact = dalloc<int2>(ecap);
-    unsigned int *d_ecount = dalloc<unsigned int>(1);
-    HIPCHECK(hipMemsetAsync(d_ecount, 0, sizeof(unsigned int), st));
+    unsigned short *d_exact = dalloc<unsigned short>(ecap);
+    int4 *d_esegs = dalloc<int4>((size_t)ntt * nts);
+    unsigned int *d_ecount = dalloc<unsigned int>(2);
+    HIPCHECK(hipMemsetAsync(d_ecount, 0, 2 * sizeof(unsigned int), st));
     adj_far_pairs<<<dim3(ntt, nts), 256, 0, st>>>(P, rt, d_tf, d_el, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc, d_tsn,
-                                                  d_cols, d_slotnode, d_lam, d_mu, vx, vy, vz, d_G, d_exact, d_ecount,
-                                                  ecap);
+                                                  d_cols, d_slotnode, d_lam, d_mu, vx, vy, vz, d_G, d_exact, d_esegs,
+                                                  d_ecount, ecap);
     HIPCHECK(hipGetLastError());
-    unsigned int nexact = 0;
-    HIPCHECK(hipMemcpyAsync(&nexact, d_ecount, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    unsigned int ecnt[2] = {0, 0};
+    HIPCHECK(hipMemcpyAsync(ecnt, d_ecount, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    if (nexact > ecap) {
+    auto free_exact = [&] {
         (void)hipFree(d_exact);
+        (void)hipFree(d_esegs);
         (void)hipFree(d_ecount);
+    };
+    if (ecnt[0] > ecap) {
+        free_exact();
         throw std::runtime_error("adjoint: more closed-form far pairs than the forward's closed-form far points");
     }
-    if (nexact) {
-        adj_far_exact<<<(nexact + 255) / 256, 256, 0, st>>>(P, rt, d_tf, d_el, d_cols, d_nodes, d_exact, nexact, d_lam,
-                                                            d_mu, d_cs, vx, vy, vz, d_G);
+    if (ecnt[1]) {
+        adj_far_exact<<<ecnt[1], 256, 0, st>>>(P, rt, d_tf, d_el, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc, d_tsn,
+                                               d_cols, d_slotnode, d_lam, d_mu, vx, vy, vz, d_G, d_exact, d_esegs);
         HIPCHECK(hipGetLastError());
     }
     HIPCHECK(hipStreamSynchronize(st));
-    (void)hipFree(d_exact);
-    (void)hipFree(d_ecount);
+    free_exact();
     if (nseg) {
         const long long np = (long long)nt * nseg;
         adj_seg_far<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(P, rt, d_tf, d_el, d_cs, d_nodes, d_seg, d_segcol, d_lam,
