@@ -69,6 +69,10 @@ def test_adjoint_against_oracle_differences(case, objective, corner, tol):
     fd = (loads(g.x, ALPHA + ha) - loads(g.x, ALPHA - ha)) / (2 * ha)
     print(f"  alpha: fd {fd:.10e} adjoint {dalpha:.10e}", flush=True)
     assert abs(fd - dalpha) < 1e-6 * abs(fd) + 1e-9
+    # the backward pass's lamᵀ db/dalpha against lamᵀ of a second assembly's b along dV∞/dalpha
+    ref = float(adj.last_lam @ adj._db_dalpha())
+    assert abs(adj.last_lam_db - ref) < 1e-12 * np.abs(adj.last_lam).sum() * np.abs(adj._db_dalpha()).max(), \
+        (adj.last_lam_db, ref)
     adj.close()
 
 

@@ -63,6 +63,19 @@ def test_wing_with_wake():
     _compare(g, dm, V, w=w)
 
 
+def test_nonfar_list_regrowth(monkeypatch):
+    """Work lists budgeted too small: the far kernels' enumerate-only pass refills the grown non-far list without
+    adding the far blocks twice, and the rule generator refills the grown rule-piece lists without counting its
+    pairs twice (same entries, tiers and counts)"""
+    monkeypatch.setenv("VP_NONFAR_CAP", "100")
+    monkeypatch.setenv("VP_LEAF_CAP", "3")
+    g = mesh.naca_wing(mesh.WingSpec(code="0012", n_around=16, n_span=6, aspect=4.0))
+    dm = dofmap.build_dofmap(g)
+    V = np.array([np.cos(np.radians(5)), 0, np.sin(np.radians(5))])
+    w = wk.build_wake(g, dm, V, wk.WakeOptions(length=2.0))
+    _compare(g, dm, V, w=w)
+
+
 def test_options_deeper_rules():
     g = mesh.icosphere(1)
     _compare(g, dofmap.build_dofmap(g), VG, api.AssemblyOptions(far_ratio=8.0, near_ratio=4.0, near_depth=5,

@@ -61,6 +61,7 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
                                                      const int *__restrict__ ts_ncol, const int *__restrict__ src_cols,
                                                      const int *__restrict__ slot_node, const double *__restrict__ lam,
                                                      const double *__restrict__ mu, double vx, double vy, double vz,
+                                                     V3<double> dV, double *__restrict__ bdot,
                                                      double *__restrict__ G, unsigned short *__restrict__ exact_list,
                                                      int4 *__restrict__ exact_segs, unsigned int *__restrict__ exact_count,
                                                      unsigned int exact_cap)
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
     const int s = ts_elems[st * TS + lane];
     const bool valid = s >= 0;
     V3<double> XS[3], Gam = mk3(0.0, 0.0, 0.0), sc = mk3(0.0, 0.0, 0.0);
-    double muS[3] = {0, 0, 0}, sigS = 0.0, sdiam = 0.0;
+    double muS[3] = {0, 0, 0}, sigS = 0.0, sdiam = 0.0, dVn = 0.0;
     int uloc[3] = {0, 0, 0};
     if (valid) {
         const Elem &S = el[s];
@@ -89,9 +90,11 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
         }
         Gam = sheet_of(XS, muS);
         sigS = s < P.nt ? -dot(Vinf, ld3(S.n)) : 0.0;
+        dVn = s < P.nt ? dot(dV, ld3(S.n)) : 0.0;
         sc = ld3(S.c);
         sdiam = S.diam;
     }
+    double bdsum = 0.0;
     __syncthreads();
     const int off = rt.off[R_FAR];
     for (int i = wave; i < TT_MAX; i += 4) {
@@ -142,6 +145,7 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
         }
         V3<double> gT[3], gS[3];
         finish_tri_pair(XT, nT, F.area, XS, muS, sigS, Gam, s < P.nt, Vinf, Q, p, gT, gS);
+        bdsum += dVn * F.area * dot(p.Sv, nT);
         for (int k = 0; k < 3; k++) {
             atomicAdd(&GT[vl[k]][0], gT[k].x);
             atomicAdd(&GT[vl[k]][1], gT[k].y);
@@ -160,6 +164,7 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
     __syncthreads();
     for (unsigned int j = threadIdx.x; j < nexq; j += blockDim.x)
         if (exbase + j < exact_cap) exact_list[exbase + j] = exq[j];
+    block_add(bdsum, bdot);
     far_tile_flush(P, GT, GU, tt, st, tt_rows, tt_nrow, ts_cols, ts_ncol, slot_node, G);
 }
 
@@ -174,6 +179,7 @@ __global__ __launch_bounds__(256) void adj_far_exact(AsmParams P, RuleTab rt, co
                                                      const int *__restrict__ ts_ncol, const int *__restrict__ src_cols,
                                                      const int *__restrict__ slot_node, const double *__restrict__ lam,
                                                      const double *__restrict__ mu, double vx, double vy, double vz,
+                                                     V3<double> dV, double *__restrict__ bdot,
                                                      double *__restrict__ G, const unsigned short *__restrict__ list,
                                                      const int4 *__restrict__ segs)
 {
@@ -186,6 +192,7 @@ __global__ __launch_bounds__(256) void adj_far_exact(AsmParams P, RuleTab rt, co
     __syncthreads();
     const V3<double> Vinf = mk3(vx, vy, vz);
     const int off = rt.off[R_FAR];
+    double bdsum = 0.0;
 #pragma unroll 1
     for (int j = threadIdx.x; j < sg.w; j += blockDim.x) {
         const int code = list[sg.z + j], i = code / TS, ls = code - i * TS;
@@ -226,6 +233,7 @@ __global__ __launch_bounds__(256) void adj_far_exact(AsmParams P, RuleTab rt, co
         const V3<double> XT[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
         V3<double> gT[3], gS[3];
         finish_tri_pair(XT, nT, F.area, XS, muS, sigS, Gam, s < P.nt, Vinf, Q, p, gT, gS);
+        if (s < P.nt) bdsum += dot(dV, ld3(S.n)) * F.area * dot(p.Sv, nT);
         for (int k = 0; k < 3; k++) {
             atomicAdd(&GT[vl[k]][0], gT[k].x);
             atomicAdd(&GT[vl[k]][1], gT[k].y);
@@ -235,7 +243,7 @@ __global__ __launch_bounds__(256) void adj_far_exact(AsmParams P, RuleTab rt, co
             atomicAdd(&GU[uloc[k]][2], gS[k].z);
         }
     }
-    __syncthreads();
+    block_add(bdsum, bdot);
     far_tile_flush(P, GT, GU, tt, st, tt_rows, tt_nrow, ts_cols, ts_ncol, slot_node, G);
 }
 
@@ -289,10 +297,12 @@ __global__ __launch_bounds__(256) void adj_group_eval(AsmParams P, RuleTab rt, c
                                                       const int *__restrict__ segcol, const Leaf *__restrict__ leaves,
                                                       const int4 *__restrict__ groups, unsigned int ngroups,
                                                       const double *__restrict__ lam, const double *__restrict__ mu,
-                                                      double vx, double vy, double vz, double *__restrict__ G)
+                                                      double vx, double vy, double vz, V3<double> dV,
+                                                      double *__restrict__ bdot, double *__restrict__ G)
 {
     const int lane = threadIdx.x & (WAVE - 1), wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const V3<double> Vinf = mk3(vx, vy, vz);
+    double bdsum = 0.0;
     for (unsigned int gi = blockIdx.x * 4 + wave; gi < ngroups; gi += gridDim.x * 4) {
         const int4 grp = groups[gi];
         const int t = leaves[grp.x].t, s = leaves[grp.x].s, ntot = grp.z;
@@ -370,31 +380,40 @@ __global__ __launch_bounds__(256) void adj_group_eval(AsmParams P, RuleTab rt, c
             } else {
                 finish_tri_pair(XT, nT, T.area, XS, muS, sigS, Gam, s < P.nt, Vinf, Q, p, gT, gS);
                 for (int k = 0; k < 3; k++) gadd(G, nodes[3 * s + k], gS[k]);
+                if (s < P.nt) bdsum += dot(dV, ld3(el[s].n)) * T.area * dot(p.Sv, nT);
             }
             for (int k = 0; k < 3; k++) gadd(G, nodes[3 * t + k], gT[k]);
         }
     }
+    block_add(bdsum, bdot);
 }
 
-// The freestream term -(V∞.n_T) A_T int(psi_a)/A_T (assemble.jl:422) contracted with lambda: -(ti/2) Lambda V∞.N_T.
-__global__ void adj_freestream(const Elem *__restrict__ el, const int *__restrict__ corner_slot, const int *__restrict__ nodes,
-                               int nt, int test, const double *__restrict__ lam, double vx, double vy, double vz,
-                               double *__restrict__ G)
+// The freestream term -(V∞.n_T) A_T int(psi_a)/A_T (assemble.jl:422) contracted with lambda: -(ti/2) Lambda V∞.N_T;
+// its value along dV∞ goes to bdot.
+__global__ __launch_bounds__(256) void adj_freestream(const Elem *__restrict__ el, const int *__restrict__ corner_slot,
+                                                      const int *__restrict__ nodes, int nt, int test,
+                                                      const double *__restrict__ lam, double vx, double vy, double vz,
+                                                      V3<double> dV, double *__restrict__ bdot, double *__restrict__ G)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    const Elem &E = el[t];
-    const V3<double> X[3] = {ld3(E.v[0]), ld3(E.v[1]), ld3(E.v[2])};
-    const double ti = test == 1 ? 1.0 : 1.0 / 3;
-    const double Lam = lam[corner_slot[3 * t]] + lam[corner_slot[3 * t + 1]] + lam[corner_slot[3 * t + 2]];
-    V3<double> g[3] = {mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0)};
-    cross_adj(X, (-0.5 * ti * Lam) * mk3(vx, vy, vz), g);
-    for (int k = 0; k < 3; k++) gadd(G, nodes[3 * t + k], g[k]);
+    double bd = 0.0;
+    if (t < nt) {
+        const Elem &E = el[t];
+        const V3<double> X[3] = {ld3(E.v[0]), ld3(E.v[1]), ld3(E.v[2])};
+        const double ti = test == 1 ? 1.0 : 1.0 / 3;
+        const double Lam = lam[corner_slot[3 * t]] + lam[corner_slot[3 * t + 1]] + lam[corner_slot[3 * t + 2]];
+        V3<double> g[3] = {mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0), mk3(0.0, 0.0, 0.0)};
+        cross_adj(X, (-0.5 * ti * Lam) * mk3(vx, vy, vz), g);
+        for (int k = 0; k < 3; k++) gadd(G, nodes[3 * t + k], g[k]);
+        bd = -ti * Lam * E.area * dot(dV, ld3(E.n));
+    }
+    block_add(bd, bdot);
 }
 
 // ------------------------------------------------------------------------------------------ host ------------------
 
-void Plan::adjoint(const double *lam_h, const double *mu_h, const double *vinf, double *G_h)
+void Plan::adjoint(const double *lam_h, const double *mu_h, const double *vinf, const double *dvinf, double *G_h,
+                   double *bdot_h)
 {
     hipStream_t st = ctx->stream;
     if (!assembled) throw std::invalid_argument("the plan has not been assembled (vp_plan_assemble first)");
@@ -403,27 +422,33 @@ void Plan::adjoint(const double *lam_h, const double *mu_h, const double *vinf, 
         d_mu = dalloc<double>(ncol);
         d_G = dalloc<double>((size_t)3 * (nn + nw));
         d_slotnode = dalloc<int>(nslot);
-        owned.insert(owned.end(), {d_lam, d_mu, d_G, d_slotnode});
+        d_bdot = dalloc<double>(1);
+        owned.insert(owned.end(), {d_lam, d_mu, d_G, d_slotnode, d_bdot});
         slot_nodes<<<(nt + 255) / 256, 256, 0, st>>>(d_cs, d_nodes, nt, d_slotnode);
         HIPCHECK(hipGetLastError());
     }
     HIPCHECK(hipMemcpyAsync(d_lam, lam_h, sizeof(double) * nslot, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemcpyAsync(d_mu, mu_h, sizeof(double) * ncol, hipMemcpyHostToDevice, st));
     HIPCHECK(hipMemsetAsync(d_G, 0, sizeof(double) * 3 * (nn + nw), st));
+    HIPCHECK(hipMemsetAsync(d_bdot, 0, sizeof(double), st));
     const AsmParams P = params();
     const double vx = vinf[0], vy = vinf[1], vz = vinf[2];
+    V3<double> dV;
+    dV.x = dvinf ? dvinf[0] : 0.0;
+    dV.y = dvinf ? dvinf[1] : 0.0;
+    dV.z = dvinf ? dvinf[2] : 0.0;
+    double *bd = dvinf ? d_bdot : nullptr;
     // far pairs with a closed-form point: at most the forward's count of closed-form far points
     unsigned long long hst[16];
-    HIPCHECK(hipMemcpyAsync(hst, d_stats, sizeof(hst), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
+    read_stats(hst);
     const unsigned int ecap = (unsigned int)std::min<unsigned long long>(hst[12] + 1024, 1ull << 31);
     unsigned short *d_exact = dalloc<unsigned short>(ecap);
     int4 *d_esegs = dalloc<int4>((size_t)ntt * nts);
     unsigned int *d_ecount = dalloc<unsigned int>(2);
     HIPCHECK(hipMemsetAsync(d_ecount, 0, 2 * sizeof(unsigned int), st));
     adj_far_pairs<<<dim3(ntt, nts), 256, 0, st>>>(P, rt, d_tf, d_el, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc, d_tsn,
-                                                  d_cols, d_slotnode, d_lam, d_mu, vx, vy, vz, d_G, d_exact, d_esegs,
-                                                  d_ecount, ecap);
+                                                  d_cols, d_slotnode, d_lam, d_mu, vx, vy, vz, dV, bd, d_G, d_exact,
+                                                  d_esegs, d_ecount, ecap);
     HIPCHECK(hipGetLastError());
     unsigned int ecnt[2] = {0, 0};
     HIPCHECK(hipMemcpyAsync(ecnt, d_ecount, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
@@ -439,7 +464,8 @@ void Plan::adjoint(const double *lam_h, const double *mu_h, const double *vinf, 
     }
     if (ecnt[1]) {
         adj_far_exact<<<ecnt[1], 256, 0, st>>>(P, rt, d_tf, d_el, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc, d_tsn,
-                                               d_cols, d_slotnode, d_lam, d_mu, vx, vy, vz, d_G, d_exact, d_esegs);
+                                               d_cols, d_slotnode, d_lam, d_mu, vx, vy, vz, dV, bd, d_G, d_exact,
+                                               d_esegs);
         HIPCHECK(hipGetLastError());
     }
     HIPCHECK(hipStreamSynchronize(st));
@@ -451,7 +477,7 @@ void Plan::adjoint(const double *lam_h, const double *mu_h, const double *vinf, 
         HIPCHECK(hipGetLastError());
     }
     // the non-far pairs of the last assembly, through the forward's own routing and kernels in adjoint mode
-    AdjArgs aa{d_lam, d_mu, d_G, d_nodes, vx, vy, vz};
+    AdjArgs aa{d_lam, d_mu, d_G, d_nodes, vx, vy, vz, dV.x, dV.y, dV.z, bd};
     if (nsimple) {
         touch_simple<true><<<std::min<unsigned int>((nsimple + 3) / 4, 65535u), 256, 0, st>>>(
             P, rt, d_el, d_sigma, d_cs, d_cols, d_simple, nsimple, nullptr, nullptr, nullptr, nullptr, aa);
@@ -469,12 +495,14 @@ void Plan::adjoint(const double *lam_h, const double *mu_h, const double *vinf, 
     }
     if (ngroups) {
         adj_group_eval<<<std::min<unsigned int>((ngroups + 3) / 4, 65535u), 256, 0, st>>>(
-            P, rt, d_el, d_cs, d_nodes, d_cols, d_seg, d_segcol, d_leaves, d_groups, ngroups, d_lam, d_mu, vx, vy, vz, d_G);
+            P, rt, d_el, d_cs, d_nodes, d_cols, d_seg, d_segcol, d_leaves, d_groups, ngroups, d_lam, d_mu, vx, vy, vz, dV, bd,
+            d_G);
         HIPCHECK(hipGetLastError());
     }
-    adj_freestream<<<(nt + 255) / 256, 256, 0, st>>>(d_el, d_cs, d_nodes, nt, opts.test, d_lam, vx, vy, vz, d_G);
+    adj_freestream<<<(nt + 255) / 256, 256, 0, st>>>(d_el, d_cs, d_nodes, nt, opts.test, d_lam, vx, vy, vz, dV, bd, d_G);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(G_h, d_G, sizeof(double) * 3 * (nn + nw), hipMemcpyDeviceToHost, st));
+    if (bdot_h) HIPCHECK(hipMemcpyAsync(bdot_h, d_bdot, sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
 }
 

@@ -319,12 +319,36 @@ __device__ __forceinline__ double wsum(double v)
 
 // What the evaluation kernels need in adjoint mode (null lam in the forward assembly)
 struct AdjArgs {
-    const double *lam;  // adjoint weights per slot
-    const double *mu;   // doublet values per column (slots, then wake nodes)
-    double *G;          // gradient per node (body nodes, then wake nodes), 3 each
-    const int *nodes;   // gradient index of every element corner (body nodes, wake nodes after them)
-    double vx, vy, vz;  // V∞ (sigma = -V∞.n)
+    const double *lam;               // adjoint weights per slot
+    const double *mu;                // doublet values per column (slots, then wake nodes)
+    double *G;                       // gradient per node (body nodes, then wake nodes), 3 each
+    const int *nodes;                // gradient index of every element corner (body nodes, wake nodes after them)
+    double vx, vy, vz;               // V∞ (sigma = -V∞.n)
+    double dx = 0, dy = 0, dz = 0;   // a direction dV∞ for ...
+    double *bdot = nullptr;          // ... lambdaᵀ b(sigma = -dV∞.n, dV∞), summed here when not null
 };
+
+// A body pair's part of lambdaᵀ b for the source strength sigma' = -d.n_S: -sigma' A_T (Sv . n_T), since
+// Sv = sum_q a_q w_q with A_T a_q = sum_a lambda_a W_q psi_a (pair_block's b entries, assemble.jl:253-254)
+__device__ __forceinline__ double pair_bdot(const AdjArgs &aa, bool body, V3<double> nS, double AT, V3<double> nT,
+                                            V3<double> Sv)
+{
+    return body ? dot(mk3(aa.dx, aa.dy, aa.dz), nS) * AT * dot(Sv, nT) : 0.0;
+}
+
+// Adds every thread's v to *dst (one atomic per workgroup); every thread of the workgroup calls it
+__device__ __forceinline__ void block_add(double v, double *dst)
+{
+    __shared__ double part[4];
+    v = wsum(v);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0 && dst) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += part[w];
+        if (t != 0.0) atomicAdd(dst, t);
+    }
+}
 
 // A (test triangle, source triangle) pair in adjoint mode: the contraction Q, the weights and the running sums
 struct AdjPair {
@@ -375,8 +399,9 @@ __device__ __forceinline__ void adj_wave_reduce(PairSums &p)
     }
 }
 
-__device__ __forceinline__ void adj_pair_end(const AdjArgs &aa, int nt, const Elem &T, const V3<double> *XT, int t, int s,
-                                             const int *nodes, const AdjPair &ap)
+// Returns the pair's part of lambdaᵀ b along dV∞ (pair_bdot)
+__device__ __forceinline__ double adj_pair_end(const AdjArgs &aa, int nt, const Elem &T, const V3<double> *XT, int t,
+                                               int s, const int *nodes, const AdjPair &ap, const Elem &S)
 {
     V3<double> gT[3], gS[3];
     finish_tri_pair(XT, ld3(T.n), T.area, ap.XS, ap.muS, ap.sigS, ap.Gam, s < nt, mk3(aa.vx, aa.vy, aa.vz), ap.Q, ap.p,
@@ -385,6 +410,7 @@ __device__ __forceinline__ void adj_pair_end(const AdjArgs &aa, int nt, const El
         gadd(aa.G, nodes[3 * t + k], gT[k]);
         gadd(aa.G, nodes[3 * s + k], gS[k]);
     }
+    return pair_bdot(aa, s < nt, ld3(S.n), T.area, ld3(T.n), ap.p.Sv);
 }
 
 

@@ -24,6 +24,8 @@
 //   freestream     b_i -= (Vinf.n) A int psi_i / A
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -255,7 +257,7 @@ __global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__r
                                                  double *__restrict__ bvec, int2 *__restrict__ near_list,
                                                  unsigned int *__restrict__ near_count, unsigned int near_cap,
                                                  unsigned long long *__restrict__ far_count, uint8_t *__restrict__ tiers,
-                                                 int st0)
+                                                 int st0, bool enum_only)
 {
     __shared__ double R[VMAX * UMAX];
     __shared__ double rbacc[VMAX];
@@ -288,6 +290,7 @@ __global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__r
             far = cls_norm(d) > P.far_ratio * fmax(F.diam, f.diam);
         }
         append_pair(valid && !far, t, s, near_list, near_count, near_cap);
+        if (enum_only) continue;  // the list's second pass after it grew: the far blocks are already in A
         if (tiers && valid && far) tiers[(size_t)t * P.nsrc + s] = T_FAR;
         if (far) {
             myfar++;
@@ -340,7 +343,7 @@ __global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__r
 __global__ void seg_pairs(AsmParams P, const TestFar *__restrict__ tf, const int *__restrict__ corner_slot,
                           const double *__restrict__ seg, const int *__restrict__ segcol, double *__restrict__ A,
                           int2 *__restrict__ near_list, unsigned int *__restrict__ near_count, unsigned int near_cap,
-                          unsigned long long *__restrict__ far_count)
+                          unsigned long long *__restrict__ far_count, bool enum_only)
 {
 #pragma clang fp contract(off)
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -358,6 +361,7 @@ __global__ void seg_pairs(AsmParams P, const TestFar *__restrict__ tf, const int
         far = cls_norm(mid - ld3(F.c)) > P.far_ratio * fmax(F.diam, L);
     }
     append_pair(in && !far, t, P.nsrc + f, near_list, near_count, near_cap);
+    if (enum_only) return;
     unsigned long long m = __ballot(far);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(far_count, (unsigned long long)__popcll(m));
     if (!far) return;
@@ -988,6 +992,7 @@ __global__ __launch_bounds__(256) void touch_simple(AsmParams P, RuleTab rt, con
     __shared__ unsigned long long wst[12];
     const int lane = threadIdx.x & (WAVE - 1), wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (threadIdx.x < 12) wst[threadIdx.x] = 0;
+    double bdsum = 0.0;
     __syncthreads();
     for (unsigned int pi = blockIdx.x * 4 + wave; pi < npairs; pi += gridDim.x * 4) {
         const int4 e = list[pi];
@@ -1056,7 +1061,7 @@ __global__ __launch_bounds__(256) void touch_simple(AsmParams P, RuleTab rt, con
                 adj_point(ap, X, S, side, P.kernel_far, rt.w[off + q] * bl[9], l, P.test);
             }
             adj_wave_reduce(ap.p);
-            if (lane == 0) adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap);
+            if (lane == 0) bdsum += adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap, S);
             continue;
         }
         double acc[12];
@@ -1117,6 +1122,7 @@ __global__ __launch_bounds__(256) void touch_simple(AsmParams P, RuleTab rt, con
         }
     }
     __syncthreads();
+    if constexpr (ADJ) block_add(bdsum, aa.bdot);
     if (!ADJ && threadIdx.x < 12 && wst[threadIdx.x]) atomicAdd(&stats[threadIdx.x], wst[threadIdx.x]);
 }
 
@@ -1186,6 +1192,7 @@ __global__ __launch_bounds__(256) void near_thread(AsmParams P, RuleTab rt, cons
     __syncthreads();
     const int nr = rt.n[R_NEAR], off = rt.off[R_NEAR];
     unsigned long long mypairs = 0;
+    double bdsum = 0.0;
     for (unsigned int pi = blockIdx.x * blockDim.x + threadIdx.x; pi < npairs; pi += gridDim.x * blockDim.x) {
         const int2 pr = list[pi];
         const int t = pr.x, s = pr.y;
@@ -1198,7 +1205,7 @@ __global__ __launch_bounds__(256) void near_thread(AsmParams P, RuleTab rt, cons
             adj_pair_begin(aa, P.nt, S, s, corner_slot + 3 * t, src_cols + 3 * s, ld3(T.n), ap);
 #pragma unroll 1
             for (int q = 0; q < nr; q++) adj_point(ap, X, S, 0, P.kernel_far, rt.w[off + q], rt.l + 3 * (off + q), P.test);
-            adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap);
+            bdsum += adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap, S);
             continue;
         }
         double acc[12];
@@ -1212,6 +1219,7 @@ __global__ __launch_bounds__(256) void near_thread(AsmParams P, RuleTab rt, cons
     }
     if (mypairs) atomicAdd(&wst, mypairs);
     __syncthreads();
+    if constexpr (ADJ) block_add(bdsum, aa.bdot);
     if (!ADJ && threadIdx.x == 0 && wst) {
         atomicAdd(&stats[T_NEAR], wst);
         atomicAdd(&stats[6 + T_NEAR], wst * (unsigned long long)nr);
@@ -1282,6 +1290,7 @@ __global__ __launch_bounds__(256) void near_group(AsmParams P, RuleTab rt, const
     }
     const int nr = rt.n[R_NEAR], off = rt.off[R_NEAR];
     unsigned long long mypairs = 0, mypts = 0;
+    double bdsum = 0.0;
     for (unsigned int pi = blockIdx.x * 4 + wave; pi < npairs; pi += gridDim.x * 4) {
         const int2 pr = list[pi];
         const int t = pr.x, s = pr.y;
@@ -1349,7 +1358,7 @@ __global__ __launch_bounds__(256) void near_group(AsmParams P, RuleTab rt, const
                 adj_point(ap, X, S, 0, P.kernel_far, rt.w[off + q] * frac, l, P.test);
             }
             adj_wave_reduce(ap.p);
-            if (lane == 0) adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap);
+            if (lane == 0) bdsum += adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap, S);
             continue;
         }
         double acc[12];
@@ -1382,6 +1391,7 @@ __global__ __launch_bounds__(256) void near_group(AsmParams P, RuleTab rt, const
         atomicAdd(&wst[1], mypts);
     }
     __syncthreads();
+    if constexpr (ADJ) block_add(bdsum, aa.bdot);
     if (!ADJ && threadIdx.x == 0 && wst[0]) {
         atomicAdd(&stats[T_NEAR], wst[0]);
         atomicAdd(&stats[6 + T_NEAR], wst[1]);
@@ -1676,12 +1686,15 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
     ownsA = true;
     d_b = dalloc<double>(nslot);
     d_sigma = dalloc<double>(nt);
-    d_stats = dalloc<unsigned long long>(16);
+    d_stats = dalloc<unsigned long long>(32);
     d_count = dalloc<unsigned int>(7);
     owned.insert(owned.end(), {d_b, d_sigma, d_stats, d_count});
+    // The non-far list's first budget (it grows on demand); VP_NONFAR_CAP sets it, for tests of the regrowth
     nonfar_cap = (unsigned int)std::min<long long>((long long)nt * 160 + 1024, 1ll << 31);
+    if (const char *cap = std::getenv("VP_NONFAR_CAP")) nonfar_cap = (unsigned int)std::max(1ll, std::atoll(cap));
     leaf_cap = (unsigned int)std::min<long long>((long long)nt * 24 + 1024, 1ll << 31);
     group_cap = (unsigned int)std::min<long long>((long long)nt * 24 + 1024, 1ll << 31);
+    if (const char *cap = std::getenv("VP_LEAF_CAP")) leaf_cap = group_cap = (unsigned int)std::max(1ll, std::atoll(cap));
     d_nonfar = dalloc<int2>(nonfar_cap);
     d_touchp = dalloc<int2>(nonfar_cap);
     d_near = dalloc<int2>(nonfar_cap);
@@ -1722,120 +1735,133 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
 {
     hipStream_t st = ctx->stream;
     HIPCHECK(hipMemcpyAsync(d_sigma, sigma, sizeof(double) * nt, host_sigma ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, st));
-    // The work lists grow when a pass finds more pairs than budgeted, and the assembly is redone; each grows at
-    // most once per list, so three redos cover every case.
-    for (int attempt = 0;; attempt++) {
-        if (attempt == 4) throw std::runtime_error("work lists did not fit after regrowing");
-        HIPCHECK(hipMemsetAsync(A, 0, sizeof(double) * (size_t)lda * ncol, st));
-        HIPCHECK(hipMemsetAsync(d_b, 0, sizeof(double) * nslot, st));
-        HIPCHECK(hipMemsetAsync(d_stats, 0, sizeof(unsigned long long) * 16, st));
-        HIPCHECK(hipMemsetAsync(d_count, 0, sizeof(unsigned int) * 7, st));
-        if (d_tiers) HIPCHECK(hipMemsetAsync(d_tiers, 255, (size_t)nt * nsrc, st));
-        const AsmParams P = params();
-        const int st0 = wake_only ? nts_body : 0;
-        dim3 grid(ntt, nts - st0);
-        if (timing) HIPCHECK(hipEventRecord(ev[0], st));
+    // The work lists grow when a pass finds more entries than budgeted: a non-far list that overflowed is refilled by
+    // the far kernels' enumerate-only pass (their far blocks are in A already), a rule-piece list by running the
+    // touching pairs' rule generator again (its tier counts are kept apart, in stats[16..31], and reset for it).
+    HIPCHECK(hipMemsetAsync(A, 0, sizeof(double) * (size_t)lda * ncol, st));
+    HIPCHECK(hipMemsetAsync(d_b, 0, sizeof(double) * nslot, st));
+    HIPCHECK(hipMemsetAsync(d_stats, 0, sizeof(unsigned long long) * 32, st));
+    HIPCHECK(hipMemsetAsync(d_count, 0, sizeof(unsigned int) * 7, st));
+    if (d_tiers) HIPCHECK(hipMemsetAsync(d_tiers, 255, (size_t)nt * nsrc, st));
+    const AsmParams P = params();
+    const int st0 = wake_only ? nts_body : 0;
+    dim3 grid(ntt, nts - st0);
+    if (timing) HIPCHECK(hipEventRecord(ev[0], st));
+    auto far_pass = [&](bool enum_only) {
         if (nts - st0 > 0) {
             if (opts.precision == 1)
-                far_pairs<float><<<grid, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc,
-                                                       d_tsn, A, d_b, d_nonfar, d_count, nonfar_cap, d_stats + 5, d_tiers, st0);
+                far_pairs<float><<<grid, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl,
+                                                       d_tsc, d_tsn, A, d_b, d_nonfar, d_count, nonfar_cap, d_stats + 5,
+                                                       d_tiers, st0, enum_only);
             else
                 far_pairs<double><<<grid, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl,
                                                         d_tsc, d_tsn, A, d_b, d_nonfar, d_count, nonfar_cap, d_stats + 5,
-                                                        d_tiers, st0);
+                                                        d_tiers, st0, enum_only);
             HIPCHECK(hipGetLastError());
         }
         if (nseg) {
             long long np = (long long)nt * nseg;
             seg_pairs<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(P, d_tf, d_cs, d_seg, d_segcol, A, d_nonfar, d_count,
-                                                                    nonfar_cap, d_stats + 5);
+                                                                    nonfar_cap, d_stats + 5, enum_only);
             HIPCHECK(hipGetLastError());
         }
-        if (timing) HIPCHECK(hipEventRecord(ev[1], st));
-        unsigned int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+    };
+    far_pass(false);
+    unsigned int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+    HIPCHECK(hipMemcpyAsync(cnt, d_count, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (cnt[0] > nonfar_cap) {  // more non-far pairs than budgeted: grow every pair list and enumerate again
+        for (int2 **pp : {&d_nonfar, &d_touchp, &d_near, &d_nearm}) {
+            owned.erase(std::find(owned.begin(), owned.end(), (void *)*pp));
+            (void)hipFree(*pp);
+        }
+        owned.erase(std::find(owned.begin(), owned.end(), (void *)d_simple));
+        (void)hipFree(d_simple);
+        nonfar_cap = cnt[0] + 1024;
+        d_nonfar = dalloc<int2>(nonfar_cap);
+        d_touchp = dalloc<int2>(nonfar_cap);
+        d_near = dalloc<int2>(nonfar_cap);
+        d_nearm = dalloc<int2>(nonfar_cap);
+        d_simple = dalloc<int4>(nonfar_cap);
+        owned.insert(owned.end(), {d_nonfar, d_touchp, d_near, d_nearm, d_simple});
+        HIPCHECK(hipMemsetAsync(d_count, 0, sizeof(unsigned int), st));
+        far_pass(true);
         HIPCHECK(hipMemcpyAsync(cnt, d_count, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
-        if (cnt[0] > nonfar_cap) {  // more non-far pairs than budgeted: grow every pair list and redo
-            for (int2 **pp : {&d_nonfar, &d_touchp, &d_near, &d_nearm}) {
-                owned.erase(std::find(owned.begin(), owned.end(), (void *)*pp));
-                (void)hipFree(*pp);
-            }
-            owned.erase(std::find(owned.begin(), owned.end(), (void *)d_simple));
-            (void)hipFree(d_simple);
-            nonfar_cap = cnt[0] + 1024;
-            d_nonfar = dalloc<int2>(nonfar_cap);
-            d_touchp = dalloc<int2>(nonfar_cap);
-            d_near = dalloc<int2>(nonfar_cap);
-            d_nearm = dalloc<int2>(nonfar_cap);
-            d_simple = dalloc<int4>(nonfar_cap);
-            owned.insert(owned.end(), {d_nonfar, d_touchp, d_near, d_nearm, d_simple});
-            continue;
-        }
-        nnonfar = cnt[0];
-        if (cnt[0]) {
-            triage<<<std::min<unsigned int>((cnt[0] + 255) / 256, 65535u), 256, 0, st>>>(
-                P, d_el, d_nodes, d_nonfar, cnt[0], d_touch, ntouch, d_touchp, d_near, d_nearm, d_simple, d_count + 1,
-                nonfar_cap);
-            HIPCHECK(hipGetLastError());
-        }
-        HIPCHECK(hipMemcpyAsync(cnt, d_count, 5 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-        nsimple = cnt[4];
-        ntouchp = cnt[1];
-        nnear = cnt[2];
-        nnearm = cnt[3];
+        if (cnt[0] > nonfar_cap) throw std::runtime_error("the non-far pair count changed between passes");
+    }
+    if (timing) HIPCHECK(hipEventRecord(ev[1], st));
+    nnonfar = cnt[0];
+    if (cnt[0]) {
+        triage<<<std::min<unsigned int>((cnt[0] + 255) / 256, 65535u), 256, 0, st>>>(
+            P, d_el, d_nodes, d_nonfar, cnt[0], d_touch, ntouch, d_touchp, d_near, d_nearm, d_simple, d_count + 1,
+            nonfar_cap);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipMemcpyAsync(cnt, d_count, 5 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    nsimple = cnt[4];
+    ntouchp = cnt[1];
+    nnear = cnt[2];
+    nnearm = cnt[3];
+    auto gen_pass = [&]() {
         if (cnt[1]) {
             touch_gen<<<std::min<unsigned int>((cnt[1] + 255) / 256, 65535u), 256, 0, st>>>(
                 P, rt, d_el, d_nodes, d_seg, d_touchp, cnt[1], d_touch, ntouch, d_leaves, leaf_cap, d_groups, group_cap,
-                d_count + 5, d_stats, d_tiers);
+                d_count + 5, d_stats + 16, d_tiers);
             HIPCHECK(hipGetLastError());
         }
-        if (timing) HIPCHECK(hipEventRecord(ev[2], st));
-        if (nsimple) {
-            touch_simple<false><<<std::min<unsigned int>((nsimple + 3) / 4, 65535u), 256, 0, st>>>(
-                P, rt, d_el, d_sigma, d_cs, d_cols, d_simple, nsimple, A, d_b, d_stats, d_tiers, AdjArgs{});
-            HIPCHECK(hipGetLastError());
-        }
-        if (cnt[2]) {
-            near_thread<false><<<std::min<unsigned int>((cnt[2] + 255) / 256, 65535u), 256, 0, st>>>(
-                P, rt, d_el, d_sigma, d_cs, d_cols, d_near, cnt[2], A, d_b, d_stats, d_tiers, AdjArgs{});
-            HIPCHECK(hipGetLastError());
-        }
-        if (cnt[3]) {
-            near_group<false><<<std::min<unsigned int>((cnt[3] + 3) / 4, 65535u), 256, 0, st>>>(
-                P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, cnt[3], A, d_b, d_stats, d_tiers, AdjArgs{});
-            HIPCHECK(hipGetLastError());
-        }
-        if (timing) HIPCHECK(hipEventRecord(ev[3], st));
+    };
+    gen_pass();
+    if (timing) HIPCHECK(hipEventRecord(ev[2], st));
+    if (nsimple) {
+        touch_simple<false><<<std::min<unsigned int>((nsimple + 3) / 4, 65535u), 256, 0, st>>>(
+            P, rt, d_el, d_sigma, d_cs, d_cols, d_simple, nsimple, A, d_b, d_stats, d_tiers, AdjArgs{});
+        HIPCHECK(hipGetLastError());
+    }
+    if (cnt[2]) {
+        near_thread<false><<<std::min<unsigned int>((cnt[2] + 255) / 256, 65535u), 256, 0, st>>>(
+            P, rt, d_el, d_sigma, d_cs, d_cols, d_near, cnt[2], A, d_b, d_stats, d_tiers, AdjArgs{});
+        HIPCHECK(hipGetLastError());
+    }
+    if (cnt[3]) {
+        near_group<false><<<std::min<unsigned int>((cnt[3] + 3) / 4, 65535u), 256, 0, st>>>(
+            P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, cnt[3], A, d_b, d_stats, d_tiers, AdjArgs{});
+        HIPCHECK(hipGetLastError());
+    }
+    if (timing) HIPCHECK(hipEventRecord(ev[3], st));
+    HIPCHECK(hipMemcpyAsync(cnt + 5, d_count + 5, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (cnt[5] > leaf_cap || cnt[6] > group_cap) {  // more rule pieces than budgeted: grow and generate again
+        owned.erase(std::find(owned.begin(), owned.end(), (void *)d_leaves));
+        owned.erase(std::find(owned.begin(), owned.end(), (void *)d_groups));
+        (void)hipFree(d_leaves);
+        (void)hipFree(d_groups);
+        leaf_cap = std::max(leaf_cap, cnt[5] + 1024);
+        group_cap = std::max(group_cap, cnt[6] + 1024);
+        d_leaves = dalloc<Leaf>(leaf_cap);
+        d_groups = dalloc<int4>(group_cap);
+        owned.insert(owned.end(), {d_leaves, d_groups});
+        HIPCHECK(hipMemsetAsync(d_count + 5, 0, 2 * sizeof(unsigned int), st));
+        HIPCHECK(hipMemsetAsync(d_stats + 16, 0, sizeof(unsigned long long) * 16, st));
+        gen_pass();
         HIPCHECK(hipMemcpyAsync(cnt + 5, d_count + 5, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
-        if (cnt[5] > leaf_cap || cnt[6] > group_cap) {  // more rule pieces than budgeted: grow and redo
-            owned.erase(std::find(owned.begin(), owned.end(), (void *)d_leaves));
-            owned.erase(std::find(owned.begin(), owned.end(), (void *)d_groups));
-            (void)hipFree(d_leaves);
-            (void)hipFree(d_groups);
-            leaf_cap = std::max(leaf_cap, cnt[5] + 1024);
-            group_cap = std::max(group_cap, cnt[6] + 1024);
-            d_leaves = dalloc<Leaf>(leaf_cap);
-            d_groups = dalloc<int4>(group_cap);
-            owned.insert(owned.end(), {d_leaves, d_groups});
-            continue;
-        }
-        nleaves = cnt[5];
-        ngroups = cnt[6];
-        if (cnt[6]) {
-            group_eval<<<std::min<unsigned int>((cnt[6] + 3) / 4, 65535u), 256, 0, st>>>(
-                P, rt, d_el, d_sigma, d_cs, d_cols, d_seg, d_segcol, d_leaves, d_groups, cnt[6], A, d_b);
-            HIPCHECK(hipGetLastError());
-        }
-        if (timing) HIPCHECK(hipEventRecord(ev[4], st));
-        if (!wake_only) {
-            freestream<<<(nt + 255) / 256, 256, 0, st>>>(d_el, d_cs, nt, opts.test, vinf[0], vinf[1], vinf[2], d_b);
-            HIPCHECK(hipGetLastError());
-        }
-        if (timing) HIPCHECK(hipEventRecord(ev[5], st));
-        break;
+        if (cnt[5] > leaf_cap || cnt[6] > group_cap) throw std::runtime_error("the rule-piece count changed between passes");
     }
+    nleaves = cnt[5];
+    ngroups = cnt[6];
+    if (cnt[6]) {
+        group_eval<<<std::min<unsigned int>((cnt[6] + 3) / 4, 65535u), 256, 0, st>>>(
+            P, rt, d_el, d_sigma, d_cs, d_cols, d_seg, d_segcol, d_leaves, d_groups, cnt[6], A, d_b);
+        HIPCHECK(hipGetLastError());
+    }
+    if (timing) HIPCHECK(hipEventRecord(ev[4], st));
+    if (!wake_only) {
+        freestream<<<(nt + 255) / 256, 256, 0, st>>>(d_el, d_cs, nt, opts.test, vinf[0], vinf[1], vinf[2], d_b);
+        HIPCHECK(hipGetLastError());
+    }
+    if (timing) HIPCHECK(hipEventRecord(ev[5], st));
     assembled = true;
 }
 
@@ -1850,15 +1876,23 @@ void Plan::kernel_ms(double *ms)
     ms[5] = (double)nnear;
     ms[6] = (double)nleaves;
     unsigned long long h[16];
-    HIPCHECK(hipMemcpy(h, d_stats, sizeof(h), hipMemcpyDeviceToHost));
+    read_stats(h);
     ms[7] = (double)h[12];
+}
+
+// The assembly's counters: the evaluation kernels' (stats[0..15]) plus the rule generator's (stats[16..31])
+void Plan::read_stats(unsigned long long *h)
+{
+    unsigned long long both[32];
+    HIPCHECK(hipMemcpyAsync(both, d_stats, sizeof(both), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < 16; k++) h[k] = both[k] + both[16 + k];
 }
 
 void Plan::stats(long long *pairs, long long *points)
 {
     unsigned long long h[16];
-    HIPCHECK(hipMemcpyAsync(h, d_stats, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    read_stats(h);
     for (int k = 0; k < 6; k++) {
         pairs[k] = (long long)h[k];
         points[k] = (long long)h[6 + k];

@@ -162,14 +162,15 @@ int vp_plan_device_A(vp_plan *plan, double **A, double **b, long long *lda)
     return VP_OK;
 }
 
-int vp_plan_adjoint(vp_plan *plan, const double *lambda, const double *mu, const double *vinf, double *grad)
+int vp_plan_adjoint(vp_plan *plan, const double *lambda, const double *mu, const double *vinf, const double *dvinf,
+                    double *grad, double *lambda_b_dot)
 {
     if (!plan || !plan->p.ctx) return VP_E_ARGUMENT;
     vp::Ctx *c = plan->p.ctx;
-    if (!lambda || !mu || !vinf || !grad) return fail(c, VP_E_ARGUMENT, "null argument");
+    if (!lambda || !mu || !vinf || !grad || (dvinf && !lambda_b_dot)) return fail(c, VP_E_ARGUMENT, "null argument");
     return guard(c, [&] {
         (void)hipSetDevice(c->device);
-        plan->p.adjoint(lambda, mu, vinf, grad);
+        plan->p.adjoint(lambda, mu, vinf, dvinf, grad, dvinf ? lambda_b_dot : nullptr);
     });
 }
 

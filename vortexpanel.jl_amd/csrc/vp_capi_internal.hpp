@@ -66,6 +66,7 @@ struct Plan {
     // backward assembly (adjoint) buffers, made on first use
     double *d_lam = nullptr, *d_mu = nullptr, *d_G = nullptr;
     int *d_slotnode = nullptr;
+    double *d_bdot = nullptr;
     int2 *d_nonfar = nullptr;
     struct Leaf *d_leaves = nullptr;
 
@@ -77,7 +78,8 @@ struct Plan {
     AsmParams params() const;
     void wake_product(const double *mu_nodes, double *out);
     void blob_rhs(int nb, const double *bx, const double *ba, const double *bd, double *out);
-    void adjoint(const double *lam, const double *mu, const double *vinf, double *G);
+    void read_stats(unsigned long long *h);
+    void adjoint(const double *lam, const double *mu, const double *vinf, const double *dvinf, double *G, double *bdot);
 };
 
 // The factored bordered system (solve.jl:65-87): the original FP64 bordered matrix M (row-major, npad x npad, the

@@ -151,13 +151,18 @@ class WingAdjoint:
         t0 = time.perf_counter()
         els = build_elements(g)
         self.plan = AssemblyPlan(g, dm, wake, opts or AssemblyOptions(precision=precision), ctx=self.ctx)
+        t1 = time.perf_counter()
         self.plan.assemble(source_strengths(els, V), V)
         self.b = self.plan.read_b()
+        t2 = time.perf_counter()
         self.W = wake_rows(dm, wake.te)
         self.F = factorize_galerkin(dm, self.plan, W=self.W, mode=precision)
+        self.ctx.sync()
+        t3 = time.perf_counter()
         sol = solve_factored(self.F, self.b)
         self.ctx.sync()
-        self.timings["forward_s"] = time.perf_counter() - t0
+        t4 = time.perf_counter()
+        self.timings.update(forward_s=t4 - t0, plan_s=t1 - t0, assemble_s=t2 - t1, factor_s=t3 - t2, solve_s=t4 - t3)
         self.q, self.mu = sol["q"], sol["mu"]
         self._bd = None
         self.mu_w = dm.K @ self.q                   # trailing edge doublet jumps
@@ -175,24 +180,30 @@ class WingAdjoint:
         X = F.solve_bordered_t(R)
         return F.C @ X[:F.nq]
 
+    def _dV(self):
+        a = np.radians(self.alpha)
+        return np.array([-np.sin(a), 0.0, np.cos(a)])  # dV∞/dalpha per radian
+
     def _backward(self, lam):
-        """d/dx [lamᵀ (b - A [mu; mu_wake])] on the device, wake node gradients moved to their trailing edge nodes"""
+        """d/dx [lamᵀ (b - A [mu; mu_wake])] on the device, wake node gradients moved to their trailing edge nodes;
+        and lamᵀ db/dalpha (per radian) from the same pass (b is linear in V∞, so db/dalpha is b of dV∞/dalpha)"""
         nn = len(self.g.x)
         mu_full = L.f64(np.concatenate([self.mu, self.mu_wn]))
         G = np.zeros((nn + len(self.wake.x), 3))
+        lbd = C.c_double()
         _check(self.ctx.h, self.ctx.lib.vp_plan_adjoint(self.plan.h, L.ptr(L.f64(lam)), L.ptr(mu_full),
-                                                        L.ptr(L.f64(self.V)), L.ptr(G)))
+                                                        L.ptr(L.f64(self.V)), L.ptr(L.f64(self._dV())), L.ptr(G),
+                                                        C.byref(lbd)))
         gx = G[:nn].copy()
         np.add.at(gx, self.dm.wake_te.node[self.wake.te], G[nn:])
-        return gx
+        return gx, lbd.value
 
     def _db_dalpha(self):
-        """db/dalpha at fixed geometry and wake (per radian), made once: b is linear in V∞ (sources and freestream
-        term), so it is the b of dV∞/dalpha.  The re-assembly leaves the plan's rule lists as they were (they do not
+        """db/dalpha at fixed geometry and wake (per radian) by a second assembly, made once (a check of the
+        backward pass's lamᵀ db/dalpha).  The re-assembly leaves the plan's rule lists as they were (they do not
         depend on V∞), which the backward assembly reads."""
         if self._bd is None:
-            a = np.radians(self.alpha)
-            Vd = np.array([-np.sin(a), 0.0, np.cos(a)])
+            Vd = self._dV()
             self.plan.assemble(source_strengths(build_elements(self.g), Vd), Vd)
             self._bd = self.plan.read_b()
         return self._bd
@@ -211,10 +222,11 @@ class WingAdjoint:
         t1 = time.perf_counter()
         lam = self._adjoint_weights(dJdq)
         t2 = time.perf_counter()
-        gx = dx + self._backward(lam)
-        self.ctx.sync()
+        gb, lbd = self._backward(lam)
+        gx = dx + gb
+        self.last_lam, self.last_lam_db = lam, lbd
         t3 = time.perf_counter()
-        dalpha = da + float(lam @ self._db_dalpha()) * np.pi / 180
+        dalpha = da + lbd * np.pi / 180
         self.timings.update({f"{objective}_explicit_s": t1 - t0, f"{objective}_adjoint_solve_s": t2 - t1,
                              f"{objective}_backward_s": t3 - t2})
         return J, gx, dalpha
