@@ -107,14 +107,15 @@ int vp_plan_tiers(vp_plan *plan, uint8_t *tiers);
  * (non-multipole) point evaluations (8 values). */
 int vp_plan_timings(vp_plan *plan, double *ms);
 /* Backward assembly (no counterpart in the reference, which has no derivatives; it serves the adjoint dJ/dx of a
- * Galerkin solution): the gradient of lambdaᵀ (b(x) - A(x) mu) with respect to the node coordinates, for adjoint
- * weights lambda (nslot) and doublet values mu (nslot + nw: slots, then wake nodes), with the rules of the last
- * vp_plan_assemble and sigma = -Vinf.n (source_strengths, assemble.jl:364).  grad: (nn + nw) x 3, body nodes then
- * wake nodes.  With a direction dvinf (3 values, or null) it also returns lambdaᵀ b(sigma = -dvinf.n, dvinf) in
- * *lambda_b_dot: the derivative of lambdaᵀ b along dvinf (b is linear in Vinf), e.g. for d/dalpha, from the same
- * pass without a second assembly. */
-int vp_plan_adjoint(vp_plan *plan, const double *lambda, const double *mu, const double *vinf, const double *dvinf,
-                    double *grad, double *lambda_b_dot);
+ * Galerkin solution): the gradient of lambdaᵀ (b(x) - A(x) mu) with respect to the node coordinates, for nlam sets of
+ * adjoint weights lambda (nlam x nslot, row-major: one per objective) and doublet values mu (nslot + nw: slots, then
+ * wake nodes), with the rules of the last vp_plan_assemble and sigma = -Vinf.n (source_strengths, assemble.jl:364).
+ * grad: nlam x (nn + nw) x 3, body nodes then wake nodes.  Two sets share one pass of the pair integrals.  With a
+ * direction dvinf (3 values, or null) it also returns lambdaᵀ b(sigma = -dvinf.n, dvinf) per set in lambda_b_dot
+ * (nlam): the derivative of lambdaᵀ b along dvinf (b is linear in Vinf), e.g. for d/dalpha, without a second
+ * assembly. */
+int vp_plan_adjoint(vp_plan *plan, int nlam, const double *lambda, const double *mu, const double *vinf,
+                    const double *dvinf, double *grad, double *lambda_b_dot);
 /* `explicit_rhs` (src/assembly/explicit.jl:31-86) of known sheets and filaments: out = A_wake mu_wake (nslot values;
  * the right hand side is its negative), the wake columns of the last vp_plan_assemble times the doublets of the wake's
  * nodes (a wake-only plan whose wake holds the known sheets and filaments, one node per corner or filament end). */

@@ -317,15 +317,20 @@ __device__ __forceinline__ double wsum(double v)
     return v;
 }
 
-// What the evaluation kernels need in adjoint mode (null lam in the forward assembly)
+// What the evaluation kernels need in adjoint mode (null lam in the forward assembly).  The kernels take up to
+// ADJ_MAX sets of adjoint weights at once (a template count NL): the point derivatives of the kernel do not depend on
+// lambda, which only weights their sums, so several objectives share one pass.
+constexpr int ADJ_MAX = 2;
 struct AdjArgs {
-    const double *lam;               // adjoint weights per slot
+    const double *lam;               // adjoint weights per slot, NL sets lstride apart
     const double *mu;                // doublet values per column (slots, then wake nodes)
-    double *G;                       // gradient per node (body nodes, then wake nodes), 3 each
+    double *G;                       // gradient per node (body nodes, then wake nodes), 3 each; NL sets gstride apart
     const int *nodes;                // gradient index of every element corner (body nodes, wake nodes after them)
     double vx, vy, vz;               // V∞ (sigma = -V∞.n)
     double dx = 0, dy = 0, dz = 0;   // a direction dV∞ for ...
-    double *bdot = nullptr;          // ... lambdaᵀ b(sigma = -dV∞.n, dV∞), summed here when not null
+    double *bdot = nullptr;          // ... lambdaᵀ b(sigma = -dV∞.n, dV∞) per set, summed here when not null
+    int lstride = 0;
+    long long gstride = 0;
 };
 
 // A body pair's part of lambdaᵀ b for the source strength sigma' = -d.n_S: -sigma' A_T (Sv . n_T), since
@@ -348,46 +353,70 @@ __device__ __forceinline__ void block_add(double v, double *dst)
         for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += part[w];
         if (t != 0.0) atomicAdd(dst, t);
     }
+    __syncthreads();
+}
+
+template <int NL> __device__ __forceinline__ void block_add_n(const double *v, double *dst)
+{
+    for (int l = 0; l < NL; l++) block_add(v[l], dst ? dst + l : nullptr);
+}
+
+__device__ __forceinline__ void zero_sums(PairSums &p)
+{
+    p.Sv = mk3(0.0, 0.0, 0.0);
+    for (int k = 0; k < 3; k++) p.GS[k] = p.GX[k] = p.Sv;
+}
+
+// One point's contribution, weight a, to a pair's sums: value wv, vertex derivatives g, l the point's barycentrics
+__device__ __forceinline__ void add_point(PairSums &p, double a, V3<double> wv, const V3<double> *g, V3<double> gx,
+                                          const double *l)
+{
+    p.Sv = p.Sv + a * wv;
+    for (int k = 0; k < 3; k++) {
+        p.GS[k] = p.GS[k] + a * g[k];
+        p.GX[k] = p.GX[k] + (a * l[k]) * gx;
+    }
 }
 
 // A (test triangle, source triangle) pair in adjoint mode: the contraction Q, the weights and the running sums
-struct AdjPair {
+template <int NL> struct AdjPair {
     V3<double> XS[3], Gam, Q;
-    double la[3], muS[3], sigS;
-    PairSums p;
+    double la[NL][3], muS[3], sigS;
+    PairSums p[NL];
 };
 
+template <int NL>
 __device__ __forceinline__ void adj_pair_begin(const AdjArgs &aa, int nt, const Elem &S, int s, const int *ct,
-                                               const int *cs, V3<double> nT, AdjPair &ap)
+                                               const int *cs, V3<double> nT, AdjPair<NL> &ap)
 {
     for (int k = 0; k < 3; k++) {
         ap.XS[k] = ld3(S.v[k]);
         ap.muS[k] = aa.mu[cs[k]];
-        ap.la[k] = aa.lam[ct[k]];
+        for (int l = 0; l < NL; l++) ap.la[l][k] = aa.lam[l * aa.lstride + ct[k]];
     }
     ap.Gam = sheet_of(ap.XS, ap.muS);
     ap.sigS = s < nt ? -dot(mk3(aa.vx, aa.vy, aa.vz), ld3(S.n)) : 0.0;
     ap.Q = ap.sigS * nT + cross(nT, ap.Gam);
-    ap.p.Sv = mk3(0.0, 0.0, 0.0);
-    for (int k = 0; k < 3; k++) ap.p.GS[k] = ap.p.GX[k] = ap.p.Sv;
+    for (int l = 0; l < NL; l++) zero_sums(ap.p[l]);
 }
 
 // One rule point (barycentric l on the test triangle, weight W0 = rule weight x piece area fraction)
-__device__ __forceinline__ void adj_point(AdjPair &ap, const V3<double> *XT, const Elem &S, int side, double kernel_far,
-                                          double W0, const double *l, int test)
+template <int NL>
+__device__ __forceinline__ void adj_point(AdjPair<NL> &ap, const V3<double> *XT, const Elem &S, int side,
+                                          double kernel_far, double W0, const double *l, int test)
 {
-    double psi[3];
+    double psi[3], a[NL];
     test_weights(test, l, psi);
-    const double a = W0 * ((ap.la[0] * psi[0] + ap.la[1] * psi[1]) + ap.la[2] * psi[2]);
-    if (a == 0.0) return;
+    bool any = false;
+    for (int j = 0; j < NL; j++) {
+        a[j] = W0 * ((ap.la[j][0] * psi[0] + ap.la[j][1] * psi[1]) + ap.la[j][2] * psi[2]);
+        any = any || a[j] != 0.0;
+    }
+    if (!any) return;
     V3<double> wv, g[3];
     tri_point_grad(ap.XS, S, cls_point(XT, l), ap.Q, side, kernel_far, wv, g);
     const V3<double> gx = -1.0 * ((g[0] + g[1]) + g[2]);
-    ap.p.Sv = ap.p.Sv + a * wv;
-    for (int k = 0; k < 3; k++) {
-        ap.p.GS[k] = ap.p.GS[k] + a * g[k];
-        ap.p.GX[k] = ap.p.GX[k] + (a * l[k]) * gx;
-    }
+    for (int j = 0; j < NL; j++) add_point(ap.p[j], a[j], wv, g, gx, l);
 }
 
 __device__ __forceinline__ void adj_wave_reduce(PairSums &p)
@@ -399,19 +428,22 @@ __device__ __forceinline__ void adj_wave_reduce(PairSums &p)
     }
 }
 
-// Returns the pair's part of lambdaᵀ b along dV∞ (pair_bdot)
-__device__ __forceinline__ double adj_pair_end(const AdjArgs &aa, int nt, const Elem &T, const V3<double> *XT, int t,
-                                               int s, const int *nodes, const AdjPair &ap, const Elem &S)
+// Adds the pair's parts of lambdaᵀ b along dV∞ (pair_bdot) to bd
+template <int NL>
+__device__ __forceinline__ void adj_pair_end(const AdjArgs &aa, int nt, const Elem &T, const V3<double> *XT, int t,
+                                             int s, const int *nodes, const AdjPair<NL> &ap, const Elem &S, double *bd)
 {
-    V3<double> gT[3], gS[3];
-    finish_tri_pair(XT, ld3(T.n), T.area, ap.XS, ap.muS, ap.sigS, ap.Gam, s < nt, mk3(aa.vx, aa.vy, aa.vz), ap.Q, ap.p,
-                    gT, gS);
-    for (int k = 0; k < 3; k++) {
-        gadd(aa.G, nodes[3 * t + k], gT[k]);
-        gadd(aa.G, nodes[3 * s + k], gS[k]);
+    for (int l = 0; l < NL; l++) {
+        V3<double> gT[3], gS[3];
+        finish_tri_pair(XT, ld3(T.n), T.area, ap.XS, ap.muS, ap.sigS, ap.Gam, s < nt, mk3(aa.vx, aa.vy, aa.vz), ap.Q,
+                        ap.p[l], gT, gS);
+        double *G = aa.G + l * aa.gstride;
+        for (int k = 0; k < 3; k++) {
+            gadd(G, nodes[3 * t + k], gT[k]);
+            gadd(G, nodes[3 * s + k], gS[k]);
+        }
+        bd[l] += pair_bdot(aa, s < nt, ld3(S.n), T.area, ld3(T.n), ap.p[l].Sv);
     }
-    return pair_bdot(aa, s < nt, ld3(S.n), T.area, ld3(T.n), ap.p.Sv);
 }
-
 
 }  // namespace vp

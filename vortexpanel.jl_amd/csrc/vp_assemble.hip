@@ -980,7 +980,7 @@ __device__ __forceinline__ double wave_sum2(double v)
     return v;
 }
 
-template <bool ADJ>
+template <int NADJ>  // 0: the forward assembly; 1..ADJ_MAX: backward with that many sets of adjoint weights
 __global__ __launch_bounds__(256) void touch_simple(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                     const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                     const int *__restrict__ src_cols, const int4 *__restrict__ list,
@@ -992,7 +992,7 @@ __global__ __launch_bounds__(256) void touch_simple(AsmParams P, RuleTab rt, con
     __shared__ unsigned long long wst[12];
     const int lane = threadIdx.x & (WAVE - 1), wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (threadIdx.x < 12) wst[threadIdx.x] = 0;
-    double bdsum = 0.0;
+    double bdsum[NADJ > 0 ? NADJ : 1] = {};
     __syncthreads();
     for (unsigned int pi = blockIdx.x * 4 + wave; pi < npairs; pi += gridDim.x * 4) {
         const int4 e = list[pi];
@@ -1048,8 +1048,8 @@ __global__ __launch_bounds__(256) void touch_simple(AsmParams P, RuleTab rt, con
         const int side = s == t ? 1 : 0;
         const V3<double> tn = ld3(T.n), sc = ld3(S.c);
         const double fd = P.kernel_far * S.diam;
-        if constexpr (ADJ) {  // backward assembly: the same pieces, the pair's gradient sums
-            AdjPair ap;
+        if constexpr (NADJ > 0) {  // backward assembly: the same pieces, the pair's gradient sums
+            AdjPair<NADJ> ap;
             adj_pair_begin(aa, P.nt, S, s, corner_slot + 3 * t, src_cols + 3 * s, tn, ap);
 #pragma unroll 1
             for (int pp = lane; pp < ntot; pp += WAVE) {
@@ -1060,8 +1060,8 @@ __global__ __launch_bounds__(256) void touch_simple(AsmParams P, RuleTab rt, con
                 for (int j = 0; j < 3; j++) l[j] = (m[0] * bl[j] + m[1] * bl[3 + j]) + m[2] * bl[6 + j];
                 adj_point(ap, X, S, side, P.kernel_far, rt.w[off + q] * bl[9], l, P.test);
             }
-            adj_wave_reduce(ap.p);
-            if (lane == 0) bdsum += adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap, S);
+            for (int l = 0; l < NADJ; l++) adj_wave_reduce(ap.p[l]);
+            if (lane == 0) adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap, S, bdsum);
             continue;
         }
         double acc[12];
@@ -1122,8 +1122,8 @@ __global__ __launch_bounds__(256) void touch_simple(AsmParams P, RuleTab rt, con
         }
     }
     __syncthreads();
-    if constexpr (ADJ) block_add(bdsum, aa.bdot);
-    if (!ADJ && threadIdx.x < 12 && wst[threadIdx.x]) atomicAdd(&stats[threadIdx.x], wst[threadIdx.x]);
+    if constexpr (NADJ > 0) block_add_n<NADJ>(bdsum, aa.bdot);
+    if (NADJ == 0 && threadIdx.x < 12 && wst[threadIdx.x]) atomicAdd(&stats[threadIdx.x], wst[threadIdx.x]);
 }
 
 // One point of a plain near pair's rule: accumulates W psi_a (w x n_T) (acc[3a..3a+2]) and W psi_a (w . n_T)
@@ -1179,7 +1179,7 @@ __device__ __forceinline__ void near_scatter(const AsmParams &P, int t, int s, c
 
 // Plain near pairs whose near rule is the single root piece (near_rule, assemble.jl:69-73): one thread per pair,
 // the 7-point rule on the whole test triangle.
-template <bool ADJ>
+template <int NADJ>  // 0: the forward assembly; 1..ADJ_MAX: backward with that many sets of adjoint weights
 __global__ __launch_bounds__(256) void near_thread(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                    const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                    const int *__restrict__ src_cols, const int2 *__restrict__ list,
@@ -1192,7 +1192,7 @@ __global__ __launch_bounds__(256) void near_thread(AsmParams P, RuleTab rt, cons
     __syncthreads();
     const int nr = rt.n[R_NEAR], off = rt.off[R_NEAR];
     unsigned long long mypairs = 0;
-    double bdsum = 0.0;
+    double bdsum[NADJ > 0 ? NADJ : 1] = {};
     for (unsigned int pi = blockIdx.x * blockDim.x + threadIdx.x; pi < npairs; pi += gridDim.x * blockDim.x) {
         const int2 pr = list[pi];
         const int t = pr.x, s = pr.y;
@@ -1200,12 +1200,12 @@ __global__ __launch_bounds__(256) void near_thread(AsmParams P, RuleTab rt, cons
         const V3<double> X[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
         const V3<double> SX[3] = {ld3(S.v[0]), ld3(S.v[1]), ld3(S.v[2])};
         const double fd = P.kernel_far * S.diam;
-        if constexpr (ADJ) {
-            AdjPair ap;
+        if constexpr (NADJ > 0) {
+            AdjPair<NADJ> ap;
             adj_pair_begin(aa, P.nt, S, s, corner_slot + 3 * t, src_cols + 3 * s, ld3(T.n), ap);
 #pragma unroll 1
             for (int q = 0; q < nr; q++) adj_point(ap, X, S, 0, P.kernel_far, rt.w[off + q], rt.l + 3 * (off + q), P.test);
-            bdsum += adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap, S);
+            adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap, S, bdsum);
             continue;
         }
         double acc[12];
@@ -1219,8 +1219,8 @@ __global__ __launch_bounds__(256) void near_thread(AsmParams P, RuleTab rt, cons
     }
     if (mypairs) atomicAdd(&wst, mypairs);
     __syncthreads();
-    if constexpr (ADJ) block_add(bdsum, aa.bdot);
-    if (!ADJ && threadIdx.x == 0 && wst) {
+    if constexpr (NADJ > 0) block_add_n<NADJ>(bdsum, aa.bdot);
+    if (NADJ == 0 && threadIdx.x == 0 && wst) {
         atomicAdd(&stats[T_NEAR], wst);
         atomicAdd(&stats[6 + T_NEAR], wst * (unsigned long long)nr);
     }
@@ -1266,7 +1266,7 @@ __device__ __forceinline__ void node_of(int nid, int &L, int &idx)  // level-ord
 // full quadtree at once across its lanes (each test depends on that node's own corners only, so this is the
 // recursive near_pieces! exactly), collects the leaves -- nodes whose ancestors all split and that do not split or
 // reach the depth -- by ballot, and evaluates their 7-point rules flattened over the lanes.
-template <bool ADJ>
+template <int NADJ>  // 0: the forward assembly; 1..ADJ_MAX: backward with that many sets of adjoint weights
 __global__ __launch_bounds__(256) void near_group(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                   const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                   const int *__restrict__ src_cols, const int2 *__restrict__ list,
@@ -1290,7 +1290,7 @@ __global__ __launch_bounds__(256) void near_group(AsmParams P, RuleTab rt, const
     }
     const int nr = rt.n[R_NEAR], off = rt.off[R_NEAR];
     unsigned long long mypairs = 0, mypts = 0;
-    double bdsum = 0.0;
+    double bdsum[NADJ > 0 ? NADJ : 1] = {};
     for (unsigned int pi = blockIdx.x * 4 + wave; pi < npairs; pi += gridDim.x * 4) {
         const int2 pr = list[pi];
         const int t = pr.x, s = pr.y;
@@ -1340,8 +1340,8 @@ __global__ __launch_bounds__(256) void near_group(AsmParams P, RuleTab rt, const
         }
         // 3. the leaves' points
         const double fd = P.kernel_far * S.diam;
-        if constexpr (ADJ) {
-            AdjPair ap;
+        if constexpr (NADJ > 0) {
+            AdjPair<NADJ> ap;
             adj_pair_begin(aa, P.nt, S, s, corner_slot + 3 * t, src_cols + 3 * s, ld3(T.n), ap);
 #pragma unroll 1
             for (int pp = lane; pp < nl * nr; pp += WAVE) {
@@ -1357,8 +1357,8 @@ __global__ __launch_bounds__(256) void near_group(AsmParams P, RuleTab rt, const
                 for (int i = 0; i < L; i++) frac /= 4;
                 adj_point(ap, X, S, 0, P.kernel_far, rt.w[off + q] * frac, l, P.test);
             }
-            adj_wave_reduce(ap.p);
-            if (lane == 0) bdsum += adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap, S);
+            for (int l = 0; l < NADJ; l++) adj_wave_reduce(ap.p[l]);
+            if (lane == 0) adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap, S, bdsum);
             continue;
         }
         double acc[12];
@@ -1391,8 +1391,8 @@ __global__ __launch_bounds__(256) void near_group(AsmParams P, RuleTab rt, const
         atomicAdd(&wst[1], mypts);
     }
     __syncthreads();
-    if constexpr (ADJ) block_add(bdsum, aa.bdot);
-    if (!ADJ && threadIdx.x == 0 && wst[0]) {
+    if constexpr (NADJ > 0) block_add_n<NADJ>(bdsum, aa.bdot);
+    if (NADJ == 0 && threadIdx.x == 0 && wst[0]) {
         atomicAdd(&stats[T_NEAR], wst[0]);
         atomicAdd(&stats[6 + T_NEAR], wst[1]);
     }
@@ -1815,17 +1815,17 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
     gen_pass();
     if (timing) HIPCHECK(hipEventRecord(ev[2], st));
     if (nsimple) {
-        touch_simple<false><<<std::min<unsigned int>((nsimple + 3) / 4, 65535u), 256, 0, st>>>(
+        touch_simple<0><<<std::min<unsigned int>((nsimple + 3) / 4, 65535u), 256, 0, st>>>(
             P, rt, d_el, d_sigma, d_cs, d_cols, d_simple, nsimple, A, d_b, d_stats, d_tiers, AdjArgs{});
         HIPCHECK(hipGetLastError());
     }
     if (cnt[2]) {
-        near_thread<false><<<std::min<unsigned int>((cnt[2] + 255) / 256, 65535u), 256, 0, st>>>(
+        near_thread<0><<<std::min<unsigned int>((cnt[2] + 255) / 256, 65535u), 256, 0, st>>>(
             P, rt, d_el, d_sigma, d_cs, d_cols, d_near, cnt[2], A, d_b, d_stats, d_tiers, AdjArgs{});
         HIPCHECK(hipGetLastError());
     }
     if (cnt[3]) {
-        near_group<false><<<std::min<unsigned int>((cnt[3] + 3) / 4, 65535u), 256, 0, st>>>(
+        near_group<0><<<std::min<unsigned int>((cnt[3] + 3) / 4, 65535u), 256, 0, st>>>(
             P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, cnt[3], A, d_b, d_stats, d_tiers, AdjArgs{});
         HIPCHECK(hipGetLastError());
     }

@@ -162,15 +162,16 @@ int vp_plan_device_A(vp_plan *plan, double **A, double **b, long long *lda)
     return VP_OK;
 }
 
-int vp_plan_adjoint(vp_plan *plan, const double *lambda, const double *mu, const double *vinf, const double *dvinf,
-                    double *grad, double *lambda_b_dot)
+int vp_plan_adjoint(vp_plan *plan, int nlam, const double *lambda, const double *mu, const double *vinf,
+                    const double *dvinf, double *grad, double *lambda_b_dot)
 {
     if (!plan || !plan->p.ctx) return VP_E_ARGUMENT;
     vp::Ctx *c = plan->p.ctx;
+    if (nlam < 1) return fail(c, VP_E_ARGUMENT, "nlam must be at least 1");
     if (!lambda || !mu || !vinf || !grad || (dvinf && !lambda_b_dot)) return fail(c, VP_E_ARGUMENT, "null argument");
     return guard(c, [&] {
         (void)hipSetDevice(c->device);
-        plan->p.adjoint(lambda, mu, vinf, dvinf, grad, dvinf ? lambda_b_dot : nullptr);
+        plan->p.adjoint(nlam, lambda, mu, vinf, dvinf, grad, dvinf ? lambda_b_dot : nullptr);
     });
 }
 

@@ -12,6 +12,7 @@ namespace vp {
 struct Elem;
 struct TestFar;
 struct AsmParams;
+struct AdjArgs;
 
 struct Ctx {
     int device = 0;
@@ -79,7 +80,9 @@ struct Plan {
     void wake_product(const double *mu_nodes, double *out);
     void blob_rhs(int nb, const double *bx, const double *ba, const double *bd, double *out);
     void read_stats(unsigned long long *h);
-    void adjoint(const double *lam, const double *mu, const double *vinf, const double *dvinf, double *G, double *bdot);
+    void adjoint(int nlam, const double *lam, const double *mu, const double *vinf, const double *dvinf, double *G,
+                 double *bdot);
+    template <int NL> void adjoint_pass(const AdjArgs &aa);
 };
 
 // The factored bordered system (solve.jl:65-87): the original FP64 bordered matrix M (row-major, npad x npad, the

@@ -184,19 +184,23 @@ class WingAdjoint:
         a = np.radians(self.alpha)
         return np.array([-np.sin(a), 0.0, np.cos(a)])  # dV∞/dalpha per radian
 
-    def _backward(self, lam):
-        """d/dx [lamᵀ (b - A [mu; mu_wake])] on the device, wake node gradients moved to their trailing edge nodes;
-        and lamᵀ db/dalpha (per radian) from the same pass (b is linear in V∞, so db/dalpha is b of dV∞/dalpha)"""
-        nn = len(self.g.x)
+    def _backward(self, lams):
+        """d/dx [lamᵀ (b - A [mu; mu_wake])] for each row of lams on the device (two per pass of the pair integrals),
+        wake node gradients moved to their trailing edge nodes; and lamᵀ db/dalpha (per radian) from the same pass
+        (b is linear in V∞, so db/dalpha is b of dV∞/dalpha)"""
+        lams = np.atleast_2d(np.asarray(lams, dtype=np.float64))
+        nl, nn = len(lams), len(self.g.x)
         mu_full = L.f64(np.concatenate([self.mu, self.mu_wn]))
-        G = np.zeros((nn + len(self.wake.x), 3))
-        lbd = C.c_double()
-        _check(self.ctx.h, self.ctx.lib.vp_plan_adjoint(self.plan.h, L.ptr(L.f64(lam)), L.ptr(mu_full),
+        G = np.zeros((nl, nn + len(self.wake.x), 3))
+        lbd = np.zeros(nl)
+        _check(self.ctx.h, self.ctx.lib.vp_plan_adjoint(self.plan.h, nl, L.ptr(L.f64(lams)), L.ptr(mu_full),
                                                         L.ptr(L.f64(self.V)), L.ptr(L.f64(self._dV())), L.ptr(G),
-                                                        C.byref(lbd)))
-        gx = G[:nn].copy()
-        np.add.at(gx, self.dm.wake_te.node[self.wake.te], G[nn:])
-        return gx, lbd.value
+                                                        L.ptr(lbd)))
+        gx = G[:, :nn].copy()
+        te = self.dm.wake_te.node[self.wake.te]
+        for k in range(nl):
+            np.add.at(gx[k], te, G[k, nn:])
+        return gx, lbd
 
     def _db_dalpha(self):
         """db/dalpha at fixed geometry and wake (per radian) by a second assembly, made once (a check of the
@@ -208,28 +212,34 @@ class WingAdjoint:
             self._bd = self.plan.read_b()
         return self._bd
 
-    def gradient(self, objective="CL"):
-        """dJ/dx (nn, 3) and dJ/dalpha (per degree) of J = CL or CDi"""
-        t0 = time.perf_counter()
+    def _explicit(self, objective):
         if objective == "CL":
             J, dmu, dx, da = cl_explicit(self.g, self.dm, self.g.x, self.mu, self.alpha, self.S)
-            dJdq = self.F.C.T @ dmu
-        elif objective == "CDi":
+            return J, self.F.C.T @ dmu, dx, da
+        if objective == "CDi":
             J, dmw, dx, da = cdi_explicit(self.dm, self.g.x, self.mu_w, self.alpha, self.S)
-            dJdq = self.dm.K.T @ dmw
-        else:
-            raise ValueError("objective must be CL or CDi")
+            return J, self.dm.K.T @ dmw, dx, da
+        raise ValueError("objective must be CL or CDi")
+
+    def gradients(self, objectives=("CL", "CDi")):
+        """{objective: (J, dJ/dx (nn, 3), dJ/dalpha per degree)} for J in objectives (CL, CDi): one adjoint solve
+        each, one backward pass for every two of them"""
+        t0 = time.perf_counter()
+        ex = [self._explicit(o) for o in objectives]
         t1 = time.perf_counter()
-        lam = self._adjoint_weights(dJdq)
+        lams = np.stack([self._adjoint_weights(e[1]) for e in ex])
         t2 = time.perf_counter()
-        gb, lbd = self._backward(lam)
-        gx = dx + gb
-        self.last_lam, self.last_lam_db = lam, lbd
+        gb, lbd = self._backward(lams)
         t3 = time.perf_counter()
-        dalpha = da + lbd * np.pi / 180
-        self.timings.update({f"{objective}_explicit_s": t1 - t0, f"{objective}_adjoint_solve_s": t2 - t1,
-                             f"{objective}_backward_s": t3 - t2})
-        return J, gx, dalpha
+        self.last_lam, self.last_lam_db = lams, lbd
+        key = "+".join(objectives)
+        self.timings.update({f"{key}_explicit_s": t1 - t0, f"{key}_adjoint_solve_s": t2 - t1,
+                             f"{key}_backward_s": t3 - t2})
+        return {o: (e[0], e[2] + gb[k], e[3] + lbd[k] * np.pi / 180) for k, (o, e) in enumerate(zip(objectives, ex))}
+
+    def gradient(self, objective="CL"):
+        """dJ/dx (nn, 3) and dJ/dalpha (per degree) of J = CL or CDi"""
+        return self.gradients((objective,))[objective]
 
     def close(self):
         self.F.close()
