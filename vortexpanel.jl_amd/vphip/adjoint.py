@@ -41,96 +41,152 @@ def _dot(a, b):
 
 def cl_terms(X, m, V, eL, area_ref):
     """Per-triangle contributions to CL = F.eL/S with F = -sum Cp A n (pressure_force, forces.jl:11-19) and the
-    gradient-route Cp (Post.jl:72-84), as an analytic function of the vertices X (nt, 3, 3) and corner doublets m
-    (nt, 3): complex arguments give complex-step derivatives."""
-    a, b, c = X[:, 0], X[:, 1], X[:, 2]
+    gradient-route Cp (Post.jl:72-84), as an analytic function of the vertices X (..., nt, 3, 3) and corner doublets m
+    (nt, 3): complex arguments give complex-step derivatives, leading axes a batch of them."""
+    a, b, c = X[..., 0, :], X[..., 1, :], X[..., 2, :]
     N = _cross(b - a, c - a)
     A2 = np.sqrt(_dot(N, N))
-    n = N / A2[:, None]
-    e = np.stack([c - b, a - c, b - a], axis=1)
-    gradl = _cross(n[:, None, :], e) / A2[:, None, None]
-    u = V - _dot(n, V)[:, None] * n + np.einsum("tk,tkj->tj", m, gradl)
+    n = N / A2[..., None]
+    e = np.stack([c - b, a - c, b - a], axis=-2)
+    gradl = _cross(n[..., None, :], e) / A2[..., None, None]
+    u = V - _dot(n, V)[..., None] * n + np.einsum("tk,...tkj->...tj", m, gradl)
     cp = 1 - _dot(u, u) / _dot(V, V)
     return -cp * (A2 / 2) * _dot(n, eL) / area_ref, u, gradl, A2, n
 
 
 def cl_explicit(g, dm, x, mu, alpha_deg, area_ref):
-    """CL and its partial derivatives at fixed doublets: d/dmu (nslot), d/dx (nn, 3), d/dalpha (per degree)"""
+    """CL and its partial derivatives at fixed doublets: d/dmu (nslot), d/dx (nn, 3), d/dalpha (per degree).
+
+    With N = (b - a) x (c - a) (q = N.N), E = sum_k m_k e_k (e_k the edge opposite corner k) each term is
+    f = -(1 - |u|^2/|V|^2) (N.eL)/(2S), u = V - (N.V) N/q + (N x E)/q, so d/dx goes through d/dN and d/dE by hand."""
     V, eL = wing_freestream(alpha_deg)
     tris = g.tris
     X = x[tris]
     m = np.asarray(mu)[dm.corner_slot]
     f, u, gradl, A2, n = cl_terms(X, m, V, eL, area_ref)
     CL = float(np.sum(f))
+    VV = np.dot(V, V)
     # d/du of each term, then d/dmu through grad(lambda_k)
-    dfdu = (2 * (A2 / 2) * _dot(n, eL) / (area_ref * _dot(V, V)))[:, None] * u
-    dmu = np.zeros(dm.nslot)
-    np.add.at(dmu, dm.corner_slot, np.einsum("tj,tkj->tk", dfdu, gradl))
+    dfdu = (2 * (A2 / 2) * _dot(n, eL) / (area_ref * VV))[:, None] * u
+    dmu = np.bincount(dm.corner_slot.ravel(), np.einsum("tj,tkj->tk", dfdu, gradl).ravel(), minlength=dm.nslot)
+    a, b, c = X[:, 0], X[:, 1], X[:, 2]
+    N = _cross(b - a, c - a)
+    q = _dot(N, N)
+    E = m[:, 0, None] * (c - b) + m[:, 1, None] * (a - c) + m[:, 2, None] * (b - a)
+    gu = dfdu  # df/du
+    NV, gN, NE = N @ V, _dot(gu, N), _cross(N, E)
+    dN = (-(1 - _dot(u, u) / VV) / (2 * area_ref))[:, None] * eL \
+        - (gN / q)[:, None] * V - (NV / q)[:, None] * gu + (2 * NV * gN / q ** 2)[:, None] * N \
+        + _cross(E, gu) / q[:, None] - (2 * _dot(gu, NE) / q ** 2)[:, None] * N
+    dE = _cross(gu, N) / q[:, None]
+    db, dc = _cross(c - a, dN), _cross(dN, b - a)
+    grads = [-(db + dc) + (m[:, 1] - m[:, 2])[:, None] * dE, db + (m[:, 2] - m[:, 0])[:, None] * dE,
+             dc + (m[:, 0] - m[:, 1])[:, None] * dE]
     dx = np.zeros_like(x)
     for k in range(3):
         for d in range(3):
-            Xc = X.astype(complex)
-            Xc[:, k, d] += 1j * H
-            fk, *_ = cl_terms(Xc, m, V, eL, area_ref)
-            np.add.at(dx[:, d], tris[:, k], fk.imag / H)
-    a = np.radians(alpha_deg) + 1j * H
-    fa, *_ = cl_terms(X, m, np.array([np.cos(a), 0, np.sin(a)]), np.array([-np.sin(a), 0, np.cos(a)]), area_ref)
+            dx[:, d] += np.bincount(tris[:, k], grads[k][:, d], minlength=len(x))
+    al = np.radians(alpha_deg) + 1j * H
+    fa, *_ = cl_terms(X, m, np.array([np.cos(al), 0, np.sin(al)]), np.array([-np.sin(al), 0, np.cos(al)]), area_ref)
     dalpha = float(np.sum(fa).imag / H) * np.pi / 180
     return CL, dmu, dx, dalpha
+
+
+def _trefftz_topology(lines, closed):
+    """The trailing edge lines as index arrays: segment ends (ia, ib) and, per trailing vortex point, its node and the
+    segments after and before it along its line (-1: none)"""
+    ia, ib, vnode, vafter, vbefore = [], [], [], [], []
+    base = 0
+    for line, cl in zip(lines, closed):
+        line = list(line)
+        m = len(line)
+        nseg = m if cl else m - 1
+        for i in range(nseg):
+            ia.append(line[i])
+            ib.append(line[(i + 1) % m])
+        for i in range(m):
+            vnode.append(line[i])
+            vbefore.append(base + i - 1 if i > 0 else (base + nseg - 1 if cl else -1))
+            vafter.append(base + i if i < nseg else -1)
+        base += nseg
+    return tuple(np.asarray(v, dtype=np.int64) for v in (ia, ib, vnode, vafter, vbefore))
+
+
+def _trefftz_batch(P, mu_w, topo, d):
+    """trefftz_drag's D (forces.jl:53-81, before the 2/|V∞|^2 factor) for a batch: P (B, nte, 3), mu_w (B, nte)"""
+    ia, ib, vnode, vafter, vbefore = topo
+    Pp = P - _dot(P, d)[..., None] * d
+    G = (mu_w[:, ia] + mu_w[:, ib]) / 2
+    Gz = np.concatenate([G, np.zeros((len(G), 1), dtype=G.dtype)], axis=1)  # column -1: no segment
+    vg = Gz[:, vafter] - Gz[:, vbefore]
+    sa, sb = Pp[:, ia], Pp[:, ib]
+    c = (sa + sb) / 2
+    ell = np.sqrt(_dot(sb - sa, sb - sa))
+    n = _cross(d, (sb - sa) / ell[..., None])
+    r = c[:, :, None, :] - Pp[:, vnode][:, None, :, :]
+    v = np.einsum("bk,bskj->bsj", vg / (4 * np.pi), _cross(d, r) / _dot(r, r)[..., None])
+    return np.sum(G * _dot(n, v) * ell, axis=-1)
 
 
 def trefftz_of(P, lines, closed, mu_w, d):
     """trefftz_drag (forces.jl:53-81) as an analytic function of the trailing edge node positions P and mu_w (complex
     steps welcome), D in units of q∞ before the 2/|V∞|^2 factor"""
-    Pp = P - _dot(P, d)[:, None] * d
-    vp, vg, sa, sb, sg = [], [], [], [], []
-    for line, cl in zip(lines, closed):
-        line = list(line)
-        m = len(line)
-        nseg = m if cl else m - 1
-        G = [(mu_w[line[i]] + mu_w[line[(i + 1) % m]]) / 2 for i in range(nseg)]
-        for i in range(m):
-            before = G[i - 1] if i > 0 else (G[nseg - 1] if cl else 0.0)
-            after = G[i] if i < nseg else 0.0
-            vp.append(Pp[line[i]])
-            vg.append(after - before)
-        for i in range(nseg):
-            sa.append(Pp[line[i]])
-            sb.append(Pp[line[(i + 1) % m]])
-            sg.append(G[i])
-    vp, vg, sa, sb, sg = (np.asarray(v) for v in (vp, vg, sa, sb, sg))
-    c = (sa + sb) / 2
-    ell = np.sqrt(_dot(sb - sa, sb - sa))
-    n = _cross(d, (sb - sa) / ell[:, None])
-    r = c[:, None, :] - vp[None, :, :]
-    v = np.einsum("k,skj->sj", vg / (4 * np.pi), _cross(d, r) / _dot(r, r)[:, :, None])
-    return np.sum(sg * _dot(n, v) * ell)
+    topo = _trefftz_topology(lines, closed)
+    return _trefftz_batch(np.asarray(P)[None], np.asarray(mu_w)[None], topo, np.asarray(d))[0]
 
 
 def cdi_explicit(dm, x, mu_w, alpha_deg, area_ref):
     """CDi (Trefftz) and its partial derivatives: d/dmu_w (nte), d/dx (nn, 3) through the trailing edge nodes,
-    d/dalpha (per degree)"""
+    d/dalpha (per degree).
+
+    In the plane normal to d (points projected there) each (segment s, trailing vortex k) term of D is
+    G_s g_k (l_s . r_sk) / (4 pi |r_sk|^2), l_s the segment, r_sk from the vortex to the segment's middle, since
+    ell n.(d x r) = (d x l).(d x r) = l.r for vectors normal to d; its derivatives are taken by hand."""
     wte = dm.wake_te
     V, _ = wing_freestream(alpha_deg)
     scale = 2 / np.dot(V, V) / area_ref
     d = V / np.sqrt(np.dot(V, V))
-    P = np.asarray(x)[wte.node]
+    P = np.asarray(x)[wte.node].astype(float)
     mw = np.asarray(mu_w, dtype=float)
-    CD = float(trefftz_of(P, wte.lines, wte.closed, mw, d)) * scale
-    dmw = np.zeros(len(mw))
-    for j in range(len(mw)):
-        e = mw.astype(complex)
-        e[j] += 1j * H
-        dmw[j] = trefftz_of(P, wte.lines, wte.closed, e, d).imag / H * scale
+    nte = len(mw)
+    topo = _trefftz_topology(wte.lines, wte.closed)
+    ia, ib, vnode, vafter, vbefore = topo
+    Pp = P - (P @ d)[:, None] * d
+    G = (mw[ia] + mw[ib]) / 2
+    Gz = np.append(G, 0.0)
+    vg = Gz[vafter] - Gz[vbefore]
+    sa, sb = Pp[ia], Pp[ib]
+    ls = sb - sa
+    r = ((sa + sb) / 2)[:, None, :] - Pp[vnode][None, :, :]  # (nseg, nv, 3)
+    ir2 = 1 / _dot(r, r)
+    lr = _dot(ls[:, None, :], r)
+    Mk = lr * ir2 / (4 * np.pi)  # (nseg, nv)
+    CD = float(G @ Mk @ vg) * scale
+    # d/dmu_w: through G_s and g_k
+    dG = Mk @ vg
+    dvg = G @ Mk
+    dGz = np.zeros(len(G) + 1)
+    np.add.at(dGz, vafter, dvg)
+    np.add.at(dGz, vbefore, -dvg)
+    dG = dG + dGz[:-1]
+    dmw = (np.bincount(ia, dG / 2, minlength=nte) + np.bincount(ib, dG / 2, minlength=nte)) * scale
+    # d/dx: through the segment ends and the vortex points (projected positions), then the projection
+    w = (G[:, None] * vg[None, :]) / (4 * np.pi)  # (nseg, nv)
+    dT_dl = (w * ir2)[..., None] * r
+    dT_dr = (w * ir2)[..., None] * ls[:, None, :] - (2 * w * lr * ir2 ** 2)[..., None] * r
+    gl = dT_dl.sum(1)
+    gr = dT_dr.sum(1)
+    dPp = np.zeros_like(P)
+    np.add.at(dPp, ia, -gl + gr / 2)
+    np.add.at(dPp, ib, gl + gr / 2)
+    np.add.at(dPp, vnode, -dT_dr.sum(0))
+    dP = (dPp - (dPp @ d)[:, None] * d) * scale
     dx = np.zeros_like(x)
-    for j in range(len(P)):
-        for k in range(3):
-            Pc = P.astype(complex)
-            Pc[j, k] += 1j * H
-            dx[wte.node[j], k] += trefftz_of(Pc, wte.lines, wte.closed, mw, d).imag / H * scale
-    a = np.radians(alpha_deg) + 1j * H
-    Vc = np.array([np.cos(a), 0, np.sin(a)])
-    da = trefftz_of(P, wte.lines, wte.closed, mw, Vc / np.sqrt(np.sum(Vc * Vc))) * 2 / np.sum(Vc * Vc) / area_ref
+    np.add.at(dx, wte.node, dP)
+    al = np.radians(alpha_deg) + 1j * H
+    Vc = np.array([np.cos(al), 0, np.sin(al)])
+    da = _trefftz_batch(P[None].astype(complex), mw[None].astype(complex), topo, Vc / np.sqrt(np.sum(Vc * Vc)))[0]
+    da = da * 2 / np.sum(Vc * Vc) / area_ref
     return CD, dmw, dx, float(da.imag / H) * np.pi / 180
 
 
