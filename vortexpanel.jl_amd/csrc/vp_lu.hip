@@ -52,6 +52,9 @@ template <> struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
+constexpr int NB = 64;    // diagonal block width of the LU (the blocks whose inverses the triangular solves use)
+constexpr int NB2 = 256;  // outer panel width of the LU: the K of its trailing updates
+
 // ------------------------------------------------------------------------------------------ GEMM (MFMA) ------------
 
 // C = A B (SET) or C -= A B (SUB), row-major, arbitrary M, N, K; with TA, A is read transposed (element (m, k) at
@@ -199,7 +202,7 @@ void launch_gemm_cfg(int cfg, hipStream_t s, int M, int N, int K, const T *A, lo
 }
 
 // The autotuner: the trailing update's tile shape per precision, chosen by timing every candidate (HIP events, three
-// runs after a warm-up) on the first factorization's own update shape (C -= A B with K = 64, M = N = the matrix size
+// runs after a warm-up) on the first factorization's own update shape (C -= A B with K = NB2, M = N = the matrix size
 // less one panel, at most 8192) in its factor buffer before the matrix is written there; kept for the process.
 // VP_GEMM_TILE=<index> forces a candidate.
 struct GemmTuning {
@@ -221,18 +224,18 @@ template <typename T> int tuned_cfg(hipStream_t s, T *scratch, long long ld, int
         tu.cfg = (c >= 0 && c < kNumTiles) ? c : 0;
         return tu.cfg;
     }
-    const int m = std::min(npad - 64, 8192);
+    const int m = std::min(npad - NB2, 8192);
     if (m < 1024) return 0;  // too small to time well: the default shape, tuned later on a larger matrix
-    HIPCHECK2(hipMemsetAsync(scratch, 0, sizeof(T) * (size_t)ld * (m + 64), s));
+    HIPCHECK2(hipMemsetAsync(scratch, 0, sizeof(T) * (size_t)ld * (m + NB2), s));
     hipEvent_t e0, e1;
     HIPCHECK2(hipEventCreate(&e0));
     HIPCHECK2(hipEventCreate(&e1));
     int best = 0;
     for (int c = 0; c < kNumTiles; c++) {
-        T *A = scratch + 64 * ld, *B = scratch + 64, *C = scratch + 64 * ld + 64;
-        launch_gemm_cfg<T, true, false>(c, s, m, m, 64, A, ld, B, ld, C, ld);
+        T *A = scratch + NB2 * ld, *B = scratch + NB2, *C = scratch + NB2 * ld + NB2;
+        launch_gemm_cfg<T, true, false>(c, s, m, m, NB2, A, ld, B, ld, C, ld);
         HIPCHECK2(hipEventRecord(e0, s));
-        for (int r = 0; r < 3; r++) launch_gemm_cfg<T, true, false>(c, s, m, m, 64, A, ld, B, ld, C, ld);
+        for (int r = 0; r < 3; r++) launch_gemm_cfg<T, true, false>(c, s, m, m, NB2, A, ld, B, ld, C, ld);
         HIPCHECK2(hipEventRecord(e1, s));
         HIPCHECK2(hipEventSynchronize(e1));
         float t = 0;
@@ -274,10 +277,11 @@ void launch_gemm(hipStream_t s, int M, int N, int K, const T *A, long long lda, 
 
 // ------------------------------------------------------------------------------------------ LU pieces --------------
 
-constexpr int NB = 64;  // panel width
 
 // Factor the NB x NB diagonal block at (k0, k0) in place without pivoting (D = L U, L unit lower), and write the
 // inverses of L and U (NB x NB, row-major) for the panel's triangular solves.  A zero or non-finite pivot sets *bad.
+// Every stage is right-looking over the whole workgroup: the elimination, then L^-1 and U^-1 from the identity by
+// rank-1 updates (row k of the inverse is final once the updates of the rows before it, or after it for U, are in).
 template <typename T>
 __global__ __launch_bounds__(256) void diag_lu(T *G, long long ld, int k0, int nb, T *Linv, T *Uinv, int *bad)
 {
@@ -287,14 +291,16 @@ __global__ __launch_bounds__(256) void diag_lu(T *G, long long ld, int k0, int n
     for (int idx = tid; idx < NB * NB; idx += 256) {
         int i = idx / NB, j = idx % NB;
         D[i][j] = (i < nb && j < nb) ? G[(long long)(k0 + i) * ld + k0 + j] : T(i == j ? 1 : 0);
+        X[i][j] = T(i == j ? 1 : 0);
     }
     __syncthreads();
     for (int k = 0; k < nb; k++) {
-        T p = D[k][k];
+        const T p = D[k][k];
         if (tid == 0 && (!(p != T(0)) || !isfinite((double)p))) atomicExch(bad, 1);
-        for (int i = k + 1 + tid; i < nb; i += 256) D[i][k] /= p;
+        const T ip = T(1) / p;
+        for (int i = k + 1 + tid; i < nb; i += 256) D[i][k] *= ip;
         __syncthreads();
-        int m = nb - k - 1;
+        const int m = nb - k - 1;
         for (int idx = tid; idx < m * m; idx += 256) {
             int i = k + 1 + idx / m, j = k + 1 + idx % m;
             D[i][j] -= D[i][k] * D[k][j];
@@ -305,30 +311,31 @@ __global__ __launch_bounds__(256) void diag_lu(T *G, long long ld, int k0, int n
         int i = idx / nb, j = idx % nb;
         G[(long long)(k0 + i) * ld + k0 + j] = D[i][j];
     }
-    // inverse of the unit lower L: column j by forward substitution, one thread per column
-    for (int j = tid; j < NB; j += 256) {
-        for (int i = 0; i < NB; i++) {
-            T s = T(i == j ? 1 : 0);
-            for (int k = j; k < i; k++) s -= D[i][k] * X[k][j];
-            X[i][j] = i < j ? T(0) : s;
+    // L^-1 (unit lower): after step k, rows i > k have lost L[i][k] x (row k)
+    for (int k = 0; k < NB - 1; k++) {
+        const int w = k + 1, cnt = (NB - 1 - k) * w;
+        for (int idx = tid; idx < cnt; idx += 256) {
+            const int i = k + 1 + idx / w, j = idx % w;
+            X[i][j] -= D[i][k] * X[k][j];
         }
+        __syncthreads();
     }
-    __syncthreads();
     for (int idx = tid; idx < NB * NB; idx += 256) Linv[idx] = X[idx / NB][idx % NB];
     __syncthreads();
-    // inverse of the upper U: column j by back substitution
-    for (int j = tid; j < NB; j += 256) {
-        for (int i = NB - 1; i >= 0; i--) {
-            if (i > j) {
-                X[i][j] = T(0);
-                continue;
-            }
-            T s = T(i == j ? 1 : 0);
-            for (int k = i + 1; k <= j; k++) s -= D[i][k] * X[k][j];
-            X[i][j] = s / D[i][i];
-        }
-    }
+    for (int idx = tid; idx < NB * NB; idx += 256) X[idx / NB][idx % NB] = T(idx / NB == idx % NB ? 1 : 0);
     __syncthreads();
+    // U^-1: from the last row up, row k is scaled by 1/U[k][k], then rows i < k lose U[i][k] x (row k)
+    for (int k = NB - 1; k >= 0; k--) {
+        const T ip = T(1) / D[k][k];
+        for (int j = k + tid; j < NB; j += 256) X[k][j] *= ip;
+        __syncthreads();
+        const int w = NB - k, cnt = k * w;
+        for (int idx = tid; idx < cnt; idx += 256) {
+            const int i = idx / w, j = k + idx % w;
+            X[i][j] -= D[i][k] * X[k][j];
+        }
+        __syncthreads();
+    }
     for (int idx = tid; idx < NB * NB; idx += 256) Uinv[idx] = X[idx / NB][idx % NB];
 }
 
@@ -583,22 +590,42 @@ Factor::~Factor()
 
 static unsigned blocks(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
+// Right-looking blocked LU without pivoting, two levels: panels of NB2 columns factored in steps of NB (updates kept
+// inside the panel), then the panel's row block U12 = L11^-1 A12 by block forward substitution, then the trailing
+// update A22 -= L21 U12 with K = NB2 (the autotuned tile shape), which keeps the update's HBM traffic per flop low.
 template <typename T> void lu_factor(hipStream_t s, T *G, long long ld, int npad, T *linv, T *uinv, int *bad, int cfg)
 {
-    for (int k0 = 0; k0 < npad; k0 += NB) {
-        int nb = std::min(NB, npad - k0);
-        diag_lu<T><<<1, 256, 0, s>>>(G, ld, k0, nb, linv + (size_t)(k0 / NB) * NB * NB, uinv + (size_t)(k0 / NB) * NB * NB, bad);
-        HIPCHECK2(hipGetLastError());
-        int m = npad - k0 - nb;
-        if (m <= 0) break;
-        T *D = G + (long long)k0 * ld + k0;
-        // L21 = A21 U11^-1 (in place: one workgroup column covers the panel's NB columns)
-        launch_gemm<T, false>(s, m, nb, nb, D + (long long)nb * ld, ld, uinv + (size_t)(k0 / NB) * NB * NB, NB,
-                              D + (long long)nb * ld, ld);
-        // U12 = L11^-1 A12 (in place: one workgroup row covers the panel's NB rows)
-        launch_gemm<T, false>(s, nb, m, nb, linv + (size_t)(k0 / NB) * NB * NB, NB, D + nb, ld, D + nb, ld);
-        // A22 -= L21 U12 (the autotuned tile shape)
-        launch_gemm_cfg<T, true, false>(cfg, s, m, m, nb, D + (long long)nb * ld, ld, D + nb, ld, D + (long long)nb * ld + nb, ld);
+    auto Li = [&](int k0) { return linv + (size_t)(k0 / NB) * NB * NB; };
+    auto Ui = [&](int k0) { return uinv + (size_t)(k0 / NB) * NB * NB; };
+    for (int K0 = 0; K0 < npad; K0 += NB2) {
+        const int w = std::min(NB2, npad - K0);
+        for (int k0 = K0; k0 < K0 + w; k0 += NB) {
+            const int nb = std::min(NB, npad - k0);
+            diag_lu<T><<<1, 256, 0, s>>>(G, ld, k0, nb, Li(k0), Ui(k0), bad);
+            HIPCHECK2(hipGetLastError());
+            const int m = npad - k0 - nb, wr = K0 + w - k0 - nb;
+            if (m <= 0) break;
+            T *D = G + (long long)k0 * ld + k0;
+            // L21 = A21 U11^-1 (in place: one workgroup column covers the block's NB columns)
+            launch_gemm<T, false>(s, m, nb, nb, D + (long long)nb * ld, ld, Ui(k0), NB, D + (long long)nb * ld, ld);
+            if (wr > 0) {
+                // U12 = L11^-1 A12 and A22 -= L21 U12, both within the panel's columns
+                launch_gemm<T, false>(s, nb, wr, nb, Li(k0), NB, D + nb, ld, D + nb, ld);
+                launch_gemm<T, true>(s, m, wr, nb, D + (long long)nb * ld, ld, D + nb, ld, D + (long long)nb * ld + nb, ld);
+            }
+        }
+        const int nr = npad - K0 - w;
+        if (nr <= 0) break;
+        T *R = G + (long long)K0 * ld + K0 + w;  // the panel's row block right of it
+        for (int k0 = K0; k0 < K0 + w; k0 += NB) {
+            T *Xk = R + (long long)(k0 - K0) * ld;
+            launch_gemm<T, false>(s, NB, nr, NB, Li(k0), NB, Xk, ld, Xk, ld);
+            const int below = K0 + w - k0 - NB;
+            if (below > 0)
+                launch_gemm<T, true>(s, below, nr, NB, G + (long long)(k0 + NB) * ld + k0, ld, Xk, ld, Xk + (long long)NB * ld, ld);
+        }
+        launch_gemm_cfg<T, true, false>(cfg, s, nr, nr, w, G + (long long)(K0 + w) * ld + K0, ld, R, ld,
+                                        G + (long long)(K0 + w) * ld + K0 + w, ld);
     }
 }
 
