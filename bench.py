@@ -228,7 +228,7 @@ def main():
     tm = p.timings()
     value = pairs / (dt / args.steps)
     log(f"fp64 assemble {ms_step:.2f} ms/step, {value:.3e} pair-integrals/s, kernels {tm}")
-    # roofline of the dominant kernel (far tier)
+    # roofline of the dominant kernel (the far tier)
     far_pairs_n = p.pairs["far"]
     ex = tm["far_exact_points"]
     flops = ex * (FLOPS_EXACT + FLOPS_BLOCK) + (3 * far_pairs_n - ex) * (FLOPS_MULTIPOLE + FLOPS_BLOCK)

@@ -1,0 +1,61 @@
+"""The device's own FP64 log and atan2 (the closed-form kernel's transcendentals) against the host C library's, through
+the C ABI's probe: within 2 ulp over the whole range, the special values and signed zeros as the library gives them."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from vphip import _lib as L, api
+
+pytestmark = pytest.mark.gpu
+
+
+def _probe(fn, a, b=None):
+    ctx = api.default_context()
+    a = L.f64(a)
+    b = L.f64(b) if b is not None else None
+    out = np.zeros_like(a)
+    rc = ctx.lib.vp_math_probe(ctx.h, fn, len(a), L.ptr(a), L.ptr(b) if b is not None else None, L.ptr(out))
+    assert rc == 0
+    return out
+
+
+def _ulps(out, ref):
+    return np.abs(out - ref) / np.spacing(np.abs(ref))
+
+
+def test_log():
+    print("device log", flush=True)
+    rng = np.random.default_rng(3)
+    x = np.concatenate([10.0 ** rng.uniform(-307, 307, 20000), 1 + rng.uniform(-1e-3, 1e-3, 5000),
+                        rng.uniform(0.5, 2.0, 5000), 1 + rng.uniform(-1e-12, 1e-12, 1000), [1.0, 2.0, 0.5, np.e],
+                        [5e-324, 2.2e-308, 1.7e308, 0.7071067811865476, 1.4142135623730951]])
+    out, ref = _probe(0, x), np.log(x)
+    nz = ref != 0
+    assert np.max(_ulps(out[nz], ref[nz])) <= 2.0
+    assert out[~nz].max() == 0.0
+    sp = np.array([0.0, -0.0, np.inf, -1.0, np.nan])
+    o = _probe(0, sp)
+    assert o[0] == -np.inf and o[1] == -np.inf and o[2] == np.inf and np.isnan(o[3]) and np.isnan(o[4])
+
+
+def test_atan2():
+    print("device atan2", flush=True)
+    rng = np.random.default_rng(4)
+    n = 40000
+    y = rng.standard_normal(n) * 10.0 ** rng.uniform(-20, 20, n)
+    x = rng.standard_normal(n) * 10.0 ** rng.uniform(-20, 20, n)
+    t = np.tan(np.linspace(-np.pi / 2 + 1e-9, np.pi / 2 - 1e-9, 2001))  # every octant boundary region
+    y = np.concatenate([y, t, -t, np.ones_like(t)])
+    x = np.concatenate([x, np.ones_like(t), -np.ones_like(t), t])
+    out, ref = _probe(1, y, x), np.arctan2(y, x)
+    nz = ref != 0
+    assert np.max(_ulps(out[nz], ref[nz])) <= 2.0
+    zs = np.array([0.0, -0.0])
+    for yy in zs:
+        for xx in np.concatenate([zs, [1.0, -1.0]]):
+            o = _probe(1, np.array([yy]), np.array([xx]))[0]
+            r = np.arctan2(yy, xx)
+            assert o == r and np.signbit(o) == np.signbit(r), (yy, xx, o, r)
+    o = _probe(1, np.array([1.0, -2.0]), np.array([0.0, -0.0]))
+    assert np.allclose(o, [np.pi / 2, -np.pi / 2], rtol=0, atol=1e-16)

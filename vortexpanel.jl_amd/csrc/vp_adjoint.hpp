@@ -13,12 +13,11 @@
 //
 //     P_ts = -A_T sum_q w_q psi_lambda(q) Q.w(x_q; S),   Q = sigma_S n_T + n_T x Gamma_S,
 //
-// (pair_block, assemble.jl:245-258, contracted with lambda and mu) is differentiated: the kernel w by forward-mode dual
-// numbers with respect to the three vertices of S (three passes of three tangents; the derivative with respect to the
-// point is minus their sum, by translation invariance) or, for the multipole, its reverse derivative by hand; the test
-// triangle's area and normal, the source's normal
-// (sigma) and sheet (Gamma) by their closed-form adjoints.  Filaments (segment_block, :312-321) and the freestream
-// term (:422) likewise.  Gradients are added per node: body nodes, then wake nodes (offset by the body's node count).
+// (pair_block, assemble.jl:245-258, contracted with lambda and mu) is differentiated: the kernel w (closed form or
+// multipole) by its reverse derivative by hand with respect to the three vertices of S (the derivative with respect
+// to the point is minus their sum, by translation invariance); the test triangle's area and normal, the source's
+// normal (sigma) and sheet (Gamma) by their closed-form adjoints; filaments (segment_block, :312-321) by dual numbers;
+// the freestream term (:422) by hand.  Gradients are added per node: body nodes, then wake nodes (offset by the body's node count).
 //
 // This file is included by vp_assemble.hip (it uses the plan's tiles, lists and rule generator).
 

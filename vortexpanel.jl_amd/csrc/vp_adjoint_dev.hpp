@@ -1,6 +1,6 @@
-// Device helpers of the backward assembly (vp_adjoint.hpp has its kernels and the host side): dual numbers, the
-// source kernel from its vertices in dual numbers, the octupole multipole's reverse derivative, the closed-form
-// adjoints of a triangle's normal, area and sheet, and the per-pair sums.  Included by vp_assemble.hip ahead of the
+// Device helpers of the backward assembly (vp_adjoint.hpp has its kernels and the host side): dual numbers (for the
+// filament), the reverse derivatives of the octupole multipole and of the closed form, the closed-form adjoints of a
+// triangle's normal, area and sheet, and the per-pair sums.  Included by vp_assemble.hip ahead of the
 // evaluation kernels, which take an adjoint mode.
 #pragma once
 
@@ -57,47 +57,6 @@ __device__ __forceinline__ V3<D3> dconst(V3<double> x) { return mk3(D3(x.x), D3(
 __device__ __forceinline__ V3<D3> dseed(V3<double> x, double on)
 {
     return mk3(D3(x.x, on, 0.0, 0.0), D3(x.y, 0.0, on, 0.0), D3(x.z, 0.0, 0.0, on));
-}
-
-// source_velocity_exact (src/kernels/source_triangle.jl:75-89) with the triangle's data formed from its vertices
-// (the constructor, :18-27), in dual numbers.  Off the own element the solid angle is always the atan2 form (its
-// derivative across the plane is the one the closed form has; the in-plane branch's zero has none).
-__device__ __forceinline__ V3<D3> sv_exact_vertices(const V3<D3> *v, V3<double> x, int side, const Elem *self,
-                                                    double diam)
-{
-    const V3<D3> e0 = v[1] - v[0], e1 = v[2] - v[1], e2 = v[0] - v[2];
-    const V3<D3> nA = cross(e0, v[2] - v[0]);
-    const D3 A2 = dsqrt(dot(nA, nA));
-    const V3<D3> n = (D3(1.0) / A2) * nA;
-    const D3 l0 = dsqrt(dot(e0, e0)), l1 = dsqrt(dot(e1, e1)), l2 = dsqrt(dot(e2, e2));
-    const V3<D3> xd = dconst(x);
-    const V3<D3> ra = xd - v[0], rb = xd - v[1], rc = xd - v[2];
-    const V3<D3> nu0 = (D3(1.0) / l0) * cross(e0, n), nu1 = (D3(1.0) / l1) * cross(e1, n),
-                 nu2 = (D3(1.0) / l2) * cross(e2, n);
-    const V3<D3> wt = edge_log(ra, rb, l0) * nu0 + edge_log(rb, rc, l1) * nu1 + edge_log(rc, ra, l2) * nu2;
-    D3 Om;
-    const V3<D3> c = D3(1.0 / 3) * ((v[0] + v[1]) + v[2]);
-    const D3 z = dot(xd - c, n);
-    if (side != 0 && fabs(z.v) <= 1e-12 * diam) {
-        Om = D3(side * 2 * kPi * inplane_fraction(*self, x, 1e-12));
-    } else {
-        const D3 la = dsqrt(dot(ra, ra)), lb = dsqrt(dot(rb, rb)), lc = dsqrt(dot(rc, rc));
-        const D3 N = dot(ra, cross(rb, rc));
-        const D3 D = la * lb * lc + dot(ra, rb) * lc + dot(ra, rc) * lb + dot(rb, rc) * la;
-        Om = D3(2.0) * datan2(N, D);
-    }
-    return D3(1.0 / (4 * kPi)) * (wt + Om * n);
-}
-
-// source_velocity_far (source_triangle.jl:100-119) from the vertices, in dual numbers
-__device__ __forceinline__ V3<D3> sv_far_vertices(const V3<D3> *v, V3<double> x)
-{
-    const V3<D3> c = D3(1.0 / 3) * ((v[0] + v[1]) + v[2]);
-    const V3<D3> nA = cross(v[1] - v[0], v[2] - v[0]);
-    SrcT<D3> s;
-    for (int k = 0; k < 3; k++) s.v[k] = v[k] - c;
-    s.area = D3(0.5) * dsqrt(dot(nA, nA));
-    return sv_far_rel(s, dconst(x) - c);
 }
 
 // ------------------------------------------------------------------------------------------ triangle adjoints ------

@@ -247,25 +247,78 @@ __device__ __forceinline__ V3<double> far_eval(const FarSrc<T> &f, const Elem *_
     return sv_exact_abs(s, ra, rb, rc, dot(r, s.n), 0, nullptr, x);
 }
 
+// The pair_block of a far pair (assemble.jl:245-258) from its three far-rule points: blk[a][b] += psi_a (w x n).gamma_b,
+// rb[a] -= sigma_S psi_a (w.n), with the weights W psi folded into F.wpsi; EXACT admits closed-form points
+template <typename T, bool EXACT>
+__device__ __forceinline__ void far_block(const FarSrc<T> &f, const Elem *__restrict__ E, const TestFar &F, double kfar2,
+                                          double (*blk)[3], double *rb, unsigned int &nexact)
+{
+    const V3<double> n = ld3(F.n);
+#pragma unroll 1
+    for (int q = 0; q < 3; q++) {
+        V3<double> w;
+        if constexpr (EXACT) {
+            w = far_eval<T>(f, E, ld3(F.x[q]), kfar2, nexact);
+        } else {
+            const V3<double> r = ld3(F.x[q]) - f.c;
+            const V3<T> wt = sv_far_rel(f.s, mk3(T(r.x), T(r.y), T(r.z)));
+            w = mk3((double)wt.x, (double)wt.y, (double)wt.z);
+        }
+        V3<double> vv = cross(w, n);  // n.(gamma x w) = gamma.(w x n), assemble.jl:251
+        double c0 = dot(vv, mk3((double)f.g[0].x, (double)f.g[0].y, (double)f.g[0].z));
+        double c1 = dot(vv, mk3((double)f.g[1].x, (double)f.g[1].y, (double)f.g[1].z));
+        double c2 = dot(vv, mk3((double)f.g[2].x, (double)f.g[2].y, (double)f.g[2].z));
+        double wn = (double)f.sigma * dot(w, n);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            double psi = F.wpsi[q][a];
+            blk[a][0] += psi * c0;
+            blk[a][1] += psi * c1;
+            blk[a][2] += psi * c2;
+            rb[a] -= wn * psi;
+        }
+    }
+}
+
+// A far tile's LDS sums (blocks per local test slot x local source column, RHS per local test slot) into A and b
+__device__ __forceinline__ void far_flush(const AsmParams &P, const double *R, const double *rbacc, int tt, int st,
+                                          const int *tt_rows, const int *tt_nrow, const int *ts_cols, const int *ts_ncol,
+                                          double *A, double *bvec)
+{
+    const int nrow = tt_nrow[tt], ncol = ts_ncol[st];
+    for (int i = threadIdx.x; i < nrow * UMAX; i += blockDim.x) {
+        int v = i / UMAX, u = i - v * UMAX;
+        double val = R[v * UMAX + u];
+        if (u < ncol && val != 0.0) unsafeAtomicAdd(&A[(long long)ts_cols[st * UMAX + u] * P.lda + tt_rows[tt * VMAX + v]], val);
+    }
+    for (int v = threadIdx.x; v < nrow; v += blockDim.x)
+        if (rbacc[v] != 0.0) unsafeAtomicAdd(&bvec[tt_rows[tt * VMAX + v]], rbacc[v]);
+}
+
+// The far tier over tiles of TT_MAX test triangles (Morton order, waves) x TS source elements (lanes): pair_rule's far
+// test (FP64, no contraction: bit-exact tiers), non-far pairs appended to the near list, far pairs evaluated at their
+// three far-rule points with source_velocity's switch and summed per local test slot and source column in LDS, one
+// flush per tile.  The closed form is rare here but carries most of the kernel's registers; capping the kernel at two
+// waves per SIMD (a few registers spill in that path) beats the one wave it would otherwise get.  enum_only: the near
+// list alone, its second pass after it grew.
 template <typename T>
-__global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__restrict__ tf, const Elem *__restrict__ el,
-                                                 const double *__restrict__ sigma, const int *__restrict__ tt_elems,
-                                                 const uint8_t *__restrict__ tt_loc, const int *__restrict__ tt_rows,
-                                                 const int *__restrict__ tt_nrow, const int *__restrict__ ts_elems,
-                                                 const uint8_t *__restrict__ ts_loc, const int *__restrict__ ts_cols,
-                                                 const int *__restrict__ ts_ncol, double *__restrict__ A,
-                                                 double *__restrict__ bvec, int2 *__restrict__ near_list,
-                                                 unsigned int *__restrict__ near_count, unsigned int near_cap,
-                                                 unsigned long long *__restrict__ far_count, uint8_t *__restrict__ tiers,
-                                                 int st0, bool enum_only)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void far_pairs(
+    AsmParams P, const TestFar *__restrict__ tf, const Elem *__restrict__ el, const double *__restrict__ sigma,
+    const int *__restrict__ tt_elems, const uint8_t *__restrict__ tt_loc, const int *__restrict__ tt_rows,
+    const int *__restrict__ tt_nrow, const int *__restrict__ ts_elems, const uint8_t *__restrict__ ts_loc,
+    const int *__restrict__ ts_cols, const int *__restrict__ ts_ncol, double *__restrict__ A, double *__restrict__ bvec,
+    int2 *__restrict__ near_list, unsigned int *__restrict__ near_count, unsigned int near_cap,
+    unsigned long long *__restrict__ far_count, uint8_t *__restrict__ tiers, int st0, bool enum_only)
 {
     __shared__ double R[VMAX * UMAX];
     __shared__ double rbacc[VMAX];
     __shared__ unsigned int nfar, nexw;
     const int tt = blockIdx.x, st = blockIdx.y + st0;
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < VMAX * UMAX; i += blockDim.x) R[i] = 0.0;
-    if (threadIdx.x < VMAX) rbacc[threadIdx.x] = 0.0;
+    if (!enum_only) {
+        for (int i = threadIdx.x; i < VMAX * UMAX; i += blockDim.x) R[i] = 0.0;
+        if (threadIdx.x < VMAX) rbacc[threadIdx.x] = 0.0;
+    }
     if (threadIdx.x == 0) nfar = nexw = 0;
     const int s = ts_elems[st * TS + lane];
     const bool valid = s >= 0;
@@ -282,7 +335,6 @@ __global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__r
         const int t = tt_elems[tt * TT_MAX + i];
         if (t < 0) break;
         const TestFar &F = tf[t];
-        const int v0 = tt_loc[(tt * TT_MAX + i) * 3], v1 = tt_loc[(tt * TT_MAX + i) * 3 + 1], v2 = tt_loc[(tt * TT_MAX + i) * 3 + 2];
         bool far = false;
         if (valid && s != t) {
             // pair_rule's far test, src/assembly/assemble.jl:269 (bit-exact: FP64, no contraction)
@@ -290,30 +342,14 @@ __global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__r
             far = cls_norm(d) > P.far_ratio * fmax(F.diam, f.diam);
         }
         append_pair(valid && !far, t, s, near_list, near_count, near_cap);
-        if (enum_only) continue;  // the list's second pass after it grew: the far blocks are already in A
+        if (enum_only) continue;  // the far blocks are in A already
         if (tiers && valid && far) tiers[(size_t)t * P.nsrc + s] = T_FAR;
         if (far) {
             myfar++;
-            V3<double> n = ld3(F.n);
             double blk[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, rb[3] = {0, 0, 0};
-#pragma unroll 1
-            for (int q = 0; q < 3; q++) {
-                V3<double> w = far_eval<T>(f, el + s, ld3(F.x[q]), kfar2, myex);
-                V3<double> vv = cross(w, n);  // n.(gamma x w) = gamma.(w x n), assemble.jl:251
-                double c0 = dot(vv, mk3((double)f.g[0].x, (double)f.g[0].y, (double)f.g[0].z));
-                double c1 = dot(vv, mk3((double)f.g[1].x, (double)f.g[1].y, (double)f.g[1].z));
-                double c2 = dot(vv, mk3((double)f.g[2].x, (double)f.g[2].y, (double)f.g[2].z));
-                double wn = (double)f.sigma * dot(w, n);
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    double psi = F.wpsi[q][a];
-                    blk[a][0] += psi * c0;
-                    blk[a][1] += psi * c1;
-                    blk[a][2] += psi * c2;
-                    rb[a] -= wn * psi;
-                }
-            }
-            const int vl[3] = {v0, v1, v2};
+            far_block<T, true>(f, el + s, F, kfar2, blk, rb, myex);
+            const int vl[3] = {tt_loc[(tt * TT_MAX + i) * 3], tt_loc[(tt * TT_MAX + i) * 3 + 1],
+                               tt_loc[(tt * TT_MAX + i) * 3 + 2]};
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -322,17 +358,11 @@ __global__ __launch_bounds__(256) void far_pairs(AsmParams P, const TestFar *__r
             for (int a = 0; a < 3; a++) atomicAdd(&rbacc[vl[a]], rb[a]);
         }
     }
+    if (enum_only) return;
     if (myfar) atomicAdd(&nfar, myfar);
     if (myex) atomicAdd(&nexw, myex);
     __syncthreads();
-    const int nrow = tt_nrow[tt], ncol = ts_ncol[st];
-    for (int i = threadIdx.x; i < nrow * UMAX; i += blockDim.x) {
-        int v = i / UMAX, u = i - v * UMAX;
-        double val = R[v * UMAX + u];
-        if (u < ncol && val != 0.0) unsafeAtomicAdd(&A[(long long)ts_cols[st * UMAX + u] * P.lda + tt_rows[tt * VMAX + v]], val);
-    }
-    for (int v = threadIdx.x; v < nrow; v += blockDim.x)
-        if (rbacc[v] != 0.0) unsafeAtomicAdd(&bvec[tt_rows[tt * VMAX + v]], rbacc[v]);
+    far_flush(P, R, rbacc, tt, st, tt_rows, tt_nrow, ts_cols, ts_ncol, A, bvec);
     if (threadIdx.x == 0 && nfar) atomicAdd(far_count, (unsigned long long)nfar);
     if (threadIdx.x == 0 && nexw) atomicAdd(far_count + 7, (unsigned long long)nexw);  // stats[12]: exact far points
 }
@@ -1767,6 +1797,7 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
         }
     };
     far_pass(false);
+    if (timing) HIPCHECK(hipEventRecord(ev[1], st));
     unsigned int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
     HIPCHECK(hipMemcpyAsync(cnt, d_count, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
@@ -1790,7 +1821,6 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
         HIPCHECK(hipStreamSynchronize(st));
         if (cnt[0] > nonfar_cap) throw std::runtime_error("the non-far pair count changed between passes");
     }
-    if (timing) HIPCHECK(hipEventRecord(ev[1], st));
     nnonfar = cnt[0];
     if (cnt[0]) {
         triage<<<std::min<unsigned int>((cnt[0] + 255) / 256, 65535u), 256, 0, st>>>(
@@ -1875,6 +1905,7 @@ void Plan::kernel_ms(double *ms)
     }
     ms[5] = (double)nnear;
     ms[6] = (double)nleaves;
+
     unsigned long long h[16];
     read_stats(h);
     ms[7] = (double)h[12];

@@ -119,6 +119,7 @@ void body_velocity_host(Ctx *c, const vp_surface &g, const double *sigma, const 
 void blob_velocities_host(Ctx *c, int nb, const double *bx, const double *ba, const double *bd, int np,
                           const double *xs, int precision, double *u);
 int dense_solve_host(Ctx *c, int n, const double *M, int mode, const double *r, double *x);
+void math_probe_host(Ctx *c, int fn, int n, const double *a, const double *b, double *out);
 
 }  // namespace vp
 

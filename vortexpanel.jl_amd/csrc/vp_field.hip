@@ -440,4 +440,26 @@ void blob_velocities_host(Ctx *c, int nb, const double *bx, const double *ba, co
     FCHECK(hipStreamSynchronize(st));
 }
 
+// ------------------------------------------------------------------------------------------ test hook -----------
+
+// The device's FP64 log (fn 0, of x) and atan2 (fn 1, of y = x[i], x = y[i]) as the kernels use them
+__global__ void math_probe(int fn, int n, const double *__restrict__ a, const double *__restrict__ b, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = fn == 0 ? dlog(a[i]) : datan2(a[i], b[i]);
+}
+
+void math_probe_host(Ctx *c, int fn, int n, const double *a, const double *b, double *out)
+{
+    hipStream_t st = c->stream;
+    Owned o;
+    o.st = st;
+    double *da = dput(a, (size_t)n, st, o.v), *db = dput(fn == 1 ? b : a, (size_t)n, st, o.v);
+    double *dout = dnew<double>((size_t)n, o.v);
+    math_probe<<<(n + 255) / 256, 256, 0, st>>>(fn, n, da, db, dout);
+    FCHECK(hipGetLastError());
+    FCHECK(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    FCHECK(hipStreamSynchronize(st));
+}
+
 }  // namespace vp
