@@ -454,8 +454,9 @@ template <int NL> void Plan::adjoint_pass(const AdjArgs &aa)
             P, rt, d_el, d_sigma, d_cs, d_cols, d_near, nnear, nullptr, nullptr, nullptr, nullptr, aa);
         HIPCHECK(hipGetLastError());
     }
-    // near_group takes one set per launch: built for two sets (ROCm 7.2, gfx950) its gradients differed from run to
-    // run by up to 4e-2 of their scale while every other kernel here matched its one-set build to 1e-15
+    // near_group takes one set per launch: with two it spills (560 bytes per lane under its two-wave cap) and runs
+    // slower than two launches; built for two sets without that cap (ROCm 7.2, gfx950) its gradients were not
+    // reproducible from run to run, while every other kernel here matched its one-set build to 1e-15
     for (int l = 0; nnearm && l < NL; l++) {
         AdjArgs a1 = aa;
         a1.lam += (size_t)l * aa.lstride;

@@ -902,7 +902,7 @@ __device__ __forceinline__ double wave_sum(double v)
 // One wavefront per pair group: the points of all the pair's rule pieces flattened over the 64 lanes (a lane walks
 // the pieces with a cursor), in FP64 (pair_block, assemble.jl:245-258, or segment_block, :312-321), reduced over the
 // wave and added to the pair's block of A and its rows of b.
-__global__ __launch_bounds__(256) void group_eval(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void group_eval(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                   const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                   const int *__restrict__ src_cols, const double *__restrict__ seg,
                                                   const int *__restrict__ segcol, const Leaf *__restrict__ leaves,
@@ -1295,9 +1295,10 @@ __device__ __forceinline__ void node_of(int nid, int &L, int &idx)  // level-ord
 // Plain near pairs whose root piece splits: one wavefront per pair evaluates the split test of every node of the
 // full quadtree at once across its lanes (each test depends on that node's own corners only, so this is the
 // recursive near_pieces! exactly), collects the leaves -- nodes whose ancestors all split and that do not split or
-// reach the depth -- by ballot, and evaluates their 7-point rules flattened over the lanes.
+// reach the depth -- by ballot, and evaluates their 7-point rules flattened over the lanes.  Capped at two waves per SIMD
+// (measured faster than the register allocation the compiler picks by itself, in both modes).
 template <int NADJ>  // 0: the forward assembly; 1..ADJ_MAX: backward with that many sets of adjoint weights
-__global__ __launch_bounds__(256) void near_group(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void near_group(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                   const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                   const int *__restrict__ src_cols, const int2 *__restrict__ list,
                                                   unsigned int npairs, double *__restrict__ A, double *__restrict__ bvec,
