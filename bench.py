@@ -181,6 +181,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the assembly metric (no wing, sweep, N-body)")
+    ap.add_argument("--extras", default="nbody,alpha_sweep,wing_adjoint,free_wake",
+                    help="which of the extra workloads to run (comma-separated)")
     ap.add_argument("--no-solve", action="store_true", help="no factorizations or solves (with --no-extras: the "
                     "assemblies alone, the program the far kernel's counter passes profile)")
     ap.add_argument("--copy-calibration", action="store_true", help="only copy 1 GiB device to device three times "
@@ -278,17 +280,18 @@ def main():
         F.close()
     dmu = float(np.max(np.abs(solves["mixed"]["mu"] - solves["fp64"]["mu"])) / np.max(np.abs(solves["fp64"]["mu"])))
     extras = {}
-    if not args.no_extras:
+    want = set() if args.no_extras else set(args.extras.split(","))
+    if "nbody" in want:
         extras["nbody"] = nbody_bench(g, dm, V, solves["fp64"]["mu"])
     for m in solves:
         solves[m].pop("mu")
         log(f"{m} solve {solves[m]}")
     for p_ in plans.values():
         p_.close()
-    if not args.no_extras:
-        extras["alpha_sweep"] = alpha_sweep_bench()
-        extras["wing_adjoint"] = wing_adjoint_bench()
-        extras["free_wake"] = free_wake_bench()
+    for name, fn in (("alpha_sweep", alpha_sweep_bench), ("wing_adjoint", wing_adjoint_bench),
+                     ("free_wake", free_wake_bench)):
+        if name in want:
+            extras[name] = fn()
     cpu = None if args.no_cpu else cpu_baseline(g, dm, V)
     if cpu:
         log(f"cpu baseline {cpu}")
