@@ -1,5 +1,6 @@
 """The device's own FP64 log and atan2 (the closed-form kernel's transcendentals) against the host C library's, through
-the C ABI's probe: within 2 ulp over the whole range, the special values and signed zeros as the library gives them."""
+the C ABI's probe: within 2 ulp over the whole range, the special values and signed zeros as the library gives them;
+and its 1/sqrt of the multipole's distances (normal positive arguments) within 2 ulp."""
 import ctypes as C
 
 import numpy as np
@@ -59,3 +60,11 @@ def test_atan2():
             assert o == r and np.signbit(o) == np.signbit(r), (yy, xx, o, r)
     o = _probe(1, np.array([1.0, -2.0]), np.array([0.0, -0.0]))
     assert np.allclose(o, [np.pi / 2, -np.pi / 2], rtol=0, atol=1e-16)
+
+
+def test_rsqrt():
+    print("device rsqrt", flush=True)
+    rng = np.random.default_rng(5)
+    x = np.concatenate([10.0 ** rng.uniform(-300, 300, 20000), rng.uniform(0.25, 4.0, 20000), [1.0, 4.0, 2.0, 0.5]])
+    out, ref = _probe(2, x), 1.0 / np.sqrt(x)
+    assert np.max(_ulps(out, ref)) <= 2.0

@@ -116,7 +116,7 @@ __device__ __forceinline__ void far_point_grad(const V3<double> *XS, const Elem 
     const V3<double> c = ld3(S.c), n = ld3(S.n);
     const double A = S.area, s = A * (1.0 / (4 * kPi));
     const V3<double> r = x - c;
-    const double R2 = dot(r, r), iR = 1.0 / dsqrt(R2);
+    const double R2 = dot(r, r), iR = vrsqrt(R2);
     const double iR2 = iR * iR, iR3 = iR2 * iR, iR5 = iR3 * iR2, iR7 = iR5 * iR2, iR9 = iR7 * iR2, iR11 = iR9 * iR2;
     const double qr = dot(Q, r);
     double cr = 0.0, rr = -3.0 * qr * iR5;

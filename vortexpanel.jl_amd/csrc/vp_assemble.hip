@@ -201,7 +201,7 @@ __device__ __forceinline__ void append_pair(bool want, int t, int s, int2 *list,
 template <typename T> struct FarSrc {  // per-lane source element data for the far tier
     V3<double> c;                      // centroid (FP64: the classification and the relative offsets)
     double diam, reach;
-    SrcT<T> s;                         // vertex offsets from the centroid, normal, edges ... in the compute type
+    FarMom<T> m;                       // the multipole's vertex moments about the centroid, in the compute type
     V3<T> g[3];
     T sigma;
 };
@@ -212,11 +212,12 @@ template <typename T> __device__ __forceinline__ FarSrc<T> load_far_src(const El
     f.c = ld3(E.c);
     f.diam = E.diam;
     f.reach = E.reach;
-    for (int k = 0; k < 3; k++) {  // the multipole needs the vertex offsets and the area only
-        f.s.v[k] = mk3(T(E.v[k][0] - E.c[0]), T(E.v[k][1] - E.c[1]), T(E.v[k][2] - E.c[2]));
+    V3<T> d[3];
+    for (int k = 0; k < 3; k++) {
+        d[k] = mk3(T(E.v[k][0] - E.c[0]), T(E.v[k][1] - E.c[1]), T(E.v[k][2] - E.c[2]));
         f.g[k] = mk3(T(E.g[k][0]), T(E.g[k][1]), T(E.g[k][2]));
     }
-    f.s.area = T(E.area);
+    f.m = far_moments(d, T(E.area));
     f.sigma = T(sigma);
     return f;
 }
@@ -230,7 +231,7 @@ __device__ __forceinline__ V3<double> far_eval(const FarSrc<T> &f, const Elem *_
 {
     V3<double> r = x - f.c;
     if (cls_dot(r, r) > kfar2) {
-        V3<T> w = sv_far_rel(f.s, mk3(T(r.x), T(r.y), T(r.z)));
+        V3<T> w = sv_far_mom(f.m, mk3(T(r.x), T(r.y), T(r.z)));
         return mk3((double)w.x, (double)w.y, (double)w.z);
     }
     SrcT<double> s;
@@ -261,7 +262,7 @@ __device__ __forceinline__ void far_block(const FarSrc<T> &f, const Elem *__rest
             w = far_eval<T>(f, E, ld3(F.x[q]), kfar2, nexact);
         } else {
             const V3<double> r = ld3(F.x[q]) - f.c;
-            const V3<T> wt = sv_far_rel(f.s, mk3(T(r.x), T(r.y), T(r.z)));
+            const V3<T> wt = sv_far_mom(f.m, mk3(T(r.x), T(r.y), T(r.z)));
             w = mk3((double)wt.x, (double)wt.y, (double)wt.z);
         }
         V3<double> vv = cross(w, n);  // n.(gamma x w) = gamma.(w x n), assemble.jl:251

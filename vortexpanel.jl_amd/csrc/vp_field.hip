@@ -442,11 +442,12 @@ void blob_velocities_host(Ctx *c, int nb, const double *bx, const double *ba, co
 
 // ------------------------------------------------------------------------------------------ test hook -----------
 
-// The device's FP64 log (fn 0, of x) and atan2 (fn 1, of y = x[i], x = y[i]) as the kernels use them
+// The device's FP64 log (fn 0, of a), atan2 (fn 1, of y = a[i], x = b[i]) and 1/sqrt (fn 2, of a) as the kernels
+// use them
 __global__ void math_probe(int fn, int n, const double *__restrict__ a, const double *__restrict__ b, double *out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = fn == 0 ? dlog(a[i]) : datan2(a[i], b[i]);
+    if (i < n) out[i] = fn == 0 ? dlog(a[i]) : fn == 1 ? datan2(a[i], b[i]) : vrsqrt(a[i]);
 }
 
 void math_probe_host(Ctx *c, int fn, int n, const double *a, const double *b, double *out)
