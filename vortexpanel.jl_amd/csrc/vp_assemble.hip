@@ -902,8 +902,9 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // One wavefront per pair group: the points of all the pair's rule pieces flattened over the 64 lanes (a lane walks
 // the pieces with a cursor), in FP64 (pair_block, assemble.jl:245-258, or segment_block, :312-321), reduced over the
-// wave and added to the pair's block of A and its rows of b.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void group_eval(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+// wave and added to the pair's block of A and its rows of b.  Capped at three waves per SIMD (C2: 2.12 -> 1.73 ms
+// against two; a few registers spill).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void group_eval(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                   const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                   const int *__restrict__ src_cols, const double *__restrict__ seg,
                                                   const int *__restrict__ segcol, const Leaf *__restrict__ leaves,
@@ -1003,6 +1004,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // The self pair and shared-edge / shared-corner pairs (no contact): one wavefront per pair.  Its pieces are the
 // stored rule on the whole triangle (self, edge k, vertex k when the corner is at most 90 degrees) or vertex_pieces!'
 // equal-angle wedges of a wide corner (assemble.jl:102-125), kept in LDS; their points are flattened over the lanes.
+// The forward kernel is capped at three waves per SIMD (C2: 10.2 -> 7.9 ms against the two its 249 registers give).
 constexpr int MAXWEDGE = 16;
 
 __device__ __forceinline__ double wave_sum2(double v)
@@ -1012,7 +1014,7 @@ __device__ __forceinline__ double wave_sum2(double v)
 }
 
 template <int NADJ>  // 0: the forward assembly; 1..ADJ_MAX: backward with that many sets of adjoint weights
-__global__ __launch_bounds__(256) void touch_simple(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ? 3 : 1, NADJ == 0 ? 3 : 1))) void touch_simple(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                     const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                     const int *__restrict__ src_cols, const int4 *__restrict__ list,
                                                     unsigned int npairs, double *__restrict__ A, double *__restrict__ bvec,
@@ -1296,10 +1298,11 @@ __device__ __forceinline__ void node_of(int nid, int &L, int &idx)  // level-ord
 // Plain near pairs whose root piece splits: one wavefront per pair evaluates the split test of every node of the
 // full quadtree at once across its lanes (each test depends on that node's own corners only, so this is the
 // recursive near_pieces! exactly), collects the leaves -- nodes whose ancestors all split and that do not split or
-// reach the depth -- by ballot, and evaluates their 7-point rules flattened over the lanes.  Capped at two waves per SIMD
-// (measured faster than the register allocation the compiler picks by itself, in both modes).
+// reach the depth -- by ballot, and evaluates their 7-point rules flattened over the lanes.  Capped at three waves per
+// SIMD forward (C2: 7.77 -> 6.50 ms against two) and two backward (measured faster than the register allocation the
+// compiler picks by itself).
 template <int NADJ>  // 0: the forward assembly; 1..ADJ_MAX: backward with that many sets of adjoint weights
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void near_group(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ? 3 : 2, NADJ == 0 ? 3 : 2))) void near_group(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                   const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                   const int *__restrict__ src_cols, const int2 *__restrict__ list,
                                                   unsigned int npairs, double *__restrict__ A, double *__restrict__ bvec,
