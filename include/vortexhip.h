@@ -188,6 +188,20 @@ int vp_body_velocity(vp_ctx *ctx, const vp_surface *g, const double *sigma, cons
 int vp_blob_velocities(vp_ctx *ctx, int nb, const double *bx, const double *alpha, const double *delta, int np,
                        const double *xs, int precision, double *u);
 
+/* The free-wake step's velocity sums, `wake_velocities` (src/freewake/step.jl:166-173): base (V∞, or null for 0)
+ * + the body's sources and sheets (as vp_body_velocity) + nb blobs (as vp_blob_velocities) at np points, as one HIP
+ * graph: the body and blob fields are independent nodes joined by one reduction.  The body's elements are set up at
+ * create; the graph is instantiated once (again only if blobs appear or vanish) and each eval rewrites its nodes' grids
+ * and arguments, so the particle count may change on every call.  sigma (nt) and gamma (nt x 3) are passed per call.
+ * One call at a time per handle (it shares the context's stream).  info: counts[0] graphs built, [1] node updates,
+ * [2] graph launches. */
+typedef struct vp_wake_field vp_wake_field;
+int vp_wake_field_create(vp_ctx *ctx, const vp_surface *g, double kernel_far, int precision, vp_wake_field **out);
+int vp_wake_field_eval(vp_wake_field *h, const double *sigma, const double *gamma, int np, const double *xs, int nb,
+                       const double *bx, const double *alpha, const double *delta, const double *base, double *u);
+int vp_wake_field_info(const vp_wake_field *h, long long *counts);
+void vp_wake_field_destroy(vp_wake_field *h);
+
 /* Test entry points (host arrays, row-major): C = op(A) B or C -= op(A) B with the library's MFMA GEMM (prec 0 FP64,
  * 1 FP32; B K x N, C M x N, all doubles on the host).  flags: bit 0 subtract; bit 1 transposed A (A stored K x M,
  * op(A) = Aᵀ), else A is M x K; bits 4-7 the tile shape's index among the autotuner's five candidates (0 is the

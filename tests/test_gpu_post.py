@@ -1,4 +1,4 @@
-"""GPU N-body sums (vp_surface_velocity, vp_body_velocity, vp_blob_velocities) against the FP64 oracle
+"""GPU N-body sums (vp_surface_velocity, vp_body_velocity, vp_blob_velocities, vp_wake_field_*) against the FP64 oracle
 (src/post/Post.jl:32-112, src/freewake/step.jl:176-187).  FP64: within 1e-12 of the largest velocity; FP32 kernels
 with FP64 sums: within 2e-5 (source triangles) and 5e-5 (blobs) of it."""
 import numpy as np
@@ -88,3 +88,42 @@ def test_empty_and_bad_inputs():
     dm = dofmap.build_dofmap(g)
     with pytest.raises(ValueError):
         post.surface_velocity(g, dm, np.zeros(g.nt), np.zeros(dm.nslot + 1), [1.0, 0, 0])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_wake_field_graph(precision):
+    """The free-wake step's velocity sums as one HIP graph (vp_wake_field_*): every replay, with the target and blob
+    counts changing from call to call (and the blobs vanishing and coming back), against the eager body_velocity +
+    blob_velocities of the same inputs run between the replays, and the first ones against the FP64 oracle."""
+    print(f"wake field graph {precision}", flush=True)
+    g = mesh.uv_ellipsoid(12, 24, (2.0, 1.0, 0.5))
+    rng = np.random.default_rng(7)
+    nt = len(g.tris)
+    sig = rng.standard_normal(nt)
+    gam = rng.standard_normal((nt, 3))
+    V = np.array([1.0, 0.0, 0.1])
+    field = post.WakeField(g, precision=precision)
+    tol = 1e-13 if precision == "fp64" else 1e-12   # the same kernels and chunks: only the final sums' order differs
+    sizes = [(300, 0), (300, 257), (2049, 1000), (500, 40), (1, 1), (7000, 3000), (64, 0), (1500, 800)]
+    for k, (npts, nb) in enumerate(sizes):
+        xs = 3.0 * rng.standard_normal((npts, 3))
+        bx = 3.0 * rng.standard_normal((nb, 3))
+        ba = rng.standard_normal((nb, 3))
+        bd = rng.uniform(0.05, 0.3, nb)
+        u = field(sig, gam, xs, bx, ba, bd, base=V) if nb else field(sig, gam, xs, base=V)
+        ref = V + post.body_velocity(g, sig, gam, xs, precision=precision)
+        if nb:
+            ref = ref + post.blob_velocities(bx, ba, bd, xs, precision=precision)
+        assert _err(u, ref) < tol, (k, npts, nb, _err(u, ref))
+        if k < 3:
+            o = V + O.body_velocity(g, sig, gam, xs) + (O.blob_velocities(bx, ba, bd, xs) if nb else 0.0)
+            assert _err(u, o) < TOL[precision] * (2.5 if nb else 1.0), (k, _err(u, o))
+    builds, updates, launches = field.info()
+    assert launches == len(sizes)
+    assert builds == 4 and updates == len(sizes) - builds   # rebuilt only when the blobs appear or vanish
+    assert field(sig, gam, np.zeros((0, 3))).shape == (0, 3)
+    with pytest.raises(ValueError):
+        field(sig[:-1], gam, np.ones((2, 3)))
+    with pytest.raises(ValueError):
+        field(sig, gam, np.ones((2, 3)), np.ones((2, 3)), np.ones((3, 3)), np.ones(2))
+    field.close()

@@ -364,6 +364,57 @@ int vp_blob_velocities(vp_ctx *ctx, int nb, const double *bx, const double *alph
     });
 }
 
+int vp_wake_field_create(vp_ctx *ctx, const vp_surface *g, double kernel_far, int precision, vp_wake_field **out)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if (!out || !g || !g->x || !g->tris) return fail(&ctx->c, VP_E_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (precision != 0 && precision != 1) return fail(&ctx->c, VP_E_ARGUMENT, "precision must be 0 (fp64) or 1 (mixed)");
+    if (g->nt <= 0 || g->nn <= 0) return fail(&ctx->c, VP_E_DIMENSION, "empty surface");
+    auto *h = new vp_wake_field();
+    int rc = guard(&ctx->c, [&] {
+        (void)hipSetDevice(ctx->c.device);
+        h->w.init(&ctx->c, *g, kernel_far, precision);
+    });
+    if (rc) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return VP_OK;
+}
+
+int vp_wake_field_eval(vp_wake_field *h, const double *sigma, const double *gamma, int np, const double *xs, int nb,
+                       const double *bx, const double *alpha, const double *delta, const double *base, double *u)
+{
+    if (!h || !h->w.c) return VP_E_ARGUMENT;
+    vp::Ctx *c = h->w.c;
+    if (!sigma || !gamma || (np > 0 && (!xs || !u)) || (nb > 0 && (!bx || !alpha || !delta)))
+        return fail(c, VP_E_ARGUMENT, "null argument");
+    if (np < 0 || nb < 0) return fail(c, VP_E_DIMENSION, "negative size");
+    if (np == 0) return VP_OK;
+    return guard(c, [&] {
+        (void)hipSetDevice(c->device);
+        h->w.eval(sigma, gamma, np, xs, nb, bx, alpha, delta, base, u);
+    });
+}
+
+int vp_wake_field_info(const vp_wake_field *h, long long *counts)
+{
+    if (!h || !counts) return VP_E_ARGUMENT;
+    counts[0] = h->w.builds;
+    counts[1] = h->w.updates;
+    counts[2] = h->w.launches;
+    return VP_OK;
+}
+
+void vp_wake_field_destroy(vp_wake_field *h)
+{
+    if (!h) return;
+    if (h->w.c) (void)hipSetDevice(h->w.device);
+    delete h;
+}
+
 int vp_assemble(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const double *sigma, const double *vinf,
                 const vp_assembly_options *opts, double *A, double *b, long long *pairs, long long *points)
 {

@@ -118,6 +118,29 @@ void body_velocity_host(Ctx *c, const vp_surface &g, const double *sigma, const 
                         double kernel_far, int precision, double *u);
 void blob_velocities_host(Ctx *c, int nb, const double *bx, const double *ba, const double *bd, int np,
                           const double *xs, int precision, double *u);
+// The free-wake step's velocity sums (the body's kernels and the wake's blobs at np targets, plus a base velocity)
+// replayed as one HIP graph per node structure, its nodes' grids and arguments rewritten on every call.
+struct WakeField {
+    Ctx *c = nullptr;
+    int device = 0;                 // the context's device and stream, kept for the destructor
+    hipStream_t stream = nullptr;
+    int nt = 0, precision = 0;
+    double kernel_far = 10.0;
+    Elem *el = nullptr;
+    double *sig = nullptr, *gam = nullptr, *X = nullptr, *u = nullptr, *part = nullptr;
+    double *bx = nullptr, *ba = nullptr, *bd = nullptr;
+    size_t capX = 0, capU = 0, capP = 0, capB3 = 0, capA3 = 0, capB1 = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t node[3] = {nullptr, nullptr, nullptr};
+    int shape = -1;  // the instantiated graph's nodes: 0 body field and reduction, 1 with the blob field too
+    long long builds = 0, updates = 0, launches = 0;
+
+    ~WakeField();
+    void init(Ctx *ctx, const vp_surface &g, double kernel_far, int precision);
+    void eval(const double *sigma, const double *gamma, int np, const double *xs, int nb, const double *bx,
+              const double *alpha, const double *delta, const double *base, double *u);
+};
 int dense_solve_host(Ctx *c, int n, const double *M, int mode, const double *r, double *x);
 void math_probe_host(Ctx *c, int fn, int n, const double *a, const double *b, double *out);
 
@@ -132,4 +155,7 @@ struct vp_ctx {
 };
 struct vp_plan {
     vp::Plan p;
+};
+struct vp_wake_field {
+    vp::WakeField w;
 };

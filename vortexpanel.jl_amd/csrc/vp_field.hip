@@ -440,6 +440,143 @@ void blob_velocities_host(Ctx *c, int nb, const double *bx, const double *ba, co
     FCHECK(hipStreamSynchronize(st));
 }
 
+// ------------------------------------------------------------------------------------------ wake field graph ------
+
+// The free-wake step's velocity sums (wake_velocities, src/freewake/step.jl:166-173: V∞ + the body's sources and
+// sheets + every wake blob) at np targets as one HIP graph of three kernel nodes: the body field and the blob field
+// are independent nodes (they run side by side on the device) writing disjoint chunks of one partial buffer, and the
+// reduction node adds V∞ and the chunks in chunk order.  The graph is built and instantiated once per node structure
+// (with or without blobs); every later call rewrites the three nodes' grids and arguments in the executable graph, so
+// a particle count that changes every step needs no new graph.  The body's elements are set up once; the device
+// buffers grow by half again when a call needs more and are kept between calls.
+WakeField::~WakeField()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (void *p : {(void *)el, (void *)sig, (void *)gam, (void *)X, (void *)bx, (void *)ba, (void *)bd, (void *)part,
+                    (void *)u})
+        if (p) (void)hipFree(p);
+}
+
+namespace {
+
+// (re)allocate *p for at least need elements, keeping none of its contents
+template <typename T> void grow(Ctx *c, T *&p, size_t &cap, size_t need)
+{
+    if (need <= cap && p) return;
+    const size_t n = std::max<size_t>(need + need / 2, 1024);
+    if (p) {
+        FCHECK(hipStreamSynchronize(c->stream));
+        FCHECK(hipFree(p));
+        p = nullptr;
+    }
+    FCHECK(hipMalloc((void **)&p, n * sizeof(T)));
+    cap = n;
+}
+
+}  // namespace
+
+void WakeField::init(Ctx *ctx, const vp_surface &g, double kfar, int prec)
+{
+    c = ctx;
+    device = ctx->device;
+    stream = ctx->stream;
+    nt = g.nt;
+    kernel_far = kfar;
+    precision = prec;
+    hipStream_t st = c->stream;
+    std::vector<void *> tmp;
+    double *d_x = dput(g.x, (size_t)g.nn * 3, st, tmp);
+    int *d_tris = dput(g.tris, (size_t)nt * 3, st, tmp);
+    FCHECK(hipMalloc((void **)&el, sizeof(Elem) * nt));
+    FCHECK(hipMalloc((void **)&sig, sizeof(double) * nt));
+    FCHECK(hipMalloc((void **)&gam, sizeof(double) * 3 * nt));
+    elem_setup<<<(nt + 255) / 256, 256, 0, st>>>(d_x, d_tris, nt, el);
+    FCHECK(hipGetLastError());
+    FCHECK(hipStreamSynchronize(st));
+    for (void *p : tmp) FCHECK(hipFree(p));
+}
+
+void WakeField::eval(const double *sigma, const double *gamma, int np, const double *xs, int nb, const double *hbx,
+                     const double *hba, const double *hbd, const double *base, double *hu)
+{
+    hipStream_t st = c->stream;
+    const int chunk_s = chunk_of(nt, np, FS), nchunk_s = (nt + chunk_s - 1) / chunk_s;
+    const int chunk_b = nb > 0 ? chunk_of(nb, np, BT) : 0, nchunk_b = nb > 0 ? (nb + chunk_b - 1) / chunk_b : 0;
+    grow(c, X, capX, (size_t)np * 3);
+    grow(c, u, capU, (size_t)np * 3);
+    grow(c, part, capP, (size_t)(nchunk_s + nchunk_b) * np * 3);
+    if (nb > 0) {
+        grow(c, bx, capB3, (size_t)nb * 3);
+        grow(c, ba, capA3, (size_t)nb * 3);
+        grow(c, bd, capB1, (size_t)nb);
+        FCHECK(hipMemcpyAsync(bx, hbx, sizeof(double) * 3 * nb, hipMemcpyHostToDevice, st));
+        FCHECK(hipMemcpyAsync(ba, hba, sizeof(double) * 3 * nb, hipMemcpyHostToDevice, st));
+        FCHECK(hipMemcpyAsync(bd, hbd, sizeof(double) * nb, hipMemcpyHostToDevice, st));
+    }
+    FCHECK(hipMemcpyAsync(sig, sigma, sizeof(double) * nt, hipMemcpyHostToDevice, st));
+    FCHECK(hipMemcpyAsync(gam, gamma, sizeof(double) * 3 * nt, hipMemcpyHostToDevice, st));
+    FCHECK(hipMemcpyAsync(X, xs, sizeof(double) * 3 * np, hipMemcpyHostToDevice, st));
+
+    // the nodes' arguments, in each kernel's parameter order
+    const Elem *a_el = el;
+    const double *a_sig = sig, *a_gam = gam, *a_X = X, *a_bx = bx, *a_ba = ba, *a_bd = bd, *a_part = part;
+    double *a_pb = part + (size_t)nchunk_s * np * 3, *a_u = u, *a_ps = part;
+    int a_nt = nt, a_cs = chunk_s, a_np = np, a_self = 0, a_nb = nb, a_cb = chunk_b, a_nch = nchunk_s + nchunk_b;
+    int a_nfil = 0, a_acc = 0;
+    double a_kf = kernel_far, b0 = base ? base[0] : 0.0, b1 = base ? base[1] : 0.0, b2 = base ? base[2] : 0.0;
+    const double *a_null = nullptr;
+    const int *a_inull = nullptr;
+    void *src_args[] = {&a_el, &a_sig, &a_gam, &a_nt, &a_cs, &a_X, &a_np, &a_self, &a_kf, &a_ps};
+    void *blob_args[] = {&a_bx, &a_ba, &a_bd, &a_nb, &a_cb, &a_X, &a_np, &a_pb};
+    void *red_args[] = {&a_part, &a_nch, &a_np, &b0, &b1, &b2, &a_null, &a_inull, &a_null, &a_nfil, &a_X, &a_acc, &a_u};
+    hipKernelNodeParams kp[3] = {};
+    kp[0].func = precision == 1 ? reinterpret_cast<void *>(&source_field<float>)
+                                : reinterpret_cast<void *>(&source_field<double>);
+    kp[0].gridDim = dim3((np + FT - 1) / FT, nchunk_s);
+    kp[0].blockDim = dim3(FT);
+    kp[0].kernelParams = src_args;
+    kp[1].func = precision == 1 ? reinterpret_cast<void *>(&blob_field<float>)
+                                : reinterpret_cast<void *>(&blob_field<double>);
+    kp[1].gridDim = dim3((np + FT - 1) / FT, std::max(nchunk_b, 1));
+    kp[1].blockDim = dim3(FT);
+    kp[1].kernelParams = blob_args;
+    kp[2].func = reinterpret_cast<void *>(&field_reduce);
+    kp[2].gridDim = dim3((np + 255) / 256);
+    kp[2].blockDim = dim3(256);
+    kp[2].kernelParams = red_args;
+
+    const int want = nb > 0 ? 1 : 0;
+    if (!exec || shape != want) {
+        if (exec) FCHECK(hipGraphExecDestroy(exec));
+        if (graph) FCHECK(hipGraphDestroy(graph));
+        exec = nullptr;
+        graph = nullptr;
+        FCHECK(hipGraphCreate(&graph, 0));
+        FCHECK(hipGraphAddKernelNode(&node[0], graph, nullptr, 0, &kp[0]));
+        hipGraphNode_t deps[2] = {node[0], nullptr};
+        int ndep = 1;
+        if (want) {
+            FCHECK(hipGraphAddKernelNode(&node[1], graph, nullptr, 0, &kp[1]));
+            deps[ndep++] = node[1];
+        }
+        FCHECK(hipGraphAddKernelNode(&node[2], graph, deps, ndep, &kp[2]));
+        FCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        shape = want;
+        builds++;
+    } else {
+        FCHECK(hipGraphExecKernelNodeSetParams(exec, node[0], &kp[0]));
+        if (want) FCHECK(hipGraphExecKernelNodeSetParams(exec, node[1], &kp[1]));
+        FCHECK(hipGraphExecKernelNodeSetParams(exec, node[2], &kp[2]));
+        updates++;
+    }
+    FCHECK(hipGraphLaunch(exec, st));
+    launches++;
+    FCHECK(hipMemcpyAsync(hu, u, sizeof(double) * 3 * np, hipMemcpyDeviceToHost, st));
+    FCHECK(hipStreamSynchronize(st));
+}
+
 // ------------------------------------------------------------------------------------------ test hook -----------
 
 // The device's FP64 log (fn 0, of a), atan2 (fn 1, of y = a[i], x = b[i]) and 1/sqrt (fn 2, of a) as the kernels

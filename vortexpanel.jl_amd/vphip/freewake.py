@@ -430,7 +430,7 @@ class FreeWakeSim:
     with the body and the bordered matrix factored once; `step()` changes only the right hand side.
 
     precision: 'fp64' or 'mixed' (the factorization in FP32 with FP64 refinement, FP32 velocity kernels with FP64 sums).
-    The device work is in `_explicit_rhs`, `_solve` and `_velocities`."""
+    The device work is in `_explicit_rhs`, `_solve` and `_velocities` (the velocity sums replay one HIP graph)."""
 
     def __init__(self, g, dm, vinf, opts: FreeWakeOptions, assembly: AssemblyOptions | None = None,
                  precision: str = "fp64", ctx: Context | None = None):
@@ -483,6 +483,7 @@ class FreeWakeSim:
         plan.close()
         self.body_plan = AssemblyPlan(self.g, self.dm, None, self.assembly, ctx=ctx, wake_only=True)
         self.body_plan.assemble(np.zeros(len(self.g.tris)), np.zeros(3))
+        self.field = post.WakeField(self.g, kernel_far=self.assembly.kernel_far, precision=self.precision, ctx=ctx)
 
     def _explicit_rhs(self, we, particles):
         return explicit_rhs(self.g, self.dm, we, particles, self.assembly, ctx=self.ctx, body_plan=self.body_plan)
@@ -492,10 +493,10 @@ class FreeWakeSim:
         return out["q"], out["mu"], out["lam"]
 
     def _velocities(self, gam, blobs, X):
-        ub = post.body_velocity(self.g, self.sigma, gam, X, kernel_far=self.assembly.kernel_far,
-                                precision=self.precision, ctx=self.ctx)
-        uw = post.blob_velocities(blobs.x, blobs.alpha, blobs.delta, X, precision=self.precision, ctx=self.ctx)
-        return ub + uw
+        """the body's kernels and every wake blob at X: one replay of the wake field's HIP graph"""
+        if len(blobs):
+            return self.field(self.sigma, gam, X, blobs.x, blobs.alpha, blobs.delta)
+        return self.field(self.sigma, gam, X)
 
     # --- the step ---
     def step(self):
@@ -580,3 +581,4 @@ class FreeWakeSim:
     def close(self):
         self.F.close()
         self.body_plan.close()
+        self.field.close()

@@ -86,6 +86,58 @@ def blob_velocities(bx, alpha, delta, xs, precision="fp64", ctx: Context | None 
     return u
 
 
+class WakeField:
+    """The free-wake step's velocity sums (wake_velocities, src/freewake/step.jl:166-173) on the device as one HIP graph
+    (vp_wake_field_*): the body's sources and sheets (as body_velocity) plus the wake's blobs (as blob_velocities)
+    at the targets, plus an optional base velocity.  The body's elements are set up once; each call rewrites the
+    graph's node arguments, so the target and blob counts may change from call to call."""
+
+    def __init__(self, g, kernel_far=10.0, precision="fp64", ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        x, tris = L.f64(g.x), L.i32(g.tris)
+        s = L.SurfaceC(len(x), L.ptr(x), len(tris), L.ptr(tris, C.c_int), None, 0, 0, None)
+        h = C.c_void_p()
+        _check(self.ctx.h, self.ctx.lib.vp_wake_field_create(self.ctx.h, C.byref(s), C.c_double(kernel_far),
+                                                              _prec(precision), C.byref(h)))
+        self.h = h
+        self.nt = len(tris)
+
+    def __call__(self, sigma, gamma, xs, bx=None, alpha=None, delta=None, base=None):
+        sig, gam = L.f64(sigma), L.f64(gamma).reshape(-1, 3)
+        if sig.shape != (self.nt,) or gam.shape != (self.nt, 3):
+            raise ValueError("sigma and gamma must have one entry per triangle")
+        X = L.f64(xs).reshape(-1, 3)
+        if bx is None:
+            bx, ba, bd = np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0)
+        else:
+            bx, ba, bd = L.f64(bx).reshape(-1, 3), L.f64(alpha).reshape(-1, 3), L.f64(delta).reshape(-1)
+        if not (len(bx) == len(ba) == len(bd)):
+            raise ValueError("blob arrays disagree in length")
+        b = L.f64(base) if base is not None else None
+        u = np.zeros_like(X)
+        rc = self.ctx.lib.vp_wake_field_eval(self.h, L.ptr(sig), L.ptr(gam), len(X), L.ptr(X), len(bd), L.ptr(bx),
+                                             L.ptr(ba), L.ptr(bd), L.ptr(b), L.ptr(u))
+        _check(self.ctx.h, rc)
+        return u
+
+    def info(self):
+        """(graphs built, node updates, graph launches)"""
+        c = np.zeros(3, dtype=np.int64)
+        _check(self.ctx.h, self.ctx.lib.vp_wake_field_info(self.h, L.ptr(c, C.c_longlong)))
+        return tuple(int(v) for v in c)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.vp_wake_field_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def gradient_velocity(els, dm, mu, vinf):
     """V∞ - (V∞.n)n + grad(mu) per element, Post.jl:72-73"""
     V = np.asarray(vinf, dtype=np.float64)
