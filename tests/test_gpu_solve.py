@@ -141,3 +141,29 @@ def test_wing_transposed_solve(mode):
     ref_cond = np.linalg.cond(M, 1)
     assert 0.3 < F.cond / ref_cond <= 1.0 + 1e-6, (F.cond, ref_cond)
     F.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_repeated_solves_replay(mode):
+    """The factors' inverse is applied eagerly on its first call for a direction and number of right hand sides,
+    captured as a HIP graph on the second and replayed after that: every solve of a sequence that changes the
+    direction and the batch width (forcing new captures), with eager work between them, against numpy's solve of
+    the oracle's bordered matrix; the residual GEMV (1-4 right hand sides) and the MFMA GEMM (more) both take part."""
+    g = mesh.naca_wing(mesh.WingSpec(code="2412", n_around=20, n_span=6, aspect=5.0))
+    dm = dofmap.build_dofmap(g)
+    V = np.array([np.cos(0.07), 0.0, np.sin(0.07)])
+    w = wk.build_wake(g, dm, V, wk.WakeOptions(length=2.0))
+    A, b, W = _case(g, dm, V, w)
+    print(f"repeated solves mode={mode}", flush=True)
+    M, _, _ = O.bordered_system(dm, A, b, W)
+    F = solve.factorize_galerkin(dm, A, W=W, mode=mode)
+    rng = np.random.default_rng(11)
+    for k, (nrhs, trans) in enumerate([(1, False)] * 4 + [(1, True)] * 3 + [(3, False)] * 3 + [(1, False)] * 3
+                                      + [(6, True)] * 3 + [(2, True)] * 3):
+        R = rng.standard_normal((F.n, nrhs))
+        X = F.solve_bordered_t(R) if trans else F.solve_bordered(R)
+        ref = np.linalg.solve(M.T if trans else M, R)
+        assert np.abs(X - ref).max() / np.abs(ref).max() < 1e-11, (k, nrhs, trans)
+        assert F.last_resid < 1e-14, (k, F.last_resid)
+        np.linalg.norm(M @ ref - R)   # host work between the device calls
+    F.close()

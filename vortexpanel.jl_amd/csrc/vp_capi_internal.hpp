@@ -94,9 +94,16 @@ struct Factor {
     double *M = nullptr, *d_ru = nullptr, *d_rv = nullptr;
     double *G64 = nullptr, *linv64 = nullptr, *uinv64 = nullptr;
     float *G32 = nullptr, *linv32 = nullptr, *uinv32 = nullptr;
-    double *X = nullptr, *R = nullptr, *Y = nullptr, *B = nullptr, *nrm = nullptr;
+    double *X = nullptr, *R = nullptr, *Y = nullptr, *B = nullptr, *nrm = nullptr, *part = nullptr;
     float *Xf = nullptr;
-    int ws_nrhs = 0, last_iters = 0;
+    int ws_nrhs = 0, ws_epoch = 0, last_iters = 0;
+    // apply_inverse replayed as a HIP graph per direction (captured on its second call with the same nrhs and
+    // workspace): the triangular solves are a chain of two kernels per 64-row panel, bound by launch cost
+    struct InvGraph {
+        hipGraphExec_t exec = nullptr;
+        int nrhs = -1, epoch = -1, calls = 0;
+    } inv_graph[2];
+    long long graph_launches = 0;
     double anorm1 = 0.0, anorm_inf = -1.0, cond = -1.0;
     std::vector<double> se;
     std::vector<void *> owned;
@@ -106,6 +113,7 @@ struct Factor {
     void init_from_dense(const double *Md, long long ldd, int n, int npad, int mode, double anorm);
     void ensure_ws(int nrhs);
     void apply_inverse(int nrhs, bool trans);
+    void apply_inverse_replay(int nrhs, bool trans);
     int solve(const double *B, int nrhs, double *X, bool device_io, int max_iter, double *resid, bool trans = false);
     double cond_estimate();
 };

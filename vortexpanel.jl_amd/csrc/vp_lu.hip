@@ -518,6 +518,81 @@ __global__ void axpy_inplace(double *x, const double *dx, long long n)
     if (i < n) x[i] += dx[i];
 }
 
+// The refinement residual for a few right hand sides (nrhs <= GEMV_MAX), HBM-bound (M is read once, 8 bytes per
+// entry): Y -= M X with one wavefront per row of M (row-major, ld), its lanes striding over the columns, the sums
+// reduced over the wave; and Y -= Mᵀ X in two passes, a thread per column of M summing a slice of KSPLIT rows into
+// part[slice][column] (coalesced along the rows), then the slices added in slice order (deterministic).
+constexpr int GEMV_MAX = 4, KSPLIT = 32;
+
+template <int NR>
+__global__ __launch_bounds__(256) void gemv_sub(const double *__restrict__ M, long long ld, int nrow, int ncol,
+                                                const double *__restrict__ X, double *__restrict__ Y)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= nrow) return;
+    const double *m = M + (long long)row * ld;
+    double acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; c++) acc[c] = 0.0;
+#pragma unroll 4
+    for (int k = lane; k < ncol; k += 64) {
+        const double a = m[k];
+#pragma unroll
+        for (int c = 0; c < NR; c++) acc[c] = fma(a, X[(long long)k * NR + c], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NR; c++)
+        for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o);
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < NR; c++) Y[(long long)row * NR + c] -= acc[c];
+}
+
+template <int NR>
+__global__ __launch_bounds__(256) void gemv_t_part(const double *__restrict__ M, long long ld, int nrow, int ncol,
+                                                   const double *__restrict__ X, int kchunk, double *__restrict__ part)
+{
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    if (col >= ncol) return;
+    const int k0 = blockIdx.y * kchunk, k1 = min(nrow, k0 + kchunk);
+    double acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; c++) acc[c] = 0.0;
+#pragma unroll 4
+    for (int k = k0; k < k1; k++) {
+        const double a = M[(long long)k * ld + col];
+#pragma unroll
+        for (int c = 0; c < NR; c++) acc[c] = fma(a, X[(long long)k * NR + c], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NR; c++) part[((long long)blockIdx.y * ncol + col) * NR + c] = acc[c];
+}
+
+__global__ void gemv_t_reduce(const double *__restrict__ part, int nsplit, long long len, double *__restrict__ Y)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    double s = 0.0;
+    for (int k = 0; k < nsplit; k++) s += part[(long long)k * len + i];
+    Y[i] -= s;
+}
+
+template <int NR>
+void launch_residual(hipStream_t s, bool trans, const double *M, long long ld, int npad, const double *X, double *Y,
+                     double *part)
+{
+    if (!trans) {
+        gemv_sub<NR><<<(npad + 3) / 4, 256, 0, s>>>(M, ld, npad, npad, X, Y);
+    } else {
+        const int kchunk = (npad + KSPLIT - 1) / KSPLIT;
+        gemv_t_part<NR><<<dim3((npad + 255) / 256, KSPLIT), 256, 0, s>>>(M, ld, npad, npad, X, kchunk, part);
+        HIPCHECK2(hipGetLastError());
+        const long long len = (long long)npad * NR;
+        gemv_t_reduce<<<(unsigned)((len + 255) / 256), 256, 0, s>>>(part, KSPLIT, len, Y);
+    }
+    HIPCHECK2(hipGetLastError());
+}
+
 // per right hand side: max |r|, max |x| (columns of n x nrhs row-major), into out[2*c], out[2*c+1]
 __global__ void colmax2(const double *r, const double *x, int n, int nrhs, double *out)
 {
@@ -585,6 +660,8 @@ template <typename T> T *dmalloc(size_t n)
 
 Factor::~Factor()
 {
+    for (auto &g : inv_graph)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     for (void *p : owned) (void)hipFree(p);
 }
 
@@ -645,6 +722,165 @@ template <typename T> void lu_solve(hipStream_t s, const T *G, long long ld, int
         T *xk = X + (long long)k0 * nrhs;
         launch_gemm<T, false>(s, nb, nrhs, nb, uinv + (size_t)(k0 / NB) * NB * NB, NB, xk, nrhs, xk, nrhs);
         if (k0 > 0) launch_gemm<T, true>(s, k0, nrhs, nb, G + k0, ld, xk, nrhs, X, nrhs);
+    }
+}
+
+// ------------------------------------------------------------------------------------------ narrow solves ---------
+
+// Triangular solves for a few right hand sides (nrhs <= GEMV_MAX) in panels of NS = 256 rows: per panel one workgroup
+// solves the panel's diagonal block (four 64-row steps through the stored 64 x 64 inverses, with the updates inside
+// the panel between them), then one kernel subtracts the panel's contribution from every row still to be solved.
+// That is two launches per 256 rows instead of two per 64: a narrow solve is bound by the latency of each launch.
+constexpr int NS = 256;
+
+// T_kk x_k = x_k for the panel of nb rows (a multiple of NB) at k0.  LOWER: forward over the 64-row blocks, else
+// backward.  TRANS: the triangle is Gᵀ's (its entry (i, j) is G[j][i]) and the block inverses are applied transposed.
+template <typename T, int NR, bool LOWER, bool TRANS>
+__global__ __launch_bounds__(256) void trsv_panel(const T *__restrict__ G, long long ld, const T *__restrict__ dinv,
+                                                  int k0, int nb, T *__restrict__ X)
+{
+    __shared__ T xs[NS][NR];
+    __shared__ T ys[NB][NR];
+    const int tid = threadIdx.x;
+    if (tid < nb)
+        for (int c = 0; c < NR; c++) xs[tid][c] = X[(long long)(k0 + tid) * NR + c];
+    __syncthreads();
+    const int nsub = nb / NB;
+    for (int t = 0; t < nsub; t++) {
+        const int sb = LOWER ? t : nsub - 1 - t;
+        const T *D = dinv + (size_t)(k0 / NB + sb) * NB * NB;
+        if (tid < NB) {
+            T acc[NR];
+            for (int c = 0; c < NR; c++) acc[c] = T(0);
+            for (int j = 0; j < NB; j++) {
+                const T d = TRANS ? D[j * NB + tid] : D[tid * NB + j];
+                for (int c = 0; c < NR; c++) acc[c] += d * xs[sb * NB + j][c];
+            }
+            for (int c = 0; c < NR; c++) ys[tid][c] = acc[c];
+        }
+        __syncthreads();
+        if (tid < NB)
+            for (int c = 0; c < NR; c++) xs[sb * NB + tid][c] = ys[tid][c];
+        __syncthreads();
+        // the rows of the panel still to be solved take this block's contribution
+        const int r = (LOWER ? (sb + 1) * NB : 0) + tid, r1 = LOWER ? nb : sb * NB;
+        if (r < r1) {
+            T acc[NR];
+            for (int c = 0; c < NR; c++) acc[c] = T(0);
+            const long long gi = k0 + r;
+            for (int j = 0; j < NB; j++) {
+                const long long gj = k0 + sb * NB + j;
+                const T g = TRANS ? G[gj * ld + gi] : G[gi * ld + gj];
+                for (int c = 0; c < NR; c++) acc[c] += g * xs[sb * NB + j][c];
+            }
+            for (int c = 0; c < NR; c++) xs[r][c] -= acc[c];
+        }
+        __syncthreads();
+    }
+    if (tid < nb)
+        for (int c = 0; c < NR; c++) X[(long long)(k0 + tid) * NR + c] = xs[tid][c];
+}
+
+// x_i -= sum over the panel's columns j in [j0, j0 + nj) of G[i][j] x_j for the rows i in [i0, i1): a wavefront per
+// row, coalesced along it.
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void trsv_update(const T *__restrict__ G, long long ld, int j0, int nj, int i0, int i1,
+                                                   T *__restrict__ X)
+{
+    __shared__ T xs[NS][NR];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < nj)
+        for (int c = 0; c < NR; c++) xs[tid][c] = X[(long long)(j0 + tid) * NR + c];
+    __syncthreads();
+    const long long i = i0 + (long long)blockIdx.x * 4 + wave;
+    if (i >= i1) return;
+    const T *g = G + i * ld + j0;
+    T acc[NR];
+    for (int c = 0; c < NR; c++) acc[c] = T(0);
+    for (int j = lane; j < nj; j += 64) {
+        const T a = g[j];
+        for (int c = 0; c < NR; c++) acc[c] += a * xs[j][c];
+    }
+    for (int c = 0; c < NR; c++)
+        for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o);
+    if (lane == 0)
+        for (int c = 0; c < NR; c++) X[i * NR + c] -= acc[c];
+}
+
+// The same with G[j][i] for G[i][j]: a thread per row i (coalesced along G's rows), the panel's columns split over
+// the four wavefronts and their sums added in wavefront order.
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void trsv_update_t(const T *__restrict__ G, long long ld, int j0, int nj, int i0,
+                                                     int i1, T *__restrict__ X)
+{
+    __shared__ T xs[NS][NR];
+    __shared__ T part[4][64][NR];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < nj)
+        for (int c = 0; c < NR; c++) xs[tid][c] = X[(long long)(j0 + tid) * NR + c];
+    __syncthreads();
+    const long long i = i0 + (long long)blockIdx.x * 64 + lane;
+    T acc[NR];
+    for (int c = 0; c < NR; c++) acc[c] = T(0);
+    if (i < i1) {
+        const int jb = wave * (NS / 4), je = min(nj, jb + NS / 4);
+        for (int j = jb; j < je; j++) {
+            const T a = G[(long long)(j0 + j) * ld + i];
+            for (int c = 0; c < NR; c++) acc[c] += a * xs[j][c];
+        }
+    }
+    for (int c = 0; c < NR; c++) part[wave][lane][c] = acc[c];
+    __syncthreads();
+    if (wave == 0 && i < i1)
+        for (int c = 0; c < NR; c++)
+            X[i * NR + c] -= ((part[0][lane][c] + part[1][lane][c]) + part[2][lane][c]) + part[3][lane][c];
+}
+
+// G^-1 X (trans: G^-T X) for G = L U in place, npad a multiple of NS: forward over L (trans: Uᵀ), backward over U
+// (trans: Lᵀ), as lu_solve and lu_solve_t do with the GEMM.
+template <typename T, int NR>
+void lu_solve_narrow(hipStream_t s, const T *G, long long ld, int npad, const T *linv, const T *uinv, T *X, bool trans)
+{
+    const int np = npad / NS;
+    for (int p = 0; p < np; p++) {
+        const int k0 = p * NS, i0 = k0 + NS;
+        if (trans) {
+            trsv_panel<T, NR, true, true><<<1, 256, 0, s>>>(G, ld, uinv, k0, NS, X);
+            if (i0 < npad) trsv_update_t<T, NR><<<(npad - i0 + 63) / 64, 256, 0, s>>>(G, ld, k0, NS, i0, npad, X);
+        } else {
+            trsv_panel<T, NR, true, false><<<1, 256, 0, s>>>(G, ld, linv, k0, NS, X);
+            if (i0 < npad) trsv_update<T, NR><<<(npad - i0 + 3) / 4, 256, 0, s>>>(G, ld, k0, NS, i0, npad, X);
+        }
+        HIPCHECK2(hipGetLastError());
+    }
+    for (int p = np - 1; p >= 0; p--) {
+        const int k0 = p * NS;
+        if (trans) {
+            trsv_panel<T, NR, false, true><<<1, 256, 0, s>>>(G, ld, linv, k0, NS, X);
+            if (k0 > 0) trsv_update_t<T, NR><<<(k0 + 63) / 64, 256, 0, s>>>(G, ld, k0, NS, 0, k0, X);
+        } else {
+            trsv_panel<T, NR, false, false><<<1, 256, 0, s>>>(G, ld, uinv, k0, NS, X);
+            if (k0 > 0) trsv_update<T, NR><<<(k0 + 3) / 4, 256, 0, s>>>(G, ld, k0, NS, 0, k0, X);
+        }
+        HIPCHECK2(hipGetLastError());
+    }
+}
+
+template <typename T>
+void lu_solve_any(hipStream_t s, const T *G, long long ld, int npad, const T *linv, const T *uinv, T *X, int nrhs,
+                  bool trans)
+{
+    if (npad % NS != 0) throw std::logic_error("the padded size must be a multiple of 256");
+    switch (nrhs <= GEMV_MAX ? nrhs : 0) {
+    case 1: return lu_solve_narrow<T, 1>(s, G, ld, npad, linv, uinv, X, trans);
+    case 2: return lu_solve_narrow<T, 2>(s, G, ld, npad, linv, uinv, X, trans);
+    case 3: return lu_solve_narrow<T, 3>(s, G, ld, npad, linv, uinv, X, trans);
+    case 4: return lu_solve_narrow<T, 4>(s, G, ld, npad, linv, uinv, X, trans);
+    default:
+        if (trans)
+            lu_solve_t<T>(s, G, ld, npad, linv, uinv, X, nrhs);
+        else
+            lu_solve<T>(s, G, ld, npad, linv, uinv, X, nrhs);
     }
 }
 
@@ -783,7 +1019,7 @@ void Factor::build(Ctx *c, const double *A, long long lda, const vp_constraints 
 void Factor::ensure_ws(int nrhs)
 {
     if (nrhs <= ws_nrhs) return;
-    for (void *p : {(void *)X, (void *)R, (void *)Y, (void *)B, (void *)Xf, (void *)nrm}) {
+    for (void *p : {(void *)X, (void *)R, (void *)Y, (void *)B, (void *)Xf, (void *)nrm, (void *)part}) {
         if (!p) continue;
         owned.erase(std::find(owned.begin(), owned.end(), p));
         (void)hipFree(p);
@@ -795,8 +1031,10 @@ void Factor::ensure_ws(int nrhs)
     B = dmalloc<double>(sz);
     Xf = dmalloc<float>(sz);
     nrm = dmalloc<double>(2 * (size_t)nrhs);
-    owned.insert(owned.end(), {X, R, Y, B, Xf, nrm});
+    part = dmalloc<double>((size_t)KSPLIT * npad * std::min(nrhs, GEMV_MAX));
+    owned.insert(owned.end(), {X, R, Y, B, Xf, nrm, part});
     ws_nrhs = nrhs;
+    ws_epoch++;
 }
 
 // y = M^-1 r, or M^-T r with trans, for the current workspace block (in: Y holds r, out: Y holds the product),
@@ -809,20 +1047,51 @@ void Factor::apply_inverse(int nrhs, bool trans)
     rbt_rows<<<blocks(h * nrhs), 256, 0, s>>>(Y, npad, nrhs, trans ? d_rv : d_ru, R);
     HIPCHECK2(hipGetLastError());
     if (mode == 0) {
-        if (trans)
-            lu_solve_t<double>(s, G64, ld, npad, linv64, uinv64, R, nrhs);
-        else
-            lu_solve<double>(s, G64, ld, npad, linv64, uinv64, R, nrhs);
+        lu_solve_any<double>(s, G64, ld, npad, linv64, uinv64, R, nrhs, trans);
     } else {
         convert<double, float><<<blocks((long long)npad * nrhs), 256, 0, s>>>(R, Xf, (long long)npad * nrhs);
-        if (trans)
-            lu_solve_t<float>(s, G32, ld, npad, linv32, uinv32, Xf, nrhs);
-        else
-            lu_solve<float>(s, G32, ld, npad, linv32, uinv32, Xf, nrhs);
+        lu_solve_any<float>(s, G32, ld, npad, linv32, uinv32, Xf, nrhs, trans);
         convert<float, double><<<blocks((long long)npad * nrhs), 256, 0, s>>>(Xf, R, (long long)npad * nrhs);
     }
     rbt_apply_v<<<blocks(h * nrhs), 256, 0, s>>>(R, npad, nrhs, trans ? d_ru : d_rv, Y);
     HIPCHECK2(hipGetLastError());
+}
+
+// apply_inverse through a HIP graph: the first call for a direction, nrhs and workspace runs eagerly, the second
+// captures the same launches from the stream and instantiates them, later ones replay the graph.  The captured work is
+// kernels only (no memset or copy nodes); it reads the factors and reads and writes only the workspace.
+void Factor::apply_inverse_replay(int nrhs, bool trans)
+{
+    hipStream_t s = ctx->stream;
+    InvGraph &g = inv_graph[trans ? 1 : 0];
+    if (g.nrhs != nrhs || g.epoch != ws_epoch) {
+        if (g.exec) HIPCHECK2(hipGraphExecDestroy(g.exec));
+        g.exec = nullptr;
+        g.nrhs = nrhs;
+        g.epoch = ws_epoch;
+        g.calls = 0;
+    }
+    if (!g.exec && ++g.calls < 2) {
+        apply_inverse(nrhs, trans);
+        return;
+    }
+    if (!g.exec) {
+        hipGraph_t graph = nullptr;
+        HIPCHECK2(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            apply_inverse(nrhs, trans);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        HIPCHECK2(hipStreamEndCapture(s, &graph));
+        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHECK2(e);
+    }
+    HIPCHECK2(hipGraphLaunch(g.exec, s));
+    graph_launches++;
 }
 
 // Solve M X = Bh, or Mᵀ X = Bh with trans (n x nrhs row-major host or device), with FP64 iterative refinement (LAPACK
@@ -835,7 +1104,7 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
     HIPCHECK2(hipMemsetAsync(B, 0, sizeof(double) * sz, s));
     HIPCHECK2(hipMemcpyAsync(B, Bin, sizeof(double) * nn, device_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     HIPCHECK2(hipMemcpyAsync(Y, B, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
-    apply_inverse(nrhs, trans);
+    apply_inverse_replay(nrhs, trans);
     HIPCHECK2(hipMemcpyAsync(X, Y, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
     if (trans && anorm1 <= 0) {
         double *dcol = dmalloc<double>(npad);
@@ -861,12 +1130,19 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
     int it = 0;
     double worst = 0.0;
     for (;; it++) {
-        // R = B - M X (FP64, MFMA)
+        // R = B - M X in FP64: a GEMV pass over M for a few right hand sides, else the MFMA GEMM
         HIPCHECK2(hipMemcpyAsync(Y, B, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
-        if (trans)
-            launch_gemm_cfg<double, true, true>(0, s, npad, nrhs, npad, M, ld, X, nrhs, Y, nrhs);
-        else
-            launch_gemm<double, true>(s, npad, nrhs, npad, M, ld, X, nrhs, Y, nrhs);
+        switch (nrhs <= GEMV_MAX ? nrhs : 0) {
+        case 1: launch_residual<1>(s, trans, M, ld, npad, X, Y, part); break;
+        case 2: launch_residual<2>(s, trans, M, ld, npad, X, Y, part); break;
+        case 3: launch_residual<3>(s, trans, M, ld, npad, X, Y, part); break;
+        case 4: launch_residual<4>(s, trans, M, ld, npad, X, Y, part); break;
+        default:
+            if (trans)
+                launch_gemm_cfg<double, true, true>(0, s, npad, nrhs, npad, M, ld, X, nrhs, Y, nrhs);
+            else
+                launch_gemm<double, true>(s, npad, nrhs, npad, M, ld, X, nrhs, Y, nrhs);
+        }
         colmax2<<<nrhs, 256, 0, s>>>(Y, X, n, nrhs, nrm);
         HIPCHECK2(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, s));
         HIPCHECK2(hipStreamSynchronize(s));
@@ -877,7 +1153,7 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
             worst = std::max(worst, hn[2 * c] / std::max(hn[2 * c + 1] * opnorm, 1e-300));
         }
         if (done || it >= max_iter) break;
-        apply_inverse(nrhs, trans);
+        apply_inverse_replay(nrhs, trans);
         axpy_inplace<<<blocks((long long)sz), 256, 0, s>>>(X, Y, (long long)sz);
         HIPCHECK2(hipGetLastError());
     }
