@@ -1210,8 +1210,62 @@ __device__ __forceinline__ void near_scatter(const AsmParams &P, int t, int s, c
     }
 }
 
-// Plain near pairs whose near rule is the single root piece (near_rule, assemble.jl:69-73): one thread per pair,
-// the 7-point rule on the whole test triangle.
+// Plain near pairs whose near rule is the single root piece (near_rule, assemble.jl:69-73), forward: one lane per
+// rule point, WAVE / n pairs of n points per wavefront (nine of the default seven).  The lanes of a pair are summed in
+// point order (the reference's accumulation order) by shuffles, and the pair's first lane scatters its block.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void near_points(
+    AsmParams P, RuleTab rt, const Elem *__restrict__ el, const double *__restrict__ sigma,
+    const int *__restrict__ corner_slot, const int *__restrict__ src_cols, const int2 *__restrict__ list,
+    unsigned int npairs, double *__restrict__ A, double *__restrict__ bvec, unsigned long long *__restrict__ stats,
+    uint8_t *__restrict__ tiers)
+{
+    __shared__ unsigned long long wst;
+    if (threadIdx.x == 0) wst = 0;
+    __syncthreads();
+    const int nr = rt.n[R_NEAR], off = rt.off[R_NEAR];
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+    const int per = WAVE / nr, slot = lane / nr, q = lane - slot * nr;
+    unsigned long long mypairs = 0;
+    const unsigned long long stride = (unsigned long long)gridDim.x * 4 * per;
+    for (unsigned long long base = ((unsigned long long)blockIdx.x * 4 + wave) * per; base < npairs; base += stride) {
+        const unsigned long long pi = base + slot;
+        const bool live = slot < per && pi < npairs;
+        double acc[12];
+        for (int i = 0; i < 12; i++) acc[i] = 0.0;
+        int t = 0, s = 0;
+        if (live) {
+            const int2 pr = list[pi];
+            t = pr.x;
+            s = pr.y;
+            const Elem &T = el[t], &S = el[s];
+            const V3<double> X[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
+            const V3<double> SX[3] = {ld3(S.v[0]), ld3(S.v[1]), ld3(S.v[2])};
+            const double fd = P.kernel_far * S.diam;
+            near_point(X, SX, S, ld3(T.n), ld3(S.c), fd * fd, rt.w[off + q] * T.area, rt.l + 3 * (off + q), P.test, acc);
+        }
+        double tot[12];
+        const int first = (slot < per ? slot : 0) * nr;
+        for (int i = 0; i < 12; i++) {
+            double sum = 0.0;
+            for (int k = 0; k < nr; k++) sum += __shfl(acc[i], first + k);
+            tot[i] = sum;
+        }
+        if (live && q == 0) {
+            near_scatter(P, t, s, el[s], tot, sigma, corner_slot, src_cols, A, bvec);
+            if (tiers) tiers[(size_t)t * P.nsrc + s] = (uint8_t)T_NEAR;
+            mypairs++;
+        }
+    }
+    if (mypairs) atomicAdd(&wst, mypairs);
+    __syncthreads();
+    if (threadIdx.x == 0 && wst) {
+        atomicAdd(&stats[T_NEAR], wst);
+        atomicAdd(&stats[6 + T_NEAR], wst * (unsigned long long)nr);
+    }
+}
+
+// Plain near pairs whose near rule is the single root piece, backward: one thread per pair, the rule's points in
+// turn.
 template <int NADJ>  // 0: the forward assembly; 1..ADJ_MAX: backward with that many sets of adjoint weights
 __global__ __launch_bounds__(256) void near_thread(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                    const double *__restrict__ sigma, const int *__restrict__ corner_slot,
@@ -1855,8 +1909,9 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
         HIPCHECK(hipGetLastError());
     }
     if (cnt[2]) {
-        near_thread<0><<<std::min<unsigned int>((cnt[2] + 255) / 256, 65535u), 256, 0, st>>>(
-            P, rt, d_el, d_sigma, d_cs, d_cols, d_near, cnt[2], A, d_b, d_stats, d_tiers, AdjArgs{});
+        const unsigned int per = WAVE / std::max(1, rt.n[R_NEAR]), waves = (cnt[2] + per - 1) / per;
+        near_points<<<std::min<unsigned int>((waves + 3) / 4, 65535u), 256, 0, st>>>(
+            P, rt, d_el, d_sigma, d_cs, d_cols, d_near, cnt[2], A, d_b, d_stats, d_tiers);
         HIPCHECK(hipGetLastError());
     }
     if (cnt[3]) {

@@ -178,8 +178,18 @@ class WakeElements:
 
 
 def nondegenerate(p, q, r):
-    """state.jl:171"""
-    return _norm(np.cross(q - p, r - p)) > 1e-12 * max(np.dot(q - p, q - p), np.dot(r - p, r - p), np.dot(r - q, r - q))
+    """state.jl:171: |(q - p) x (r - p)| > 1e-12 max squared edge, on Python floats (a 3-vector np.cross costs more
+    than the arithmetic)"""
+    p0, p1, p2 = p.tolist()
+    q0, q1, q2 = q.tolist()
+    r0, r1, r2 = r.tolist()
+    u0, u1, u2 = q0 - p0, q1 - p1, q2 - p2
+    v0, v1, v2 = r0 - p0, r1 - p1, r2 - p2
+    w0, w1, w2 = r0 - q0, r1 - q1, r2 - q2
+    c0, c1, c2 = u1 * v2 - u2 * v1, u2 * v0 - u0 * v2, u0 * v1 - u1 * v0
+    return math.sqrt(c0 * c0 + c1 * c1 + c2 * c2) > 1e-12 * max(u0 * u0 + u1 * u1 + u2 * u2,
+                                                                 v0 * v0 + v1 * v1 + v2 * v2,
+                                                                 w0 * w0 + w1 * w1 + w2 * w2)
 
 
 def add_sheet(e, p, q, r, mp, mq, mr):
