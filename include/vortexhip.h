@@ -120,6 +120,10 @@ int vp_plan_adjoint(vp_plan *plan, int nlam, const double *lambda, const double 
  * the right hand side is its negative), the wake columns of the last vp_plan_assemble times the doublets of the wake's
  * nodes (a wake-only plan whose wake holds the known sheets and filaments, one node per corner or filament end). */
 int vp_plan_wake_product(vp_plan *plan, const double *mu_wake, double *out);
+/* New coordinates (nw x 3) for the plan's wake nodes, same triangles and filaments (no counterpart in the reference:
+ * a free wake's known sheets move every step, `step!`, src/freewake/step.jl:113-146, while their topology stays):
+ * the wake's element records are rebuilt in place and the plan is reused; assemble again before reading. */
+int vp_plan_update_wake(vp_plan *plan, const double *wake_x);
 /* `explicit_rhs` of nb vortex particles (positions bx and strengths alpha nb x 3, cores delta): their right hand side
  * (nslot values, added to b) on the plan's body: the far rule beyond far_ratio max(diam, core), else near_rule. */
 int vp_plan_blob_rhs(vp_plan *plan, int nb, const double *bx, const double *alpha, const double *delta, double *out);

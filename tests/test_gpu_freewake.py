@@ -37,6 +37,34 @@ def test_explicit_rhs(wing):
     assert np.abs(only - oracle_explicit_rhs(g, dm, None, blobs, api.AssemblyOptions())).max() < 1e-12 * np.abs(only).max()
 
 
+def test_explicit_rhs_moved_sheets(wing):
+    """explicit_rhs with a plan cache: known sheets of one topology moved twice (the plan's wake updated in place,
+    vp_plan_update_wake) against the oracle at each position, then a different topology (a new plan)."""
+    print("explicit_rhs cached sheet plan", flush=True)
+    g, dm = wing
+    rng = np.random.default_rng(6)
+    cache = {}
+    for k, length in enumerate((0.5, 0.5, 0.5, 0.8)):
+        w = wk.build_wake(g, dm, V, wk.WakeOptions(length=length))
+        if k in (1, 2):   # same topology, nodes moved downstream and sheared
+            w.x = w.x + np.array([0.05 * k, 0.0, 0.02 * k]) + 0.01 * k * np.outer(w.x[:, 1], [0.0, 0.0, 1.0])
+        e = FW.rigid_elements(w, rng.standard_normal(dm.wake_te.n))
+        got = FW.explicit_rhs(g, dm, e, None, cache=cache)
+        ref = oracle_explicit_rhs(g, dm, e, None, api.AssemblyOptions())
+        assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max(), (k, np.abs(got - ref).max() / np.abs(ref).max())
+        if k in (1, 2):
+            assert cache["plan"] is plan, k   # reused
+        plan = cache["plan"]
+    FW.close_cache(cache)
+    assert not cache
+    with pytest.raises(ValueError):
+        p = api.AssemblyPlan(g, dm, w, api.AssemblyOptions(), wake_only=True)
+        try:
+            p.update_wake(np.zeros((p.nw + 1, 3)))
+        finally:
+            p.close()
+
+
 def test_free_wake_steps_match_oracle(wing):
     print("free wake 8 steps fp64 vs oracle", flush=True)
     g, dm = wing

@@ -1689,6 +1689,7 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
             cols[3 * (nt + t) + k] = nslot + wk->tris[3 * t + k];
         }
     double *dx = dupload(xall, st);
+    d_x = dx;
     d_nodes = dupload(en, st);
     d_cols = dupload(cols, st);
     d_cs = dupload(std::vector<int>(g.corner_slot, g.corner_slot + 3 * nt), st);
@@ -1721,6 +1722,7 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
     }
     d_seg = dupload(segx, st);
     d_segcol = dupload(segc, st);
+    if (nseg) h_fil.assign(wk->filaments, wk->filaments + 2 * nseg);
     owned.insert(owned.end(), {d_seg, d_segcol});
 
     // touch pairs, sorted with both orders (assemble.jl:383-386)

@@ -183,6 +183,18 @@ int vp_plan_wake_product(vp_plan *plan, const double *mu_wake, double *out)
     return guard(c, [&] { plan->p.wake_product(mu_wake, out); });
 }
 
+int vp_plan_update_wake(vp_plan *plan, const double *wake_x)
+{
+    if (!plan || !plan->p.ctx) return VP_E_ARGUMENT;
+    vp::Ctx *c = plan->p.ctx;
+    if (!wake_x) return fail(c, VP_E_ARGUMENT, "null argument");
+    if (plan->p.nw <= 0) return fail(c, VP_E_DIMENSION, "the plan has no wake");
+    return guard(c, [&] {
+        (void)hipSetDevice(c->device);
+        plan->p.update_wake(wake_x);
+    });
+}
+
 int vp_plan_blob_rhs(vp_plan *plan, int nb, const double *bx, const double *alpha, const double *delta, double *out)
 {
     if (!plan || !plan->p.ctx) return VP_E_ARGUMENT;

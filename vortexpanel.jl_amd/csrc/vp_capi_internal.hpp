@@ -39,8 +39,9 @@ struct Plan {
     Elem *d_el = nullptr;
     TestFar *d_tf = nullptr;
     int *d_nodes = nullptr, *d_cols = nullptr, *d_cs = nullptr;
-    double *d_seg = nullptr;
+    double *d_seg = nullptr, *d_x = nullptr;  // filament ends; every node's coordinates (body, then wake)
     int *d_segcol = nullptr;
+    std::vector<int> h_fil;                    // the wake's filaments (node pairs), kept for update_wake
     int2 *d_touch = nullptr;
     int2 *d_near = nullptr;
     int *d_tte = nullptr, *d_ttr = nullptr, *d_ttn = nullptr, *d_tse = nullptr, *d_tsc = nullptr, *d_tsn = nullptr;
@@ -78,6 +79,7 @@ struct Plan {
     void kernel_ms(double *ms);
     AsmParams params() const;
     void wake_product(const double *mu_nodes, double *out);
+    void update_wake(const double *wake_x);
     void blob_rhs(int nb, const double *bx, const double *ba, const double *bd, double *out);
     void read_stats(unsigned long long *h);
     void adjoint(int nlam, const double *lam, const double *mu, const double *vinf, const double *dvinf, double *G,

@@ -163,6 +163,36 @@ __global__ void rows_to_slots(const double *__restrict__ rows, const int *__rest
     if (i < 3 * nt) unsafeAtomicAdd(&b[corner_slot[i]], rows[i]);
 }
 
+// New coordinates for the wake's nodes, same triangles and filaments (a free wake between steps): the wake's element
+// records and filament ends are rebuilt in place.  The tiles and the work lists depend on the topology alone (every
+// pair is classified again at each assembly), so the plan is reused without a rebuild.
+void Plan::update_wake(const double *wake_x)
+{
+    if (!nw) throw std::invalid_argument("the plan has no wake");
+    hipStream_t st = ctx->stream;
+    HIPCHECK(hipMemcpyAsync(d_x + 3 * (size_t)nn, wake_x, sizeof(double) * 3 * nw, hipMemcpyHostToDevice, st));
+    if (nwt) {
+        elem_setup<<<(nwt + 255) / 256, 256, 0, st>>>(d_x, d_nodes + 3 * (size_t)nt, nwt, d_el + nt);
+        HIPCHECK(hipGetLastError());
+        std::vector<Elem> hel(nwt);
+        HIPCHECK(hipMemcpyAsync(hel.data(), d_el + nt, sizeof(Elem) * nwt, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        for (int e = 0; e < nwt; e++)
+            if (hel[e].pad0 != 0.0) throw std::invalid_argument("degenerate triangle " + std::to_string(nt + e));
+    }
+    if (nseg) {
+        std::vector<double> segx((size_t)6 * nseg);
+        for (int f = 0; f < nseg; f++)
+            for (int k = 0; k < 3; k++) {
+                segx[6 * f + k] = wake_x[3 * h_fil[2 * f] + k];
+                segx[6 * f + 3 + k] = wake_x[3 * h_fil[2 * f + 1] + k];
+            }
+        HIPCHECK(hipMemcpyAsync(d_seg, segx.data(), sizeof(double) * segx.size(), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    assembled = false;
+}
+
 void Plan::wake_product(const double *mu_nodes, double *out)
 {
     hipStream_t st = ctx->stream;

@@ -213,6 +213,14 @@ class AssemblyPlan:
         _check(self.ctx.h, self.ctx.lib.vp_plan_wake_product(self.h, L.ptr(mw) if self.nw else None, L.ptr(out)))
         return out
 
+    def update_wake(self, x):
+        """New coordinates (nw x 3) for the wake's nodes, same triangles and filaments: the plan is reused (assemble
+        again before reading)"""
+        wx = L.f64(x).reshape(-1, 3)
+        if wx.shape != (self.nw, 3):
+            raise ValueError(f"wake coordinates have {wx.shape}, expected ({self.nw}, 3)")
+        _check(self.ctx.h, self.ctx.lib.vp_plan_update_wake(self.h, L.ptr(wx)))
+
     def blob_rhs(self, bx, alpha, delta):
         """explicit_rhs of vortex particles on the plan's body (nslot values), src/assembly/explicit.jl:60-79"""
         bx, ba = L.f64(bx).reshape(-1, 3), L.f64(alpha).reshape(-1, 3)

@@ -379,21 +379,32 @@ def _known_wake(e):
 
 
 def explicit_rhs(g, dm, e: WakeElements | None, blobs: Blobs | None, opts: AssemblyOptions | None = None,
-                 ctx: Context | None = None, body_plan: AssemblyPlan | None = None):
+                 ctx: Context | None = None, body_plan: AssemblyPlan | None = None, cache: dict | None = None):
     """-int psi_i n.u_src dA for every slot (src/assembly/explicit.jl:31-86) on the device.  The sheets and filaments
     take the assembly's tiers (a wake-only plan: exactly their assembled columns times their doublets); the particles
     the far rule or near_rule (the plan's particle kernels).  body_plan: a wake-only plan of the body alone, reused for
-    the particles."""
+    the particles.  cache: a dict that keeps the sheets' plan between calls; a call with as many sheets and filaments
+    as the last moves that plan's wake (AssemblyPlan.update_wake) instead of building a new one (close it with
+    close_cache)."""
     ctx = ctx or default_context()
     opts = opts or AssemblyOptions()
     b = np.zeros(dm.nslot)
     zero = np.zeros(len(g.tris))
     if e is not None and (e.sheets or e.segments):
         w, mu = _known_wake(e)
-        plan = AssemblyPlan(g, dm, w, opts, ctx=ctx, wake_only=True)
+        key = (len(e.sheets), len(e.segments))
+        plan = cache.get("plan") if cache is not None and cache.get("key") == key else None
+        if plan is not None:
+            plan.update_wake(w.x)
+        else:
+            close_cache(cache)
+            plan = AssemblyPlan(g, dm, w, opts, ctx=ctx, wake_only=True)
         plan.assemble(zero, np.zeros(3))
         b -= plan.wake_product(mu)
-        plan.close()
+        if cache is not None:
+            cache.update(key=key, plan=plan)
+        else:
+            plan.close()
     if blobs is not None and len(blobs):
         plan = body_plan
         if plan is None:
@@ -403,6 +414,14 @@ def explicit_rhs(g, dm, e: WakeElements | None, blobs: Blobs | None, opts: Assem
         if body_plan is None:
             plan.close()
     return b
+
+
+def close_cache(cache: dict | None):
+    """release the sheets' plan an explicit_rhs cache keeps"""
+    if cache and cache.get("plan") is not None:
+        cache["plan"].close()
+    if cache is not None:
+        cache.clear()
 
 
 # ------------------------------------------------------------------------------------------ the simulation ----------
@@ -494,9 +513,11 @@ class FreeWakeSim:
         self.body_plan = AssemblyPlan(self.g, self.dm, None, self.assembly, ctx=ctx, wake_only=True)
         self.body_plan.assemble(np.zeros(len(self.g.tris)), np.zeros(3))
         self.field = post.WakeField(self.g, kernel_far=self.assembly.kernel_far, precision=self.precision, ctx=ctx)
+        self._sheet_plan = {}  # the free rows' sheets: one wake-only plan, moved from step to step
 
     def _explicit_rhs(self, we, particles):
-        return explicit_rhs(self.g, self.dm, we, particles, self.assembly, ctx=self.ctx, body_plan=self.body_plan)
+        return explicit_rhs(self.g, self.dm, we, particles, self.assembly, ctx=self.ctx, body_plan=self.body_plan,
+                            cache=self._sheet_plan)
 
     def _solve(self, rhs):
         out = solve_factored(self.F, rhs)
@@ -592,3 +613,4 @@ class FreeWakeSim:
         self.F.close()
         self.body_plan.close()
         self.field.close()
+        close_cache(self._sheet_plan)
