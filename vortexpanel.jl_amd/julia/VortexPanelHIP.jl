@@ -370,6 +370,60 @@ function blob_velocities(blobs::AbstractVector, X::AbstractVector{<:SVector{3}};
     return [SVector(u[1, i], u[2, i], u[3, i]) for i in axes(u, 2)]
 end
 
+# ---- the free wake's velocity sums as one HIP graph -----------------------------------------------------------------
+
+"""
+    WakeField(g; kernel_far, precision)
+
+The free-wake step's velocity sums (`wake_velocities`, src/freewake/step.jl:166-173) as one HIP graph on the device:
+the body's sources and sheets and the wake's blobs at the targets.  The body's elements are set up once; every call
+rewrites the graph's node arguments, so the particle count may change from step to step.
+"""
+mutable struct WakeField
+    ptr::Ptr{Cvoid}
+    ctx::Context
+    nt::Int
+end
+
+function WakeField(g::SurfaceGeometry; kernel_far::Real=10.0, precision::Symbol=:fp64, ctx::Context=default_context())
+    x = flat3(g.x, Cdouble)
+    tris = Cint[t[k] - 1 for t in g.tris for k in 1:3]
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve x tris begin
+        sf = VPSurface(length(g.x), pointer(x), length(g.tris), pointer(tris), C_NULL, 0, 0, C_NULL)
+        check(ctx, ccall((:vp_wake_field_create, libvortexhip), Cint,
+            (Ptr{Cvoid}, Ref{VPSurface}, Cdouble, Cint, Ref{Ptr{Cvoid}}),
+            ctx.ptr, Ref(sf), kernel_far, precision === :mixed ? 1 : 0, out))
+    end
+    wf = WakeField(out[], ctx, length(g.tris))
+    finalizer(w -> w.ptr == C_NULL || ccall((:vp_wake_field_destroy, libvortexhip), Cvoid, (Ptr{Cvoid},), w.ptr), wf)
+    return wf
+end
+
+"""
+    (wf::WakeField)(σ, γ, X, blobs; base = nothing)
+
+base (V∞, or nothing) + the body's kernels (σ per triangle, γ the sheet strength per triangle) + the blobs at X.
+"""
+function (wf::WakeField)(σ::AbstractVector, γ::AbstractVector, X::AbstractVector{<:SVector{3}}, blobs::AbstractVector;
+                         base=nothing)
+    sig = Vector{Cdouble}(σ)
+    gam = flat3(γ, Cdouble)
+    Xf = flat3(X, Cdouble)
+    bx = flat3([b.x for b in blobs], Cdouble)
+    ba = flat3([b.α for b in blobs], Cdouble)
+    bd = Cdouble[b.δ for b in blobs]
+    bv = base === nothing ? C_NULL : Cdouble[base[1], base[2], base[3]]
+    u = zeros(Cdouble, 3, length(X))
+    GC.@preserve sig gam Xf bx ba bd bv u begin
+        check(wf.ctx, ccall((:vp_wake_field_eval, libvortexhip), Cint,
+            (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Cint, Ptr{Cdouble}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble},
+             Ptr{Cdouble}, Ptr{Cdouble}),
+            wf.ptr, sig, gam, length(X), Xf, length(blobs), bx, ba, bd, bv, u))
+    end
+    return [SVector(u[1, i], u[2, i], u[3, i]) for i in axes(u, 2)]
+end
+
 # ---- the adjoint's device pieces -----------------------------------------------------------------------------------
 
 """
