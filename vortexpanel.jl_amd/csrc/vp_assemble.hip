@@ -1365,8 +1365,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
 {
     constexpr int MAXINT = (1 << (2 * NG_MAXDEPTH)) / 3 + 1;  // internal nodes of a depth-5 tree (341)
     constexpr int MAXLEAF = 1 << (2 * NG_MAXDEPTH);           // 1024
+    constexpr int LB = WAVE;  // leaves whose corners and area fraction are kept in LDS for the point loop
+    constexpr int PBMAX = 4;  // pairs whose split tests share the lanes
     __shared__ unsigned long long splitw[4][(MAXINT + 63) / 64];
     __shared__ unsigned short leaf[4][MAXLEAF];
+    __shared__ double leafb[4][LB][10];
     __shared__ unsigned long long wst[2];
     const int lane = threadIdx.x & (WAVE - 1), wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (threadIdx.x < 2) wst[threadIdx.x] = 0;
@@ -1377,33 +1380,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
         if (Lv < depth) nint += c;
         if (Lv > 0) ncand += c;
     }
+    // a batch of PB pairs per wavefront: their split tests fill the lanes together when PB * nint <= 64
+    const int PB = nint > 0 && nint <= WAVE / 2 ? min(PBMAX, WAVE / nint) : 1;
     const int nr = rt.n[R_NEAR], off = rt.off[R_NEAR];
     unsigned long long mypairs = 0, mypts = 0;
     double bdsum[NADJ > 0 ? NADJ : 1] = {};
-    for (unsigned int pi = blockIdx.x * 4 + wave; pi < npairs; pi += gridDim.x * 4) {
+    // split test of internal node nid of the near tree on test triangle X against source SX (near_pieces!,
+    // assemble.jl:131-142; FP64 without contraction, as the reference's rule builder)
+    auto split_test = [&](const V3<double> *X, const V3<double> *SX, int nid) {
+#pragma clang fp contract(off)
+        int L, idx;
+        node_of(nid, L, idx);
+        double b[3][3];
+        node_corners(L, idx, b);
+        const V3<double> p1 = cls_point(X, b[0]), p2 = cls_point(X, b[1]), p3 = cls_point(X, b[2]);
+        const double size = fmax(fmax(cls_norm(cls_sub(p2, p1)), cls_norm(cls_sub(p3, p2))), cls_norm(cls_sub(p1, p3)));
+        V3<double> cen = (p1 + p2) + p3;
+        cen = mk3(cen.x / 3.0, cen.y / 3.0, cen.z / 3.0);
+        return cls_tri_distance(SX, cen) < P.near_ratio * size;
+    };
+    for (unsigned int b0 = (blockIdx.x * 4 + wave) * PB; b0 < npairs; b0 += gridDim.x * 4 * PB) {
+        // 1a. split tests of the batch's pairs at once (one bit word per pair)
+        unsigned long long sw[PBMAX] = {0, 0, 0, 0};
+        if (PB > 1) {
+            const int bb = lane / nint, nid = lane - bb * nint;
+            bool split = false;
+            if (bb < PB && b0 + bb < npairs) {
+                const int2 pb = list[b0 + bb];
+                const Elem &Tb = el[pb.x], &Sb = el[pb.y];
+                const V3<double> Xb[3] = {ld3(Tb.v[0]), ld3(Tb.v[1]), ld3(Tb.v[2])};
+                const V3<double> SXb[3] = {ld3(Sb.v[0]), ld3(Sb.v[1]), ld3(Sb.v[2])};
+                split = split_test(Xb, SXb, nid);
+            }
+            const unsigned long long m = __ballot(split);
+            for (int k = 0; k < PB; k++) sw[k] = (m >> (k * nint)) & ((1ull << nint) - 1ull);
+        }
+        for (int bb = 0; bb < PB; bb++) {
+        const unsigned int pi = b0 + bb;
+        if (pi >= npairs) break;
         const int2 pr = list[pi];
         const int t = pr.x, s = pr.y;
         const Elem &T = el[t], &S = el[s];
         const V3<double> X[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
         const V3<double> SX[3] = {ld3(S.v[0]), ld3(S.v[1]), ld3(S.v[2])};
-        // 1. split tests of all internal nodes
-        for (int base = 0; base < nint; base += WAVE) {
-            const int nid = base + lane;
-            bool split = false;
-            if (nid < nint) {
-#pragma clang fp contract(off)
-                int L, idx;
-                node_of(nid, L, idx);
-                double b[3][3];
-                node_corners(L, idx, b);
-                const V3<double> p1 = cls_point(X, b[0]), p2 = cls_point(X, b[1]), p3 = cls_point(X, b[2]);
-                const double size = fmax(fmax(cls_norm(cls_sub(p2, p1)), cls_norm(cls_sub(p3, p2))), cls_norm(cls_sub(p1, p3)));
-                V3<double> cen = (p1 + p2) + p3;
-                cen = mk3(cen.x / 3.0, cen.y / 3.0, cen.z / 3.0);
-                split = cls_tri_distance(SX, cen) < P.near_ratio * size;
+        // 1b. one pair's split tests over the lanes (deeper trees)
+        if (PB > 1) {
+            if (lane == 0) splitw[wave][0] = sw[bb];
+        } else {
+            for (int base = 0; base < nint; base += WAVE) {
+                const int nid = base + lane;
+                const bool split = nid < nint && split_test(X, SX, nid);
+                const unsigned long long m = __ballot(split);
+                if (lane == 0) splitw[wave][base / WAVE] = m;
             }
-            const unsigned long long m = __ballot(split);
-            if (lane == 0) splitw[wave][base / WAVE] = m;
         }
         // 2. leaves: every ancestor splits, the node itself does not (or is at the depth)
         int nl = 0;
@@ -1427,6 +1456,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
             if (isleaf) leaf[wave][nl + __popcll(m & ((1ull << lane) - 1ull))] = (unsigned short)nid;
             nl += __popcll(m);
         }
+        // the first LB leaves' corners and area fractions, once per leaf for the point loop
+        if (lane < nl && lane < LB) {
+            int L, idx;
+            node_of(leaf[wave][lane], L, idx);
+            double b[3][3];
+            node_corners(L, idx, b);
+            double frac = 1.0;
+            for (int i = 0; i < L; i++) frac /= 4;
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) leafb[wave][lane][3 * i + j] = b[i][j];
+            leafb[wave][lane][9] = frac;
+        }
+        // leaf li's corners b and area fraction (LDS for the first LB leaves)
+        auto leaf_geom = [&](int li, double (*b)[3], double &frac) {
+            if (li < LB) {
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) b[i][j] = leafb[wave][li][3 * i + j];
+                frac = leafb[wave][li][9];
+            } else {
+                int L, idx;
+                node_of(leaf[wave][li], L, idx);
+                node_corners(L, idx, b);
+                frac = 1.0;
+                for (int i = 0; i < L; i++) frac /= 4;
+            }
+        };
         // 3. the leaves' points
         const double fd = P.kernel_far * S.diam;
         if constexpr (NADJ > 0) {
@@ -1435,15 +1490,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
 #pragma unroll 1
             for (int pp = lane; pp < nl * nr; pp += WAVE) {
                 const int li = pp / nr, q = pp - li * nr;
-                int L, idx;
-                node_of(leaf[wave][li], L, idx);
-                double b[3][3];
-                node_corners(L, idx, b);
+                double b[3][3], frac;
+                leaf_geom(li, b, frac);
                 const double *m = rt.l + 3 * (off + q);
                 double l[3];
                 for (int j = 0; j < 3; j++) l[j] = (m[0] * b[0][j] + m[1] * b[1][j]) + m[2] * b[2][j];
-                double frac = 1.0;
-                for (int i = 0; i < L; i++) frac /= 4;
                 adj_point(ap, X, S, 0, P.kernel_far, rt.w[off + q] * frac, l, P.test);
             }
             for (int l = 0; l < NADJ; l++) adj_wave_reduce(ap.p[l]);
@@ -1455,15 +1506,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
 #pragma unroll 1
         for (int pp = lane; pp < nl * nr; pp += WAVE) {
             const int li = pp / nr, q = pp - li * nr;
-            int L, idx;
-            node_of(leaf[wave][li], L, idx);
-            double b[3][3];
-            node_corners(L, idx, b);
+            double b[3][3], frac;
+            leaf_geom(li, b, frac);
             const double *m = rt.l + 3 * (off + q);
             double l[3];
             for (int j = 0; j < 3; j++) l[j] = (m[0] * b[0][j] + m[1] * b[1][j]) + m[2] * b[2][j];
-            double frac = 1.0;
-            for (int i = 0; i < L; i++) frac /= 4;
             near_point(X, SX, S, ld3(T.n), ld3(S.c), fd * fd, rt.w[off + q] * frac * T.area, l, P.test, acc);
         }
         double tot[12];
@@ -1474,6 +1521,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
         }
         mypairs++;
         mypts += (unsigned long long)nl * nr;
+        }
     }
     if (lane == 0 && mypairs) {
         atomicAdd(&wst[0], mypairs);
