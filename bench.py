@@ -178,7 +178,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)  # the launch autotuner times its candidates on the first two
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the assembly metric (no wing, sweep, N-body)")
     ap.add_argument("--extras", default="nbody,alpha_sweep,wing_adjoint,free_wake",
@@ -323,6 +323,7 @@ def main():
                   "max_entry_rel_diff_vs_fp64": entry_err, "max_mu_rel_diff_vs_fp64": dmu},
         "solve": solves,
         "gemm_tuning": json.loads(ctx.lib.vp_gemm_tuning().decode()),
+        "assembly_tuning": json.loads(ctx.lib.vp_assembly_tuning().decode()),
         "kernels_ms": tm,
         **extras,
     }

@@ -174,6 +174,12 @@ void vp_factor_destroy(vp_factor *f);
  * and each candidate's milliseconds.  The tuner runs inside the first vp_factorize per precision; the environment
  * variable VP_GEMM_TILE=<index> forces a candidate instead.  The string lives until the next call. */
 const char *vp_gemm_tuning(void);
+/* The assembly's launch autotuner (no counterpart in the reference), as JSON: for each forward evaluation kernel
+ * (touch_simple, near_points, near_group, group_eval) the occupancy cap chosen (waves per SIMD; null while it is still
+ * being timed) and each candidate's time in ms on the plan it was timed on.  Candidates run in turn on one plan's
+ * successive assemblies; the fastest is kept for the process.  VP_ASM_WAVES=<3|2> forces one.  The string lives until
+ * the next call. */
+const char *vp_assembly_tuning(void);
 
 /* ---- N-body velocity sums (src/post/Post.jl:32-112, src/freewake/step.jl:176-187) -------------------------- */
 

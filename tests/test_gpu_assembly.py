@@ -88,3 +88,32 @@ def test_bad_option_raises():
     els = api.build_elements(g)
     with pytest.raises(ValueError):
         api.assemble(g, dm, els, api.source_strengths(els, VG), VG, opts=api.AssemblyOptions(test="box"))
+
+
+def test_launch_autotuner():
+    """The assembly's launch autotuner: a plan large enough to time runs each occupancy candidate of the forward
+    evaluation kernels on successive assemblies, every assembly's A agrees with the first (same arithmetic, other
+    register budget) and the oracle's, and the report names a chosen cap for every kernel afterwards."""
+    import json
+    print("launch autotuner on an abused ellipsoid 36 x 72", flush=True)
+    g = mesh.abused_ellipsoid(nth=36, nph=72, axes=(2.0, 1.0, 0.5), jitter=0.25, sliver_frac=0.05, sliver_ratio=0.1,
+                              seed=5)
+    dm = dofmap.build_dofmap(g)
+    V = np.array([np.cos(0.087), 0.0, np.sin(0.087)])
+    sig = api.source_strengths(api.build_elements(g), V)
+    p = api.AssemblyPlan(g, dm, None, api.AssemblyOptions())
+    ref = None
+    for k in range(4):
+        p.assemble(sig, V)
+        A, b = p.read()
+        if ref is None:
+            ref = (A, b)
+        else:
+            assert np.abs(A - ref[0]).max() <= 1e-13 * np.abs(ref[0]).max(), k
+            assert np.abs(b - ref[1]).max() <= 1e-13 * np.abs(ref[1]).max(), k
+    p.close()
+    rep = json.loads(api.default_context().lib.vp_assembly_tuning().decode())
+    for name in ("touch_simple", "near_points", "near_group", "group_eval"):
+        assert rep[name]["chosen_waves"] in (2, 3), (name, rep[name])
+    Ao, bo, _, _ = O.assemble(g, dm, sig, V)
+    assert np.abs(ref[0] - Ao).max() < 1e-12 * np.abs(Ao).max()

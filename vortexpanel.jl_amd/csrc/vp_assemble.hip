@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -902,9 +903,10 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // One wavefront per pair group: the points of all the pair's rule pieces flattened over the 64 lanes (a lane walks
 // the pieces with a cursor), in FP64 (pair_block, assemble.jl:245-258, or segment_block, :312-321), reduced over the
-// wave and added to the pair's block of A and its rows of b.  Capped at three waves per SIMD (C2: 2.12 -> 1.73 ms
-// against two; a few registers spill).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void group_eval(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+// wave and added to the pair's block of A and its rows of b.  WPS: the occupancy cap (waves per SIMD), chosen by the
+// launch autotuner (three measured best on MI355X: C2 2.12 -> 1.73 ms against two; a few registers spill).
+template <int WPS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPS, WPS))) void group_eval(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                   const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                   const int *__restrict__ src_cols, const double *__restrict__ seg,
                                                   const int *__restrict__ segcol, const Leaf *__restrict__ leaves,
@@ -1004,7 +1006,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 // The self pair and shared-edge / shared-corner pairs (no contact): one wavefront per pair.  Its pieces are the
 // stored rule on the whole triangle (self, edge k, vertex k when the corner is at most 90 degrees) or vertex_pieces!'
 // equal-angle wedges of a wide corner (assemble.jl:102-125), kept in LDS; their points are flattened over the lanes.
-// The forward kernel is capped at three waves per SIMD (C2: 10.2 -> 7.9 ms against the two its 249 registers give).
+// The forward kernel's occupancy cap WPS is chosen by the launch autotuner (three waves per SIMD measured best on
+// MI355X: C2 10.2 -> 7.9 ms against the two its 249 registers give); the backward runs at one.
 constexpr int MAXWEDGE = 16;
 
 __device__ __forceinline__ double wave_sum2(double v)
@@ -1013,8 +1016,8 @@ __device__ __forceinline__ double wave_sum2(double v)
     return v;
 }
 
-template <int NADJ>  // 0: the forward assembly; 1..ADJ_MAX: backward with that many sets of adjoint weights
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ? 3 : 1, NADJ == 0 ? 3 : 1))) void touch_simple(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+template <int NADJ, int WPS = 3>  // NADJ 0: the forward assembly; 1..ADJ_MAX: backward with that many weight sets
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ? WPS : 1, NADJ == 0 ? WPS : 1))) void touch_simple(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                     const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                     const int *__restrict__ src_cols, const int4 *__restrict__ list,
                                                     unsigned int npairs, double *__restrict__ A, double *__restrict__ bvec,
@@ -1212,8 +1215,10 @@ __device__ __forceinline__ void near_scatter(const AsmParams &P, int t, int s, c
 
 // Plain near pairs whose near rule is the single root piece (near_rule, assemble.jl:69-73), forward: one lane per
 // rule point, WAVE / n pairs of n points per wavefront (nine of the default seven).  The lanes of a pair are summed in
-// point order (the reference's accumulation order) by shuffles, and the pair's first lane scatters its block.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void near_points(
+// point order (the reference's accumulation order) by shuffles, and the pair's first lane scatters its block.  WPS:
+// the occupancy cap, chosen by the launch autotuner.
+template <int WPS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPS, WPS))) void near_points(
     AsmParams P, RuleTab rt, const Elem *__restrict__ el, const double *__restrict__ sigma,
     const int *__restrict__ corner_slot, const int *__restrict__ src_cols, const int2 *__restrict__ list,
     unsigned int npairs, double *__restrict__ A, double *__restrict__ bvec, unsigned long long *__restrict__ stats,
@@ -1352,11 +1357,11 @@ __device__ __forceinline__ void node_of(int nid, int &L, int &idx)  // level-ord
 // Plain near pairs whose root piece splits: one wavefront per pair evaluates the split test of every node of the
 // full quadtree at once across its lanes (each test depends on that node's own corners only, so this is the
 // recursive near_pieces! exactly), collects the leaves -- nodes whose ancestors all split and that do not split or
-// reach the depth -- by ballot, and evaluates their 7-point rules flattened over the lanes.  Capped at three waves per
-// SIMD forward (C2: 7.77 -> 6.50 ms against two) and two backward (measured faster than the register allocation the
-// compiler picks by itself).
-template <int NADJ>  // 0: the forward assembly; 1..ADJ_MAX: backward with that many sets of adjoint weights
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ? 3 : 2, NADJ == 0 ? 3 : 2))) void near_group(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+// reach the depth -- by ballot, and evaluates their 7-point rules flattened over the lanes.  The forward occupancy cap
+// WPS is chosen by the launch autotuner (three waves per SIMD measured best: C2 7.77 -> 6.50 ms against two); the
+// backward is capped at two (faster than the register allocation the compiler picks by itself).
+template <int NADJ, int WPS = 3>  // NADJ 0: the forward assembly; 1..ADJ_MAX: backward with that many weight sets
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ? WPS : 2, NADJ == 0 ? WPS : 2))) void near_group(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                   const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                   const int *__restrict__ src_cols, const int2 *__restrict__ list,
                                                   unsigned int npairs, double *__restrict__ A, double *__restrict__ bvec,
@@ -1657,10 +1662,118 @@ template <typename X> X *dupload(const std::vector<X> &v, hipStream_t s)
     return p;
 }
 
+// ------------------------------------------------------------------------------------------ launch autotuner ------
+
+// The forward evaluation kernels' occupancy cap (waves per SIMD: the register budget the compiler had) is chosen by
+// timing.  Each candidate runs in turn on one plan's successive assemblies (the same lists, so the same work), timed
+// with HIP events around its launch, and the fastest is kept for the process: the rest of that plan's assemblies and
+// every later plan use it.  A launch shorter than kTuneMinMs is too short to tell the candidates apart: it runs the
+// default and leaves the timing to a larger plan.  VP_ASM_WAVES=<3|2> forces a candidate.
+namespace {
+struct AsmTuneState {
+    int choice[TK_COUNT] = {-1, -1, -1, -1};
+    float ms[TK_COUNT][2] = {{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};
+    long long plan[TK_COUNT] = {0, 0, 0, 0};  // serial of the plan whose assemblies are compared
+    int next[TK_COUNT] = {0, 0, 0, 0};        // the candidate its next assembly runs
+};
+AsmTuneState g_asm;
+std::mutex g_asm_mutex;
+long long g_plan_serial = 0;
+constexpr float kTuneMinMs = 0.2f;
+const char *kTuneNames[TK_COUNT] = {"touch_simple", "near_points", "near_group", "group_eval"};
+
+int forced_waves()
+{
+    const char *env = std::getenv("VP_ASM_WAVES");
+    if (!env) return -1;
+    const int w = std::atoi(env);
+    for (int v = 0; v < 2; v++)
+        if (kTuneWaves[v] == w) return v;
+    return -1;
+}
+}  // namespace
+
+long long next_plan_serial()
+{
+    std::lock_guard<std::mutex> lock(g_asm_mutex);
+    return ++g_plan_serial;
+}
+
+int Plan::tune_pick(int k)
+{
+    tune_timed[k] = false;
+    const int f = forced_waves();
+    if (f >= 0) return f;
+    std::lock_guard<std::mutex> lock(g_asm_mutex);
+    if (g_asm.choice[k] >= 0) return g_asm.choice[k];
+    if (g_asm.plan[k] != 0 && g_asm.plan[k] != serial) return 0;  // another plan is being timed
+    g_asm.plan[k] = serial;
+    tune_timed[k] = true;
+    return g_asm.next[k];
+}
+
+void Plan::tune_collect()
+{
+    for (int k = 0; k < TK_COUNT; k++) {
+        if (!tune_timed[k]) continue;
+        tune_timed[k] = false;
+        float t = 0;
+        HIPCHECK(hipEventSynchronize(tev[k][1]));
+        HIPCHECK(hipEventElapsedTime(&t, tev[k][0], tev[k][1]));
+        std::lock_guard<std::mutex> lock(g_asm_mutex);
+        if (g_asm.choice[k] >= 0 || g_asm.plan[k] != serial) continue;
+        const int v = g_asm.next[k];
+        if (t < kTuneMinMs) {  // too short to tell the candidates apart: left to a larger plan
+            g_asm.plan[k] = 0;
+            g_asm.next[k] = 0;
+            g_asm.ms[k][0] = g_asm.ms[k][1] = -1;
+            continue;
+        }
+        g_asm.ms[k][v] = t;
+        if (v + 1 < 2) {
+            g_asm.next[k] = v + 1;
+        } else {
+            g_asm.choice[k] = g_asm.ms[k][1] < g_asm.ms[k][0] ? 1 : 0;
+        }
+    }
+}
+
+void Plan::tune_release()
+{
+    std::lock_guard<std::mutex> lock(g_asm_mutex);
+    for (int k = 0; k < TK_COUNT; k++)
+        if (g_asm.choice[k] < 0 && g_asm.plan[k] == serial) {  // unfinished: the next plan starts over
+            g_asm.plan[k] = 0;
+            g_asm.next[k] = 0;
+            g_asm.ms[k][0] = g_asm.ms[k][1] = -1;
+        }
+}
+
+std::string asm_tuning_report()
+{
+    std::lock_guard<std::mutex> lock(g_asm_mutex);
+    const int f = forced_waves();
+    std::string out = "{";
+    for (int k = 0; k < TK_COUNT; k++) {
+        const int c = f >= 0 ? f : g_asm.choice[k];
+        out += std::string(k ? ", " : "") + "\"" + kTuneNames[k] + "\": {\"chosen_waves\": " +
+               (c < 0 ? std::string("null") : std::to_string(kTuneWaves[c])) + ", \"ms\": {";
+        for (int v = 0; v < 2; v++)
+            out += std::string(v ? ", " : "") + "\"" + std::to_string(kTuneWaves[v]) + "\": " +
+                   (g_asm.ms[k][v] < 0 ? std::string("null") : std::to_string(g_asm.ms[k][v]));
+        out += "}}";
+    }
+    return out + (f >= 0 ? ", \"forced\": true}" : "}");
+}
+
 Plan::~Plan()
 {
+    tune_release();
     for (int k = 0; k < 6; k++)
         if (ev[k]) (void)hipEventDestroy(ev[k]);
+    for (int k = 0; k < TK_COUNT; k++)
+        for (int j = 0; j < 2; j++)
+            if (tev[k][j]) (void)hipEventDestroy(tev[k][j]);
     for (void *p : owned) (void)hipFree(p);
     if (d_bnear) (void)hipFree(d_bnear);
     if (A && ownsA) (void)hipFree(A);
@@ -1843,6 +1956,9 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
     d_groups = dalloc<int4>(group_cap);
     owned.insert(owned.end(), {d_nonfar, d_touchp, d_near, d_nearm, d_simple, d_leaves, d_groups});
     for (int k = 0; k < 6; k++) HIPCHECK(hipEventCreate(&ev[k]));
+    for (int k = 0; k < TK_COUNT; k++)
+        for (int j = 0; j < 2; j++) HIPCHECK(hipEventCreate(&tev[k][j]));
+    serial = next_plan_serial();
     if (record_tiers) {
         d_tiers = dalloc<uint8_t>((size_t)nt * nsrc);
         owned.push_back(d_tiers);
@@ -1873,6 +1989,15 @@ AsmParams Plan::params() const
 void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
 {
     hipStream_t st = ctx->stream;
+    tune_collect();  // the previous assembly's timed launches
+    // a forward evaluation kernel launched with the occupancy cap the launch autotuner picks, timed while it tunes
+    auto tuned_launch = [&](int k, auto &&launch) {
+        const int v = tune_pick(k);
+        if (tune_timed[k]) HIPCHECK(hipEventRecord(tev[k][0], st));
+        launch(kTuneWaves[v]);
+        HIPCHECK(hipGetLastError());
+        if (tune_timed[k]) HIPCHECK(hipEventRecord(tev[k][1], st));
+    };
     HIPCHECK(hipMemcpyAsync(d_sigma, sigma, sizeof(double) * nt, host_sigma ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, st));
     // The work lists grow when a pass finds more entries than budgeted: a non-far list that overflowed is refilled by
     // the far kernels' enumerate-only pass (their far blocks are in A already), a rule-piece list by running the
@@ -1954,20 +2079,36 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
     gen_pass();
     if (timing) HIPCHECK(hipEventRecord(ev[2], st));
     if (nsimple) {
-        touch_simple<0><<<std::min<unsigned int>((nsimple + 3) / 4, 65535u), 256, 0, st>>>(
-            P, rt, d_el, d_sigma, d_cs, d_cols, d_simple, nsimple, A, d_b, d_stats, d_tiers, AdjArgs{});
-        HIPCHECK(hipGetLastError());
+        const unsigned int g = std::min<unsigned int>((nsimple + 3) / 4, 65535u);
+        tuned_launch(TK_TOUCH, [&](int w) {
+            if (w == 3)
+                touch_simple<0, 3><<<g, 256, 0, st>>>(P, rt, d_el, d_sigma, d_cs, d_cols, d_simple, nsimple, A, d_b,
+                                                       d_stats, d_tiers, AdjArgs{});
+            else
+                touch_simple<0, 2><<<g, 256, 0, st>>>(P, rt, d_el, d_sigma, d_cs, d_cols, d_simple, nsimple, A, d_b,
+                                                       d_stats, d_tiers, AdjArgs{});
+        });
     }
     if (cnt[2]) {
         const unsigned int per = WAVE / std::max(1, rt.n[R_NEAR]), waves = (cnt[2] + per - 1) / per;
-        near_points<<<std::min<unsigned int>((waves + 3) / 4, 65535u), 256, 0, st>>>(
-            P, rt, d_el, d_sigma, d_cs, d_cols, d_near, cnt[2], A, d_b, d_stats, d_tiers);
-        HIPCHECK(hipGetLastError());
+        const unsigned int g = std::min<unsigned int>((waves + 3) / 4, 65535u), n = cnt[2];
+        tuned_launch(TK_NEARPTS, [&](int w) {
+            if (w == 3)
+                near_points<3><<<g, 256, 0, st>>>(P, rt, d_el, d_sigma, d_cs, d_cols, d_near, n, A, d_b, d_stats, d_tiers);
+            else
+                near_points<2><<<g, 256, 0, st>>>(P, rt, d_el, d_sigma, d_cs, d_cols, d_near, n, A, d_b, d_stats, d_tiers);
+        });
     }
     if (cnt[3]) {
-        near_group<0><<<std::min<unsigned int>((cnt[3] + 3) / 4, 65535u), 256, 0, st>>>(
-            P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, cnt[3], A, d_b, d_stats, d_tiers, AdjArgs{});
-        HIPCHECK(hipGetLastError());
+        const unsigned int g = std::min<unsigned int>((cnt[3] + 3) / 4, 65535u), n = cnt[3];
+        tuned_launch(TK_NEARGRP, [&](int w) {
+            if (w == 3)
+                near_group<0, 3><<<g, 256, 0, st>>>(P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, n, A, d_b, d_stats,
+                                                     d_tiers, AdjArgs{});
+            else
+                near_group<0, 2><<<g, 256, 0, st>>>(P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, n, A, d_b, d_stats,
+                                                     d_tiers, AdjArgs{});
+        });
     }
     if (timing) HIPCHECK(hipEventRecord(ev[3], st));
     HIPCHECK(hipMemcpyAsync(cnt + 5, d_count + 5, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
@@ -1992,9 +2133,15 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
     nleaves = cnt[5];
     ngroups = cnt[6];
     if (cnt[6]) {
-        group_eval<<<std::min<unsigned int>((cnt[6] + 3) / 4, 65535u), 256, 0, st>>>(
-            P, rt, d_el, d_sigma, d_cs, d_cols, d_seg, d_segcol, d_leaves, d_groups, cnt[6], A, d_b);
-        HIPCHECK(hipGetLastError());
+        const unsigned int g = std::min<unsigned int>((cnt[6] + 3) / 4, 65535u), n = cnt[6];
+        tuned_launch(TK_GROUP, [&](int w) {
+            if (w == 3)
+                group_eval<3><<<g, 256, 0, st>>>(P, rt, d_el, d_sigma, d_cs, d_cols, d_seg, d_segcol, d_leaves, d_groups,
+                                                  n, A, d_b);
+            else
+                group_eval<2><<<g, 256, 0, st>>>(P, rt, d_el, d_sigma, d_cs, d_cols, d_seg, d_segcol, d_leaves, d_groups,
+                                                  n, A, d_b);
+        });
     }
     if (timing) HIPCHECK(hipEventRecord(ev[4], st));
     if (!wake_only) {

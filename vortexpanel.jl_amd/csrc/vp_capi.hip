@@ -287,6 +287,13 @@ const char *vp_gemm_tuning(void)
     return report.c_str();
 }
 
+const char *vp_assembly_tuning(void)
+{
+    static thread_local std::string report;
+    report = vp::asm_tuning_report();
+    return report.c_str();
+}
+
 int vp_factor_info(vp_factor *f, int *n, double *se)
 {
     if (!f) return VP_E_ARGUMENT;

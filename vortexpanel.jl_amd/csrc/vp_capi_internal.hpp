@@ -27,6 +27,13 @@ struct RuleTab {  // all rules' points (3 barycentric coordinates each) and weig
     int n[9];
 };
 
+// The forward evaluation kernels whose occupancy cap the launch autotuner picks, and its candidates (waves per SIMD;
+// index 0 is the default)
+enum TuneKernel { TK_TOUCH = 0, TK_NEARPTS = 1, TK_NEARGRP = 2, TK_GROUP = 3, TK_COUNT = 4 };
+constexpr int kTuneWaves[2] = {3, 2};
+long long next_plan_serial();
+std::string asm_tuning_report();
+
 struct Plan {
     Ctx *ctx = nullptr;
     vp_assembly_options opts{};
@@ -55,6 +62,9 @@ struct Plan {
     bool record_tiers = false;
     bool timing = false;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t tev[TK_COUNT][2] = {};       // around the launches the autotuner times
+    bool tune_timed[TK_COUNT] = {};
+    long long serial = 0;
     unsigned int nonfar_cap = 0, leaf_cap = 0, nnonfar = 0, nleaves = 0, group_cap = 0, ngroups = 0;
     int4 *d_groups = nullptr;
     int2 *d_nearm = nullptr, *d_touchp = nullptr;
@@ -80,6 +90,9 @@ struct Plan {
     AsmParams params() const;
     void wake_product(const double *mu_nodes, double *out);
     void update_wake(const double *wake_x);
+    int tune_pick(int k);
+    void tune_collect();
+    void tune_release();
     void blob_rhs(int nb, const double *bx, const double *ba, const double *bd, double *out);
     void read_stats(unsigned long long *h);
     void adjoint(int nlam, const double *lam, const double *mu, const double *vinf, const double *dvinf, double *G,

@@ -40,7 +40,7 @@ EXPORTS = ["vp_create", "vp_destroy", "vp_last_error", "vp_sync", "vp_assemble",
            "vp_surface_velocity", "vp_body_velocity", "vp_blob_velocities", "vp_plan_adjoint",
            "vp_factor_solve_transposed", "vp_plan_wake_product", "vp_plan_blob_rhs", "vp_gemm_tuning", "vp_math_probe",
            "vp_wake_field_create", "vp_wake_field_eval", "vp_wake_field_info", "vp_wake_field_destroy",
-           "vp_plan_update_wake"]
+           "vp_plan_update_wake", "vp_assembly_tuning"]
 
 
 def load():
@@ -52,6 +52,8 @@ def load():
         lib = C.CDLL(LIBPATH)
         lib.vp_gemm_tuning.restype = C.c_char_p
         lib.vp_gemm_tuning.argtypes = []
+        lib.vp_assembly_tuning.restype = C.c_char_p
+        lib.vp_assembly_tuning.argtypes = []
         lib.vp_last_error.restype = C.c_char_p
         lib.vp_last_error.argtypes = [C.c_void_p]
         lib.vp_destroy.restype = None
