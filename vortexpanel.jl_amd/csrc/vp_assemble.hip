@@ -1941,8 +1941,10 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
     d_stats = dalloc<unsigned long long>(32);
     d_count = dalloc<unsigned int>(7);
     owned.insert(owned.end(), {d_b, d_sigma, d_stats, d_count});
-    // The non-far list's first budget (it grows on demand); VP_NONFAR_CAP sets it, for tests of the regrowth
-    nonfar_cap = (unsigned int)std::min<long long>((long long)nt * 160 + 1024, 1ll << 31);
+    // The non-far list's first budget, 384 pairs per test triangle (48 bytes each over the five lists): the abused
+    // ellipsoid needs 244, so its first assembly runs no second, enumerate-only far pass; it grows on demand (the
+    // swept wing's long panels need more).  VP_NONFAR_CAP sets it, for tests of the regrowth.
+    nonfar_cap = (unsigned int)std::min<long long>((long long)nt * 384 + 1024, 1ll << 31);
     if (const char *cap = std::getenv("VP_NONFAR_CAP")) nonfar_cap = (unsigned int)std::max(1ll, std::atoll(cap));
     leaf_cap = (unsigned int)std::min<long long>((long long)nt * 24 + 1024, 1ll << 31);
     group_cap = (unsigned int)std::min<long long>((long long)nt * 24 + 1024, 1ll << 31);
