@@ -101,3 +101,57 @@ def test_adjoint_mixed_matches_fp64(case):
         adj.close()
     assert abs(out["mixed"][0] - out["fp64"][0]) < 1e-6
     assert np.abs(out["mixed"][1] - out["fp64"][1]).max() < 1e-5 * np.abs(out["fp64"][1]).max()
+
+
+def test_rrules(case):
+    """The rrule forms: the loads' pullback is linear in its cotangents and equals the gradients for (1, 0) and
+    (0, 1); the doublets' pullback against central differences of the oracle's doublets (wᵀmu along random
+    directions, the wide-corner split off, the DOFMap's slot areas rebuilt at each perturbed geometry).  A random wᵀmu
+    sees a uniform doublet, so its pullback carries the zero-mean border rows' derivative (-0.18 of 8.1 here).  The
+    differences of two touching entries are not smooth at this mesh (their central differences change by 8e-4
+    between h = 1e-6 and 1e-7), which bounds the agreement at a few 1e-6 of the scale."""
+    print("rrules: loads and doublets", flush=True)
+    g, dm, w = case
+    (CL, CDi), back = AD.wing_loads_rrule(g, dm, w, ALPHA, S)
+    ref = back.adjoint.gradients(("CL", "CDi"))   # the same forward solution, both objectives in one pass
+    assert CL == ref["CL"][0] and CDi == ref["CDi"][0]
+    for k, name in enumerate(("CL", "CDi")):
+        xb, ab = back((1.0, 0.0) if k == 0 else (0.0, 1.0))
+        scale = np.abs(ref[name][1]).max()
+        assert np.abs(xb - ref[name][1]).max() < 1e-12 * scale, name
+        assert abs(ab - ref[name][2]) < 1e-12 * max(abs(ref[name][2]), 1e-12), name
+    xb, ab = back((0.7, -2.5))
+    lin = 0.7 * ref["CL"][1] - 2.5 * ref["CDi"][1]
+    assert np.abs(xb - lin).max() < 1e-12 * np.abs(lin).max()
+    assert abs(ab - (0.7 * ref["CL"][2] - 2.5 * ref["CDi"][2])) < 1e-12 * (abs(ref["CL"][2]) + abs(ref["CDi"][2]))
+    back.close()
+
+    opts = api.AssemblyOptions(corner_angle=180.0)
+    mu, back = AD.doublets_rrule(g, dm, w, ALPHA, opts=opts)
+    rng = np.random.default_rng(77)
+    wts = rng.standard_normal(len(mu))
+    xb, ab = back(wts)
+    back.close()
+    node_of_wake = dm.wake_te.node[w.te]
+
+    def oracle_mu(x, alpha):
+        gp = dataclasses.replace(g, x=x)
+        dmp = dofmap.build_dofmap(gp)   # the slot areas of the zero-mean border rows move with the nodes
+        wp = dataclasses.replace(w, x=x[node_of_wake] + (w.x - g.x[node_of_wake]))
+        V, _ = sweep.wing_freestream(alpha)
+        el = O.elements(x, g.tris)
+        A, b, _, _ = O.assemble(gp, dmp, O.source_strengths(el, V), V, O.opts(corner_angle=180.0), wake=wp)
+        return O.solve_galerkin(dmp, A, b, dofmap.wake_rows(dmp, w.te))["mu"]
+
+    assert np.abs(mu - oracle_mu(g.x, ALPHA)).max() < 1e-10 * np.abs(mu).max()
+    h = 1e-6
+    for i in range(2):
+        v = rng.standard_normal(g.x.shape)
+        fd = wts @ (oracle_mu(g.x + h * v, ALPHA) - oracle_mu(g.x - h * v, ALPHA)) / (2 * h)
+        ad = float(np.sum(xb * v))
+        scale = float(np.sum(np.abs(xb) * np.abs(v)))
+        print(f"  direction {i}: fd {fd:.10e} adjoint {ad:.10e} rel {abs(fd - ad) / scale:.2e}", flush=True)
+        assert abs(fd - ad) < 1e-5 * scale   # measured 3e-6: a few touching entries are not smooth here
+    ha = 1e-4
+    fd = wts @ (oracle_mu(g.x, ALPHA + ha) - oracle_mu(g.x, ALPHA - ha)) / (2 * ha)
+    assert abs(fd - ab) < 1e-6 * abs(fd) + 1e-9, (fd, ab)

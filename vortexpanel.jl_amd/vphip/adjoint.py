@@ -229,12 +229,35 @@ class WingAdjoint:
         CD, *_ = cdi_explicit(self.dm, self.g.x, self.mu_w, self.alpha, self.S)
         return dict(CL=CL, CDi=CD)
 
-    def _adjoint_weights(self, dJdq):
+    def _adjoint_weights(self, dJdq, border=False):
+        """lam = C psi_q for M^T psi = [dJ/dq; 0]; with border, also psi's border part (one value per body)"""
         F = self.F
         R = np.zeros(F.n)
         R[:F.nq] = dJdq
         X = F.solve_bordered_t(R)
-        return F.C @ X[:F.nq]
+        lam = F.C @ X[:F.nq]
+        return (lam, X[F.nq:F.nq + F.nb].copy()) if border else lam
+
+    def _border_grad(self, psi_b):
+        """-psi_bᵀ d/dx (c~ q) at fixed q, (nn, 3).  The border rows c~_k = c_k scale/|c_k| (solve.jl:30-35, with
+        c_k = Cᵀ (slot_area on body k) and scale = se_k |e_k|) depend on the nodes through the slot areas, a third of
+        each corner's triangle area (build.jl:199-212); c~_k q = 0 at the solution, so only d(c_k . q)/dx is left, and
+        d area/d a = n x (c - b) / 2 for the triangle (a, b, c).  (The border column's se_k moves with A's diagonal,
+        but it multiplies the border unknown, zero to rounding for a closed body's consistent right side.)"""
+        g, dm = self.g, self.dm
+        gx = np.zeros_like(g.x)
+        X = g.x[g.tris]
+        N = _cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0])
+        n = N / np.sqrt(_dot(N, N))[:, None]
+        dA = 0.5 * np.stack([_cross(n, X[:, 2] - X[:, 1]), _cross(n, X[:, 0] - X[:, 2]), _cross(n, X[:, 1] - X[:, 0])],
+                            axis=1)
+        for k, psi in enumerate(np.atleast_1d(psi_b)):
+            on = dm.slot_body == k
+            c = dm.C.T @ (dm.slot_area * on)
+            scale = self.F.se[k] * np.sqrt(len(dm.body_q[k]))
+            mt = (self.mu * on)[dm.corner_slot].sum(1) / 3
+            np.add.at(gx, g.tris, (-psi * scale / np.linalg.norm(c) * mt)[:, None, None] * dA)
+        return gx
 
     def _dV(self):
         a = np.radians(self.alpha)
@@ -297,6 +320,59 @@ class WingAdjoint:
         """dJ/dx (nn, 3) and dJ/dalpha (per degree) of J = CL or CDi"""
         return self.gradients((objective,))[objective]
 
+    def pullback_loads(self, dCL, dCDi):
+        """The vector-Jacobian product of (CL, CDi) with cotangents (dCL, dCDi): (x̄ (nn, 3), ᾱ per degree) =
+        dCL grad CL + dCDi grad CDi, from one adjoint solve and one backward pass (the adjoint weights are linear in
+        the objective's)"""
+        eCL, eCD = self._explicit("CL"), self._explicit("CDi")
+        lam = self._adjoint_weights(dCL * eCL[1] + dCDi * eCD[1])
+        gb, lbd = self._backward(lam[None])
+        return dCL * eCL[2] + dCDi * eCD[2] + gb[0], dCL * eCL[3] + dCDi * eCD[3] + lbd[0] * np.pi / 180
+
+    def pullback_mu(self, mu_bar):
+        """The vector-Jacobian product of the slot doublets mu(x, alpha) with a cotangent mu_bar (nslot): (x̄ (nn, 3),
+        ᾱ per degree).  mu = C q, so dJ/dq = Cᵀ mu_bar; the explicit dependence of an objective on x is its own.  A
+        cotangent that is not blind to a uniform doublet also moves the zero-mean border rows (_border_grad)"""
+        lam, psi_b = self._adjoint_weights(self.F.C.T @ np.asarray(mu_bar, dtype=np.float64), border=True)
+        gb, lbd = self._backward(lam[None])
+        return gb[0] + self._border_grad(psi_b), lbd[0] * np.pi / 180
+
     def close(self):
         self.F.close()
         self.plan.close()
+
+
+# ------------------------------------------------------------------------------------------ rules (rrule form) -----
+
+def wing_loads_rrule(g, dm, wake, alpha_deg, area_ref, precision="fp64", opts: AssemblyOptions | None = None,
+                     ctx: Context | None = None):
+    """The reverse rule of (CL, CDi) = wing_loads(x, alpha) in ChainRulesCore's rrule form (Julia's analogue of
+    torch.autograd.Function): ((CL, CDi), pullback) with pullback((dCL, dCDi)) -> (x̄ (nn, 3), ᾱ per degree).  The
+    forward solve happens here; each pullback call is one adjoint solve and one backward pass.  close() on the
+    pullback releases the device state."""
+    adj = WingAdjoint(g, dm, wake, alpha_deg, area_ref, precision=precision, opts=opts, ctx=ctx)
+    loads = adj.loads()
+
+    def pullback(ybar):
+        dCL, dCDi = ybar
+        return adj.pullback_loads(float(dCL), float(dCDi))
+
+    pullback.close = adj.close
+    pullback.adjoint = adj
+    return (loads["CL"], loads["CDi"]), pullback
+
+
+def doublets_rrule(g, dm, wake, alpha_deg, precision="fp64", opts: AssemblyOptions | None = None,
+                   ctx: Context | None = None):
+    """The reverse rule of the slot doublets mu = galerkin_doublets(x, alpha) (assemble + solve_galerkin): (mu,
+    pullback) with pullback(mu_bar) -> (x̄ (nn, 3), ᾱ per degree).  An objective written on mu and x then needs only
+    its own explicit derivatives on top, which the caller's differentiation supplies.  close() on the pullback
+    releases the device state."""
+    adj = WingAdjoint(g, dm, wake, alpha_deg, 1.0, precision=precision, opts=opts, ctx=ctx)
+
+    def pullback(mu_bar):
+        return adj.pullback_mu(mu_bar)
+
+    pullback.close = adj.close
+    pullback.adjoint = adj
+    return adj.mu.copy(), pullback
