@@ -167,3 +167,28 @@ def test_repeated_solves_replay(mode):
         assert F.last_resid < 1e-14, (k, F.last_resid)
         np.linalg.norm(M @ ref - R)   # host work between the device calls
     F.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_two_bodies(mode):
+    """Two spheres of uneven size and resolution in one geometry: one set of work lists over both bodies' pairs (tier
+    counts and entries against the oracle), and the bordered solve with one zero-mean row per body (mu and both border
+    multipliers against the oracle's LAPACK solve)."""
+    print(f"two bodies mode={mode}", flush=True)
+    g = mesh.combine([mesh.icosphere(3), mesh.icosphere(1, r=0.4, center=(2.2, 0.3, -0.1))])
+    dm = dofmap.build_dofmap(g)
+    els = api.build_elements(g)
+    sig = api.source_strengths(els, VG)
+    A, b, stats = api.assemble(g, dm, els, sig, VG)
+    Ao, bo, po, _ = O.assemble(g, dm, sig, VG)
+    assert stats["pairs"] == po
+    assert np.abs(A - Ao).max() < 1e-12 * np.abs(Ao).max()
+    assert np.abs(b - bo).max() < 1e-12 * np.abs(bo).max()
+    sol = solve.solve_galerkin(dm, A, b, mode=mode)
+    ref = O.solve_galerkin(dm, Ao, bo)
+    assert np.abs(sol["mu"] - ref["mu"]).max() < 1e-10 * np.abs(ref["mu"]).max()
+    assert sol["lam"].shape == (2,) and np.abs(sol["lam"] - ref["lam"]).max() < 1e-12
+    sh = solve.solve_galerkin(dm, A, b + 1e-3, mode=mode, cond=False)   # the multipliers take up the inconsistency
+    rsh = O.solve_galerkin(dm, Ao, bo + 1e-3)
+    assert np.abs(sh["lam"] - rsh["lam"]).max() < 1e-12, (sh["lam"], rsh["lam"])
+    assert np.abs(sh["lam"] - 1e-3).max() < 1e-5, sh["lam"]

@@ -110,6 +110,30 @@ def icosphere(level: int, r: float = 1.0, center=(0.0, 0.0, 0.0)) -> Surface:
     return _closed_surface(X, np.array(tris))
 
 
+def combine(parts: list) -> Surface:
+    """Several surfaces as one geometry of as many bodies (SurfaceGeometry with tri_body, src/geometry/types.jl:174-199):
+    nodes and triangles concatenated in order, body k's triangles labelled k.  Chains, touch pairs and caps are offset
+    by the nodes and triangles before them."""
+    xs, tris, body, side, chains, touch, closed, caps = [], [], [], [], [], [], [], []
+    n0 = t0 = 0
+    for k, s in enumerate(parts):
+        if len(s.bodies_closed) != 1:
+            raise ValueError("combine takes single-body surfaces")
+        xs.append(s.x)
+        tris.append(s.tris + n0)
+        body.append(np.full(s.nt, k, np.int32))
+        side.append(s.corner_side)
+        for c in s.chains:
+            chains.append(KuttaChain(c.nodes + n0, c.tri_upper + t0, c.tri_lower + t0, c.closed))
+        touch.append(s.touch_pairs + t0)
+        closed += list(s.bodies_closed)
+        caps.append(s.tri_cap if s.tri_cap is not None else np.zeros(s.nt, bool))
+        n0 += s.nn
+        t0 += s.nt
+    return Surface(np.vstack(xs), np.vstack(tris).astype(np.int32), np.concatenate(body), np.vstack(side), chains,
+                   np.vstack(touch).astype(np.int32), closed, np.concatenate(caps))
+
+
 def uv_ellipsoid(nth: int, nph: int, axes=(1.0, 1.0, 1.0)) -> Surface:
     """Welded latitude-longitude sphere (src/meshtools/bodies.jl:82-96) scaled by `axes` (:103-108).
 
