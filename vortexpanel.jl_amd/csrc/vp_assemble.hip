@@ -1718,8 +1718,12 @@ void Plan::tune_collect()
         if (!tune_timed[k]) continue;
         tune_timed[k] = false;
         float t = 0;
-        HIPCHECK(hipEventSynchronize(tev[k][1]));
-        HIPCHECK(hipEventElapsedTime(&t, tev[k][0], tev[k][1]));
+        // an assembly that stopped on an error may not have recorded the pair: no measurement then
+        if (hipEventSynchronize(tev[k][1]) != hipSuccess || hipEventElapsedTime(&t, tev[k][0], tev[k][1]) != hipSuccess) {
+            (void)hipGetLastError();
+            tune_release();
+            continue;
+        }
         std::lock_guard<std::mutex> lock(g_asm_mutex);
         if (g_asm.choice[k] >= 0 || g_asm.plan[k] != serial) continue;
         const int v = g_asm.next[k];
