@@ -214,7 +214,7 @@ void vp_wake_field_destroy(vp_wake_field *h);
 
 /* Test entry points (host arrays, row-major): C = op(A) B or C -= op(A) B with the library's MFMA GEMM (prec 0 FP64,
  * 1 FP32; B K x N, C M x N, all doubles on the host).  flags: bit 0 subtract; bit 1 transposed A (A stored K x M,
- * op(A) = Aᵀ), else A is M x K; bits 4-7 the tile shape's index among the autotuner's five candidates (0 is the
+ * op(A) = Aᵀ), else A is M x K; bits 4-7 the tile shape's index among the autotuner's eight candidates (0 is the
  * default).  And x = M^-1 r for a dense n x n matrix through the randomized LU (mode 0 FP64, 1 FP32 factors) with
  * FP64 refinement. */
 int vp_gemm(vp_ctx *ctx, int prec, int flags, int M, int N, int K, const double *A, const double *B, double *C);

@@ -314,7 +314,7 @@ int vp_gemm(vp_ctx *ctx, int prec, int flags, int M, int N, int K, const double 
 {
     if (!ctx) return VP_E_ARGUMENT;
     if (!A || !B || !C || (prec != 0 && prec != 1) || M <= 0 || N <= 0 || K <= 0 || (flags & ~0xF3) ||
-        ((flags >> 4) & 15) >= 5)
+        ((flags >> 4) & 15) >= 8)
         return fail(&ctx->c, VP_E_ARGUMENT, "bad argument");
     return guard(&ctx->c, [&] { vp::gemm_host(ctx->c.stream, prec, flags, M, N, K, A, B, C); });
 }

@@ -164,9 +164,11 @@ __global__ __launch_bounds__(256) void gemm(int M, int N, int K, const T *A, lon
 struct TileShape {
     int bm, bn, bk;
 };
-constexpr int kNumTiles = 5;
-constexpr TileShape kTiles64[kNumTiles] = {{64, 64, 16}, {128, 64, 16}, {64, 128, 16}, {128, 128, 16}, {128, 128, 8}};
-constexpr TileShape kTiles32[kNumTiles] = {{128, 128, 16}, {64, 64, 16}, {128, 64, 16}, {128, 128, 32}, {256, 128, 16}};
+constexpr int kNumTiles = 8;
+constexpr TileShape kTiles64[kNumTiles] = {{64, 64, 16}, {128, 64, 16}, {64, 128, 16}, {128, 128, 16},
+                                           {128, 128, 8}, {64, 64, 32}, {64, 64, 8},   {128, 64, 8}};
+constexpr TileShape kTiles32[kNumTiles] = {{128, 128, 16}, {64, 64, 16},  {128, 64, 16}, {128, 128, 32},
+                                           {256, 128, 16}, {128, 64, 32}, {64, 128, 16}, {128, 64, 8}};
 
 template <typename T, int BM, int BN, int BK, bool SUB, bool TA>
 void launch_tile(hipStream_t s, int M, int N, int K, const T *A, long long lda, const T *B, long long ldb, T *C,
@@ -188,6 +190,9 @@ void launch_gemm_cfg(int cfg, hipStream_t s, int M, int N, int K, const T *A, lo
         case 2: return launch_tile<T, 64, 128, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
         case 3: return launch_tile<T, 128, 128, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
         case 4: return launch_tile<T, 128, 128, 8, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        case 5: return launch_tile<T, 64, 64, 32, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        case 6: return launch_tile<T, 64, 64, 8, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        case 7: return launch_tile<T, 128, 64, 8, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
         default: return launch_tile<T, 64, 64, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
         }
     } else {
@@ -196,6 +201,9 @@ void launch_gemm_cfg(int cfg, hipStream_t s, int M, int N, int K, const T *A, lo
         case 2: return launch_tile<T, 128, 64, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
         case 3: return launch_tile<T, 128, 128, 32, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
         case 4: return launch_tile<T, 256, 128, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        case 5: return launch_tile<T, 128, 64, 32, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        case 6: return launch_tile<T, 64, 128, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        case 7: return launch_tile<T, 128, 64, 8, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
         default: return launch_tile<T, 128, 128, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
         }
     }
@@ -207,7 +215,7 @@ void launch_gemm_cfg(int cfg, hipStream_t s, int M, int N, int K, const T *A, lo
 // VP_GEMM_TILE=<index> forces a candidate.
 struct GemmTuning {
     int cfg = -1;
-    double ms[kNumTiles] = {0, 0, 0, 0, 0};
+    double ms[kNumTiles] = {};
     int size = 0;
 };
 static GemmTuning g_tune[2];  // [0] FP64, [1] FP32
