@@ -140,3 +140,42 @@ def test_impulsive_start(wing):
     fh = free.run(20)
     assert all(r["umax"] < 3 * np.linalg.norm(V) for r in fh)
     assert abs((fh[-1]["F_kj"] @ LIFT / S) / kj[-1] - 1) < 0.03
+
+
+def test_paraview_series(wing, tmp_path):
+    """run(n, series=...) writes a ParaView frame every `every` steps and the last (src/freewake/step.jl:203-216,
+    src/freewake/export.jl): the free rows, particles, body and trailing edge panel grids with their arrays, and .pvd
+    collections listing each frame at its step's time."""
+    import xml.etree.ElementTree as ET
+    from vphip import export
+    g, dm, S = wing
+    dt = wk.te_panel_size(g)
+    sim = OracleFreeWakeSim(g, dm, V, FW.FreeWakeOptions(dt=dt, max_rows=2))
+    series = export.FreeWakeSeries(str(tmp_path), every=3)
+    hist = sim.run(7, series=series)
+    frames = sorted(p.name for p in tmp_path.glob("frame_*_body.vtu"))
+    assert frames == ["frame_00000_body.vtu", "frame_00003_body.vtu", "frame_00006_body.vtu"]
+    assert sorted(p.name for p in tmp_path.glob("*.pvd")) == ["body.pvd", "free.pvd", "particles.pvd", "te_panel.pvd"]
+    times = [float(d.get("timestep")) for d in ET.parse(tmp_path / "body.pvd").getroot().iter("DataSet")]
+    assert np.allclose(times, [hist[0]["t"], hist[3]["t"], hist[6]["t"]])
+
+    def arrays(path):
+        piece = ET.parse(path).getroot().find("UnstructuredGrid/Piece")
+        out = {a.get("Name"): np.array(a.text.split(), float) for a in piece.iter("DataArray")}
+        return int(piece.get("NumberOfPoints")), int(piece.get("NumberOfCells")), out
+
+    npts, ncell, a = arrays(tmp_path / "frame_00006_body.vtu")
+    assert (npts, ncell) == (g.nn, g.nt) and len(a["Cp"]) == g.nt and len(a["Cp_nodes"]) == g.nn
+    assert np.allclose(a["Cp"], sim.cp) and np.allclose(a["mu"], sim.mu[:g.nn])
+    assert np.all(a["types"] == 5) and np.array_equal(a["connectivity"].astype(int), g.tris.ravel())
+    # the frame's free rows are the state before its step moved them: rows 0..nr-1 of nte nodes
+    npts, ncell, a = arrays(tmp_path / "frame_00006_free.vtu")
+    nte = sim.topo.nte
+    assert npts % nte == 0 and ncell == 2 * len(sim.topo.segments) * (npts // nte - 1)
+    assert set(a["row"].astype(int)) == set(range(npts // nte))
+    npts, ncell, a = arrays(tmp_path / "frame_00006_particles.vtu")
+    assert npts == ncell and len(a["alpha"]) == 3 * npts and np.all(a["types"] == 1)
+    assert np.allclose(a["strength"], np.linalg.norm(a["alpha"].reshape(-1, 3), axis=1))
+    # the area-weighted nodal average reproduces a constant
+    els = api.build_elements(g)
+    assert np.allclose(export.nodal_average(g, els, np.full(g.nt, 2.5)), 2.5)

@@ -11,6 +11,7 @@ FP32 kernels with FP64 sums (the circulation bookkeeping stays FP64).
 from __future__ import annotations
 
 import math
+import os
 import time
 import warnings
 from dataclasses import dataclass, field
@@ -601,12 +602,23 @@ class FreeWakeSim:
         limit = max(float(x @ d) for x in self.edge) + self.opts.delete_beyond * self.te.span
         self.state.keep_particles(self.state.particles.x @ d <= limit)
 
-    def run(self, nsteps, progress=None):
-        """run!, step.jl:203-216 (without the VTK series)"""
-        for _ in range(nsteps):
+    def run(self, nsteps, progress=None, series=None):
+        """run!, step.jl:203-216: nsteps steps, progress(sim, record) after each.  With an export.FreeWakeSeries, every
+        series.every steps and the last are written as a ParaView frame: the wake as the step's solve saw it (written
+        before the step moves it), the body's pressure and the trailing edge panel from that solve, at the step's
+        time; the .pvd collections at the end"""
+        from . import export
+        for k in range(1, nsteps + 1):
+            save = series is not None and (self.n % series.every == 0 or k == nsteps)
+            stem = os.path.join(series.dir, f"frame_{self.n:05d}") if save else ""
+            wake_files = export.write_free_wake_vtk(self, stem) if save else []
             rec = self.step()
+            if save:
+                export.add_frame(series, rec["t"], export.write_body_vtk(self, stem) + wake_files)
             if progress is not None:
                 progress(self, rec)
+        if series is not None:
+            export.write_collections(series)
         return self.history
 
     def close(self):
