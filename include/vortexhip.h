@@ -220,8 +220,9 @@ void vp_wake_field_destroy(vp_wake_field *h);
 int vp_gemm(vp_ctx *ctx, int prec, int flags, int M, int N, int K, const double *A, const double *B, double *C);
 int vp_dense_solve(vp_ctx *ctx, int n, const double *M, int mode, const double *r, double *x, int *iters);
 /* Test entry point: the device's FP64 natural log (fn 0: out = log a), two-argument arctangent (fn 1:
- * out = atan2(a, b)) and reciprocal square root (fn 2: out = 1/sqrt(a), a normal and positive) as the kernels
- * evaluate them, elementwise over n host values. */
+ * out = atan2(a, b)), reciprocal square root (fn 2: out = 1/sqrt(a), a normal and positive) and square root of a
+ * squared distance (fn 3: out = sqrt(a), a zero or normal and positive) as the kernels evaluate them, elementwise
+ * over n host values. */
 int vp_math_probe(vp_ctx *ctx, int fn, int n, const double *a, const double *b, double *out);
 
 #ifdef __cplusplus

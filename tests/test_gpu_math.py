@@ -1,6 +1,7 @@
 """The device's own FP64 log and atan2 (the closed-form kernel's transcendentals) against the host C library's, through
 the C ABI's probe: within 2 ulp over the whole range, the special values and signed zeros as the library gives them;
-and its 1/sqrt of the multipole's distances (normal positive arguments) within 2 ulp."""
+and its 1/sqrt of the multipole's distances (normal positive arguments) within 2 ulp, and the square root of the closed
+form's squared distances within 3 ulp."""
 import ctypes as C
 
 import numpy as np
@@ -68,3 +69,14 @@ def test_rsqrt():
     x = np.concatenate([10.0 ** rng.uniform(-300, 300, 20000), rng.uniform(0.25, 4.0, 20000), [1.0, 4.0, 2.0, 0.5]])
     out, ref = _probe(2, x), 1.0 / np.sqrt(x)
     assert np.max(_ulps(out, ref)) <= 2.0
+
+
+def test_distance_sqrt():
+    """The closed form's vertex distances: sqrt of zero or normal positive squared distances within 3 ulp"""
+    print("device distance sqrt", flush=True)
+    rng = np.random.default_rng(6)
+    x = np.concatenate([10.0 ** rng.uniform(-300, 300, 20000), rng.uniform(0.25, 4.0, 20000), [1.0, 4.0, 2.0, 0.5],
+                        [2.2250738585072014e-308, 1.7e308]])
+    out, ref = _probe(3, x), np.sqrt(x)
+    assert np.max(_ulps(out, ref)) <= 3.0
+    assert _probe(3, np.array([0.0]))[0] == 0.0

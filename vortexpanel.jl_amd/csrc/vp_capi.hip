@@ -322,7 +322,7 @@ int vp_gemm(vp_ctx *ctx, int prec, int flags, int M, int N, int K, const double 
 int vp_math_probe(vp_ctx *ctx, int fn, int n, const double *a, const double *b, double *out)
 {
     if (!ctx) return VP_E_ARGUMENT;
-    if (!a || !out || n <= 0 || fn < 0 || fn > 2 || (fn == 1 && !b)) return fail(&ctx->c, VP_E_ARGUMENT, "bad argument");
+    if (!a || !out || n <= 0 || fn < 0 || fn > 3 || (fn == 1 && !b)) return fail(&ctx->c, VP_E_ARGUMENT, "bad argument");
     return guard(&ctx->c, [&] { vp::math_probe_host(&ctx->c, fn, n, a, b, out); });
 }
 

@@ -160,6 +160,11 @@ __device__ __forceinline__ double vrsqrt(double x)
 __device__ __forceinline__ float vrsqrt(float x) { return 1.0f / __builtin_sqrtf(x); }
 
 __device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+// The square root of a squared distance (x zero or normal positive) as x / sqrt(x) through vrsqrt: within 3 ulp of
+// the correctly rounded root (tested on the device), about half the instructions of the rounded one; the closed
+// form's vertex distances
+__device__ __forceinline__ double psqrt(double x) { return x > 0.0 ? x * vrsqrt(x) : 0.0; }
+__device__ __forceinline__ float psqrt(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ double dlog(double x) { return vlog(x); }
 __device__ __forceinline__ float dlog(float x) { return __logf(x); }
@@ -242,7 +247,7 @@ __device__ __forceinline__ V3<T> sv_exact_abs(const SrcT<T> &s, V3<T> ra, V3<T> 
                                               const Elem *ein, V3<double> xabs)
 {
     // the three vertex distances serve the three edge logs and the solid angle
-    const T la = dsqrt(dot(ra, ra)), lb = dsqrt(dot(rb, rb)), lc = dsqrt(dot(rc, rc));
+    const T la = psqrt(dot(ra, ra)), lb = psqrt(dot(rb, rb)), lc = psqrt(dot(rc, rc));
     V3<T> wt = edge_log(ra, rb, la, lb, s.len[0]) * s.nu[0] + edge_log(rb, rc, lb, lc, s.len[1]) * s.nu[1] +
                edge_log(rc, ra, lc, la, s.len[2]) * s.nu[2];
     T Om;

@@ -584,7 +584,7 @@ void WakeField::eval(const double *sigma, const double *gamma, int np, const dou
 __global__ void math_probe(int fn, int n, const double *__restrict__ a, const double *__restrict__ b, double *out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = fn == 0 ? dlog(a[i]) : fn == 1 ? datan2(a[i], b[i]) : vrsqrt(a[i]);
+    if (i < n) out[i] = fn == 0 ? dlog(a[i]) : fn == 1 ? datan2(a[i], b[i]) : fn == 2 ? vrsqrt(a[i]) : psqrt(a[i]);
 }
 
 void math_probe_host(Ctx *c, int fn, int n, const double *a, const double *b, double *out)
