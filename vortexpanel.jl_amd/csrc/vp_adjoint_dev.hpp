@@ -159,7 +159,7 @@ __device__ __forceinline__ void exact_point_grad(const V3<double> *XS, const Ele
 {
     const V3<double> n = ld3(S.n);
     const V3<double> r[3] = {x - XS[0], x - XS[1], x - XS[2]};
-    const double l[3] = {dsqrt(dot(r[0], r[0])), dsqrt(dot(r[1], r[1])), dsqrt(dot(r[2], r[2]))};
+    const double l[3] = {psqrt(dot(r[0], r[0])), psqrt(dot(r[1], r[1])), psqrt(dot(r[2], r[2]))};
     const V3<double> nQ = cross(n, Q);
     V3<double> cn = mk3(0.0, 0.0, 0.0), wt = cn;
     g[0] = g[1] = g[2] = cn;
