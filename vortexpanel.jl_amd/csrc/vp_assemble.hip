@@ -256,7 +256,8 @@ __device__ __forceinline__ void far_block(const FarSrc<T> &f, const Elem *__rest
                                           double (*blk)[3], double *rb, unsigned int &nexact)
 {
     const V3<double> n = ld3(F.n);
-#pragma unroll 1
+    // unrolled: the three points are independent (C2 far kernel 17.1 -> 16.3 ms, no spills at two waves per SIMD)
+#pragma unroll
     for (int q = 0; q < 3; q++) {
         V3<double> w;
         if constexpr (EXACT) {
