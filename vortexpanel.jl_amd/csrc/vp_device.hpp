@@ -157,7 +157,9 @@ __device__ __forceinline__ double vrsqrt(double x)
     r = fma(-x * y, y, 1.0);
     return fma(0.5 * y, r, y);
 }
-__device__ __forceinline__ float vrsqrt(float x) { return 1.0f / __builtin_sqrtf(x); }
+// FP32 (the mixed path's multipole and blobs): the hardware estimate itself, about 1 ulp (the FP32 kernels' stated
+// errors, 2e-5 and 5e-5 of the largest velocity, are far above it)
+__device__ __forceinline__ float vrsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 
 __device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
 // The square root of a squared distance (x zero or normal positive) as x / sqrt(x) through vrsqrt: within 3 ulp of
@@ -346,11 +348,13 @@ template <typename T> __device__ __forceinline__ void seg_vel(V3<T> ra, V3<T> rb
 }
 
 // blob_velocity, src/kernels/blob.jl:21-25, r = x - xp
+// (1/(4 pi)) (d2)^(-3/2) alpha x r with one reciprocal square root (the refined one in FP64, the hardware's in FP32)
+// instead of a square root and a division
 template <typename T> __device__ __forceinline__ V3<T> blob_vel(V3<T> r, V3<T> al, T d2core)
 {
-    T d2 = dot(r, r) + d2core;
-    T s = dsqrt(d2);
-    return (T(1) / (T(4 * kPi) * d2 * s)) * cross(al, r);
+    const T d2 = dot(r, r) + d2core;
+    const T i = vrsqrt(d2);
+    return (T(1.0 / (4 * kPi)) * (i * i * i)) * cross(al, r);
 }
 
 // ---------------------------------------------------------------- classification (contraction off, FP64) ----------
