@@ -38,6 +38,27 @@ def wing_loads(els, dm, mu, q, vinf, e_lift, area_ref):
                 CY=float(Fp[1]) / area_ref, mu_w=mu_w)
 
 
+def wing_loads_batch(els, dm, MU, Q, alphas, area_ref):
+    """wing_loads for many angles at once (MU nslot x na, Q nq x na): the per-element pressure route as one batched
+    contraction over the angles, the trailing-edge loads per angle"""
+    a = np.radians(np.asarray(alphas, dtype=np.float64))
+    V = np.stack([np.cos(a), np.zeros_like(a), np.sin(a)], axis=1)          # (na, 3)
+    eL = np.stack([-np.sin(a), np.zeros_like(a), np.cos(a)], axis=1)
+    n = els.n
+    M = np.asarray(MU).T[:, dm.corner_slot]                                  # (na, nt, 3) corner doublets
+    u = (V[:, None, :] - np.einsum("aj,tj->at", V, n)[:, :, None] * n[None]  # gradient_velocity, Post.jl:72-73
+         + np.einsum("atk,tkj->atj", M, els.gradl))
+    cp = 1 - np.einsum("atj,atj->at", u, u) / np.einsum("aj,aj->a", V, V)[:, None]  # pressure_coefficient, :82-84
+    F = -np.einsum("at,tj->aj", cp, els.area[:, None] * n)                 # pressure_force per angle, forces.jl:11-19
+    MW = dm.K @ np.asarray(Q)
+    CL_KJ, CDi_T = [], []
+    for k in range(len(a)):
+        CL_KJ.append(float(post.kutta_joukowski_force(dm.wake_te, MW[:, k], V[k]) @ eL[k]) / area_ref)
+        CDi_T.append(post.trefftz_drag(dm.wake_te, MW[:, k], V[k]) / area_ref)
+    return dict(CL=np.einsum("aj,aj->a", F, eL) / area_ref, CL_KJ=np.array(CL_KJ), CDi_T=np.array(CDi_T),
+                CDi_p=np.einsum("aj,aj->a", F, V) / area_ref, CY=F[:, 1] / area_ref)
+
+
 @dataclass
 class AlphaSweep:
     """One wing, one body-fixed wake, one factorization; `solve(alphas)` gives the loads of every angle."""
@@ -78,14 +99,12 @@ class AlphaSweep:
         t0 = time.perf_counter()
         out = solve_factored(self.F, self.rhs(alphas))
         t1 = time.perf_counter()
-        loads = []
-        for k, al in enumerate(np.atleast_1d(alphas)):
-            V, eL = wing_freestream(al)
-            loads.append(wing_loads(self.els, self.dm, out["mu"][:, k], out["q"][:, k], V, eL, self.area_ref))
+        al = np.atleast_1d(np.asarray(alphas, dtype=np.float64))
+        MU, Q = out["mu"].reshape(self.dm.nslot, -1), out["q"].reshape(self.dm.nq, -1)
+        ld = wing_loads_batch(self.els, self.dm, MU, Q, al, self.area_ref)
         self.timings.update(solve_s=t1 - t0, loads_s=time.perf_counter() - t1, refinement_iters=out["iters"])
-        return dict(alpha=np.atleast_1d(alphas), mu=out["mu"], q=out["q"], lam=out["lam"],
-                    CL=np.array([l["CL"] for l in loads]), CL_KJ=np.array([l["CL_KJ"] for l in loads]),
-                    CDi_T=np.array([l["CDi_T"] for l in loads]), CDi_p=np.array([l["CDi_p"] for l in loads]))
+        return dict(alpha=al, mu=out["mu"], q=out["q"], lam=out["lam"], CL=ld["CL"], CL_KJ=ld["CL_KJ"],
+                    CDi_T=ld["CDi_T"], CDi_p=ld["CDi_p"])
 
     def close(self):
         self.F.close()
