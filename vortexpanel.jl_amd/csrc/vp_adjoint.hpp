@@ -60,13 +60,13 @@ template <int NL> __device__ __forceinline__ void far_tile_zero(double (*GT)[VMA
 }
 
 // One far pair's three far-rule points (multipole only, or with the forward's kernel switch when EXACT) and its
-// gradients, per weight set, into the tile's LDS sums; bd gets the pair's lambdaᵀ b along dV∞ (dVn = dV∞.n_S).
+// gradients per weight set: gT with respect to the test triangle's vertices, gS to the source's; bd gets the pair's
+// lambdaᵀ b along dV∞ (dVn = dV∞.n_S).
 template <int NL, bool EXACT>
-__device__ __forceinline__ void far_pair_adj(const AsmParams &P, const RuleTab &rt, const TestFar &F, const Elem &T,
-                                             const V3<double> *XS, const Elem &S, bool body, const double *muS,
-                                             V3<double> Gam, double sigS, double dVn, const double (*la)[3],
-                                             const int *vl, const int *uloc, V3<double> Vinf, double (*GT)[VMAX][3],
-                                             double (*GU)[UMAX][3], double *bd)
+__device__ __forceinline__ void far_pair_grads(const AsmParams &P, const RuleTab &rt, const TestFar &F, const Elem &T,
+                                               const V3<double> *XS, const Elem &S, bool body, const double *muS,
+                                               V3<double> Gam, double sigS, double dVn, const double (*la)[3],
+                                               V3<double> Vinf, V3<double> (*gT)[3], V3<double> (*gS)[3], double *bd)
 {
     const V3<double> nT = ld3(F.n);
     const V3<double> Q = sigS * nT + cross(nT, Gam);
@@ -91,18 +91,33 @@ __device__ __forceinline__ void far_pair_adj(const AsmParams &P, const RuleTab &
     }
     const V3<double> XT[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
     for (int j = 0; j < NL; j++) {
-        V3<double> gT[3], gS[3];
-        finish_tri_pair(XT, nT, F.area, XS, muS, sigS, Gam, body, Vinf, Q, p[j], gT, gS);
+        finish_tri_pair(XT, nT, F.area, XS, muS, sigS, Gam, body, Vinf, Q, p[j], gT[j], gS[j]);
         bd[j] += dVn * F.area * dot(p[j].Sv, nT);
-        for (int k = 0; k < 3; k++) {
-            atomicAdd(&GT[j][vl[k]][0], gT[k].x);
-            atomicAdd(&GT[j][vl[k]][1], gT[k].y);
-            atomicAdd(&GT[j][vl[k]][2], gT[k].z);
-            atomicAdd(&GU[j][uloc[k]][0], gS[k].x);
-            atomicAdd(&GU[j][uloc[k]][1], gS[k].y);
-            atomicAdd(&GU[j][uloc[k]][2], gS[k].z);
-        }
     }
+}
+
+__device__ __forceinline__ void lds_add3(double *G, V3<double> v)
+{
+    atomicAdd(G, v.x);
+    atomicAdd(G + 1, v.y);
+    atomicAdd(G + 2, v.z);
+}
+
+// far_pair_grads with the gradients added to the tile's LDS sums (vl, uloc: the local test slots and source columns)
+template <int NL, bool EXACT>
+__device__ __forceinline__ void far_pair_adj(const AsmParams &P, const RuleTab &rt, const TestFar &F, const Elem &T,
+                                             const V3<double> *XS, const Elem &S, bool body, const double *muS,
+                                             V3<double> Gam, double sigS, double dVn, const double (*la)[3],
+                                             const int *vl, const int *uloc, V3<double> Vinf, double (*GT)[VMAX][3],
+                                             double (*GU)[UMAX][3], double *bd)
+{
+    V3<double> gT[NL][3], gS[NL][3];
+    far_pair_grads<NL, EXACT>(P, rt, F, T, XS, S, body, muS, Gam, sigS, dVn, la, Vinf, gT, gS, bd);
+    for (int j = 0; j < NL; j++)
+        for (int k = 0; k < 3; k++) {
+            lds_add3(GT[j][vl[k]], gT[j][k]);
+            lds_add3(GU[j][uloc[k]], gS[j][k]);
+        }
 }
 
 // The far tier (the forward's tiles and far test, far_pairs): lane = source element, wave = test triangle; the
@@ -149,6 +164,9 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
         sdiam = S.diam;
     }
     double bd[NL] = {};
+    V3<double> accS[NL][3];
+    for (int j = 0; j < NL; j++)
+        for (int k = 0; k < 3; k++) accS[j][k] = mk3(0.0, 0.0, 0.0);
     __syncthreads();
     for (int i = wave; i < TT_MAX; i += 4) {
         const int t = tt_elems[tt * TT_MAX + i];
@@ -173,13 +191,32 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
             b0 = __shfl(b0, 0);
             if (ex) exq[b0 + __popcll(em & ((1ull << lane) - 1ull))] = (unsigned short)(i * TS + lane);
         }
-        if (!multi) continue;
+        if (!__ballot(multi)) continue;
         const int vl[3] = {tt_loc[(tt * TT_MAX + i) * 3], tt_loc[(tt * TT_MAX + i) * 3 + 1], tt_loc[(tt * TT_MAX + i) * 3 + 2]};
-        double la[NL][3];
+        V3<double> gT[NL][3];
+        if (multi) {
+            double la[NL][3];
+            for (int j = 0; j < NL; j++)
+                for (int k = 0; k < 3; k++) la[j][k] = aa.lam[j * aa.lstride + tt_rows[tt * VMAX + vl[k]]];
+            V3<double> gS[NL][3];
+            far_pair_grads<NL, false>(P, rt, F, el[t], XS, el[s], s < P.nt, muS, Gam, sigS, dVn, la, Vinf, gT, gS, bd);
+            for (int j = 0; j < NL; j++)
+                for (int k = 0; k < 3; k++) accS[j][k] = accS[j][k] + gS[j][k];
+        } else {
+            for (int j = 0; j < NL; j++)
+                for (int k = 0; k < 3; k++) gT[j][k] = mk3(0.0, 0.0, 0.0);
+        }
+        // the wave's lanes share the test triangle: its gradients are summed over the wave and added once
         for (int j = 0; j < NL; j++)
-            for (int k = 0; k < 3; k++) la[j][k] = aa.lam[j * aa.lstride + tt_rows[tt * VMAX + vl[k]]];
-        far_pair_adj<NL, false>(P, rt, F, el[t], XS, el[s], s < P.nt, muS, Gam, sigS, dVn, la, vl, uloc, Vinf, GT, GU, bd);
+            for (int k = 0; k < 3; k++) {
+                const V3<double> v = mk3(wave_total(gT[j][k].x), wave_total(gT[j][k].y), wave_total(gT[j][k].z));
+                if (lane == 0) lds_add3(GT[j][vl[k]], v);
+            }
     }
+    // the lane's source gradients, summed over the tile's test triangles, into the tile's column sums
+    if (valid)
+        for (int j = 0; j < NL; j++)
+            for (int k = 0; k < 3; k++) lds_add3(GU[j][uloc[k]], accS[j][k]);
     __syncthreads();
     if (threadIdx.x == 0 && nexq) {
         exbase = atomicAdd(exact_count, nexq);

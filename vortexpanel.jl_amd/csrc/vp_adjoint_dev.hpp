@@ -276,6 +276,34 @@ __device__ __forceinline__ double wsum(double v)
     return v;
 }
 
+// x of the lane that the DPP control CTRL names within the lane's row of 16
+template <int CTRL> __device__ __forceinline__ double dpp_get(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double lane_value(double x, int l)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// The sum of v over the wave, in every lane, without LDS: DPP adds within each row of 16 lanes (lane pairs by
+// quad_perm [1,0,3,2], quads by [2,3,0,1], half rows by row_half_mirror, rows by row_mirror), then the four row sums
+// read from lanes 0, 16, 32 and 48.  Every lane of the wave must be active.
+__device__ __forceinline__ double wave_total(double v)
+{
+    v += dpp_get<0xB1>(v);
+    v += dpp_get<0x4E>(v);
+    v += dpp_get<0x141>(v);
+    v += dpp_get<0x140>(v);
+    return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+
 // What the evaluation kernels need in adjoint mode (null lam in the forward assembly).  The kernels take up to
 // ADJ_MAX sets of adjoint weights at once (a template count NL): the point derivatives of the kernel do not depend on
 // lambda, which only weights their sums, so several objectives share one pass.
