@@ -142,24 +142,26 @@ def assemble(g, dm, sigma, vinf, o: Opts | None = None, wake=None, nthreads=0, t
     return out + (tr,) if tiers else out
 
 
-def assemble_rows(g, dm, sigma, vinf, t_begin, t_end, o: Opts | None = None, wake=None, nthreads=0):
+def assemble_rows(g, dm, sigma, vinf, t_begin, t_end, o: Opts | None = None, wake=None, nthreads=0, full=False):
     """assemble restricted to the test triangles [t_begin, t_end) (the bounded sample of the CPU baseline);
-    returns the number of pair integrals done"""
+    returns the number of pair integrals done, or with full=True also the partial A (nslot x (nslot + nw)) and b,
+    whose rows are complete for every slot whose triangles all lie in the range (the geometry's touch pairs included)"""
     o = o or opts()
     nwx, wx, nwt, wt, nwf, wf = _wake_arrays(wake)
     ns = dm.nslot
     A = np.zeros((ns + nwx, ns))
     b = np.zeros(ns)
     pairs, points = np.zeros(6, np.int64), np.zeros(6, np.int64)
-    tp = np.zeros((1, 2), np.int32)
+    ntp = len(g.touch_pairs) if full else 0
+    tp = _i32(g.touch_pairs.reshape(-1, 2)) if ntp else np.zeros((1, 2), np.int32)
     rc = lib().vpo_assemble_rows(g.nn, _p(_f64(g.x)), g.nt, _p(_i32(g.tris), C.c_int),
-                                 _p(_i32(dm.corner_slot), C.c_int), ns, _p(_f64(sigma)), _p(_f64(vinf)), 0,
+                                 _p(_i32(dm.corner_slot), C.c_int), ns, _p(_f64(sigma)), _p(_f64(vinf)), ntp,
                                  _p(tp, C.c_int), nwx, _p(wx), nwt, _p(wt, C.c_int), nwf, _p(wf, C.c_int), C.byref(o),
                                  nthreads, _p(A), _p(b), _p(pairs, C.c_longlong), _p(points, C.c_longlong), None,
                                  int(t_begin), int(t_end))
     if rc:
         raise ValueError(f"vpo_assemble_rows failed ({rc})")
-    return int(pairs.sum())
+    return (int(pairs.sum()), A.T, b) if full else int(pairs.sum())
 
 
 def pair(g, sigma, t, s, o: Opts | None = None, wake=None, touch=0):
