@@ -53,3 +53,32 @@ def test_alpha_batch_loads(wing, precision, tl, td):
     for k in (0, 5):
         assert np.abs(Bn[:, k] - ref[k][2]).max() <= tb * np.abs(ref[k][2]).max()
     sw.close()
+
+
+def test_alpha_batch_full_size():
+    """The benchmark's batch at full size (rectangular NACA 0012 wing, AR 8, 160 x 60 quads: 19,516 triangles, wake to
+    20 chords; 64 angles from -4 to 12 deg plus 0 and +-4): mixed within 1e-6 (CL) and 1e-7 (CDi) of FP64 at every
+    angle; the symmetric section gives |CL(0)| and |CL(4) + CL(-4)| below 1e-4; the lift slope lies in the lifting-line
+    range for AR 8 (0.080-0.092 per degree) and the Trefftz drag's span efficiency CL²/(π AR CDi) in (0.9, 1.05)"""
+    spec = mesh.WingSpec(code="0012", n_around=160, n_span=60, aspect=8.0)
+    g = mesh.naca_wing(spec)
+    dm = dofmap.build_dofmap(g)
+    w = wk.build_wake(g, dm, np.array([1.0, 0.0, 0.0]), wk.WakeOptions(length=2.5))
+    print(f"case alpha batch full size nt={g.nt}", flush=True)
+    alphas = np.concatenate([np.linspace(-4.0, 12.0, 64), [0.0, 4.0]])
+    out = {}
+    for p in ("fp64", "mixed"):
+        sw = sweep.AlphaSweep(g, dm, w, spec.aspect * spec.mean_chord ** 2, precision=p)
+        out[p] = sw.solve(alphas)
+        sw.close()
+    a, m = out["fp64"], out["mixed"]
+    dcl, dcd = np.abs(a["CL"] - m["CL"]).max(), np.abs(a["CDi_T"] - m["CDi_T"]).max()
+    print(f"  mixed vs fp64: |dCL| {dcl:.2e} |dCDi| {dcd:.2e}", flush=True)
+    assert dcl < 1e-6 and dcd < 1e-7, (dcl, dcd)
+    cl0, cl4, clm4 = a["CL"][64], a["CL"][65], a["CL"][0]
+    slope = np.polyfit(alphas[16:40], a["CL"][16:40], 1)[0]
+    e = cl4 ** 2 / (np.pi * spec.aspect * a["CDi_T"][65])
+    print(f"  CL(0) {cl0:.2e} CL(4)+CL(-4) {cl4 + clm4:.2e} slope {slope:.5f}/deg e {e:.4f}", flush=True)
+    assert abs(cl0) < 1e-4 and abs(cl4 + clm4) < 1e-4
+    assert 0.080 < slope < 0.092
+    assert 0.9 < e < 1.05
