@@ -185,8 +185,8 @@ def test_swept_wing_full_size():
     16,560 triangles and 138 filaments): every pair is classified once (Σ pairs = nt (nt + wake triangles +
     filaments)); the rows of A (all 29,161 slot and 8,470 wake-node columns) and b of the 362 slots whose triangles
     all lie among test triangles 0..1199 equal the oracle's assembly of those rows to 2e-12 of the largest entry
-    (measured 6e-13: FP64 sums in another order, near the thin trailing edge where upper and lower panels' large
-    contributions cancel); the mixed assembly's rows there stay within 1e-8 of FP64"""
+    (measured 6e-13; the device's own log and atan2 and its order of FP64 sums differ from the oracle's); the mixed
+    assembly's rows there stay within 1e-8 of FP64"""
     from vphip import sweep
     spec = mesh.WingSpec(code="2412", n_around=240, n_span=120, aspect=10.0, taper=0.5, sweep_c4_deg=25.0)
     g = mesh.naca_wing(spec)
