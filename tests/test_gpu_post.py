@@ -127,3 +127,22 @@ def test_wake_field_graph(precision):
     with pytest.raises(ValueError):
         field(sig, gam, np.ones((2, 3)), np.ones((2, 3)), np.ones((3, 3)), np.ones(2))
     field.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_blob_velocities_full_size(precision):
+    """The free wake's end size: 84,000 blobs (the benchmark's spread and strengths, cores 0.01) summed at all 84,000
+    blob positions on the device; 600 of the targets (every 140th) against the oracle's FP64 sums over all 84,000
+    blobs, to the same tolerances as the small cases"""
+    print(f"blob_velocities full size {precision}", flush=True)
+    rng = np.random.default_rng(0)
+    nb = 84000
+    bx = rng.standard_normal((nb, 3)) * np.array([3.0, 3.0, 0.3])
+    ba = rng.standard_normal((nb, 3)) * 1e-3
+    bd = np.full(nb, 0.01)
+    u = post.blob_velocities(bx, ba, bd, bx, precision=precision)
+    pick = np.arange(0, nb, 140)
+    ref = O.blob_velocities(bx, ba, bd, bx[pick])
+    e = _err(u[pick], ref)
+    print(f"  {len(pick)} sampled targets: {e:.2e}", flush=True)
+    assert e < (1e-12 if precision == "fp64" else 5e-5), e
