@@ -146,3 +146,23 @@ def test_blob_velocities_full_size(precision):
     e = _err(u[pick], ref)
     print(f"  {len(pick)} sampled targets: {e:.2e}", flush=True)
     assert e < (1e-12 if precision == "fp64" else 5e-5), e
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_body_velocity_full_size(precision):
+    """body_velocity of the abused ellipsoid at its full benchmark size (19,880 triangles, random source and sheet
+    strengths) at 777 points inside and outside, off the surface, against the oracle's sums over every triangle"""
+    print(f"body_velocity full size {precision}", flush=True)
+    g = mesh.abused_ellipsoid(nth=71, nph=142, axes=(2.0, 1.0, 0.5), jitter=0.25, sliver_frac=0.05, sliver_ratio=0.1,
+                              seed=20261203)
+    rng = np.random.default_rng(11)
+    sig = rng.standard_normal(g.nt)
+    gam = rng.standard_normal((g.nt, 3))
+    d = rng.standard_normal((777, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    xs = d * rng.uniform(0.6, 6.0, 777)[:, None] * np.array([2.0, 1.0, 0.5])
+    u = post.body_velocity(g, sig, gam, xs, precision=precision)
+    ref = O.body_velocity(g, sig, gam, xs)
+    e = _err(u, ref)
+    print(f"  {e:.2e}", flush=True)
+    assert e < TOL[precision], e
