@@ -19,7 +19,9 @@ enum {
     VP_E_ARGUMENT = -1, /* bad option or input: the reference throws ArgumentError */
     VP_E_DIMENSION = -2, /* array sizes disagree: the reference throws DimensionMismatch */
     VP_E_DEVICE = -3,   /* HIP runtime error (allocation, launch) */
-    VP_E_SINGULAR = -4  /* zero pivot in the factorization: the reference's LAPACK getrf reports info > 0 */
+    VP_E_SINGULAR = -4, /* zero pivot in the factorization: the reference's LAPACK getrf reports info > 0 */
+    VP_E_CONVERGENCE = -5 /* FP64 iterative refinement did not meet its stopping test (no counterpart: the
+                             reference's LAPACK solve has no refinement) */
 };
 
 typedef struct vp_ctx vp_ctx;
@@ -159,15 +161,26 @@ int vp_factorize(vp_ctx *ctx, const double *A, long long lda, int A_on_device, c
                  vp_factor **out);
 /* Solve M X = R for nrhs right hand sides (`solve_galerkin(F, b)`, solve.jl:95-99): R and X are n x nrhs
  * row-major host arrays (R[i*nrhs + r]; the caller forms [Cᵀ b; 0]).  iters: refinement steps taken; resid: the
- * largest |R - M X|_inf / (|M|_inf |X|_inf). */
+ * largest relative residual |R - M X|_2 / |R|_2 over the right hand sides (the reference's `residual`, solve.jl:58).
+ * Refinement stops by LAPACK dsgesv's test (|r|_inf <= |x|_inf |M|_inf eps sqrt(n)) within 30 steps.  Mixed factors
+ * that do not meet it are replaced by an FP64 factorization of the same M (dsgesv's fallback; vp_factor_info reports
+ * it) and the solve is repeated; FP64 factors that do not meet it return VP_E_CONVERGENCE. */
 int vp_factor_solve(vp_factor *f, const double *R, int nrhs, double *X, int *iters, double *resid);
 /* Solve Mᵀ X = R (the adjoint system; no counterpart in the reference): same layout as vp_factor_solve, through the
  * same factors read transposed (no second factorization), FP64 refinement against Mᵀ. */
 int vp_factor_solve_transposed(vp_factor *f, const double *R, int nrhs, double *X, int *iters, double *resid);
 /* 1-norm condition number estimate of M (1 / gecon's rcond, solve.jl:85). */
 int vp_factor_cond(vp_factor *f, double *cond);
+/* `consistency(dm, A, b; W)` (solve.jl:111-115): with ACf = A [C; W] (the k's C and W; its border vectors are not
+ * read), out[0] = max |row sums of ACf| / max |ACf|, out[1] = max |column sums of ACf| / max |ACf|, out[2] =
+ * |sum b| / sum |b|.  A is column-major nslot x (nslot + nw), on the device when A_on_device (vp_plan_device_A); b is
+ * a host array of nslot values. */
+int vp_consistency(vp_ctx *ctx, const double *A, long long lda, int A_on_device, const double *b,
+                   const vp_constraints *k, double *out);
 /* The border column scales se (nb values, solve.jl:33) and n. */
 int vp_factor_info(vp_factor *f, int *n, double *se);
+/* The factorization's current precision mode (0 FP64, 1 FP32 factors) and the number of FP64 fallbacks taken. */
+int vp_factor_mode(vp_factor *f, int *mode, int *fallbacks);
 void vp_factor_destroy(vp_factor *f);
 /* The LU trailing update's GEMM tile shapes chosen by the autotuner (no counterpart in the reference), as JSON:
  * per precision the chosen BMxBNxBK (null before the first factorization large enough to time), the update size timed

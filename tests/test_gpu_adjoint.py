@@ -181,3 +181,21 @@ def test_swept_wing_full_size_mixed_matches_fp64():
         assert abs(a[0] - b[0]) < 1e-6, (o, a[0], b[0])
         assert np.abs(a[1] - b[1]).max() < 1e-5 * np.abs(a[1]).max(), o
     assert 0.5 < out["fp64"]["CL"][0] < 0.6
+
+
+def test_near_group_two_sets_one_launch(case, monkeypatch):
+    """near_group's backward build for two sets of adjoint weights in one launch (one wave per SIMD, no spills), whose
+    lanes hand split words, leaves and leaf corners to each other through LDS, against the per-set launches: the same
+    gradients (to FP64 atomic-order rounding) on three runs in a row."""
+    g, dm, w = case
+    print("adjoint near_group: two sets in one launch vs one per launch", flush=True)
+    adj = AD.WingAdjoint(g, dm, w, ALPHA, S, precision="fp64")
+    ref = adj.gradients(("CL", "CDi"))
+    monkeypatch.setenv("VP_ADJ_NEAR_ONE_LAUNCH", "1")
+    for run in range(3):
+        got = adj.gradients(("CL", "CDi"))
+        for o in ("CL", "CDi"):
+            scale = np.abs(ref[o][1]).max()
+            assert np.abs(got[o][1] - ref[o][1]).max() < 1e-13 * scale, (run, o)
+            assert abs(got[o][2] - ref[o][2]) < 1e-13 * abs(ref[o][2]) + 1e-15, (run, o)
+    adj.close()

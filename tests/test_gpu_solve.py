@@ -86,6 +86,7 @@ def test_sphere_solve(mode):
     assert e < 1e-11, e
     assert abs(sol["lam"][0] - ref["lam"][0]) < 1e-14
     assert abs(sol["cond"] / ref["cond"] - 1) < 0.05, (sol["cond"], ref["cond"])
+    assert sol["residual"] < 1e-12, sol["residual"]  # test/assembly/test_sphere.jl:52
     # lambda takes up an incompatible right hand side (test/assembly/test_sphere.jl:50-51)
     sh = solve.solve_galerkin(dm, A, b + 1e-3, mode=mode, cond=False)
     assert abs(sh["lam"][0] - 1e-3) < 1e-9
@@ -134,12 +135,14 @@ def test_wing_transposed_solve(mode):
     X = F.solve_bordered_t(R)
     ref = np.linalg.solve(M.T, R)
     assert np.abs(X - ref).max() / np.abs(ref).max() < 1e-11
-    assert F.last_resid < 1e-14
+    assert 0 <= F.last_resid < 1e-12  # |R - Mᵀ X|_2 / |R|_2, the reference's residual bound (test_sphere.jl:52)
+    assert abs(F.last_resid - np.linalg.norm(R - M.T @ X) / np.linalg.norm(R)) < 1e-13
     r, z = rng.standard_normal(F.n), rng.standard_normal(F.n)
     lhs, rhs = z @ F.solve_bordered(r), F.solve_bordered_t(z) @ r
     assert abs(lhs - rhs) < 1e-11 * (abs(lhs) + 1)
     ref_cond = np.linalg.cond(M, 1)
-    assert 0.3 < F.cond / ref_cond <= 1.0 + 1e-6, (F.cond, ref_cond)
+    # a lower bound (Hager-Higham); FP32 factors give it to about 1e-4
+    assert 0.3 < F.cond / ref_cond <= 1.0 + (1e-6 if mode == "fp64" else 1e-3), (F.cond, ref_cond)
     F.close()
 
 
@@ -164,7 +167,7 @@ def test_repeated_solves_replay(mode):
         X = F.solve_bordered_t(R) if trans else F.solve_bordered(R)
         ref = np.linalg.solve(M.T if trans else M, R)
         assert np.abs(X - ref).max() / np.abs(ref).max() < 1e-11, (k, nrhs, trans)
-        assert F.last_resid < 1e-14, (k, F.last_resid)
+        assert F.last_resid < 1e-12, (k, F.last_resid)
         np.linalg.norm(M @ ref - R)   # host work between the device calls
     F.close()
 
@@ -192,3 +195,63 @@ def test_two_bodies(mode):
     rsh = O.solve_galerkin(dm, Ao, bo + 1e-3)
     assert np.abs(sh["lam"] - rsh["lam"]).max() < 1e-12, (sh["lam"], rsh["lam"])
     assert np.abs(sh["lam"] - 1e-3).max() < 1e-5, sh["lam"]
+
+
+def _dense_solve(M, r, mode):
+    ctx = api.default_context()
+    n = M.shape[0]
+    Mc, rc = np.ascontiguousarray(M), np.ascontiguousarray(r)
+    x, it = np.zeros(n), C.c_int()
+    st = ctx.lib.vp_dense_solve(ctx.h, n, Mc.ctypes.data_as(C.POINTER(C.c_double)), mode,
+                                rc.ctypes.data_as(C.POINTER(C.c_double)), x.ctypes.data_as(C.POINTER(C.c_double)),
+                                C.byref(it))
+    return st, x, it.value
+
+
+def test_mixed_refinement_falls_back_to_fp64():
+    """A matrix whose condition (1e10) is beyond FP32 refinement: the FP32 factors' refinement cannot meet LAPACK
+    dsgesv's stopping test, so the solve replaces them by an FP64 factorization of the same matrix (dsgesv's fallback)
+    and returns the FP64 solution; a well conditioned one converges on the FP32 factors."""
+    rng = np.random.default_rng(3)
+    n = 300
+    Q1, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    x0 = rng.standard_normal(n)
+    for lo, fallback in ((1e-1, False), (1e-10, True)):
+        M = (Q1 * np.logspace(0, np.log10(lo), n)) @ Q2.T
+        r = M @ x0
+        print(f"dense solve mode=mixed cond={1 / lo:.0e}", flush=True)
+        st, x, it = _dense_solve(M, r, 1)
+        assert st == 0, api.default_context().lib.vp_last_error(api.default_context().h)
+        ref = np.linalg.solve(M, r)
+        assert (it > 30) == fallback, it  # the FP32 attempt's 30 refinements, then the FP64 solve's
+        assert np.linalg.norm(M @ x - r) / np.linalg.norm(r) < 1e-13
+        assert np.abs(x - ref).max() / np.abs(ref).max() < 1e-4
+
+
+def test_consistency_matches_oracle():
+    """`consistency` (src/assembly/solve.jl:111-115) on the device, from a host A and from an assembly plan's device A,
+    against the oracle's: a closed sphere (row and rhs defects at rounding level, test/assembly/test_sphere.jl:44-47)
+    and a wing whose wake rows W join A's wake columns."""
+    g = mesh.icosphere(3)
+    dm = dofmap.build_dofmap(g)
+    A, b, _ = _case(g, dm, VG)
+    print("consistency icosphere(3)", flush=True)
+    got, ref = solve.consistency(dm, A, b), O.consistency(dm, A, b)
+    for key in ("rows", "columns", "rhs"):
+        assert abs(got[key] - ref[key]) < 1e-14 + 1e-9 * ref[key], (key, got[key], ref[key])
+    assert got["rows"] < 1e-12 and got["rhs"] < 1e-12  # test/assembly/test_sphere.jl:44-47
+    gw = mesh.naca_wing(mesh.WingSpec(code="2412", n_around=20, n_span=6, aspect=5.0))
+    dmw = dofmap.build_dofmap(gw)
+    V = np.array([np.cos(0.07), 0.0, np.sin(0.07)])
+    w = wk.build_wake(gw, dmw, V, wk.WakeOptions(length=2.0))
+    print("consistency wing with wake", flush=True)
+    plan = api.AssemblyPlan(gw, dmw, w)
+    plan.assemble(api.source_strengths(api.build_elements(gw), V), V)
+    Aw, bw = plan.read()
+    W = dofmap.wake_rows(dmw, w.te)
+    ref = O.consistency(dmw, Aw, bw, W)
+    for src in (Aw, plan):
+        got = solve.consistency(dmw, src, bw, W)
+        for key in ("rows", "columns", "rhs"):
+            assert abs(got[key] - ref[key]) < 1e-14 + 1e-9 * ref[key], (key, got[key], ref[key])

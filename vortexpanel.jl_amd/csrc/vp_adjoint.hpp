@@ -494,9 +494,16 @@ template <int NL> void Plan::adjoint_pass(const AdjArgs &aa)
         HIPCHECK(hipGetLastError());
     }
     // near_group takes one set per launch: with two it spills (560 bytes per lane under its two-wave cap) and runs
-    // slower than two launches; built for two sets without that cap (ROCm 7.2, gfx950) its gradients were not
-    // reproducible from run to run, while every other kernel here matched its one-set build to 1e-15
-    for (int l = 0; nnearm && l < NL; l++) {
+    // slower than two launches.  VP_ADJ_NEAR_ONE_LAUNCH=1 takes every set in one launch of the build at one wave per
+    // SIMD (no spills), which the tests compare with the per-set launches.
+    const char *one_env = getenv("VP_ADJ_NEAR_ONE_LAUNCH");
+    const bool one_launch = NL > 1 && one_env && one_env[0] == '1';
+    if (nnearm && one_launch) {
+        near_group<NL, 1><<<std::min<unsigned int>((nnearm + 3) / 4, 65535u), 256, 0, st>>>(
+            P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, nnearm, nullptr, nullptr, nullptr, nullptr, aa);
+        HIPCHECK(hipGetLastError());
+    }
+    for (int l = 0; nnearm && !one_launch && l < NL; l++) {
         AdjArgs a1 = aa;
         a1.lam += (size_t)l * aa.lstride;
         a1.G += l * aa.gstride;

@@ -1037,6 +1037,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
         int rid = e.z, npc = 1;
         const Elem &T = el[t], &S = el[s];
         const V3<double> X[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
+        wave_lds_sync();  // the previous pair's reads of wedge are done before it is rewritten
         for (int i = 0; i < 3; i++)
             for (int j = 0; j < 3; j++) wedge[wave][0][3 * i + j] = i == j ? 1.0 : 0.0;
         wedge[wave][0][9] = 1.0;
@@ -1081,6 +1082,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
                 }
             }
         }
+        wave_lds_sync();  // the pieces' corners, written by every lane, read at other lanes' points below
         const int nr = rt.n[rid], off = rt.off[rid], ntot = npc * nr;
         const int side = s == t ? 1 : 0;
         const V3<double> tn = ld3(T.n), sc = ld3(S.c);
@@ -1361,8 +1363,9 @@ __device__ __forceinline__ void node_of(int nid, int &L, int &idx)  // level-ord
 // reach the depth -- by ballot, and evaluates their 7-point rules flattened over the lanes.  The forward occupancy cap
 // WPS is chosen by the launch autotuner (three waves per SIMD measured best: C2 7.77 -> 6.50 ms against two); the
 // backward is capped at two (faster than the register allocation the compiler picks by itself).
+// The backward builds are capped at two waves per SIMD, or at one with WPS = 1 (all weight sets in one launch).
 template <int NADJ, int WPS = 3>  // NADJ 0: the forward assembly; 1..ADJ_MAX: backward with that many weight sets
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ? WPS : 2, NADJ == 0 ? WPS : 2))) void near_group(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ? WPS : (WPS == 1 ? 1 : 2), NADJ == 0 ? WPS : (WPS == 1 ? 1 : 2)))) void near_group(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                   const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                   const int *__restrict__ src_cols, const int2 *__restrict__ list,
                                                   unsigned int npairs, double *__restrict__ A, double *__restrict__ bvec,
@@ -1429,6 +1432,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
         const Elem &T = el[t], &S = el[s];
         const V3<double> X[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
         const V3<double> SX[3] = {ld3(S.v[0]), ld3(S.v[1]), ld3(S.v[2])};
+        wave_lds_sync();  // the previous pair's reads of splitw, leaf and leafb are done before they are rewritten
         // 1b. one pair's split tests over the lanes (deeper trees)
         if (PB > 1) {
             if (lane == 0) splitw[wave][0] = sw[bb];
@@ -1440,6 +1444,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
                 if (lane == 0) splitw[wave][base / WAVE] = m;
             }
         }
+        wave_lds_sync();  // lane 0's split words, read by every lane below
         // 2. leaves: every ancestor splits, the node itself does not (or is at the depth)
         int nl = 0;
         for (int base = 0; base < ncand; base += WAVE) {
@@ -1462,6 +1467,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
             if (isleaf) leaf[wave][nl + __popcll(m & ((1ull << lane) - 1ull))] = (unsigned short)nid;
             nl += __popcll(m);
         }
+        wave_lds_sync();  // the leaf list, written by the leaves' lanes and read by others
         // the first LB leaves' corners and area fractions, once per leaf for the point loop
         if (lane < nl && lane < LB) {
             int L, idx;
@@ -1474,6 +1480,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
                 for (int j = 0; j < 3; j++) leafb[wave][lane][3 * i + j] = b[i][j];
             leafb[wave][lane][9] = frac;
         }
+        wave_lds_sync();  // the leaves' corners, written by lanes below LB and read by every lane
         // leaf li's corners b and area fraction (LDS for the first LB leaves)
         auto leaf_geom = [&](int li, double (*b)[3], double &frac) {
             if (li < LB) {

@@ -31,6 +31,8 @@ template <typename F> int guard(vp::Ctx *c, F &&f)
         return fail(c, VP_E_DIMENSION, e.what());
     } catch (const std::domain_error &e) {
         return fail(c, VP_E_SINGULAR, e.what());
+    } catch (const vp::ConvergenceError &e) {
+        return fail(c, VP_E_CONVERGENCE, e.what());
     } catch (const std::bad_alloc &) {
         return fail(c, VP_E_DEVICE, "host allocation failed");
     } catch (const std::exception &e) {
@@ -249,6 +251,35 @@ int vp_factorize(vp_ctx *ctx, const double *A, long long lda, int A_on_device, c
     return VP_OK;
 }
 
+int vp_consistency(vp_ctx *ctx, const double *A, long long lda, int A_on_device, const double *b,
+                   const vp_constraints *k, double *out)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if (!A || !b || !k || !out || !k->C_rowptr || !k->C_col || !k->C_val ||
+        (k->nw > 0 && (!k->W_rowptr || !k->W_col || !k->W_val)))
+        return fail(&ctx->c, VP_E_ARGUMENT, "null argument");
+    if (k->nslot <= 0 || k->nq <= 0 || k->nw < 0) return fail(&ctx->c, VP_E_ARGUMENT, "empty system");
+    if (lda < k->nslot) return fail(&ctx->c, VP_E_DIMENSION, "lda is smaller than nslot");
+    double *tmp = nullptr;
+    int rc = guard(&ctx->c, [&] {
+        (void)hipSetDevice(ctx->c.device);
+        const double *Ad = A;
+        if (!A_on_device) {
+            size_t bytes = sizeof(double) * (size_t)lda * (size_t)(k->nslot + k->nw);
+            if (hipMalloc((void **)&tmp, bytes) != hipSuccess) throw std::runtime_error("device allocation of A failed");
+            if (hipMemcpyAsync(tmp, A, bytes, hipMemcpyHostToDevice, ctx->c.stream) != hipSuccess)
+                throw std::runtime_error("copy of A failed");
+            Ad = tmp;
+        }
+        vp::consistency_host(&ctx->c, Ad, lda, b, *k, out);
+    });
+    if (tmp) {
+        (void)hipStreamSynchronize(ctx->c.stream);
+        (void)hipFree(tmp);
+    }
+    return rc;
+}
+
 int vp_factor_solve(vp_factor *f, const double *R, int nrhs, double *X, int *iters, double *resid)
 {
     if (!f || !f->f.ctx) return VP_E_ARGUMENT;
@@ -300,6 +331,14 @@ int vp_factor_info(vp_factor *f, int *n, double *se)
     if (n) *n = f->f.n;
     if (se)
         for (size_t i = 0; i < f->f.se.size(); i++) se[i] = f->f.se[i];
+    return VP_OK;
+}
+
+int vp_factor_mode(vp_factor *f, int *mode, int *fallbacks)
+{
+    if (!f) return VP_E_ARGUMENT;
+    if (mode) *mode = f->f.mode;
+    if (fallbacks) *fallbacks = f->f.fallbacks;
     return VP_OK;
 }
 

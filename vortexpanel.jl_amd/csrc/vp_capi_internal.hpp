@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -13,6 +14,11 @@ struct Elem;
 struct TestFar;
 struct AsmParams;
 struct AdjArgs;
+
+// Iterative refinement that does not meet its stopping test (VP_E_CONVERGENCE at the C ABI)
+struct ConvergenceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
 
 struct Ctx {
     int device = 0;
@@ -119,6 +125,7 @@ struct Factor {
         int nrhs = -1, epoch = -1, calls = 0;
     } inv_graph[2];
     long long graph_launches = 0;
+    int fallbacks = 0;  // FP32 factorizations replaced by FP64 ones (refinement that did not converge)
     double anorm1 = 0.0, anorm_inf = -1.0, cond = -1.0;
     std::vector<double> se;
     std::vector<void *> owned;
@@ -129,7 +136,9 @@ struct Factor {
     void ensure_ws(int nrhs);
     void apply_inverse(int nrhs, bool trans);
     void apply_inverse_replay(int nrhs, bool trans);
-    int solve(const double *B, int nrhs, double *X, bool device_io, int max_iter, double *resid, bool trans = false);
+    int solve(const double *B, int nrhs, double *X, bool device_io, int max_iter, double *resid, bool trans = false,
+              bool strict = true);
+    void refactor_fp64();
     double cond_estimate();
 };
 
@@ -165,6 +174,8 @@ struct WakeField {
               const double *alpha, const double *delta, const double *base, double *u);
 };
 int dense_solve_host(Ctx *c, int n, const double *M, int mode, const double *r, double *x);
+// consistency of A (on the device, column-major lda) and b (host), out: rows, columns, rhs
+void consistency_host(Ctx *c, const double *A, long long lda, const double *b, const vp_constraints &k, double *out);
 void math_probe_host(Ctx *c, int fn, int n, const double *a, const double *b, double *out);
 
 }  // namespace vp

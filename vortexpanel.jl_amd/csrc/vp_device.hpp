@@ -31,6 +31,17 @@ template <typename T> __device__ __forceinline__ V3<T> cross(V3<T> a, V3<T> b)
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
+// Orders LDS accesses between the lanes of one wavefront: what any lane wrote before the call is visible to every
+// lane after it, and no access is moved across it by the compiler (a release fence at wavefront scope, the
+// scheduling barrier, an acquire fence).  Kernels that hand values from some lanes to others through LDS, with no
+// workgroup barrier in between, call it after the writes and again before the same locations are written anew.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // FP64 natural logarithm: x = m 2^e with m scaled into [sqrt(1/2), sqrt(2)), then log m = 2 atanh(s),
 // s = (m - 1)/(m + 1), |s| <= 3 - 2 sqrt(2) < 0.1716, by eleven odd terms of the series (the next is below 2^-53
 // of the sum), and e ln 2 with ln 2 split into a 32-bit head (exact times any exponent) and its tail.  About 40

@@ -601,29 +601,38 @@ void launch_residual(hipStream_t s, bool trans, const double *M, long long ld, i
     HIPCHECK2(hipGetLastError());
 }
 
-// per right hand side: max |r|, max |x| (columns of n x nrhs row-major), into out[2*c], out[2*c+1]
-__global__ void colmax2(const double *r, const double *x, int n, int nrhs, double *out)
+// per right hand side (columns of n x nrhs row-major): max |r|, max |x|, sum r^2 and sum rhs^2 into out[4c..4c+3]
+__global__ void colmax2(const double *r, const double *x, const double *rhs, int n, int nrhs, double *out)
 {
-    __shared__ double s1[256], s2[256];
+    __shared__ double s1[256], s2[256], s3[256], s4[256];
     int c = blockIdx.x;
-    double a = 0.0, b = 0.0;
+    double a = 0.0, b = 0.0, r2 = 0.0, b2 = 0.0;
     for (int i = threadIdx.x; i < n; i += 256) {
-        a = fmax(a, fabs(r[(long long)i * nrhs + c]));
+        const double ri = r[(long long)i * nrhs + c], bi = rhs[(long long)i * nrhs + c];
+        a = fmax(a, fabs(ri));
         b = fmax(b, fabs(x[(long long)i * nrhs + c]));
+        r2 += ri * ri;
+        b2 += bi * bi;
     }
     s1[threadIdx.x] = a;
     s2[threadIdx.x] = b;
+    s3[threadIdx.x] = r2;
+    s4[threadIdx.x] = b2;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o) {
             s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + o]);
             s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + o]);
+            s3[threadIdx.x] += s3[threadIdx.x + o];
+            s4[threadIdx.x] += s4[threadIdx.x + o];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out[2 * c] = s1[0];
-        out[2 * c + 1] = s2[0];
+        out[4 * c] = s1[0];
+        out[4 * c + 1] = s2[0];
+        out[4 * c + 2] = s3[0];
+        out[4 * c + 3] = s4[0];
     }
 }
 
@@ -1038,7 +1047,7 @@ void Factor::ensure_ws(int nrhs)
     Y = dmalloc<double>(sz);
     B = dmalloc<double>(sz);
     Xf = dmalloc<float>(sz);
-    nrm = dmalloc<double>(2 * (size_t)nrhs);
+    nrm = dmalloc<double>(4 * (size_t)nrhs);
     part = dmalloc<double>((size_t)KSPLIT * npad * std::min(nrhs, GEMV_MAX));
     owned.insert(owned.end(), {X, R, Y, B, Xf, nrm, part});
     ws_nrhs = nrhs;
@@ -1103,8 +1112,14 @@ void Factor::apply_inverse_replay(int nrhs, bool trans)
 }
 
 // Solve M X = Bh, or Mᵀ X = Bh with trans (n x nrhs row-major host or device), with FP64 iterative refinement (LAPACK
-// dsgesv's stopping test: |r|_inf <= |x|_inf |op(M)|_inf eps sqrt(n), at most max_iter refinements).
-int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int max_iter, double *resid_out, bool trans)
+// dsgesv's stopping test: |r|_inf <= |x|_inf |op(M)|_inf eps sqrt(n), at most max_iter refinements).  resid_out: the
+// largest relative residual |R - op(M) X|_2 / |R|_2 over the right hand sides (the reference's residual, solve.jl:58).
+// When the test is not met within max_iter, the FP32 factors are replaced by an FP64 factorization of the same M and
+// the solve is repeated (dsgesv's fallback); FP64 factors that still do not meet it raise ConvergenceError.
+// max_iter 0 applies the factors once, with no refinement and no test; strict false refines up to max_iter and keeps
+// what it has (the condition estimate's products).
+int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int max_iter, double *resid_out, bool trans,
+                  bool strict)
 {
     hipStream_t s = ctx->stream;
     ensure_ws(nrhs);
@@ -1114,6 +1129,13 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
     HIPCHECK2(hipMemcpyAsync(Y, B, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
     apply_inverse_replay(nrhs, trans);
     HIPCHECK2(hipMemcpyAsync(X, Y, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
+    if (max_iter <= 0) {
+        if (resid_out) *resid_out = -1.0;
+        HIPCHECK2(hipMemcpyAsync(Xout, X, sizeof(double) * nn, device_io ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+        HIPCHECK2(hipStreamSynchronize(s));
+        last_iters = 0;
+        return 0;
+    }
     if (trans && anorm1 <= 0) {
         double *dcol = dmalloc<double>(npad);
         col_abs_sum<<<blocks(n), 256, 0, s>>>(M, ld, n, dcol);
@@ -1134,8 +1156,9 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
     }
     const double opnorm = trans ? anorm1 : anorm_inf;  // |Mᵀ|_inf = |M|_1
     const double eps = 1.1102230246251565e-16, tolf = opnorm * eps * std::sqrt((double)n);
-    std::vector<double> hn(2 * (size_t)nrhs);
+    std::vector<double> hn(4 * (size_t)nrhs);
     int it = 0;
+    bool done = false;
     double worst = 0.0;
     for (;; it++) {
         // R = B - M X in FP64: a GEMV pass over M for a few right hand sides, else the MFMA GEMM
@@ -1151,19 +1174,32 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
             else
                 launch_gemm<double, true>(s, npad, nrhs, npad, M, ld, X, nrhs, Y, nrhs);
         }
-        colmax2<<<nrhs, 256, 0, s>>>(Y, X, n, nrhs, nrm);
-        HIPCHECK2(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 2 * nrhs, hipMemcpyDeviceToHost, s));
+        colmax2<<<nrhs, 256, 0, s>>>(Y, X, B, n, nrhs, nrm);
+        HIPCHECK2(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 4 * nrhs, hipMemcpyDeviceToHost, s));
         HIPCHECK2(hipStreamSynchronize(s));
-        bool done = true;
+        done = true;
         worst = 0.0;
         for (int c = 0; c < nrhs; c++) {
-            if (hn[2 * c] > hn[2 * c + 1] * tolf) done = false;
-            worst = std::max(worst, hn[2 * c] / std::max(hn[2 * c + 1] * opnorm, 1e-300));
+            if (!(hn[4 * c] <= hn[4 * c + 1] * tolf)) done = false;  // a NaN residual fails the test too
+            const double rb = std::sqrt(hn[4 * c + 3]);
+            worst = std::max(worst, rb > 0 ? std::sqrt(hn[4 * c + 2]) / rb : std::sqrt(hn[4 * c + 2]));
         }
         if (done || it >= max_iter) break;
         apply_inverse_replay(nrhs, trans);
         axpy_inplace<<<blocks((long long)sz), 256, 0, s>>>(X, Y, (long long)sz);
         HIPCHECK2(hipGetLastError());
+    }
+    if (!done && strict) {
+        if (mode == 1) {
+            refactor_fp64();
+            const int it2 = solve(Bin, nrhs, Xout, device_io, max_iter, resid_out, trans);
+            return it + 1 + it2;
+        }
+        char msg[200];
+        snprintf(msg, sizeof msg,
+                 "iterative refinement did not converge: relative residual %.3e after %d refinements of the FP64 factors",
+                 worst, it);
+        throw ConvergenceError(msg);
     }
     if (resid_out) *resid_out = worst;
     HIPCHECK2(hipMemcpyAsync(Xout, X, sizeof(double) * nn, device_io ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
@@ -1172,15 +1208,55 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
     return it;
 }
 
+// dsgesv's fallback: the FP32 factors did not give a converged solve, so M (kept in FP64) is factored again in FP64
+// through the same butterflies, and the factorization stays FP64 from then on.
+void Factor::refactor_fp64()
+{
+    hipStream_t s = ctx->stream;
+    HIPCHECK2(hipStreamSynchronize(s));
+    for (InvGraph &g : inv_graph) {
+        if (g.exec) HIPCHECK2(hipGraphExecDestroy(g.exec));
+        g = InvGraph();
+    }
+    int *dbad = dmalloc<int>(1);
+    owned.push_back(dbad);
+    HIPCHECK2(hipMemsetAsync(dbad, 0, sizeof(int), s));
+    const size_t nblk = (size_t)(npad / NB) * NB * NB;
+    const long long h = npad / 4;
+    G64 = dmalloc<double>((size_t)ld * npad);
+    linv64 = dmalloc<double>(nblk);
+    uinv64 = dmalloc<double>(nblk);
+    owned.insert(owned.end(), {G64, linv64, uinv64});
+    const int cfg = tuned_cfg<double>(s, G64, ld, npad);
+    rbt_both<double><<<blocks(h * h), 256, 0, s>>>(M, ld, npad, d_ru, d_rv, G64, ld);
+    HIPCHECK2(hipGetLastError());
+    lu_factor<double>(s, G64, ld, npad, linv64, uinv64, dbad, cfg);
+    int hbad = 0;
+    HIPCHECK2(hipMemcpyAsync(&hbad, dbad, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK2(hipStreamSynchronize(s));
+    if (hbad) throw std::domain_error("zero or non-finite pivot in the randomized LU (singular bordered matrix)");
+    for (void *p : {(void *)G32, (void *)linv32, (void *)uinv32}) {
+        if (!p) continue;
+        owned.erase(std::find(owned.begin(), owned.end(), p));
+        (void)hipFree(p);
+    }
+    G32 = nullptr;
+    linv32 = uinv32 = nullptr;
+    mode = 0;
+    fallbacks++;
+}
+
 // Hager-Higham estimate of |M^-1|_1 (the iteration of LAPACK dlacn2, which gecon runs on the reference's factors).
 // It needs products with M^-1 and M^-T, both from the one factorization.
 double Factor::cond_estimate()
 {
+    // products with the factors alone, as gecon takes them (FP32 factors give the estimate to about 1e-4)
+    const int kit = 0;
     std::vector<double> x(n, 1.0 / n), y(n), z(n);
     double est = 0.0;
     int jlast = -1;
     for (int iter = 0; iter < 5; iter++) {
-        solve(x.data(), 1, y.data(), false, 30, nullptr);  // y = M^-1 x
+        solve(x.data(), 1, y.data(), false, kit, nullptr, false, false);  // y = M^-1 x
         double ny = 0.0;
         for (double v : y) ny += std::fabs(v);
         if (iter > 0 && ny <= est) {
@@ -1189,7 +1265,7 @@ double Factor::cond_estimate()
         }
         est = ny;
         for (int i = 0; i < n; i++) z[i] = y[i] >= 0 ? 1.0 : -1.0;
-        solve(z.data(), 1, x.data(), false, 30, nullptr, true);  // x = M^-T z
+        solve(z.data(), 1, x.data(), false, kit, nullptr, true, false);  // x = M^-T z
         int j = 0;
         for (int i = 1; i < n; i++)
             if (std::fabs(x[i]) > std::fabs(x[j])) j = i;
@@ -1201,7 +1277,7 @@ double Factor::cond_estimate()
     // the alternative estimate of dlacn2 (altsgn)
     std::vector<double> alt(n);
     for (int i = 0; i < n; i++) alt[i] = (i % 2 ? -1.0 : 1.0) * (1.0 + (double)i / (n > 1 ? n - 1 : 1));
-    solve(alt.data(), 1, y.data(), false, 30, nullptr);
+    solve(alt.data(), 1, y.data(), false, kit, nullptr, false, false);
     double t = 0.0;
     for (double v : y) t += std::fabs(v);
     t = 2 * t / (3.0 * n);
@@ -1317,4 +1393,143 @@ int dense_solve_host(Ctx *c, int n, const double *Mh, int mode, const double *r,
     int it = f.solve(r, 1, x, false, 30, nullptr);
     return it;
 }
+
+// ------------------------------------------------------------------------------------------ consistency ------------
+// `consistency(dm, A, b; W)` (src/assembly/solve.jl:111-115): with ACf = A [C; W] (nslot x nq), the largest absolute
+// row sum and column sum of ACf over its largest absolute entry, and |sum b| / sum |b|.
+
+// One workgroup per column q of Cf = [C; W] (CSC): ACf[:, q] = sum over the column's entries of v A[:, j]; its
+// largest absolute entry and its sum (threads over the rows, coalesced down A's columns; fixed reduction order).
+__global__ __launch_bounds__(256) void acf_columns(const double *__restrict__ A, long long lda, int nrow,
+                                                   const int *__restrict__ fp, const int *__restrict__ fi,
+                                                   const double *__restrict__ fv, double *__restrict__ colmax,
+                                                   double *__restrict__ colsum)
+{
+    __shared__ double smax[256], ssum[256];
+    const int q = blockIdx.x, a0 = fp[q], a1 = fp[q + 1];
+    double mx = 0.0, sm = 0.0;
+    for (int i = threadIdx.x; i < nrow; i += 256) {
+        double v = 0.0;
+        for (int a = a0; a < a1; a++) v += fv[a] * A[i + (long long)fi[a] * lda];
+        mx = fmax(mx, fabs(v));
+        sm += v;
+    }
+    smax[threadIdx.x] = mx;
+    ssum[threadIdx.x] = sm;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + o]);
+            ssum[threadIdx.x] += ssum[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        colmax[q] = smax[0];
+        colsum[q] = ssum[0];
+    }
+}
+
+// Row sums of ACf as A (Cf 1): partial sums over column chunks (grid.y), thread per row, then added in chunk order
+__global__ __launch_bounds__(256) void av_rows(const double *__restrict__ A, long long lda, int nrow, int ncol,
+                                               const double *__restrict__ v, int chunk, double *__restrict__ part)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
+    if (i >= nrow) return;
+    const int j1 = min(ncol, (c + 1) * chunk);
+    double s = 0.0;
+    for (int j = c * chunk; j < j1; j++) s += A[i + (long long)j * lda] * v[j];
+    part[(long long)c * nrow + i] = s;
+}
+
+__global__ void av_rows_finish(const double *__restrict__ part, int nrow, int nchunk, double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrow) return;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; c++) s += part[(long long)c * nrow + i];
+    out[i] = s;
+}
+
+void consistency_host(Ctx *c, const double *A, long long lda, const double *b, const vp_constraints &k, double *out)
+{
+    hipStream_t s = c->stream;
+    const int nslot = k.nslot, nq = k.nq, nw = k.nw, ncol = nslot + nw;
+    std::vector<std::vector<std::pair<int, double>>> fcol(nq);
+    std::vector<double> v1(ncol, 0.0);  // Cf 1
+    for (int i = 0; i < nslot; i++)
+        for (int a = k.C_rowptr[i]; a < k.C_rowptr[i + 1]; a++) {
+            const int q = k.C_col[a];
+            if (q < 0 || q >= nq) throw std::out_of_range("C column index out of range");
+            fcol[q].push_back({i, k.C_val[a]});
+            v1[i] += k.C_val[a];
+        }
+    for (int i = 0; i < nw; i++)
+        for (int a = k.W_rowptr[i]; a < k.W_rowptr[i + 1]; a++) {
+            const int q = k.W_col[a];
+            if (q < 0 || q >= nq) throw std::out_of_range("W column index out of range");
+            fcol[q].push_back({nslot + i, k.W_val[a]});
+            v1[nslot + i] += k.W_val[a];
+        }
+    std::vector<int> fp(1, 0), fi;
+    std::vector<double> fv;
+    for (auto &col : fcol) {
+        for (auto &e : col) {
+            fi.push_back(e.first);
+            fv.push_back(e.second);
+        }
+        fp.push_back((int)fi.size());
+    }
+    const int chunk = 512, nchunk = (ncol + chunk - 1) / chunk;
+    std::vector<void *> mine;
+    auto dev = [&](size_t bytes) {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 8)) != hipSuccess) throw std::runtime_error("device allocation failed");
+        mine.push_back(p);
+        return p;
+    };
+    try {
+        int *dfp = (int *)dev(fp.size() * 4), *dfi = (int *)dev(fi.size() * 4);
+        double *dfv = (double *)dev(fv.size() * 8), *dv = (double *)dev((size_t)ncol * 8);
+        double *cmax = (double *)dev((size_t)nq * 8), *csum = (double *)dev((size_t)nq * 8);
+        double *part = (double *)dev((size_t)nchunk * nslot * 8), *rsum = (double *)dev((size_t)nslot * 8);
+        HIPCHECK2(hipMemcpyAsync(dfp, fp.data(), fp.size() * 4, hipMemcpyHostToDevice, s));
+        if (!fi.empty()) {
+            HIPCHECK2(hipMemcpyAsync(dfi, fi.data(), fi.size() * 4, hipMemcpyHostToDevice, s));
+            HIPCHECK2(hipMemcpyAsync(dfv, fv.data(), fv.size() * 8, hipMemcpyHostToDevice, s));
+        }
+        HIPCHECK2(hipMemcpyAsync(dv, v1.data(), (size_t)ncol * 8, hipMemcpyHostToDevice, s));
+        acf_columns<<<nq, 256, 0, s>>>(A, lda, nslot, dfp, dfi, dfv, cmax, csum);
+        HIPCHECK2(hipGetLastError());
+        av_rows<<<dim3((nslot + 255) / 256, nchunk), 256, 0, s>>>(A, lda, nslot, ncol, dv, chunk, part);
+        HIPCHECK2(hipGetLastError());
+        av_rows_finish<<<(nslot + 255) / 256, 256, 0, s>>>(part, nslot, nchunk, rsum);
+        HIPCHECK2(hipGetLastError());
+        std::vector<double> hmax(nq), hsum(nq), hrow(nslot);
+        HIPCHECK2(hipMemcpyAsync(hmax.data(), cmax, (size_t)nq * 8, hipMemcpyDeviceToHost, s));
+        HIPCHECK2(hipMemcpyAsync(hsum.data(), csum, (size_t)nq * 8, hipMemcpyDeviceToHost, s));
+        HIPCHECK2(hipMemcpyAsync(hrow.data(), rsum, (size_t)nslot * 8, hipMemcpyDeviceToHost, s));
+        HIPCHECK2(hipStreamSynchronize(s));
+        double nA = 0.0, cs = 0.0, rs = 0.0;
+        for (int q = 0; q < nq; q++) {
+            nA = std::max(nA, hmax[q]);
+            cs = std::max(cs, std::fabs(hsum[q]));
+        }
+        for (int i = 0; i < nslot; i++) rs = std::max(rs, std::fabs(hrow[i]));
+        double bs = 0.0, ba = 0.0;
+        for (int i = 0; i < nslot; i++) {
+            bs += b[i];
+            ba += std::fabs(b[i]);
+        }
+        out[0] = rs / nA;
+        out[1] = cs / nA;
+        out[2] = std::fabs(bs) / ba;
+    } catch (...) {
+        (void)hipStreamSynchronize(s);
+        for (void *p : mine) (void)hipFree(p);
+        throw;
+    }
+    for (void *p : mine) (void)hipFree(p);
+}
+
 }  // namespace vp

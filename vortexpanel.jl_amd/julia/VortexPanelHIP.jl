@@ -32,6 +32,7 @@ const VP_E_ARGUMENT = Cint(-1)
 const VP_E_DIMENSION = Cint(-2)
 const VP_E_DEVICE = Cint(-3)
 const VP_E_SINGULAR = Cint(-4)
+const VP_E_CONVERGENCE = Cint(-5)
 const TIER_NAMES = (:self, :edge, :vertex, :contact, :near, :far)   # src/assembly/assemble.jl:260
 
 # ---- C structs (field order and types of include/vortexhip.h) ---------------------------------------------------
@@ -110,6 +111,11 @@ default_context() = something(DEFAULT[], (DEFAULT[] = Context(0)))
 
 last_error(ctx::Context) = unsafe_string(ccall((:vp_last_error, libvortexhip), Cstring, (Ptr{Cvoid},), ctx.ptr))
 
+"""FP64 iterative refinement of a solve did not meet its stopping test (no counterpart: LAPACK's solve has none)."""
+struct ConvergenceError <: Exception
+    msg::String
+end
+
 """Rethrow a status as the exception the reference raises for the same condition."""
 function check(ctx::Context, rc::Cint)
     rc == VP_OK && return nothing
@@ -117,6 +123,7 @@ function check(ctx::Context, rc::Cint)
     rc == VP_E_ARGUMENT && throw(ArgumentError(msg))          # check_options, assemble.jl:43-46
     rc == VP_E_DIMENSION && throw(DimensionMismatch(msg))     # reduce_system, src/dofmap/system.jl:28
     rc == VP_E_SINGULAR && throw(SingularException(0))         # getrf info > 0
+    rc == VP_E_CONVERGENCE && throw(ConvergenceError(msg))
     error("vortexhip: $msg")
 end
 
@@ -171,7 +178,8 @@ c_wake(w::WakeArrays) = VPWake(length(w.x) ÷ 3, pointer(w.x), length(w.tris) ÷
     assemble(g, dm, els, σ, V∞; opts=AssemblyOptions(), external=nothing, wake=nothing, precision=:fp64, ctx)
 
 The reference's `assemble` on the GPU: returns `(A, b, stats)` with A nslot × (nslot + nwake) (row = test slot,
-column = source slot then wake node), b and `stats = (pairs, points, time)` keyed by tier. `els` is accepted for
+column = source slot then wake node), b and `stats = (pairs, points, time)` with `pairs` and `points` a
+`NamedTuple{TIER_NAMES}` of counts (assemble.jl:445), so `stats.pairs.contact` and `sum(stats.pairs)` work as there. `els` is accepted for
 signature compatibility; the library builds its element records from `g.x`/`g.tris` (the same construction as
 `build_elements`). `external` (an arbitrary closure) cannot cross the C boundary: pass `nothing`.
 """
@@ -200,7 +208,7 @@ function assemble(g::SurfaceGeometry, dm::DOFMapping, els, σ::AbstractVector, V
             ctx.ptr, sc, wc, sig, vinf, Ref(o), A, b, pairs, points)
         check(ctx, rc)
     end
-    stats = (pairs=Dict(zip(TIER_NAMES, Int.(pairs))), points=Dict(zip(TIER_NAMES, Int.(points))), time=t)
+    stats = (pairs=NamedTuple{TIER_NAMES}(Tuple(Int.(pairs))), points=NamedTuple{TIER_NAMES}(Tuple(Int.(points))), time=t)
     return A, b, stats
 end
 
@@ -266,7 +274,10 @@ end
 `solve_galerkin(F, b)` of solve.jl:95-99: `(q, μ, λ, cond)`. `b` may also be a matrix whose columns are right hand
 sides (an angle-of-attack sweep solved as one batch); q, μ, λ then have one column per right hand side.
 """
-function solve_galerkin(F::GalerkinFactorizationHIP, b::AbstractVecOrMat)
+solve_galerkin(F::GalerkinFactorizationHIP, b::AbstractVecOrMat) = Base.structdiff(solve_residual(F, b), NamedTuple{(:residual,)})
+
+"""The solve of `solve_galerkin(F, b)` plus the library's relative residual |M x - rhs|₂ / |rhs|₂ (`residual`)."""
+function solve_residual(F::GalerkinFactorizationHIP, b::AbstractVecOrMat)
     B = reshape(Matrix{Cdouble}(reshape(b, size(b, 1), :)), size(b, 1), :)
     nrhs = size(B, 2)
     n = F.nq + F.nb
@@ -283,8 +294,8 @@ function solve_galerkin(F::GalerkinFactorizationHIP, b::AbstractVecOrMat)
     X = Matrix(transpose(Xr))
     q = X[1:F.nq, :]
     λ = F.se .* X[F.nq+1:n, :]
-    b isa AbstractVector && return (q=vec(q), μ=F.C * vec(q), λ=vec(λ), cond=F.cond)
-    return (q=q, μ=F.C * q, λ=λ, cond=F.cond)
+    b isa AbstractVector && return (q=vec(q), μ=F.C * vec(q), λ=vec(λ), cond=F.cond, residual=resid[])
+    return (q=q, μ=F.C * q, λ=λ, cond=F.cond, residual=resid[])
 end
 
 """
@@ -295,10 +306,32 @@ The one-shot form of solve.jl:51-59: `(q, μ, λ, cond, residual)`.
 function solve_galerkin(dm::DOFMapping, A::AbstractMatrix, b::AbstractVector; W=nothing, precision::Symbol=:fp64,
                         ctx::Context=default_context())
     F = factorize_galerkin(dm, A; W=W, precision=precision, ctx=ctx)
-    s = solve_galerkin(F, b)
-    M, rhs, _ = VortexPanel.bordered_system(dm, A, b; W=W)
-    x = [s.q; s.λ ./ F.se]
-    return (q=s.q, μ=s.μ, λ=s.λ, cond=s.cond, residual=norm(M * x - rhs) / norm(rhs))
+    return solve_residual(F, b)
+end
+
+"""
+    consistency(dm, A, b; W=nothing)
+
+solve.jl:111-115 on the device: `(rows, columns, rhs)`, the largest absolute row and column sums of A [C; W] over its
+largest absolute entry, and |Σb| / Σ|b|.
+"""
+function consistency(dm::DOFMapping, A::AbstractMatrix, b::AbstractVector; W=nothing, ctx::Context=default_context())
+    nw = W === nothing ? 0 : size(W, 1)
+    size(A) == (dm.nslot, dm.nslot + nw) || throw(DimensionMismatch("A is $(size(A)), expected ($(dm.nslot), $(dm.nslot + nw))"))
+    length(b) == dm.nslot || throw(DimensionMismatch("b has $(length(b)) entries, expected $(dm.nslot)"))
+    Cr, Cc, Cv = csr(dm.C)
+    Wr, Wc, Wv = W === nothing ? (Cint[0], Cint[], Cdouble[]) : csr(sparse(W))
+    Ad, bd = Matrix{Cdouble}(A), Vector{Cdouble}(b)
+    out = zeros(Cdouble, 3)
+    GC.@preserve Cr Cc Cv Wr Wc Wv Ad bd out begin
+        k = VPConstraints(dm.nslot, dm.nq, nw, 0, pointer(Cr), pointer(Cc), pointer(Cv),
+            nw == 0 ? C_NULL : pointer(Wr), nw == 0 ? C_NULL : pointer(Wc), nw == 0 ? C_NULL : pointer(Wv),
+            C_NULL, C_NULL)
+        check(ctx, ccall((:vp_consistency, libvortexhip), Cint,
+            (Ptr{Cvoid}, Ptr{Cdouble}, Clonglong, Cint, Ptr{Cdouble}, Ref{VPConstraints}, Ptr{Cdouble}),
+            ctx.ptr, Ad, dm.nslot, 0, bd, Ref(k), out))
+    end
+    return (rows=out[1], columns=out[2], rhs=out[3])
 end
 
 # ---- N-body velocity sums (src/post/Post.jl:32-112, src/freewake/step.jl:176-187) --------------------------------

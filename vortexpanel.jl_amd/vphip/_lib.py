@@ -10,7 +10,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIBPATH = os.path.join(PKG, "lib", "libvortexhip.so")
 
-VP_OK, VP_E_ARGUMENT, VP_E_DIMENSION, VP_E_DEVICE, VP_E_SINGULAR = 0, -1, -2, -3, -4
+VP_OK, VP_E_ARGUMENT, VP_E_DIMENSION, VP_E_DEVICE, VP_E_SINGULAR, VP_E_CONVERGENCE = 0, -1, -2, -3, -4, -5
 
 
 class AssemblyOptionsC(C.Structure):
@@ -40,7 +40,8 @@ EXPORTS = ["vp_create", "vp_destroy", "vp_last_error", "vp_sync", "vp_assemble",
            "vp_surface_velocity", "vp_body_velocity", "vp_blob_velocities", "vp_plan_adjoint",
            "vp_factor_solve_transposed", "vp_plan_wake_product", "vp_plan_blob_rhs", "vp_gemm_tuning", "vp_math_probe",
            "vp_wake_field_create", "vp_wake_field_eval", "vp_wake_field_info", "vp_wake_field_destroy",
-           "vp_plan_update_wake", "vp_assembly_tuning"]
+           "vp_plan_update_wake", "vp_assembly_tuning", "vp_factor_mode",
+           "vp_consistency"]
 
 
 def load():

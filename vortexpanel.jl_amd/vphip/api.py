@@ -23,6 +23,10 @@ class VortexHipError(RuntimeError):
     pass
 
 
+class ConvergenceError(VortexHipError):
+    """FP64 iterative refinement did not meet its stopping test (VP_E_CONVERGENCE)"""
+
+
 @dataclass
 class AssemblyOptions:
     """src/assembly/assemble.jl:26-41 (`threaded` has no effect on the device); precision 'fp64' or 'mixed'"""
@@ -59,6 +63,8 @@ def _check(ctx, rc):
         msg = L.load().vp_last_error(ctx).decode() if ctx else "no context"
         if rc in (L.VP_E_ARGUMENT, L.VP_E_DIMENSION):
             raise ValueError(msg)
+        if rc == L.VP_E_CONVERGENCE:
+            raise ConvergenceError(f"vortexhip status {rc}: {msg}")
         raise VortexHipError(f"vortexhip status {rc}: {msg}")
 
 

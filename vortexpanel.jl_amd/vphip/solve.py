@@ -47,7 +47,9 @@ def _constraints(dm: DOFMapping, W):
 
 @dataclass
 class GalerkinFactorization:
-    """src/assembly/solve.jl:65-72: the device factorization handle plus se, C, nq, nb (cond on request)"""
+    """src/assembly/solve.jl:65-72: the device factorization handle plus se, C, nq, nb, and cond (estimated on its
+    first use and kept).  last_resid: the reference's relative residual |R - M X|_2 / |R|_2 of the last solve (its
+    worst right hand side)."""
     h: object
     ctx: Context
     se: np.ndarray
@@ -68,6 +70,14 @@ class GalerkinFactorization:
             _check(self.ctx.h, self.ctx.lib.vp_factor_cond(self.h, C.byref(c)))
             self._cond = c.value
         return self._cond
+
+    @property
+    def precision_info(self) -> tuple[str, int]:
+        """(the factors' current precision, FP64 fallbacks taken): mixed factors whose refinement does not converge
+        are replaced by an FP64 factorization of the same matrix (LAPACK dsgesv's fallback)"""
+        m, fb = C.c_int(), C.c_int()
+        _check(self.ctx.h, self.ctx.lib.vp_factor_mode(self.h, C.byref(m), C.byref(fb)))
+        return ("fp64" if m.value == 0 else "mixed"), fb.value
 
     def solve_bordered(self, R):
         """X = M^-1 R for R (n,) or (n, nrhs)"""
@@ -134,8 +144,10 @@ def factorize_galerkin(dm: DOFMapping, A, W=None, mode: str = "fp64", ctx: Conte
     return GalerkinFactorization(h=h, ctx=ctx, se=se[:nb], C=dm.C.tocsr(), nq=dm.nq, nb=nb, n=n.value, mode=mode)
 
 
-def solve_factored(F: GalerkinFactorization, b):
-    """`solve_galerkin(F, b)`, src/assembly/solve.jl:95-99; b (nslot,) or (nslot, nrhs) for a batch"""
+def solve_factored(F: GalerkinFactorization, b, cond: bool = True):
+    """`solve_galerkin(F, b)`, src/assembly/solve.jl:95-99; b (nslot,) or (nslot, nrhs) for a batch.  Returns q, mu,
+    lam and cond as the reference does (cond is estimated on the factorization's first solve; cond=False leaves it
+    out, for callers that do not return it), plus the refinement's iters and relative residual resid."""
     b = np.asarray(b, dtype=np.float64)
     one = b.ndim == 1
     B = b.reshape(b.shape[0], -1)
@@ -145,19 +157,46 @@ def solve_factored(F: GalerkinFactorization, b):
     q = X[:F.nq]
     out = dict(q=q[:, 0] if one else q, mu=(F.C @ q)[:, 0] if one else F.C @ q,
                lam=(F.se[:, None] * X[F.nq:F.nq + F.nb])[:, 0] if one else F.se[:, None] * X[F.nq:F.nq + F.nb],
-               cond=F._cond, iters=F.last_iters, resid=F.last_resid)
+               cond=F.cond if cond else None, iters=F.last_iters, resid=F.last_resid)
     return out
 
 
 def solve_galerkin(dm_or_F, A=None, b=None, W=None, mode: str = "fp64", ctx: Context | None = None, cond: bool = True):
-    """`solve_galerkin(dm, A, b; W)` (src/assembly/solve.jl:51-59) -> q, mu, lambda, cond, residual; or
-    `solve_galerkin(F, b)` when the first argument is a GalerkinFactorization."""
+    """`solve_galerkin(dm, A, b; W)` (src/assembly/solve.jl:51-59) -> q, mu, lambda, cond, residual (the relative
+    residual |M x - rhs|_2 / |rhs|_2 of the bordered system); or `solve_galerkin(F, b)` when the first argument is a
+    GalerkinFactorization.  cond=False skips the condition estimate (cond is then None)."""
     if isinstance(dm_or_F, GalerkinFactorization):
         return solve_factored(dm_or_F, A if b is None else b)
     F = factorize_galerkin(dm_or_F, A, W=W, mode=mode, ctx=ctx)
-    out = solve_factored(F, b)
-    if cond:
-        out["cond"] = F.cond
+    out = solve_factored(F, b, cond=cond)
     out["residual"] = F.last_resid
     F.close()
     return out
+
+
+def consistency(dm: DOFMapping, A, b, W=None, ctx: Context | None = None) -> dict:
+    """`consistency(dm, A, b; W)`, src/assembly/solve.jl:111-115, on the device: the largest absolute row sum and
+    column sum of A [C; W] over its largest absolute entry (rows, columns) and |sum b| / sum |b| (rhs).  A: host array
+    nslot x (nslot + nw) or an AssemblyPlan (its device matrix is read in place)."""
+    nw = 0 if W is None else W.shape[0]
+    k, keep, _ = _constraints(dm, W)
+    b = L.f64(b)
+    if b.shape != (dm.nslot,):
+        raise ValueError(f"b has shape {b.shape}, expected ({dm.nslot},)")
+    if isinstance(A, AssemblyPlan):
+        ctx = A.ctx
+        if A.nslot != dm.nslot or A.nw != nw:
+            raise ValueError(f"A is ({A.nslot}, {A.nslot + A.nw}), expected ({dm.nslot}, {dm.nslot + nw})")
+        dA, db, lda = C.c_void_p(), C.c_void_p(), C.c_longlong()
+        _check(ctx.h, ctx.lib.vp_plan_device_A(A.h, C.byref(dA), C.byref(db), C.byref(lda)))
+        Aptr, ondev, ld = C.cast(dA, C.POINTER(C.c_double)), 1, lda.value
+    else:
+        ctx = ctx or default_context()
+        A = np.asarray(A, dtype=np.float64)
+        if A.shape != (dm.nslot, dm.nslot + nw):
+            raise ValueError(f"A is {A.shape}, expected ({dm.nslot}, {dm.nslot + nw})")
+        keep["A"] = np.asfortranarray(A)
+        Aptr, ondev, ld = L.ptr(keep["A"]), 0, dm.nslot
+    out = np.zeros(3)
+    _check(ctx.h, ctx.lib.vp_consistency(ctx.h, Aptr, ld, ondev, L.ptr(b), C.byref(k), L.ptr(out)))
+    return dict(rows=float(out[0]), columns=float(out[1]), rhs=float(out[2]))

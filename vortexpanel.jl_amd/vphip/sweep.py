@@ -97,7 +97,7 @@ class AlphaSweep:
     def solve(self, alphas):
         import time
         t0 = time.perf_counter()
-        out = solve_factored(self.F, self.rhs(alphas))
+        out = solve_factored(self.F, self.rhs(alphas), cond=False)
         t1 = time.perf_counter()
         al = np.atleast_1d(np.asarray(alphas, dtype=np.float64))
         MU, Q = out["mu"].reshape(self.dm.nslot, -1), out["q"].reshape(self.dm.nq, -1)
