@@ -93,6 +93,61 @@ def blob_velocity(x, xp, al, delta):
     return u
 
 
+def source_velocity_many(a, b, c, xs, side=0, far=10.0, plane_tol=1e-12, mode=0):
+    """source_velocity at the points xs (n, 3) -> (n, 3)"""
+    xs = _f64(np.asarray(xs).reshape(-1, 3))
+    w = np.zeros_like(xs)
+    rc = lib().vpo_source_velocity_many(_p(_f64(a)), _p(_f64(b)), _p(_f64(c)), len(xs), _p(xs), C.c_int(side),
+                                        C.c_double(far), C.c_double(plane_tol), C.c_int(mode), _p(w))
+    if rc:
+        raise ValueError("degenerate triangle")
+    return w
+
+
+def segment_velocity_many(a, b, xs):
+    """segment_velocity at the points xs (n, 3) -> va, vb (n, 3) each"""
+    xs = _f64(np.asarray(xs).reshape(-1, 3))
+    va, vb = np.zeros_like(xs), np.zeros_like(xs)
+    lib().vpo_segment_velocity_many(_p(_f64(a)), _p(_f64(b)), len(xs), _p(xs), _p(va), _p(vb))
+    return va, vb
+
+
+RULE_KINDS = dict(classify=-1, far=0, near_base=1, self=2, edge0=3, edge1=4, edge2=5, vertex0=6, vertex1=7, vertex2=8,
+                  near=9, vertex_rule0=10, vertex_rule1=11, vertex_rule2=12, contact=13)
+
+
+def pair_geom(tx, sx, tid=(0, 1, 2), sid=(3, 4, 5), kind="classify", sigma=0.0, self_pair=False, o: Opts | None = None):
+    """The rule and pair_block of test triangle tx and source triangle sx (3 corners each), shared corners given by
+    equal node ids: kind 'classify' (pair_rule; wake_pair_rule when no id is shared) or a fixed rule (RULE_KINDS).
+    Returns (tier or None, lam (n, 3), w (n,), blk (3, 3), rb (3,)), or None when kind 'contact' gives no rule."""
+    o = o or opts()
+    tier = C.c_int(-1)
+    tx, sx = _f64(np.asarray(tx).reshape(3, 3)), _f64(np.asarray(sx).reshape(3, 3))
+    ti, si = _i32(tid), _i32(sid)
+    args = (_p(tx), _p(sx), _p(ti, C.c_int), _p(si, C.c_int), int(self_pair), RULE_KINDS[kind], C.byref(o),
+            C.c_double(sigma), C.byref(tier))
+    n = lib().vpo_pair_geom(*args, 0, None, None, None, None)
+    if n == -2:
+        return None
+    if n < 0:
+        raise ValueError(f"vpo_pair_geom failed ({n})")
+    lam, w, blk, rb = np.zeros((n, 3)), np.zeros(n), np.zeros(9), np.zeros(3)
+    lib().vpo_pair_geom(*args, n, _p(lam), _p(w), _p(blk), _p(rb))
+    return (TIER_NAMES[tier.value] if tier.value >= 0 else None), lam, w, blk.reshape(3, 3), rb
+
+
+def segment_geom(tx, fa, fb, o: Opts | None = None):
+    """segment_rule and segment_block of test triangle tx and the filament fa -> fb: (tier, npoints, ca, cb)"""
+    o = o or opts()
+    tier = C.c_int(-1)
+    ca, cb = np.zeros(3), np.zeros(3)
+    n = lib().vpo_segment_geom(_p(_f64(np.asarray(tx).reshape(3, 3))), _p(_f64(fa)), _p(_f64(fb)), C.byref(o),
+                               C.byref(tier), _p(ca), _p(cb))
+    if n < 0:
+        raise ValueError(f"vpo_segment_geom failed ({n})")
+    return TIER_NAMES[tier.value], n, ca, cb
+
+
 def rule(kind: int, o: Opts | None = None):
     """kind 0 far, 1 near, 2 self, 3..5 edge k, 6..8 vertex k (TierRules, src/elements/rules.jl:161-174)"""
     o = o or opts()
@@ -140,6 +195,23 @@ def assemble(g, dm, sigma, vinf, o: Opts | None = None, wake=None, nthreads=0, t
         raise ValueError(f"vpo_assemble failed ({rc})")
     out = (A.T, b, dict(zip(TIER_NAMES, pairs.tolist())), dict(zip(TIER_NAMES, points.tolist())))
     return out + (tr,) if tiers else out
+
+
+def classify(g, o: Opts | None = None, wake=None, nthreads=0, tiers=True):
+    """assemble's classification alone (pair_rule, wake_pair_rule, segment_rule; no pair blocks): the tier of every
+    (body test triangle, source triangle) pair (nt, nt + wake triangles) uint8 and the pairs and points per tier"""
+    o = o or opts()
+    nwx, wx, nwt, wt, nwf, wf = _wake_arrays(wake)
+    pairs = np.zeros(6, np.int64)
+    points = np.zeros(6, np.int64)
+    tp = _i32(g.touch_pairs.reshape(-1, 2)) if len(g.touch_pairs) else np.zeros((1, 2), np.int32)
+    tr = np.zeros((g.nt, g.nt + nwt), np.uint8) if tiers else None
+    rc = lib().vpo_classify(g.nn, _p(_f64(g.x)), g.nt, _p(_i32(g.tris), C.c_int), len(g.touch_pairs), _p(tp, C.c_int),
+                            nwx, _p(wx), nwt, _p(wt, C.c_int), nwf, _p(wf, C.c_int), C.byref(o), nthreads,
+                            _p(pairs, C.c_longlong), _p(points, C.c_longlong), _p(tr, C.c_ubyte) if tiers else None)
+    if rc:
+        raise ValueError(f"vpo_classify failed ({rc})")
+    return tr, dict(zip(TIER_NAMES, pairs.tolist())), dict(zip(TIER_NAMES, points.tolist()))
 
 
 def assemble_rows(g, dm, sigma, vinf, t_begin, t_end, o: Opts | None = None, wake=None, nthreads=0, full=False):

@@ -943,6 +943,76 @@ int vpo_assemble(int nn, const double *x, int nt, const int *tris, const int *co
                              nwf, wfil, o, nthreads, A, bvec, pairs, points, tiers, 0, nt);
 }
 
+/* The classification of assemble alone (pair_rule, wake_pair_rule and segment_rule per pair, no pair blocks): the
+ * tier of every (body test triangle, source triangle) pair into tiers (nt x (nt + nwt), when not null) and the pair and
+ * point counts per tier, with the same rules and arithmetic as vpo_assemble; for full-size tier checks. */
+int vpo_classify(int nn, const double *x, int nt, const int *tris, int ntouch, const int *touch, int nwx,
+                 const double *wx, int nwt, const int *wtris, int nwf, const int *wfil, const vpo_opts *o,
+                 int nthreads, long long *pairs, long long *points, unsigned char *tiers)
+{
+    tier_rules_t R;
+    if (tier_rules(o, &R)) return -3;
+    el_t *els = (el_t *)malloc(sizeof(el_t) * (nt > 0 ? nt : 1));
+    el_t *wel = (el_t *)malloc(sizeof(el_t) * (nwt > 0 ? nwt : 1));
+    int rc = build_els(nn, x, nt, tris, els);
+    if (!rc && nwt) rc = build_els(nwx, wx, nwt, wtris, wel);
+    int *tp = NULL;
+    if (!rc && ntouch) {
+        tp = (int *)malloc(sizeof(int) * 4 * ntouch);
+        for (int i = 0; i < ntouch; i++) {
+            tp[4 * i] = touch[2 * i]; tp[4 * i + 1] = touch[2 * i + 1];
+            tp[4 * i + 2] = touch[2 * i + 1]; tp[4 * i + 3] = touch[2 * i];
+        }
+        qsort(tp, 2 * ntouch, sizeof(int) * 2, cmp_pair);
+    }
+    if (rc) { free(els); free(wel); free(tp); tier_rules_free(&R); return rc; }
+    for (int k = 0; k < 6; k++) pairs[k] = points[k] = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel
+    {
+        long long mp[6] = {0}, mq[6] = {0};
+        rule_t tmp;
+        rule_init(&tmp);
+#pragma omp for schedule(dynamic, 16)
+        for (int t = 0; t < nt; t++) {
+            const el_t *T = els + t;
+            const int *tri = tris + 3 * t;
+            for (int s = 0; s < nt; s++) {
+                int tier;
+                const rule_t *r = pair_rule(T, els + s, t, s, tri, tris + 3 * s, &R, o, is_touch(tp, 2 * ntouch, t, s), &tmp, &tier);
+                mp[tier]++;
+                mq[tier] += r->n;
+                if (tiers) tiers[(size_t)t * (nt + nwt) + s] = (unsigned char)tier;
+            }
+            for (int s = 0; s < nwt; s++) {
+                const int *wt = wtris + 3 * s;
+                int gs[3] = {wt[0] + nn, wt[1] + nn, wt[2] + nn};
+                int tier;
+                const rule_t *r = pair_rule(T, wel + s, t, -1 - s, tri, gs, &R, o, 0, &tmp, &tier);
+                mp[tier]++;
+                mq[tier] += r->n;
+                if (tiers) tiers[(size_t)t * (nt + nwt) + nt + s] = (unsigned char)tier;
+            }
+            for (int f = 0; f < nwf; f++) {
+                int tier;
+                const rule_t *r = segment_rule(T, ld3(wx + 3 * wfil[2 * f]), ld3(wx + 3 * wfil[2 * f + 1]), &R, o, &tmp, &tier);
+                mp[tier]++;
+                mq[tier] += r->n;
+            }
+        }
+#pragma omp critical(vpo_cstats)
+        for (int k = 0; k < 6; k++) { pairs[k] += mp[k]; points[k] += mq[k]; }
+        rule_free(&tmp);
+    }
+    free(els);
+    free(wel);
+    free(tp);
+    tier_rules_free(&R);
+    return 0;
+}
+
 /* One pair block, for the per-pair parity tests: tier, block and RHS of test triangle t and source triangle s
  * (s >= nt: wake triangle s - nt).  Same rules and classification as vpo_assemble. */
 int vpo_pair(int nn, const double *x, int nt, const int *tris, const double *sigma, int nwx, const double *wx,
@@ -980,6 +1050,116 @@ int vpo_pair(int nn, const double *x, int nt, const int *tris, const double *sig
     rule_free(&tmp);
     tier_rules_free(&R);
     return tier;
+}
+
+/* ----------------------------------------------------------- test exports ---------------------------------- */
+
+/* source_velocity at n points (xs, n x 3 -> w, n x 3), for cubature references in the tests */
+int vpo_source_velocity_many(const double *a, const double *b, const double *c, int n, const double *xs, int side,
+                             double far, double plane_tol, int mode, double *w)
+{
+    src_t s;
+    if (make_src(ld3(a), ld3(b), ld3(c), &s)) return -1;
+    for (int i = 0; i < n; i++) {
+        v3 x = ld3(xs + 3 * i);
+        v3 r = mode == 1 ? sv_exact(&s, x, side, plane_tol) : mode == 2 ? sv_far(&s, x) : sv(&s, x, side, far, plane_tol);
+        st3(w + 3 * i, r);
+    }
+    return 0;
+}
+
+/* segment_velocity at n points: va, vb (n x 3 each) */
+void vpo_segment_velocity_many(const double *a, const double *b, int n, const double *xs, double *va, double *vb)
+{
+    for (int i = 0; i < n; i++) {
+        v3 p, q;
+        seg_vel(ld3(a), ld3(b), ld3(xs + 3 * i), &p, &q);
+        st3(va + 3 * i, p);
+        st3(vb + 3 * i, q);
+    }
+}
+
+/* The rule and pair_block of a test triangle T (tx: 3 corners) and a source triangle S (sx), given directly, with node
+ * ids tid and sid (equal ids are shared corners; a wake source's ids match no body node, which makes pair_rule the
+ * reference's wake_pair_rule).  self_pair: S is T (side +1).  kind -1: the classification (pair_rule, tier set);
+ * otherwise a fixed rule: 0 far, 1 near (the stored 7-point base), 2 self, 3..5 edge k, 6..8 vertex k (stored), 9
+ * near_rule (the recursive split), 10..12 vertex_rule k (wide corners in wedges), 13 contact_rule (-2 when it gives
+ * none).  Returns the rule's point count; fills up to cap points into lam (3 each) and w, and blk (3x3 row-major, test
+ * corner by source corner) and rb of pair_block with source strength sig. */
+int vpo_pair_geom(const double *tx, const double *sx, const int *tid, const int *sid, int self_pair, int kind,
+                  const vpo_opts *o, double sig, int *tier, int cap, double *lam, double *w, double *blk, double *rb)
+{
+    tier_rules_t R;
+    if (tier_rules(o, &R)) return -3;
+    el_t T, S;
+    if (make_el(ld3(tx), ld3(tx + 3), ld3(tx + 6), &T) || make_el(ld3(sx), ld3(sx + 3), ld3(sx + 6), &S)) {
+        tier_rules_free(&R);
+        return -1;
+    }
+    rule_t tmp;
+    rule_init(&tmp);
+    const rule_t *r = NULL;
+    int shared[3];
+    for (int k = 0; k < 3; k++) shared[k] = tid[k] == sid[0] || tid[k] == sid[1] || tid[k] == sid[2];
+    *tier = -1;
+    if (kind == -1) {
+        r = pair_rule(&T, &S, 0, self_pair ? 0 : 1, tid, sid, &R, o, 0, &tmp, tier);
+    } else if (kind == 0) {
+        r = &R.far;
+    } else if (kind == 1) {
+        r = &R.near;
+    } else if (kind == 2) {
+        r = &R.self;
+    } else if (kind <= 5) {
+        r = &R.edge[kind - 3];
+    } else if (kind <= 8) {
+        r = &R.vertex[kind - 6];
+    } else if (kind == 9) {
+        srcref_t sr = {0, &S, {0, 0, 0}, {0, 0, 0}};
+        near_rule(&tmp, &T, &sr, &R.near, o->near_ratio, o->near_depth);
+        r = &tmp;
+    } else if (kind <= 12) {
+        r = vertex_rule(&T, kind - 10, &R, o, &tmp);
+    } else if (kind == 13) {
+        if (contact_rule(&T, &S, shared, &R, o, &tmp)) r = &tmp;
+    }
+    int n = -2;
+    if (r) {
+        n = r->n;
+        for (int i = 0; i < n && i < cap; i++) {
+            if (lam) memcpy(lam + 3 * i, r->l + 3 * i, sizeof(double) * 3);
+            if (w) w[i] = r->w[i];
+        }
+        double b3[3][3], r3[3];
+        pair_block(&T, &S, r, sig, o->test, self_pair ? 1 : 0, o->kernel_far, b3, r3);
+        if (blk) memcpy(blk, b3, sizeof(double) * 9);
+        if (rb) memcpy(rb, r3, sizeof(double) * 3);
+    }
+    rule_free(&tmp);
+    tier_rules_free(&R);
+    return n;
+}
+
+/* The filament rows of a test triangle T (tx) for a filament from fa to fb: segment_rule (tier set) and
+ * segment_block's ca, cb (src/assembly/assemble.jl:299-321); returns the rule's point count */
+int vpo_segment_geom(const double *tx, const double *fa, const double *fb, const vpo_opts *o, int *tier, double *ca,
+                     double *cb)
+{
+    tier_rules_t R;
+    if (tier_rules(o, &R)) return -3;
+    el_t T;
+    if (make_el(ld3(tx), ld3(tx + 3), ld3(tx + 6), &T)) {
+        tier_rules_free(&R);
+        return -1;
+    }
+    rule_t tmp;
+    rule_init(&tmp);
+    const rule_t *r = segment_rule(&T, ld3(fa), ld3(fb), &R, o, &tmp, tier);
+    const int n = r->n;
+    segment_block(&T, ld3(fa), ld3(fb), r, o->test, ca, cb);
+    rule_free(&tmp);
+    tier_rules_free(&R);
+    return n;
 }
 
 /* ------------------------------------------------------------- velocities ---------------------------------- */

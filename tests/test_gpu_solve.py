@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from vphip import _lib as L, api, dofmap, mesh, solve, wake as wk
+from vphip import _lib as L, api, dofmap, mesh, post, solve, wake as wk
 
 pytestmark = pytest.mark.gpu
 
@@ -255,3 +255,31 @@ def test_consistency_matches_oracle():
         got = solve.consistency(dmw, src, bw, W)
         for key in ("rows", "columns", "rhs"):
             assert abs(got[key] - ref[key]) < 1e-14 + 1e-9 * ref[key], (key, got[key], ref[key])
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_sphere_exact_flow_configured(mode):
+    """The icosphere(3) sphere at its configured freestream V = (1, 0, 0) on the device path (assembly, bordered solve,
+    gradient-route Cp) against the exact flow mu = V.x/2 and Cp = 1 - (9/4) sin^2(theta), at the reference's published
+    accuracy (README.md:210-217: mu L2 5.8e-4, max 1.9e-3 of the doublet scale 1/2, Cp L2 9.8e-3, cond 640-650,
+    lambda ~ 0), and against the oracle's solution"""
+    V = np.array([1.0, 0.0, 0.0])
+    g = mesh.icosphere(3)
+    dm = dofmap.build_dofmap(g)
+    els = api.build_elements(g)
+    sig = api.source_strengths(els, V)
+    print(f"sphere exact flow V=(1,0,0) mode={mode}", flush=True)
+    A, b, _ = api.assemble(g, dm, els, sig, V, api.AssemblyOptions(precision=mode))
+    sol = solve.solve_galerkin(dm, A, b, mode=mode)
+    err = sol["mu"] - g.x[dm.slot_node] @ V / 2
+    err -= np.sum(dm.slot_area * err) / np.sum(dm.slot_area)
+    muL2 = np.sqrt(np.sum(dm.slot_area * err ** 2) / np.sum(dm.slot_area)) / 0.5
+    cp = post.pressure_coefficient(post.gradient_velocity(els, dm, sol["mu"], V), V)
+    c = els.centroid / np.linalg.norm(els.centroid, axis=1)[:, None]
+    cpL2 = np.sqrt(np.sum(els.area * (cp - (1 - 9 / 4 * (1 - (c @ V) ** 2))) ** 2) / np.sum(els.area))
+    assert abs(muL2 - 5.8e-4) < 0.05e-4 and np.abs(err).max() / 0.5 < 1.9e-3, (muL2, np.abs(err).max() / 0.5)
+    assert abs(cpL2 - 9.8e-3) < 0.3e-3, cpL2
+    assert 640 < sol["cond"] < 650 and abs(sol["lam"][0]) < 1e-14 and sol["residual"] < 1e-12
+    Ao, bo, _, _ = O.assemble(g, dm, O.source_strengths(O.elements(g.x, g.tris), V), V)
+    ref = O.solve_galerkin(dm, Ao, bo)
+    assert np.abs(sol["mu"] - ref["mu"]).max() < (1e-11 if mode == "fp64" else 1e-7) * np.abs(ref["mu"]).max()
