@@ -211,6 +211,21 @@ int vp_body_velocity(vp_ctx *ctx, const vp_surface *g, const double *sigma, cons
 int vp_blob_velocities(vp_ctx *ctx, int nb, const double *bx, const double *alpha, const double *delta, int np,
                        const double *xs, int precision, double *u);
 
+/* vp_blob_velocities on device arrays, allocating nothing.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops or harnesses that forbid hipMalloc).
+ * bx, alpha (nb x 3), delta (nb), xs (np x 3) and u (np x 3) are device pointers on the context's device; the work is
+ * queued on the context's stream and not waited for (vp_sync waits).  The partial sums live in one fixed 32 MiB device
+ * array per GPU, held until the process ends: np is at most 1,398,101 (VP_E_DIMENSION beyond).  Every call on a GPU
+ * shares that array, so calls that overlap on one GPU (from other contexts or threads) race. */
+int vp_blob_velocities_device(vp_ctx *ctx, int nb, const double *bx, const double *alpha, const double *delta, int np,
+                              const double *xs, int precision, double *u);
+/* Device buffers on the context's device for the device-pointer entry points (hipMalloc / hipFree), and blocking copies
+ * to and from them on the context's stream. */
+int vp_device_alloc(vp_ctx *ctx, long long bytes, void **out);
+int vp_device_free(vp_ctx *ctx, void *p);
+int vp_copy_to_device(vp_ctx *ctx, void *dst, const void *src, long long bytes);
+int vp_copy_from_device(vp_ctx *ctx, void *dst, const void *src, long long bytes);
+
 /* The free-wake step's velocity sums, `wake_velocities` (src/freewake/step.jl:166-173): base (V∞, or null for 0)
  * + the body's sources and sheets (as vp_body_velocity) + nb blobs (as vp_blob_velocities) at np points, as one HIP
  * graph: the body and blob fields are independent nodes joined by one reduction.  The body's elements are set up at

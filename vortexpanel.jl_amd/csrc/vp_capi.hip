@@ -58,6 +58,9 @@ int vp_create(int device, vp_ctx **out)
         delete c;
         return VP_E_DEVICE;
     }
+    // the device address of the no-allocation entry point's scratch array, once per device (a failure is reported
+    // by that entry point, which alone needs it)
+    (void)vp::noalloc_init(device);
     *out = c;
     return VP_OK;
 }
@@ -420,6 +423,65 @@ int vp_blob_velocities(vp_ctx *ctx, int nb, const double *bx, const double *alph
         (void)hipSetDevice(ctx->c.device);
         vp::blob_velocities_host(&ctx->c, nb, bx, alpha, delta, np, xs, precision, u);
     });
+}
+
+int vp_blob_velocities_device(vp_ctx *ctx, int nb, const double *bx, const double *alpha, const double *delta, int np,
+                              const double *xs, int precision, double *u)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if (nb < 0 || np < 0) return fail(&ctx->c, VP_E_DIMENSION, "negative size");
+    if ((size_t)np * 3 > vp::NOALLOC_DOUBLES)
+        return fail(&ctx->c, VP_E_DIMENSION,
+                    "np = " + std::to_string(np) + " targets exceed the no-allocation scratch array (at most " +
+                        std::to_string(vp::NOALLOC_DOUBLES / 3) + ")");
+    if ((nb > 0 && (!bx || !alpha || !delta)) || (np > 0 && (!xs || !u)))
+        return fail(&ctx->c, VP_E_ARGUMENT, "null argument");
+    if (precision != 0 && precision != 1) return fail(&ctx->c, VP_E_ARGUMENT, "precision must be 0 (fp64) or 1 (mixed)");
+    if (np == 0) return VP_OK;
+    return guard(&ctx->c, [&] {
+        (void)hipSetDevice(ctx->c.device);
+        vp::blob_velocities_noalloc(&ctx->c, nb, bx, alpha, delta, np, xs, precision, u);
+    });
+}
+
+int vp_device_alloc(vp_ctx *ctx, long long bytes, void **out)
+{
+    if (!ctx || !out) return VP_E_ARGUMENT;
+    *out = nullptr;
+    if (bytes <= 0) return fail(&ctx->c, VP_E_ARGUMENT, "bytes must be positive");
+    (void)hipSetDevice(ctx->c.device);
+    if (hipMalloc(out, (size_t)bytes) != hipSuccess) return fail(&ctx->c, VP_E_DEVICE, "device allocation failed");
+    return VP_OK;
+}
+
+int vp_device_free(vp_ctx *ctx, void *p)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if (!p) return VP_OK;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    if (hipFree(p) != hipSuccess) return fail(&ctx->c, VP_E_DEVICE, "device free failed");
+    return VP_OK;
+}
+
+int vp_copy_to_device(vp_ctx *ctx, void *dst, const void *src, long long bytes)
+{
+    if (!ctx || !dst || !src || bytes < 0) return VP_E_ARGUMENT;
+    (void)hipSetDevice(ctx->c.device);
+    if (hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, ctx->c.stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->c.stream) != hipSuccess)
+        return fail(&ctx->c, VP_E_DEVICE, "copy to the device failed");
+    return VP_OK;
+}
+
+int vp_copy_from_device(vp_ctx *ctx, void *dst, const void *src, long long bytes)
+{
+    if (!ctx || !dst || !src || bytes < 0) return VP_E_ARGUMENT;
+    (void)hipSetDevice(ctx->c.device);
+    if (hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, ctx->c.stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->c.stream) != hipSuccess)
+        return fail(&ctx->c, VP_E_DEVICE, "copy from the device failed");
+    return VP_OK;
 }
 
 int vp_wake_field_create(vp_ctx *ctx, const vp_surface *g, double kernel_far, int precision, vp_wake_field **out)

@@ -148,6 +148,12 @@ void surface_velocity_host(Ctx *c, const vp_surface &g, const vp_wake *w, const 
                            const double *mu_wake, const double *vinf, double kernel_far, int precision, double *u);
 void body_velocity_host(Ctx *c, const vp_surface &g, const double *sigma, const double *gam, int np, const double *xs,
                         double kernel_far, int precision, double *u);
+// The no-allocation blob sums (vp_blob_velocities_device): the size of the per-GPU scratch array in doubles (32 MiB),
+// its device address per device (taken at vp_create), and the sums on device arrays.
+constexpr size_t NOALLOC_DOUBLES = (size_t)1 << 22;
+hipError_t noalloc_init(int device);
+void blob_velocities_noalloc(Ctx *c, int nb, const double *bx, const double *ba, const double *bd, int np,
+                             const double *xs, int precision, double *u);
 void blob_velocities_host(Ctx *c, int nb, const double *bx, const double *ba, const double *bd, int np,
                           const double *xs, int precision, double *u);
 // The free-wake step's velocity sums (the body's kernels and the wake's blobs at np targets, plus a base velocity)

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -599,6 +600,66 @@ void math_probe_host(Ctx *c, int fn, int n, const double *a, const double *b, do
     FCHECK(hipGetLastError());
     FCHECK(hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     FCHECK(hipStreamSynchronize(st));
+}
+
+// ------------------------------------------------------------------------------------------ no-allocation variant --
+// vp_blob_velocities_device keeps its per-chunk partial sums in `scratch`, a fixed array that each GPU holds for the
+// process.  In host code the array's name is not a device address: the address comes from hipGetSymbolAddress, taken
+// once per device with that device current (at vp_create) and kept in scratch_dev; only that pointer is passed to
+// kernels.  Calls on one GPU share the array, so calls that overlap on a GPU race.
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+std::mutex scratch_mu;
+constexpr int MAX_DEVICES = 64;
+double *scratch_dev[MAX_DEVICES] = {};
+}  // namespace
+
+hipError_t noalloc_init(int device)
+{
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    if (scratch_dev[device]) return hipSuccess;
+    double *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) scratch_dev[device] = p;
+    return e;
+}
+
+// The blob sums of blob_velocities_host on device arrays, with no allocation: the chunking of the allocating path
+// (the same partial sums, added in the same order) unless it would not fit the scratch array, when the blobs are split
+// into fewer, longer chunks.  Queued on the context's stream, not waited for.
+void blob_velocities_noalloc(Ctx *c, int nb, const double *bx, const double *ba, const double *bd, int np,
+                             const double *xs, int precision, double *u)
+{
+    double *scr = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(scratch_mu);
+        if (c->device >= 0 && c->device < MAX_DEVICES) scr = scratch_dev[c->device];
+    }
+    if (!scr) throw std::runtime_error("the scratch array's device address is not set (hipGetSymbolAddress failed at vp_create)");
+    if ((size_t)np * 3 > NOALLOC_DOUBLES)
+        throw std::length_error("np = " + std::to_string(np) + " targets exceed the no-allocation scratch array (at most " +
+                                std::to_string(NOALLOC_DOUBLES / 3) + ")");
+    if (np == 0) return;
+    hipStream_t st = c->stream;
+    int chunk = chunk_of(nb, np, BT);
+    int nchunk = nb > 0 ? (nb + chunk - 1) / chunk : 0;
+    const size_t cap = NOALLOC_DOUBLES / ((size_t)np * 3);
+    if ((size_t)nchunk > cap) {
+        const int ntiles = (nb + BT - 1) / BT, per = (int)((ntiles + cap - 1) / cap);
+        chunk = per * BT;
+        nchunk = (nb + chunk - 1) / chunk;
+    }
+    if (nchunk) {
+        const dim3 grid((np + FT - 1) / FT, nchunk);
+        if (precision == 1)
+            blob_field<float><<<grid, FT, 0, st>>>(bx, ba, bd, nb, chunk, xs, np, scr);
+        else
+            blob_field<double><<<grid, FT, 0, st>>>(bx, ba, bd, nb, chunk, xs, np, scr);
+        FCHECK(hipGetLastError());
+    }
+    field_reduce_device(st, scr, nchunk, np, nullptr, nullptr, nullptr, nullptr, 0, xs, 0, u);
 }
 
 }  // namespace vp

@@ -41,7 +41,8 @@ EXPORTS = ["vp_create", "vp_destroy", "vp_last_error", "vp_sync", "vp_assemble",
            "vp_factor_solve_transposed", "vp_plan_wake_product", "vp_plan_blob_rhs", "vp_gemm_tuning", "vp_math_probe",
            "vp_wake_field_create", "vp_wake_field_eval", "vp_wake_field_info", "vp_wake_field_destroy",
            "vp_plan_update_wake", "vp_assembly_tuning", "vp_factor_mode",
-           "vp_consistency"]
+           "vp_consistency", "vp_blob_velocities_device", "vp_device_alloc", "vp_device_free", "vp_copy_to_device",
+           "vp_copy_from_device"]
 
 
 def load():
