@@ -373,6 +373,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // ------------------------------------------------------------------------------------------ filaments --------------
 
 // segment_rule's far test (src/assembly/assemble.jl:299-301) and the far segment_block (:312-321)
+// An upper bound of the entries the far pass appends to the non-far list: far_pairs' test norm(c_S - c_T) >
+// far_ratio max(diam_T, diam_S) over the sources [s_begin, s_end), and seg_pairs' with a filament's midpoint and length,
+// taken on squared distances with a relative margin of 1e-12 (the FP64 tests' rounding is a few ulps), so that no pair
+// they keep is missed.  Thread per test triangle, sources in LDS tiles over grid.y chunks; one atomic per workgroup.
+constexpr int BOUND_CHUNK = 4096;
+__global__ __launch_bounds__(256) void nonfar_bound(const Elem *__restrict__ el, int nt, int s_begin, int s_end,
+                                                    const double *__restrict__ seg, int nseg, double far_ratio,
+                                                    unsigned long long *__restrict__ count)
+{
+    __shared__ double S[256][4];
+    __shared__ unsigned int part[4];
+    const int t = blockIdx.x * 256 + threadIdx.x, ne = s_end - s_begin, ns = ne + nseg;
+    const bool live = t < nt;
+    const V3<double> tc = live ? ld3(el[t].c) : mk3(0.0, 0.0, 0.0);
+    const double td = live ? el[t].diam : 0.0;
+    const int j0 = blockIdx.y * BOUND_CHUNK, j1 = min(ns, j0 + BOUND_CHUNK);
+    unsigned int c = 0;
+    for (int base = j0; base < j1; base += 256) {
+        __syncthreads();
+        const int j = base + threadIdx.x;
+        if (j < j1) {
+            if (j < ne) {
+                const Elem &E = el[s_begin + j];
+                S[threadIdx.x][0] = E.c[0];
+                S[threadIdx.x][1] = E.c[1];
+                S[threadIdx.x][2] = E.c[2];
+                S[threadIdx.x][3] = E.diam;
+            } else {
+                const V3<double> fa = ld3(seg + 6 * (j - ne)), fb = ld3(seg + 6 * (j - ne) + 3);
+                S[threadIdx.x][0] = (fa.x + fb.x) / 2;
+                S[threadIdx.x][1] = (fa.y + fb.y) / 2;
+                S[threadIdx.x][2] = (fa.z + fb.z) / 2;
+                S[threadIdx.x][3] = cls_norm(fb - fa);
+            }
+        }
+        __syncthreads();
+        const int nk = min(256, j1 - base);
+        if (live)
+            for (int k = 0; k < nk; k++) {
+                const double dx = S[k][0] - tc.x, dy = S[k][1] - tc.y, dz = S[k][2] - tc.z;
+                const double r = far_ratio * fmax(td, S[k][3]);
+                c += dx * dx + dy * dy + dz * dz <= r * r * (1.0 + 1e-12);
+            }
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long tot = (unsigned long long)part[0] + part[1] + part[2] + part[3];
+        if (tot) atomicAdd(count, tot);
+    }
+}
+
 __global__ void seg_pairs(AsmParams P, const TestFar *__restrict__ tf, const int *__restrict__ corner_slot,
                           const double *__restrict__ seg, const int *__restrict__ segcol, double *__restrict__ A,
                           int2 *__restrict__ near_list, unsigned int *__restrict__ near_count, unsigned int near_cap,
@@ -1953,11 +2006,15 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
     d_stats = dalloc<unsigned long long>(32);
     d_count = dalloc<unsigned int>(7);
     owned.insert(owned.end(), {d_b, d_sigma, d_stats, d_count});
-    // The non-far list's first budget, 384 pairs per test triangle (48 bytes each over the five lists): the abused
-    // ellipsoid needs 244, so its first assembly runs no second, enumerate-only far pass; it grows on demand (the
-    // swept wing's long panels need more).  VP_NONFAR_CAP sets it, for tests of the regrowth.
+    // The non-far list's first budget, 384 pairs per test triangle (48 bytes each over the five lists; the abused
+    // ellipsoid needs 244).  The first assembly grows it to an upper bound of the count (nonfar_bound) before its far
+    // pass (the swept wing's long panels need ~1,130); later assemblies grow it on demand, refilling the lists by an
+    // enumerate-only far pass.  VP_NONFAR_CAP sets it and skips the bound, for tests of the regrowth.
     nonfar_cap = (unsigned int)std::min<long long>((long long)nt * 384 + 1024, 1ll << 31);
-    if (const char *cap = std::getenv("VP_NONFAR_CAP")) nonfar_cap = (unsigned int)std::max(1ll, std::atoll(cap));
+    if (const char *cap = std::getenv("VP_NONFAR_CAP")) {
+        nonfar_cap = (unsigned int)std::max(1ll, std::atoll(cap));
+        nonfar_cap_forced = true;  // the tests of list regrowth: no sizing pass
+    }
     leaf_cap = (unsigned int)std::min<long long>((long long)nt * 24 + 1024, 1ll << 31);
     group_cap = (unsigned int)std::min<long long>((long long)nt * 24 + 1024, 1ll << 31);
     if (const char *cap = std::getenv("VP_LEAF_CAP")) leaf_cap = group_cap = (unsigned int)std::max(1ll, std::atoll(cap));
@@ -2044,25 +2101,48 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
             HIPCHECK(hipGetLastError());
         }
     };
-    far_pass(false);
-    if (timing) HIPCHECK(hipEventRecord(ev[1], st));
-    unsigned int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
-    HIPCHECK(hipMemcpyAsync(cnt, d_count, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    if (cnt[0] > nonfar_cap) {  // more non-far pairs than budgeted: grow every pair list and enumerate again
+    // every pair list grown to hold cap entries (contents dropped)
+    auto grow_pair_lists = [&](unsigned int cap) {
+        HIPCHECK(hipStreamSynchronize(st));
         for (int2 **pp : {&d_nonfar, &d_touchp, &d_near, &d_nearm}) {
             owned.erase(std::find(owned.begin(), owned.end(), (void *)*pp));
             (void)hipFree(*pp);
         }
         owned.erase(std::find(owned.begin(), owned.end(), (void *)d_simple));
         (void)hipFree(d_simple);
-        nonfar_cap = cnt[0] + 1024;
+        nonfar_cap = cap;
         d_nonfar = dalloc<int2>(nonfar_cap);
         d_touchp = dalloc<int2>(nonfar_cap);
         d_near = dalloc<int2>(nonfar_cap);
         d_nearm = dalloc<int2>(nonfar_cap);
         d_simple = dalloc<int4>(nonfar_cap);
         owned.insert(owned.end(), {d_nonfar, d_touchp, d_near, d_nearm, d_simple});
+    };
+    if (!lists_sized && !nonfar_cap_forced) {
+        // the plan's first assembly: the lists sized from an upper bound of the non-far count, so that the far pass
+        // never has to run again in its enumerate-only mode (a wing's slender panels have ~1,100 non-far pairs each)
+        const int s_begin = wake_only ? nt : 0;
+        const int nsb = (nsrc - s_begin) + nseg;
+        if (nt > 0 && nsb > 0) {
+            HIPCHECK(hipMemsetAsync(d_stats + 15, 0, sizeof(unsigned long long), st));
+            nonfar_bound<<<dim3((nt + 255) / 256, (nsb + BOUND_CHUNK - 1) / BOUND_CHUNK), 256, 0, st>>>(
+                d_el, nt, s_begin, nsrc, d_seg, nseg, opts.far_ratio, d_stats + 15);
+            HIPCHECK(hipGetLastError());
+            unsigned long long bound = 0;
+            HIPCHECK(hipMemcpyAsync(&bound, d_stats + 15, sizeof(bound), hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+            if (bound > 0xFFFF0000ull) throw std::length_error("more than 2^32 non-far pairs");
+            if (bound > nonfar_cap) grow_pair_lists((unsigned int)bound + 1024);
+        }
+        lists_sized = true;
+    }
+    far_pass(false);
+    if (timing) HIPCHECK(hipEventRecord(ev[1], st));
+    unsigned int cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+    HIPCHECK(hipMemcpyAsync(cnt, d_count, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (cnt[0] > nonfar_cap) {  // more non-far pairs than budgeted: grow every pair list and enumerate again
+        grow_pair_lists(cnt[0] + 1024);
         HIPCHECK(hipMemsetAsync(d_count, 0, sizeof(unsigned int), st));
         far_pass(true);
         HIPCHECK(hipMemcpyAsync(cnt, d_count, sizeof(unsigned int), hipMemcpyDeviceToHost, st));

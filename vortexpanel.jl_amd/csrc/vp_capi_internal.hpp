@@ -77,6 +77,8 @@ struct Plan {
     unsigned int nnearm = 0, ntouchp = 0, nsimple = 0;
     int4 *d_simple = nullptr;
     bool assembled = false;
+    bool lists_sized = false;        // the non-far lists sized by an upper bound (the first assembly)
+    bool nonfar_cap_forced = false;  // a first budget set by VP_NONFAR_CAP (no sizing pass)
     bool wake_only = false;  // assemble only the body test triangles' pairs with the wake's sources
     int nts_body = 0;        // source tiles of the body (the wake's follow)
     int2 *d_bnear = nullptr; // near (test triangle, particle) pairs of blob_rhs
