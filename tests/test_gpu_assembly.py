@@ -146,7 +146,8 @@ def test_abused_ellipsoid_full_size():
     """The abused ellipsoid at its full benchmark size (2:1:0.5, uv 71 x 142: 19,880 triangles, 5% slivers, jitter
     0.25): every pair is classified once (tier pair counts sum to nt²); the rows of A and b of every slot whose
     triangles all lie among test triangles 0..1135 equal the oracle's assembly of those test rows (all 9,942 columns,
-    1e-12 of the largest entry); the mixed assembly (FP32 multipole) stays within 1e-8 of FP64 everywhere; both
+    1e-12 of the largest entry); the mixed assembly (FP32 multipole far field and FP32 points of the split near pairs)
+    stays within 2e-6 of the largest entry of FP64 everywhere (measured 4.0e-7); both
     bordered solves give the exact potential flow's doublet (mean removed) to 1e-3 of its scale in the area-weighted
     L2 norm (the discretization error, measured 2.8e-4) and agree with each other to 1e-6"""
     g = mesh.abused_ellipsoid(nth=71, nph=142, axes=(2.0, 1.0, 0.5), jitter=0.25, sliver_frac=0.05, sliver_ratio=0.1,
@@ -187,7 +188,7 @@ def test_abused_ellipsoid_full_size():
             Am, bm = plan.read()
             em = np.abs(Am - A).max() / scale
             print(f"  mixed vs FP64 entries: {em:.2e}", flush=True)
-            assert em < 1e-8, em
+            assert em < 2e-6, em
         F = solve.factorize_galerkin(dm, plan, mode=m)
         mus[m] = solve.solve_galerkin(F, bm if m == "mixed" else b)["mu"]
         F.close()
@@ -208,7 +209,7 @@ def test_swept_wing_full_size():
     filaments)); the rows of A (all 29,161 slot and 8,470 wake-node columns) and b of the 362 slots whose triangles
     all lie among test triangles 0..1199 equal the oracle's assembly of those rows to 1e-12 of the largest entry
     (measured 6e-13; the device's own log and atan2 and its order of FP64 sums differ from the oracle's); the mixed
-    assembly's rows there stay within 1e-8 of FP64"""
+    assembly's rows there stay within 2e-6 of FP64's largest entry (measured 7.5e-7)"""
     from vphip import sweep
     spec = mesh.WingSpec(code="2412", n_around=240, n_span=120, aspect=10.0, taper=0.5, sweep_c4_deg=25.0)
     g = mesh.naca_wing(spec)
@@ -242,4 +243,4 @@ def test_swept_wing_full_size():
     em = np.abs(got["mixed"][0] - got["fp64"][0]).max() / scale
     print(f"  {len(rows)} complete slots: A {ea:.2e}, b {eb:.2e}; mixed vs FP64 {em:.2e}", flush=True)
     assert ea < 1e-12 and eb < 1e-12, (ea, eb)
-    assert em < 1e-8, em
+    assert em < 2e-6, em

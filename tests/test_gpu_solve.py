@@ -209,9 +209,10 @@ def _dense_solve(M, r, mode):
 
 
 def test_mixed_refinement_falls_back_to_fp64():
-    """A matrix whose condition (1e10) is beyond FP32 refinement: the FP32 factors' refinement cannot meet LAPACK
-    dsgesv's stopping test, so the solve replaces them by an FP64 factorization of the same matrix (dsgesv's fallback)
-    and returns the FP64 solution; a well conditioned one converges on the FP32 factors."""
+    """A matrix whose condition (1e10) is beyond FP32 refinement: the FP32 factors' refinement stops reducing the
+    residual, so the solve replaces them by an FP64 factorization of the same matrix (dsgesv's fallback) and returns
+    the FP64 solution, whose residual (1e-13 relative) FP32 factors cannot reach at that condition; a well conditioned
+    one converges on the FP32 factors within a few steps."""
     rng = np.random.default_rng(3)
     n = 300
     Q1, _ = np.linalg.qr(rng.standard_normal((n, n)))
@@ -224,7 +225,7 @@ def test_mixed_refinement_falls_back_to_fp64():
         st, x, it = _dense_solve(M, r, 1)
         assert st == 0, api.default_context().lib.vp_last_error(api.default_context().h)
         ref = np.linalg.solve(M, r)
-        assert (it > 30) == fallback, it  # the FP32 attempt's 30 refinements, then the FP64 solve's
+        assert fallback or it <= 5, it
         assert np.linalg.norm(M @ x - r) / np.linalg.norm(r) < 1e-13
         assert np.abs(x - ref).max() / np.abs(ref).max() < 1e-4
 

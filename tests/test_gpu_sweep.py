@@ -46,10 +46,10 @@ def test_alpha_batch_loads(wing, precision, tl, td):
     assert np.abs(out["CDi_T"] - CD).max() <= td, np.abs(out["CDi_T"] - CD).max()
     # lift grows with the angle and the two force routes agree to discretisation accuracy
     assert np.all(np.diff(out["CL"]) > 0) and np.abs(out["CL"] - out["CL_KJ"]).max() < 0.05 * np.abs(CL).max()
-    # the batch's right hand sides are the oracle's assembled ones (b is linear in V∞); the mixed path's FP32
-    # multipole far field moves b by about 1e-10 of its largest entry
+    # the batch's right hand sides are the oracle's assembled ones (b is linear in V∞); the mixed path's FP32 far
+    # field and split near pairs move b by about 1e-7 of its largest entry
     Bn = sw.rhs(ALPHAS)
-    tb = 1e-12 if precision == "fp64" else 1e-8
+    tb = 1e-12 if precision == "fp64" else 1e-6
     for k in (0, 5):
         assert np.abs(Bn[:, k] - ref[k][2]).max() <= tb * np.abs(ref[k][2]).max()
     sw.close()

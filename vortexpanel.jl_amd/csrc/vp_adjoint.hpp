@@ -62,7 +62,7 @@ template <int NL> __device__ __forceinline__ void far_tile_zero(double (*GT)[VMA
 // One far pair's three far-rule points (multipole only, or with the forward's kernel switch when EXACT) and its
 // gradients per weight set: gT with respect to the test triangle's vertices, gS to the source's; bd gets the pair's
 // lambdaᵀ b along dV∞ (dVn = dV∞.n_S).
-template <int NL, bool EXACT>
+template <int NL, bool EXACT, typename CT = double>
 __device__ __forceinline__ void far_pair_grads(const AsmParams &P, const RuleTab &rt, const TestFar &F, const Elem &T,
                                                const V3<double> *XS, const Elem &S, bool body, const double *muS,
                                                V3<double> Gam, double sigS, double dVn, const double (*la)[3],
@@ -79,9 +79,9 @@ __device__ __forceinline__ void far_pair_grads(const AsmParams &P, const RuleTab
     for (int q = 0; q < 3; q++) {
         V3<double> wv, g[3];
         if constexpr (EXACT)
-            tri_point_grad(XS, S, ld3(F.x[q]), Q, 0, P.kernel_far, wv, g);
+            tri_point_grad<CT>(XS, S, ld3(F.x[q]), Q, 0, P.kernel_far, wv, g);
         else
-            far_point_grad(XS, S, ld3(F.x[q]), Q, wv, g);
+            far_point_grad<CT>(XS, S, ld3(F.x[q]), Q, wv, g);
         const V3<double> gx = -1.0 * ((g[0] + g[1]) + g[2]);
         const double *l = rt.l + 3 * (off + q);
         for (int j = 0; j < NL; j++) {
@@ -104,7 +104,7 @@ __device__ __forceinline__ void lds_add3(double *G, V3<double> v)
 }
 
 // far_pair_grads with the gradients added to the tile's LDS sums (vl, uloc: the local test slots and source columns)
-template <int NL, bool EXACT>
+template <int NL, bool EXACT, typename CT = double>
 __device__ __forceinline__ void far_pair_adj(const AsmParams &P, const RuleTab &rt, const TestFar &F, const Elem &T,
                                              const V3<double> *XS, const Elem &S, bool body, const double *muS,
                                              V3<double> Gam, double sigS, double dVn, const double (*la)[3],
@@ -112,7 +112,7 @@ __device__ __forceinline__ void far_pair_adj(const AsmParams &P, const RuleTab &
                                              double (*GU)[UMAX][3], double *bd)
 {
     V3<double> gT[NL][3], gS[NL][3];
-    far_pair_grads<NL, EXACT>(P, rt, F, T, XS, S, body, muS, Gam, sigS, dVn, la, Vinf, gT, gS, bd);
+    far_pair_grads<NL, EXACT, CT>(P, rt, F, T, XS, S, body, muS, Gam, sigS, dVn, la, Vinf, gT, gS, bd);
     for (int j = 0; j < NL; j++)
         for (int k = 0; k < 3; k++) {
             lds_add3(GT[j][vl[k]], gT[j][k]);
@@ -124,7 +124,9 @@ __device__ __forceinline__ void far_pair_adj(const AsmParams &P, const RuleTab &
 // gradients are summed per local test slot and local source column in LDS and flushed per tile.  Pairs with a
 // closed-form point are queued in LDS as (test index in the tile) x TS + lane and written out as one contiguous segment
 // per tile (exact_segs: tile, first entry, count), for adj_far_exact, which keeps the tile's LDS sums.
-template <int NL>
+// Both builds keep the compiler's own register allocation (FP64 208 VGPRs, FP32 176; capping the FP32 build at three
+// waves per SIMD left it 84 VGPRs and 840 bytes of spills, 1.8x slower).
+template <int NL, typename CT = double>
 __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, const TestFar *__restrict__ tf,
                                                      const Elem *__restrict__ el, const int *__restrict__ tt_elems,
                                                      const uint8_t *__restrict__ tt_loc, const int *__restrict__ tt_rows,
@@ -199,7 +201,7 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
             for (int j = 0; j < NL; j++)
                 for (int k = 0; k < 3; k++) la[j][k] = aa.lam[j * aa.lstride + tt_rows[tt * VMAX + vl[k]]];
             V3<double> gS[NL][3];
-            far_pair_grads<NL, false>(P, rt, F, el[t], XS, el[s], s < P.nt, muS, Gam, sigS, dVn, la, Vinf, gT, gS, bd);
+            far_pair_grads<NL, false, CT>(P, rt, F, el[t], XS, el[s], s < P.nt, muS, Gam, sigS, dVn, la, Vinf, gT, gS, bd);
             for (int j = 0; j < NL; j++)
                 for (int k = 0; k < 3; k++) accS[j][k] = accS[j][k] + gS[j][k];
         } else {
@@ -233,7 +235,7 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
 // Far pairs with a closed-form point (within kernel_far diameters of the source), one workgroup per tile segment
 // queued by adj_far_pairs: thread per pair, the three far points by tri_point_grad, the gradients summed in LDS per
 // local test slot and source column as in adj_far_pairs and flushed once.
-template <int NL>
+template <int NL, typename CT = double>
 __global__ __launch_bounds__(256) void adj_far_exact(AsmParams P, RuleTab rt, const TestFar *__restrict__ tf,
                                                      const Elem *__restrict__ el, const int *__restrict__ tt_elems,
                                                      const uint8_t *__restrict__ tt_loc, const int *__restrict__ tt_rows,
@@ -269,7 +271,7 @@ __global__ __launch_bounds__(256) void adj_far_exact(AsmParams P, RuleTab rt, co
         }
         const bool body = s < P.nt;
         const double sigS = body ? -dot(Vinf, ld3(S.n)) : 0.0, dVn = body ? dot(dV, ld3(S.n)) : 0.0;
-        far_pair_adj<NL, true>(P, rt, tf[t], el[t], XS, S, body, muS, sheet_of(XS, muS), sigS, dVn, la, vl, uloc, Vinf,
+        far_pair_adj<NL, true, CT>(P, rt, tf[t], el[t], XS, S, body, muS, sheet_of(XS, muS), sigS, dVn, la, vl, uloc, Vinf,
                                GT, GU, bd);
     }
     __syncthreads();
@@ -459,59 +461,68 @@ template <int NL> void Plan::adjoint_pass(const AdjArgs &aa)
         (void)hipFree(d_esegs);
         (void)hipFree(d_ecount);
     };
-    HIPCHECK(hipMemsetAsync(d_ecount, 0, 2 * sizeof(unsigned int), st));
-    adj_far_pairs<NL><<<dim3(ntt, nts), 256, 0, st>>>(P, rt, d_tf, d_el, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc,
-                                                      d_tsn, d_cols, d_slotnode, aa, d_exact, d_esegs, d_ecount, ecap);
-    HIPCHECK(hipGetLastError());
-    unsigned int ecnt[2] = {0, 0};
-    HIPCHECK(hipMemcpyAsync(ecnt, d_ecount, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    if (ecnt[0] > ecap) {
+    // the far and plain near tiers' points in the plan's precision (FP32 on the mixed path); the touching tiers, the
+    // filaments and the freestream in FP64
+    auto run = [&](auto tag) {
+        using CT = decltype(tag);
+        HIPCHECK(hipMemsetAsync(d_ecount, 0, 2 * sizeof(unsigned int), st));
+        adj_far_pairs<NL, CT><<<dim3(ntt, nts), 256, 0, st>>>(P, rt, d_tf, d_el, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc,
+                                                          d_tsn, d_cols, d_slotnode, aa, d_exact, d_esegs, d_ecount, ecap);
+        HIPCHECK(hipGetLastError());
+        unsigned int ecnt[2] = {0, 0};
+        HIPCHECK(hipMemcpyAsync(ecnt, d_ecount, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        if (ecnt[0] > ecap) {
+            free_exact();
+            throw std::runtime_error("adjoint: more closed-form far pairs than the forward's closed-form far points");
+        }
+        if (ecnt[1]) {
+            adj_far_exact<NL, CT><<<ecnt[1], 256, 0, st>>>(P, rt, d_tf, d_el, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc,
+                                                       d_tsn, d_cols, d_slotnode, aa, d_exact, d_esegs);
+            HIPCHECK(hipGetLastError());
+        }
+        HIPCHECK(hipStreamSynchronize(st));
         free_exact();
-        throw std::runtime_error("adjoint: more closed-form far pairs than the forward's closed-form far points");
-    }
-    if (ecnt[1]) {
-        adj_far_exact<NL><<<ecnt[1], 256, 0, st>>>(P, rt, d_tf, d_el, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc,
-                                                   d_tsn, d_cols, d_slotnode, aa, d_exact, d_esegs);
-        HIPCHECK(hipGetLastError());
-    }
-    HIPCHECK(hipStreamSynchronize(st));
-    free_exact();
-    if (nseg) {
-        const long long np = (long long)nt * nseg;
-        adj_seg_far<NL><<<(unsigned)((np + 255) / 256), 256, 0, st>>>(P, rt, d_tf, d_el, d_cs, d_seg, d_segcol, aa);
-        HIPCHECK(hipGetLastError());
-    }
-    // the non-far pairs of the last assembly, through the forward's own routing and kernels in adjoint mode
-    if (nsimple) {
-        touch_simple<NL><<<std::min<unsigned int>((nsimple + 3) / 4, 65535u), 256, 0, st>>>(
-            P, rt, d_el, d_sigma, d_cs, d_cols, d_simple, nsimple, nullptr, nullptr, nullptr, nullptr, aa);
-        HIPCHECK(hipGetLastError());
-    }
-    if (nnear) {
-        near_thread<NL><<<std::min<unsigned int>((nnear + 255) / 256, 65535u), 256, 0, st>>>(
-            P, rt, d_el, d_sigma, d_cs, d_cols, d_near, nnear, nullptr, nullptr, nullptr, nullptr, aa);
-        HIPCHECK(hipGetLastError());
-    }
-    // near_group takes one set per launch: with two it spills (560 bytes per lane under its two-wave cap) and runs
-    // slower than two launches.  VP_ADJ_NEAR_ONE_LAUNCH=1 takes every set in one launch of the build at one wave per
-    // SIMD (no spills), which the tests compare with the per-set launches.
-    const char *one_env = getenv("VP_ADJ_NEAR_ONE_LAUNCH");
-    const bool one_launch = NL > 1 && one_env && one_env[0] == '1';
-    if (nnearm && one_launch) {
-        near_group<NL, 1><<<std::min<unsigned int>((nnearm + 3) / 4, 65535u), 256, 0, st>>>(
-            P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, nnearm, nullptr, nullptr, nullptr, nullptr, aa);
-        HIPCHECK(hipGetLastError());
-    }
-    for (int l = 0; nnearm && !one_launch && l < NL; l++) {
-        AdjArgs a1 = aa;
-        a1.lam += (size_t)l * aa.lstride;
-        a1.G += l * aa.gstride;
-        if (a1.bdot) a1.bdot += l;
-        near_group<1><<<std::min<unsigned int>((nnearm + 3) / 4, 65535u), 256, 0, st>>>(
-            P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, nnearm, nullptr, nullptr, nullptr, nullptr, a1);
-        HIPCHECK(hipGetLastError());
-    }
+        if (nseg) {
+            const long long np = (long long)nt * nseg;
+            adj_seg_far<NL><<<(unsigned)((np + 255) / 256), 256, 0, st>>>(P, rt, d_tf, d_el, d_cs, d_seg, d_segcol, aa);
+            HIPCHECK(hipGetLastError());
+        }
+        // the non-far pairs of the last assembly, through the forward's own routing and kernels in adjoint mode
+        if (nsimple) {
+            touch_simple<NL><<<std::min<unsigned int>((nsimple + 3) / 4, 65535u), 256, 0, st>>>(
+                P, rt, d_el, d_sigma, d_cs, d_cols, d_simple, nsimple, nullptr, nullptr, nullptr, nullptr, aa);
+            HIPCHECK(hipGetLastError());
+        }
+        if (nnear) {
+            near_thread<NL, CT><<<std::min<unsigned int>((nnear + 255) / 256, 65535u), 256, 0, st>>>(
+                P, rt, d_el, d_sigma, d_cs, d_cols, d_near, nnear, nullptr, nullptr, nullptr, nullptr, aa);
+            HIPCHECK(hipGetLastError());
+        }
+        // near_group takes one set per launch: with two it spills (560 bytes per lane under its two-wave cap) and runs
+        // slower than two launches.  VP_ADJ_NEAR_ONE_LAUNCH=1 takes every set in one launch of the build at one wave per
+        // SIMD (no spills), which the tests compare with the per-set launches.
+        const char *one_env = getenv("VP_ADJ_NEAR_ONE_LAUNCH");
+        const bool one_launch = NL > 1 && one_env && one_env[0] == '1';
+        if (nnearm && one_launch) {
+            near_group<NL, 1, CT><<<std::min<unsigned int>((nnearm + 3) / 4, 65535u), 256, 0, st>>>(
+                P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, nnearm, nullptr, nullptr, nullptr, nullptr, aa);
+            HIPCHECK(hipGetLastError());
+        }
+        for (int l = 0; nnearm && !one_launch && l < NL; l++) {
+            AdjArgs a1 = aa;
+            a1.lam += (size_t)l * aa.lstride;
+            a1.G += l * aa.gstride;
+            if (a1.bdot) a1.bdot += l;
+            near_group<1, 3, CT><<<std::min<unsigned int>((nnearm + 3) / 4, 65535u), 256, 0, st>>>(
+                P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, nnearm, nullptr, nullptr, nullptr, nullptr, a1);
+            HIPCHECK(hipGetLastError());
+        }
+    };
+    if (opts.precision == 1)
+        run(float{});
+    else
+        run(double{});
     if (ngroups) {
         adj_group_eval<NL><<<std::min<unsigned int>((ngroups + 3) / 4, 65535u), 256, 0, st>>>(
             P, rt, d_el, d_cs, d_cols, d_seg, d_segcol, d_leaves, d_groups, ngroups, aa);

@@ -105,47 +105,58 @@ __device__ __forceinline__ void sheet_adj(const V3<double> *X, const double *mu,
     area2_adj(X, dot(d, E) / (A2 * A2), g);
 }
 
+template <typename T> __device__ __forceinline__ V3<T> cvt3(V3<double> v) { return mk3(T(v.x), T(v.y), T(v.z)); }
+template <typename T> __device__ __forceinline__ V3<double> dbl3(V3<T> v)
+{
+    return mk3((double)v.x, (double)v.y, (double)v.z);
+}
+
 // The octupole multipole (source_velocity_far, source_triangle.jl:100-119) and the reverse derivative of
 // phi = Q.w with respect to the source's vertices, by hand: phi depends on r = x - c, the vertex offsets d_k = v_k - c
 // and the area A; with dr_k = d_k.r, dd_k = d_k.d_k and iRn = |r|^-n,
 //   d phi/d v_j = d phi/d d_j - (1/3) sum_k d phi/d d_k - (1/3) d phi/d r + (phi/A) dA/d v_j.
-// About 200 flops per point against ~3000 for dual numbers.
-__device__ __forceinline__ void far_point_grad(const V3<double> *XS, const Elem &S, V3<double> x, V3<double> Q,
+// About 200 flops per point against ~3000 for dual numbers.  T: the compute type (float on the mixed path: the
+// offsets r and d_k are formed in FP64 first, so FP32 carries only their rounding).
+template <typename T = double>
+__device__ __forceinline__ void far_point_grad(const V3<double> *XS, const Elem &S, V3<double> x, V3<double> Qd,
                                                V3<double> &wv, V3<double> *g)
 {
-    const V3<double> c = ld3(S.c), n = ld3(S.n);
-    const double A = S.area, s = A * (1.0 / (4 * kPi));
-    const V3<double> r = x - c;
-    const double R2 = dot(r, r), iR = vrsqrt(R2);
-    const double iR2 = iR * iR, iR3 = iR2 * iR, iR5 = iR3 * iR2, iR7 = iR5 * iR2, iR9 = iR7 * iR2, iR11 = iR9 * iR2;
-    const double qr = dot(Q, r);
-    double cr = 0.0, rr = -3.0 * qr * iR5;
-    V3<double> w = mk3(0.0, 0.0, 0.0), gr = w, gd[3];
+    const V3<double> cd = ld3(S.c);
+    const V3<T> n = cvt3<T>(ld3(S.n)), Q = cvt3<T>(Qd);
+    const T A = T(S.area), s = A * T(1.0 / (4 * kPi));
+    const V3<T> r = cvt3<T>(x - cd);
+    const T R2 = dot(r, r), iR = vrsqrt(R2);
+    const T iR2 = iR * iR, iR3 = iR2 * iR, iR5 = iR3 * iR2, iR7 = iR5 * iR2, iR9 = iR7 * iR2, iR11 = iR9 * iR2;
+    const T qr = dot(Q, r);
+    T cr = T(0), rr = T(-3) * qr * iR5;
+    V3<T> w = mk3(T(0), T(0), T(0)), gr = w, gd[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        const V3<double> d = XS[k] - c;
-        const double dr = dot(d, r), dd = dot(d, d), qd = dot(Q, d);
-        cr += (15 * dr * dr * iR7 - 3 * dd * iR5) * (1.0 / 24) - (45 * dd * dr * iR7 - 105 * dr * dr * dr * iR9) * (1.0 / 180);
-        const double cd = -6 * dr * iR5 * (1.0 / 24) - (45 * dr * dr * iR7 - 9 * dd * iR5) * (1.0 / 180);
-        w = w + cd * d;
-        const double pdr = qr * (30 * dr * iR7 * (1.0 / 24) - (45 * dd * iR7 - 315 * dr * dr * iR9) * (1.0 / 180)) +
-                           qd * (-6 * iR5 * (1.0 / 24) - 90 * dr * iR7 * (1.0 / 180));
-        const double pdd = qr * (-3 * iR5 * (1.0 / 24) - 45 * dr * iR7 * (1.0 / 180)) + qd * (9 * iR5 * (1.0 / 180));
+        const V3<T> d = cvt3<T>(XS[k] - cd);
+        const T dr = dot(d, r), dd = dot(d, d), qd = dot(Q, d);
+        cr += (T(15) * dr * dr * iR7 - T(3) * dd * iR5) * T(1.0 / 24) -
+              (T(45) * dd * dr * iR7 - T(105) * dr * dr * dr * iR9) * T(1.0 / 180);
+        const T cd_ = T(-6) * dr * iR5 * T(1.0 / 24) - (T(45) * dr * dr * iR7 - T(9) * dd * iR5) * T(1.0 / 180);
+        w = w + cd_ * d;
+        const T pdr = qr * (T(30) * dr * iR7 * T(1.0 / 24) - (T(45) * dd * iR7 - T(315) * dr * dr * iR9) * T(1.0 / 180)) +
+                      qd * (T(-6) * iR5 * T(1.0 / 24) - T(90) * dr * iR7 * T(1.0 / 180));
+        const T pdd = qr * (T(-3) * iR5 * T(1.0 / 24) - T(45) * dr * iR7 * T(1.0 / 180)) + qd * (T(9) * iR5 * T(1.0 / 180));
         gr = gr + pdr * d;
-        rr += qr * (-105 * dr * dr * iR9 * (1.0 / 24) + 15 * dd * iR7 * (1.0 / 24) + 315 * dd * dr * iR9 * (1.0 / 180) -
-                    945 * dr * dr * dr * iR11 * (1.0 / 180)) +
-              qd * (30 * dr * iR7 * (1.0 / 24) + 315 * dr * dr * iR9 * (1.0 / 180) - 45 * dd * iR7 * (1.0 / 180));
-        gd[k] = s * (cd * Q + pdr * r + (2 * pdd) * d);
+        rr += qr * (T(-105) * dr * dr * iR9 * T(1.0 / 24) + T(15) * dd * iR7 * T(1.0 / 24) +
+                    T(315) * dd * dr * iR9 * T(1.0 / 180) - T(945) * dr * dr * dr * iR11 * T(1.0 / 180)) +
+              qd * (T(30) * dr * iR7 * T(1.0 / 24) + T(315) * dr * dr * iR9 * T(1.0 / 180) - T(45) * dd * iR7 * T(1.0 / 180));
+        gd[k] = s * (cd_ * Q + pdr * r + (T(2) * pdd) * d);
     }
     w = w + (iR3 + cr) * r;
-    wv = s * w;
-    const double phi = dot(Q, wv);
-    const V3<double> g_r = s * ((iR3 + cr) * Q + gr + rr * r);
-    const V3<double> base = (-1.0 / 3) * ((gd[0] + gd[1]) + gd[2] + g_r);
-    const double pa = 0.5 * phi / A;
-    g[0] = gd[0] + base + pa * cross(n, XS[2] - XS[1]);
-    g[1] = gd[1] + base + pa * cross(XS[2] - XS[0], n);
-    g[2] = gd[2] + base + pa * cross(n, XS[1] - XS[0]);
+    const V3<T> wt = s * w;
+    const T phi = dot(Q, wt);
+    const V3<T> g_r = s * ((iR3 + cr) * Q + gr + rr * r);
+    const V3<T> base = T(-1.0 / 3) * ((gd[0] + gd[1]) + gd[2] + g_r);
+    const T pa = T(0.5) * phi / A;
+    wv = dbl3(wt);
+    g[0] = dbl3(gd[0] + base + pa * cross(n, cvt3<T>(XS[2] - XS[1])));
+    g[1] = dbl3(gd[1] + base + pa * cross(cvt3<T>(XS[2] - XS[0]), n));
+    g[2] = dbl3(gd[2] + base + pa * cross(n, cvt3<T>(XS[1] - XS[0])));
 }
 
 // The closed form (source_velocity_exact, source_triangle.jl:75-89) and the reverse derivative of phi = Q.w with
@@ -207,15 +218,82 @@ __device__ __forceinline__ void exact_point_grad(const V3<double> *XS, const Ele
     g[2] = s * g[2];
 }
 
+// exact_point_grad in FP32 for points off the source (the mixed path's far and plain near tiers; no in-plane branch),
+// with the forms of sv_exact_f32 that keep FP32's digits: each edge log as log1p(2 d t / s2d2), the solid angle's
+// numerator as 2 A z, and the cross products of its derivative as rb x (b - c), rc x (c - a), ra x (a - b) (equal to
+// rb x rc, rc x ra, ra x rb, without their cancellation for nearly parallel offsets).  The normal's adjoint is taken
+// in FP64.  (exact_point_grad with side 0.)
+__device__ __forceinline__ void exact_point_grad_f32(const V3<double> *XS, const Elem &S, V3<double> x, V3<double> Qd,
+                                                     V3<double> &wv, V3<double> *g)
+{
+    const V3<float> n = cvt3<float>(ld3(S.n)), Q = cvt3<float>(Qd);
+    V3<float> r[3], e[3], gf[3];
+    float l[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        r[k] = cvt3<float>(x - XS[k]);
+        e[k] = cvt3<float>(XS[k == 2 ? 0 : k + 1] - XS[k]);  // edge k: vertex k -> k + 1
+        l[k] = __builtin_amdgcn_sqrtf(dot(r[k], r[k]));
+        gf[k] = mk3(0.f, 0.f, 0.f);
+    }
+    const V3<float> nQ = cross(n, Q);
+    V3<float> cn = mk3(0.f, 0.f, 0.f), wt = cn;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int i = k, j = k == 2 ? 0 : k + 1;
+        const float d = (float)S.len[k], a = l[i], b = l[j], ab = a * b, p = dot(r[i], r[j]);
+        const V3<float> c = cross(r[i], r[j]);
+        const float s2d2 = p < 0.f ? 2.f * dot(c, c) * __builtin_amdgcn_rcpf(ab - p) : 2.f * (ab + p);
+        const float is2 = __builtin_amdgcn_rcpf(s2d2), t = a + b + d, Le = flog1p(2.f * d * t * is2);
+        const V3<float> nu = cvt3<float>(ld3(S.nu[k]));
+        wt = wt + Le * nu;
+        const float f = dot(Q, nu), id = __builtin_amdgcn_rcpf(d);
+        const float dLa = -2.f * d * is2, dLd = 2.f * (a + b) * is2;
+        const V3<float> ge = Le * (id * nQ - (f * id * id) * e[k]);
+        gf[i] = gf[i] + (f * (-dLa * __builtin_amdgcn_rcpf(a))) * r[i] - (f * dLd * id) * e[k] - ge;
+        gf[j] = gf[j] + (f * (-dLa * __builtin_amdgcn_rcpf(b))) * r[j] + (f * dLd * id) * e[k] + ge;
+        cn = cn + (Le * id) * cross(Q, e[k]);
+    }
+    // the atan2 form differentiated everywhere, in the plane too (where the forward takes 0 and N = 0), as the FP64
+    // adjoint does for points off the own element
+    const float qn = dot(Q, n);
+    float Om;
+    const double z = dot(x - ld3(S.c), ld3(S.n));
+    {
+        const V3<float> &ra = r[0], &rb = r[1], &rc = r[2];
+        const float la = l[0], lb = l[1], lc = l[2];
+        const float N = (float)(2.0 * S.area * z);
+        const float D = la * lb * lc + dot(ra, rb) * lc + dot(ra, rc) * lb + dot(rb, rc) * la;
+        Om = 2.f * fatan2(N, D);
+        const float h = 2.f * __builtin_amdgcn_rcpf(N * N + D * D), dN = h * D * qn, dD = -h * N * qn;
+        // rb x rc = rb x (b - c) = -rb x e[1]; rc x ra = -rc x e[2]; ra x rb = -ra x e[0]
+        gf[0] = gf[0] + dN * cross(rb, e[1]) - dD * (((lb * lc + dot(rb, rc)) * __builtin_amdgcn_rcpf(la)) * ra + lc * rb + lb * rc);
+        gf[1] = gf[1] + dN * cross(rc, e[2]) - dD * (((la * lc + dot(ra, rc)) * __builtin_amdgcn_rcpf(lb)) * rb + lc * ra + la * rc);
+        gf[2] = gf[2] + dN * cross(ra, e[0]) - dD * (((la * lb + dot(ra, rb)) * __builtin_amdgcn_rcpf(lc)) * rc + lb * ra + la * rb);
+    }
+    cn = cn + Om * Q;
+    const double s4 = 1.0 / (4 * kPi);
+    for (int k = 0; k < 3; k++) g[k] = dbl3(gf[k]);
+    normal_adj(XS, dbl3(cn), g);
+    wv = s4 * dbl3(wt + Om * n);
+    g[0] = s4 * g[0];
+    g[1] = s4 * g[1];
+    g[2] = s4 * g[2];
+}
+
 // phi = Q.w(x; S) (w = source_velocity, source_triangle.jl:134-138, with the forward's switch) and d phi/d(vertex k of
-// S) for k = 0, 1, 2; the value w(x) in wv.  Both branches by their reverse derivatives.
+// S) for k = 0, 1, 2; the value w(x) in wv.  Both branches by their reverse derivatives; T float: the mixed path's
+// FP32 forms (points off the source only: side 0).
+template <typename T = double>
 __device__ __forceinline__ void tri_point_grad(const V3<double> *XS, const Elem &S, V3<double> x, V3<double> Q, int side,
                                                double kernel_far, V3<double> &wv, V3<double> *g)
 {
     const V3<double> r0 = cls_sub(x, ld3(S.c));
     const double fd = kernel_far * S.diam;
     if (cls_dot(r0, r0) > fd * fd)
-        far_point_grad(XS, S, x, Q, wv, g);
+        far_point_grad<T>(XS, S, x, Q, wv, g);
+    else if constexpr (std::is_same<T, float>::value)
+        exact_point_grad_f32(XS, S, x, Q, wv, g);
     else
         exact_point_grad(XS, S, x, Q, side, wv, g);
 }
@@ -388,7 +466,7 @@ __device__ __forceinline__ void adj_pair_begin(const AdjArgs &aa, int nt, const 
 }
 
 // One rule point (barycentric l on the test triangle, weight W0 = rule weight x piece area fraction)
-template <int NL>
+template <int NL, typename T = double>
 __device__ __forceinline__ void adj_point(AdjPair<NL> &ap, const V3<double> *XT, const Elem &S, int side,
                                           double kernel_far, double W0, const double *l, int test)
 {
@@ -401,7 +479,7 @@ __device__ __forceinline__ void adj_point(AdjPair<NL> &ap, const V3<double> *XT,
     }
     if (!any) return;
     V3<double> wv, g[3];
-    tri_point_grad(ap.XS, S, cls_point(XT, l), ap.Q, side, kernel_far, wv, g);
+    tri_point_grad<T>(ap.XS, S, cls_point(XT, l), ap.Q, side, kernel_far, wv, g);
     const V3<double> gx = -1.0 * ((g[0] + g[1]) + g[2]);
     for (int j = 0; j < NL; j++) add_point(ap.p[j], a[j], wv, g, gx, l);
 }

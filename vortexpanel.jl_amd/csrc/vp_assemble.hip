@@ -33,6 +33,7 @@
 #include <numeric>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "vp_capi_internal.hpp"
@@ -224,8 +225,9 @@ template <typename T> __device__ __forceinline__ FarSrc<T> load_far_src(const El
 }
 
 // One kernel evaluation of source_velocity (src/kernels/source_triangle.jl:134-138) at x (FP64) for the far tier:
-// the multipole beyond kernel_far diameters in T, the closed form in FP64 otherwise (its edge-log sum cancels
-// like r/diam, which single precision cannot carry).
+// the multipole beyond kernel_far diameters in T, the closed form in FP64 otherwise.  (The FP32 closed form,
+// sv_exact_f32, measured no faster here on the swept wing, 208 against 210 ms: the far pass is not bound by its
+// arithmetic there; so the mixed path keeps these points' FP64 accuracy.)
 template <typename T>
 __device__ __forceinline__ V3<double> far_eval(const FarSrc<T> &f, const Elem *__restrict__ E, V3<double> x, double kfar2,
                                                unsigned int &nexact)
@@ -283,16 +285,20 @@ __device__ __forceinline__ void far_block(const FarSrc<T> &f, const Elem *__rest
     }
 }
 
-// A far tile's LDS sums (blocks per local test slot x local source column, RHS per local test slot) into A and b
+// A far tile's LDS sums (blocks per local source column x local test slot, R[u * RST + v]; RHS per local test slot)
+// into A and b.  Consecutive lanes take consecutive test slots of one source column: the tile's slots are sorted, so
+// a wave's atomic adds fall on ascending addresses of one column of A (column-major) rather than one cache line per
+// lane (C4: the far pass was bound by those scattered atomics).
+constexpr int RST = VMAX + 1;  // the tile's column stride in LDS: odd, so that the lanes' columns spread over the banks
 __device__ __forceinline__ void far_flush(const AsmParams &P, const double *R, const double *rbacc, int tt, int st,
                                           const int *tt_rows, const int *tt_nrow, const int *ts_cols, const int *ts_ncol,
                                           double *A, double *bvec)
 {
     const int nrow = tt_nrow[tt], ncol = ts_ncol[st];
-    for (int i = threadIdx.x; i < nrow * UMAX; i += blockDim.x) {
-        int v = i / UMAX, u = i - v * UMAX;
-        double val = R[v * UMAX + u];
-        if (u < ncol && val != 0.0) unsafeAtomicAdd(&A[(long long)ts_cols[st * UMAX + u] * P.lda + tt_rows[tt * VMAX + v]], val);
+    for (int i = threadIdx.x; i < nrow * ncol; i += blockDim.x) {
+        const int u = i / nrow, v = i - u * nrow;
+        const double val = R[u * RST + v];
+        if (val != 0.0) unsafeAtomicAdd(&A[(long long)ts_cols[st * UMAX + u] * P.lda + tt_rows[tt * VMAX + v]], val);
     }
     for (int v = threadIdx.x; v < nrow; v += blockDim.x)
         if (rbacc[v] != 0.0) unsafeAtomicAdd(&bvec[tt_rows[tt * VMAX + v]], rbacc[v]);
@@ -313,13 +319,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     int2 *__restrict__ near_list, unsigned int *__restrict__ near_count, unsigned int near_cap,
     unsigned long long *__restrict__ far_count, uint8_t *__restrict__ tiers, int st0, bool enum_only)
 {
-    __shared__ double R[VMAX * UMAX];
+    __shared__ double R[UMAX * RST];
     __shared__ double rbacc[VMAX];
     __shared__ unsigned int nfar, nexw;
     const int tt = blockIdx.x, st = blockIdx.y + st0;
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
     if (!enum_only) {
-        for (int i = threadIdx.x; i < VMAX * UMAX; i += blockDim.x) R[i] = 0.0;
+        for (int i = threadIdx.x; i < UMAX * RST; i += blockDim.x) R[i] = 0.0;
         if (threadIdx.x < VMAX) rbacc[threadIdx.x] = 0.0;
     }
     if (threadIdx.x == 0) nfar = nexw = 0;
@@ -356,7 +362,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
-                for (int b = 0; b < 3; b++) atomicAdd(&R[vl[a] * UMAX + uloc[b]], blk[a][b]);
+                for (int b = 0; b < 3; b++) atomicAdd(&R[uloc[b] * RST + vl[a]], blk[a][b]);
 #pragma unroll
             for (int a = 0; a < 3; a++) atomicAdd(&rbacc[vl[a]], rb[a]);
         }
@@ -1220,6 +1226,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
 
 // One point of a plain near pair's rule: accumulates W psi_a (w x n_T) (acc[3a..3a+2]) and W psi_a (w . n_T)
 // (acc[9+a]); the sheets gamma_b and the source strength are applied once per pair (pair_block, assemble.jl:245-258).
+// T float: the mixed path (multipole and closed form in FP32, the sums in FP64); the rule's points stay FP64.
+template <typename T = double>
 __device__ __forceinline__ void near_point(const V3<double> *X, const V3<double> *SX, const Elem &S, V3<double> tn,
                                            V3<double> sc, double fd2, double W, const double *l, int test, double *acc)
 {
@@ -1228,7 +1236,21 @@ __device__ __forceinline__ void near_point(const V3<double> *X, const V3<double>
     const V3<double> x = cls_point(X, l);
     const V3<double> r = x - sc;
     V3<double> w;
-    if (cls_dot(r, r) > fd2) {
+    if constexpr (std::is_same<T, float>::value) {
+        V3<float> wf;
+        if (cls_dot(r, r) > fd2) {
+            SrcT<float> sr;
+            for (int k = 0; k < 3; k++) {
+                const V3<double> d = SX[k] - sc;
+                sr.v[k] = mk3((float)d.x, (float)d.y, (float)d.z);
+            }
+            sr.area = (float)S.area;
+            wf = sv_far_rel(sr, mk3((float)r.x, (float)r.y, (float)r.z));
+        } else {
+            wf = sv_exact_f32(S, SX, x, dot(r, ld3(S.n)));
+        }
+        w = mk3((double)wf.x, (double)wf.y, (double)wf.z);
+    } else if (cls_dot(r, r) > fd2) {
         SrcT<double> sr;
         for (int k = 0; k < 3; k++) sr.v[k] = SX[k] - sc;
         sr.area = S.area;
@@ -1272,8 +1294,8 @@ __device__ __forceinline__ void near_scatter(const AsmParams &P, int t, int s, c
 // Plain near pairs whose near rule is the single root piece (near_rule, assemble.jl:69-73), forward: one lane per
 // rule point, WAVE / n pairs of n points per wavefront (nine of the default seven).  The lanes of a pair are summed in
 // point order (the reference's accumulation order) by shuffles, and the pair's first lane scatters its block.  WPS:
-// the occupancy cap, chosen by the launch autotuner.
-template <int WPS>
+// the occupancy cap, chosen by the launch autotuner.  CT: the points' compute type (float on the mixed path).
+template <int WPS, typename CT = double>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPS, WPS))) void near_points(
     AsmParams P, RuleTab rt, const Elem *__restrict__ el, const double *__restrict__ sigma,
     const int *__restrict__ corner_slot, const int *__restrict__ src_cols, const int2 *__restrict__ list,
@@ -1302,7 +1324,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPS, WPS)))
             const V3<double> X[3] = {ld3(T.v[0]), ld3(T.v[1]), ld3(T.v[2])};
             const V3<double> SX[3] = {ld3(S.v[0]), ld3(S.v[1]), ld3(S.v[2])};
             const double fd = P.kernel_far * S.diam;
-            near_point(X, SX, S, ld3(T.n), ld3(S.c), fd * fd, rt.w[off + q] * T.area, rt.l + 3 * (off + q), P.test, acc);
+            near_point<CT>(X, SX, S, ld3(T.n), ld3(S.c), fd * fd, rt.w[off + q] * T.area, rt.l + 3 * (off + q), P.test, acc);
         }
         double tot[12];
         const int first = (slot < per ? slot : 0) * nr;
@@ -1327,7 +1349,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPS, WPS)))
 
 // Plain near pairs whose near rule is the single root piece, backward: one thread per pair, the rule's points in
 // turn.
-template <int NADJ>  // 0: the forward assembly; 1..ADJ_MAX: backward with that many sets of adjoint weights
+template <int NADJ, typename CT = double>  // NADJ 0: the forward assembly; 1..ADJ_MAX: backward with that many weight sets
 __global__ __launch_bounds__(256) void near_thread(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                    const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                    const int *__restrict__ src_cols, const int2 *__restrict__ list,
@@ -1352,7 +1374,8 @@ __global__ __launch_bounds__(256) void near_thread(AsmParams P, RuleTab rt, cons
             AdjPair<NADJ> ap;
             adj_pair_begin(aa, P.nt, S, s, corner_slot + 3 * t, src_cols + 3 * s, ld3(T.n), ap);
 #pragma unroll 1
-            for (int q = 0; q < nr; q++) adj_point(ap, X, S, 0, P.kernel_far, rt.w[off + q], rt.l + 3 * (off + q), P.test);
+            for (int q = 0; q < nr; q++)
+                adj_point<NADJ, CT>(ap, X, S, 0, P.kernel_far, rt.w[off + q], rt.l + 3 * (off + q), P.test);
             adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap, S, bdsum);
             continue;
         }
@@ -1417,7 +1440,8 @@ __device__ __forceinline__ void node_of(int nid, int &L, int &idx)  // level-ord
 // WPS is chosen by the launch autotuner (three waves per SIMD measured best: C2 7.77 -> 6.50 ms against two); the
 // backward is capped at two (faster than the register allocation the compiler picks by itself).
 // The backward builds are capped at two waves per SIMD, or at one with WPS = 1 (all weight sets in one launch).
-template <int NADJ, int WPS = 3>  // NADJ 0: the forward assembly; 1..ADJ_MAX: backward with that many weight sets
+// CT: the forward points' compute type (float on the mixed path).
+template <int NADJ, int WPS = 3, typename CT = double>  // NADJ 0: the forward assembly; 1..ADJ_MAX: backward with that many weight sets
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ? WPS : (WPS == 1 ? 1 : 2), NADJ == 0 ? WPS : (WPS == 1 ? 1 : 2)))) void near_group(AsmParams P, RuleTab rt, const Elem *__restrict__ el,
                                                   const double *__restrict__ sigma, const int *__restrict__ corner_slot,
                                                   const int *__restrict__ src_cols, const int2 *__restrict__ list,
@@ -1561,7 +1585,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
                 const double *m = rt.l + 3 * (off + q);
                 double l[3];
                 for (int j = 0; j < 3; j++) l[j] = (m[0] * b[0][j] + m[1] * b[1][j]) + m[2] * b[2][j];
-                adj_point(ap, X, S, 0, P.kernel_far, rt.w[off + q] * frac, l, P.test);
+                adj_point<NADJ, CT>(ap, X, S, 0, P.kernel_far, rt.w[off + q] * frac, l, P.test);
             }
             for (int l = 0; l < NADJ; l++) adj_wave_reduce(ap.p[l]);
             if (lane == 0) adj_pair_end(aa, P.nt, T, X, t, s, aa.nodes, ap, S, bdsum);
@@ -1577,7 +1601,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
             const double *m = rt.l + 3 * (off + q);
             double l[3];
             for (int j = 0; j < 3; j++) l[j] = (m[0] * b[0][j] + m[1] * b[1][j]) + m[2] * b[2][j];
-            near_point(X, SX, S, ld3(T.n), ld3(S.c), fd * fd, rt.w[off + q] * frac * T.area, l, P.test, acc);
+            near_point<CT>(X, SX, S, ld3(T.n), ld3(S.c), fd * fd, rt.w[off + q] * frac * T.area, l, P.test, acc);
         }
         double tot[12];
         for (int i = 0; i < 12; i++) tot[i] = wave_sum(acc[i]);
@@ -1673,6 +1697,15 @@ void build_tiles(const std::vector<int> &order, const std::vector<int> &cols3, i
             for (int k = 0; k < 3; k++) tl.push_back((uint8_t)idx[k]);
             i++;
         }
+        // the tile's slots in ascending order, so that a far tile's flush walks each column of A in address order
+        std::vector<int> perm(uc.size()), rank(uc.size());
+        std::iota(perm.begin(), perm.end(), 0);
+        std::sort(perm.begin(), perm.end(), [&](int a, int b) { return uc[a] < uc[b]; });
+        for (size_t k = 0; k < perm.size(); k++) rank[perm[k]] = (int)k;
+        std::vector<int> us(uc.size());
+        for (size_t k = 0; k < perm.size(); k++) us[k] = uc[perm[k]];
+        for (auto &x : tl) x = (uint8_t)rank[x];
+        uc.swap(us);
         for (size_t k = 0; k < (size_t)cap; k++) {
             elems.push_back(k < tile.size() ? tile[k] : -1);
             for (int j = 0; j < 3; j++) loc.push_back(k < tile.size() ? tl[3 * k + j] : 0);
@@ -2187,6 +2220,7 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
         const unsigned int per = WAVE / std::max(1, rt.n[R_NEAR]), waves = (cnt[2] + per - 1) / per;
         const unsigned int g = std::min<unsigned int>((waves + 3) / 4, 65535u), n = cnt[2];
         tuned_launch(TK_NEARPTS, [&](int w) {
+            // FP64 in both paths: its FP32 build measured slower on the swept wing (24.1 against 23.1 ms)
             if (w == 3)
                 near_points<3><<<g, 256, 0, st>>>(P, rt, d_el, d_sigma, d_cs, d_cols, d_near, n, A, d_b, d_stats, d_tiers);
             else
@@ -2196,7 +2230,13 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
     if (cnt[3]) {
         const unsigned int g = std::min<unsigned int>((cnt[3] + 3) / 4, 65535u), n = cnt[3];
         tuned_launch(TK_NEARGRP, [&](int w) {
-            if (w == 3)
+            if (opts.precision == 1 && w == 3)
+                near_group<0, 3, float><<<g, 256, 0, st>>>(P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, n, A, d_b, d_stats,
+                                                            d_tiers, AdjArgs{});
+            else if (opts.precision == 1)
+                near_group<0, 2, float><<<g, 256, 0, st>>>(P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, n, A, d_b, d_stats,
+                                                            d_tiers, AdjArgs{});
+            else if (w == 3)
                 near_group<0, 3><<<g, 256, 0, st>>>(P, rt, d_el, d_sigma, d_cs, d_cols, d_nearm, n, A, d_b, d_stats,
                                                      d_tiers, AdjArgs{});
             else

@@ -342,6 +342,84 @@ template <typename T> __device__ __forceinline__ V3<T> sv_far_mom(const FarMom<T
     return (m.area * T(1.0 / (4 * kPi))) * w;
 }
 
+// ---- FP32 closed form (the mixed path's far-tier closed-form points and plain near pairs) ----
+
+// atan2 in FP32: atan t on [0, 1] as t P(t^2), P of degree 8 fitted here by least squares in relative error on 4000
+// Chebyshev nodes of [0, 1] (1.7e-8 of atan in exact arithmetic, about 1.5 ulp evaluated in FP32), with the octant
+// reduction atan2 = pi/2 - atan(1/t) above the diagonal and pi - atan for x < 0.
+__device__ __forceinline__ float fatan2(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y), mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float t = mx > 0.f ? mn * __builtin_amdgcn_rcpf(mx) : 0.f, z = t * t;
+    float p = 2.878729539e-03f;
+    p = fmaf(p, z, -1.619798713e-02f);
+    p = fmaf(p, z, 4.293066915e-02f);
+    p = fmaf(p, z, -7.527754620e-02f);
+    p = fmaf(p, z, 1.065401495e-01f);
+    p = fmaf(p, z, -1.420775599e-01f);
+    p = fmaf(p, z, 1.999329349e-01f);
+    p = fmaf(p, z, -3.333312076e-01f);
+    p = fmaf(p, z, 9.999999930e-01f);
+    float r = t * p;
+    if (ay > ax) r = 1.57079633f - r;
+    if (x < 0.f) r = 3.14159265f - r;
+    return copysignf(r, y);
+}
+
+// log(1 + u) for u >= 0 in FP32: below u = 0.5 as 2 atanh(s), s = u / (2 + u), by five terms of the odd series
+// (within 1e-8 relative), above it the hardware logarithm of 1 + u (no cancellation there)
+__device__ __forceinline__ float flog1p(float u)
+{
+    if (u < 0.5f) {
+        const float s = u * __builtin_amdgcn_rcpf(2.f + u), z = s * s;
+        float p = 1.f / 9;
+        p = fmaf(p, z, 1.f / 7);
+        p = fmaf(p, z, 1.f / 5);
+        p = fmaf(p, z, 1.f / 3);
+        p = fmaf(p, z, 1.f);
+        return 2.f * s * p;
+    }
+    return __logf(1.f + u);
+}
+
+// source_velocity_exact (src/kernels/source_triangle.jl:75-89) in FP32 at a point off the source, from its offsets
+// from the vertices SX and its height z over the plane, both taken in FP64:
+//  - each edge_log as log1p(2 d t / s2d2): with t = a + b + d and s2d2 = (a + b)^2 - d^2 (its cancellation-free
+//    form), t^2 - s2d2 = 2 d t exactly because |ri - rj| = d, so the ratio's excess over one needs no subtraction;
+//  - the solid angle's numerator ra.(rb x rc) as 2 A z, which it equals exactly ((x - a).((b - a) x (c - a)), the
+//    other terms of the triple product vanish), instead of a triple product of nearly parallel offsets;
+// so that only the tangential sum's own cancellation (its terms ~ diam/r, their sum ~ (diam/r)^2) costs digits:
+// about 1e-6 of the FP64 closed form out to the multipole switch.  In the plane (|z| <= 1e-12 diam) the solid angle
+// is 0, as the FP64 form's off-element branch.
+__device__ __forceinline__ V3<float> sv_exact_f32(const Elem &S, const V3<double> *SX, V3<double> x, double z)
+{
+    V3<float> r[3];
+    float l[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const V3<double> d = x - SX[k];
+        r[k] = mk3((float)d.x, (float)d.y, (float)d.z);
+        l[k] = __builtin_amdgcn_sqrtf(dot(r[k], r[k]));
+    }
+    V3<float> wt = mk3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int i = k, j = k == 2 ? 0 : k + 1;
+        const float d = (float)S.len[k], ab = l[i] * l[j], p = dot(r[i], r[j]);
+        const V3<float> c = cross(r[i], r[j]);
+        const float s2d2 = p < 0.f ? 2.f * dot(c, c) * __builtin_amdgcn_rcpf(ab - p) : 2.f * (ab + p);
+        const float L = flog1p(2.f * d * (l[i] + l[j] + d) * __builtin_amdgcn_rcpf(s2d2));
+        wt = wt + L * mk3((float)S.nu[k][0], (float)S.nu[k][1], (float)S.nu[k][2]);
+    }
+    float Om = 0.f;
+    if (!(fabs(z) <= 1e-12 * S.diam)) {
+        const float N = (float)(2.0 * S.area * z);
+        const float D = l[0] * l[1] * l[2] + dot(r[0], r[1]) * l[2] + dot(r[0], r[2]) * l[1] + dot(r[1], r[2]) * l[0];
+        Om = 2.f * fatan2(N, D);
+    }
+    return (float)(1.0 / (4 * kPi)) * (wt + Om * mk3((float)S.n[0], (float)S.n[1], (float)S.n[2]));
+}
+
 // segment_velocity, src/kernels/segment.jl:15-25
 template <typename T> __device__ __forceinline__ void seg_vel(V3<T> ra, V3<T> rb, V3<T> &va, V3<T> &vb)
 {

@@ -601,7 +601,8 @@ void launch_residual(hipStream_t s, bool trans, const double *M, long long ld, i
     HIPCHECK2(hipGetLastError());
 }
 
-// per right hand side (columns of n x nrhs row-major): max |r|, max |x|, sum r^2 and sum rhs^2 into out[4c..4c+3]
+// per right hand side (columns of n x nrhs row-major): max |r|, max |x|, sum r^2 and sum rhs^2 into out[4c..4c+3];
+// a non-finite r or x makes max |r| infinite (fmax alone would drop a NaN)
 __global__ void colmax2(const double *r, const double *x, const double *rhs, int n, int nrhs, double *out)
 {
     __shared__ double s1[256], s2[256], s3[256], s4[256];
@@ -609,8 +610,9 @@ __global__ void colmax2(const double *r, const double *x, const double *rhs, int
     double a = 0.0, b = 0.0, r2 = 0.0, b2 = 0.0;
     for (int i = threadIdx.x; i < n; i += 256) {
         const double ri = r[(long long)i * nrhs + c], bi = rhs[(long long)i * nrhs + c];
-        a = fmax(a, fabs(ri));
-        b = fmax(b, fabs(x[(long long)i * nrhs + c]));
+        const double xi = x[(long long)i * nrhs + c];
+        a = fmax(a, (isfinite(ri) && isfinite(xi)) ? fabs(ri) : INFINITY);
+        b = fmax(b, fabs(xi));
         r2 += ri * ri;
         b2 += bi * bi;
     }
@@ -1158,8 +1160,9 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
     const double eps = 1.1102230246251565e-16, tolf = opnorm * eps * std::sqrt((double)n);
     std::vector<double> hn(4 * (size_t)nrhs);
     int it = 0;
-    bool done = false;
+    bool done = false, diverged = false;
     double worst = 0.0;
+    std::vector<double> prev(nrhs, INFINITY);
     for (;; it++) {
         // R = B - M X in FP64: a GEMV pass over M for a few right hand sides, else the MFMA GEMM
         HIPCHECK2(hipMemcpyAsync(Y, B, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
@@ -1180,11 +1183,15 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
         done = true;
         worst = 0.0;
         for (int c = 0; c < nrhs; c++) {
-            if (!(hn[4 * c] <= hn[4 * c + 1] * tolf)) done = false;  // a NaN residual fails the test too
-            const double rb = std::sqrt(hn[4 * c + 3]);
-            worst = std::max(worst, rb > 0 ? std::sqrt(hn[4 * c + 2]) / rb : std::sqrt(hn[4 * c + 2]));
+            if (!(hn[4 * c] <= hn[4 * c + 1] * tolf)) done = false;  // a non-finite residual fails the test too
+            const double rb = std::sqrt(hn[4 * c + 3]), rr = std::sqrt(hn[4 * c + 2]);
+            const double rel = rb > 0 ? rr / rb : rr;
+            worst = std::isnan(rel) || std::isnan(worst) ? NAN : std::max(worst, rel);
+            // dsgesv stops the FP32 factors early when a refinement step does not reduce the residual
+            if (it > 0 && strict && mode == 1 && !(hn[4 * c] < prev[c])) diverged = true;
+            prev[c] = hn[4 * c];
         }
-        if (done || it >= max_iter) break;
+        if (done || it >= max_iter || (diverged && !done)) break;
         apply_inverse_replay(nrhs, trans);
         axpy_inplace<<<blocks((long long)sz), 256, 0, s>>>(X, Y, (long long)sz);
         HIPCHECK2(hipGetLastError());
@@ -1197,8 +1204,8 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
         }
         char msg[200];
         snprintf(msg, sizeof msg,
-                 "iterative refinement did not converge: relative residual %.3e after %d refinements of the FP64 factors",
-                 worst, it);
+                 "iterative refinement did not converge: relative residual %.3e after %d refinements of the FP64 factors%s",
+                 worst, it, diverged ? " (the residual stopped decreasing)" : "");
         throw ConvergenceError(msg);
     }
     if (resid_out) *resid_out = worst;

@@ -230,10 +230,12 @@ class WingAdjoint:
         return dict(CL=CL, CDi=CD)
 
     def _adjoint_weights(self, dJdq, border=False):
-        """lam = C psi_q for M^T psi = [dJ/dq; 0]; with border, also psi's border part (one value per body)"""
+        """lam = C psi_q for M^T psi = [dJ/dq; 0] (dJdq (nq,), or (nq, k) for k objectives solved as one batch); with
+        border, also psi's border part (one value per body)"""
         F = self.F
-        R = np.zeros(F.n)
-        R[:F.nq] = dJdq
+        dJ = np.asarray(dJdq, dtype=np.float64)
+        R = np.zeros((F.n,) + dJ.shape[1:])
+        R[:F.nq] = dJ
         X = F.solve_bordered_t(R)
         lam = F.C @ X[:F.nq]
         return (lam, X[F.nq:F.nq + F.nb].copy()) if border else lam
@@ -301,12 +303,12 @@ class WingAdjoint:
         raise ValueError("objective must be CL or CDi")
 
     def gradients(self, objectives=("CL", "CDi")):
-        """{objective: (J, dJ/dx (nn, 3), dJ/dalpha per degree)} for J in objectives (CL, CDi): one adjoint solve
-        each, one backward pass for every two of them"""
+        """{objective: (J, dJ/dx (nn, 3), dJ/dalpha per degree)} for J in objectives (CL, CDi): their adjoint systems
+        as one batched solve, one backward pass for every two of them"""
         t0 = time.perf_counter()
         ex = [self._explicit(o) for o in objectives]
         t1 = time.perf_counter()
-        lams = np.stack([self._adjoint_weights(e[1]) for e in ex])
+        lams = np.ascontiguousarray(self._adjoint_weights(np.stack([e[1] for e in ex], 1)).T)  # one solve for all
         t2 = time.perf_counter()
         gb, lbd = self._backward(lams)
         t3 = time.perf_counter()
