@@ -136,13 +136,13 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
                                                      const int *__restrict__ slot_node, AdjArgs aa,
                                                      unsigned short *__restrict__ exact_list,
                                                      int4 *__restrict__ exact_segs, unsigned int *__restrict__ exact_count,
-                                                     unsigned int exact_cap)
+                                                     unsigned int exact_cap, int st0)
 {
     __shared__ double GT[NL][VMAX][3];
     __shared__ double GU[NL][UMAX][3];
     __shared__ unsigned short exq[TT_MAX * TS];
     __shared__ unsigned int nexq, exbase;
-    const int tt = blockIdx.x, st = blockIdx.y;
+    const int tt = blockIdx.x, st = blockIdx.y + st0;  // st0: the first source tile of this launch's slice
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
     far_tile_zero<NL>(GT, GU);
     if (threadIdx.x == 0) nexq = 0;
@@ -449,12 +449,21 @@ template <int NL> void Plan::adjoint_pass(const AdjArgs &aa)
 {
     hipStream_t st = ctx->stream;
     const AsmParams P = params();
-    // far pairs with a closed-form point: at most the forward's count of closed-form far points
+    // far pairs with a closed-form point: at most the forward's count of closed-form far points.  When that exceeds
+    // the queue's budget (2^28 entries; a kernel_far of 1e300 makes every far point closed-form: 1.3e10 on the swept
+    // wing), the far tier goes in slices of source tiles, each queueing at most what its pairs can add.
     unsigned long long hst[16];
     read_stats(hst);
-    const unsigned int ecap = (unsigned int)std::min<unsigned long long>(hst[12] + 1024, 1ull << 31);
+    const unsigned long long budget = 1ull << 28, per_col = (unsigned long long)ntt * TT_MAX * TS;
+    int slice = nts;
+    unsigned long long ecap64 = hst[12] + 1024;
+    if (ecap64 > budget) {
+        slice = (int)std::max<unsigned long long>(1, budget / per_col);
+        ecap64 = std::min(budget, per_col * (unsigned long long)slice);
+    }
+    const unsigned int ecap = (unsigned int)ecap64;
     unsigned short *d_exact = dalloc<unsigned short>(ecap);
-    int4 *d_esegs = dalloc<int4>((size_t)ntt * nts);
+    int4 *d_esegs = dalloc<int4>((size_t)ntt * slice);
     unsigned int *d_ecount = dalloc<unsigned int>(2);
     auto free_exact = [&] {
         (void)hipFree(d_exact);
@@ -465,23 +474,26 @@ template <int NL> void Plan::adjoint_pass(const AdjArgs &aa)
     // filaments and the freestream in FP64
     auto run = [&](auto tag) {
         using CT = decltype(tag);
-        HIPCHECK(hipMemsetAsync(d_ecount, 0, 2 * sizeof(unsigned int), st));
-        adj_far_pairs<NL, CT><<<dim3(ntt, nts), 256, 0, st>>>(P, rt, d_tf, d_el, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc,
-                                                          d_tsn, d_cols, d_slotnode, aa, d_exact, d_esegs, d_ecount, ecap);
-        HIPCHECK(hipGetLastError());
-        unsigned int ecnt[2] = {0, 0};
-        HIPCHECK(hipMemcpyAsync(ecnt, d_ecount, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-        if (ecnt[0] > ecap) {
-            free_exact();
-            throw std::runtime_error("adjoint: more closed-form far pairs than the forward's closed-form far points");
-        }
-        if (ecnt[1]) {
-            adj_far_exact<NL, CT><<<ecnt[1], 256, 0, st>>>(P, rt, d_tf, d_el, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc,
-                                                       d_tsn, d_cols, d_slotnode, aa, d_exact, d_esegs);
+        for (int sb = 0; sb < nts; sb += slice) {
+            HIPCHECK(hipMemsetAsync(d_ecount, 0, 2 * sizeof(unsigned int), st));
+            adj_far_pairs<NL, CT><<<dim3(ntt, std::min(slice, nts - sb)), 256, 0, st>>>(
+                P, rt, d_tf, d_el, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl, d_tsc, d_tsn, d_cols, d_slotnode, aa, d_exact,
+                d_esegs, d_ecount, ecap, sb);
             HIPCHECK(hipGetLastError());
+            unsigned int ecnt[2] = {0, 0};
+            HIPCHECK(hipMemcpyAsync(ecnt, d_ecount, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipStreamSynchronize(st));
+            if (ecnt[0] > ecap) {
+                free_exact();
+                throw std::runtime_error("adjoint: more closed-form far pairs than the forward's closed-form far points");
+            }
+            if (ecnt[1]) {
+                adj_far_exact<NL, CT><<<ecnt[1], 256, 0, st>>>(P, rt, d_tf, d_el, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl,
+                                                               d_tsc, d_tsn, d_cols, d_slotnode, aa, d_exact, d_esegs);
+                HIPCHECK(hipGetLastError());
+            }
+            HIPCHECK(hipStreamSynchronize(st));
         }
-        HIPCHECK(hipStreamSynchronize(st));
         free_exact();
         if (nseg) {
             const long long np = (long long)nt * nseg;
