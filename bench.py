@@ -11,6 +11,7 @@ sample of the same assembly (rows of test triangles).
 python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
+import copy
 import json
 import os
 import sys
@@ -45,13 +46,22 @@ def workload():
     return g, dm, V
 
 
-def cpu_baseline(g, dm, V, target_s=12.0):
-    """The FP64 oracle (C, OpenMP) on the host's cores over a bounded sample of test-triangle rows"""
+def cpu_threads():
+    """Host threads of the CPU legs: the process's CPU share (OMP_NUM_THREADS where it is set, else the CPUs this
+    process may run on), or VP_CPU_THREADS"""
+    n = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        n = min(n, int(os.environ["OMP_NUM_THREADS"]))
+    return int(os.environ.get("VP_CPU_THREADS", n))
+
+
+def cpu_baseline(g, dm, V, threads, target_s=12.0):
+    """The FP64 oracle (C, OpenMP) on the host's cores over the same assembly: every test-triangle row when a pilot
+    puts it within target_s, else a bounded sample of rows"""
     from oracle import oracle as O
-    threads = int(os.environ.get("VP_CPU_THREADS", min(16, os.cpu_count() or 1)))
     el = O.elements(g.x, g.tris)
     sig = O.source_strengths(el, V)
-    pilot = max(threads * 4, 64)
+    pilot = min(g.nt, max(threads * 32, 512))
     t0 = time.perf_counter()
     O.assemble_rows(g, dm, sig, V, 0, pilot, nthreads=threads)
     tp = time.perf_counter() - t0
@@ -62,6 +72,74 @@ def cpu_baseline(g, dm, V, target_s=12.0):
     return dict(value=npairs / dt, unit="pair-integrals/s", cores=threads, kind="port",
                 sample=f"FP64 C oracle assemble over test triangles 0..{rows - 1} of {g.nt} ({npairs:.3e} pair "
                        f"integrals, {dt:.1f} s)")
+
+
+def cpu_lu_leg(dm, A, b, threads, gpu):
+    """LU + solve of the ellipsoid's bordered system on the host: the oracle's solve_galerkin (LAPACK getrf, gecon,
+    getrs, the routines the reference calls) on `threads` BLAS threads, beside the device's factorization + solve"""
+    from threadpoolctl import threadpool_limits
+    from oracle import oracle as O
+    with threadpool_limits(limits=threads):
+        t0 = time.perf_counter()
+        O.solve_galerkin(dm, A, b)
+        dt = time.perf_counter() - t0
+    n = dm.nq + 1
+    return dict(cpu_s=dt, gpu_fp64_s=gpu["fp64"], gpu_mixed_s=gpu["mixed"], cores=threads, kind="port",
+                sample=f"bordered LU (n = {n}) + condition estimate + one solve, FP64 LAPACK")
+
+
+def cpu_alpha_leg(threads):
+    """The 64-angle batch on the host: the FP64 oracle's assembly of the wing with its body-fixed wake (timed once,
+    counted twice: b at V = e_x and at e_z span every angle's right hand side), LAPACK LU of the bordered matrix and
+    the 64 solves.  (Row samples of this assembly are not used: each call clears the whole 1 GB A, which skews them.)"""
+    from threadpoolctl import threadpool_limits
+    from oracle import oracle as O
+    from vphip import dofmap, mesh, wake as wk
+    spec = mesh.WingSpec(code="0012", n_around=160, n_span=60, aspect=8.0)
+    g = mesh.naca_wing(spec)
+    dm = dofmap.build_dofmap(g)
+    w = wk.build_wake(g, dm, np.array([1.0, 0.0, 0.0]), wk.WakeOptions(length=2.5))
+    el = O.elements(g.x, g.tris)
+    ex = np.array([1.0, 0.0, 0.0])
+    t0 = time.perf_counter()
+    A, b, pairs, _ = O.assemble(g, dm, O.source_strengths(el, ex), ex, wake=w, nthreads=threads)
+    ta = time.perf_counter() - t0
+    with threadpool_limits(limits=threads):
+        t0 = time.perf_counter()
+        M, rhs, _ = O.bordered_system(dm, A, b, dofmap.wake_rows(dm, w.te))
+        lu = O.sla.lu_factor(M)
+        O.sla.lu_solve(lu, np.repeat(rhs[:, None], 64, axis=1))
+        tl = time.perf_counter() - t0
+    return dict(cpu_s=2 * ta + tl, cpu_assemble_s=ta, cpu_lu_solves_s=tl, cores=threads, kind="port",
+                sample=f"FP64 oracle assemble of the {g.nt}-triangle wing with its wake ({sum(pairs.values()):.3e} "
+                       f"pair integrals, {ta:.1f} s, counted twice), bordered LU + 64 solves ({tl:.1f} s)")
+
+
+def cpu_free_wake_leg(fw, threads):
+    """One free-wake step on the host at the particle counts of the device run's snapshots: the same host logic over
+    the FP64 oracle (explicit_rhs, LAPACK solve with the factors kept, body and blob velocity sums), from the device
+    run's state before that step"""
+    import copy
+    from oracle.freewake_oracle import OracleFreeWakeSim
+    from vphip import freewake as FW
+    OracleFreeWakeSim.nthreads = threads
+    g, dm, V, opts, snaps = fw
+    t0 = time.perf_counter()
+    sim = OracleFreeWakeSim(g, dm, V, opts)
+    tsetup = time.perf_counter() - t0
+    out = []
+    for snap in snaps:
+        for k, v in snap["state"].items():
+            setattr(sim, k, copy.deepcopy(v))
+        t0 = time.perf_counter()
+        r = sim.step()
+        dt = time.perf_counter() - t0
+        out.append(dict(step=snap["step"], particles=snap["particles"], cpu_s=dt, gpu_s=snap["gpu_s"],
+                        cpu_t_rhs=r["t_rhs"], cpu_t_vel=r["t_vel"]))
+        log(f"cpu free-wake step {out[-1]}")
+    return dict(steps=out, cpu_setup_s=tsetup, cores=threads, kind="port",
+                sample="one step each from the device run's state at about 10k, 40k and 84k particles (FP64 oracle; "
+                       "the device run is the mixed path)")
 
 
 def wing_adjoint_bench(repeats=3):
@@ -139,11 +217,19 @@ def free_wake_bench(nsteps=1000):
     t0 = time.perf_counter()
     sim = FW.FreeWakeSim(g, dm, V, FW.FreeWakeOptions(dt=0.02, core=float(0.1 * edges.mean())), precision="mixed")
     t1 = time.perf_counter()
-    steps, every100 = [], []
+    steps, every100, snaps, marks = [], [], [], [10_000, 40_000, 80_000]
     for n in range(1, nsteps + 1):
+        snap = None
+        if marks and len(sim.state.particles) >= marks[0]:  # the state before this step, for the CPU leg
+            marks.pop(0)
+            snap = dict(step=n, particles=len(sim.state.particles),
+                        state={k: copy.deepcopy(getattr(sim, k)) for k in ("state", "n", "t", "mu", "mu_w", "I_prev")})
         ts = time.perf_counter()
         r = sim.step()
         steps.append(time.perf_counter() - ts)
+        if snap is not None:
+            snap["gpu_s"] = steps[-1]
+            snaps.append(snap)
         if n % 100 == 0:
             every100.append(dict(step=n, particles=r["particles"], mean_step_ms=1e3 * float(np.mean(steps[-100:])),
                                  CL_kj=float(r["F_kj"] @ L / spec.aspect)))
@@ -154,7 +240,7 @@ def free_wake_bench(nsteps=1000):
                CL_pressure=float(r["F_pressure"] @ L / spec.aspect), t_rhs_last=r["t_rhs"], t_vel_last=r["t_vel"],
                every_100=every100)
     log(f"free wake: {out}")
-    return out
+    return out, (g, dm, V, FW.FreeWakeOptions(dt=0.02, core=float(0.1 * edges.mean())), snaps)
 
 
 def nbody_bench(g, dm, V, mu):
@@ -301,13 +387,29 @@ def main():
         log(f"{m} solve {solves[m]}")
     for p_ in plans.values():
         p_.close()
+    fw = None
     for name, fn in (("alpha_sweep", alpha_sweep_bench), ("wing_adjoint", wing_adjoint_bench),
                      ("free_wake", free_wake_bench)):
         if name in want:
             extras[name] = fn()
-    cpu = None if args.no_cpu else cpu_baseline(g, dm, V)
-    if cpu:
+            if name == "free_wake":
+                extras[name], fw = extras[name]
+    # CPU legs: the FP64 oracle on the host's CPU share, beside the device figures above
+    cpu, legs = None, None
+    if not args.no_cpu:
+        threads = cpu_threads()
+        cpu = cpu_baseline(g, dm, V, threads)
         log(f"cpu baseline {cpu}")
+        legs = {"lu_solve": cpu_lu_leg(dm, A64, b64, threads, {m: solves[m]["factor_s"] + solves[m]["solve_s"]
+                                                               for m in solves})}
+        log(f"cpu lu leg {legs['lu_solve']}")
+        if "alpha_sweep" in extras:
+            legs["alpha_batch"] = dict(cpu_alpha_leg(threads), gpu_s=extras["alpha_sweep"]["total_s"])
+            log(f"cpu alpha leg {legs['alpha_batch']}")
+        if fw is not None:
+            legs["free_wake_step"] = cpu_free_wake_leg(fw, threads)
+        legs["adjoint"] = "no CPU leg: the reference computes no derivatives (its only route, finite differences, " \
+                          "takes two solves per node coordinate: 174,240 for the swept wing)"
     line = {
         "metric": "pair-integrals/s",
         "value": value,
@@ -334,6 +436,7 @@ def main():
                      "kernel": "far_pairs<double>", "kernel_ms": far_avg * 1e3,
                      "algorithmic_flops": flops, "far_exact_points": ex, "far_points": 3 * far_pairs_n},
         "cpu_baseline": cpu,
+        "cpu_legs": legs,
         "mixed": {"assemble_ms": mixed_ms, "pair_integrals_per_s": pairs / (mixed_ms / 1e3),
                   "max_entry_rel_diff_vs_fp64": entry_err, "max_mu_rel_diff_vs_fp64": dmu},
         "solve": solves,

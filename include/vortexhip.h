@@ -133,6 +133,24 @@ int vp_plan_blob_rhs(vp_plan *plan, int nb, const double *bx, const double *alph
 int vp_plan_device_A(vp_plan *plan, double **A, double **b, long long *lda);
 void vp_plan_destroy(vp_plan *plan);
 
+/* `explicit_rhs(els, dm, src::ExplicitSources; opts)` (src/assembly/explicit.jl:31-86; ExplicitSources :10-18) as a
+ * handle on one body.  A call takes nsheet flat triangles (sheet_x nsheet x 9: corners p, q, r) with corner doublets
+ * (sheet_mu nsheet x 3), nseg filaments (seg_x nseg x 6: ends a, b) with end circulations (seg_gamma nseg x 2) and nblob
+ * particles (bx, alpha nblob x 3, cores delta) and returns -∫ψ_i n·u dA for every slot (out, nslot values; added to
+ * assemble's b).  The sheets and filaments take the assembly's tiers: out is minus their assembled columns (one wake
+ * node per corner and filament end) times their doublets; the particles take the far rule or near_rule.  The handle
+ * keeps a plan of the known elements and moves it in place while nsheet and nseg stay (the free wake's rows move every
+ * step, step!, src/freewake/step.jl:113-146; new plans only when rows are added), and a plan of the body for the
+ * particles.  One call at a time per handle. */
+typedef struct vp_explicit vp_explicit;
+int vp_explicit_create(vp_ctx *ctx, const vp_surface *g, const vp_assembly_options *opts, vp_explicit **out);
+int vp_explicit_rhs(vp_explicit *h, int nsheet, const double *sheet_x, const double *sheet_mu, int nseg,
+                    const double *seg_x, const double *seg_gamma, int nblob, const double *bx, const double *alpha,
+                    const double *delta, double *out);
+/* counts[0] plans of known elements built, [1] moved in place, [2] calls */
+int vp_explicit_info(const vp_explicit *h, long long *counts);
+void vp_explicit_destroy(vp_explicit *h);
+
 /* ---- bordered solve (src/assembly/solve.jl:15-99) ---------------------------------------------------------- */
 
 /* The DOFMapping pieces of the bordered system: C (nslot x nq) and the wake rows W (nw x nq, `wake_rows`,

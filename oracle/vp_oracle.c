@@ -209,6 +209,8 @@ typedef struct { int n, cap; double *l; /* 3 per point */ double *w; } rule_t;
 
 static void rule_init(rule_t *r) { r->n = 0; r->cap = 0; r->l = NULL; r->w = NULL; }
 static void rule_free(rule_t *r) { free(r->l); free(r->w); rule_init(r); }
+/* empty a rule for reuse, keeping its buffers (the per-thread scratch rule of every pair) */
+static void rule_clear(rule_t *r) { r->n = 0; }
 static void rule_push(rule_t *r, double l1, double l2, double l3, double w)
 {
     if (r->n == r->cap) {
@@ -555,7 +557,7 @@ static void near_pieces(rule_t *out, const el_t *T, const srcref_t *S, const rul
 /* near_rule, src/assembly/assemble.jl:69-73 */
 static void near_rule(rule_t *out, const el_t *T, const srcref_t *S, const rule_t *base, double ratio, int depth)
 {
-    rule_init(out);
+    rule_clear(out);
     near_pieces(out, T, S, base, ratio, depth, CORNERS, 1.0, 0);
 }
 
@@ -601,7 +603,7 @@ static const rule_t *vertex_rule(const el_t *T, int k, const tier_rules_t *R, co
 {
     v3 u = sub(T->x[(k + 1) % 3], T->x[k]), v = sub(T->x[(k + 2) % 3], T->x[k]);
     if (dot(u, v) >= 0) return &R->vertex[k];
-    rule_init(out);
+    rule_clear(out);
     vertex_pieces(out, T->x, k, &R->vertex[0], CORNERS, 1.0, o->corner_angle);
     return out;
 }
@@ -690,7 +692,7 @@ static int contact_rule(const el_t *T, const el_t *S, const int shared[3], const
     for (int k = 0; k < 3 && !extra; k++) extra = !shared[k] && tri_distance(S->x, T->x[k]) <= tol;
     if (!extra) extra = splits_an_edge(T->x, S, tol);
     if (!extra) return 0;
-    rule_init(out);
+    rule_clear(out);
     contact_pieces(out, T, S, R, o, tol, CORNERS, 1.0, 0);
     return 1;
 }

@@ -8,7 +8,7 @@ import warnings
 import numpy as np
 import pytest
 
-from freewake_oracle import OracleFreeWakeSim, oracle_explicit_rhs, oracle_opts
+from oracle.freewake_oracle import OracleFreeWakeSim, oracle_explicit_rhs, oracle_opts
 from oracle import oracle as O
 from vphip import api, dofmap, freewake as FW, mesh, sweep, wake as wk
 

@@ -2,10 +2,12 @@
 particles (near and far) within 1e-12 of the largest entry, and the simulation step by step (trailing edge mu_w,
 Kutta-Joukowski force, particle positions) within 1e-10 in FP64 over steps with conversions; the mixed path's mu_w
 within 1e-6."""
+import types
+
 import numpy as np
 import pytest
 
-from freewake_oracle import OracleFreeWakeSim, oracle_explicit_rhs
+from oracle.freewake_oracle import OracleFreeWakeSim, oracle_explicit_rhs
 from vphip import api, dofmap, freewake as FW, mesh, sweep, wake as wk
 
 pytestmark = pytest.mark.gpu
@@ -38,25 +40,39 @@ def test_explicit_rhs(wing):
 
 
 def test_explicit_rhs_moved_sheets(wing):
-    """explicit_rhs with a plan cache: known sheets of one topology moved twice (the plan's wake updated in place,
-    vp_plan_update_wake) against the oracle at each position, then a different topology (a new plan)."""
-    print("explicit_rhs cached sheet plan", flush=True)
+    """explicit_rhs through one vp_explicit handle (the C ABI the Julia binding calls): known sheets of one topology
+    moved twice (the handle's plan moved in place) against the oracle at each position, then a different topology (a
+    new plan), then particles alone and with the sheets (the body plan, made once)."""
+    print("explicit_rhs handle, moved sheets", flush=True)
     g, dm = wing
     rng = np.random.default_rng(6)
-    cache = {}
+    h = FW.ExplicitRHS(g, dm)
     for k, length in enumerate((0.5, 0.5, 0.5, 0.8)):
         w = wk.build_wake(g, dm, V, wk.WakeOptions(length=length))
         if k in (1, 2):   # same topology, nodes moved downstream and sheared
             w.x = w.x + np.array([0.05 * k, 0.0, 0.02 * k]) + 0.01 * k * np.outer(w.x[:, 1], [0.0, 0.0, 1.0])
         e = FW.rigid_elements(w, rng.standard_normal(dm.wake_te.n))
-        got = FW.explicit_rhs(g, dm, e, None, cache=cache)
+        got = h(e, None)
         ref = oracle_explicit_rhs(g, dm, e, None, api.AssemblyOptions())
         assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max(), (k, np.abs(got - ref).max() / np.abs(ref).max())
-        if k in (1, 2):
-            assert cache["plan"] is plan, k   # reused
-        plan = cache["plan"]
-    FW.close_cache(cache)
-    assert not cache
+    assert h.info() == (2, 2, 4)
+    n = 300
+    blobs = FW.Blobs(np.column_stack([rng.uniform(1.0, 2.0, n), rng.uniform(-3.0, 3.0, n), rng.normal(0.0, 0.1, n)]),
+                     1e-3 * rng.standard_normal((n, 3)), np.full(n, 0.05))
+    for ee in (None, e):
+        got = h(ee, blobs)
+        ref = oracle_explicit_rhs(g, dm, ee, blobs, api.AssemblyOptions())
+        assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    assert h.info() == (2, 3, 6)
+    assert np.array_equal(h(None, None), np.zeros(dm.nslot))
+    # the body as a triangle soup (its elements' own corners, as the Julia binding passes `els`): the same values
+    soup = types.SimpleNamespace(x=g.x[g.tris].reshape(-1, 3), tris=np.arange(3 * len(g.tris)).reshape(-1, 3),
+                                 touch_pairs=[])
+    hs = FW.ExplicitRHS(soup, dm)
+    ref = h(e, blobs)
+    assert np.abs(hs(e, blobs) - ref).max() <= 1e-15 * np.abs(ref).max()
+    hs.close()
+    h.close()
     with pytest.raises(ValueError):
         p = api.AssemblyPlan(g, dm, w, api.AssemblyOptions(), wake_only=True)
         try:

@@ -1,6 +1,8 @@
 """GPU N-body sums (vp_surface_velocity, vp_body_velocity, vp_blob_velocities, vp_wake_field_*) against the FP64 oracle
 (src/post/Post.jl:32-112, src/freewake/step.jl:176-187).  FP64: within 1e-12 of the largest velocity; FP32 kernels
 with FP64 sums: within 2e-5 (source triangles) and 5e-5 (blobs) of it."""
+import types
+
 import numpy as np
 import pytest
 
@@ -122,6 +124,13 @@ def test_wake_field_graph(precision):
     assert launches == len(sizes)
     assert builds == 4 and updates == len(sizes) - builds   # rebuilt only when the blobs appear or vanish
     assert field(sig, gam, np.zeros((0, 3))).shape == (0, 3)
+    # the body as a triangle soup (each element with its own corners, as the Julia binding passes `els`)
+    soup = types.SimpleNamespace(x=g.x[g.tris].reshape(-1, 3), tris=np.arange(3 * nt).reshape(-1, 3))
+    fs = post.WakeField(soup, precision=precision)
+    xs = 3.0 * rng.standard_normal((400, 3))
+    bx, ba, bd = 3.0 * rng.standard_normal((90, 3)), rng.standard_normal((90, 3)), rng.uniform(0.05, 0.3, 90)
+    assert _err(fs(sig, gam, xs, bx, ba, bd, base=V), field(sig, gam, xs, bx, ba, bd, base=V)) <= 1e-15
+    fs.close()
     with pytest.raises(ValueError):
         field(sig[:-1], gam, np.ones((2, 3)))
     with pytest.raises(ValueError):

@@ -218,6 +218,129 @@ void vp_plan_destroy(vp_plan *plan)
     delete plan;
 }
 
+}  // extern "C"
+
+// explicit_rhs as a handle: copies of the body's arrays (a new plan of the known elements needs them again), the
+// two wake-only plans and the known elements as a wake (one node per corner and filament end)
+struct vp_explicit {
+    vp_ctx *ctx = nullptr;
+    vp_assembly_options opts{};
+    std::vector<double> x;
+    std::vector<int> tris, cs, touch;
+    vp_surface g{};
+    vp_plan *body = nullptr, *known = nullptr;
+    int nsheet = -1, nseg = -1;
+    std::vector<double> wx, wmu, tmp, zero;
+    std::vector<int> wtris, wfil;
+    long long counts[3] = {0, 0, 0};
+    ~vp_explicit()
+    {
+        vp_plan_destroy(known);
+        vp_plan_destroy(body);
+    }
+};
+
+extern "C" {
+
+int vp_explicit_create(vp_ctx *ctx, const vp_surface *g, const vp_assembly_options *opts, vp_explicit **out)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if (!g || !opts || !out || !g->x || !g->tris || !g->corner_slot || (g->ntouch > 0 && !g->touch))
+        return fail(&ctx->c, VP_E_ARGUMENT, "null argument");
+    if (g->nn <= 0 || g->nt <= 0 || g->nslot <= 0 || g->ntouch < 0) return fail(&ctx->c, VP_E_DIMENSION, "empty body");
+    *out = nullptr;
+    vp_explicit *h = new (std::nothrow) vp_explicit;
+    if (!h) return fail(&ctx->c, VP_E_DEVICE, "host allocation failed");
+    const int rc = guard(&ctx->c, [&] {
+        h->ctx = ctx;
+        h->opts = *opts;
+        h->x.assign(g->x, g->x + 3 * (size_t)g->nn);
+        h->tris.assign(g->tris, g->tris + 3 * (size_t)g->nt);
+        h->cs.assign(g->corner_slot, g->corner_slot + 3 * (size_t)g->nt);
+        if (g->ntouch > 0) h->touch.assign(g->touch, g->touch + 2 * (size_t)g->ntouch);
+        h->g = *g;
+        h->g.x = h->x.data();
+        h->g.tris = h->tris.data();
+        h->g.corner_slot = h->cs.data();
+        h->g.touch = h->touch.empty() ? nullptr : h->touch.data();
+        h->tmp.assign(g->nslot, 0.0);
+        h->zero.assign(g->nt, 0.0);
+    });
+    if (rc) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return VP_OK;
+}
+
+int vp_explicit_rhs(vp_explicit *h, int nsheet, const double *sheet_x, const double *sheet_mu, int nseg,
+                    const double *seg_x, const double *seg_gamma, int nblob, const double *bx, const double *alpha,
+                    const double *delta, double *out)
+{
+    if (!h || !h->ctx) return VP_E_ARGUMENT;
+    vp::Ctx *c = &h->ctx->c;
+    if (!out || nsheet < 0 || nseg < 0 || nblob < 0 || (nsheet > 0 && (!sheet_x || !sheet_mu)) ||
+        (nseg > 0 && (!seg_x || !seg_gamma)) || (nblob > 0 && (!bx || !alpha || !delta)))
+        return fail(c, VP_E_ARGUMENT, "null argument");
+    const int nslot = h->g.nslot;
+    for (int i = 0; i < nslot; i++) out[i] = 0.0;
+    h->counts[2]++;
+    static const double vzero[3] = {0.0, 0.0, 0.0};
+    if (nsheet + nseg > 0) {
+        const int nw = 3 * nsheet + 2 * nseg;
+        h->wx.resize(3 * (size_t)nw);
+        h->wmu.resize(nw);
+        std::memcpy(h->wx.data(), sheet_x, sizeof(double) * 9 * (size_t)nsheet);
+        std::memcpy(h->wmu.data(), sheet_mu, sizeof(double) * 3 * (size_t)nsheet);
+        std::memcpy(h->wx.data() + 9 * (size_t)nsheet, seg_x, sizeof(double) * 6 * (size_t)nseg);
+        std::memcpy(h->wmu.data() + 3 * (size_t)nsheet, seg_gamma, sizeof(double) * 2 * (size_t)nseg);
+        int rc;
+        if (h->known && h->nsheet == nsheet && h->nseg == nseg) {
+            if ((rc = vp_plan_update_wake(h->known, h->wx.data()))) return rc;
+            h->counts[1]++;
+        } else {
+            vp_plan_destroy(h->known);
+            h->known = nullptr;
+            h->wtris.resize(3 * (size_t)nsheet);
+            for (int i = 0; i < 3 * nsheet; i++) h->wtris[i] = i;
+            h->wfil.resize(2 * (size_t)nseg);
+            for (int i = 0; i < 2 * nseg; i++) h->wfil[i] = 3 * nsheet + i;
+            const vp_wake w{nw, h->wx.data(), nsheet, h->wtris.data(), nseg, h->wfil.data()};
+            if ((rc = vp_plan_create_ex(h->ctx, &h->g, &w, &h->opts, VP_PLAN_WAKE_ONLY, &h->known))) return rc;
+            h->nsheet = nsheet;
+            h->nseg = nseg;
+            h->counts[0]++;
+        }
+        if ((rc = vp_plan_assemble(h->known, h->zero.data(), vzero, nullptr, nullptr))) return rc;
+        if ((rc = vp_plan_wake_product(h->known, h->wmu.data(), h->tmp.data()))) return rc;
+        for (int i = 0; i < nslot; i++) out[i] -= h->tmp[i];
+    }
+    if (nblob > 0) {
+        int rc;
+        if (!h->body) {
+            if ((rc = vp_plan_create_ex(h->ctx, &h->g, nullptr, &h->opts, VP_PLAN_WAKE_ONLY, &h->body))) return rc;
+            if ((rc = vp_plan_assemble(h->body, h->zero.data(), vzero, nullptr, nullptr))) {
+                vp_plan_destroy(h->body);
+                h->body = nullptr;
+                return rc;
+            }
+        }
+        if ((rc = vp_plan_blob_rhs(h->body, nblob, bx, alpha, delta, h->tmp.data()))) return rc;
+        for (int i = 0; i < nslot; i++) out[i] += h->tmp[i];
+    }
+    return VP_OK;
+}
+
+int vp_explicit_info(const vp_explicit *h, long long *counts)
+{
+    if (!h || !counts) return VP_E_ARGUMENT;
+    for (int k = 0; k < 3; k++) counts[k] = h->counts[k];
+    return VP_OK;
+}
+
+void vp_explicit_destroy(vp_explicit *h) { delete h; }
+
 int vp_factorize(vp_ctx *ctx, const double *A, long long lda, int A_on_device, const vp_constraints *k, int mode,
                  vp_factor **out)
 {

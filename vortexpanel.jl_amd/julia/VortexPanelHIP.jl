@@ -23,7 +23,7 @@ using LinearAlgebra
 using SparseArrays
 using StaticArrays
 import VortexPanel
-import VortexPanel: AssemblyOptions, DOFMapping, RigidWake, SurfaceGeometry, bordered_rows
+import VortexPanel: AssemblyOptions, DOFMapping, RigidWake, SurfaceGeometry, bordered_rows, ExplicitSources
 
 const libvortexhip = get(ENV, "VORTEXHIP_LIB", normpath(joinpath(@__DIR__, "..", "lib", "libvortexhip.so")))
 
@@ -418,17 +418,24 @@ mutable struct WakeField
     nt::Int
 end
 
-function WakeField(g::SurfaceGeometry; kernel_far::Real=10.0, precision::Symbol=:fp64, ctx::Context=default_context())
-    x = flat3(g.x, Cdouble)
-    tris = Cint[t[k] - 1 for t in g.tris for k in 1:3]
+WakeField(g::SurfaceGeometry; kernel_far::Real=10.0, precision::Symbol=:fp64, ctx::Context=default_context()) =
+    WakeField(flat3(g.x, Cdouble), Cint[t[k] - 1 for t in g.tris for k in 1:3]; kernel_far, precision, ctx)
+
+# from the body's elements (as `wake_velocities` passes `sim.els`): each triangle with its own corners
+WakeField(els::AbstractVector; kernel_far::Real=10.0, precision::Symbol=:fp64, ctx::Context=default_context()) =
+    WakeField(Cdouble[e.x[k][d] for e in els for k in 1:3 for d in 1:3], Cint[0:3length(els)-1;]; kernel_far, precision,
+              ctx)
+
+function WakeField(x::Vector{Cdouble}, tris::Vector{Cint}; kernel_far::Real=10.0, precision::Symbol=:fp64,
+                   ctx::Context=default_context())
     out = Ref{Ptr{Cvoid}}(C_NULL)
     GC.@preserve x tris begin
-        sf = VPSurface(length(g.x), pointer(x), length(g.tris), pointer(tris), C_NULL, 0, 0, C_NULL)
+        sf = VPSurface(length(x) ÷ 3, pointer(x), length(tris) ÷ 3, pointer(tris), C_NULL, 0, 0, C_NULL)
         check(ctx, ccall((:vp_wake_field_create, libvortexhip), Cint,
             (Ptr{Cvoid}, Ref{VPSurface}, Cdouble, Cint, Ref{Ptr{Cvoid}}),
             ctx.ptr, Ref(sf), kernel_far, precision === :mixed ? 1 : 0, out))
     end
-    wf = WakeField(out[], ctx, length(g.tris))
+    wf = WakeField(out[], ctx, length(tris) ÷ 3)
     finalizer(w -> w.ptr == C_NULL || ccall((:vp_wake_field_destroy, libvortexhip), Cvoid, (Ptr{Cvoid},), w.ptr), wf)
     return wf
 end
@@ -455,6 +462,99 @@ function (wf::WakeField)(σ::AbstractVector, γ::AbstractVector, X::AbstractVect
             wf.ptr, sig, gam, length(X), Xf, length(blobs), bx, ba, bd, bv, u))
     end
     return [SVector(u[1, i], u[2, i], u[3, i]) for i in axes(u, 2)]
+end
+
+# ---- explicit_rhs (src/assembly/explicit.jl:31-86) ------------------------------------------------------------------
+
+"""
+    ExplicitRHS(els, dm; opts=AssemblyOptions(), ctx)
+
+A `vp_explicit` handle on one body: `explicit_rhs` of known sheets, filaments and particles with the body's plans kept
+on the device between calls (the plan of the known elements is moved in place while their counts stay, as a free
+wake's rows move every step).  The body is given by its elements (`FlatTriangle`s, as the reference passes `sim.els`):
+explicit_rhs pairs them only with the known elements, so they cross as triangles with their own corners.
+"""
+mutable struct ExplicitRHS
+    ptr::Ptr{Cvoid}
+    ctx::Context
+    nslot::Int
+end
+
+function ExplicitRHS(els::AbstractVector, dm::DOFMapping; opts::AssemblyOptions=AssemblyOptions(),
+                     ctx::Context=default_context())
+    nt = length(els)
+    length(dm.corner_slot) == nt || throw(DimensionMismatch("dm has $(length(dm.corner_slot)) triangles, els $nt"))
+    x = Cdouble[e.x[k][d] for e in els for k in 1:3 for d in 1:3]
+    tris = Cint[0:3nt-1;]
+    cs = Cint[c[k] - 1 for c in dm.corner_slot for k in 1:3]
+    o = c_options(opts, :fp64)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve x tris cs begin
+        sf = VPSurface(3nt, pointer(x), nt, pointer(tris), pointer(cs), dm.nslot, 0, C_NULL)
+        check(ctx, ccall((:vp_explicit_create, libvortexhip), Cint,
+            (Ptr{Cvoid}, Ref{VPSurface}, Ref{VPAssemblyOptions}, Ref{Ptr{Cvoid}}), ctx.ptr, Ref(sf), Ref(o), out))
+    end
+    h = ExplicitRHS(out[], ctx, dm.nslot)
+    finalizer(h -> h.ptr == C_NULL || ccall((:vp_explicit_destroy, libvortexhip), Cvoid, (Ptr{Cvoid},), h.ptr), h)
+    return h
+end
+
+function (h::ExplicitRHS)(src::ExplicitSources)
+    sx = Cdouble[t.x[k][d] for t in src.sheets for k in 1:3 for d in 1:3]
+    sm = Cdouble[m[k] for m in src.sheet_μ for k in 1:3]
+    fx = Cdouble[p[d] for s in src.segments for p in (s.a, s.b) for d in 1:3]
+    fg = Cdouble[g[k] for g in src.segment_Γ for k in 1:2]
+    bx = flat3([b.x for b in src.blobs], Cdouble)
+    ba = flat3([b.α for b in src.blobs], Cdouble)
+    bd = Cdouble[b.δ for b in src.blobs]
+    length(sm) == length(sx) ÷ 3 || throw(DimensionMismatch("one sheet_μ per sheet"))
+    length(fg) == length(fx) ÷ 3 || throw(DimensionMismatch("one segment_Γ per segment"))
+    b = zeros(Cdouble, h.nslot)
+    GC.@preserve sx sm fx fg bx ba bd b begin
+        check(h.ctx, ccall((:vp_explicit_rhs, libvortexhip), Cint,
+            (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Cint, Ptr{Cdouble},
+             Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}),
+            h.ptr, length(src.sheets), sx, sm, length(src.segments), fx, fg, length(src.blobs), bx, ba, bd, b))
+    end
+    return b
+end
+
+# one handle per body element vector, so the reference's step! (which calls explicit_rhs(sim.els, dm, src; opts)
+# every step) reuses its plans
+const EXPLICIT_HANDLES = WeakKeyDict{Any,Tuple{DOFMapping,AssemblyOptions,ExplicitRHS}}()
+
+"""
+    explicit_rhs(els, dm, src::ExplicitSources; opts=AssemblyOptions(), ctx)
+
+The reference's `explicit_rhs` (src/assembly/explicit.jl:31) on the GPU: -∫ψ_i n·u_src dA for every slot.  The handle
+of `els` is kept between calls (while `dm` and `opts` stay the same), so a free-wake loop pays for its plans once.
+"""
+function explicit_rhs(els::AbstractVector, dm::DOFMapping, src::ExplicitSources; opts::AssemblyOptions=AssemblyOptions(),
+                      ctx::Context=default_context())
+    e = get(EXPLICIT_HANDLES, els, nothing)
+    if e === nothing || e[1] !== dm || e[2] != opts || e[3].ctx !== ctx
+        e = (dm, opts, ExplicitRHS(els, dm; opts=opts, ctx=ctx))
+        EXPLICIT_HANDLES[els] = e
+    end
+    return e[3](src)
+end
+
+const WAKE_FIELDS = WeakKeyDict{Any,Tuple{Float64,Symbol,WakeField}}()
+
+"""
+    wake_field(els; kernel_far=10.0, precision=:fp64, ctx)
+
+The `WakeField` of a body's element vector, made on the first call and kept while `els` lives: the handle the
+reference's `wake_velocities` (src/freewake/step.jl:166-173) uses every step,
+`wake_field(sim.els; kernel_far)(sim.σ, sheet_strengths(sim.els, sim.dm, sol.μ), X, blobs; base=sim.V∞)`.
+"""
+function wake_field(els::AbstractVector; kernel_far::Real=10.0, precision::Symbol=:fp64, ctx::Context=default_context())
+    e = get(WAKE_FIELDS, els, nothing)
+    if e === nothing || e[1] != kernel_far || e[2] !== precision || e[3].ctx !== ctx
+        e = (Float64(kernel_far), precision, WakeField(els; kernel_far, precision, ctx))
+        WAKE_FIELDS[els] = e
+    end
+    return e[3]
 end
 
 # ---- the adjoint's device pieces -----------------------------------------------------------------------------------

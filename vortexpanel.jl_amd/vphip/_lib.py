@@ -42,7 +42,7 @@ EXPORTS = ["vp_create", "vp_destroy", "vp_last_error", "vp_sync", "vp_assemble",
            "vp_wake_field_create", "vp_wake_field_eval", "vp_wake_field_info", "vp_wake_field_destroy",
            "vp_plan_update_wake", "vp_assembly_tuning", "vp_factor_mode",
            "vp_consistency", "vp_blob_velocities_device", "vp_device_alloc", "vp_device_free", "vp_copy_to_device",
-           "vp_copy_from_device"]
+           "vp_copy_from_device", "vp_explicit_create", "vp_explicit_rhs", "vp_explicit_info", "vp_explicit_destroy"]
 
 
 def load():
@@ -61,6 +61,7 @@ def load():
         lib.vp_destroy.restype = None
         lib.vp_plan_destroy.restype = None
         lib.vp_wake_field_destroy.restype = None
+        lib.vp_explicit_destroy.restype = None
         _lib = lib
     return _lib
 

@@ -10,6 +10,7 @@ FP32 kernels with FP64 sums (the circulation bookkeeping stays FP64).
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 import os
 import time
@@ -18,8 +19,10 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
+from . import _lib as L
 from . import post
-from .api import AssemblyOptions, AssemblyPlan, Context, build_elements, default_context, source_strengths
+from .api import (AssemblyOptions, AssemblyPlan, Context, _check, _surface_c, build_elements, default_context,
+                  source_strengths)
 from .dofmap import wake_rows
 from .solve import factorize_galerkin, solve_factored
 from .wake import te_panel
@@ -360,69 +363,67 @@ def vorticity_moments(b):
 
 # ------------------------------------------------------------------------------------------ explicit rhs -------------
 
-def _known_wake(e):
-    """The known sheets and filaments as a wake: one node per corner and per filament end, with their doublets"""
-    from .wake import RigidWake, WakeOptions
-    ns, nf = len(e.sheets), len(e.segments)
-    x = np.zeros((3 * ns + 2 * nf, 3))
-    mu = np.zeros(3 * ns + 2 * nf)
-    for i, ((p, q, r), m) in enumerate(zip(e.sheets, e.sheet_mu)):
-        x[3 * i], x[3 * i + 1], x[3 * i + 2] = p, q, r
-        mu[3 * i:3 * i + 3] = m
-    for j, ((a, b), (ga, gb)) in enumerate(zip(e.segments, e.segment_gamma)):
-        x[3 * ns + 2 * j], x[3 * ns + 2 * j + 1] = a, b
-        mu[3 * ns + 2 * j], mu[3 * ns + 2 * j + 1] = ga, gb
-    tris = np.arange(3 * ns, dtype=np.int32).reshape(-1, 3)
-    fil = (3 * ns + np.arange(2 * nf, dtype=np.int32)).reshape(-1, 2)
-    w = RigidWake(x=x, te=np.zeros(len(x), np.int64), row=np.zeros(len(x), np.int64), tris=tris, filaments=fil,
-                  rows=np.zeros(1), span=1.0, options=WakeOptions())
-    return w, mu
+class ExplicitRHS:
+    """explicit_rhs (src/assembly/explicit.jl:31-86) on one body, through the vp_explicit_* handle: the known sheets and
+    filaments take the assembly's tiers (a wake-only plan: exactly their assembled columns times their doublets, one
+    wake node per corner and filament end), the particles the far rule or near_rule.  The handle moves its plan of the
+    known elements in place while their counts stay (the free rows move every step, keep their triangles) and keeps
+    a plan of the body for the particles."""
+
+    def __init__(self, g, dm, opts: AssemblyOptions | None = None, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.nslot = dm.nslot
+        s, _keep = _surface_c(g, dm)  # the handle copies the arrays
+        oc = (opts or AssemblyOptions()).to_c()
+        h = C.c_void_p()
+        _check(self.ctx.h, self.ctx.lib.vp_explicit_create(self.ctx.h, C.byref(s), C.byref(oc), C.byref(h)))
+        self.h = h
+
+    def __call__(self, e: WakeElements | None, blobs: Blobs | None):
+        ns = len(e.sheets) if e is not None else 0
+        nf = len(e.segments) if e is not None else 0
+        sx = L.f64(np.asarray(e.sheets, dtype=np.float64).reshape(ns, 9)) if ns else None
+        sm = L.f64(np.asarray(e.sheet_mu, dtype=np.float64).reshape(ns, 3)) if ns else None
+        fx = L.f64(np.asarray(e.segments, dtype=np.float64).reshape(nf, 6)) if nf else None
+        fg = L.f64(np.asarray(e.segment_gamma, dtype=np.float64).reshape(nf, 2)) if nf else None
+        nb = len(blobs) if blobs is not None else 0
+        bx = L.f64(blobs.x) if nb else None
+        ba = L.f64(blobs.alpha) if nb else None
+        bd = L.f64(blobs.delta) if nb else None
+        out = np.zeros(self.nslot)
+        rc = self.ctx.lib.vp_explicit_rhs(self.h, ns, L.ptr(sx), L.ptr(sm), nf, L.ptr(fx), L.ptr(fg), nb, L.ptr(bx),
+                                          L.ptr(ba), L.ptr(bd), L.ptr(out))
+        _check(self.ctx.h, rc)
+        return out
+
+    def info(self):
+        """(plans of known elements built, moved in place, calls)"""
+        c = np.zeros(3, dtype=np.int64)
+        _check(self.ctx.h, self.ctx.lib.vp_explicit_info(self.h, L.ptr(c, C.c_longlong)))
+        return tuple(int(v) for v in c)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.vp_explicit_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def explicit_rhs(g, dm, e: WakeElements | None, blobs: Blobs | None, opts: AssemblyOptions | None = None,
-                 ctx: Context | None = None, body_plan: AssemblyPlan | None = None, cache: dict | None = None):
-    """-int psi_i n.u_src dA for every slot (src/assembly/explicit.jl:31-86) on the device.  The sheets and filaments
-    take the assembly's tiers (a wake-only plan: exactly their assembled columns times their doublets); the particles
-    the far rule or near_rule (the plan's particle kernels).  body_plan: a wake-only plan of the body alone, reused for
-    the particles.  cache: a dict that keeps the sheets' plan between calls; a call with as many sheets and filaments
-    as the last moves that plan's wake (AssemblyPlan.update_wake) instead of building a new one (close it with
-    close_cache)."""
-    ctx = ctx or default_context()
-    opts = opts or AssemblyOptions()
-    b = np.zeros(dm.nslot)
-    zero = np.zeros(len(g.tris))
-    if e is not None and (e.sheets or e.segments):
-        w, mu = _known_wake(e)
-        key = (len(e.sheets), len(e.segments))
-        plan = cache.get("plan") if cache is not None and cache.get("key") == key else None
-        if plan is not None:
-            plan.update_wake(w.x)
-        else:
-            close_cache(cache)
-            plan = AssemblyPlan(g, dm, w, opts, ctx=ctx, wake_only=True)
-        plan.assemble(zero, np.zeros(3))
-        b -= plan.wake_product(mu)
-        if cache is not None:
-            cache.update(key=key, plan=plan)
-        else:
-            plan.close()
-    if blobs is not None and len(blobs):
-        plan = body_plan
-        if plan is None:
-            plan = AssemblyPlan(g, dm, None, opts, ctx=ctx, wake_only=True)
-            plan.assemble(zero, np.zeros(3))
-        b += plan.blob_rhs(blobs.x, blobs.alpha, blobs.delta)
-        if body_plan is None:
-            plan.close()
-    return b
-
-
-def close_cache(cache: dict | None):
-    """release the sheets' plan an explicit_rhs cache keeps"""
-    if cache and cache.get("plan") is not None:
-        cache["plan"].close()
-    if cache is not None:
-        cache.clear()
+                 ctx: Context | None = None, handle: ExplicitRHS | None = None):
+    """-int psi_i n.u_src dA for every slot (src/assembly/explicit.jl:31-86) on the device: one call of an
+    ExplicitRHS handle (a temporary one unless `handle` is given)"""
+    h = handle or ExplicitRHS(g, dm, opts, ctx)
+    try:
+        return h(e, blobs)
+    finally:
+        if handle is None:
+            h.close()
 
 
 # ------------------------------------------------------------------------------------------ the simulation ----------
@@ -511,14 +512,11 @@ class FreeWakeSim:
         self.b0 = plan.read_b()
         self.F = factorize_galerkin(self.dm, plan, W=wake_rows(self.dm, self.te.te), mode=self.precision)
         plan.close()
-        self.body_plan = AssemblyPlan(self.g, self.dm, None, self.assembly, ctx=ctx, wake_only=True)
-        self.body_plan.assemble(np.zeros(len(self.g.tris)), np.zeros(3))
+        self.explicit = ExplicitRHS(self.g, self.dm, self.assembly, ctx)  # its plans are moved from step to step
         self.field = post.WakeField(self.g, kernel_far=self.assembly.kernel_far, precision=self.precision, ctx=ctx)
-        self._sheet_plan = {}  # the free rows' sheets: one wake-only plan, moved from step to step
 
     def _explicit_rhs(self, we, particles):
-        return explicit_rhs(self.g, self.dm, we, particles, self.assembly, ctx=self.ctx, body_plan=self.body_plan,
-                            cache=self._sheet_plan)
+        return self.explicit(we, particles)
 
     def _solve(self, rhs):
         out = solve_factored(self.F, rhs)
@@ -623,6 +621,5 @@ class FreeWakeSim:
 
     def close(self):
         self.F.close()
-        self.body_plan.close()
+        self.explicit.close()
         self.field.close()
-        close_cache(self._sheet_plan)
