@@ -65,12 +65,13 @@ def test_explicit_rhs_moved_sheets(wing):
         assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
     assert h.info() == (2, 3, 6)
     assert np.array_equal(h(None, None), np.zeros(dm.nslot))
-    # the body as a triangle soup (its elements' own corners, as the Julia binding passes `els`): the same values
+    # the body as a triangle soup (its elements' own corners, as the Julia binding passes `els`): the same values to
+    # rounding (the atomic sums' order differs; measured 1.3e-15)
     soup = types.SimpleNamespace(x=g.x[g.tris].reshape(-1, 3), tris=np.arange(3 * len(g.tris)).reshape(-1, 3),
                                  touch_pairs=[])
     hs = FW.ExplicitRHS(soup, dm)
     ref = h(e, blobs)
-    assert np.abs(hs(e, blobs) - ref).max() <= 1e-15 * np.abs(ref).max()
+    assert np.abs(hs(e, blobs) - ref).max() <= 1e-13 * np.abs(ref).max()
     hs.close()
     h.close()
     with pytest.raises(ValueError):

@@ -129,7 +129,7 @@ def test_wake_field_graph(precision):
     fs = post.WakeField(soup, precision=precision)
     xs = 3.0 * rng.standard_normal((400, 3))
     bx, ba, bd = 3.0 * rng.standard_normal((90, 3)), rng.standard_normal((90, 3)), rng.uniform(0.05, 0.3, 90)
-    assert _err(fs(sig, gam, xs, bx, ba, bd, base=V), field(sig, gam, xs, bx, ba, bd, base=V)) <= 1e-15
+    assert _err(fs(sig, gam, xs, bx, ba, bd, base=V), field(sig, gam, xs, bx, ba, bd, base=V)) <= 1e-13
     fs.close()
     with pytest.raises(ValueError):
         field(sig[:-1], gam, np.ones((2, 3)))
