@@ -1,6 +1,7 @@
 """GPU N-body sums (vp_surface_velocity, vp_body_velocity, vp_blob_velocities, vp_wake_field_*) against the FP64 oracle
 (src/post/Post.jl:32-112, src/freewake/step.jl:176-187).  FP64: within 1e-12 of the largest velocity; FP32 kernels
 with FP64 sums: within 2e-5 (source triangles) and 5e-5 (blobs) of it."""
+import os
 import types
 
 import numpy as np
@@ -131,6 +132,15 @@ def test_wake_field_graph(precision):
     bx, ba, bd = 3.0 * rng.standard_normal((90, 3)), rng.standard_normal((90, 3)), rng.uniform(0.05, 0.3, 90)
     assert _err(fs(sig, gam, xs, bx, ba, bd, base=V), field(sig, gam, xs, bx, ba, bd, base=V)) <= 1e-13
     fs.close()
+    # VP_GRAPHS=0: the same three kernels launched one by one, no graph launch
+    launches = field.info()[2]
+    os.environ["VP_GRAPHS"] = "0"
+    try:
+        eager = field(sig, gam, xs, bx, ba, bd, base=V)
+    finally:
+        del os.environ["VP_GRAPHS"]
+    assert field.info()[2] == launches
+    assert _err(eager, field(sig, gam, xs, bx, ba, bd, base=V)) <= 1e-15
     with pytest.raises(ValueError):
         field(sig[:-1], gam, np.ones((2, 3)))
     with pytest.raises(ValueError):

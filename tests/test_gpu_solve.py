@@ -146,12 +146,14 @@ def test_wing_transposed_solve(mode):
     F.close()
 
 
-@pytest.mark.parametrize("mode", ["fp64", "mixed"])
-def test_repeated_solves_replay(mode):
+@pytest.mark.parametrize("mode,graphs", [("fp64", "1"), ("mixed", "1"), ("mixed", "0")])
+def test_repeated_solves_replay(mode, graphs, monkeypatch):
     """The factors' inverse is applied eagerly on its first call for a direction and number of right hand sides,
     captured as a HIP graph on the second and replayed after that: every solve of a sequence that changes the
     direction and the batch width (forcing new captures), with eager work between them, against numpy's solve of
-    the oracle's bordered matrix; the residual GEMV (1-4 right hand sides) and the MFMA GEMM (more) both take part."""
+    the oracle's bordered matrix; the residual GEMV (1-4 right hand sides) and the MFMA GEMM (more) both take part.
+    With VP_GRAPHS=0 every application is eager."""
+    monkeypatch.setenv("VP_GRAPHS", graphs)
     g = mesh.naca_wing(mesh.WingSpec(code="2412", n_around=20, n_span=6, aspect=5.0))
     dm = dofmap.build_dofmap(g)
     V = np.array([np.cos(0.07), 0.0, np.sin(0.07)])

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -38,6 +39,15 @@ struct RuleTab {  // all rules' points (3 barycentric coordinates each) and weig
 enum TuneKernel { TK_TOUCH = 0, TK_NEARPTS = 1, TK_NEARGRP = 2, TK_GROUP = 3, TK_COUNT = 4 };
 constexpr int kTuneWaves[2] = {3, 2};
 long long next_plan_serial();
+
+// HIP graph replay of the narrow solves and of the wake field (default on); VP_GRAPHS=0 launches the same kernels one
+// by one instead (for tools that cannot follow graph launches: rocprofv3's kernel trace has crashed inside
+// hipGraphLaunch here)
+inline bool graphs_enabled()
+{
+    const char *e = std::getenv("VP_GRAPHS");  // read per call (cheap), so a process may switch it
+    return !(e && e[0] == '0');
+}
 std::string asm_tuning_report();
 
 struct Plan {

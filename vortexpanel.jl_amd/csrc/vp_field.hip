@@ -550,6 +550,14 @@ void WakeField::eval(const double *sigma, const double *gamma, int np, const dou
     kp[2].kernelParams = red_args;
 
     const int want = nb > 0 ? 1 : 0;
+    if (!graphs_enabled()) {  // the same three kernels, launched one by one
+        for (int k = 0; k < 3; k++)
+            if (k != 1 || want)
+                FCHECK(hipLaunchKernel(kp[k].func, kp[k].gridDim, kp[k].blockDim, kp[k].kernelParams, 0, st));
+        FCHECK(hipMemcpyAsync(hu, u, sizeof(double) * 3 * np, hipMemcpyDeviceToHost, st));
+        FCHECK(hipStreamSynchronize(st));
+        return;
+    }
     if (!exec || shape != want) {
         if (exec) FCHECK(hipGraphExecDestroy(exec));
         if (graph) FCHECK(hipGraphDestroy(graph));

@@ -1082,6 +1082,10 @@ void Factor::apply_inverse(int nrhs, bool trans)
 void Factor::apply_inverse_replay(int nrhs, bool trans)
 {
     hipStream_t s = ctx->stream;
+    if (!graphs_enabled()) {
+        apply_inverse(nrhs, trans);
+        return;
+    }
     InvGraph &g = inv_graph[trans ? 1 : 0];
     if (g.nrhs != nrhs || g.epoch != ws_epoch) {
         if (g.exec) HIPCHECK2(hipGraphExecDestroy(g.exec));
