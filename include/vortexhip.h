@@ -260,15 +260,16 @@ void vp_wake_field_destroy(vp_wake_field *h);
 
 /* Test entry points (host arrays, row-major): C = op(A) B or C -= op(A) B with the library's MFMA GEMM (prec 0 FP64,
  * 1 FP32; B K x N, C M x N, all doubles on the host).  flags: bit 0 subtract; bit 1 transposed A (A stored K x M,
- * op(A) = Aᵀ), else A is M x K; bits 4-7 the tile shape's index among the autotuner's eight candidates (0 is the
- * default).  And x = M^-1 r for a dense n x n matrix through the randomized LU (mode 0 FP64, 1 FP32 factors) with
+ * op(A) = Aᵀ), else A is M x K; bits 4-7 the autotuner's candidate: 0-7 the tile shapes (0 is the default), 8 the
+ * library GEMM (rocBLAS) for subtract without transposed A, the default tile otherwise.  And x = M^-1 r for a dense n x n matrix through the randomized LU (mode 0 FP64, 1 FP32 factors) with
  * FP64 refinement. */
 int vp_gemm(vp_ctx *ctx, int prec, int flags, int M, int N, int K, const double *A, const double *B, double *C);
 int vp_dense_solve(vp_ctx *ctx, int n, const double *M, int mode, const double *r, double *x, int *iters);
 /* Test entry point: the device's FP64 natural log (fn 0: out = log a), two-argument arctangent (fn 1:
  * out = atan2(a, b)), reciprocal square root (fn 2: out = 1/sqrt(a), a normal and positive) and square root of a
- * squared distance (fn 3: out = sqrt(a), a zero or normal and positive) as the kernels evaluate them, elementwise
- * over n host values. */
+ * squared distance (fn 3: out = sqrt(a), a zero or normal and positive) as the kernels evaluate them, and the mixed
+ * path's FP32 atan2 (fn 4: out = atan2(a, b), both rounded to FP32) and log(1 + u) (fn 5: u = a >= 0 rounded to FP32),
+ * elementwise over n host values. */
 int vp_math_probe(vp_ctx *ctx, int fn, int n, const double *a, const double *b, double *out);
 
 #ifdef __cplusplus

@@ -80,3 +80,21 @@ def test_distance_sqrt():
     out, ref = _probe(3, x), np.sqrt(x)
     assert np.max(_ulps(out, ref)) <= 3.0
     assert _probe(3, np.array([0.0]))[0] == 0.0
+
+
+def test_fp32_atan2_log1p():
+    """The mixed path's FP32 atan2 (a degree-8 polynomial fitted here, with octant reduction) and log(1 + u) (odd
+    series below 0.5, the hardware log above) against numpy's FP64 values of the same FP32 inputs: within 5e-7 of
+    atan2 (relative, and 2e-7 absolute near 0) in every octant and at the axes, within 5e-7 relative of log1p (a few
+    FP32 ulp: the hardware reciprocal and logarithm are within about 1 ulp)"""
+    print("device fp32 atan2 and log1p", flush=True)
+    rng = np.random.default_rng(8)
+    y = np.concatenate([rng.standard_normal(20000) * 10.0 ** rng.uniform(-6, 6, 20000), [0.0, 1.0, -1.0, 0.0, 3.0]])
+    x = np.concatenate([rng.standard_normal(20000) * 10.0 ** rng.uniform(-6, 6, 20000), [1.0, 0.0, 0.0, -2.0, 3.0]])
+    y32, x32 = y.astype(np.float32).astype(np.float64), x.astype(np.float32).astype(np.float64)
+    out, ref = _probe(4, y32, x32), np.arctan2(y32, x32)
+    assert np.all(np.abs(out - ref) <= 5e-7 * np.abs(ref) + 2e-7), np.max(np.abs(out - ref) / (np.abs(ref) + 1e-300))
+    u = np.concatenate([10.0 ** rng.uniform(-12, 6, 20000), [0.0, 0.4999999, 0.5, 1.0]]).astype(np.float32)
+    u = u.astype(np.float64)
+    out, ref = _probe(5, u), np.log1p(u)
+    assert np.all(np.abs(out - ref) <= 5e-7 * ref + 1e-300), np.max(np.abs(out - ref) / np.maximum(ref, 1e-300))

@@ -17,7 +17,9 @@ VG = np.array([1.0, 0.3, -0.2]) / np.linalg.norm([1.0, 0.3, -0.2])
 @pytest.mark.parametrize("shape", [(16, 16, 4), (64, 64, 64), (130, 70, 33), (300, 1, 257), (17, 200, 64),
                                    (260, 300, 64)])
 def test_gemm(prec, shape):
-    """Every autotuner tile candidate, plain and transposed A, set and subtract, against numpy."""
+    """Every autotuner candidate (the eight tile shapes and the library GEMM, index 8, which takes the trailing
+    update's form, subtract without transposed A, and falls back to the default tile otherwise), plain and
+    transposed A, set and subtract, against numpy."""
     M, N, K = shape
     print(f"gemm prec={prec} M={M} N={N} K={K}", flush=True)
     rng = np.random.default_rng(M * 7 + N * 3 + K)
@@ -26,7 +28,7 @@ def test_gemm(prec, shape):
         A, B, Cm = (x.astype(np.float32).astype(np.float64) for x in (A, B, Cm))
     At = np.ascontiguousarray(A.T)
     ctx = api.default_context()
-    for tile in range(8):
+    for tile in range(9):
         for ta in (0, 1):
             for sub in (0, 1):
                 out = np.ascontiguousarray(Cm.copy())
@@ -43,7 +45,7 @@ def test_gemm(prec, shape):
 def test_gemm_rejects_bad_flags():
     ctx = api.default_context()
     A = np.zeros(4)
-    for flags in (4, 8 << 4, 1 << 8):
+    for flags in (4, 9 << 4, 1 << 8):
         rc = ctx.lib.vp_gemm(ctx.h, 0, flags, 2, 2, 2, *(A.ctypes.data_as(C.POINTER(C.c_double)) for _ in range(3)))
         assert rc == L.VP_E_ARGUMENT
 

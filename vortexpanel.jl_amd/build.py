@@ -13,13 +13,15 @@ SOURCES = ["vp_assemble.hip", "vp_lu.hip", "vp_field.hip", "vp_capi.hip"]
 # -packed-fp32-ops off: no v_pk_*_f32 instructions anywhere.
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-munsafe-fp-atomics",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# rocBLAS: the LU autotuner's library candidate for the trailing update (a plain GEMM)
+LIBS = ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocblas"]
 
 
 def build_library(verbose: bool = False) -> str:
     os.makedirs(os.path.dirname(OUT), exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = OUT + ".tmp"
-    cmd = ["hipcc", *HIPFLAGS, "-o", tmp, *srcs]
+    cmd = ["hipcc", *HIPFLAGS, "-o", tmp, *srcs, *LIBS]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)

@@ -1276,6 +1276,26 @@ __device__ __forceinline__ void near_point(const V3<double> *X, const V3<double>
     }
 }
 
+// One of the twelve adds of near_scatter (entries 0-8: block row a = e / 3 and source column b = e % 3; 9-11: right
+// hand side row e - 9), so that a pair's adds can be spread over lanes that all hold its sums: e differs from lane to
+// lane, so the sums are picked with constant indices (a register array indexed by e would go to scratch memory).
+__device__ __forceinline__ void near_scatter_entry(const AsmParams &P, int e, int t, int s, const Elem &S,
+                                                   const double *tot, const double *sigma, const int *corner_slot,
+                                                   const int *src_cols, double *A, double *bvec)
+{
+    const int *ct = corner_slot + 3 * t;
+    if (e < 9) {
+        const int a = e / 3, bb = e - 3 * a;
+        V3<double> va = mk3(tot[0], tot[1], tot[2]);
+        if (a == 1) va = mk3(tot[3], tot[4], tot[5]);
+        if (a == 2) va = mk3(tot[6], tot[7], tot[8]);
+        unsafeAtomicAdd(&A[(long long)src_cols[3 * s + bb] * P.lda + ct[a]], dot(va, ld3(S.g[bb])));
+    } else if (s < P.nt) {
+        const double w = e == 9 ? tot[9] : e == 10 ? tot[10] : tot[11];
+        unsafeAtomicAdd(&bvec[ct[e - 9]], -sigma[s] * w);
+    }
+}
+
 // The pair's block from the accumulated sums: A[ct_a, cs_b] += (sum W psi_a w x n) . gamma_b, b[ct_a] -= sigma sum ...
 __device__ __forceinline__ void near_scatter(const AsmParams &P, int t, int s, const Elem &S, const double *tot,
                                              const double *sigma, const int *corner_slot, const int *src_cols, double *A,
@@ -1333,10 +1353,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPS, WPS)))
             for (int k = 0; k < nr; k++) sum += __shfl(acc[i], first + k);
             tot[i] = sum;
         }
-        if (live && q == 0) {
-            near_scatter(P, t, s, el[s], tot, sigma, corner_slot, src_cols, A, bvec);
-            if (tiers) tiers[(size_t)t * P.nsrc + s] = (uint8_t)T_NEAR;
-            mypairs++;
+        if (live) {
+            // the pair's twelve adds spread over its lanes (entries q and q + nr): two atomic instructions per wave
+            // instead of twelve from one lane per pair (the adds took 1.8 of the kernel's 2.8 ms on the abused
+            // ellipsoid; spread, it runs 2.54 ms, with the lane-indexed sums staged in LDS by the compiler)
+            for (int e = q; e < 12; e += nr) near_scatter_entry(P, e, t, s, el[s], tot, sigma, corner_slot, src_cols, A, bvec);
+            if (q == 0) {
+                if (tiers) tiers[(size_t)t * P.nsrc + s] = (uint8_t)T_NEAR;
+                mypairs++;
+            }
         }
     }
     if (mypairs) atomicAdd(&wst, mypairs);
@@ -1605,6 +1630,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NADJ == 0 ?
         }
         double tot[12];
         for (int i = 0; i < 12; i++) tot[i] = wave_sum(acc[i]);
+        // one lane adds the pair's twelve entries: spread over twelve lanes, the lane-indexed sums went to LDS and
+        // scratch and the kernel ran 5.50 -> 6.05 ms on the abused ellipsoid
         if (lane == 0) {
             near_scatter(P, t, s, S, tot, sigma, corner_slot, src_cols, A, bvec);
             if (tiers) tiers[(size_t)t * P.nsrc + s] = (uint8_t)T_NEAR;

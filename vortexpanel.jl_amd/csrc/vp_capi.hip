@@ -479,7 +479,7 @@ int vp_gemm(vp_ctx *ctx, int prec, int flags, int M, int N, int K, const double 
 {
     if (!ctx) return VP_E_ARGUMENT;
     if (!A || !B || !C || (prec != 0 && prec != 1) || M <= 0 || N <= 0 || K <= 0 || (flags & ~0xF3) ||
-        ((flags >> 4) & 15) >= 8)
+        ((flags >> 4) & 15) > 8)
         return fail(&ctx->c, VP_E_ARGUMENT, "bad argument");
     return guard(&ctx->c, [&] { vp::gemm_host(ctx->c.stream, prec, flags, M, N, K, A, B, C); });
 }
@@ -487,7 +487,7 @@ int vp_gemm(vp_ctx *ctx, int prec, int flags, int M, int N, int K, const double 
 int vp_math_probe(vp_ctx *ctx, int fn, int n, const double *a, const double *b, double *out)
 {
     if (!ctx) return VP_E_ARGUMENT;
-    if (!a || !out || n <= 0 || fn < 0 || fn > 3 || (fn == 1 && !b)) return fail(&ctx->c, VP_E_ARGUMENT, "bad argument");
+    if (!a || !out || n <= 0 || fn < 0 || fn > 5 || ((fn == 1 || fn == 4) && !b)) return fail(&ctx->c, VP_E_ARGUMENT, "bad argument");
     return guard(&ctx->c, [&] { vp::math_probe_host(&ctx->c, fn, n, a, b, out); });
 }
 

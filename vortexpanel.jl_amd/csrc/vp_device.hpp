@@ -344,22 +344,23 @@ template <typename T> __device__ __forceinline__ V3<T> sv_far_mom(const FarMom<T
 
 // ---- FP32 closed form (the mixed path's far-tier closed-form points and plain near pairs) ----
 
-// atan2 in FP32: atan t on [0, 1] as t P(t^2), P of degree 8 fitted here by least squares in relative error on 4000
-// Chebyshev nodes of [0, 1] (1.7e-8 of atan in exact arithmetic, about 1.5 ulp evaluated in FP32), with the octant
+// atan2 in FP32: atan t on [0, 1] as t P(t^2), P of degree 8 fitted here to atan(t)/t in relative error on 4000
+// Chebyshev nodes of [0, 1], by Lawson's iteratively reweighted least squares towards the minimax fit (1.5e-8 of atan
+// in exact arithmetic, 1.6e-7 evaluated in FP32), with the octant
 // reduction atan2 = pi/2 - atan(1/t) above the diagonal and pi - atan for x < 0.
 __device__ __forceinline__ float fatan2(float y, float x)
 {
     const float ax = fabsf(x), ay = fabsf(y), mx = fmaxf(ax, ay), mn = fminf(ax, ay);
     const float t = mx > 0.f ? mn * __builtin_amdgcn_rcpf(mx) : 0.f, z = t * t;
-    float p = 2.878729539e-03f;
-    p = fmaf(p, z, -1.619798713e-02f);
-    p = fmaf(p, z, 4.293066915e-02f);
-    p = fmaf(p, z, -7.527754620e-02f);
-    p = fmaf(p, z, 1.065401495e-01f);
-    p = fmaf(p, z, -1.420775599e-01f);
-    p = fmaf(p, z, 1.999329349e-01f);
-    p = fmaf(p, z, -3.333312076e-01f);
-    p = fmaf(p, z, 9.999999930e-01f);
+    float p = 2.849886427e-03f;
+    p = fmaf(p, z, -1.606861502e-02f);
+    p = fmaf(p, z, 4.269149899e-02f);
+    p = fmaf(p, z, -7.504292578e-02f);
+    p = fmaf(p, z, 1.064093336e-01f);
+    p = fmaf(p, z, -1.420364380e-01f);
+    p = fmaf(p, z, 1.999261975e-01f);
+    p = fmaf(p, z, -3.333307207e-01f);
+    p = fmaf(p, z, 1.000000000e+00f);
     float r = t * p;
     if (ay > ax) r = 1.57079633f - r;
     if (x < 0.f) r = 3.14159265f - r;

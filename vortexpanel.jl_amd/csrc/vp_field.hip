@@ -594,7 +594,15 @@ void WakeField::eval(const double *sigma, const double *gamma, int np, const dou
 __global__ void math_probe(int fn, int n, const double *__restrict__ a, const double *__restrict__ b, double *out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = fn == 0 ? dlog(a[i]) : fn == 1 ? datan2(a[i], b[i]) : fn == 2 ? vrsqrt(a[i]) : psqrt(a[i]);
+    if (i >= n) return;
+    switch (fn) {
+    case 0: out[i] = dlog(a[i]); break;
+    case 1: out[i] = datan2(a[i], b[i]); break;
+    case 2: out[i] = vrsqrt(a[i]); break;
+    case 3: out[i] = psqrt(a[i]); break;
+    case 4: out[i] = fatan2((float)a[i], (float)b[i]); break;
+    default: out[i] = flog1p((float)a[i]); break;
+    }
 }
 
 void math_probe_host(Ctx *c, int fn, int n, const double *a, const double *b, double *out)
@@ -602,7 +610,7 @@ void math_probe_host(Ctx *c, int fn, int n, const double *a, const double *b, do
     hipStream_t st = c->stream;
     Owned o;
     o.st = st;
-    double *da = dput(a, (size_t)n, st, o.v), *db = dput(fn == 1 ? b : a, (size_t)n, st, o.v);
+    double *da = dput(a, (size_t)n, st, o.v), *db = dput(fn == 1 || fn == 4 ? b : a, (size_t)n, st, o.v);
     double *dout = dnew<double>((size_t)n, o.v);
     math_probe<<<(n + 255) / 256, 256, 0, st>>>(fn, n, da, db, dout);
     FCHECK(hipGetLastError());

@@ -12,6 +12,7 @@
 //
 // Layouts: all device matrices are row-major with a leading dimension; right hand sides are n x nrhs row-major.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <cstdlib>
 #include <mutex>
@@ -179,11 +180,50 @@ void launch_tile(hipStream_t s, int M, int N, int K, const T *A, long long lda, 
     HIPCHECK2(hipGetLastError());
 }
 
+// The library candidate of the trailing update, index kNumTiles after the tile shapes: rocBLAS's GEMM for the plain
+// C -= A B (row-major here, so rocBLAS's column-major Cᵀ -= Bᵀ Aᵀ), one handle per device made on first use and set
+// to the caller's stream.  Only the factorization's trailing update uses it (never inside a captured graph).
+constexpr int kBlasCfg = kNumTiles;
+static std::mutex g_blas_mutex;
+static rocblas_handle g_blas[64] = {};
+
+static rocblas_handle blas_handle(hipStream_t s)
+{
+    int dev = 0;
+    HIPCHECK2(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) throw std::runtime_error("rocBLAS: device index out of range");
+    std::lock_guard<std::mutex> lock(g_blas_mutex);
+    if (!g_blas[dev] && rocblas_create_handle(&g_blas[dev]) != rocblas_status_success)
+        throw std::runtime_error("rocblas_create_handle failed");
+    if (rocblas_set_stream(g_blas[dev], s) != rocblas_status_success) throw std::runtime_error("rocblas_set_stream failed");
+    return g_blas[dev];
+}
+
+template <typename T>
+void blas_gemm_sub(hipStream_t s, int M, int N, int K, const T *A, long long lda, const T *B, long long ldb, T *C,
+                   long long ldc)
+{
+    rocblas_handle h = blas_handle(s);
+    const T alpha = T(-1), beta = T(1);
+    rocblas_status st;
+    if constexpr (sizeof(T) == 8)
+        st = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, N, M, K, &alpha, B, (int)ldb, A, (int)lda,
+                           &beta, C, (int)ldc);
+    else
+        st = rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, N, M, K, &alpha, B, (int)ldb, A, (int)lda,
+                           &beta, C, (int)ldc);
+    if (st != rocblas_status_success) throw std::runtime_error("rocBLAS GEMM failed");
+}
+
 template <typename T, bool SUB, bool TA>
 void launch_gemm_cfg(int cfg, hipStream_t s, int M, int N, int K, const T *A, long long lda, const T *B, long long ldb,
                      T *C, long long ldc)
 {
     if (M <= 0 || N <= 0 || K <= 0) return;
+    if (cfg == kBlasCfg) {
+        if constexpr (SUB && !TA) return blas_gemm_sub<T>(s, M, N, K, A, lda, B, ldb, C, ldc);
+        cfg = 0;  // the library candidate covers the trailing update's form only
+    }
     if constexpr (sizeof(T) == 8) {
         switch (cfg) {
         case 1: return launch_tile<T, 128, 64, 16, SUB, TA>(s, M, N, K, A, lda, B, ldb, C, ldc);
@@ -209,13 +249,13 @@ void launch_gemm_cfg(int cfg, hipStream_t s, int M, int N, int K, const T *A, lo
     }
 }
 
-// The autotuner: the trailing update's tile shape per precision, chosen by timing every candidate (HIP events, three
-// runs after a warm-up) on the first factorization's own update shape (C -= A B with K = NB2, M = N = the matrix size
-// less one panel, at most 8192) in its factor buffer before the matrix is written there; kept for the process.
-// VP_GEMM_TILE=<index> forces a candidate.
+// The autotuner: the trailing update's GEMM per precision (one of the tile shapes, or rocBLAS), chosen by timing every
+// candidate (HIP events, three runs after a warm-up) on the first factorization's own update shape (C -= A B with
+// K = NB2, M = N = the matrix size less one panel, at most 8192) in its factor buffer before the matrix is written
+// there; kept for the process.  VP_GEMM_TILE=<index> forces a candidate (8: rocBLAS).
 struct GemmTuning {
     int cfg = -1;
-    double ms[kNumTiles] = {};
+    double ms[kNumTiles + 1] = {};
     int size = 0;
 };
 static GemmTuning g_tune[2];  // [0] FP64, [1] FP32
@@ -229,7 +269,7 @@ template <typename T> int tuned_cfg(hipStream_t s, T *scratch, long long ld, int
     if (tu.cfg >= 0) return tu.cfg;
     if (const char *env = std::getenv("VP_GEMM_TILE")) {
         const int c = std::atoi(env);
-        tu.cfg = (c >= 0 && c < kNumTiles) ? c : 0;
+        tu.cfg = (c >= 0 && c <= kBlasCfg) ? c : 0;
         return tu.cfg;
     }
     const int m = std::min(npad - NB2, 8192);
@@ -239,7 +279,7 @@ template <typename T> int tuned_cfg(hipStream_t s, T *scratch, long long ld, int
     HIPCHECK2(hipEventCreate(&e0));
     HIPCHECK2(hipEventCreate(&e1));
     int best = 0;
-    for (int c = 0; c < kNumTiles; c++) {
+    for (int c = 0; c <= kBlasCfg; c++) {
         T *A = scratch + NB2 * ld, *B = scratch + NB2, *C = scratch + NB2 * ld + NB2;
         launch_gemm_cfg<T, true, false>(c, s, m, m, NB2, A, ld, B, ld, C, ld);
         HIPCHECK2(hipEventRecord(e0, s));
@@ -266,12 +306,13 @@ std::string tuning_report()
         const GemmTuning &tu = g_tune[p];
         const TileShape *sh = p == 0 ? kTiles64 : kTiles32;
         auto name = [&](int c) {
+            if (c == kBlasCfg) return std::string("rocblas");
             return std::to_string(sh[c].bm) + "x" + std::to_string(sh[c].bn) + "x" + std::to_string(sh[c].bk);
         };
         out += std::string(p ? ", " : "") + (p == 0 ? "\"fp64\": {" : "\"fp32\": {");
         out += "\"chosen\": " + (tu.cfg < 0 ? std::string("null") : "\"" + name(tu.cfg) + "\"");
         out += ", \"tuned_on\": " + std::to_string(tu.size) + ", \"ms\": {";
-        for (int c = 0; c < kNumTiles; c++) out += std::string(c ? ", " : "") + "\"" + name(c) + "\": " + std::to_string(tu.ms[c]);
+        for (int c = 0; c <= kBlasCfg; c++) out += std::string(c ? ", " : "") + "\"" + name(c) + "\": " + std::to_string(tu.ms[c]);
         out += "}}";
     }
     return out + "}";
