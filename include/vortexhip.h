@@ -151,6 +151,70 @@ int vp_explicit_rhs(vp_explicit *h, int nsheet, const double *sheet_x, const dou
 int vp_explicit_info(const vp_explicit *h, long long *counts);
 void vp_explicit_destroy(vp_explicit *h);
 
+/* ---- the free-wake step (FreeWakeSim / step!, src/freewake/step.jl:35-146) --------------------------------------
+ * A native engine for step!: it keeps the free rows (node rows with their frozen doublets), their conversion into
+ * particles with strip merging (state.jl:292-357), the loads of every step (step_loads, step.jl:149-163), forward-Euler
+ * advection and delete_far! (step.jl:190-195), and does each step's device work through the library's own handles:
+ * explicit_rhs of the known elements (vp_explicit), the solve with the factorization given here (vp_factor_solve) and
+ * the velocity sums as one HIP graph (vp_wake_field).  The caller makes the set-up the reference's constructor makes
+ * (step.jl:35-58): the trailing edge panel, its assembly with the body and the bordered factorization, and the
+ * resolved options. */
+typedef struct vp_freewake vp_freewake;
+typedef struct vp_freewake_setup {
+    const double *vinf;               /* V∞ (3) */
+    double dt;                        /* FreeWakeOptions.dt */
+    int advection;                    /* 0 free, 1 freestream */
+    double delete_beyond;             /* spans downstream of the trailing edge; non-finite: no deletion */
+    double span;                      /* the trailing edge panel's span (RigidWake.span) */
+    int keep_rows;                    /* free rows kept before the oldest is converted (max_rows, resolved) */
+    int core_mode;                    /* CoreRule (state.jl:68-92): 0 fixed core_value, 1 auto, 2 scaled */
+    double core_value, overlap, core_factor, core_floor;
+    const vp_assembly_options *opts;  /* the assembly options of explicit_rhs (kernel_far also for the velocities) */
+    int field_precision;              /* the velocity sums: 0 FP64, 1 FP32 kernels with FP64 sums */
+    vp_factor *factor;                /* the bordered factorization of the body with the trailing edge panel (kept) */
+    const double *b0;                 /* assemble's b with the trailing edge panel (nslot) */
+    int nq;                           /* DOFs (the factorization's size less its border rows) */
+    const int *C_rowptr, *C_col;      /* C (nslot x nq), CSR */
+    const double *C_val;
+    int nte;                          /* wake trailing edge nodes */
+    const int *K_rowptr, *K_col;      /* K (nte x nq), CSR: mu_w = K q */
+    const double *K_val;
+    const double *te_x;               /* the wake trailing edge nodes' positions (nte x 3), for kutta_joukowski_force */
+    int nteseg;                       /* its segments along the lines (te_segments, forces.jl:22-23), pairs */
+    const int *teseg;
+    const double *edge;               /* the trailing edge panel's downstream edge (nte x 3): where rows are shed */
+    int nstrip;                       /* Topology (state.jl:99-121): strips (pairs of trailing edge nodes) */
+    const int *strips;
+    const int *merge;                 /* rows merged per strip (nstrip), or NULL for no merging */
+    int nfirst;                       /* the first and last nodes of open lines (the tips) */
+    const int *first;
+    int nlast;
+    const int *last;
+    const vp_wake *panel;             /* the trailing edge panel as a wake (nodes, triangles, filaments) */
+    const int *panel_te;              /* its nodes' trailing edge node (RigidWake.te) */
+} vp_freewake_setup;
+/* step!'s record (step.jl:138-140): n, t, F_pressure, M_pressure, F_kj, F_impulse (NaN on the first step), Ω, I,
+ * rows, particles, λ (the first border multiplier), umax, t_rhs and t_vel (s); iters: the solve's refinements. */
+typedef struct vp_freewake_record {
+    int n, rows, particles, iters;
+    double t;
+    double F_pressure[3], M_pressure[3], F_kj[3], F_impulse[3], Omega[3], I[3];
+    double lambda, umax, t_rhs, t_vel;
+} vp_freewake_record;
+int vp_freewake_create(vp_ctx *ctx, const vp_surface *g, const vp_freewake_setup *setup, vp_freewake **out);
+/* One step; mu_w (nte, or NULL) receives the step's wake doublets */
+int vp_freewake_step(vp_freewake *h, vp_freewake_record *rec, double *mu_w);
+/* counts: [0] node rows, [1] particles, [2] nte, [3] strips, [4] steps taken, [5] conversions */
+int vp_freewake_counts(const vp_freewake *h, long long *counts);
+/* The state (any pointer may be NULL): node rows (rows x nte x 3, newest first) and their doublets (rows x nte),
+ * mu_line (nte), particles (positions and strengths np x 3, cores np), their merge tags and groups (np), the strips'
+ * merge batches (rows, length, width: nstrip each), the last step's mu (nslot), mu_w (nte) and element Cp (nt), the
+ * impulse I of the last loads (3) and the time t. */
+int vp_freewake_read(const vp_freewake *h, double *rows_x, double *rows_mu, double *mu_line, double *px, double *palpha,
+                     double *pdelta, long long *tag, int *group, long long *batch_rows, double *batch_length,
+                     double *batch_width, double *mu, double *mu_w, double *cp, double *I_prev, double *t);
+void vp_freewake_destroy(vp_freewake *h);
+
 /* ---- bordered solve (src/assembly/solve.jl:15-99) ---------------------------------------------------------- */
 
 /* The DOFMapping pieces of the bordered system: C (nslot x nq) and the wake rows W (nw x nq, `wake_rows`,

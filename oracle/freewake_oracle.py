@@ -26,7 +26,7 @@ def oracle_explicit_rhs(g, dm, we, particles, a, nthreads=0):
     return O.explicit_rhs(g, dm, sheets, segments, blobs, oracle_opts(a), nthreads=nthreads)
 
 
-class OracleFreeWakeSim(FW.FreeWakeSim):
+class OracleFreeWakeSim(FW.FreeWakeHost):
     nthreads = 0
 
     def _setup(self):

@@ -103,6 +103,62 @@ def test_free_wake_steps_match_oracle(wing):
     sim.close()
 
 
+@pytest.mark.parametrize("variant", ["merge_delete_scaled", "auto_freestream", "mixed"])
+def test_native_engine_matches_host_mirror(wing, variant):
+    """The native step (vp_freewake_*: rows, conversions with strip merging, loads, advection and deletion in C++)
+    against the host mirror of step! (FreeWakeHost) over the same device calls, twelve steps: every record field,
+    the rows, the particles with their merge tags and groups, mu and Cp, to 1e-11 of each quantity's scale (the total
+    vorticity and the border multiplier, zero to rounding, to 1e-12 and 1e-15 absolute; only the host sums' order differs); the first step's impulse force is
+    NaN in both."""
+    print(f"free wake native engine vs host mirror {variant}", flush=True)
+    g, dm = wing
+    dt = wk.te_panel_size(g) * 2
+    prec = "mixed" if variant == "mixed" else "fp64"
+    if variant == "merge_delete_scaled":
+        o = FW.FreeWakeOptions(dt=dt, max_rows=2, merge_rows=2, core="scaled", delete_beyond=0.04)
+    elif variant == "auto_freestream":
+        o = FW.FreeWakeOptions(dt=dt, max_rows=3, core="auto", advection="freestream")
+    else:
+        o = FW.FreeWakeOptions(dt=dt, max_rows=2, merge_rows="auto")
+    sim = FW.FreeWakeSim(g, dm, V, o, precision=prec)
+    ref = FW.FreeWakeHost(g, dm, V, o, precision=prec)
+    for k in range(12):
+        a, b = sim.step(), ref.step()
+        assert a["n"] == b["n"] and a["rows"] == b["rows"] and a["particles"] == b["particles"], (k, a, b)
+        assert a["t"] == pytest.approx(b["t"], rel=1e-15)
+        for key in ("mu_w", "F_pressure", "M_pressure", "F_kj", "F_impulse", "Omega", "I"):
+            x, y = np.asarray(a[key]), np.asarray(b[key])
+            if k == 0 and key == "F_impulse":
+                assert np.all(np.isnan(x)) and np.all(np.isnan(y))
+                continue
+            # the total vorticity is zero to rounding (Kelvin): compared in absolute terms
+            tol = 1e-12 if key == "Omega" else 1e-11 * max(np.abs(y).max(), 1e-300)
+            assert np.abs(x - y).max() <= tol, (k, key, x, y)
+        assert abs(a["umax"] - b["umax"]) <= 1e-11 * b["umax"], (k, a["umax"], b["umax"])
+        # the border multiplier is zero but for rounding (3e-8 here): compared in absolute terms
+        assert abs(a["lam"] - b["lam"]) <= 1e-15, (k, a["lam"], b["lam"])
+    sa, sb = sim.state, ref.state
+    assert sa.conversions == sb.conversions > 0 and len(sa.x) == len(sb.x)
+    for r1, r2, m1, m2 in zip(sa.x, sb.x, sa.mu, sb.mu):
+        assert np.abs(r1 - r2).max() <= 1e-11 * np.abs(r2).max()
+        assert np.abs(m1 - m2).max() <= 1e-11 * max(np.abs(m2).max(), 1e-300)
+    assert np.array_equal(sa.tag, sb.tag) and np.array_equal(sa.group, sb.group)
+    assert np.array_equal(sa.batch_rows, sb.batch_rows)
+    for f in ("x", "alpha", "delta"):
+        u, v = getattr(sa.particles, f), getattr(sb.particles, f)
+        assert np.abs(u - v).max() <= 1e-11 * np.abs(v).max(), f
+    assert np.abs(sim.mu - ref.mu).max() <= 1e-11 * np.abs(ref.mu).max()
+    assert np.abs(sim.cp - ref.cp).max() <= 1e-11 * np.abs(ref.cp).max()
+    if variant == "merge_delete_scaled":  # merges made, and particles beyond the limit deleted
+        assert sb.conversions >= 4
+        d = V / np.linalg.norm(V)
+        lim = max(float(x @ d) for x in ref.edge) + o.delete_beyond * ref.te.span
+        print(f"  particles {len(sb.particles)}, furthest {np.max(sb.particles.x @ d) - lim:+.3e} from the limit",
+              flush=True)
+    sim.close()
+    ref.close()
+
+
 def test_free_wake_mixed(wing):
     print("free wake 6 steps mixed vs fp64", flush=True)
     g, dm = wing

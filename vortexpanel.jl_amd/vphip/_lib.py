@@ -31,6 +31,30 @@ class WakeC(C.Structure):
                 ("nfil", C.c_int), ("filaments", C.POINTER(C.c_int))]
 
 
+_PD, _PI = C.POINTER(C.c_double), C.POINTER(C.c_int)
+
+
+class FreeWakeSetupC(C.Structure):
+    """vp_freewake_setup"""
+    _fields_ = [("vinf", _PD), ("dt", C.c_double), ("advection", C.c_int), ("delete_beyond", C.c_double),
+                ("span", C.c_double), ("keep_rows", C.c_int), ("core_mode", C.c_int), ("core_value", C.c_double),
+                ("overlap", C.c_double), ("core_factor", C.c_double), ("core_floor", C.c_double),
+                ("opts", C.POINTER(AssemblyOptionsC)), ("field_precision", C.c_int), ("factor", C.c_void_p),
+                ("b0", _PD), ("nq", C.c_int), ("C_rowptr", _PI), ("C_col", _PI), ("C_val", _PD), ("nte", C.c_int),
+                ("K_rowptr", _PI), ("K_col", _PI), ("K_val", _PD), ("te_x", _PD), ("nteseg", C.c_int),
+                ("teseg", _PI), ("edge", _PD), ("nstrip", C.c_int), ("strips", _PI), ("merge", _PI),
+                ("nfirst", C.c_int), ("first", _PI), ("nlast", C.c_int), ("last", _PI), ("panel", C.POINTER(WakeC)),
+                ("panel_te", _PI)]
+
+
+class FreeWakeRecordC(C.Structure):
+    """vp_freewake_record"""
+    _fields_ = [("n", C.c_int), ("rows", C.c_int), ("particles", C.c_int), ("iters", C.c_int), ("t", C.c_double),
+                ("F_pressure", C.c_double * 3), ("M_pressure", C.c_double * 3), ("F_kj", C.c_double * 3),
+                ("F_impulse", C.c_double * 3), ("Omega", C.c_double * 3), ("I", C.c_double * 3),
+                ("lam", C.c_double), ("umax", C.c_double), ("t_rhs", C.c_double), ("t_vel", C.c_double)]
+
+
 _lib = None
 
 # every symbol include/vortexhip.h declares
@@ -42,7 +66,8 @@ EXPORTS = ["vp_create", "vp_destroy", "vp_last_error", "vp_sync", "vp_assemble",
            "vp_wake_field_create", "vp_wake_field_eval", "vp_wake_field_info", "vp_wake_field_destroy",
            "vp_plan_update_wake", "vp_assembly_tuning", "vp_factor_mode",
            "vp_consistency", "vp_blob_velocities_device", "vp_device_alloc", "vp_device_free", "vp_copy_to_device",
-           "vp_copy_from_device", "vp_explicit_create", "vp_explicit_rhs", "vp_explicit_info", "vp_explicit_destroy"]
+           "vp_copy_from_device", "vp_explicit_create", "vp_explicit_rhs", "vp_explicit_info", "vp_explicit_destroy",
+           "vp_freewake_create", "vp_freewake_step", "vp_freewake_counts", "vp_freewake_read", "vp_freewake_destroy"]
 
 
 def load():
@@ -62,6 +87,7 @@ def load():
         lib.vp_plan_destroy.restype = None
         lib.vp_wake_field_destroy.restype = None
         lib.vp_explicit_destroy.restype = None
+        lib.vp_freewake_destroy.restype = None
         _lib = lib
     return _lib
 

@@ -456,12 +456,13 @@ def check_conversion_distance(max_rows, core, edge, topo, te_len, row):
     return d
 
 
-class FreeWakeSim:
-    """FreeWakeSim (step.jl:8-58): the body at rest in V∞, started impulsively; the trailing edge panel is assembled
-    with the body and the bordered matrix factored once; `step()` changes only the right hand side.
+class FreeWakeHost:
+    """FreeWakeSim (step.jl:8-58) with its step (step!, step.jl:113-146) written out on the host, state.jl's
+    bookkeeping line by line; the device work is in `_explicit_rhs`, `_solve` and `_velocities`.  It is the readable
+    mirror of the reference's step, and the checker's frame: the oracle's simulation overrides those three methods
+    with the FP64 CPU oracle.  FreeWakeSim runs the same step in the native engine (vp_freewake_*).
 
-    precision: 'fp64' or 'mixed' (the factorization in FP32 with FP64 refinement, FP32 velocity kernels with FP64 sums).
-    The device work is in `_explicit_rhs`, `_solve` and `_velocities` (the velocity sums replay one HIP graph)."""
+    precision: 'fp64' or 'mixed' (the factorization in FP32 with FP64 refinement, FP32 velocity kernels with FP64 sums)."""
 
     def __init__(self, g, dm, vinf, opts: FreeWakeOptions, assembly: AssemblyOptions | None = None,
                  precision: str = "fp64", ctx: Context | None = None):
@@ -623,3 +624,144 @@ class FreeWakeSim:
         self.F.close()
         self.explicit.close()
         self.field.close()
+
+
+# ------------------------------------------------------------------------------------------ the native engine -------
+
+class FreeWakeSim(FreeWakeHost):
+    """FreeWakeSim (step.jl:8-58) whose step runs in the native engine (vp_freewake_*): the rows, conversions,
+    merging, loads, advection and deletion in C++, the device work through the library's handles (explicit_rhs, the
+    kept factorization, the velocity sums as one HIP graph).  The set-up (trailing edge panel, its assembly with the
+    body, the bordered factorization, the resolved options) is FreeWakeHost's; `state`, `mu`, `mu_w`, `cp` are read
+    back from the engine."""
+
+    def __init__(self, g, dm, vinf, opts: FreeWakeOptions, assembly: AssemblyOptions | None = None,
+                 precision: str = "fp64", ctx: Context | None = None):
+        self._h = None
+        super().__init__(g, dm, vinf, opts, assembly, precision, ctx)
+        self._make_engine()
+
+    def _setup(self):
+        """the body with the trailing edge panel, assembled and factored once (step.jl:38-46)"""
+        ctx = self.ctx or default_context()
+        self.ctx = ctx
+        o = AssemblyOptions(**{**self.assembly.__dict__, "precision": self.precision})
+        plan = AssemblyPlan(self.g, self.dm, self.te, o, ctx=ctx)
+        plan.assemble(self.sigma, self.V)
+        self.b0 = plan.read_b()
+        self.F = factorize_galerkin(self.dm, plan, W=wake_rows(self.dm, self.te.te), mode=self.precision)
+        plan.close()
+
+    def _make_engine(self):
+        import scipy.sparse as sp
+        from .api import _wake_c
+        dm, topo, core = self.dm, self.topo, self.core
+        i32 = lambda a: L.i32(np.asarray(a, dtype=np.int64).reshape(-1)) if len(a) else np.zeros(1, np.int32)
+        Cm, Km = sp.csr_matrix(dm.C), sp.csr_matrix(dm.K)
+        from .post import te_segments
+        k = dict(V=L.f64(self.V), b0=L.f64(self.b0), Cp=L.i32(Cm.indptr), Ci=L.i32(Cm.indices), Cv=L.f64(Cm.data),
+                 Kp=L.i32(Km.indptr), Ki=L.i32(Km.indices), Kv=L.f64(Km.data), tex=L.f64(dm.wake_te.x),
+                 teseg=i32(te_segments(dm.wake_te)), edge=L.f64(self.edge), strips=i32(topo.segments),
+                 merge=i32(self.merge) if self.merge is not None else None, first=i32(topo.first_nodes),
+                 last=i32(topo.last_nodes), pte=L.i32(self.te.te), opts=self.assembly.to_c())
+        wc, k["wk"] = _wake_c(self.te)
+        k["wc"] = wc
+        mode = {"fixed": 0, "auto": 1, "scaled": 2}[core.mode]
+        su = L.FreeWakeSetupC(
+            L.ptr(k["V"]), self.opts.dt, 1 if self.opts.advection == "freestream" else 0, float(self.opts.delete_beyond),
+            float(self.te.span), int(self.keep_rows), mode, core.value, core.overlap, core.factor, core.floor,
+            C.pointer(k["opts"]), 1 if self.precision == "mixed" else 0, self.F.h, L.ptr(k["b0"]), dm.nq,
+            L.ptr(k["Cp"], C.c_int), L.ptr(k["Ci"], C.c_int), L.ptr(k["Cv"]), topo.nte, L.ptr(k["Kp"], C.c_int),
+            L.ptr(k["Ki"], C.c_int), L.ptr(k["Kv"]), L.ptr(k["tex"]), len(te_segments(dm.wake_te)),
+            L.ptr(k["teseg"], C.c_int), L.ptr(k["edge"]), len(topo.segments), L.ptr(k["strips"], C.c_int),
+            L.ptr(k["merge"], C.c_int) if k["merge"] is not None else None, len(topo.first_nodes),
+            L.ptr(k["first"], C.c_int), len(topo.last_nodes), L.ptr(k["last"], C.c_int), C.pointer(wc),
+            L.ptr(k["pte"], C.c_int))
+        s, keep = _surface_c(self.g, dm)
+        h = C.c_void_p()
+        _check(self.ctx.h, self.ctx.lib.vp_freewake_create(self.ctx.h, C.byref(s), C.byref(su), C.byref(h)))
+        self._h = h
+        self._state_cache = None
+
+    def step(self):
+        """step!, step.jl:113-146, in the engine"""
+        rec = L.FreeWakeRecordC()
+        mw = np.zeros(self.topo.nte)
+        _check(self.ctx.h, self.ctx.lib.vp_freewake_step(self._h, C.byref(rec), L.ptr(mw)))
+        out = dict(n=rec.n, t=rec.t, mu_w=mw, F_pressure=np.array(rec.F_pressure), M_pressure=np.array(rec.M_pressure),
+                   F_kj=np.array(rec.F_kj), F_impulse=np.array(rec.F_impulse), Omega=np.array(rec.Omega),
+                   I=np.array(rec.I), rows=rec.rows, particles=rec.particles, lam=rec.lam, umax=rec.umax,
+                   t_rhs=rec.t_rhs, t_vel=rec.t_vel, iters=rec.iters)
+        self.history.append(out)
+        self.n += 1
+        self.t += self.opts.dt
+        self._state_cache = None
+        return out
+
+    def _read(self):
+        """the engine's state as a FreeWakeState, with mu, mu_w, cp and I_prev"""
+        if self._state_cache is not None:
+            return self._state_cache
+        c = np.zeros(6, dtype=np.int64)
+        _check(self.ctx.h, self.ctx.lib.vp_freewake_counts(self._h, L.ptr(c, C.c_longlong)))
+        nr, npt, nte, ns = (int(v) for v in c[:4])
+        rx, rm, ml = np.zeros((nr, nte, 3)), np.zeros((nr, nte)), np.zeros(nte)
+        px, pa, pd = np.zeros((npt, 3)), np.zeros((npt, 3)), np.zeros(npt)
+        tag, grp = np.zeros(npt, np.int64), np.zeros(npt, np.int32)
+        br, bl, bw = np.zeros(ns, np.int64), np.zeros(ns), np.zeros(ns)
+        mu, mw, cp, Ip, t = np.zeros(self.dm.nslot), np.zeros(nte), np.zeros(len(self.g.tris)), np.zeros(3), C.c_double()
+        _check(self.ctx.h, self.ctx.lib.vp_freewake_read(
+            self._h, L.ptr(rx), L.ptr(rm), L.ptr(ml), L.ptr(px), L.ptr(pa), L.ptr(pd), L.ptr(tag, C.c_longlong),
+            L.ptr(grp, C.c_int), L.ptr(br, C.c_longlong), L.ptr(bl), L.ptr(bw), L.ptr(mu), L.ptr(mw), L.ptr(cp),
+            L.ptr(Ip), C.byref(t)))
+        st = FreeWakeState([r for r in rx], [r for r in rm], ml, Blobs(px, pa, pd), int(c[5]), tag, grp.astype(np.int8),
+                           br, bl, bw)
+        self._state_cache = (st, mu, mw, cp, Ip)
+        return self._state_cache
+
+    # the engine's state, read back on access (assignments before the engine exists set up the host's empty state)
+    @property
+    def state(self):
+        return self._read()[0] if getattr(self, "_h", None) else self._state0
+
+    @state.setter
+    def state(self, v):
+        self._state0 = v
+
+    @property
+    def mu(self):
+        return self._read()[1] if getattr(self, "_h", None) else self._mu0
+
+    @mu.setter
+    def mu(self, v):
+        self._mu0 = v
+
+    @property
+    def mu_w(self):
+        return self._read()[2] if getattr(self, "_h", None) else self._muw0
+
+    @mu_w.setter
+    def mu_w(self, v):
+        self._muw0 = v
+
+    @property
+    def cp(self):
+        return self._read()[3] if getattr(self, "_h", None) else self._cp0
+
+    @cp.setter
+    def cp(self, v):
+        self._cp0 = v
+
+    @property
+    def I_prev(self):
+        return self._read()[4] if getattr(self, "_h", None) else self._I0
+
+    @I_prev.setter
+    def I_prev(self, v):
+        self._I0 = v
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.ctx.lib.vp_freewake_destroy(self._h)
+            self._h = None
+        self.F.close()
