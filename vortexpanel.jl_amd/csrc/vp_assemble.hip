@@ -1776,6 +1776,25 @@ template <typename X> X *dalloc(size_t n)
     return p;
 }
 
+double *Plan::scratch_d(int slot, size_t n)
+{
+    if ((int)sbuf.size() <= slot) {
+        sbuf.resize(slot + 1, nullptr);
+        scap.resize(slot + 1, 0);
+    }
+    if (scap[slot] < n) {
+        if (sbuf[slot]) {
+            HIPCHECK(hipStreamSynchronize(ctx->stream));  // the last call's work on it is done
+            (void)hipFree(sbuf[slot]);
+            sbuf[slot] = nullptr;
+        }
+        n = std::max(n, scap[slot] + scap[slot] / 2);
+        sbuf[slot] = dalloc<double>(n);
+        scap[slot] = n;
+    }
+    return (double *)sbuf[slot];
+}
+
 template <typename X> X *dupload(const std::vector<X> &v, hipStream_t s)
 {
     X *p = dalloc<X>(v.size());
@@ -1900,6 +1919,8 @@ Plan::~Plan()
         for (int j = 0; j < 2; j++)
             if (tev[k][j]) (void)hipEventDestroy(tev[k][j]);
     for (void *p : owned) (void)hipFree(p);
+    for (void *p : sbuf)
+        if (p) (void)hipFree(p);
     if (d_bnear) (void)hipFree(d_bnear);
     if (A && ownsA) (void)hipFree(A);
 }

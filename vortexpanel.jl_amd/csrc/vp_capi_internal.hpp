@@ -100,6 +100,12 @@ struct Plan {
     int2 *d_nonfar = nullptr;
     struct Leaf *d_leaves = nullptr;
 
+    // grow-only device buffers of the per-call entries (wake_product, blob_rhs), kept with the plan: a free-wake
+    // step's calls allocate nothing once their sizes have been seen
+    std::vector<void *> sbuf;
+    std::vector<size_t> scap;
+    double *scratch_d(int slot, size_t n);
+
     ~Plan();
     void build(Ctx *ctx, const vp_surface &g, const vp_wake *wake, const vp_assembly_options &o);
     void assemble(const double *sigma, const double *vinf, bool host_sigma);

@@ -197,42 +197,32 @@ void Plan::wake_product(const double *mu_nodes, double *out)
 {
     hipStream_t st = ctx->stream;
     if (!assembled) throw std::invalid_argument("the plan has not been assembled (vp_plan_assemble first)");
-    double *dmu = dalloc<double>(std::max(nw, 1)), *dout = dalloc<double>(nslot);
+    double *dmu = scratch_d(7, std::max(nw, 1)), *dout = scratch_d(8, nslot);
     if (nw) HIPCHECK(hipMemcpyAsync(dmu, mu_nodes, sizeof(double) * nw, hipMemcpyHostToDevice, st));
     wake_matvec<<<(nslot + 255) / 256, 256, 0, st>>>(A, lda, nslot, nw, dmu, dout);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(out, dout, sizeof(double) * nslot, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    (void)hipFree(dmu);
-    (void)hipFree(dout);
 }
 
 void Plan::blob_rhs(int nb, const double *bx_h, const double *ba_h, const double *bd_h, double *out)
 {
     hipStream_t st = ctx->stream;
     const AsmParams P = params();
-    std::vector<void *> tmp;
-    auto alloc = [&](size_t n) {
-        double *p = dalloc<double>(n);
-        tmp.push_back(p);
-        return p;
-    };
-    struct Free {
-        std::vector<void *> &v;
+    // the plan's grow-only buffers (the call ends with a synchronization, so the next call may reuse them); on an
+    // error the stream is drained before the buffers can be reused
+    struct Drain {
         hipStream_t s;
-        ~Free()
-        {
-            (void)hipStreamSynchronize(s);
-            for (void *p : v) (void)hipFree(p);
-        }
-    } guard_{tmp, st};
-    double *bx = alloc((size_t)3 * std::max(nb, 1)), *ba = alloc((size_t)3 * std::max(nb, 1)), *bd = alloc(std::max(nb, 1));
+        ~Drain() { (void)hipStreamSynchronize(s); }
+    } drain_{st};
+    double *bx = scratch_d(0, (size_t)3 * std::max(nb, 1)), *ba = scratch_d(1, (size_t)3 * std::max(nb, 1)),
+           *bd = scratch_d(2, std::max(nb, 1));
     if (nb) {
         HIPCHECK(hipMemcpyAsync(bx, bx_h, sizeof(double) * 3 * nb, hipMemcpyHostToDevice, st));
         HIPCHECK(hipMemcpyAsync(ba, ba_h, sizeof(double) * 3 * nb, hipMemcpyHostToDevice, st));
         HIPCHECK(hipMemcpyAsync(bd, bd_h, sizeof(double) * nb, hipMemcpyHostToDevice, st));
     }
-    double *rows = alloc((size_t)3 * nt), *b = alloc(nslot);
+    double *rows = scratch_d(3, (size_t)3 * nt), *b = scratch_d(4, nslot);
     HIPCHECK(hipMemsetAsync(rows, 0, sizeof(double) * 3 * nt, st));
     HIPCHECK(hipMemsetAsync(b, 0, sizeof(double) * nslot, st));
     if (nb) {
@@ -240,8 +230,8 @@ void Plan::blob_rhs(int nb, const double *bx_h, const double *ba_h, const double
         const int nchunk = std::max(1, std::min(ntiles, (2048 + tblocks - 1) / tblocks));
         const int chunk = ((ntiles + nchunk - 1) / nchunk) * BLOB_TILE;
         const int nch = (nb + chunk - 1) / chunk;
-        double *partial = alloc((size_t)3 * nt * nch);
-        unsigned int *cnt = (unsigned int *)alloc(1);
+        double *partial = scratch_d(5, (size_t)3 * nt * nch);
+        unsigned int *cnt = (unsigned int *)scratch_d(6, 1);
         for (int attempt = 0;; attempt++) {
             if (attempt == 3) throw std::runtime_error("near blob list did not fit after regrowing");
             if (!d_bnear) {
