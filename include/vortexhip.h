@@ -329,6 +329,10 @@ void vp_wake_field_destroy(vp_wake_field *h);
  * FP64 refinement. */
 int vp_gemm(vp_ctx *ctx, int prec, int flags, int M, int N, int K, const double *A, const double *B, double *C);
 int vp_dense_solve(vp_ctx *ctx, int n, const double *M, int mode, const double *r, double *x, int *iters);
+/* Test entry point: the LU's diagonal-block kernel on one 64 x 64 block (A row-major, factored in its top-left nb x nb
+ * part, the rest the identity; prec 0 FP64, 1 FP32): the packed factors (L unit lower below the diagonal, U on and
+ * above it) and the inverses of L and U, row-major 64 x 64. */
+int vp_diag_lu_probe(vp_ctx *ctx, int prec, int nb, const double *A, double *LU, double *Linv, double *Uinv);
 /* Test entry point: the device's FP64 natural log (fn 0: out = log a), two-argument arctangent (fn 1:
  * out = atan2(a, b)), reciprocal square root (fn 2: out = 1/sqrt(a), a normal and positive) and square root of a
  * squared distance (fn 3: out = sqrt(a), a zero or normal and positive) as the kernels evaluate them, and the mixed

@@ -491,6 +491,14 @@ int vp_math_probe(vp_ctx *ctx, int fn, int n, const double *a, const double *b, 
     return guard(&ctx->c, [&] { vp::math_probe_host(&ctx->c, fn, n, a, b, out); });
 }
 
+int vp_diag_lu_probe(vp_ctx *ctx, int prec, int nb, const double *A, double *LU, double *Linv, double *Uinv)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if (!A || !LU || !Linv || !Uinv || (prec != 0 && prec != 1) || nb < 1 || nb > 64)
+        return fail(&ctx->c, VP_E_ARGUMENT, "bad argument");
+    return guard(&ctx->c, [&] { vp::diag_lu_host(ctx->c.stream, prec, nb, A, LU, Linv, Uinv); });
+}
+
 int vp_dense_solve(vp_ctx *ctx, int n, const double *M, int mode, const double *r, double *x, int *iters)
 {
     if (!ctx) return VP_E_ARGUMENT;

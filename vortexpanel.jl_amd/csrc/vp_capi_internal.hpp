@@ -162,6 +162,7 @@ struct Factor {
 
 std::string tuning_report();
 void gemm_host(hipStream_t s, int prec, int flags, int M, int N, int K, const double *A, const double *B, double *C);
+void diag_lu_host(hipStream_t s, int prec, int nb, const double *A, double *LU, double *Linv, double *Uinv);
 void surface_velocity_host(Ctx *c, const vp_surface &g, const vp_wake *w, const double *sigma, const double *mu,
                            const double *mu_wake, const double *vinf, double kernel_far, int precision, double *u);
 void body_velocity_host(Ctx *c, const vp_surface &g, const double *sigma, const double *gam, int np, const double *xs,

@@ -331,6 +331,8 @@ void launch_gemm(hipStream_t s, int M, int N, int K, const T *A, long long lda, 
 // inverses of L and U (NB x NB, row-major) for the panel's triangular solves.  A zero or non-finite pivot sets *bad.
 // Every stage is right-looking over the whole workgroup: the elimination, then L^-1 and U^-1 from the identity by
 // rank-1 updates (row k of the inverse is final once the updates of the rows before it, or after it for U, are in).
+// (A one-wave version, lane per column with no workgroup barriers, ran 2x slower: its serial LDS loops leave the
+// latency unhidden; C4's FP64 factorization 0.40 -> 0.47 s.)
 template <typename T>
 __global__ __launch_bounds__(256) void diag_lu(T *G, long long ld, int k0, int nb, T *Linv, T *Uinv, int *bad)
 {
@@ -1427,6 +1429,36 @@ void gemm_host(hipStream_t s, int prec, int flags, int M_, int N_, int K_, const
     (void)hipFree(dA);
     (void)hipFree(dB);
     (void)hipFree(dC);
+}
+
+// Test hook: diag_lu on one block (NB x NB row-major A, nb <= NB), its packed factors and both inverses back
+void diag_lu_host(hipStream_t s, int prec, int nb, const double *A, double *LU, double *Linv, double *Uinv)
+{
+    const size_t nn = (size_t)NB * NB;
+    double *dA = dmalloc<double>(nn), *dL = dmalloc<double>(nn), *dU = dmalloc<double>(nn);
+    int *bad = dmalloc<int>(1);
+    HIPCHECK2(hipMemcpyAsync(dA, A, nn * 8, hipMemcpyHostToDevice, s));
+    HIPCHECK2(hipMemsetAsync(bad, 0, sizeof(int), s));
+    if (prec == 0) {
+        diag_lu<double><<<1, 256, 0, s>>>(dA, NB, 0, nb, dL, dU, bad);
+    } else {
+        float *fA = dmalloc<float>(nn), *fL = dmalloc<float>(nn), *fU = dmalloc<float>(nn);
+        convert<double, float><<<blocks(nn), 256, 0, s>>>(dA, fA, nn);
+        diag_lu<float><<<1, 256, 0, s>>>(fA, NB, 0, nb, fL, fU, bad);
+        convert<float, double><<<blocks(nn), 256, 0, s>>>(fA, dA, nn);
+        convert<float, double><<<blocks(nn), 256, 0, s>>>(fL, dL, nn);
+        convert<float, double><<<blocks(nn), 256, 0, s>>>(fU, dU, nn);
+        HIPCHECK2(hipStreamSynchronize(s));
+        (void)hipFree(fA);
+        (void)hipFree(fL);
+        (void)hipFree(fU);
+    }
+    HIPCHECK2(hipGetLastError());
+    HIPCHECK2(hipMemcpyAsync(LU, dA, nn * 8, hipMemcpyDeviceToHost, s));
+    HIPCHECK2(hipMemcpyAsync(Linv, dL, nn * 8, hipMemcpyDeviceToHost, s));
+    HIPCHECK2(hipMemcpyAsync(Uinv, dU, nn * 8, hipMemcpyDeviceToHost, s));
+    HIPCHECK2(hipStreamSynchronize(s));
+    for (void *p : {(void *)dA, (void *)dL, (void *)dU, (void *)bad}) (void)hipFree(p);
 }
 
 int dense_solve_host(Ctx *c, int n, const double *Mh, int mode, const double *r, double *x)
