@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from vphip import api, dofmap, mesh, solve, wake as wk
+from vphip import api, dofmap, geometry, mesh, solve, wake as wk
 from vphip.exact import EllipsoidFlow
 
 pytestmark = pytest.mark.gpu
@@ -61,6 +61,33 @@ def test_wing_with_wake():
     dm = dofmap.build_dofmap(g)
     V = np.array([np.cos(np.radians(5)), 0, np.sin(np.radians(5))])
     w = wk.build_wake(g, dm, V, wk.WakeOptions(length=2.0))
+    _compare(g, dm, V, w=w)
+
+
+def test_pipeline_wing_with_wake():
+    """A swept, tapered wing built by the geometry pipeline from a raw quad mesh (split trailing edge welded, tips
+    capped by the ladder, diagonals by the default rule), with its wake"""
+    spec = mesh.WingSpec(code="2412", n_around=16, n_span=7, aspect=5.0, taper=0.6, sweep_c4_deg=20.0)
+    g = geometry.build_geometry(*mesh.naca_wing_raw(spec, split_te=True))
+    dm = dofmap.build_dofmap(g)
+    V = np.array([np.cos(np.radians(5)), 0, np.sin(np.radians(5))])
+    w = wk.build_wake(g, dm, V, wk.WakeOptions(length=2.0))
+    _compare(g, dm, V, w=w)
+
+
+def test_pipeline_notched_trailing_edge():
+    """A prism whose section has a V notch in its trailing edge, through the geometry pipeline: two Kutta chains (the
+    notch tips), two wake sheets, caps with the notch in them"""
+    sec = [(0.0, 0.0), (0.5, 0.1), (1.0, 0.05), (0.7, 0.0), (1.0, -0.05), (0.5, -0.1)]
+    n, m = 6, len(sec)
+    X = np.array([(px, -1.5 + 3 * j / n, pz) for j in range(n + 1) for px, pz in sec])
+    faces = [[m * j + k, m * j + (k + 1) % m, m * (j + 1) + (k + 1) % m, m * (j + 1) + k]
+             for j in range(n) for k in range(m)]
+    g = geometry.build_geometry(X, faces)
+    assert len(g.chains) == 2
+    dm = dofmap.build_dofmap(g)
+    V = np.array([np.cos(np.radians(3)), 0, np.sin(np.radians(3))])
+    w = wk.build_wake(g, dm, V, wk.WakeOptions(length=1.5))
     _compare(g, dm, V, w=w)
 
 

@@ -32,6 +32,10 @@ class KuttaChain:
     tri_upper: np.ndarray
     tri_lower: np.ndarray
     closed: bool = False
+    s: np.ndarray | None = None        # arclength at the nodes
+    edges: np.ndarray | None = None    # edge ids of the segments (in the geometry pipeline's edge numbering)
+    end_type: tuple | None = None      # "cap", "open", "branch" or "dangling" at the first and last node
+    end_cap_tri: tuple | None = None   # a cap triangle at each end node, -1 if none
 
 
 @dataclass
@@ -45,6 +49,7 @@ class Surface:
     touch_pairs: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int32))
     bodies_closed: list = field(default_factory=lambda: [True])
     tri_cap: np.ndarray | None = None  # (nt,) bool, cap triangles
+    report: object | None = None       # what the geometry pipeline did, when it built the surface
 
     @property
     def nn(self):
@@ -67,8 +72,10 @@ def _normalize(v):
 
 
 def _orient_outward(X, f, outward):
-    """src/meshtools/bodies.jl:4-8"""
+    """src/meshtools/bodies.jl:4-8 (a quad adds the normal of its second half)"""
     n = np.cross(X[f[1]] - X[f[0]], X[f[2]] - X[f[0]])
+    if len(f) == 4:
+        n = n + np.cross(X[f[2]] - X[f[0]], X[f[3]] - X[f[0]])
     return f[::-1] if np.dot(n, outward) < 0 else f
 
 
@@ -253,6 +260,51 @@ def abused_ellipsoid(nth=71, nph=142, axes=(2.0, 1.0, 0.5), jitter=0.25, sliver_
     return s
 
 
+# --------------------------------------------------------------------------------------------------------------------
+# raw meshes (nodes and faces before repair), the input of the geometry pipeline
+
+
+def cube_sphere_raw(n: int, r: float = 1.0):
+    """src/meshtools/bodies.jl:43-74: each cube face an n x n equiangular grid projected onto the sphere, faces built
+    on their own nodes (the cube edges and corners duplicated), outward quads"""
+    X, faces = [], []
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            a, b = [k for k in range(3) if k != axis]
+            base = len(X)
+            xi = np.tan(np.pi / 4 * (2 * np.arange(n + 1) / n - 1))
+            for j in range(n + 1):
+                for i in range(n + 1):
+                    p = np.zeros(3)
+                    p[axis], p[a], p[b] = sgn, xi[i], xi[j]
+                    X.append(r * p / np.linalg.norm(p))
+            nid = lambda i, j: base + j * (n + 1) + i
+            Xa = np.array(X)
+            for j in range(n):
+                for i in range(n):
+                    f = [nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1)]
+                    faces.append(list(_orient_outward(Xa, np.array(f), Xa[f].mean(axis=0))))
+    return np.array(X), faces
+
+
+def uv_sphere_raw(nth: int, nph: int, r: float = 1.0):
+    """src/meshtools/bodies.jl:76-101: latitude-longitude quads with every coincident node repeated (poles nph + 1
+    times, the seam meridian twice), so the pole quads collapse to triangles once welded"""
+    X = []
+    for i in range(nth + 1):
+        for j in range(nph + 1):
+            th, ph = np.pi * i / nth, 2 * np.pi * j / nph
+            X.append(r * np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)]))
+    X = np.array(X)
+    nid = lambda i, j: i * (nph + 1) + j
+    faces = []
+    for i in range(nth):
+        for j in range(nph):
+            f = [nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1)]
+            faces.append(list(_orient_outward(X, np.array(f), X[f].mean(axis=0))))
+    return X, faces
+
+
 def naca4(code: str, xc: np.ndarray):
     """Camber line and half thickness of a NACA 4-digit section, closed trailing edge coefficient -0.1036"""
     m, p, t = int(code[0]) / 100, int(code[1]) / 10, int(code[2:]) / 100
@@ -275,13 +327,8 @@ class WingSpec:
     mean_chord: float = 1.0
 
 
-def naca_wing(spec: WingSpec) -> Surface:
-    """Structured wing, tip caps, one Kutta chain along the trailing edge.
-
-    Section ring: index 0 the trailing edge, 1..nh-1 the lower surface toward the leading edge (index nh), then the
-    upper surface back toward the trailing edge; cosine spacing.  Stations y from -b/2 to b/2 (uniform); chord
-    c(y) = c_r (1 - (1 - taper) |y|/(b/2)), quarter-chord line swept by sweep_c4_deg.
-    """
+def _wing_nodes(spec: WingSpec):
+    """Section ring nodes of every station, (n_span + 1) * n_around rows (see naca_wing for the numbering)"""
     N = spec.n_around
     assert N % 2 == 0 and N >= 8
     nh = N // 2
@@ -303,7 +350,19 @@ def naca_wing(spec: WingSpec) -> Surface:
         xle = abs(y) * tan_s + 0.25 * cr - 0.25 * c
         for k in range(N):
             X.append((xle + c * sx[k], y, c * sz[k]))
-    X = np.array(X, dtype=np.float64)
+    return np.array(X, dtype=np.float64)
+
+
+def naca_wing(spec: WingSpec) -> Surface:
+    """Structured wing, tip caps, one Kutta chain along the trailing edge.
+
+    Section ring: index 0 the trailing edge, 1..nh-1 the lower surface toward the leading edge (index nh), then the
+    upper surface back toward the trailing edge; cosine spacing.  Stations y from -b/2 to b/2 (uniform); chord
+    c(y) = c_r (1 - (1 - taper) |y|/(b/2)), quarter-chord line swept by sweep_c4_deg.
+    """
+    N = spec.n_around
+    nh = N // 2
+    X = _wing_nodes(spec)
     nid = lambda j, k: j * N + (k % N)
     tris, side, cap = [], [], []
     for j in range(spec.n_span):
@@ -372,3 +431,20 @@ def side_of(tri, side, t, te):
         if side[t][k] != NO_TE:
             return side[t][k]
     return NO_TE
+
+
+def naca_wing_raw(spec: WingSpec, split_te: bool = False):
+    """The wing of `naca_wing` as a raw quad mesh with open tips: quad (j, k) joins ring points k and k + 1 of
+    stations j and j + 1, counterclockwise about the outward normal.  With `split_te` the upper surface ends on a
+    second copy of every trailing edge node (as meshes with a split trailing edge come), which welding merges."""
+    N = spec.n_around
+    X = _wing_nodes(spec)
+    ns = spec.n_span + 1
+    if split_te:
+        X = np.concatenate([X, X[::N]])
+        nid = lambda j, k: j * N + k if k < N else ns * N + j
+    else:
+        nid = lambda j, k: j * N + (k % N)
+    faces = [[nid(j, k), nid(j, k + 1), nid(j + 1, k + 1), nid(j + 1, k)]
+             for j in range(spec.n_span) for k in range(N)]
+    return X, faces
