@@ -446,3 +446,141 @@ def explicit_rhs(g, dm, sheets=None, segments=None, blobs=None, o: Opts | None =
     if rc:
         raise ValueError("explicit_rhs failed")
     return b
+
+
+# ------------------------------------------------------------------------------------------------------------------
+# Isoparametric quads and mixed meshes (vp_oracle_quad.c: src/elements/iso_quad.jl, quad_field.jl, quad_rules.jl,
+# src/assembly/mixed.jl)
+
+SQUARE_KINDS = dict(far=0, near=1, self=2, edge0=3, edge1=4, edge2=5, edge3=6, vertex0=7, vertex1=8, vertex2=9,
+                    vertex3=10, vertex_base=11)
+
+
+def quad_field(qx, x, nx, vinf, side=0, polar=1.0, P=None, d=0.0):
+    """quad_field(Q, x, nx, V; side, polar) (src/elements/quad_field.jl:40-46), or polar_field about P at distance d:
+    returns (c[4], s)"""
+    c = np.zeros(4)
+    s = np.zeros(1)
+    rc = lib().vpo_quad_field(_p(_f64(qx)), _p(_f64(x)), _p(_f64(nx)), _p(_f64(vinf)), C.c_int(side),
+                              C.c_double(polar), _p(_f64(P)) if P is not None else None, C.c_double(d), _p(c), _p(s))
+    if rc == -1:
+        raise ValueError("quad_field: x lies on the quad's boundary, where the sheet's field is singular")
+    if rc:
+        raise ValueError("degenerate quadrilateral")
+    return c, float(s[0])
+
+
+def quad_project(qx, p):
+    """project(Q, p), src/elements/iso_quad.jl:108-146: (P, d)"""
+    P = np.zeros(2)
+    d = np.zeros(1)
+    if lib().vpo_quad_project(_p(_f64(qx)), _p(_f64(p)), _p(P), _p(d), None):
+        raise ValueError("degenerate quadrilateral")
+    return P, float(d[0])
+
+
+def quad_info(qx):
+    """IsoQuad's derived data (src/elements/iso_quad.jl:24-37): dict of area, diam, reach, centroid, n"""
+    out = np.zeros(9)
+    if lib().vpo_quad_project(_p(_f64(qx)), None, None, None, _p(out)):
+        raise ValueError("degenerate quadrilateral")
+    return dict(area=out[0], diam=out[1], reach=out[2], centroid=out[3:6].copy(), n=out[6:9].copy())
+
+
+def square_rule(kind, o: Opts | None = None, quad_levels=4):
+    """A rule of SquareTiers (src/elements/quad_rules.jl:73-87, square_tiers of src/assembly/mixed.jl:70-71):
+    points (n, 2) and weights; vertex_base is barycentric (n, 3)"""
+    o = o or opts()
+    k = SQUARE_KINDS[kind] if isinstance(kind, str) else kind
+    n = lib().vpo_square_rule(C.byref(o), quad_levels, k, None, None)
+    st = 3 if k == 11 else 2
+    p, w = np.zeros((n, st)), np.zeros(n)
+    lib().vpo_square_rule(C.byref(o), quad_levels, k, _p(p), _p(w))
+    return p, w
+
+
+QUAD_RULE_KINDS = dict(quad=0, tri=1, wake=2, segment=3, vertex_square=4, contact=5)
+
+
+def quad_rule(tx, kind, sx=None, tn=(0, 1, 2, 3), sn=(4, 5, 6, 7), self_pair=False, touch=False,
+              o: Opts | None = None, quad_levels=4, corner=0):
+    """The tier and rule of test quad tx for a source (quad_pair_rule, quad_wake_rule, quad_segment_rule, or
+    vertex_square / square_contact_rule alone; src/assembly/mixed.jl:118-369): (tier name, points (n, 2), weights);
+    the rule is None for a contact test that finds no contact"""
+    o = o or opts()
+    k = QUAD_RULE_KINDS[kind]
+    tier = C.c_int(0)
+    sxa = _f64(sx) if sx is not None else np.zeros(12)
+    sel = corner if k == 4 else int(self_pair)
+    args = (_p(_f64(tx)), _p(_i32(tn), C.c_int), k, _p(sxa), _p(_i32(sn), C.c_int), sel, int(touch), C.byref(o),
+            quad_levels, C.byref(tier))
+    n = lib().vpo_quad_rule(*args, None, None, 0)
+    if n < 0:
+        raise ValueError("degenerate element")
+    p, w = np.zeros((n, 2)), np.zeros(n)
+    lib().vpo_quad_rule(*args, _p(p), _p(w), n)
+    if k == 5 and n == 0:
+        return TIER_NAMES[tier.value], None, None
+    return TIER_NAMES[tier.value], p, w
+
+
+def quad_block(tx, sx, p, w, vinf, test=0, side=0):
+    """block(T::IsoQuad, S::IsoQuad, rule, V, test, side), src/assembly/mixed.jl:376-388: (4 x 4 block, rb[4])"""
+    p, w = _f64(np.asarray(p).reshape(-1, 2)), _f64(w)
+    blk, rb = np.zeros((4, 4)), np.zeros(4)
+    rc = lib().vpo_quad_block(_p(_f64(tx)), _p(_f64(sx)), len(w), _p(p), _p(w), _p(_f64(vinf)), C.c_int(test),
+                              C.c_int(side), _p(blk), _p(rb))
+    if rc:
+        raise ValueError(f"vpo_quad_block failed ({rc})")
+    return blk, rb
+
+
+def assemble_mixed(x, mesh, nslot, vinf, touch=None, o: Opts | None = None, wake=None, quad_levels=4, nthreads=0,
+                   tiers=False):
+    """assemble(g, dm, mesh::MixedMesh, V∞; opts, wake), src/assembly/mixed.jl:444-505.  mesh: quad_nodes,
+    quad_slots (nq, 4), tri_nodes, tri_slots (nt, 3), quads first; touch: element pairs (k, 2).  Returns A
+    (nslot x (nslot + nw)), b, pairs and points per tier (and the tiers, ne x (ne + wake triangles))."""
+    o = o or opts()
+    nwx, wx, nwt, wt, nwf, wf = _wake_arrays(wake)
+    qn, qs = _i32(mesh.quad_nodes).reshape(-1, 4), _i32(mesh.quad_slots).reshape(-1, 4)
+    tn, ts = _i32(mesh.tri_nodes).reshape(-1, 3), _i32(mesh.tri_slots).reshape(-1, 3)
+    nq, nt = len(qn), len(tn)
+    A = np.zeros((nslot + nwx, nslot))
+    b = np.zeros(nslot)
+    pairs, points = np.zeros(6, np.int64), np.zeros(6, np.int64)
+    tp = _i32(np.asarray(touch if touch is not None else np.zeros((0, 2)), dtype=np.int64).reshape(-1, 2))
+    ntp = len(tp)
+    if ntp == 0:
+        tp = np.zeros((1, 2), np.int32)
+    tr = np.zeros((nq + nt, nq + nt + nwt), np.uint8) if tiers else None
+    x = _f64(np.asarray(x).reshape(-1, 3))
+    rc = lib().vpo_assemble_mixed(len(x), _p(x), nq, _p(qn if nq else np.zeros((1, 4), np.int32), C.c_int),
+                                  _p(qs if nq else np.zeros((1, 4), np.int32), C.c_int), nt,
+                                  _p(tn if nt else np.zeros((1, 3), np.int32), C.c_int),
+                                  _p(ts if nt else np.zeros((1, 3), np.int32), C.c_int), nslot, _p(_f64(vinf)),
+                                  ntp, _p(tp, C.c_int), nwx, _p(wx), nwt, _p(wt, C.c_int), nwf, _p(wf, C.c_int),
+                                  C.byref(o), quad_levels, nthreads, _p(A), _p(b), _p(pairs, C.c_longlong),
+                                  _p(points, C.c_longlong), _p(tr, C.c_ubyte) if tiers else None)
+    if rc == -4:
+        raise ValueError("quad_field: a test point lies on a source quad's boundary")
+    if rc:
+        raise ValueError(f"vpo_assemble_mixed failed ({rc})")
+    out = (A.T, b, dict(zip(TIER_NAMES, pairs.tolist())), dict(zip(TIER_NAMES, points.tolist())))
+    return out + (tr,) if tiers else out
+
+
+def quad_field_many(qx, xs, nxs, vinf, side=0, P=None):
+    """quad_field (or, with P (n, 2), polar_field about P at distance 0) at the points xs with directions nxs:
+    returns c (n, 4), s (n)"""
+    xs = _f64(np.asarray(xs).reshape(-1, 3))
+    nxs = _f64(np.broadcast_to(np.asarray(nxs, dtype=np.float64).reshape(-1, 3), xs.shape))
+    n = len(xs)
+    c, s = np.zeros((n, 4)), np.zeros(n)
+    Pa = _f64(np.asarray(P).reshape(-1, 2)) if P is not None else None
+    rc = lib().vpo_quad_field_many(_p(_f64(qx)), n, _p(xs), _p(nxs), _p(_f64(vinf)), C.c_int(side),
+                                   _p(Pa) if Pa is not None else None, _p(c), _p(s))
+    if rc == -1:
+        raise ValueError("quad_field: x lies on the quad's boundary, where the sheet's field is singular")
+    if rc:
+        raise ValueError("degenerate quadrilateral")
+    return c, s

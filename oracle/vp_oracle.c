@@ -1327,3 +1327,6 @@ int vpo_explicit_rhs(int nn, const double *x, int nt, const int *tris, const int
     tier_rules_free(&R);
     return 0;
 }
+
+/* the mixed-mesh (isoparametric quad) oracle shares this translation unit */
+#include "vp_oracle_quad.c"
