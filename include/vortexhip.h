@@ -30,6 +30,7 @@ typedef struct vp_factor vp_factor;
 
 /* AssemblyOptions (src/assembly/assemble.jl:26-41) plus the precision of the path.
  * test: 0 hat, 1 onering, 2 dual.  precision: 0 plain FP64, 1 mixed (FP32 multipole far field).
+ * quad_touch_levels: the grading levels of the touching rules on isoparametric quads (vp_assemble_mixed only).
  * `threaded` has no counterpart (the device is always parallel); `external` is not crossable (a Julia closure). */
 typedef struct vp_assembly_options {
     int test;
@@ -45,6 +46,7 @@ typedef struct vp_assembly_options {
     double contact_rel;
     double kernel_far;
     int precision;
+    int quad_touch_levels;
 } vp_assembly_options;
 
 /* The SurfaceGeometry and DOFMapping arrays assemble reads (src/geometry/types.jl:174-199,
@@ -85,6 +87,34 @@ int vp_sync(vp_ctx *ctx);
  * tier in the reference's TIER_NAMES order (self, edge, vertex, contact, near, far).  wake may be NULL. */
 int vp_assemble(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const double *sigma, const double *vinf,
                 const vp_assembly_options *opts, double *A, double *b, long long *pairs, long long *points);
+
+/* MixedMesh (src/assembly/mixed.jl:13-58), 0-based: nquad isoparametric quads (corner nodes in their outward order and
+ * their DOF slots, nquad x 4), then ntri flat triangles (nodes and slots, ntri x 3); element touch pairs (ntouch x 2,
+ * element ids with the quads first, either order). */
+typedef struct vp_mixed_mesh {
+    int nn;
+    const double *x;
+    int nquad;
+    const int *quad_nodes;
+    const int *quad_slots;
+    int ntri;
+    const int *tri_nodes;
+    const int *tri_slots;
+    int nslot;
+    int ntouch;
+    const int *touch;
+} vp_mixed_mesh;
+
+/* `assemble(g, dm, mesh::MixedMesh, V; opts, wake)` (src/assembly/mixed.jl:444-505): the quads' inner integrals by
+ * quad_field (tensor Gauss or polar), their test integrals on the parameter square's tiers, the triangles' as in
+ * vp_assemble; sources of strength -V.n (the quads' along their curved surface).  A column-major nslot x
+ * (nslot + nw), b, pairs/points per tier as vp_assemble; tiers (may be NULL) receives the tier of every (element,
+ * body element or wake triangle) pair, (nquad + ntri) x (nquad + ntri + wake triangles) bytes row-major.  FP64 only
+ * (opts->precision is ignored).  VP_E_ARGUMENT for a degenerate element, bad options, or a test point on a source
+ * quad's boundary (the reference's quad_field raises there). */
+int vp_assemble_mixed(vp_ctx *ctx, const vp_mixed_mesh *m, const vp_wake *wake, const double *vinf,
+                      const vp_assembly_options *opts, double *A, double *b, long long *pairs, long long *points,
+                      uint8_t *tiers);
 
 /* The same in steps, with the matrix kept in device memory: a plan holds the geometry, element records, tiles and
  * rule tables; vp_plan_assemble fills the plan's A and b (on the device) for new sigma and Vinf. */

@@ -678,4 +678,23 @@ int vp_assemble(vp_ctx *ctx, const vp_surface *g, const vp_wake *wake, const dou
     return rc;
 }
 
+int vp_assemble_mixed(vp_ctx *ctx, const vp_mixed_mesh *m, const vp_wake *wake, const double *vinf,
+                      const vp_assembly_options *opts, double *A, double *b, long long *pairs, long long *points,
+                      uint8_t *tiers)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if (!m || !opts || !vinf || !A || !b || !m->x) return fail(&ctx->c, VP_E_ARGUMENT, "null argument");
+    if (m->nquad < 0 || m->ntri < 0 || m->ntouch < 0) return fail(&ctx->c, VP_E_DIMENSION, "negative size");
+    if ((m->nquad > 0 && (!m->quad_nodes || !m->quad_slots)) || (m->ntri > 0 && (!m->tri_nodes || !m->tri_slots)) ||
+        (m->ntouch > 0 && !m->touch))
+        return fail(&ctx->c, VP_E_ARGUMENT, "null mesh array");
+    const vp_wake *w = wake && wake->nw > 0 ? wake : nullptr;
+    if (w && (!w->x || (w->ntri > 0 && !w->tris) || (w->nfil > 0 && !w->filaments)))
+        return fail(&ctx->c, VP_E_ARGUMENT, "null wake argument");
+    return guard(&ctx->c, [&] {
+        (void)hipSetDevice(ctx->c.device);
+        vp::mixed_assemble_host(&ctx->c, *m, w, vinf, *opts, A, b, pairs, points, tiers);
+    });
+}
+
 }  // extern "C"

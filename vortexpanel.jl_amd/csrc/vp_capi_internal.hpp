@@ -202,6 +202,10 @@ int dense_solve_host(Ctx *c, int n, const double *M, int mode, const double *r, 
 // consistency of A (on the device, column-major lda) and b (host), out: rows, columns, rhs
 void consistency_host(Ctx *c, const double *A, long long lda, const double *b, const vp_constraints &k, double *out);
 void math_probe_host(Ctx *c, int fn, int n, const double *a, const double *b, double *out);
+// assemble over a mixed mesh of isoparametric quads and flat triangles (vp_mixed.hip)
+void mixed_assemble_host(Ctx *c, const vp_mixed_mesh &m, const vp_wake *w, const double *vinf,
+                         const vp_assembly_options &o, double *A, double *b, long long *pairs, long long *points,
+                         uint8_t *tiers);
 
 }  // namespace vp
 

@@ -51,6 +51,7 @@ struct VPAssemblyOptions      # vp_assembly_options
     contact_rel::Cdouble
     kernel_far::Cdouble
     precision::Cint
+    quad_touch_levels::Cint
 end
 
 struct VPSurface              # vp_surface
@@ -138,7 +139,7 @@ function c_options(o::AssemblyOptions, precision::Symbol)
     precision in (:fp64, :mixed) || throw(ArgumentError("precision must be :fp64 or :mixed, not $precision"))
     return VPAssemblyOptions(test_code(o.test), o.far_ratio, o.far_points, o.near_points, o.near_ratio, o.near_depth,
         o.touch_levels, o.touch_ratio, o.touch_order, o.corner_angle, o.contact_rel, o.kernel_far,
-        precision === :mixed ? 1 : 0)
+        precision === :mixed ? 1 : 0, o.quad_touch_levels)
 end
 
 struct SurfaceArrays

@@ -17,7 +17,16 @@ class AssemblyOptionsC(C.Structure):
     _fields_ = [("test", C.c_int), ("far_ratio", C.c_double), ("far_points", C.c_int), ("near_points", C.c_int),
                 ("near_ratio", C.c_double), ("near_depth", C.c_int), ("touch_levels", C.c_int),
                 ("touch_ratio", C.c_double), ("touch_order", C.c_int), ("corner_angle", C.c_double),
-                ("contact_rel", C.c_double), ("kernel_far", C.c_double), ("precision", C.c_int)]
+                ("contact_rel", C.c_double), ("kernel_far", C.c_double), ("precision", C.c_int),
+                ("quad_touch_levels", C.c_int)]
+
+
+class MixedMeshC(C.Structure):
+    """vp_mixed_mesh"""
+    _fields_ = [("nn", C.c_int), ("x", C.POINTER(C.c_double)), ("nquad", C.c_int), ("quad_nodes", C.POINTER(C.c_int)),
+                ("quad_slots", C.POINTER(C.c_int)), ("ntri", C.c_int), ("tri_nodes", C.POINTER(C.c_int)),
+                ("tri_slots", C.POINTER(C.c_int)), ("nslot", C.c_int), ("ntouch", C.c_int),
+                ("touch", C.POINTER(C.c_int))]
 
 
 class SurfaceC(C.Structure):
@@ -63,7 +72,7 @@ EXPORTS = ["vp_create", "vp_destroy", "vp_last_error", "vp_sync", "vp_assemble",
            "vp_factor_solve", "vp_factor_cond", "vp_factor_info", "vp_factor_destroy", "vp_gemm", "vp_dense_solve",
            "vp_surface_velocity", "vp_body_velocity", "vp_blob_velocities", "vp_plan_adjoint",
            "vp_factor_solve_transposed", "vp_plan_wake_product", "vp_plan_blob_rhs", "vp_gemm_tuning", "vp_math_probe",
-           "vp_wake_field_create", "vp_wake_field_eval", "vp_wake_field_info", "vp_wake_field_destroy",
+           "vp_assemble_mixed", "vp_wake_field_create", "vp_wake_field_eval", "vp_wake_field_info", "vp_wake_field_destroy",
            "vp_plan_update_wake", "vp_assembly_tuning", "vp_factor_mode",
            "vp_consistency", "vp_blob_velocities_device", "vp_device_alloc", "vp_device_free", "vp_copy_to_device",
            "vp_copy_from_device", "vp_explicit_create", "vp_explicit_rhs", "vp_explicit_info", "vp_explicit_destroy",

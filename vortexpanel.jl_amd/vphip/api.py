@@ -42,6 +42,7 @@ class AssemblyOptions:
     corner_angle: float = 75.0
     contact_rel: float = 0.1
     kernel_far: float = 10.0
+    quad_touch_levels: int = 4
     threaded: bool = True
     precision: str = "fp64"
 
@@ -55,7 +56,7 @@ class AssemblyOptions:
         return L.AssemblyOptionsC(TESTS[self.test], self.far_ratio, self.far_points, self.near_points, self.near_ratio,
                                   self.near_depth, self.touch_levels, self.touch_ratio, self.touch_order,
                                   self.corner_angle, self.contact_rel, kf if np.isfinite(kf) else 1e300,
-                                  0 if self.precision == "fp64" else 1)
+                                  0 if self.precision == "fp64" else 1, self.quad_touch_levels)
 
 
 def _check(ctx, rc):
@@ -280,3 +281,48 @@ def assemble(g, dm, els, sigma, vinf, opts: AssemblyOptions | None = None, exter
     stats = dict(pairs=plan.pairs, points=plan.points, time=dt)
     plan.close()
     return A, b, stats
+
+
+def assemble_mixed(g, dm, mesh, vinf, opts: AssemblyOptions | None = None, external=None, wake=None,
+                   ctx: Context | None = None, tiers: bool = False):
+    """`assemble(g, dm, mesh::MixedMesh, Vinf; opts, external, wake)` -> (A, b, stats), src/assembly/mixed.jl:444-505,
+    on the device (vp_assemble_mixed): the quads' inner integrals by quad_field, their test integrals on the
+    parameter-square tiers, the triangles' as `assemble`.  `mesh` is a `vphip.mixed.MixedMesh`; the sources are
+    -Vinf.n on every element (so no sigma argument).  FP64.  With tiers=True, stats["tiers"] holds the tier of every
+    (element, body element or wake triangle) pair."""
+    from .mixed import element_touch_pairs
+
+    if external is not None:
+        raise ValueError("external velocity closures are not supported on the device; use explicit_rhs")
+    o = (opts or AssemblyOptions()).to_c()
+    ctx = ctx or default_context()
+    lib = L.load()
+    keep = dict(x=L.f64(mesh.x), qn=L.i32(mesh.quad_nodes.reshape(-1, 4)), qs=L.i32(mesh.quad_slots.reshape(-1, 4)),
+                tn=L.i32(mesh.tri_nodes.reshape(-1, 3)), ts=L.i32(mesh.tri_slots.reshape(-1, 3)),
+                tp=L.i32(element_touch_pairs(g, mesh).reshape(-1, 2)))
+    for k, w in (("qn", 4), ("qs", 4), ("tn", 3), ("ts", 3), ("tp", 2)):
+        if len(keep[k]) == 0:
+            keep[k] = np.zeros((1, w), np.int32)
+    nq, nt = mesh.nq, mesh.ntri
+    ntp = len(element_touch_pairs(g, mesh))
+    m = L.MixedMeshC(len(mesh.x), L.ptr(keep["x"]), nq, L.ptr(keep["qn"], C.c_int), L.ptr(keep["qs"], C.c_int), nt,
+                     L.ptr(keep["tn"], C.c_int), L.ptr(keep["ts"], C.c_int), dm.nslot, ntp, L.ptr(keep["tp"], C.c_int))
+    wc, wkeep = _wake_c(wake)
+    nw = len(wake.x) if wake is not None else 0
+    nwt = len(wake.tris) if wake is not None else 0
+    A = np.zeros((dm.nslot + nw, dm.nslot))  # column-major nslot x ncol
+    b = np.zeros(dm.nslot)
+    pairs, points = np.zeros(6, np.int64), np.zeros(6, np.int64)
+    tr = np.zeros((nq + nt, nq + nt + nwt), np.uint8) if tiers else None
+    import time
+    t0 = time.perf_counter()
+    rc = lib.vp_assemble_mixed(ctx.h, C.byref(m), C.byref(wc) if wc is not None else None, L.ptr(L.f64(vinf)),
+                               C.byref(o), L.ptr(A), L.ptr(b), L.ptr(pairs, C.c_longlong), L.ptr(points, C.c_longlong),
+                               L.ptr(tr, C.c_uint8) if tiers else None)
+    dt = time.perf_counter() - t0
+    _check(ctx.h, rc)
+    names = ("self", "edge", "vertex", "contact", "near", "far")
+    stats = dict(pairs=dict(zip(names, pairs.tolist())), points=dict(zip(names, points.tolist())), time=dt)
+    if tiers:
+        stats["tiers"] = tr
+    return A.T, b, stats
