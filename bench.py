@@ -243,6 +243,73 @@ def free_wake_bench(nsteps=1000):
     return out, (g, dm, V, FW.FreeWakeOptions(dt=0.02, core=float(0.1 * edges.mean())), snaps)
 
 
+def mixed_wing():
+    """The 64-angle batch's wing (NACA 0012, AR 8, 160 x 60 quads) from its raw quads through the geometry pipeline,
+    its quads kept whole (isoparametric) and its tip caps flat triangles, with the body-fixed rigid wake"""
+    from vphip import dofmap, geometry, mesh, mixed, wake as wk
+    X, F = mesh.naca_wing_raw(mesh.WingSpec(code="0012", n_around=160, n_span=60, aspect=8.0))
+    g = geometry.build_geometry(X, F)
+    dm = dofmap.build_dofmap(g)
+    w = wk.build_wake(g, dm, np.array([1.0, 0.0, 0.0]), wk.WakeOptions(length=2.5))
+    return g, dm, w, mixed.mixed_mesh(g, dm)
+
+
+def mixed_mesh_bench(repeats=3):
+    """Mixed-mesh assembly (src/assembly/mixed.jl:444-598) of mixed_wing() at alpha 4 deg on the device, FP64, the
+    median of `repeats`; beside it the same wing split into triangles through the triangle path"""
+    from vphip import api, sweep
+    g, dm, w, m = mixed_wing()
+    V, _ = sweep.wing_freestream(4.0)
+    ts, st = [], None
+    for _ in range(repeats + 1):
+        t0 = time.perf_counter()
+        A, b, st = api.assemble_mixed(g, dm, m, V, wake=w)
+        ts.append(time.perf_counter() - t0)
+    ts = ts[1:]
+    els = api.build_elements(g)
+    sig = api.source_strengths(els, V)
+    plan = api.AssemblyPlan(g, dm, w, api.AssemblyOptions())
+    for _ in range(2):
+        plan.assemble(sig, V)
+    plan.ctx.sync()
+    t0 = time.perf_counter()
+    plan.assemble(sig, V)
+    plan.ctx.sync()
+    tt = time.perf_counter() - t0
+    tri_pairs, tri_points = sum(plan.pairs.values()), sum(plan.points.values())
+    plan.close()
+    npairs, npts = sum(st["pairs"].values()), sum(st["points"].values())
+    out = dict(quads=m.nq, triangles=m.ntri, nslot=dm.nslot, wake_nodes=len(w.x), assemble_s=float(np.median(ts)),
+               assemble_s_all=ts, pairs=st["pairs"], points=st["points"], pair_integrals_per_s=npairs / np.median(ts),
+               points_per_s=npts / np.median(ts), split_triangles_assemble_s=tt, split_triangle_pairs=tri_pairs,
+               split_triangle_points=tri_points, quads_over_triangles_time=float(np.median(ts)) / tt)
+    log(f"mixed mesh: {out}")
+    return out
+
+
+def cpu_mixed_leg(threads, gpu, target_s=12.0):
+    """The mixed-mesh assembly on the host: the FP64 oracle (C, OpenMP) over a bounded sample of test elements of
+    mixed_wing(), scaled to pair integrals per second"""
+    from oracle import oracle as O
+    from vphip import mixed, sweep
+    g, dm, w, m = mixed_wing()
+    V, _ = sweep.wing_freestream(4.0)
+    touch = mixed.element_touch_pairs(g, m)
+    pilot = max(threads * 4, 64)
+    t0 = time.perf_counter()
+    O.assemble_mixed(g.x, m, dm.nslot, V, touch=touch, wake=w, nthreads=threads, rows=(0, pilot))
+    tp = time.perf_counter() - t0
+    rows = int(min(m.ne, max(pilot, pilot * target_s / max(tp, 1e-3))))
+    t0 = time.perf_counter()
+    _, _, pairs, points = O.assemble_mixed(g.x, m, dm.nslot, V, touch=touch, wake=w, nthreads=threads, rows=(0, rows))
+    dt = time.perf_counter() - t0
+    npairs = sum(pairs.values())
+    return dict(value=npairs / dt, unit="pair-integrals/s", cpu_points_per_s=sum(points.values()) / dt,
+                gpu_pair_integrals_per_s=gpu["pair_integrals_per_s"], cores=threads, kind="port",
+                sample=f"FP64 C oracle mixed assemble over test elements 0..{rows - 1} of {m.ne} ({npairs:.3e} pair "
+                       f"integrals, {dt:.1f} s)")
+
+
 def nbody_bench(g, dm, V, mu):
     """surface_velocity on the ellipsoid solution and blob sums at the free wake's end size (84,000 particles)"""
     from vphip import api, post
@@ -279,7 +346,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)  # the launch autotuner times its candidates on the first two
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the assembly metric (no wing, sweep, N-body)")
-    ap.add_argument("--extras", default="nbody,alpha_sweep,wing_adjoint,free_wake",
+    ap.add_argument("--extras", default="nbody,alpha_sweep,wing_adjoint,free_wake,mixed_mesh",
                     help="which of the extra workloads to run (comma-separated)")
     ap.add_argument("--no-solve", action="store_true", help="no factorizations or solves (with --no-extras: the "
                     "assemblies alone, the program the far kernel's counter passes profile)")
@@ -389,7 +456,7 @@ def main():
         p_.close()
     fw = None
     for name, fn in (("alpha_sweep", alpha_sweep_bench), ("wing_adjoint", wing_adjoint_bench),
-                     ("free_wake", free_wake_bench)):
+                     ("free_wake", free_wake_bench), ("mixed_mesh", mixed_mesh_bench)):
         if name in want:
             extras[name] = fn()
             if name == "free_wake":
@@ -408,6 +475,9 @@ def main():
             log(f"cpu alpha leg {legs['alpha_batch']}")
         if fw is not None:
             legs["free_wake_step"] = cpu_free_wake_leg(fw, threads)
+        if "mixed_mesh" in extras:
+            legs["mixed_mesh"] = cpu_mixed_leg(threads, extras["mixed_mesh"])
+            log(f"cpu mixed leg {legs['mixed_mesh']}")
         legs["adjoint"] = "no CPU leg: the reference computes no derivatives (its only route, finite differences, " \
                           "takes two solves per node coordinate: 174,240 for the swept wing)"
     line = {

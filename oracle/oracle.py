@@ -536,10 +536,11 @@ def quad_block(tx, sx, p, w, vinf, test=0, side=0):
 
 
 def assemble_mixed(x, mesh, nslot, vinf, touch=None, o: Opts | None = None, wake=None, quad_levels=4, nthreads=0,
-                   tiers=False):
+                   tiers=False, rows=None):
     """assemble(g, dm, mesh::MixedMesh, V∞; opts, wake), src/assembly/mixed.jl:444-505.  mesh: quad_nodes,
     quad_slots (nq, 4), tri_nodes, tri_slots (nt, 3), quads first; touch: element pairs (k, 2).  Returns A
-    (nslot x (nslot + nw)), b, pairs and points per tier (and the tiers, ne x (ne + wake triangles))."""
+    (nslot x (nslot + nw)), b, pairs and points per tier (and the tiers, ne x (ne + wake triangles)).  rows = (e0, e1)
+    assembles the rows of test elements e0..e1-1 only."""
     o = o or opts()
     nwx, wx, nwt, wt, nwf, wf = _wake_arrays(wake)
     qn, qs = _i32(mesh.quad_nodes).reshape(-1, 4), _i32(mesh.quad_slots).reshape(-1, 4)
@@ -560,7 +561,8 @@ def assemble_mixed(x, mesh, nslot, vinf, touch=None, o: Opts | None = None, wake
                                   _p(ts if nt else np.zeros((1, 3), np.int32), C.c_int), nslot, _p(_f64(vinf)),
                                   ntp, _p(tp, C.c_int), nwx, _p(wx), nwt, _p(wt, C.c_int), nwf, _p(wf, C.c_int),
                                   C.byref(o), quad_levels, nthreads, _p(A), _p(b), _p(pairs, C.c_longlong),
-                                  _p(points, C.c_longlong), _p(tr, C.c_ubyte) if tiers else None)
+                                  _p(points, C.c_longlong), _p(tr, C.c_ubyte) if tiers else None,
+                                  *(rows if rows is not None else (0, nq + nt)))
     if rc == -4:
         raise ValueError("quad_field: a test point lies on a source quad's boundary")
     if rc:

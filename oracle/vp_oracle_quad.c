@@ -1095,14 +1095,15 @@ int vpo_quad_block(const double *tx, const double *sx, int n, const double *p, c
  * assemble(g, dm, mesh::MixedMesh, V; opts, wake), src/assembly/mixed.jl:444-505, with test_quad! (:508-560) and
  * test_tri! (:563-598).  Elements: nq quads (quad_nodes, quad_slots: nq x 4) first, then nt triangles (tri_nodes,
  * tri_slots: nt x 3).  touch: ntouch element pairs (0-based element ids, quads first), made symmetric here.  A is
- * column-major nslot x (nslot + nwx) as in vpo_assemble; tiers (optional) ne x (ne + nwt) bytes.
+ * column-major nslot x (nslot + nwx) as in vpo_assemble; tiers (optional) ne x (ne + nwt) bytes.  Only the rows of
+ * test elements [e_begin, e_end) are assembled (the bounded sample the CPU baseline times; 0, ne for all).
  * Returns 0, -1 degenerate element, -2 bad index, -3 bad option, -4 a test point on a source quad's boundary.
  */
 int vpo_assemble_mixed(int nn, const double *x, int nq, const int *quad_nodes, const int *quad_slots, int nt,
                        const int *tri_nodes, const int *tri_slots, int nslot, const double *vinf, int ntouch,
                        const int *touch, int nwx, const double *wx, int nwt, const int *wtris, int nwf,
                        const int *wfil, const vpo_opts *o, int quad_levels, int nthreads, double *A, double *bvec,
-                       long long *pairs, long long *points, unsigned char *tiers)
+                       long long *pairs, long long *points, unsigned char *tiers, int e_begin, int e_end)
 {
     if (o->test < 0 || o->test > 2) return -3;
     tier_rules_t R;
@@ -1154,7 +1155,7 @@ int vpo_assemble_mixed(int nn, const double *x, int nq, const int *quad_nodes, c
         rule_t tmp;
         rule_init(&tmp);
 #pragma omp for schedule(dynamic, 1)
-        for (int e = 0; e < ne; e++) {
+        for (int e = e_begin < 0 ? 0 : e_begin; e < (e_end > ne ? ne : e_end); e++) {
             if (fail) continue;
             memset(buf, 0, sizeof(double) * 4 * (size_t)ncol);
             double rows[4] = {0, 0, 0, 0};

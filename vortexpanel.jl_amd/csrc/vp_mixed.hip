@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -901,6 +902,35 @@ __device__ __forceinline__ int qf_pieces(double a, double b, double brk, int bas
     return np;
 }
 
+// The integrand of polar_field along one ray p = P + rho e from the closest point P.  The bilinear surface makes
+// the source point quadratic in rho and the area normal and the tangents linear in it:
+//   x(p) = x(P) + rho D + rho^2 E  (D = e1 a1(P) + e2 a2(P), E = e1 e2 W, W = x1 - x2 + x3 - x4),
+//   a1(p) = a1(P) + rho e2 W,  a2(p) = a2(P) + rho e1 W,  a1 x a2 (p) = a1 x a2 (P) + rho (e1 a1(P) x W + e2 W x a2(P)),
+// so with r = r0 - rho (D + rho E), r0 = x - x(P), the kernel_terms of src/elements/quad_field.jl:17-27 become
+//   t.a2 = |r|^-3 r.(nx x a2(p)), t.a1 = |r|^-3 r.(nx x a1(p)),  s = -V.(a1 x a2)(p) nx.r |r|^-3
+// (the same integrand, about half the arithmetic of evaluating the bilinear map at every point).
+struct PolarRay {
+    V3<double> r0, D, E, Q1, Q2, Wn, nx;
+    double P0, P1, e0, e1, vn0, vn1;
+};
+
+__device__ __forceinline__ void ray_terms(const PolarRay &R, double rho, double *c)
+{
+    const V3<double> r = R.r0 - rho * (R.D + rho * R.E);
+    const double ir = vrsqrt(dot(r, r));
+    const double ir3 = ir * ir * ir;
+    const double rw = dot(r, R.Wn);
+    const double ta2 = ir3 * (dot(r, R.Q2) + (rho * R.e0) * rw);
+    const double ta1 = ir3 * (dot(r, R.Q1) + (rho * R.e1) * rw);
+    const double xi = R.P0 + rho * R.e0, eta = R.P1 + rho * R.e1;
+    const double oe = 1 - eta, ox = 1 - xi;
+    c[0] = -oe * ta2 + ox * ta1;
+    c[1] = oe * ta2 + xi * ta1;
+    c[2] = eta * ta2 - xi * ta1;
+    c[3] = -eta * ta2 - ox * ta1;
+    c[4] = -(R.vn0 + rho * R.vn1) * dot(R.nx, r) * ir3;
+}
+
 // polar_field, src/elements/quad_field.jl:103-154 with ray_pv (:160-171) and ray_near (:174-192); out[0..4] with
 // 1/4pi.  Returns false for a target on the quad's boundary.
 __device__ bool polar_field(const V3<double> *X, V3<double> x, V3<double> nx, V3<double> V, double P0, double P1,
@@ -908,22 +938,27 @@ __device__ bool polar_field(const V3<double> *X, V3<double> x, V3<double> nx, V3
 {
     const bool onq = d <= 1e-12 * diam;
     if (onq && fmin(fmin(P0, P1), fmin(1 - P0, 1 - P1)) <= 1e-12) return false;
-    V3<double> a1s, a2s;
-    {
-        const double oe = 1 - P1, ox = 1 - P0;
-        a1s = oe * (X[1] - X[0]) + P1 * (X[2] - X[3]);
-        a2s = ox * (X[3] - X[0]) + P0 * (X[2] - X[1]);
-    }
+    const V3<double> W = (X[0] - X[1]) + (X[2] - X[3]);
+    const V3<double> a1s = (X[1] - X[0]) + P1 * W, a2s = (X[3] - X[0]) + P0 * W;
     const double g11 = dot(a1s, a1s), g12 = dot(a1s, a2s), g22 = dot(a2s, a2s);
     const double l11 = sqrt(g11);
     const double l12 = g12 / l11, l22 = sqrt(g22 - g12 * g12 / g11);
     const double J = l11 * l22;
-    // sheet densities at P (sheet_densities, iso_quad.jl:69-73): g_b = N_b,xi a2 - N_b,eta a1
-    const double Nx[4] = {-(1 - P1), 1 - P1, P1, -P1}, Ne[4] = {-(1 - P0), -P0, P0, 1 - P0};
-    V3<double> g0[4];
-    for (int b = 0; b < 4; b++) g0[b] = Nx[b] * a2s - Ne[b] * a1s;
     const V3<double> nJs = cross(a1s, a2s);
-    const double sJ0 = -dot(V, nJs);
+    PolarRay ray;
+    ray.P0 = P0;
+    ray.P1 = P1;
+    ray.nx = nx;
+    ray.r0 = x - (X[0] + P0 * (X[1] - X[0]) + P1 * (X[3] - X[0]) + (P0 * P1) * W);
+    ray.Q1 = cross(nx, a1s);
+    ray.Q2 = cross(nx, a2s);
+    ray.Wn = cross(nx, W);
+    ray.vn0 = dot(V, nJs);
+    const double va = dot(V, cross(a1s, W)), vb = dot(V, cross(W, a2s));
+    const double sJ0 = -ray.vn0;
+    // f_b = -nx.(g0_b x A)/|A|^3 with g0_b = N_b,xi a2 - N_b,eta a1 (the sheet densities at P): nx.(g0_b x A) =
+    // N_b,xi nx.(a2 x A) - N_b,eta nx.(a1 x A)
+    const double Nx[4] = {-(1 - P1), 1 - P1, P1, -P1}, Ne[4] = {-(1 - P0), -P0, P0, 1 - P0};
     double acc[5] = {0, 0, 0, 0, 0};
     const double SQ[4][2] = {{0.0, 0.0}, {1.0, 0.0}, {1.0, 1.0}, {0.0, 1.0}};
     for (int k = 0; k < 4; k++) {
@@ -939,28 +974,33 @@ __device__ bool polar_field(const V3<double> *X, V3<double> x, V3<double> nx, V3
         int nv[2];
         const int np = qf_pieces(asinh((A0 * et0 + A1 * et1) / h), asinh((B0 * et0 + B1 * et1) / h), 0.0, 5, 1.5, lo, hi, nv);
         for (int pp = 0; pp < np; pp++) {
-            const double va = lo[pp], vb = hi[pp];
+            const double va_ = lo[pp], vb_ = hi[pp];
             const int n = nv[pp], o = gofs(n);
             for (int i = 0; i < n; i++) {
-                const double v = va + (vb - va) * gx[o + i], wq = gw[o + i];
+                const double v = va_ + (vb_ - va_) * gx[o + i], wq = gw[o + i];
                 const double ch = cosh(v), sh = sinh(v);
                 const double R = h * ch;
                 const double z0 = (nk0 + sh * et0) / ch, z1 = (nk1 + sh * et1) / ch;
                 const double e0 = (z0 - l12 * z1 / l22) / l11, e1 = z1 / l22;
-                const V3<double> At = e0 * a1s + e1 * a2s;
-                const double nA = sqrt(dot(At, At));
-                const double W = wq * (vb - va) / (ch * J);
+                ray.e0 = e0;
+                ray.e1 = e1;
+                ray.D = e0 * a1s + e1 * a2s;
+                ray.E = (e0 * e1) * W;
+                ray.vn1 = e0 * va + e1 * vb;
+                const double nA = sqrt(dot(ray.D, ray.D));
+                const double W_ = wq * (vb_ - va_) / (ch * J);
                 double dc[5] = {0, 0, 0, 0, 0};
                 if (onq) {  // ray_pv with the 1/rho terms added back analytically
                     const double nA3 = nA * nA * nA;
+                    const double u2 = dot(nx, cross(a2s, ray.D)), u1 = dot(nx, cross(a1s, ray.D));
                     double f[5];
-                    for (int b = 0; b < 4; b++) f[b] = -dot(nx, cross(g0[b], At)) / nA3;
-                    f[4] = -sJ0 * dot(nx, At) / nA3;
+                    for (int b = 0; b < 4; b++) f[b] = -(Nx[b] * u2 - Ne[b] * u1) / nA3;
+                    f[4] = -sJ0 * dot(nx, ray.D) / nA3;
                     const int o12 = gofs(12);
                     for (int q = 0; q < 12; q++) {
                         const double rho = R * gx[o12 + q], wr = gw[o12 + q] * R;
                         double kt[5];
-                        kt_terms(X, x, nx, V, P0 + rho * e0, P1 + rho * e1, kt);
+                        ray_terms(ray, rho, kt);
                         for (int b = 0; b < 5; b++) dc[b] += wr * (rho * kt[b] - f[b] / rho);
                     }
                     const double lg = log(R * nA);
@@ -976,7 +1016,7 @@ __device__ bool polar_field(const V3<double> *X, V3<double> x, V3<double> nx, V3
                             const double rho = rho0 * sinh(sv);
                             const double jac = gw[orr + q] * sb * rho0 * cosh(sv) * rho;
                             double kt[5];
-                            kt_terms(X, x, nx, V, P0 + rho * e0, P1 + rho * e1, kt);
+                            ray_terms(ray, rho, kt);
                             for (int b = 0; b < 5; b++) dc[b] += jac * kt[b];
                         }
                     } else {
@@ -985,21 +1025,22 @@ __device__ bool polar_field(const V3<double> *X, V3<double> x, V3<double> nx, V3
                             const double rho = R * gx[o12 + q];
                             const double wr = gw[o12 + q] * R * rho;
                             double kt[5];
-                            kt_terms(X, x, nx, V, P0 + rho * e0, P1 + rho * e1, kt);
+                            ray_terms(ray, rho, kt);
                             for (int b = 0; b < 5; b++) dc[b] += wr * kt[b];
                         }
                     }
                 }
-                for (int b = 0; b < 5; b++) acc[b] += W * dc[b];
+                for (int b = 0; b < 5; b++) acc[b] += W_ * dc[b];
             }
         }
     }
     const double i4 = 1.0 / (4 * kPi);
     for (int b = 0; b < 5; b++) out[b] = i4 * acc[b];
-    if (onq && side != 0) {
+    if (onq && side != 0) {  // half the jumps: (g0_b x nP) for the sheets, sigma nP for the source
         const V3<double> nP = (1.0 / sqrt(dot(nJs, nJs))) * nJs;
         const double hs = side / 2.0;
-        for (int b = 0; b < 4; b++) out[b] += hs * (dot(nx, cross(g0[b], nP)) / J);
+        const double w2 = dot(nx, cross(a2s, nP)), w1 = dot(nx, cross(a1s, nP));
+        for (int b = 0; b < 4; b++) out[b] += hs * ((Nx[b] * w2 - Ne[b] * w1) / J);
         out[4] += side * (-dot(V, nP)) / 2 * dot(nx, nP);
     }
     return true;
@@ -1094,6 +1135,8 @@ __device__ __forceinline__ void mx_tri_weights(int test, const double *l, double
 // of the right-hand side, body sources only).  Test point: a quad at parameters (xi, eta) with rule weight w (times
 // the area element J), or a triangle at barycentric l with weight w (times its area).  Returns false on a boundary
 // target (the pair block's quad_field raises there).
+// KIND 1: the source is a quad; 0: a triangle or filament; -1: either (the far tier)
+template <int KIND>
 __device__ bool mx_point(const MxArgs &P, int e, int s, bool tquad, double xi, double eta, const double *l, double w,
                          double *ent)
 {
@@ -1123,7 +1166,7 @@ __device__ bool mx_point(const MxArgs &P, int e, int s, bool tquad, double xi, d
     const V3<double> V = mk3(P.V[0], P.V[1], P.V[2]);
     double c[4] = {0, 0, 0, 0}, src = 0.0;
     int nb;
-    if (s < P.nq) {  // a quad source
+    if (KIND == 1 || (KIND < 0 && s < P.nq)) {  // a quad source
         const QuadRec &S = P.quad[s];
         V3<double> SX[4];
         for (int k = 0; k < 4; k++) SX[k] = mq_corner(S, k);
@@ -1152,8 +1195,14 @@ __device__ bool mx_point(const MxArgs &P, int e, int s, bool tquad, double xi, d
         c[1] = dot(vb, nx);
         nb = 2;
     }
-    for (int a = 0; a < na; a++) {
-        for (int b = 0; b < nb; b++) ent[4 * a + b] += psi[a] * c[b];
+    (void)na;
+    (void)nb;
+    // all sixteen products with constant indices (the unused rows and columns are zero: psi[3] for a triangle test,
+    // c[b] past the source's corners), so that ent stays in registers
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+#pragma unroll
+        for (int b = 0; b < 4; b++) ent[4 * a + b] += psi[a] * c[b];
         ent[16 + a] -= src * psi[a];
     }
     return true;
@@ -1185,9 +1234,13 @@ __device__ void mx_scatter(const MxArgs &P, int e, int s, const double *ent)
         nb = 2;
         rhs = false;
     }
-    for (int a = 0; a < na; a++) {
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        if (a >= na) break;
         const long long r = rows[a];
-        for (int b = 0; b < nb; b++) unsafeAtomicAdd(&P.A[(long long)cols[b] * P.lda + r], ent[4 * a + b]);
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            if (b < nb) unsafeAtomicAdd(&P.A[(long long)cols[b] * P.lda + r], ent[4 * a + b]);
         if (rhs) unsafeAtomicAdd(&P.b[r], ent[16 + a]);
     }
 }
@@ -1322,7 +1375,7 @@ __device__ __forceinline__ void mixed_far_row(const MxArgs &P, int e, int2 *list
     bool ok = true;
     for (int q = 0; q < P.rn[rule] && ok; q++) {
         const double *l = P.rl + 3 * (o + q);
-        ok = mx_point(P, e, s, tquad, l[0], l[1], l, P.rw[o + q], ent);
+        ok = mx_point<-1>(P, e, s, tquad, l[0], l[1], l, P.rw[o + q], ent);
     }
     if (!ok) {
         atomicExch(P.err, 1);
@@ -1357,7 +1410,7 @@ __global__ __launch_bounds__(64) void mixed_rules(MxArgs P, const int2 *__restri
     em.rn = P.rn;
     const int t = mx_pair_rule(em, P, pr.x, pr.y);
     if (PASS == 0) {
-        tier[i] = t;
+        tier[i] = t < 0 ? t : t | (pr.y < P.nq ? 8 : 0);  // bit 3: a quad source (its points go to mixed_eval<1>)
         npieces[i] = em.npieces;
         npts[i] = em.npts;
         if (t < 0) atomicExch(P.err, 2);
@@ -1365,12 +1418,15 @@ __global__ __launch_bounds__(64) void mixed_rules(MxArgs P, const int2 *__restri
     }
 }
 
-// Thread per rule point of the non-far pairs; a wave whose lanes share a pair sums its entries by shuffles
-__global__ __launch_bounds__(256) void mixed_eval(MxArgs P, const int2 *__restrict__ list, const MxPiece *__restrict__ pieces,
-                                                  long long npieces, long long ntotal)
+// Thread per rule point of the non-far pairs whose sources are quads (KIND 1: the numerical inner integrals) or
+// triangles and filaments (KIND 0: closed forms), points pt_base .. pt_base + ntotal - 1 and their pieces; a wave
+// whose lanes share a pair sums its entries by shuffles
+template <int KIND, int WPS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPS, WPS == 1 ? 8 : WPS))) void mixed_eval(MxArgs P, const int2 *__restrict__ list, const MxPiece *__restrict__ pieces,
+                                                  long long npieces, long long pt_base, long long ntotal)
 {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = g < ntotal;
+    const long long g = pt_base + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = g < pt_base + ntotal;
     int pair = -1, e = 0, s = 0;
     double ent[20];
     for (int i = 0; i < 20; i++) ent[i] = 0.0;
@@ -1394,15 +1450,15 @@ __global__ __launch_bounds__(256) void mixed_eval(MxArgs P, const int2 *__restri
         bool ok;
         if (pc.kind == 0) {  // append_rect!: lo + d .* q, w d1 d2
             const double d0 = pc.c[2] - pc.c[0], d1 = pc.c[3] - pc.c[1];
-            ok = mx_point(P, e, s, true, pc.c[0] + d0 * bl[0], pc.c[1] + d1 * bl[1], nullptr, bw * d0 * d1, ent);
+            ok = mx_point<KIND>(P, e, s, true, pc.c[0] + d0 * bl[0], pc.c[1] + d1 * bl[1], nullptr, bw * d0 * d1, ent);
         } else if (pc.kind == 1) {  // append_triangle!: l1 a + l2 b + l3 c, w frac
             const double xi = (bl[0] * pc.c[0] + bl[1] * pc.c[2]) + bl[2] * pc.c[4];
             const double eta = (bl[0] * pc.c[1] + bl[1] * pc.c[3]) + bl[2] * pc.c[5];
-            ok = mx_point(P, e, s, true, xi, eta, nullptr, bw * pc.frac, ent);
+            ok = mx_point<KIND>(P, e, s, true, xi, eta, nullptr, bw * pc.frac, ent);
         } else {  // append_piece!: sum_i m_i b_i, w frac
             double l[3];
             for (int j = 0; j < 3; j++) l[j] = (bl[0] * pc.c[j] + bl[1] * pc.c[3 + j]) + bl[2] * pc.c[6 + j];
-            ok = mx_point(P, e, s, tquad, 0.0, 0.0, l, bw * pc.frac, ent);
+            ok = mx_point<KIND>(P, e, s, tquad, 0.0, 0.0, l, bw * pc.frac, ent);
         }
         if (!ok) atomicExch(P.err, 1);
     }
@@ -1729,27 +1785,46 @@ void mixed_assemble_host(Ctx *c, const vp_mixed_mesh &m, const vp_wake *w, const
     long long pr[6] = {0, 0, 0, 0, 0, 0}, pt[6] = {0, 0, 0, 0, 0, 0};
     pr[MT_FAR] = (long long)hc[2];
     pt[MT_FAR] = (long long)hc[3];
+    // pieces and points of the pairs with quad sources first (mixed_eval<1>), then the rest (mixed_eval<0>)
     std::vector<long long> poff(np), qoff(np);
-    long long totp = 0, totq = 0;
-    for (int i = 0; i < np; i++) {
-        poff[i] = totp;
-        qoff[i] = totq;
-        totp += h_npc[i];
-        totq += h_npt[i];
-        pr[h_tier[i]]++;
-        pt[h_tier[i]] += h_npt[i];
+    long long totp = 0, totq = 0, heavy_p = 0, heavy_q = 0;
+    for (int pass = 1; pass >= 0; pass--) {
+        for (int i = 0; i < np; i++) {
+            if (((h_tier[i] & 8) != 0) != (pass == 1)) continue;
+            poff[i] = totp;
+            qoff[i] = totq;
+            totp += h_npc[i];
+            totq += h_npt[i];
+            const int t = h_tier[i] & 7;
+            pr[t]++;
+            pt[t] += h_npt[i];
+        }
+        if (pass == 1) {
+            heavy_p = totp;
+            heavy_q = totq;
+        }
     }
     long long *d_poff = D.upload(poff.data(), np, st), *d_qoff = D.upload(qoff.data(), np, st);
     MxPiece *d_pieces = D.alloc<MxPiece>(totp);
     if (np)
         mixed_rules<1><<<(np + 63) / 64, 64, 0, st>>>(P, d_list, np, d_tier, d_npc, d_npt, d_poff, d_qoff, d_pieces);
     MCHECK(hipGetLastError());
-    if (totq > 0) {
-        const long long nblk = (totq + 255) / 256;
-        if (nblk > INT32_MAX) throw std::length_error("too many rule points");
-        mixed_eval<<<(unsigned)nblk, 256, 0, st>>>(P, d_list, d_pieces, totp, totq);
-        MCHECK(hipGetLastError());
+    if ((totq + 255) / 256 > INT32_MAX) throw std::length_error("too many rule points");
+    // the quad-source kernel's occupancy cap in waves per SIMD (VP_MIXED_WAVES, 1 to 4)
+    const char *mw = std::getenv("VP_MIXED_WAVES");
+    const int wps = mw && mw[0] >= '1' && mw[0] <= '4' ? mw[0] - '0' : 2;
+    const unsigned hb = (unsigned)((heavy_q + 255) / 256);
+    if (heavy_q > 0) {
+        if (wps == 1) mixed_eval<1, 1><<<hb, 256, 0, st>>>(P, d_list, d_pieces, heavy_p, 0, heavy_q);
+        else if (wps == 2) mixed_eval<1, 2><<<hb, 256, 0, st>>>(P, d_list, d_pieces, heavy_p, 0, heavy_q);
+        else if (wps == 3) mixed_eval<1, 3><<<hb, 256, 0, st>>>(P, d_list, d_pieces, heavy_p, 0, heavy_q);
+        else mixed_eval<1, 4><<<hb, 256, 0, st>>>(P, d_list, d_pieces, heavy_p, 0, heavy_q);
     }
+    MCHECK(hipGetLastError());
+    if (totq > heavy_q)
+        mixed_eval<0, 1><<<(unsigned)((totq - heavy_q + 255) / 256), 256, 0, st>>>(
+            P, d_list, d_pieces + heavy_p, totp - heavy_p, heavy_q, totq - heavy_q);
+    MCHECK(hipGetLastError());
     mixed_freestream<<<(ne + 255) / 256, 256, 0, st>>>(P);
     MCHECK(hipGetLastError());
     MCHECK(hipMemcpyAsync(&herr, P.err, sizeof(int), hipMemcpyDeviceToHost, st));
