@@ -23,7 +23,7 @@ using LinearAlgebra
 using SparseArrays
 using StaticArrays
 import VortexPanel
-import VortexPanel: AssemblyOptions, DOFMapping, RigidWake, SurfaceGeometry, bordered_rows, ExplicitSources
+import VortexPanel: AssemblyOptions, DOFMapping, RigidWake, SurfaceGeometry, bordered_rows, ExplicitSources, MixedMesh
 
 const libvortexhip = get(ENV, "VORTEXHIP_LIB", normpath(joinpath(@__DIR__, "..", "lib", "libvortexhip.so")))
 
@@ -72,6 +72,20 @@ struct VPWake                 # vp_wake
     tris::Ptr{Cint}
     nfil::Cint
     filaments::Ptr{Cint}
+end
+
+struct VPMixedMesh            # vp_mixed_mesh
+    nn::Cint
+    x::Ptr{Cdouble}
+    nquad::Cint
+    quad_nodes::Ptr{Cint}
+    quad_slots::Ptr{Cint}
+    ntri::Cint
+    tri_nodes::Ptr{Cint}
+    tri_slots::Ptr{Cint}
+    nslot::Cint
+    ntouch::Cint
+    touch::Ptr{Cint}
 end
 
 struct VPConstraints          # vp_constraints
@@ -207,6 +221,52 @@ function assemble(g::SurfaceGeometry, dm::DOFMapping, els, σ::AbstractVector, V
             (Ptr{Cvoid}, Ref{VPSurface}, Ptr{VPWake}, Ptr{Cdouble}, Ptr{Cdouble}, Ref{VPAssemblyOptions},
              Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Clonglong}, Ptr{Clonglong}),
             ctx.ptr, sc, wc, sig, vinf, Ref(o), A, b, pairs, points)
+        check(ctx, rc)
+    end
+    stats = (pairs=NamedTuple{TIER_NAMES}(Tuple(Int.(pairs))), points=NamedTuple{TIER_NAMES}(Tuple(Int.(points))), time=t)
+    return A, b, stats
+end
+
+"""
+    assemble(g, dm, mesh::MixedMesh, V∞; opts=AssemblyOptions(), external=nothing, wake=nothing, ctx)
+
+The reference's mixed-mesh `assemble` (src/assembly/mixed.jl:444-505) on the GPU: the quads of `mesh` kept whole as
+isoparametric quads (inner integrals by quad_field's tensor Gauss and polar rules, test integrals on the parameter
+square's tiers with `opts.quad_touch_levels`), every other face a flat triangle; sources -V∞·n. Returns `(A, b,
+stats)` as the triangle method, FP64. The element touch pairs are formed from `g.touch_pairs` through
+`mesh.element_of` as the reference does (mixed.jl:453-457).
+"""
+function assemble(g::SurfaceGeometry, dm::DOFMapping, mesh::MixedMesh, V∞::SVector{3};
+                  opts::AssemblyOptions=AssemblyOptions(), external=nothing, wake::Union{Nothing,RigidWake}=nothing,
+                  ctx::Context=default_context())
+    external === nothing || throw(ArgumentError("external velocity closures are not supported by the GPU path"))
+    o = c_options(opts, :fp64)
+    x = flat3(g.x, Cdouble)
+    qn = Cint[v - 1 for q in mesh.quad_nodes for v in q]
+    qs = Cint[v - 1 for q in mesh.quad_slots for v in q]
+    tn = Cint[v - 1 for t in mesh.tri_nodes for v in t]
+    ts = Cint[v - 1 for t in mesh.tri_slots for v in t]
+    tp = Cint[]
+    for (a, b) in g.touch_pairs
+        ea, eb = mesh.element_of[a], mesh.element_of[b]
+        ea == eb || append!(tp, (ea - 1, eb - 1))
+    end
+    w = wake === nothing ? nothing : WakeArrays(wake)
+    nw = wake === nothing ? 0 : length(wake.x)
+    A = Matrix{Cdouble}(undef, dm.nslot, dm.nslot + nw)
+    b = Vector{Cdouble}(undef, dm.nslot)
+    vinf = Cdouble[V∞[1], V∞[2], V∞[3]]
+    pairs = zeros(Clonglong, 6)
+    points = zeros(Clonglong, 6)
+    t = @elapsed GC.@preserve x qn qs tn ts tp w A b vinf pairs points begin
+        m = Ref(VPMixedMesh(length(g.x), pointer(x), length(mesh.quads), pointer(qn), pointer(qs), length(mesh.tris),
+                            pointer(tn), pointer(ts), dm.nslot, length(tp) ÷ 2, pointer(tp)))
+        wref = Ref(w === nothing ? VPWake(0, C_NULL, 0, C_NULL, 0, C_NULL) : c_wake(w))
+        wc = w === nothing ? Ptr{VPWake}(C_NULL) : Base.unsafe_convert(Ptr{VPWake}, wref)
+        rc = GC.@preserve wref ccall((:vp_assemble_mixed, libvortexhip), Cint,
+            (Ptr{Cvoid}, Ref{VPMixedMesh}, Ptr{VPWake}, Ptr{Cdouble}, Ref{VPAssemblyOptions}, Ptr{Cdouble},
+             Ptr{Cdouble}, Ptr{Clonglong}, Ptr{Clonglong}, Ptr{UInt8}),
+            ctx.ptr, m, wc, vinf, Ref(o), A, b, pairs, points, C_NULL)
         check(ctx, rc)
     end
     stats = (pairs=NamedTuple{TIER_NAMES}(Tuple(Int.(pairs))), points=NamedTuple{TIER_NAMES}(Tuple(Int.(points))), time=t)
