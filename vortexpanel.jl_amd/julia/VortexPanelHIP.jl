@@ -302,21 +302,28 @@ function factorize_galerkin(dm::DOFMapping, A::AbstractMatrix; W=nothing, precis
     precision in (:fp64, :mixed) || throw(ArgumentError("precision must be :fp64 or :mixed, not $precision"))
     nw = W === nothing ? 0 : size(W, 1)
     size(A) == (dm.nslot, dm.nslot + nw) || throw(DimensionMismatch("A is $(size(A)), expected ($(dm.nslot), $(dm.nslot + nw))"))
+    Ad = Matrix{Cdouble}(A)
+    return GC.@preserve Ad factorize_pointer(dm, pointer(Ad), dm.nslot, false, W, precision, ctx)
+end
+
+# vp_factorize of A at `Aptr` (column-major, leading dimension lda; device memory when on_device, as a plan's A)
+function factorize_pointer(dm::DOFMapping, Aptr::Ptr{Cdouble}, lda::Integer, on_device::Bool, W, precision::Symbol,
+                           ctx::Context)
+    nw = W === nothing ? 0 : size(W, 1)
     rows = bordered_rows(dm)
     nb = length(rows)
     Cr, Cc, Cv = csr(dm.C)
     Wr, Wc, Wv = W === nothing ? (Cint[0], Cint[], Cdouble[]) : csr(sparse(W))
     e = Cdouble[r.e[i] for r in rows for i in 1:dm.nq]
     c = Cdouble[r.c[i] for r in rows for i in 1:dm.nq]
-    Ad = Matrix{Cdouble}(A)
     out = Ref{Ptr{Cvoid}}(C_NULL)
-    GC.@preserve Cr Cc Cv Wr Wc Wv e c Ad begin
+    GC.@preserve Cr Cc Cv Wr Wc Wv e c begin
         k = VPConstraints(dm.nslot, dm.nq, nw, nb, pointer(Cr), pointer(Cc), pointer(Cv),
             nw == 0 ? C_NULL : pointer(Wr), nw == 0 ? C_NULL : pointer(Wc), nw == 0 ? C_NULL : pointer(Wv),
             pointer(e), pointer(c))
         rc = ccall((:vp_factorize, libvortexhip), Cint,
             (Ptr{Cvoid}, Ptr{Cdouble}, Clonglong, Cint, Ref{VPConstraints}, Cint, Ref{Ptr{Cvoid}}),
-            ctx.ptr, Ad, dm.nslot, 0, Ref(k), precision === :mixed ? 1 : 0, out)
+            ctx.ptr, Aptr, lda, on_device ? 1 : 0, Ref(k), precision === :mixed ? 1 : 0, out)
         check(ctx, rc)
     end
     n = Ref{Cint}(0)
@@ -633,5 +640,135 @@ function solve_transposed(F::GalerkinFactorizationHIP, r::AbstractVector)
         (Ptr{Cvoid}, Ptr{Cdouble}, Cint, Ptr{Cdouble}, Ref{Cint}, Ref{Cdouble}), F.ptr, R, 1, X, iters, resid))
     return X
 end
+
+# ---- galerkin_doublets and its reverse rule ------------------------------------------------------------------------
+
+"""
+    DoubletState(g, dm, wake, V∞, dV∞; opts, precision, ctx)
+
+The forward state of `galerkin_doublets`: a device plan of `g` and `wake` assembled at V∞ with σ = -V∞·n
+(`vp_plan_create`, `vp_plan_assemble`), the bordered system factored on the device from the plan's own A
+(`vp_plan_device_A`, `vp_factorize` with `A_on_device`), and the solve: the unknowns q, the slot doublets μ = C q
+and the wake nodes' doublets W q. The pullback keeps the plan (its rules and A) and the factors.
+"""
+mutable struct DoubletState
+    ctx::Context
+    plan::Ptr{Cvoid}
+    F::Any
+    dm::DOFMapping
+    wake::RigidWake
+    nn::Int
+    V::Vector{Cdouble}
+    dV::Vector{Cdouble}
+    q::Vector{Float64}
+    μ::Vector{Float64}
+    μ_wake::Vector{Float64}
+    tris::Vector{SVector{3,Int}}
+    x::Vector{SVector{3,Float64}}
+end
+
+function DoubletState(g::SurfaceGeometry, dm::DOFMapping, wake::RigidWake, V∞::SVector{3}, dV∞::SVector{3};
+                      opts::AssemblyOptions=AssemblyOptions(), precision::Symbol=:fp64, ctx::Context=default_context())
+    o = c_options(opts, precision)
+    s = SurfaceArrays(g, dm)
+    w = WakeArrays(wake)
+    plan = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve s w begin
+        check(ctx, ccall((:vp_plan_create, libvortexhip), Cint,
+            (Ptr{Cvoid}, Ref{VPSurface}, Ref{VPWake}, Ref{VPAssemblyOptions}, Ref{Ptr{Cvoid}}),
+            ctx.ptr, Ref(c_surface(s)), Ref(c_wake(w)), Ref(o), plan))
+    end
+    σ = Cdouble[-dot(V∞, normalize(cross(g.x[t[2]] - g.x[t[1]], g.x[t[3]] - g.x[t[1]]))) for t in g.tris]
+    V = Cdouble[V∞[1], V∞[2], V∞[3]]
+    pairs, points = zeros(Clonglong, 6), zeros(Clonglong, 6)
+    check(ctx, ccall((:vp_plan_assemble, libvortexhip), Cint,
+        (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Clonglong}, Ptr{Clonglong}), plan[], σ, V, pairs, points))
+    b = zeros(Cdouble, dm.nslot)
+    check(ctx, ccall((:vp_plan_read, libvortexhip), Cint, (Ptr{Cvoid}, Ptr{Cdouble}, Ptr{Cdouble}), plan[], C_NULL, b))
+    Ad, bd, lda = Ref{Ptr{Cdouble}}(C_NULL), Ref{Ptr{Cdouble}}(C_NULL), Ref{Clonglong}(0)
+    check(ctx, ccall((:vp_plan_device_A, libvortexhip), Cint,
+        (Ptr{Cvoid}, Ref{Ptr{Cdouble}}, Ref{Ptr{Cdouble}}, Ref{Clonglong}), plan[], Ad, bd, lda))
+    W = VortexPanel.wake_rows(dm, wake.te)
+    F = factorize_pointer(dm, Ad[], lda[], true, W, precision, ctx)
+    sol = solve_galerkin(F, b)
+    st = DoubletState(ctx, plan[], F, dm, wake, length(g.x), V, Cdouble[dV∞[1], dV∞[2], dV∞[3]], sol.q, sol.μ,
+                      Vector(W * sol.q), copy(g.tris), copy(g.x))
+    finalizer(x -> x.plan == C_NULL || ccall((:vp_plan_destroy, libvortexhip), Cvoid, (Ptr{Cvoid},), x.plan), st)
+    return st
+end
+
+"""
+    galerkin_doublets(g, dm, wake, V∞; opts, precision, ctx)
+
+The slot doublets μ of `assemble` + `solve_galerkin` for geometry g with its rigid wake at freestream V∞ (σ = -V∞·n):
+the primal of the reverse rule below, which differentiates them with respect to the node coordinates `g.x` and to
+V∞ along a direction `dV∞` (for d/dα, dV∞ = dV∞/dα).
+"""
+galerkin_doublets(g::SurfaceGeometry, dm::DOFMapping, wake::RigidWake, V∞::SVector{3}; kw...) =
+    DoubletState(g, dm, wake, V∞, zero(V∞); kw...).μ
+
+"""
+    doublets_pullback(st, μ̄) -> (x̄, V̄)
+
+The vector-Jacobian product of μ(x, V∞) with a cotangent μ̄ (nslot): x̄ (a vector of SVector{3} over the nodes) and
+V̄, the derivative along `st.dV` (per unit of the direction's parameter). The adjoint system Mᵀψ = [Cᵀμ̄; 0] is solved
+with the forward factors (`vp_factor_solve_transposed`); λ = C ψ_q weights the backward assembly (`vp_plan_adjoint`:
+d/dx [λᵀ(b - A[μ; μ_wake])] with the plan's rules, and λᵀ db along dV∞); the wake nodes' gradients go to their
+trailing edge nodes (the rigid wake moves with them); the zero-mean border rows add -ψ_b ∂(c̃_k·q)/∂x through the
+slot areas. The same steps as `vphip.adjoint.WingAdjoint.pullback_mu`.
+"""
+function doublets_pullback(st::DoubletState, μ̄::AbstractVector)
+    dm, F = st.dm, st.F
+    n = F.nq + F.nb
+    R = zeros(Cdouble, n)
+    R[1:F.nq] = dm.C' * Vector{Float64}(μ̄)
+    ψ = solve_transposed(F, R)
+    λ = Vector{Cdouble}(dm.C * ψ[1:F.nq])
+    μfull = vcat(Vector{Cdouble}(st.μ), Vector{Cdouble}(st.μ_wake))
+    nw = length(st.wake.x)
+    G = zeros(Cdouble, 3 * (st.nn + nw))
+    λb = zeros(Cdouble, 1)
+    check(st.ctx, ccall((:vp_plan_adjoint, libvortexhip), Cint,
+        (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}),
+        st.plan, 1, λ, μfull, st.V, st.dV, G, λb))
+    x̄ = [SVector(G[3i-2], G[3i-1], G[3i]) for i in 1:st.nn]
+    for k in 1:nw   # a wake node follows its trailing edge node
+        j = dm.wake_te.node[st.wake.te[k]]
+        x̄[j] += SVector(G[3(st.nn+k)-2], G[3(st.nn+k)-1], G[3(st.nn+k)])
+    end
+    # the border rows c̃_k = c_k scale/|c_k| (solve.jl:30-35), c_k = Cᵀ(slot_area on body k): only d(c_k·q)/dx at
+    # fixed q is left (c̃_k·q = 0), through the slot areas (a third of each corner's triangle), d area/d a = n × (c - b)/2
+    ψb = ψ[F.nq+1:n]
+    for (k, r) in enumerate(bordered_rows(dm))
+        on = dm.slot_body .== r.body
+        scale = F.se[k] * sqrt(length(dm.body_q[r.body]))
+        coef = -ψb[k] * scale / norm(r.c)
+        for (t, tri) in enumerate(st.tris)
+            a, b, c = st.x[tri[1]], st.x[tri[2]], st.x[tri[3]]
+            nrm = normalize(cross(b - a, c - a))
+            cs = dm.corner_slot[t]
+            mt = sum(on[cs[j]] ? st.μ[cs[j]] : 0.0 for j in 1:3) / 3
+            f = coef * mt / 2
+            x̄[tri[1]] += f * cross(nrm, c - b)
+            x̄[tri[2]] += f * cross(nrm, a - c)
+            x̄[tri[3]] += f * cross(nrm, b - a)
+        end
+    end
+    return x̄, λb[1]
+end
+
+# The reverse rule for ChainRulesCore, defined when the caller has loaded it:
+#
+#     using ChainRulesCore
+#     function ChainRulesCore.rrule(::typeof(VortexPanelHIP.galerkin_doublets), g, dm, wake, V∞; dV∞, kw...)
+#         st = VortexPanelHIP.DoubletState(g, dm, wake, V∞, dV∞; kw...)
+#         function pullback(μ̄)
+#             x̄, v̄ = VortexPanelHIP.doublets_pullback(st, unthunk(μ̄))
+#             return NoTangent(), Tangent{typeof(g)}(; x=x̄), NoTangent(), NoTangent(), v̄ * dV∞
+#         end
+#         return st.μ, pullback
+#     end
+#
+# (V̄ = v̄ dV∞ is the cotangent's projection on the direction dV∞: with dV∞ = dV∞/dα the derivative in α is v̄.)
 
 end # module
