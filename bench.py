@@ -243,6 +243,43 @@ def free_wake_bench(nsteps=1000):
     return out, (g, dm, V, FW.FreeWakeOptions(dt=0.02, core=float(0.1 * edges.mean())), snaps)
 
 
+def uneven_bodies_bench(steps=5):
+    """Bodies of uneven size in one geometry, one assembly: the abused ellipsoid (19,880 triangles), a sphere of
+    5,120 triangles and one of 80, apart; FP64 pair integrals per second of the combined assembly against the
+    ellipsoid's own, with the far tier as a grid of tiles and as one persistent work queue (VP_FAR_QUEUE)"""
+    from vphip import api, dofmap, mesh
+    g0, _, V = workload()
+    parts = [g0, mesh.icosphere(4, r=0.8, center=(4.0, 0.5, 0.0)), mesh.icosphere(1, r=0.2, center=(-3.0, 0.0, 0.5))]
+    out = dict(bodies=[p_.nt for p_ in parts])
+    for name, geo in (("ellipsoid", g0), ("three_bodies", mesh.combine(parts))):
+        dm = dofmap.build_dofmap(geo)
+        els = api.build_elements(geo)
+        sig = api.source_strengths(els, V)
+        for form in ("grid", "queue"):
+            os.environ["VP_FAR_QUEUE"] = "1" if form == "queue" else "0"
+            plan = api.AssemblyPlan(geo, dm, None, api.AssemblyOptions(), timing=True)
+            for _ in range(2):
+                plan.assemble(sig, V)
+            plan.ctx.sync()
+            t0 = time.perf_counter()
+            far = []
+            for _ in range(steps):
+                plan.assemble(sig, V)
+                far.append(plan.timings()["far"])
+            plan.ctx.sync()
+            dt = (time.perf_counter() - t0) / steps
+            npairs = sum(plan.pairs.values())
+            out[f"{name}_{form}"] = dict(nt=geo.nt, pairs=npairs, assemble_ms=1e3 * dt, pair_integrals_per_s=npairs / dt,
+                                         far_ms=float(np.median(far)), far_pairs_per_ms=plan.pairs["far"] / float(np.median(far)))
+            plan.close()
+    os.environ.pop("VP_FAR_QUEUE", None)
+    for form in ("grid", "queue"):
+        out[f"rate_three_bodies_over_ellipsoid_{form}"] = (out[f"three_bodies_{form}"]["pair_integrals_per_s"] /
+                                                           out[f"ellipsoid_{form}"]["pair_integrals_per_s"])
+    log(f"uneven bodies: {out}")
+    return out
+
+
 def mixed_wing():
     """The 64-angle batch's wing (NACA 0012, AR 8, 160 x 60 quads) from its raw quads through the geometry pipeline,
     its quads kept whole (isoparametric) and its tip caps flat triangles, with the body-fixed rigid wake"""
@@ -346,7 +383,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)  # the launch autotuner times its candidates on the first two
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the assembly metric (no wing, sweep, N-body)")
-    ap.add_argument("--extras", default="nbody,alpha_sweep,wing_adjoint,free_wake,mixed_mesh",
+    ap.add_argument("--extras", default="nbody,alpha_sweep,wing_adjoint,free_wake,mixed_mesh,uneven_bodies",
                     help="which of the extra workloads to run (comma-separated)")
     ap.add_argument("--no-solve", action="store_true", help="no factorizations or solves (with --no-extras: the "
                     "assemblies alone, the program the far kernel's counter passes profile)")
@@ -456,7 +493,8 @@ def main():
         p_.close()
     fw = None
     for name, fn in (("alpha_sweep", alpha_sweep_bench), ("wing_adjoint", wing_adjoint_bench),
-                     ("free_wake", free_wake_bench), ("mixed_mesh", mixed_mesh_bench)):
+                     ("free_wake", free_wake_bench), ("mixed_mesh", mixed_mesh_bench),
+                     ("uneven_bodies", uneven_bodies_bench)):
         if name in want:
             extras[name] = fn()
             if name == "free_wake":
@@ -503,7 +541,7 @@ def main():
                      "frac": achieved / PEAK_FP64, "traffic": traffic,
                      "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE passes, copy-calibrated, "
                                        "profiles/far_pairs_abused_ellipsoid_pmc.md" if traffic else None,
-                     "kernel": "far_pairs<double>", "kernel_ms": far_avg * 1e3,
+                     "kernel": "far_pairs_queue<double>", "kernel_ms": far_avg * 1e3,
                      "algorithmic_flops": flops, "far_exact_points": ex, "far_points": 3 * far_pairs_n},
         "cpu_baseline": cpu,
         "cpu_legs": legs,

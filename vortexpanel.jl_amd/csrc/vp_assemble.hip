@@ -310,19 +310,23 @@ __device__ __forceinline__ void far_flush(const AsmParams &P, const double *R, c
 // flush per tile.  The closed form is rare here but carries most of the kernel's registers; capping the kernel at two
 // waves per SIMD (a few registers spill in that path) beats the one wave it would otherwise get.  enum_only: the near
 // list alone, its second pass after it grew.
+#define FAR_ARGS                                                                                                    \
+    AsmParams P, const TestFar *__restrict__ tf, const Elem *__restrict__ el, const double *__restrict__ sigma,        \
+        const int *__restrict__ tt_elems, const uint8_t *__restrict__ tt_loc, const int *__restrict__ tt_rows,         \
+        const int *__restrict__ tt_nrow, const int *__restrict__ ts_elems, const uint8_t *__restrict__ ts_loc,         \
+        const int *__restrict__ ts_cols, const int *__restrict__ ts_ncol, double *__restrict__ A,                      \
+        double *__restrict__ bvec, int2 *__restrict__ near_list, unsigned int *__restrict__ near_count,                \
+        unsigned int near_cap, unsigned long long *__restrict__ far_count, uint8_t *__restrict__ tiers, int st0,      \
+        bool enum_only
+#define FAR_PASS                                                                                                    \
+    P, tf, el, sigma, tt_elems, tt_loc, tt_rows, tt_nrow, ts_elems, ts_loc, ts_cols, ts_ncol, A, bvec, near_list,      \
+        near_count, near_cap, far_count, tiers, st0, enum_only
+
+// One tile (test tile tt, source tile st) of the far tier, by the whole workgroup, with its LDS sums R and rbacc
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void far_pairs(
-    AsmParams P, const TestFar *__restrict__ tf, const Elem *__restrict__ el, const double *__restrict__ sigma,
-    const int *__restrict__ tt_elems, const uint8_t *__restrict__ tt_loc, const int *__restrict__ tt_rows,
-    const int *__restrict__ tt_nrow, const int *__restrict__ ts_elems, const uint8_t *__restrict__ ts_loc,
-    const int *__restrict__ ts_cols, const int *__restrict__ ts_ncol, double *__restrict__ A, double *__restrict__ bvec,
-    int2 *__restrict__ near_list, unsigned int *__restrict__ near_count, unsigned int near_cap,
-    unsigned long long *__restrict__ far_count, uint8_t *__restrict__ tiers, int st0, bool enum_only)
+__device__ __forceinline__ void far_tile(FAR_ARGS, int tt, int st, double *R, double *rbacc, unsigned int &nfar,
+                                         unsigned int &nexw)
 {
-    __shared__ double R[UMAX * RST];
-    __shared__ double rbacc[VMAX];
-    __shared__ unsigned int nfar, nexw;
-    const int tt = blockIdx.x, st = blockIdx.y + st0;
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
     if (!enum_only) {
         for (int i = threadIdx.x; i < UMAX * RST; i += blockDim.x) R[i] = 0.0;
@@ -374,6 +378,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     far_flush(P, R, rbacc, tt, st, tt_rows, tt_nrow, ts_cols, ts_ncol, A, bvec);
     if (threadIdx.x == 0 && nfar) atomicAdd(far_count, (unsigned long long)nfar);
     if (threadIdx.x == 0 && nexw) atomicAdd(far_count + 7, (unsigned long long)nexw);  // stats[12]: exact far points
+}
+
+// The far tier as a grid of tiles: test tile blockIdx.x, source tile blockIdx.y + st0
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void far_pairs(FAR_ARGS)
+{
+    __shared__ double R[UMAX * RST];
+    __shared__ double rbacc[VMAX];
+    __shared__ unsigned int nfar, nexw;
+    far_tile<T>(FAR_PASS, blockIdx.x, blockIdx.y + st0, R, rbacc, nfar, nexw);
+}
+
+// The far tier as one persistent work queue: a grid sized to the chip's resident workgroups takes tiles in order
+// (test tile fastest, as the grid form) from one counter until the ntt x nst tiles are done; every workgroup leaves
+// when the counter passes the last tile.  Tiles of bodies of any size share the queue.
+template <typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void far_pairs_queue(
+    FAR_ARGS, int ntt, int nst, unsigned int *__restrict__ queue)
+{
+    __shared__ double R[UMAX * RST];
+    __shared__ double rbacc[VMAX];
+    __shared__ unsigned int nfar, nexw, tile;
+    const unsigned int ntiles = (unsigned int)ntt * (unsigned int)nst;
+    for (;;) {
+        if (threadIdx.x == 0) tile = atomicAdd(queue, 1u);
+        __syncthreads();
+        const unsigned int k = tile;
+        if (k >= ntiles) break;
+        far_tile<T>(FAR_PASS, (int)(k % (unsigned int)ntt), (int)(k / (unsigned int)ntt) + st0, R, rbacc, nfar, nexw);
+        __syncthreads();  // the flush has read R and the counters before the next tile clears them
+    }
 }
 
 // ------------------------------------------------------------------------------------------ filaments --------------
@@ -1930,6 +1965,15 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
     ctx = ctx_;
     hipStream_t st = ctx->stream;
     opts = o;
+    {  // the far tier's form: one persistent work queue over every body's tiles (two workgroups per CU, the
+       // residency at its two-wave cap; C2 far tier 14.56 -> 14.27 ms, three uneven bodies 22.1 -> 21.3 ms against
+       // the grid), or with VP_FAR_QUEUE=0 a grid of tiles
+        const char *q = std::getenv("VP_FAR_QUEUE");
+        int ncu = 0;
+        if (!(q && q[0] == '0') &&
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess)
+            far_queue_wgs = 2LL * std::max(ncu, 1);
+    }
     if (o.test < 0 || o.test > 2) throw std::invalid_argument("test must be hat (0), onering (1) or dual (2)");
     if (o.precision != 0 && o.precision != 1) throw std::invalid_argument("precision must be 0 (fp64) or 1 (mixed)");
     if (g.nt <= 0 || g.nn <= 0 || g.nslot <= 0) throw std::invalid_argument("empty surface");
@@ -2085,7 +2129,7 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
     d_b = dalloc<double>(nslot);
     d_sigma = dalloc<double>(nt);
     d_stats = dalloc<unsigned long long>(32);
-    d_count = dalloc<unsigned int>(7);
+    d_count = dalloc<unsigned int>(8);  // [7]: the far tier's work-queue counter
     owned.insert(owned.end(), {d_b, d_sigma, d_stats, d_count});
     // The non-far list's first budget, 384 pairs per test triangle (48 bytes each over the five lists; the abused
     // ellipsoid needs 244).  The first assembly grows it to an upper bound of the count (nonfar_bound) before its far
@@ -2164,7 +2208,21 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
     dim3 grid(ntt, nts - st0);
     if (timing) HIPCHECK(hipEventRecord(ev[0], st));
     auto far_pass = [&](bool enum_only) {
-        if (nts - st0 > 0) {
+        if (nts - st0 > 0 && far_queue_wgs > 0) {  // one persistent work queue over every tile
+            HIPCHECK(hipMemsetAsync(d_count + 7, 0, sizeof(unsigned int), st));
+            const unsigned int nwg = (unsigned int)std::min<long long>((long long)ntt * (nts - st0), far_queue_wgs);
+            if (opts.precision == 1)
+                far_pairs_queue<float><<<nwg, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse,
+                                                            d_tsl, d_tsc, d_tsn, A, d_b, d_nonfar, d_count, nonfar_cap,
+                                                            d_stats + 5, d_tiers, st0, enum_only, ntt, nts - st0,
+                                                            d_count + 7);
+            else
+                far_pairs_queue<double><<<nwg, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse,
+                                                             d_tsl, d_tsc, d_tsn, A, d_b, d_nonfar, d_count, nonfar_cap,
+                                                             d_stats + 5, d_tiers, st0, enum_only, ntt, nts - st0,
+                                                             d_count + 7);
+            HIPCHECK(hipGetLastError());
+        } else if (nts - st0 > 0) {
             if (opts.precision == 1)
                 far_pairs<float><<<grid, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse, d_tsl,
                                                        d_tsc, d_tsn, A, d_b, d_nonfar, d_count, nonfar_cap, d_stats + 5,
