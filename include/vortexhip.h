@@ -293,6 +293,10 @@ int vp_consistency(vp_ctx *ctx, const double *A, long long lda, int A_on_device,
 int vp_factor_info(vp_factor *f, int *n, double *se);
 /* The factorization's current precision mode (0 FP64, 1 FP32 factors) and the number of FP64 fallbacks taken. */
 int vp_factor_mode(vp_factor *f, int *mode, int *fallbacks);
+/* GMRES-IR (no counterpart in the reference): the solves of a mixed factorization that plain refinement left short and
+ * restarted GMRES on M, preconditioned by the FP32 factors, finished without an FP64 refactorization, and the GMRES
+ * iterations taken (environment VP_GMRES=0 turns it off). */
+int vp_factor_gmres(vp_factor *f, long long *solves, long long *iterations);
 void vp_factor_destroy(vp_factor *f);
 /* The LU trailing update's GEMM tile shapes chosen by the autotuner (no counterpart in the reference), as JSON:
  * per precision the chosen BMxBNxBK (null before the first factorization large enough to time), the update size timed

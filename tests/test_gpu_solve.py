@@ -340,3 +340,57 @@ def test_sphere_exact_flow_configured(mode):
     Ao, bo, _, _ = O.assemble(g, dm, O.source_strengths(O.elements(g.x, g.tris), V), V)
     ref = O.solve_galerkin(dm, Ao, bo)
     assert np.abs(sol["mu"] - ref["mu"]).max() < (1e-11 if mode == "fp64" else 1e-7) * np.abs(ref["mu"]).max()
+
+
+def _conditioned(n, kappa, seed=7):
+    """A dense n x n matrix with singular values spaced logarithmically from 1 to 1/kappa"""
+    rng = np.random.default_rng(seed)
+    Q1, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return (Q1 * np.logspace(0, -np.log10(kappa), n)) @ Q2.T, rng.standard_normal(n)
+
+
+@pytest.mark.parametrize("kappa", [1e4, 1e5])
+def test_gmres_ir_where_refinement_stops_short(kappa):
+    """A dense 642 x 642 matrix of condition 1e4 and 1e5: plain refinement with the unpivoted FP32 factors stops
+    short (with VP_GMRES=0 the solve takes the FP64 refactorization), and GMRES on M in FP64 preconditioned by the
+    FP32 factors (GMRES-IR) finishes it in mixed precision, its solution the FP64 factorization's to 1e-10."""
+    import os
+    g = mesh.icosphere(3)
+    dm = dofmap.build_dofmap(g)
+    A, b = _conditioned(dm.nslot, kappa)
+    ref = solve.solve_galerkin(dm, A, b, mode="fp64")
+    scale = np.abs(ref["mu"]).max()
+    print(f"gmres-ir kappa {kappa:.0e}: mixed factors", flush=True)
+    F = solve.factorize_galerkin(dm, A, mode="mixed")
+    out = solve.solve_factored(F, b, cond=False)
+    nsolve, nit = F.gmres_info
+    prec, fb = F.precision_info
+    print(f"  gmres solves {nsolve} iterations {nit} fallbacks {fb} resid {F.last_resid:.2e}", flush=True)
+    assert nsolve == 1 and 0 < nit <= 240 and fb == 0 and prec == "mixed"
+    assert np.abs(out["mu"] - ref["mu"]).max() < 1e-10 * scale
+    F.close()
+    os.environ["VP_GMRES"] = "0"
+    try:
+        F = solve.factorize_galerkin(dm, A, mode="mixed")
+        out = solve.solve_factored(F, b, cond=False)
+        assert F.gmres_info == (0, 0) and F.precision_info == ("fp64", 1)
+        assert np.abs(out["mu"] - ref["mu"]).max() < 1e-10 * scale
+        F.close()
+    finally:
+        os.environ.pop("VP_GMRES", None)
+
+
+def test_gmres_ir_gives_up_to_the_fp64_refactorization():
+    """Condition 1e8: GMRES-IR runs its 8 restarts of 30 steps without meeting the stopping test, and the solve
+    falls back to the FP64 refactorization, whose solution is returned"""
+    g = mesh.icosphere(3)
+    dm = dofmap.build_dofmap(g)
+    A, b = _conditioned(dm.nslot, 1e8)
+    ref = solve.solve_galerkin(dm, A, b, mode="fp64")
+    F = solve.factorize_galerkin(dm, A, mode="mixed")
+    out = solve.solve_factored(F, b, cond=False)
+    print(f"gmres-ir kappa 1e8: {F.gmres_info} {F.precision_info}", flush=True)
+    assert F.gmres_info == (0, 240) and F.precision_info == ("fp64", 1)
+    assert np.abs(out["mu"] - ref["mu"]).max() < 1e-10 * np.abs(ref["mu"]).max()
+    F.close()

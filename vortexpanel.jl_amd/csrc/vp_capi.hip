@@ -468,6 +468,14 @@ int vp_factor_mode(vp_factor *f, int *mode, int *fallbacks)
     return VP_OK;
 }
 
+int vp_factor_gmres(vp_factor *f, long long *solves, long long *iterations)
+{
+    if (!f) return VP_E_ARGUMENT;
+    if (solves) *solves = f->f.gmres_solves;
+    if (iterations) *iterations = f->f.gmres_iters;
+    return VP_OK;
+}
+
 void vp_factor_destroy(vp_factor *f)
 {
     if (!f) return;

@@ -43,6 +43,15 @@ long long next_plan_serial();
 // HIP graph replay of the narrow solves and of the wake field (default on); VP_GRAPHS=0 launches the same kernels one
 // by one instead (for tools that cannot follow graph launches: rocprofv3's kernel trace has crashed inside
 // hipGraphLaunch here)
+// GMRES-IR before the FP64 refactorization of a mixed factorization whose refinement stops short (default on);
+// VP_GMRES=0 goes straight to the refactorization
+inline bool gmres_enabled()
+{
+    const char *e = std::getenv("VP_GMRES");
+    return !(e && e[0] == '0');
+}
+constexpr int GMRES_M = 30, GMRES_RESTARTS = 8;
+
 inline bool graphs_enabled()
 {
     const char *e = std::getenv("VP_GRAPHS");  // read per call (cheap), so a process may switch it
@@ -145,6 +154,10 @@ struct Factor {
     } inv_graph[2];
     long long graph_launches = 0;
     int fallbacks = 0;  // FP32 factorizations replaced by FP64 ones (refinement that did not converge)
+    // GMRES-IR (refinement by preconditioned GMRES when plain refinement with the FP32 factors stops short)
+    double *gV = nullptr, *gx = nullptr, *gw = nullptr, *gb = nullptr, *gh = nullptr;
+    long long gmres_solves = 0, gmres_iters = 0;
+    bool gmres_column(int c, int nrhs, bool trans, double tolf, int max_restarts);
     double anorm1 = 0.0, anorm_inf = -1.0, cond = -1.0;
     std::vector<double> se;
     std::vector<void *> owned;

@@ -569,6 +569,81 @@ __global__ void axpy_inplace(double *x, const double *dx, long long n)
     if (i < n) x[i] += dx[i];
 }
 
+// ---- GMRES-IR's vector kernels (one right hand side at a time, vectors of npad entries) ----
+
+// column c of an n x nrhs row-major block into a vector, and back
+__global__ void gm_gather(const double *__restrict__ src, int nrhs, int c, int n, double *__restrict__ dst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[(long long)i * nrhs + c];
+}
+__global__ void gm_scatter(const double *__restrict__ src, int nrhs, int c, int n, double *__restrict__ dst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[(long long)i * nrhs + c] = src[i];
+}
+
+// out[j] = V[:, j] . w for the ncol basis columns (V column-major, leading dimension ldv): a block per column, summed
+// in a fixed order
+__global__ void gm_dots(const double *__restrict__ V, long long ldv, const double *__restrict__ w, int n,
+                        double *__restrict__ out)
+{
+    __shared__ double sh[256];
+    const double *v = V + (long long)blockIdx.x * ldv;
+    double a = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) a += v[i] * w[i];
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+// w = alpha w + beta V[:, 0..ncol) h (alpha 0 overwrites w without reading it)
+__global__ void gm_update(const double *__restrict__ V, long long ldv, int ncol, const double *__restrict__ h,
+                          double alpha, double beta, double *__restrict__ w, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int j = 0; j < ncol; j++) acc += V[(long long)j * ldv + i] * h[j];
+    w[i] = alpha == 0.0 ? beta * acc : alpha * w[i] + beta * acc;
+}
+
+// dst = s src
+__global__ void gm_scale(const double *__restrict__ src, double s, double *__restrict__ dst, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = s * src[i];
+}
+
+// max |r| and max |x| (a non-finite entry makes the first infinite), for the stopping test
+__global__ void gm_maxabs(const double *__restrict__ r, const double *__restrict__ x, int n, double *__restrict__ out)
+{
+    __shared__ double s1[256], s2[256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        a = fmax(a, (isfinite(r[i]) && isfinite(x[i])) ? fabs(r[i]) : INFINITY);
+        b = fmax(b, fabs(x[i]));
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + o]);
+            s2[threadIdx.x] = fmax(s2[threadIdx.x], s2[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s1[0];
+        out[1] = s2[0];
+    }
+}
+
 // The refinement residual for a few right hand sides (nrhs <= GEMV_MAX), HBM-bound (M is read once, 8 bytes per
 // entry): Y -= M X with one wavefront per row of M (row-major, ld), its lanes striding over the columns, the sums
 // reduced over the wave; and Y -= Mᵀ X in two passes, a thread per column of M summing a slice of KSPLIT rows into
@@ -1243,6 +1318,35 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
         axpy_inplace<<<blocks((long long)sz), 256, 0, s>>>(X, Y, (long long)sz);
         HIPCHECK2(hipGetLastError());
     }
+    if (!done && strict && mode == 1 && gmres_enabled()) {
+        // GMRES-IR on every right hand side from the refinement's iterate, before a second factorization
+        bool all = true;
+        for (int c = 0; c < nrhs && all; c++) all = gmres_column(c, nrhs, trans, tolf, GMRES_RESTARTS);
+        if (all) {
+            gmres_solves++;
+            HIPCHECK2(hipMemcpyAsync(Y, B, sizeof(double) * sz, hipMemcpyDeviceToDevice, s));
+            switch (nrhs <= GEMV_MAX ? nrhs : 0) {
+            case 1: launch_residual<1>(s, trans, M, ld, npad, X, Y, part); break;
+            case 2: launch_residual<2>(s, trans, M, ld, npad, X, Y, part); break;
+            case 3: launch_residual<3>(s, trans, M, ld, npad, X, Y, part); break;
+            case 4: launch_residual<4>(s, trans, M, ld, npad, X, Y, part); break;
+            default:
+                if (trans)
+                    launch_gemm_cfg<double, true, true>(0, s, npad, nrhs, npad, M, ld, X, nrhs, Y, nrhs);
+                else
+                    launch_gemm<double, true>(s, npad, nrhs, npad, M, ld, X, nrhs, Y, nrhs);
+            }
+            colmax2<<<nrhs, 256, 0, s>>>(Y, X, B, n, nrhs, nrm);
+            HIPCHECK2(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * 4 * nrhs, hipMemcpyDeviceToHost, s));
+            HIPCHECK2(hipStreamSynchronize(s));
+            worst = 0.0;
+            for (int c = 0; c < nrhs; c++) {
+                const double rb = std::sqrt(hn[4 * c + 3]), rr = std::sqrt(hn[4 * c + 2]);
+                worst = std::max(worst, rb > 0 ? rr / rb : rr);
+            }
+            done = true;
+        }
+    }
     if (!done && strict) {
         if (mode == 1) {
             refactor_fp64();
@@ -1260,6 +1364,115 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
     HIPCHECK2(hipStreamSynchronize(s));
     last_iters = it;
     return it;
+}
+
+// GMRES-IR (Carson & Higham, SIAM J. Sci. Comput. 40, 2018): when refinement with the FP32 factors stops short, the
+// correction equation is solved by restarted GMRES(m) on op(M) in FP64, right-preconditioned by the factors' inverse
+// (the butterflies and the FP32 LU), from the refinement's iterate: x_{k+1} = x_k + M~^-1 V y, with V the Arnoldi basis
+// of op(M) M~^-1 (classical Gram-Schmidt twice) and y the Hessenberg least-squares solution (Givens rotations on the
+// host).  Column c of the workspace X (row-major npad x nrhs) is refined against column c of B; the stopping test is
+// the refinement's (|r|_inf <= |x|_inf |op(M)|_inf eps sqrt(n) on the true residual).  Returns true when it is met.
+bool Factor::gmres_column(int c, int nrhs, bool trans, double tolf, int max_restarts)
+{
+    hipStream_t s = ctx->stream;
+    const int m = GMRES_M;
+    if (!gV) {
+        gV = dmalloc<double>((size_t)npad * (m + 1));
+        gx = dmalloc<double>(npad);
+        gw = dmalloc<double>(npad);
+        gb = dmalloc<double>(npad);
+        gh = dmalloc<double>(m + 2);
+        owned.insert(owned.end(), {gV, gx, gw, gb, gh});
+    }
+    const unsigned nb = (unsigned)blocks(npad);
+    gm_gather<<<nb, 256, 0, s>>>(X, nrhs, c, npad, gx);
+    gm_gather<<<nb, 256, 0, s>>>(B, nrhs, c, npad, gb);
+    HIPCHECK2(hipGetLastError());
+    auto residual = [&]() {  // gw = b - op(M) x
+        HIPCHECK2(hipMemcpyAsync(gw, gb, sizeof(double) * npad, hipMemcpyDeviceToDevice, s));
+        launch_residual<1>(s, trans, M, ld, npad, gx, gw, part);
+    };
+    auto dots = [&](const double *V, int ncol, const double *w, double *h) {
+        gm_dots<<<ncol, 256, 0, s>>>(V, npad, w, npad, gh);
+        HIPCHECK2(hipGetLastError());
+        HIPCHECK2(hipMemcpyAsync(h, gh, sizeof(double) * ncol, hipMemcpyDeviceToHost, s));
+        HIPCHECK2(hipStreamSynchronize(s));
+    };
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), h(m + 1), h2(m + 1), y(m);
+    double mx[2];
+    bool converged = false;
+    for (int restart = 0; restart <= max_restarts; restart++) {
+        residual();
+        gm_maxabs<<<1, 256, 0, s>>>(gw, gx, npad, gh);
+        HIPCHECK2(hipMemcpyAsync(mx, gh, sizeof(mx), hipMemcpyDeviceToHost, s));
+        HIPCHECK2(hipStreamSynchronize(s));
+        if (mx[0] <= mx[1] * tolf) {
+            converged = true;
+            break;
+        }
+        if (!std::isfinite(mx[0]) || restart == max_restarts) break;
+        double beta;
+        dots(gw, 1, gw, &beta);
+        beta = std::sqrt(beta);
+        if (!(beta > 0)) break;
+        gm_scale<<<nb, 256, 0, s>>>(gw, 1.0 / beta, gV, npad);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int k = 0;
+        for (int j = 0; j < m; j++) {
+            // z = M~^-1 v_j (in Y), w = op(M) z
+            HIPCHECK2(hipMemcpyAsync(Y, gV + (size_t)j * npad, sizeof(double) * npad, hipMemcpyDeviceToDevice, s));
+            apply_inverse_replay(1, trans);
+            HIPCHECK2(hipMemsetAsync(gw, 0, sizeof(double) * npad, s));
+            launch_residual<1>(s, trans, M, ld, npad, Y, gw, part);  // gw = -op(M) z
+            gm_scale<<<nb, 256, 0, s>>>(gw, -1.0, gw, npad);
+            // classical Gram-Schmidt, twice
+            dots(gV, j + 1, gw, h.data());
+            HIPCHECK2(hipMemcpyAsync(gh, h.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, s));
+            gm_update<<<nb, 256, 0, s>>>(gV, npad, j + 1, gh, 1.0, -1.0, gw, npad);
+            dots(gV, j + 1, gw, h2.data());
+            HIPCHECK2(hipMemcpyAsync(gh, h2.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, s));
+            gm_update<<<nb, 256, 0, s>>>(gV, npad, j + 1, gh, 1.0, -1.0, gw, npad);
+            for (int i = 0; i <= j; i++) h[i] += h2[i];
+            double hn;
+            dots(gw, 1, gw, &hn);
+            hn = std::sqrt(hn);
+            h[j + 1] = hn;
+            if (hn > 0) gm_scale<<<nb, 256, 0, s>>>(gw, 1.0 / hn, gV + (size_t)(j + 1) * npad, npad);
+            HIPCHECK2(hipGetLastError());
+            gmres_iters++;
+            // the previous rotations on the new column, then its own
+            for (int i = 0; i < j; i++) {
+                const double t = cs[i] * h[i] + sn[i] * h[i + 1];
+                h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+                h[i] = t;
+            }
+            const double r = std::hypot(h[j], h[j + 1]);
+            cs[j] = r > 0 ? h[j] / r : 1.0;
+            sn[j] = r > 0 ? h[j + 1] / r : 0.0;
+            h[j] = r;
+            h[j + 1] = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = h[i];
+            k = j + 1;
+            if (!(hn > 0) || std::fabs(g[j + 1]) <= 1e-14 * beta) break;  // the Krylov space holds the correction
+        }
+        // y = R^-1 g, u = V y, x += M~^-1 u
+        for (int i = k - 1; i >= 0; i--) {
+            double t = g[i];
+            for (int l = i + 1; l < k; l++) t -= H[(size_t)i * m + l] * y[l];
+            y[i] = H[(size_t)i * m + i] != 0.0 ? t / H[(size_t)i * m + i] : 0.0;
+        }
+        HIPCHECK2(hipMemcpyAsync(gh, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+        gm_update<<<nb, 256, 0, s>>>(gV, npad, k, gh, 0.0, 1.0, Y, npad);
+        apply_inverse_replay(1, trans);
+        axpy_inplace<<<nb, 256, 0, s>>>(gx, Y, npad);
+        HIPCHECK2(hipGetLastError());
+    }
+    gm_scatter<<<nb, 256, 0, s>>>(gx, nrhs, c, npad, X);
+    HIPCHECK2(hipGetLastError());
+    return converged;
 }
 
 // dsgesv's fallback: the FP32 factors did not give a converged solve, so M (kept in FP64) is factored again in FP64

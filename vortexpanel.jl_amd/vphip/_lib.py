@@ -73,7 +73,7 @@ EXPORTS = ["vp_create", "vp_destroy", "vp_last_error", "vp_sync", "vp_assemble",
            "vp_surface_velocity", "vp_body_velocity", "vp_blob_velocities", "vp_plan_adjoint",
            "vp_factor_solve_transposed", "vp_plan_wake_product", "vp_plan_blob_rhs", "vp_gemm_tuning", "vp_math_probe",
            "vp_assemble_mixed", "vp_wake_field_create", "vp_wake_field_eval", "vp_wake_field_info", "vp_wake_field_destroy",
-           "vp_plan_update_wake", "vp_assembly_tuning", "vp_factor_mode",
+           "vp_plan_update_wake", "vp_assembly_tuning", "vp_factor_mode", "vp_factor_gmres",
            "vp_consistency", "vp_blob_velocities_device", "vp_device_alloc", "vp_device_free", "vp_copy_to_device",
            "vp_copy_from_device", "vp_explicit_create", "vp_explicit_rhs", "vp_explicit_info", "vp_explicit_destroy",
            "vp_diag_lu_probe", "vp_freewake_create", "vp_freewake_step", "vp_freewake_counts", "vp_freewake_read", "vp_freewake_destroy"]

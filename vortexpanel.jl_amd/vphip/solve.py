@@ -79,6 +79,14 @@ class GalerkinFactorization:
         _check(self.ctx.h, self.ctx.lib.vp_factor_mode(self.h, C.byref(m), C.byref(fb)))
         return ("fp64" if m.value == 0 else "mixed"), fb.value
 
+    @property
+    def gmres_info(self) -> tuple[int, int]:
+        """(solves finished by GMRES-IR, GMRES iterations): mixed factors whose plain refinement stops short are first
+        used as the preconditioner of restarted GMRES on M (in FP64) before an FP64 refactorization"""
+        sv, it = C.c_longlong(), C.c_longlong()
+        _check(self.ctx.h, self.ctx.lib.vp_factor_gmres(self.h, C.byref(sv), C.byref(it)))
+        return sv.value, it.value
+
     def solve_bordered(self, R):
         """X = M^-1 R for R (n,) or (n, nrhs)"""
         R = np.asarray(R, dtype=np.float64)
