@@ -335,6 +335,31 @@ int vp_blob_velocities(vp_ctx *ctx, int nb, const double *bx, const double *alph
  * shares that array, so calls that overlap on one GPU (from other contexts or threads) race. */
 int vp_blob_velocities_device(vp_ctx *ctx, int nb, const double *bx, const double *alpha, const double *delta, int np,
                               const double *xs, int precision, double *u);
+/* Fast summation of blob_velocities (no counterpart in the reference, which sums directly and defers fast summation,
+ * docs/claude_handoff.md:177): a cluster-particle treecode.  Target clusters well separated from a source cluster
+ * (box radius <= theta * distance from the box centre to the sources' box) take that cluster's field at the
+ * (degree+1)^3 Chebyshev points of their box and interpolate it (barycentric Lagrange); every other pair is summed
+ * directly.  Each blob keeps its own core radius and exact kernel.  FP64.  theta in (0, 1), degree 1..7. */
+typedef struct {
+    double theta;
+    int degree;
+} vp_tree_options;
+typedef struct {
+    long long far_clusters;  /* target clusters that interpolate */
+    long long far_pairs;     /* (target cluster, source cluster) pairs taken at the proxies */
+    long long near_pairs;    /* pairs summed directly */
+    long long proxy_evals;   /* kernel evaluations at proxies */
+    long long near_evals;    /* kernel evaluations at targets */
+    long long interp_evals;  /* proxy values read by the interpolation */
+    double build_ms;         /* host: trees, lists and sorted copies */
+    double device_ms;        /* device: the three kernels */
+} vp_tree_stats;
+int vp_blob_velocities_tree(vp_ctx *ctx, int nb, const double *bx, const double *alpha, const double *delta, int np,
+                            const double *xs, const vp_tree_options *opts, double *u, vp_tree_stats *stats);
+/* The treecode's lists on the host only (test entry): *bad = the number of target leaves whose direct ranges and far
+ * ancestors' ranges do not cover every source exactly once (0 when the lists are right). */
+int vp_tree_coverage(int nb, const double *bx, int np, const double *xs, const vp_tree_options *opts, long long *bad,
+                     vp_tree_stats *stats);
 /* Device buffers on the context's device for the device-pointer entry points (hipMalloc / hipFree), and blocking copies
  * to and from them on the context's stream. */
 int vp_device_alloc(vp_ctx *ctx, long long bytes, void **out);

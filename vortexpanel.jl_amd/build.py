@@ -7,7 +7,7 @@ import subprocess
 PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "lib", "libvortexhip.so")
-SOURCES = ["vp_assemble.hip", "vp_lu.hip", "vp_field.hip", "vp_capi.hip", "vp_freewake.hip", "vp_mixed.hip"]
+SOURCES = ["vp_assemble.hip", "vp_lu.hip", "vp_field.hip", "vp_capi.hip", "vp_freewake.hip", "vp_mixed.hip", "vp_tree.hip"]
 
 # -munsafe-fp-atomics: hardware FP64 atomic adds (global and LDS) instead of compare-and-swap loops.
 # -packed-fp32-ops off: no v_pk_*_f32 instructions anywhere.

@@ -564,6 +564,30 @@ int vp_blob_velocities(vp_ctx *ctx, int nb, const double *bx, const double *alph
     });
 }
 
+int vp_blob_velocities_tree(vp_ctx *ctx, int nb, const double *bx, const double *alpha, const double *delta, int np,
+                            const double *xs, const vp_tree_options *opts, double *u, vp_tree_stats *stats)
+{
+    if (!ctx) return VP_E_ARGUMENT;
+    if (!opts || (nb > 0 && (!bx || !alpha || !delta)) || (np > 0 && (!xs || !u)))
+        return fail(&ctx->c, VP_E_ARGUMENT, "null argument");
+    if (nb < 0 || np < 0) return fail(&ctx->c, VP_E_DIMENSION, "negative size");
+    if (stats) *stats = vp_tree_stats{};
+    if (np == 0) return VP_OK;
+    return guard(&ctx->c, [&] {
+        (void)hipSetDevice(ctx->c.device);
+        vp::blob_velocities_tree_host(&ctx->c, nb, bx, alpha, delta, np, xs, *opts, u, stats);
+    });
+}
+
+int vp_tree_coverage(int nb, const double *bx, int np, const double *xs, const vp_tree_options *opts, long long *bad,
+                     vp_tree_stats *stats)
+{
+    if (!opts || !bad || (nb > 0 && !bx) || (np > 0 && !xs)) return VP_E_ARGUMENT;
+    if (nb < 0 || np < 0) return VP_E_DIMENSION;
+    if (stats) *stats = vp_tree_stats{};
+    return guard(nullptr, [&] { *bad = vp::tree_coverage_host(nb, bx, np, xs, *opts, stats); });
+}
+
 int vp_blob_velocities_device(vp_ctx *ctx, int nb, const double *bx, const double *alpha, const double *delta, int np,
                               const double *xs, int precision, double *u)
 {

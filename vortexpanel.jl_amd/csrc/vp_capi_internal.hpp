@@ -189,6 +189,11 @@ void blob_velocities_noalloc(Ctx *c, int nb, const double *bx, const double *ba,
                              const double *xs, int precision, double *u);
 void blob_velocities_host(Ctx *c, int nb, const double *bx, const double *ba, const double *bd, int np,
                           const double *xs, int precision, double *u);
+// The blob sums by the cluster-particle treecode (vp_tree.hip), and its lists' coverage check (host only)
+void blob_velocities_tree_host(Ctx *c, int nb, const double *bx, const double *ba, const double *bd, int np,
+                               const double *xs, const vp_tree_options &o, double *u, vp_tree_stats *stats);
+long long tree_coverage_host(int nb, const double *bx, int np, const double *xs, const vp_tree_options &o,
+                             vp_tree_stats *stats);
 // The free-wake step's velocity sums (the body's kernels and the wake's blobs at np targets, plus a base velocity)
 // replayed as one HIP graph per node structure, its nodes' grids and arguments rewritten on every call.
 struct WakeField {

@@ -64,6 +64,18 @@ class FreeWakeRecordC(C.Structure):
                 ("lam", C.c_double), ("umax", C.c_double), ("t_rhs", C.c_double), ("t_vel", C.c_double)]
 
 
+class TreeOptionsC(C.Structure):
+    """vp_tree_options"""
+    _fields_ = [("theta", C.c_double), ("degree", C.c_int)]
+
+
+class TreeStatsC(C.Structure):
+    """vp_tree_stats"""
+    _fields_ = [("far_clusters", C.c_longlong), ("far_pairs", C.c_longlong), ("near_pairs", C.c_longlong),
+                ("proxy_evals", C.c_longlong), ("near_evals", C.c_longlong), ("interp_evals", C.c_longlong),
+                ("build_ms", C.c_double), ("device_ms", C.c_double)]
+
+
 _lib = None
 
 # every symbol include/vortexhip.h declares
@@ -76,7 +88,8 @@ EXPORTS = ["vp_create", "vp_destroy", "vp_last_error", "vp_sync", "vp_assemble",
            "vp_plan_update_wake", "vp_assembly_tuning", "vp_factor_mode", "vp_factor_gmres",
            "vp_consistency", "vp_blob_velocities_device", "vp_device_alloc", "vp_device_free", "vp_copy_to_device",
            "vp_copy_from_device", "vp_explicit_create", "vp_explicit_rhs", "vp_explicit_info", "vp_explicit_destroy",
-           "vp_diag_lu_probe", "vp_freewake_create", "vp_freewake_step", "vp_freewake_counts", "vp_freewake_read", "vp_freewake_destroy"]
+           "vp_diag_lu_probe", "vp_freewake_create", "vp_freewake_step", "vp_freewake_counts", "vp_freewake_read", "vp_freewake_destroy",
+           "vp_blob_velocities_tree", "vp_tree_coverage"]
 
 
 def load():
