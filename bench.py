@@ -347,6 +347,34 @@ def cpu_mixed_leg(threads, gpu, target_s=12.0):
                        f"integrals, {dt:.1f} s)")
 
 
+def blob_tree_bench(sizes=(84_000, 500_000, 1_000_000), theta=0.5, degree=5):
+    """The blob sums by the treecode against direct sums, both FP64: the benchmark's blob cloud (spread 3 x 3 x 0.3,
+    strengths 1e-3, cores 0.005 to 0.02) summed at every blob, the wall time of each (host arrays in and out), the
+    treecode's host build and device times, its kernel evaluations against the direct sum's n^2, and its largest error
+    against the direct sum relative to the largest velocity"""
+    from vphip import post
+    rng = np.random.default_rng(0)
+    post.blob_velocities_tree(rng.standard_normal((3000, 3)), rng.standard_normal((3000, 3)), np.full(3000, 0.01),
+                              rng.standard_normal((3000, 3)), theta=theta, degree=degree)
+    out = dict(theta=theta, degree=degree, runs=[])
+    for n in sizes:
+        bx = rng.standard_normal((n, 3)) * np.array([3.0, 3.0, 0.3])
+        ba = rng.standard_normal((n, 3)) * 1e-3
+        bd = rng.uniform(0.005, 0.02, n)
+        t0 = time.perf_counter()
+        ut, st = post.blob_velocities_tree(bx, ba, bd, bx, theta=theta, degree=degree)
+        tt = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ud = post.blob_velocities(bx, ba, bd, bx)
+        td = time.perf_counter() - t0
+        err = float(np.abs(ut - ud).max() / np.abs(ud).max())
+        out["runs"].append(dict(n=n, direct_s=td, tree_s=tt, speedup=td / tt, build_ms=st["build_ms"],
+                                device_ms=st["device_ms"], evals_vs_direct=(st["proxy_evals"] + st["near_evals"]) / n ** 2,
+                                far_clusters=st["far_clusters"], error=err))
+        log(f"blob tree n={n}: {out['runs'][-1]}")
+    return out
+
+
 def nbody_bench(g, dm, V, mu):
     """surface_velocity on the ellipsoid solution and blob sums at the free wake's end size (84,000 particles)"""
     from vphip import api, post
@@ -383,7 +411,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)  # the launch autotuner times its candidates on the first two
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="only the assembly metric (no wing, sweep, N-body)")
-    ap.add_argument("--extras", default="nbody,alpha_sweep,wing_adjoint,free_wake,mixed_mesh,uneven_bodies",
+    ap.add_argument("--extras", default="nbody,alpha_sweep,wing_adjoint,free_wake,mixed_mesh,uneven_bodies,blob_tree",
                     help="which of the extra workloads to run (comma-separated)")
     ap.add_argument("--no-solve", action="store_true", help="no factorizations or solves (with --no-extras: the "
                     "assemblies alone, the program the far kernel's counter passes profile)")
@@ -494,7 +522,7 @@ def main():
     fw = None
     for name, fn in (("alpha_sweep", alpha_sweep_bench), ("wing_adjoint", wing_adjoint_bench),
                      ("free_wake", free_wake_bench), ("mixed_mesh", mixed_mesh_bench),
-                     ("uneven_bodies", uneven_bodies_bench)):
+                     ("uneven_bodies", uneven_bodies_bench), ("blob_tree", blob_tree_bench)):
         if name in want:
             extras[name] = fn()
             if name == "free_wake":

@@ -222,6 +222,11 @@ typedef struct vp_freewake_setup {
     const int *last;
     const vp_wake *panel;             /* the trailing edge panel as a wake (nodes, triangles, filaments) */
     const int *panel_te;              /* its nodes' trailing edge node (RigidWake.te) */
+    /* The wake blobs' velocities by the treecode of vp_blob_velocities_tree (no counterpart in the reference):
+     * tree_theta 0 sums them directly (the reference's blob_velocities); otherwise theta in (0, 1) and tree_degree
+     * 1..7.  The body's kernels stay direct sums either way. */
+    double tree_theta;
+    int tree_degree;
 } vp_freewake_setup;
 /* step!'s record (step.jl:138-140): n, t, F_pressure, M_pressure, F_kj, F_impulse (NaN on the first step), Ω, I,
  * rows, particles, λ (the first border multiplier), umax, t_rhs and t_vel (s); iters: the solve's refinements. */
@@ -340,14 +345,14 @@ int vp_blob_velocities_device(vp_ctx *ctx, int nb, const double *bx, const doubl
  * (box radius <= theta * distance from the box centre to the sources' box) take that cluster's field at the
  * (degree+1)^3 Chebyshev points of their box and interpolate it (barycentric Lagrange); every other pair is summed
  * directly.  Each blob keeps its own core radius and exact kernel.  FP64.  theta in (0, 1), degree 1..7. */
-typedef struct {
+typedef struct vp_tree_options {
     double theta;
     int degree;
 } vp_tree_options;
-typedef struct {
+typedef struct vp_tree_stats {
     long long far_clusters;  /* target clusters that interpolate */
     long long far_pairs;     /* (target cluster, source cluster) pairs taken at the proxies */
-    long long near_pairs;    /* pairs summed directly */
+    long long near_pairs;    /* (target leaf, source range) pairs summed directly */
     long long proxy_evals;   /* kernel evaluations at proxies */
     long long near_evals;    /* kernel evaluations at targets */
     long long interp_evals;  /* proxy values read by the interpolation */

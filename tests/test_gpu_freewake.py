@@ -173,12 +173,31 @@ def test_free_wake_mixed(wing):
         assert np.abs(a["mu_w"] - b["mu_w"]).max() < 1e-6 * np.abs(b["mu_w"]).max()
 
 
+def test_free_wake_tree(wing):
+    """The native engine with the wake blobs' velocities by the treecode (theta 0.5, degree 6) against direct sums
+    over 30 steps: the same rows and particle counts, mu_w within 1e-7 of its largest value"""
+    print("free wake 30 steps tree vs direct", flush=True)
+    g, dm = wing
+    dt = wk.te_panel_size(g) * 2
+    o = FW.FreeWakeOptions(dt=dt, max_rows=3)
+    runs = {}
+    for t in (None, (0.5, 6)):
+        s = FW.FreeWakeSim(g, dm, V, o, tree=t)
+        runs[t] = s.run(30)
+        s.close()
+    for a, b in zip(runs[(0.5, 6)], runs[None]):
+        assert a["particles"] == b["particles"] and a["rows"] == b["rows"]
+        assert np.abs(a["mu_w"] - b["mu_w"]).max() < 1e-7 * np.abs(b["mu_w"]).max()
+    with pytest.raises(ValueError):
+        FW.FreeWakeSim(g, dm, V, o, tree=(1.5, 6))
+
+
 def test_free_wake_full_size():
     """The benchmark's free-wake run at full size (rectangular NACA 0012 wing, AR 6, 120 x 40 quads: 9,836
     triangles; alpha 8 deg, dt 0.02, fixed core), its first 300 steps to ~25,000 particles: mixed (FP32 velocity
-    kernels, FP32 LU with FP64 refinement) against FP64 step by step (the same particle counts and rows, CL by
-    Kutta-Joukowski within 1e-5), and the impulsive start's lift growing from one 25-step checkpoint to the next
-    (Wagner)"""
+    kernels, FP32 LU with FP64 refinement) and the wake blobs by the treecode (theta 0.5, degree 6) against FP64 direct
+    sums step by step (the same particle counts and rows, CL by Kutta-Joukowski within 1e-5), and the impulsive
+    start's lift growing from one 25-step checkpoint to the next (Wagner)"""
     spec = mesh.WingSpec(code="0012", n_around=120, n_span=40, aspect=6.0)
     g = mesh.naca_wing(spec)
     dm = dofmap.build_dofmap(g)
@@ -187,17 +206,18 @@ def test_free_wake_full_size():
     o = FW.FreeWakeOptions(dt=0.02, core=float(0.1 * edges.mean()))
     print(f"case free wake full size nt={g.nt}", flush=True)
     runs = {}
-    for p in ("fp64", "mixed"):
-        s = FW.FreeWakeSim(g, dm, V8, o, precision=p)
+    for p in ("fp64", "mixed", "tree"):
+        s = FW.FreeWakeSim(g, dm, V8, o, precision="fp64" if p == "tree" else p, tree=(0.5, 6) if p == "tree" else None)
         runs[p] = s.run(300)
         s.close()
         print(f"  {p}: {runs[p][-1]['particles']} particles, CL {runs[p][-1]['F_kj'] @ L / spec.aspect:.6f}", flush=True)
     cl = {p: np.array([r["F_kj"] @ L / spec.aspect for r in runs[p]]) for p in runs}
-    for a, b in zip(runs["mixed"], runs["fp64"]):
-        assert a["particles"] == b["particles"] and a["rows"] == b["rows"]
-    d = np.abs(cl["mixed"] - cl["fp64"]).max()
-    print(f"  mixed vs fp64: max |dCL| {d:.2e}", flush=True)
-    assert d < 1e-5, d
+    for p in ("mixed", "tree"):
+        for a, b in zip(runs[p], runs["fp64"]):
+            assert a["particles"] == b["particles"] and a["rows"] == b["rows"]
+        d = np.abs(cl[p] - cl["fp64"]).max()
+        print(f"  {p} vs fp64: max |dCL| {d:.2e}", flush=True)
+        assert d < 1e-5, d
     assert runs["fp64"][-1]["particles"] > 20000
     assert np.all(np.diff(cl["fp64"][24::25]) > 0), cl["fp64"][24::25]
 

@@ -177,6 +177,7 @@ struct vp_freewake {
     P3 V;
     double dt = 0, delete_beyond = 0, span = 0;
     int advection = 0, keep_rows = 0;
+    vp_tree_options tree{0.0, 0};  // the wake blobs by the treecode when tree.theta > 0
     Core core;
     // state (state.jl:123-143)
     std::vector<std::vector<P3>> rows;  // node rows, newest (R0) first
@@ -193,7 +194,7 @@ struct vp_freewake {
     std::vector<double> mu, mu_w, cp;
     P3 I_prev;
     // scratch
-    std::vector<double> rhs, R, X, U, Xt, bx, ba, bd, g3;
+    std::vector<double> rhs, R, X, U, Xt, bx, ba, bd, g3, Ub;
 
     ~vp_freewake()
     {
@@ -512,8 +513,17 @@ void vp_freewake::step(vp_freewake_record *rec, double *mu_w_out)
         std::vector<double> ax, aa, ad;
         flat(all, ax, aa, ad);
         const double base[3] = {V.x, V.y, V.z};
-        check(ctx, vp_wake_field_eval(field, sigma.data(), g3.data(), (int)np, Xt.data(), (int)all.size(), ax.data(),
-                                      aa.data(), ad.data(), base, U.data()));
+        if (tree.theta > 0.0) {  // the body in the graph, the wake's blobs by the treecode
+            check(ctx, vp_wake_field_eval(field, sigma.data(), g3.data(), (int)np, Xt.data(), 0, nullptr, nullptr,
+                                          nullptr, base, U.data()));
+            Ub.assign(3 * np, 0.0);
+            check(ctx, vp_blob_velocities_tree(ctx, (int)all.size(), ax.data(), aa.data(), ad.data(), (int)np,
+                                               Xt.data(), &tree, Ub.data(), nullptr));
+            for (size_t k = 0; k < 3 * np; k++) U[k] += Ub[k];
+        } else {
+            check(ctx, vp_wake_field_eval(field, sigma.data(), g3.data(), (int)np, Xt.data(), (int)all.size(),
+                                          ax.data(), aa.data(), ad.data(), base, U.data()));
+        }
     }
     const double t_vel = now() - t0;
     double umax = 0;
@@ -583,6 +593,9 @@ int vp_freewake_create(vp_ctx *ctx, const vp_surface *g, const vp_freewake_setup
     if (!(su->dt > 0) || su->keep_rows < 0 || su->core_mode < 0 || su->core_mode > 2 || su->advection < 0 ||
         su->advection > 1 || su->nte <= 0 || su->nq <= 0)
         return bad("bad option in the set-up");
+    if (su->tree_theta != 0.0 && !(su->tree_theta > 0.0 && su->tree_theta < 1.0 && su->tree_degree >= 1 &&
+                                   su->tree_degree <= 7))
+        return bad("tree_theta must be 0 (direct sums) or lie in (0, 1), with tree_degree in 1..7");
     *out = nullptr;
     vp_freewake *h = new (std::nothrow) vp_freewake;
     if (!h) return VP_E_DEVICE;
@@ -602,6 +615,7 @@ int vp_freewake_create(vp_ctx *ctx, const vp_surface *g, const vp_freewake_setup
         h->V = ld(su->vinf);
         h->dt = su->dt;
         h->advection = su->advection;
+        h->tree = vp_tree_options{su->tree_theta, su->tree_degree};
         h->delete_beyond = su->delete_beyond;
         h->span = su->span;
         h->keep_rows = su->keep_rows;

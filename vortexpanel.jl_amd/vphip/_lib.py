@@ -53,7 +53,7 @@ class FreeWakeSetupC(C.Structure):
                 ("K_rowptr", _PI), ("K_col", _PI), ("K_val", _PD), ("te_x", _PD), ("nteseg", C.c_int),
                 ("teseg", _PI), ("edge", _PD), ("nstrip", C.c_int), ("strips", _PI), ("merge", _PI),
                 ("nfirst", C.c_int), ("first", _PI), ("nlast", C.c_int), ("last", _PI), ("panel", C.POINTER(WakeC)),
-                ("panel_te", _PI)]
+                ("panel_te", _PI), ("tree_theta", C.c_double), ("tree_degree", C.c_int)]
 
 
 class FreeWakeRecordC(C.Structure):

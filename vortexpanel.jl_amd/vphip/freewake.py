@@ -636,8 +636,11 @@ class FreeWakeSim(FreeWakeHost):
     back from the engine."""
 
     def __init__(self, g, dm, vinf, opts: FreeWakeOptions, assembly: AssemblyOptions | None = None,
-                 precision: str = "fp64", ctx: Context | None = None):
+                 precision: str = "fp64", ctx: Context | None = None, tree: tuple | None = None):
+        """tree = (theta, degree): the wake blobs' velocities by the treecode (post.blob_velocities_tree) instead of
+        direct sums"""
         self._h = None
+        self.tree = tree
         super().__init__(g, dm, vinf, opts, assembly, precision, ctx)
         self._make_engine()
 
@@ -676,7 +679,7 @@ class FreeWakeSim(FreeWakeHost):
             L.ptr(k["teseg"], C.c_int), L.ptr(k["edge"]), len(topo.segments), L.ptr(k["strips"], C.c_int),
             L.ptr(k["merge"], C.c_int) if k["merge"] is not None else None, len(topo.first_nodes),
             L.ptr(k["first"], C.c_int), len(topo.last_nodes), L.ptr(k["last"], C.c_int), C.pointer(wc),
-            L.ptr(k["pte"], C.c_int))
+            L.ptr(k["pte"], C.c_int), float(self.tree[0]) if self.tree else 0.0, int(self.tree[1]) if self.tree else 0)
         s, keep = _surface_c(self.g, dm)
         h = C.c_void_p()
         _check(self.ctx.h, self.ctx.lib.vp_freewake_create(self.ctx.h, C.byref(s), C.byref(su), C.byref(h)))
