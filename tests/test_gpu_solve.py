@@ -350,11 +350,13 @@ def _conditioned(n, kappa, seed=7):
     return (Q1 * np.logspace(0, -np.log10(kappa), n)) @ Q2.T, rng.standard_normal(n)
 
 
-@pytest.mark.parametrize("kappa", [1e4, 1e5])
+@pytest.mark.parametrize("kappa", [1e2, 1e3])
 def test_gmres_ir_where_refinement_stops_short(kappa):
-    """A dense 642 x 642 matrix of condition 1e4 and 1e5: plain refinement with the unpivoted FP32 factors stops
-    short (with VP_GMRES=0 the solve takes the FP64 refactorization), and GMRES on M in FP64 preconditioned by the
-    FP32 factors (GMRES-IR) finishes it in mixed precision, its solution the FP64 factorization's to 1e-10."""
+    """A dense 642 x 642 nonsymmetric matrix of condition 1e2 and 1e3 in the bordered system: the unpivoted FP32
+    factors are poor there (their growth, not the condition), so plain refinement stops short (with VP_GMRES=0 the
+    solve takes the FP64 refactorization), and GMRES on M in FP64 preconditioned by the FP32 factors (GMRES-IR)
+    finishes it in mixed precision, in about 100 of its 240 iterations, its solution the FP64 factorization's to
+    1e-10."""
     import os
     g = mesh.icosphere(3)
     dm = dofmap.build_dofmap(g)
@@ -377,6 +379,35 @@ def test_gmres_ir_where_refinement_stops_short(kappa):
         assert F.gmres_info == (0, 0) and F.precision_info == ("fp64", 1)
         assert np.abs(out["mu"] - ref["mu"]).max() < 1e-10 * scale
         F.close()
+    finally:
+        os.environ.pop("VP_GMRES", None)
+
+
+def test_gmres_ir_alone_with_good_factors():
+    """VP_GMRES=2 runs GMRES-IR right after the first application of the mixed factors: where the factors are good
+    (the sphere's Galerkin matrix, a symmetric positive definite matrix of condition 1e4) it reaches the FP64 solution
+    in a few iterations, as a right-preconditioned GMRES must (a wrong preconditioner application would take
+    hundreds)"""
+    import os
+    g = mesh.icosphere(3)
+    dm = dofmap.build_dofmap(g)
+    V = np.array([1.0, 0.2, 0.1])
+    els = api.build_elements(g)
+    Ag, bg, _ = api.assemble(g, dm, els, api.source_strengths(els, V), V)
+    rng = np.random.default_rng(7)
+    Q, _ = np.linalg.qr(rng.standard_normal((dm.nslot, dm.nslot)))
+    As = (Q * np.logspace(0, -4, dm.nslot)) @ Q.T
+    os.environ["VP_GMRES"] = "2"
+    try:
+        for name, A, b in (("sphere", Ag, bg), ("spd 1e4", As, rng.standard_normal(dm.nslot))):
+            ref = solve.solve_galerkin(dm, A, b, mode="fp64")
+            F = solve.factorize_galerkin(dm, A, mode="mixed")
+            out = solve.solve_factored(F, b, cond=False)
+            print(f"gmres-ir alone {name}: {F.gmres_info} {F.precision_info}", flush=True)
+            nsolve, nit = F.gmres_info
+            assert nsolve == 1 and 0 < nit <= 30 and F.precision_info == ("mixed", 0)
+            assert np.abs(out["mu"] - ref["mu"]).max() < 1e-10 * np.abs(ref["mu"]).max()
+            F.close()
     finally:
         os.environ.pop("VP_GMRES", None)
 

@@ -50,6 +50,13 @@ inline bool gmres_enabled()
     const char *e = std::getenv("VP_GMRES");
     return !(e && e[0] == '0');
 }
+// VP_GMRES=2 (a check of the method, not a production setting): GMRES-IR right after the first application of the
+// mixed factors, with no plain refinement before it
+inline bool gmres_only()
+{
+    const char *e = std::getenv("VP_GMRES");
+    return e && e[0] == '2';
+}
 constexpr int GMRES_M = 30, GMRES_RESTARTS = 8;
 
 inline bool graphs_enabled()

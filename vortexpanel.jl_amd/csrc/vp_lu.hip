@@ -1313,7 +1313,7 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
             if (it > 0 && strict && mode == 1 && !(hn[4 * c] < prev[c])) diverged = true;
             prev[c] = hn[4 * c];
         }
-        if (done || it >= max_iter || (diverged && !done)) break;
+        if (done || it >= max_iter || (diverged && !done) || (strict && mode == 1 && gmres_only())) break;
         apply_inverse_replay(nrhs, trans);
         axpy_inplace<<<blocks((long long)sz), 256, 0, s>>>(X, Y, (long long)sz);
         HIPCHECK2(hipGetLastError());
