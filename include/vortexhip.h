@@ -356,8 +356,10 @@ typedef struct vp_tree_stats {
     long long proxy_evals;   /* kernel evaluations at proxies */
     long long near_evals;    /* kernel evaluations at targets */
     long long interp_evals;  /* proxy values read by the interpolation */
-    double build_ms;         /* host: trees, lists and sorted copies */
-    double device_ms;        /* device: the three kernels */
+    double build_ms;         /* everything before the kernels: sort_ms + lists_ms + uploads */
+    double device_ms;        /* device: the evaluation kernels */
+    double sort_ms;          /* inputs up, Morton codes, sort and gathers on the device, sorted codes and points down */
+    double lists_ms;         /* host: trees and lists */
 } vp_tree_stats;
 int vp_blob_velocities_tree(vp_ctx *ctx, int nb, const double *bx, const double *alpha, const double *delta, int np,
                             const double *xs, const vp_tree_options *opts, double *u, vp_tree_stats *stats);

@@ -18,8 +18,9 @@
 // into exactly one of: a far pair (T with more targets than proxies, interpolated) or the direct sums of T's leaves
 // (each leaf's near sources are the complement of its ancestors' far sources).  Device: tree_proxy evaluates the far
 // pairs at the proxies (a block per work item of about 2k sources of one target cluster, one proxy per lane),
-// tree_proxy_reduce adds each cluster's items in order, and tree_targets takes one target leaf per block of four
-// waves: its near sources directly, then the interpolants of its far ancestors.
+// tree_proxy_reduce adds each cluster's items in order, tree_near sums a target leaf's near sources directly in work
+// items of at most 4k sources (a block of four waves each), and tree_finish takes one target leaf per block: its
+// items' sums in order, then the interpolants of its far ancestors.
 // Results do not depend on scheduling (no atomics; fixed summation orders).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -46,6 +47,7 @@ namespace {
 constexpr int TREE_LEAF = 64;              // points per leaf: a target leaf is one wave
 constexpr int TREE_MAX_DEGREE = 7;         // (n+1)^3 <= 512 proxies, one per lane of a proxy block
 constexpr long long ITEM_SOURCES = 2048;   // sources per proxy work item
+constexpr int NEAR_ITEM = 4096;            // near sources per near work item
 constexpr double BOX_MIN_SIDE = 1e-2;      // a box side is at least this fraction of its longest side
 
 #define TCHECK(x)                                                                                      \
@@ -259,8 +261,28 @@ struct Lists {
     // per target leaf: its points, near source ranges (CSR) and far ancestors (CSR)
     std::vector<int> leaf_a, leaf_b, near_off, anc_off, anc;
     std::vector<Range> near_rng;
+    // each range's start within its group's concatenated sources (an item's, a leaf's near ones), and each group's
+    // total: the kernels stage a group's sources in full tiles across its ranges
+    std::vector<int> item_vst, item_tot, near_vst, near_tot;
+    // near work items of at most NEAR_ITEM sources: (leaf, first range, v0, v1), and each leaf's items (CSR)
+    std::vector<int> near_item, leaf_item_off;
     long long far_pairs = 0, near_pairs = 0, proxy_evals = 0, near_evals = 0, interp_evals = 0;
 };
+
+void virtual_starts(const std::vector<int> &off, const std::vector<Range> &rng, std::vector<int> &vst,
+                    std::vector<int> &tot)
+{
+    vst.assign(rng.size(), 0);
+    tot.assign(off.empty() ? 0 : off.size() - 1, 0);
+    for (size_t g = 0; g + 1 < off.size(); g++) {
+        int at = 0;
+        for (int r = off[g]; r < off[g + 1]; r++) {
+            vst[r] = at;
+            at += rng[r].b - rng[r].a;
+        }
+        tot[g] = at;
+    }
+}
 
 void merge_ranges(std::vector<Range> &r)  // sorted by start, touching ranges joined
 {
@@ -465,6 +487,17 @@ Lists build_lists(const Tree &tt, const Tree &st, double theta, int degree)
         if (k >= 0) path.pop_back();
     };
     if (nleaf) visit(visit, 0, std::vector<Range>{});
+    virtual_starts(L.item_off, L.item_rng, L.item_vst, L.item_tot);
+    virtual_starts(L.near_off, L.near_rng, L.near_vst, L.near_tot);
+    L.leaf_item_off.push_back(0);
+    for (int l = 0; l < nleaf; l++) {
+        int r = L.near_off[l];
+        for (int v0 = 0; v0 < L.near_tot[l]; v0 += NEAR_ITEM) {
+            while (r + 1 < L.near_off[l + 1] && L.near_vst[r + 1] <= v0) r++;
+            L.near_item.insert(L.near_item.end(), {l, r, v0, std::min(v0 + NEAR_ITEM, L.near_tot[l])});
+        }
+        L.leaf_item_off.push_back((int)L.near_item.size() / 4);
+    }
     return L;
 }
 
@@ -478,14 +511,31 @@ template <int N1> __device__ __forceinline__ double cheb_w(int j)
     return (j == 0 || j == N1 - 1) ? 0.5 * s : s;
 }
 
+// Stages the sources at positions [base, base + nk) of a group's concatenated ranges r0..r1 (starts vst) into S, one
+// per thread; rb, the range holding base, only moves forward and is the same on every thread
+__device__ __forceinline__ void stage_sources(const double *__restrict__ src, const Range *__restrict__ rng,
+                                              const int *__restrict__ vst, int r1, int &rb, int base, int nk,
+                                              double (*S)[7])
+{
+    while (rb + 1 < r1 && vst[rb + 1] <= base) rb++;
+    if ((int)threadIdx.x < nk) {
+        const int v = base + (int)threadIdx.x;
+        int r = rb;
+        while (r + 1 < r1 && vst[r + 1] <= v) r++;
+        const size_t q = 7 * (size_t)(rng[r].a + (v - vst[r]));
+        for (int c = 0; c < 7; c++) S[threadIdx.x][c] = src[q + c];
+    }
+}
+
 // Threads of a proxy block: one proxy per lane, whole waves
 template <int N1> constexpr int proxy_block() { return ((N1 * N1 * N1 + 63) / 64) * 64; }
 
-// One work item (sources of one far cluster) at that cluster's proxies: partial[item][proxy][3].  The sources pass
-// through LDS in tiles of one per thread (loaded coalesced, read by every lane as a broadcast).
+// One work item (sources of one far cluster) at that cluster's proxies: partial[item][proxy][3].  The item's sources
+// pass through LDS in full tiles of one per thread across its ranges, read by every lane as a broadcast.
 template <int N1>
 __global__ __launch_bounds__(512) void tree_proxy(const double *__restrict__ src, const int *__restrict__ item_node,
                                                   const int *__restrict__ item_off, const Range *__restrict__ item_rng,
+                                                  const int *__restrict__ item_vst, const int *__restrict__ item_tot,
                                                   const double *__restrict__ box, double *__restrict__ partial)
 {
     constexpr int P = N1 * N1 * N1, BLK = proxy_block<N1>();
@@ -499,18 +549,16 @@ __global__ __launch_bounds__(512) void tree_proxy(const double *__restrict__ src
                               bx[1] + 0.5 * (bx[4] - bx[1]) * (1.0 + cheb_pt<N1>(j)),
                               bx[2] + 0.5 * (bx[5] - bx[2]) * (1.0 + cheb_pt<N1>(l)));
     V3<double> acc = mk3(0.0, 0.0, 0.0);
-    for (int r = item_off[it]; r < item_off[it + 1]; r++) {
-        const Range R = item_rng[r];
-        for (int base = R.a; base < R.b; base += BLK) {
-            const int nk = min(BLK, R.b - base);
-            __syncthreads();
-            if ((int)threadIdx.x < nk)
-                for (int c = 0; c < 7; c++) S[threadIdx.x][c] = src[7 * (size_t)(base + threadIdx.x) + c];
-            __syncthreads();
+    const int r1 = item_off[it + 1], tot = item_tot[it];
+    int rb = item_off[it];
+    for (int base = 0; base < tot; base += BLK) {
+        const int nk = min(BLK, tot - base);
+        __syncthreads();
+        stage_sources(src, item_rng, item_vst, r1, rb, base, nk, S);
+        __syncthreads();
 #pragma unroll 4
-            for (int s = 0; s < nk; s++)
-                acc = acc + blob_vel(px - mk3(S[s][0], S[s][1], S[s][2]), mk3(S[s][3], S[s][4], S[s][5]), S[s][6]);
-        }
+        for (int s = 0; s < nk; s++)
+            acc = acc + blob_vel(px - mk3(S[s][0], S[s][1], S[s][2]), mk3(S[s][3], S[s][4], S[s][5]), S[s][6]);
     }
     if ((int)threadIdx.x < P) {
         double *o = partial + 3 * ((size_t)it * P + threadIdx.x);
@@ -562,19 +610,61 @@ template <int N1> __device__ __forceinline__ void bary(double t, double *w)
     }
 }
 
-// One target leaf per block of four waves, the leaf's targets on the lanes of every wave: the near sources pass
-// through LDS in tiles of 256, wave w taking every fourth source of a tile; the far ancestors' interpolants are
-// shared out the same way; the four partial sums are added in wave order.  u in input order.
+// Near sums of one work item: a target leaf's targets on the lanes of each of a block's four waves, against at most
+// NEAR_ITEM of the leaf's near sources (positions [v0, v1) of its concatenated ranges, starting in range rb).  The
+// sources pass through LDS in full tiles of 256, wave w taking every fourth; the four waves' sums are added in wave
+// order into partial[item][lane].
+__global__ __launch_bounds__(256) void tree_near(const double *__restrict__ src, const double *__restrict__ tx,
+                                                 const int *__restrict__ leaf_a, const int *__restrict__ leaf_b,
+                                                 const int4 *__restrict__ items, const int *__restrict__ near_off,
+                                                 const Range *__restrict__ near_rng, const int *__restrict__ near_vst,
+                                                 double *__restrict__ partial)
+{
+    __shared__ double S[256][7];
+    __shared__ double red[3][64][3];
+    const int4 itm = items[blockIdx.x];  // leaf, first range, v0, v1
+    const int leaf = itm.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int a = leaf_a[leaf], b = leaf_b[leaf];
+    const V3<double> x = ld3(tx + 3 * (size_t)min(a + lane, b - 1));
+    V3<double> acc = mk3(0.0, 0.0, 0.0);
+    const int r1 = near_off[leaf + 1];
+    int rb = itm.y;
+    for (int base = itm.z; base < itm.w; base += 256) {
+        const int nk = min(256, itm.w - base);
+        __syncthreads();
+        stage_sources(src, near_rng, near_vst, r1, rb, base, nk, S);
+        __syncthreads();
+#pragma unroll 4
+        for (int s = w; s < nk; s += 4)
+            acc = acc + blob_vel(x - mk3(S[s][0], S[s][1], S[s][2]), mk3(S[s][3], S[s][4], S[s][5]), S[s][6]);
+    }
+    if (w > 0) {
+        red[w - 1][lane][0] = acc.x;
+        red[w - 1][lane][1] = acc.y;
+        red[w - 1][lane][2] = acc.z;
+    }
+    __syncthreads();
+    if (w == 0) {
+        for (int v = 0; v < 3; v++) acc = acc + mk3(red[v][lane][0], red[v][lane][1], red[v][lane][2]);
+        double *o = partial + 3 * ((size_t)blockIdx.x * 64 + lane);
+        o[0] = acc.x;
+        o[1] = acc.y;
+        o[2] = acc.z;
+    }
+}
+
+// One target leaf per block of four waves: its near items' partial sums in item order, then the interpolants of its
+// far ancestors (shared out over the waves, their sums added in wave order); u in input order
 template <int N1>
-__global__ __launch_bounds__(256) void tree_targets(const double *__restrict__ src, const double *__restrict__ tx,
-                                                    const int *__restrict__ leaf_a, const int *__restrict__ leaf_b,
-                                                    const int *__restrict__ near_off, const Range *__restrict__ near_rng,
-                                                    const int *__restrict__ anc_off, const int *__restrict__ anc,
-                                                    const double *__restrict__ box, const double *__restrict__ proxy_u,
-                                                    const int *__restrict__ perm, double *__restrict__ u)
+__global__ __launch_bounds__(256) void tree_finish(const double *__restrict__ tx, const int *__restrict__ leaf_a,
+                                                   const int *__restrict__ leaf_b, const int *__restrict__ leaf_item_off,
+                                                   const double *__restrict__ partial, const int *__restrict__ anc_off,
+                                                   const int *__restrict__ anc, const double *__restrict__ box,
+                                                   const double *__restrict__ proxy_u, const int *__restrict__ perm,
+                                                   double *__restrict__ u)
 {
     constexpr int P = N1 * N1 * N1;
-    __shared__ double S[256][7];
     __shared__ double red[3][64][3];
     const int leaf = blockIdx.x;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -582,19 +672,9 @@ __global__ __launch_bounds__(256) void tree_targets(const double *__restrict__ s
     const int i = min(a + lane, b - 1);
     const V3<double> x = ld3(tx + 3 * (size_t)i);
     V3<double> acc = mk3(0.0, 0.0, 0.0);
-    for (int r = near_off[leaf]; r < near_off[leaf + 1]; r++) {
-        const Range R = near_rng[r];
-        for (int base = R.a; base < R.b; base += 256) {
-            const int nk = min(256, R.b - base);
-            __syncthreads();
-            if ((int)threadIdx.x < nk)
-                for (int c = 0; c < 7; c++) S[threadIdx.x][c] = src[7 * (size_t)(base + threadIdx.x) + c];
-            __syncthreads();
-#pragma unroll 4
-            for (int s = w; s < nk; s += 4)
-                acc = acc + blob_vel(x - mk3(S[s][0], S[s][1], S[s][2]), mk3(S[s][3], S[s][4], S[s][5]), S[s][6]);
-        }
-    }
+    if (w == 0)
+        for (int it = leaf_item_off[leaf]; it < leaf_item_off[leaf + 1]; it++)
+            acc = acc + ld3(partial + 3 * ((size_t)it * 64 + lane));
     for (int r = anc_off[leaf] + w; r < anc_off[leaf + 1]; r += 4) {
         const int k = anc[r];
         const double *bx = box + 6 * (size_t)k;
@@ -695,6 +775,11 @@ struct DevLists {
     const int *item_node = nullptr, *item_off = nullptr, *far_item_off = nullptr;
     const Range *item_rng = nullptr, *near_rng = nullptr;
     const int *leaf_a = nullptr, *leaf_b = nullptr, *near_off = nullptr, *anc_off = nullptr, *anc = nullptr;
+    const int *item_vst = nullptr, *item_tot = nullptr, *near_vst = nullptr, *near_tot = nullptr;
+    const int4 *near_item = nullptr;
+    const int *leaf_item_off = nullptr;
+    double *near_part = nullptr;
+    int nnear = 0;
     int nfar = 0, nitem = 0, nleaf = 0, P = 0;
 };
 
@@ -718,6 +803,14 @@ DevLists upload_lists(hipStream_t st, const Lists &L, std::vector<void *> &owned
     d.near_rng = tput(L.near_rng, st, owned);
     d.anc_off = tput(L.anc_off, st, owned);
     d.anc = tput(L.anc, st, owned);
+    d.item_vst = tput(L.item_vst, st, owned);
+    d.item_tot = tput(L.item_tot, st, owned);
+    d.near_vst = tput(L.near_vst, st, owned);
+    d.near_tot = tput(L.near_tot, st, owned);
+    d.nnear = (int)L.near_item.size() / 4;
+    d.near_item = reinterpret_cast<const int4 *>(tput(L.near_item, st, owned));
+    d.leaf_item_off = tput(L.leaf_item_off, st, owned);
+    d.near_part = tnew<double>((size_t)d.nnear * 64 * 3, owned);
     return d;
 }
 
@@ -726,16 +819,21 @@ void tree_launch(hipStream_t st, const DevLists &d, const double *d_src, const d
                  double *d_u)
 {
     if (d.nitem) {
-        tree_proxy<N1><<<d.nitem, proxy_block<N1>(), 0, st>>>(d_src, d.item_node, d.item_off, d.item_rng, d.box,
-                                                               d.part);
+        tree_proxy<N1><<<d.nitem, proxy_block<N1>(), 0, st>>>(d_src, d.item_node, d.item_off, d.item_rng, d.item_vst,
+                                                               d.item_tot, d.box, d.part);
         TCHECK(hipGetLastError());
         const long long nr = (long long)d.nfar * d.P;
         tree_proxy_reduce<<<(unsigned)((nr + 255) / 256), 256, 0, st>>>(d.part, d.far_item_off, d.P, d.nfar, d.proxy);
         TCHECK(hipGetLastError());
     }
+    if (d.nnear) {
+        tree_near<<<d.nnear, 256, 0, st>>>(d_src, d_tx, d.leaf_a, d.leaf_b, d.near_item, d.near_off, d.near_rng,
+                                            d.near_vst, d.near_part);
+        TCHECK(hipGetLastError());
+    }
     if (d.nleaf) {
-        tree_targets<N1><<<d.nleaf, 256, 0, st>>>(d_src, d_tx, d.leaf_a, d.leaf_b, d.near_off, d.near_rng, d.anc_off,
-                                                  d.anc, d.box, d.proxy, d_perm, d_u);
+        tree_finish<N1><<<d.nleaf, 256, 0, st>>>(d_tx, d.leaf_a, d.leaf_b, d.leaf_item_off, d.near_part, d.anc_off,
+                                                 d.anc, d.box, d.proxy, d_perm, d_u);
         TCHECK(hipGetLastError());
     }
 }
@@ -840,14 +938,18 @@ void blob_velocities_tree_host(Ctx *c, int nb, const double *bx, const double *b
     TCHECK(hipMemcpyAsync(hkt.data(), d_kout + ns, sizeof(uint64_t) * np, hipMemcpyDeviceToHost, st));
     TCHECK(hipMemcpyAsync(hpt.data(), d_tx, sizeof(double) * 3 * np, hipMemcpyDeviceToHost, st));
     TCHECK(hipStreamSynchronize(st));
+    const auto t1 = std::chrono::steady_clock::now();
     Tree stree, ttree;
     run_both([&] { stree = tree_from_sorted(hks.data(), hps.data(), nb); },
              [&] { ttree = tree_from_sorted(hkt.data(), hpt.data(), np); });
     const Lists L = build_lists(ttree, stree, o.theta, o.degree);
-    const double build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    const auto t2 = std::chrono::steady_clock::now();
     const int *d_tperm = d_perm + ns;
     double *d_u = tnew<double>((size_t)np * 3, ow.v);
     const DevLists d = upload_lists(st, L, ow.v);
+    TCHECK(hipStreamSynchronize(st));
+    const auto t3 = std::chrono::steady_clock::now();
+    auto ms = [](auto p, auto q) { return std::chrono::duration<double, std::milli>(q - p).count(); };
     hipEvent_t e0, e1;
     TCHECK(hipEventCreate(&e0));
     TCHECK(hipEventCreate(&e1));
@@ -864,8 +966,8 @@ void blob_velocities_tree_host(Ctx *c, int nb, const double *bx, const double *b
     TCHECK(hipEventRecord(e1, st));
     TCHECK(hipMemcpyAsync(u, d_u, sizeof(double) * (size_t)np * 3, hipMemcpyDeviceToHost, st));
     TCHECK(hipStreamSynchronize(st));
-    float ms = 0.f;
-    TCHECK(hipEventElapsedTime(&ms, e0, e1));
+    float kms = 0.f;
+    TCHECK(hipEventElapsedTime(&kms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (stats) {
@@ -875,8 +977,10 @@ void blob_velocities_tree_host(Ctx *c, int nb, const double *bx, const double *b
         stats->proxy_evals = L.proxy_evals;
         stats->near_evals = L.near_evals;
         stats->interp_evals = L.interp_evals;
-        stats->build_ms = build_ms;
-        stats->device_ms = ms;
+        stats->build_ms = ms(t0, t3);
+        stats->device_ms = kms;
+        stats->sort_ms = ms(t0, t1);
+        stats->lists_ms = ms(t1, t2);
     }
 }
 

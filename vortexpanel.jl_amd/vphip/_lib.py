@@ -73,7 +73,7 @@ class TreeStatsC(C.Structure):
     """vp_tree_stats"""
     _fields_ = [("far_clusters", C.c_longlong), ("far_pairs", C.c_longlong), ("near_pairs", C.c_longlong),
                 ("proxy_evals", C.c_longlong), ("near_evals", C.c_longlong), ("interp_evals", C.c_longlong),
-                ("build_ms", C.c_double), ("device_ms", C.c_double)]
+                ("build_ms", C.c_double), ("device_ms", C.c_double), ("sort_ms", C.c_double), ("lists_ms", C.c_double)]
 
 
 _lib = None
