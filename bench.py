@@ -348,10 +348,10 @@ def cpu_mixed_leg(threads, gpu, target_s=12.0):
 
 
 def blob_tree_bench(sizes=(84_000, 500_000, 1_000_000), theta=0.5, degree=5):
-    """The blob sums by the treecode against direct sums, both FP64: the benchmark's blob cloud (spread 3 x 3 x 0.3,
-    strengths 1e-3, cores 0.005 to 0.02) summed at every blob, the wall time of each (host arrays in and out), the
-    treecode's host build and device times, its kernel evaluations against the direct sum's n^2, and its largest error
-    against the direct sum relative to the largest velocity"""
+    """The blob sums by the treecode against direct sums, in FP64 and in mixed precision (FP32 kernels): the benchmark's
+    blob cloud (spread 3 x 3 x 0.3, strengths 1e-3, cores 0.005 to 0.02) summed at every blob, the wall time of each
+    (host arrays in and out), the treecode's host build and device times, its kernel evaluations against the direct
+    sum's n^2, and its largest error against the FP64 direct sum relative to the largest velocity"""
     from vphip import post
     rng = np.random.default_rng(0)
     post.blob_velocities_tree(rng.standard_normal((3000, 3)), rng.standard_normal((3000, 3)), np.full(3000, 0.01),
@@ -365,12 +365,20 @@ def blob_tree_bench(sizes=(84_000, 500_000, 1_000_000), theta=0.5, degree=5):
         ut, st = post.blob_velocities_tree(bx, ba, bd, bx, theta=theta, degree=degree)
         tt = time.perf_counter() - t0
         t0 = time.perf_counter()
+        um, sm = post.blob_velocities_tree(bx, ba, bd, bx, theta=theta, degree=degree, precision="mixed")
+        tm = time.perf_counter() - t0
+        t0 = time.perf_counter()
         ud = post.blob_velocities(bx, ba, bd, bx)
         td = time.perf_counter() - t0
-        err = float(np.abs(ut - ud).max() / np.abs(ud).max())
+        t0 = time.perf_counter()
+        post.blob_velocities(bx, ba, bd, bx, precision="mixed")
+        tdm = time.perf_counter() - t0
+        scale = np.abs(ud).max()
         out["runs"].append(dict(n=n, direct_s=td, tree_s=tt, speedup=td / tt, build_ms=st["build_ms"],
                                 device_ms=st["device_ms"], evals_vs_direct=(st["proxy_evals"] + st["near_evals"]) / n ** 2,
-                                far_clusters=st["far_clusters"], error=err))
+                                far_clusters=st["far_clusters"], error=float(np.abs(ut - ud).max() / scale),
+                                direct_mixed_s=tdm, tree_mixed_s=tm, speedup_mixed=tdm / tm,
+                                device_mixed_ms=sm["device_ms"], error_mixed=float(np.abs(um - ud).max() / scale)))
         log(f"blob tree n={n}: {out['runs'][-1]}")
     return out
 

@@ -344,10 +344,12 @@ int vp_blob_velocities_device(vp_ctx *ctx, int nb, const double *bx, const doubl
  * docs/claude_handoff.md:177): a cluster-particle treecode.  Target clusters well separated from a source cluster
  * (box radius <= theta * distance from the box centre to the sources' box) take that cluster's field at the
  * (degree+1)^3 Chebyshev points of their box and interpolate it (barycentric Lagrange); every other pair is summed
- * directly.  Each blob keeps its own core radius and exact kernel.  FP64.  theta in (0, 1), degree 1..7. */
+ * directly.  Each blob keeps its own core radius and exact kernel.  theta in (0, 1), degree 1..7; precision 0 FP64,
+ * 1 FP32 kernels (relative to a nearby origin) with FP64 interpolation and sums. */
 typedef struct vp_tree_options {
     double theta;
     int degree;
+    int precision;
 } vp_tree_options;
 typedef struct vp_tree_stats {
     long long far_clusters;  /* target clusters that interpolate */

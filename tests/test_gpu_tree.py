@@ -63,6 +63,20 @@ def test_full_size_against_the_direct_sum(theta, degree, bound):
     assert e < bound and eo < bound
 
 
+def test_mixed_precision():
+    """FP32 kernels relative to a nearby origin (interpolation and sums FP64): 84,000 blobs within 5e-5 of the
+    oracle's FP64 sums, the direct FP32 path's tolerance, and within 1e-5 of the FP64 treecode"""
+    bx, ba, bd = _cloud(84000)
+    print("tree mixed theta 0.5 degree 5, 84000 blobs", flush=True)
+    u, st = post.blob_velocities_tree(bx, ba, bd, bx, theta=0.5, degree=5, precision="mixed")
+    u64, _ = post.blob_velocities_tree(bx, ba, bd, bx, theta=0.5, degree=5)
+    pick = np.arange(0, 84000, 280)
+    eo = np.abs(u[pick] - O.blob_velocities(bx, ba, bd, bx[pick])).max() / np.abs(u64).max()
+    e64 = _err(u, u64)
+    print(f"  oracle sample {eo:.2e}, against the FP64 treecode {e64:.2e}, device {st['device_ms']:.2f} ms", flush=True)
+    assert eo < 5e-5 and e64 < 1e-5
+
+
 def test_separate_targets_planar_and_duplicates():
     """Targets apart from the blobs (a plane of targets beside the cloud), a flat sheet of blobs (boxes of zero
     thickness), and blobs stacked 150 deep on 40 points (clusters of identical points are never interpolated; the
@@ -120,5 +134,7 @@ def test_bad_arguments_raise():
         post.blob_velocities_tree(x, x, np.ones(4), x, theta=1.2)
     with pytest.raises(ValueError):
         post.blob_velocities_tree(x, x, np.ones(4), x, degree=9)
+    with pytest.raises(ValueError):
+        post.blob_velocities_tree(x, x, np.ones(4), x, precision="fp16")
     with pytest.raises(ValueError):
         post.blob_velocities_tree(x, x, np.ones(3), x)

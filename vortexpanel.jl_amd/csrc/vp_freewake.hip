@@ -177,7 +177,7 @@ struct vp_freewake {
     P3 V;
     double dt = 0, delete_beyond = 0, span = 0;
     int advection = 0, keep_rows = 0;
-    vp_tree_options tree{0.0, 0};  // the wake blobs by the treecode when tree.theta > 0
+    vp_tree_options tree{0.0, 0, 0};  // the wake blobs by the treecode when tree.theta > 0
     Core core;
     // state (state.jl:123-143)
     std::vector<std::vector<P3>> rows;  // node rows, newest (R0) first
@@ -615,7 +615,7 @@ int vp_freewake_create(vp_ctx *ctx, const vp_surface *g, const vp_freewake_setup
         h->V = ld(su->vinf);
         h->dt = su->dt;
         h->advection = su->advection;
-        h->tree = vp_tree_options{su->tree_theta, su->tree_degree};
+        h->tree = vp_tree_options{su->tree_theta, su->tree_degree, su->field_precision};
         h->delete_beyond = su->delete_beyond;
         h->span = su->span;
         h->keep_rows = su->keep_rows;

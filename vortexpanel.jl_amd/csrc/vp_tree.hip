@@ -12,16 +12,18 @@
 // r_T <= theta * dist(c_T, box_S): every source of S is at least r_T / theta from T's centre.
 //
 // Both point sets are sorted along a Morton curve on the device (63-bit codes, hipCUB's radix sort, the points gathered
-// in sorted order); on the host (C++) each is split top-down at the highest differing code bit (binary radix tree), to
-// leaves of at most 64 points, with tight boxes (padded so that no side is thinner than 1% of the longest).  A dual
-// traversal from (root, root) sorts every (target, source) pair
-// into exactly one of: a far pair (T with more targets than proxies, interpolated) or the direct sums of T's leaves
-// (each leaf's near sources are the complement of its ancestors' far sources).  Device: tree_proxy evaluates the far
+// in sorted order); on the host (C++) each is split top-down at the highest differing code bit of the sorted codes
+// (binary radix tree), to leaves of at most 64 points, whose tight boxes the device computes; the host unites them up
+// the tree and pads every box so that no side is thinner than 1% of the longest.  A dual traversal from (root, root)
+// sorts every (target, source) pair into exactly one of: a far pair (T with more targets than proxies, interpolated)
+// or the direct sums of T's leaves (each leaf's near sources are the complement of its ancestors' far sources).  Device: tree_proxy evaluates the far
 // pairs at the proxies (a block per work item of about 2k sources of one target cluster, one proxy per lane),
 // tree_proxy_reduce adds each cluster's items in order, tree_near sums a target leaf's near sources directly in work
 // items of at most 4k sources (a block of four waves each), and tree_finish takes one target leaf per block: its
 // items' sums in order, then the interpolants of its far ancestors.
-// Results do not depend on scheduling (no atomics; fixed summation orders).
+// Results do not depend on scheduling (no atomics; fixed summation orders).  precision 1 evaluates the kernel in FP32
+// relative to a nearby origin (a far cluster's centre, a leaf's first target), as the direct FP32 sums do; the
+// interpolation and every running sum stay FP64.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -152,18 +154,22 @@ void radix_sort(std::vector<uint64_t> &key, std::vector<int> &idx)
     }
 }
 
-// Topology and boxes from the sorted codes and the points in sorted order (pos, 3 per point)
-Tree tree_from_sorted(const uint64_t *code, const double *pos, int n)
+// Topology from the sorted codes: every node's points, children and parent, and the leaves by increasing a (the
+// boxes are set by tree_boxes)
+Tree tree_topology(const uint64_t *code, int n)
 {
     Tree t;
     if (n <= 0) return t;
     t.nodes.push_back(TNode{0, n, -1, -1, -1, {}, {}, 0.0, false});
     std::vector<int> stack{0};
-    while (!stack.empty()) {
+    while (!stack.empty()) {  // depth first, the lower half first, so leaves come out by increasing a
         const int id = stack.back();
         stack.pop_back();
         const int a = t.nodes[id].a, b = t.nodes[id].b;
-        if (b - a <= TREE_LEAF) continue;
+        if (b - a <= TREE_LEAF) {
+            t.leaves.push_back(id);
+            continue;
+        }
         int s;
         const uint64_t ca = code[a], cb = code[b - 1];
         if (ca == cb) {
@@ -181,26 +187,30 @@ Tree tree_from_sorted(const uint64_t *code, const double *pos, int n)
         stack.push_back(c0 + 1);
         stack.push_back(c0);
     }
-    // tight boxes from the children up (children are created after their parent)
-    for (int id = (int)t.nodes.size() - 1; id >= 0; id--) {
-        TNode &q = t.nodes[id];
-        if (q.c0 < 0) {
-            for (int d = 0; d < 3; d++) q.lo[d] = q.hi[d] = pos[3 * (size_t)q.a + d];
-            for (int i = q.a + 1; i < q.b; i++)
-                for (int d = 0; d < 3; d++) {
-                    q.lo[d] = std::min(q.lo[d], pos[3 * (size_t)i + d]);
-                    q.hi[d] = std::max(q.hi[d], pos[3 * (size_t)i + d]);
-                }
-            t.leaves.push_back(id);
-        } else {
-            const TNode &u = t.nodes[q.c0], &v = t.nodes[q.c1];
-            for (int d = 0; d < 3; d++) {  // the children's tight boxes (before their padding)
-                q.lo[d] = std::min(u.lo[d], v.lo[d]);
-                q.hi[d] = std::max(u.hi[d], v.hi[d]);
-            }
+    return t;
+}
+
+// Boxes from the leaves' tight boxes (6 per leaf in t.leaves' order: lo, hi): every other node's from its children,
+// then every box padded to a minimum side, and its radius.  (Every box holds its own points; a padded child box may
+// stick out of its padded parent, which nothing needs.)
+void tree_boxes(Tree &t, const double *leaf_box)
+{
+    for (size_t l = 0; l < t.leaves.size(); l++) {
+        TNode &q = t.nodes[t.leaves[l]];
+        for (int d = 0; d < 3; d++) {
+            q.lo[d] = leaf_box[6 * l + d];
+            q.hi[d] = leaf_box[6 * l + 3 + d];
         }
     }
-    // pad every box to a minimum side, and its radius
+    for (int id = (int)t.nodes.size() - 1; id >= 0; id--) {  // children are created after their parent
+        TNode &q = t.nodes[id];
+        if (q.c0 < 0) continue;
+        const TNode &u = t.nodes[q.c0], &v = t.nodes[q.c1];
+        for (int d = 0; d < 3; d++) {
+            q.lo[d] = std::min(u.lo[d], v.lo[d]);
+            q.hi[d] = std::max(u.hi[d], v.hi[d]);
+        }
+    }
     for (TNode &q : t.nodes) {
         double hw[3], m = 0.0;
         for (int d = 0; d < 3; d++) m = std::max(m, hw[d] = 0.5 * (q.hi[d] - q.lo[d]));
@@ -213,9 +223,23 @@ Tree tree_from_sorted(const uint64_t *code, const double *pos, int n)
         }
         q.rad = std::sqrt(hw[0] * hw[0] + hw[1] * hw[1] + hw[2] * hw[2]);
     }
-    // (every box holds its own points; a padded child box may stick out of its padded parent, which nothing needs)
-    std::sort(t.leaves.begin(), t.leaves.end(), [&](int p, int q) { return t.nodes[p].a < t.nodes[q].a; });
-    return t;
+}
+
+// The leaves' tight boxes from the points in sorted order (pos, 3 per point), 6 per leaf
+std::vector<double> leaf_boxes_host(const Tree &t, const double *pos)
+{
+    std::vector<double> box(6 * t.leaves.size());
+    for (size_t l = 0; l < t.leaves.size(); l++) {
+        const TNode &q = t.nodes[t.leaves[l]];
+        double *o = &box[6 * l];
+        for (int d = 0; d < 3; d++) o[d] = o[3 + d] = pos[3 * (size_t)q.a + d];
+        for (int i = q.a + 1; i < q.b; i++)
+            for (int d = 0; d < 3; d++) {
+                o[d] = std::min(o[d], pos[3 * (size_t)i + d]);
+                o[3 + d] = std::max(o[3 + d], pos[3 * (size_t)i + d]);
+            }
+    }
+    return box;
 }
 
 // The same tree sorted on the host (the coverage check, which runs without a device)
@@ -233,7 +257,9 @@ Tree build_tree(const double *x, int n)
     std::vector<double> pos((size_t)n * 3);
     for (int i = 0; i < n; i++)
         for (int d = 0; d < 3; d++) pos[3 * (size_t)i + d] = x[3 * (size_t)perm[i] + d];
-    return tree_from_sorted(code.data(), pos.data(), n);
+    Tree t = tree_topology(code.data(), n);
+    tree_boxes(t, leaf_boxes_host(t, pos.data()).data());
+    return t;
 }
 
 double box_dist(const double *c, const TNode &s)  // distance from the point c to the box of s
@@ -512,10 +538,12 @@ template <int N1> __device__ __forceinline__ double cheb_w(int j)
 }
 
 // Stages the sources at positions [base, base + nk) of a group's concatenated ranges r0..r1 (starts vst) into S, one
-// per thread; rb, the range holding base, only moves forward and is the same on every thread
+// per thread, their positions taken relative to o (formed in FP64); rb, the range holding base, only moves forward
+// and is the same on every thread
+template <typename T>
 __device__ __forceinline__ void stage_sources(const double *__restrict__ src, const Range *__restrict__ rng,
                                               const int *__restrict__ vst, int r1, int &rb, int base, int nk,
-                                              double (*S)[7])
+                                              T (*S)[7], V3<double> o)
 {
     while (rb + 1 < r1 && vst[rb + 1] <= base) rb++;
     if ((int)threadIdx.x < nk) {
@@ -523,7 +551,10 @@ __device__ __forceinline__ void stage_sources(const double *__restrict__ src, co
         int r = rb;
         while (r + 1 < r1 && vst[r + 1] <= v) r++;
         const size_t q = 7 * (size_t)(rng[r].a + (v - vst[r]));
-        for (int c = 0; c < 7; c++) S[threadIdx.x][c] = src[q + c];
+        S[threadIdx.x][0] = (T)(src[q] - o.x);
+        S[threadIdx.x][1] = (T)(src[q + 1] - o.y);
+        S[threadIdx.x][2] = (T)(src[q + 2] - o.z);
+        for (int c = 3; c < 7; c++) S[threadIdx.x][c] = (T)src[q + c];
     }
 }
 
@@ -531,40 +562,46 @@ __device__ __forceinline__ void stage_sources(const double *__restrict__ src, co
 template <int N1> constexpr int proxy_block() { return ((N1 * N1 * N1 + 63) / 64) * 64; }
 
 // One work item (sources of one far cluster) at that cluster's proxies: partial[item][proxy][3].  The item's sources
-// pass through LDS in full tiles of one per thread across its ranges, read by every lane as a broadcast.
-template <int N1>
+// pass through LDS in full tiles of one per thread across its ranges, read by every lane as a broadcast.  T = float
+// evaluates relative to the cluster's box centre and sums a tile in FP32, carrying the running sum in FP64; T = double
+// evaluates in absolute coordinates.
+template <int N1, typename T>
 __global__ __launch_bounds__(512) void tree_proxy(const double *__restrict__ src, const int *__restrict__ item_node,
                                                   const int *__restrict__ item_off, const Range *__restrict__ item_rng,
                                                   const int *__restrict__ item_vst, const int *__restrict__ item_tot,
                                                   const double *__restrict__ box, double *__restrict__ partial)
 {
     constexpr int P = N1 * N1 * N1, BLK = proxy_block<N1>();
-    __shared__ double S[BLK][7];
+    __shared__ T S[BLK][7];
     const int it = blockIdx.x;
     const int k = item_node[it];
     const double *bx = box + 6 * (size_t)k;
+    const V3<double> o = sizeof(T) == 4 ? mk3(0.5 * (bx[0] + bx[3]), 0.5 * (bx[1] + bx[4]), 0.5 * (bx[2] + bx[5]))
+                                        : mk3(0.0, 0.0, 0.0);
     const int p = min((int)threadIdx.x, P - 1);
     const int i = p / (N1 * N1), j = (p / N1) % N1, l = p % N1;
-    const V3<double> px = mk3(bx[0] + 0.5 * (bx[3] - bx[0]) * (1.0 + cheb_pt<N1>(i)),
-                              bx[1] + 0.5 * (bx[4] - bx[1]) * (1.0 + cheb_pt<N1>(j)),
-                              bx[2] + 0.5 * (bx[5] - bx[2]) * (1.0 + cheb_pt<N1>(l)));
+    const V3<T> px = mk3((T)(bx[0] + 0.5 * (bx[3] - bx[0]) * (1.0 + cheb_pt<N1>(i)) - o.x),
+                         (T)(bx[1] + 0.5 * (bx[4] - bx[1]) * (1.0 + cheb_pt<N1>(j)) - o.y),
+                         (T)(bx[2] + 0.5 * (bx[5] - bx[2]) * (1.0 + cheb_pt<N1>(l)) - o.z));
     V3<double> acc = mk3(0.0, 0.0, 0.0);
     const int r1 = item_off[it + 1], tot = item_tot[it];
     int rb = item_off[it];
     for (int base = 0; base < tot; base += BLK) {
         const int nk = min(BLK, tot - base);
         __syncthreads();
-        stage_sources(src, item_rng, item_vst, r1, rb, base, nk, S);
+        stage_sources(src, item_rng, item_vst, r1, rb, base, nk, S, o);
         __syncthreads();
+        V3<T> a = mk3(T(0), T(0), T(0));
 #pragma unroll 4
         for (int s = 0; s < nk; s++)
-            acc = acc + blob_vel(px - mk3(S[s][0], S[s][1], S[s][2]), mk3(S[s][3], S[s][4], S[s][5]), S[s][6]);
+            a = a + blob_vel(px - mk3(S[s][0], S[s][1], S[s][2]), mk3(S[s][3], S[s][4], S[s][5]), S[s][6]);
+        acc = acc + mk3((double)a.x, (double)a.y, (double)a.z);
     }
     if ((int)threadIdx.x < P) {
-        double *o = partial + 3 * ((size_t)it * P + threadIdx.x);
-        o[0] = acc.x;
-        o[1] = acc.y;
-        o[2] = acc.z;
+        double *q = partial + 3 * ((size_t)it * P + threadIdx.x);
+        q[0] = acc.x;
+        q[1] = acc.y;
+        q[2] = acc.z;
     }
 }
 
@@ -613,31 +650,37 @@ template <int N1> __device__ __forceinline__ void bary(double t, double *w)
 // Near sums of one work item: a target leaf's targets on the lanes of each of a block's four waves, against at most
 // NEAR_ITEM of the leaf's near sources (positions [v0, v1) of its concatenated ranges, starting in range rb).  The
 // sources pass through LDS in full tiles of 256, wave w taking every fourth; the four waves' sums are added in wave
-// order into partial[item][lane].
+// order into partial[item][lane].  T = float evaluates relative to the leaf's first target (the digits of x - xp kept
+// where the kernel is largest) and sums a tile in FP32, the running sum in FP64.
+template <typename T>
 __global__ __launch_bounds__(256) void tree_near(const double *__restrict__ src, const double *__restrict__ tx,
                                                  const int *__restrict__ leaf_a, const int *__restrict__ leaf_b,
                                                  const int4 *__restrict__ items, const int *__restrict__ near_off,
                                                  const Range *__restrict__ near_rng, const int *__restrict__ near_vst,
                                                  double *__restrict__ partial)
 {
-    __shared__ double S[256][7];
+    __shared__ T S[256][7];
     __shared__ double red[3][64][3];
     const int4 itm = items[blockIdx.x];  // leaf, first range, v0, v1
     const int leaf = itm.x;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int a = leaf_a[leaf], b = leaf_b[leaf];
-    const V3<double> x = ld3(tx + 3 * (size_t)min(a + lane, b - 1));
+    const V3<double> o = sizeof(T) == 4 ? ld3(tx + 3 * (size_t)a) : mk3(0.0, 0.0, 0.0);
+    const V3<double> xd = ld3(tx + 3 * (size_t)min(a + lane, b - 1));
+    const V3<T> x = mk3((T)(xd.x - o.x), (T)(xd.y - o.y), (T)(xd.z - o.z));
     V3<double> acc = mk3(0.0, 0.0, 0.0);
     const int r1 = near_off[leaf + 1];
     int rb = itm.y;
     for (int base = itm.z; base < itm.w; base += 256) {
         const int nk = min(256, itm.w - base);
         __syncthreads();
-        stage_sources(src, near_rng, near_vst, r1, rb, base, nk, S);
+        stage_sources(src, near_rng, near_vst, r1, rb, base, nk, S, o);
         __syncthreads();
+        V3<T> t = mk3(T(0), T(0), T(0));
 #pragma unroll 4
         for (int s = w; s < nk; s += 4)
-            acc = acc + blob_vel(x - mk3(S[s][0], S[s][1], S[s][2]), mk3(S[s][3], S[s][4], S[s][5]), S[s][6]);
+            t = t + blob_vel(x - mk3(S[s][0], S[s][1], S[s][2]), mk3(S[s][3], S[s][4], S[s][5]), S[s][6]);
+        acc = acc + mk3((double)t.x, (double)t.y, (double)t.z);
     }
     if (w > 0) {
         red[w - 1][lane][0] = acc.x;
@@ -647,10 +690,10 @@ __global__ __launch_bounds__(256) void tree_near(const double *__restrict__ src,
     __syncthreads();
     if (w == 0) {
         for (int v = 0; v < 3; v++) acc = acc + mk3(red[v][lane][0], red[v][lane][1], red[v][lane][2]);
-        double *o = partial + 3 * ((size_t)blockIdx.x * 64 + lane);
-        o[0] = acc.x;
-        o[1] = acc.y;
-        o[2] = acc.z;
+        double *q = partial + 3 * ((size_t)blockIdx.x * 64 + lane);
+        q[0] = acc.x;
+        q[1] = acc.y;
+        q[2] = acc.z;
     }
 }
 
@@ -744,6 +787,27 @@ __global__ void tree_gather_pts(const int *__restrict__ perm, const double *__re
     for (int d = 0; d < 3; d++) out[3 * (size_t)i + d] = x[3 * (size_t)perm[i] + d];
 }
 
+// The leaves' tight boxes from the points in sorted order: a thread per leaf of at most TREE_LEAF points, 6 per leaf
+__global__ void tree_leaf_boxes(const double *__restrict__ pos, const int2 *__restrict__ leaf, int nleaf,
+                                double *__restrict__ box)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nleaf) return;
+    const int2 r = leaf[l];
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; d++) lo[d] = hi[d] = pos[3 * (size_t)r.x + d];
+    for (int i = r.x + 1; i < r.y; i++)
+        for (int d = 0; d < 3; d++) {
+            const double v = pos[3 * (size_t)i + d];
+            lo[d] = fmin(lo[d], v);
+            hi[d] = fmax(hi[d], v);
+        }
+    for (int d = 0; d < 3; d++) {
+        box[6 * (size_t)l + d] = lo[d];
+        box[6 * (size_t)l + 3 + d] = hi[d];
+    }
+}
+
 template <typename T> T *tnew(size_t n, std::vector<void *> &owned)
 {
     T *p = nullptr;
@@ -814,21 +878,21 @@ DevLists upload_lists(hipStream_t st, const Lists &L, std::vector<void *> &owned
     return d;
 }
 
-template <int N1>
+template <int N1, typename T>
 void tree_launch(hipStream_t st, const DevLists &d, const double *d_src, const double *d_tx, const int *d_perm,
                  double *d_u)
 {
     if (d.nitem) {
-        tree_proxy<N1><<<d.nitem, proxy_block<N1>(), 0, st>>>(d_src, d.item_node, d.item_off, d.item_rng, d.item_vst,
-                                                               d.item_tot, d.box, d.part);
+        tree_proxy<N1, T><<<d.nitem, proxy_block<N1>(), 0, st>>>(d_src, d.item_node, d.item_off, d.item_rng,
+                                                                  d.item_vst, d.item_tot, d.box, d.part);
         TCHECK(hipGetLastError());
         const long long nr = (long long)d.nfar * d.P;
         tree_proxy_reduce<<<(unsigned)((nr + 255) / 256), 256, 0, st>>>(d.part, d.far_item_off, d.P, d.nfar, d.proxy);
         TCHECK(hipGetLastError());
     }
     if (d.nnear) {
-        tree_near<<<d.nnear, 256, 0, st>>>(d_src, d_tx, d.leaf_a, d.leaf_b, d.near_item, d.near_off, d.near_rng,
-                                            d.near_vst, d.near_part);
+        tree_near<T><<<d.nnear, 256, 0, st>>>(d_src, d_tx, d.leaf_a, d.leaf_b, d.near_item, d.near_off, d.near_rng,
+                                               d.near_vst, d.near_part);
         TCHECK(hipGetLastError());
     }
     if (d.nleaf) {
@@ -838,8 +902,24 @@ void tree_launch(hipStream_t st, const DevLists &d, const double *d_src, const d
     }
 }
 
+template <typename T>
+void launch_degree(int degree, hipStream_t st, const DevLists &d, const double *d_src, const double *d_tx,
+                   const int *d_perm, double *d_u)
+{
+    switch (degree + 1) {
+    case 2: tree_launch<2, T>(st, d, d_src, d_tx, d_perm, d_u); break;
+    case 3: tree_launch<3, T>(st, d, d_src, d_tx, d_perm, d_u); break;
+    case 4: tree_launch<4, T>(st, d, d_src, d_tx, d_perm, d_u); break;
+    case 5: tree_launch<5, T>(st, d, d_src, d_tx, d_perm, d_u); break;
+    case 6: tree_launch<6, T>(st, d, d_src, d_tx, d_perm, d_u); break;
+    case 7: tree_launch<7, T>(st, d, d_src, d_tx, d_perm, d_u); break;
+    default: tree_launch<8, T>(st, d, d_src, d_tx, d_perm, d_u); break;
+    }
+}
+
 void check_tree_args(int nb, int np, const vp_tree_options &o)
 {
+    if (o.precision != 0 && o.precision != 1) throw std::invalid_argument("precision must be 0 (fp64) or 1 (mixed)");
     if (!(o.theta > 0.0 && o.theta < 1.0)) throw std::invalid_argument("theta must lie in (0, 1)");
     if (o.degree < 1 || o.degree > TREE_MAX_DEGREE)
         throw std::invalid_argument("degree must lie in 1.." + std::to_string(TREE_MAX_DEGREE));
@@ -928,20 +1008,34 @@ void blob_velocities_tree_host(Ctx *c, int nb, const double *bx, const double *b
                                               st));
     tree_gather_pts<<<(unsigned)((np + 255) / 256), 256, 0, st>>>(d_perm + ns, d_xs, np, d_tx);
     TCHECK(hipGetLastError());
-    // on the host: the topology and boxes from the sorted codes and positions, then the lists
+    // on the host: the topologies from the sorted codes; the leaves' boxes on the device; then the lists
     std::vector<uint64_t> hks(nb), hkt(np);
-    std::vector<double> hps((size_t)nb * 3), hpt((size_t)np * 3);
-    if (nb) {
-        TCHECK(hipMemcpyAsync(hks.data(), d_kout, sizeof(uint64_t) * nb, hipMemcpyDeviceToHost, st));
-        TCHECK(hipMemcpyAsync(hps.data(), d_spos, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, st));
-    }
+    if (nb) TCHECK(hipMemcpyAsync(hks.data(), d_kout, sizeof(uint64_t) * nb, hipMemcpyDeviceToHost, st));
     TCHECK(hipMemcpyAsync(hkt.data(), d_kout + ns, sizeof(uint64_t) * np, hipMemcpyDeviceToHost, st));
-    TCHECK(hipMemcpyAsync(hpt.data(), d_tx, sizeof(double) * 3 * np, hipMemcpyDeviceToHost, st));
     TCHECK(hipStreamSynchronize(st));
     const auto t1 = std::chrono::steady_clock::now();
     Tree stree, ttree;
-    run_both([&] { stree = tree_from_sorted(hks.data(), hps.data(), nb); },
-             [&] { ttree = tree_from_sorted(hkt.data(), hpt.data(), np); });
+    run_both([&] { stree = tree_topology(hks.data(), nb); }, [&] { ttree = tree_topology(hkt.data(), np); });
+    const size_t nls = stree.leaves.size(), nlt = ttree.leaves.size();
+    std::vector<int> hr(2 * (nls + nlt));
+    for (size_t l = 0; l < nls; l++) {
+        hr[2 * l] = stree.nodes[stree.leaves[l]].a;
+        hr[2 * l + 1] = stree.nodes[stree.leaves[l]].b;
+    }
+    for (size_t l = 0; l < nlt; l++) {
+        hr[2 * (nls + l)] = ttree.nodes[ttree.leaves[l]].a;
+        hr[2 * (nls + l) + 1] = ttree.nodes[ttree.leaves[l]].b;
+    }
+    const int2 *d_lr = reinterpret_cast<const int2 *>(tput(hr, st, ow.v));
+    double *d_lbox = tnew<double>(6 * (nls + nlt), ow.v);
+    if (nls) tree_leaf_boxes<<<(unsigned)((nls + 255) / 256), 256, 0, st>>>(d_spos, d_lr, (int)nls, d_lbox);
+    if (nlt)
+        tree_leaf_boxes<<<(unsigned)((nlt + 255) / 256), 256, 0, st>>>(d_tx, d_lr + nls, (int)nlt, d_lbox + 6 * nls);
+    TCHECK(hipGetLastError());
+    std::vector<double> hbox(6 * (nls + nlt));
+    TCHECK(hipMemcpyAsync(hbox.data(), d_lbox, sizeof(double) * hbox.size(), hipMemcpyDeviceToHost, st));
+    TCHECK(hipStreamSynchronize(st));
+    run_both([&] { tree_boxes(stree, hbox.data()); }, [&] { tree_boxes(ttree, hbox.data() + 6 * nls); });
     const Lists L = build_lists(ttree, stree, o.theta, o.degree);
     const auto t2 = std::chrono::steady_clock::now();
     const int *d_tperm = d_perm + ns;
@@ -954,15 +1048,10 @@ void blob_velocities_tree_host(Ctx *c, int nb, const double *bx, const double *b
     TCHECK(hipEventCreate(&e0));
     TCHECK(hipEventCreate(&e1));
     TCHECK(hipEventRecord(e0, st));
-    switch (o.degree + 1) {
-    case 2: tree_launch<2>(st, d, d_src, d_tx, d_tperm, d_u); break;
-    case 3: tree_launch<3>(st, d, d_src, d_tx, d_tperm, d_u); break;
-    case 4: tree_launch<4>(st, d, d_src, d_tx, d_tperm, d_u); break;
-    case 5: tree_launch<5>(st, d, d_src, d_tx, d_tperm, d_u); break;
-    case 6: tree_launch<6>(st, d, d_src, d_tx, d_tperm, d_u); break;
-    case 7: tree_launch<7>(st, d, d_src, d_tx, d_tperm, d_u); break;
-    default: tree_launch<8>(st, d, d_src, d_tx, d_tperm, d_u); break;
-    }
+    if (o.precision == 1)
+        launch_degree<float>(o.degree, st, d, d_src, d_tx, d_tperm, d_u);
+    else
+        launch_degree<double>(o.degree, st, d, d_src, d_tx, d_tperm, d_u);
     TCHECK(hipEventRecord(e1, st));
     TCHECK(hipMemcpyAsync(u, d_u, sizeof(double) * (size_t)np * 3, hipMemcpyDeviceToHost, st));
     TCHECK(hipStreamSynchronize(st));

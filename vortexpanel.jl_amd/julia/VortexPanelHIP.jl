@@ -77,6 +77,7 @@ end
 struct VPTreeOptions          # vp_tree_options
     theta::Cdouble
     degree::Cint
+    precision::Cint
 end
 
 struct VPMixedMesh            # vp_mixed_mesh
@@ -460,8 +461,8 @@ end
     blob_velocities(blobs, X; precision=:fp64, tree=nothing, ctx)
 
 FreeWake.blob_velocities on the GPU: the velocity of Rosenhead–Moore particles (`Blob`: x, α, δ) at the points X.
-With `tree = (θ, degree)` the sums are approximated by the treecode (`vp_blob_velocities_tree`, FP64; no counterpart
-in the reference): target clusters well separated from a source cluster interpolate its field from the
+With `tree = (θ, degree)` the sums are approximated by the treecode (`vp_blob_velocities_tree`, in the same
+precision; no counterpart in the reference): target clusters well separated from a source cluster interpolate its field from the
 (degree+1)^3 Chebyshev points of their box, every other pair is summed directly, each blob with its own core.
 """
 function blob_velocities(blobs::AbstractVector, X::AbstractVector{<:SVector{3}}; precision::Symbol=:fp64,
@@ -477,11 +478,11 @@ function blob_velocities(blobs::AbstractVector, X::AbstractVector{<:SVector{3}};
                 (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Cint, Ptr{Cdouble}, Cint, Ptr{Cdouble}),
                 ctx.ptr, length(blobs), bx, ba, bd, length(X), Xf, precision === :mixed ? 1 : 0, u))
         else
-            precision === :fp64 || throw(ArgumentError("the treecode sums in FP64"))
             check(ctx, ccall((:vp_blob_velocities_tree, libvortexhip), Cint,
                 (Ptr{Cvoid}, Cint, Ptr{Cdouble}, Ptr{Cdouble}, Ptr{Cdouble}, Cint, Ptr{Cdouble}, Ref{VPTreeOptions},
                  Ptr{Cdouble}, Ptr{Cvoid}),
-                ctx.ptr, length(blobs), bx, ba, bd, length(X), Xf, Ref(VPTreeOptions(tree[1], tree[2])), u, C_NULL))
+                ctx.ptr, length(blobs), bx, ba, bd, length(X), Xf,
+                Ref(VPTreeOptions(tree[1], tree[2], precision === :mixed ? 1 : 0)), u, C_NULL))
         end
     end
     return [SVector(u[1, i], u[2, i], u[3, i]) for i in axes(u, 2)]

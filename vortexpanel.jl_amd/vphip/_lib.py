@@ -66,7 +66,7 @@ class FreeWakeRecordC(C.Structure):
 
 class TreeOptionsC(C.Structure):
     """vp_tree_options"""
-    _fields_ = [("theta", C.c_double), ("degree", C.c_int)]
+    _fields_ = [("theta", C.c_double), ("degree", C.c_int), ("precision", C.c_int)]
 
 
 class TreeStatsC(C.Structure):

@@ -90,10 +90,11 @@ def _tree_stats(st):
     return {k: getattr(st, k) for k, _ in L.TreeStatsC._fields_}
 
 
-def blob_velocities_tree(bx, alpha, delta, xs, theta=0.5, degree=6, ctx: Context | None = None):
+def blob_velocities_tree(bx, alpha, delta, xs, theta=0.5, degree=6, precision="fp64", ctx: Context | None = None):
     """blob_velocities by the cluster-particle treecode (vp_blob_velocities_tree): target clusters well separated from
     a source cluster (box radius <= theta x distance) interpolate its field from the (degree+1)^3 Chebyshev points of
-    their box; every other pair is summed directly, each blob with its own core.  FP64.  Returns (u, stats)."""
+    their box; every other pair is summed directly, each blob with its own core.  precision 'mixed' evaluates the
+    kernel in FP32 (interpolation and sums FP64).  Returns (u, stats)."""
     ctx = ctx or default_context()
     bx, ba = L.f64(bx).reshape(-1, 3), L.f64(alpha).reshape(-1, 3)
     bd = L.f64(delta).reshape(-1)
@@ -101,7 +102,7 @@ def blob_velocities_tree(bx, alpha, delta, xs, theta=0.5, degree=6, ctx: Context
         raise ValueError("blob arrays disagree in length")
     X = L.f64(xs).reshape(-1, 3)
     u = np.zeros_like(X)
-    o = L.TreeOptionsC(float(theta), int(degree))
+    o = L.TreeOptionsC(float(theta), int(degree), _prec(precision))
     st = L.TreeStatsC()
     rc = ctx.lib.vp_blob_velocities_tree(ctx.h, len(bd), L.ptr(bx), L.ptr(ba), L.ptr(bd), len(X), L.ptr(X),
                                          C.byref(o), L.ptr(u), C.byref(st))
@@ -114,7 +115,7 @@ def tree_coverage(bx, xs, theta=0.5, degree=6):
     source ranges do not cover every source exactly once, stats)"""
     lib = L.load()
     bx, X = L.f64(bx).reshape(-1, 3), L.f64(xs).reshape(-1, 3)
-    o = L.TreeOptionsC(float(theta), int(degree))
+    o = L.TreeOptionsC(float(theta), int(degree), 0)
     bad, st = C.c_longlong(), L.TreeStatsC()
     rc = lib.vp_tree_coverage(len(bx), L.ptr(bx), len(X), L.ptr(X), C.byref(o), C.byref(bad), C.byref(st))
     if rc != 0:
