@@ -344,6 +344,30 @@ template <typename F, typename G> void run_both(F &&f, G &&g)
     if (ef) std::rethrow_exception(ef);
 }
 
+// f(begin, end) over [0, n) in contiguous chunks on up to 16 host threads; an exception is rethrown on this thread
+template <typename F> void parallel_for(int n, F &&f)
+{
+    const int nth = std::max(1, std::min({16, (int)std::thread::hardware_concurrency(), n / 64}));
+    if (nth == 1) {
+        if (n > 0) f(0, n);
+        return;
+    }
+    std::vector<std::exception_ptr> errs(nth);
+    std::vector<std::thread> pool;
+    auto run = [&](int t) {
+        try {
+            f((int)((long long)n * t / nth), (int)((long long)n * (t + 1) / nth));
+        } catch (...) {
+            errs[t] = std::current_exception();
+        }
+    };
+    for (int t = 1; t < nth; t++) pool.emplace_back(run, t);
+    run(0);
+    for (auto &th : pool) th.join();
+    for (auto &e : errs)
+        if (e) std::rethrow_exception(e);
+}
+
 struct FarRec {  // a far pair: target cluster (node id) and the source cluster's points
     int t;
     Range s;
@@ -442,11 +466,13 @@ Lists build_lists(const Tree &tt, const Tree &st, double theta, int degree)
     const int nfar = (int)L.far_node.size();
     L.far_item_off.push_back(0);
     L.item_off.push_back(0);
+    parallel_for(nfar, [&](int k0, int k1) {
+        for (int k = k0; k < k1; k++) merge_ranges(farl[k]);
+    });
     for (int k = 0; k < nfar; k++) {
         const TNode &T = tt.nodes[L.far_node[k]];
         for (int d = 0; d < 3; d++) L.far_box.push_back(T.lo[d]);
         for (int d = 0; d < 3; d++) L.far_box.push_back(T.hi[d]);
-        merge_ranges(farl[k]);
         long long acc = 0;
         for (Range r : farl[k]) {
             while (r.a < r.b) {
@@ -469,11 +495,14 @@ Lists build_lists(const Tree &tt, const Tree &st, double theta, int degree)
     }
     // target leaves, in order: points, far ancestors (nearest first), and the near ranges: the complement in [0, nb)
     // of the union of the ancestors' far ranges (disjoint by construction of the traversal), carried down the tree
-    // merged (a leaf's union has about as many ranges as its complement)
-    L.near_off.push_back(0);
-    L.anc_off.push_back(0);
-    std::vector<int> path;  // far clusters from the root down
-    auto visit = [&](auto &&self, int id, const std::vector<Range> &up) -> void {
+    // merged (a leaf's union has about as many ranges as its complement).  The target tree is cut into subtrees of
+    // at most a 64th of its targets, walked on host threads, their leaves joined in order.
+    struct Frag {
+        std::vector<int> leaf_a, leaf_b, anc, anc_n, near_n;
+        std::vector<Range> near_rng;
+        long long interp = 0, pairs = 0, evals = 0;
+    };
+    auto visit = [&](auto &&self, int id, const std::vector<Range> &up, std::vector<int> &path, Frag &f) -> void {
         const TNode &T = tt.nodes[id];
         const int k = far_of_node[id];
         std::vector<Range> cur;
@@ -487,43 +516,96 @@ Lists build_lists(const Tree &tt, const Tree &st, double theta, int degree)
             path.push_back(k);
         }
         if (T.c0 >= 0) {
-            self(self, T.c0, *u);
-            self(self, T.c1, *u);
+            self(self, T.c0, *u, path, f);
+            self(self, T.c1, *u, path, f);
         } else {
-            L.leaf_a.push_back(T.a);
-            L.leaf_b.push_back(T.b);
-            for (auto it = path.rbegin(); it != path.rend(); ++it) {
-                L.anc.push_back(*it);
-                L.interp_evals += (long long)L.P * (T.b - T.a);
-            }
-            L.anc_off.push_back((int)L.anc.size());
+            f.leaf_a.push_back(T.a);
+            f.leaf_b.push_back(T.b);
+            for (auto it = path.rbegin(); it != path.rend(); ++it) f.anc.push_back(*it);
+            f.anc_n.push_back((int)path.size());
+            f.interp += (long long)L.P * (T.b - T.a) * (long long)path.size();
+            const size_t r0 = f.near_rng.size();
             int at = 0;
             for (const Range &r : *u) {
-                if (r.a > at) L.near_rng.push_back(Range{at, r.a});
+                if (r.a > at) f.near_rng.push_back(Range{at, r.a});
                 at = std::max(at, r.b);
             }
-            if (at < nb) L.near_rng.push_back(Range{at, nb});
-            const int l = (int)L.leaf_a.size() - 1;
-            L.near_off.push_back((int)L.near_rng.size());
-            for (int r = L.near_off[l]; r < L.near_off[l + 1]; r++) {
-                L.near_pairs++;
-                L.near_evals += (long long)(T.b - T.a) * (L.near_rng[r].b - L.near_rng[r].a);
+            if (at < nb) f.near_rng.push_back(Range{at, nb});
+            f.near_n.push_back((int)(f.near_rng.size() - r0));
+            for (size_t r = r0; r < f.near_rng.size(); r++) {
+                f.pairs++;
+                f.evals += (long long)(T.b - T.a) * (f.near_rng[r].b - f.near_rng[r].a);
             }
         }
         if (k >= 0) path.pop_back();
     };
-    if (nleaf) visit(visit, 0, std::vector<Range>{});
+    struct Task {
+        int node;
+        std::vector<int> path;
+        std::vector<Range> uni;
+    };
+    std::vector<Task> tasks;
+    if (nleaf) {
+        const int cut = std::max(TREE_LEAF, (tt.nodes[0].b - tt.nodes[0].a) / 64);
+        std::vector<int> path;
+        auto split = [&](auto &&self, int id, const std::vector<Range> &up) -> void {
+            const TNode &T = tt.nodes[id];
+            if (T.c0 < 0 || T.b - T.a <= cut) {
+                tasks.push_back(Task{id, path, up});
+                return;
+            }
+            const int k = far_of_node[id];
+            std::vector<Range> cur = up;
+            if (k >= 0) {
+                cur.insert(cur.end(), farl[k].begin(), farl[k].end());
+                merge_ranges(cur);
+                path.push_back(k);
+            }
+            self(self, T.c0, cur);
+            self(self, T.c1, cur);
+            if (k >= 0) path.pop_back();
+        };
+        split(split, 0, std::vector<Range>{});
+    }
+    std::vector<Frag> frags(tasks.size());
+    parallel_for((int)tasks.size(), [&](int t0, int t1) {
+        for (int t = t0; t < t1; t++) {
+            std::vector<int> path = tasks[t].path;
+            visit(visit, tasks[t].node, tasks[t].uni, path, frags[t]);
+        }
+    });
+    L.near_off.push_back(0);
+    L.anc_off.push_back(0);
+    for (const Frag &f : frags) {
+        L.leaf_a.insert(L.leaf_a.end(), f.leaf_a.begin(), f.leaf_a.end());
+        L.leaf_b.insert(L.leaf_b.end(), f.leaf_b.begin(), f.leaf_b.end());
+        L.anc.insert(L.anc.end(), f.anc.begin(), f.anc.end());
+        for (int c : f.anc_n) L.anc_off.push_back(L.anc_off.back() + c);
+        L.near_rng.insert(L.near_rng.end(), f.near_rng.begin(), f.near_rng.end());
+        for (int c : f.near_n) L.near_off.push_back(L.near_off.back() + c);
+        L.interp_evals += f.interp;
+        L.near_pairs += f.pairs;
+        L.near_evals += f.evals;
+    }
     virtual_starts(L.item_off, L.item_rng, L.item_vst, L.item_tot);
     virtual_starts(L.near_off, L.near_rng, L.near_vst, L.near_tot);
-    L.leaf_item_off.push_back(0);
-    for (int l = 0; l < nleaf; l++) {
-        int r = L.near_off[l];
-        for (int v0 = 0; v0 < L.near_tot[l]; v0 += NEAR_ITEM) {
-            while (r + 1 < L.near_off[l + 1] && L.near_vst[r + 1] <= v0) r++;
-            L.near_item.insert(L.near_item.end(), {l, r, v0, std::min(v0 + NEAR_ITEM, L.near_tot[l])});
+    L.leaf_item_off.assign(nleaf + 1, 0);
+    for (int l = 0; l < nleaf; l++)
+        L.leaf_item_off[l + 1] = L.leaf_item_off[l] + (L.near_tot[l] + NEAR_ITEM - 1) / NEAR_ITEM;
+    L.near_item.resize(4 * (size_t)L.leaf_item_off[nleaf]);
+    parallel_for(nleaf, [&](int l0, int l1) {
+        for (int l = l0; l < l1; l++) {
+            int r = L.near_off[l];
+            int *o = &L.near_item[4 * (size_t)L.leaf_item_off[l]];
+            for (int v0 = 0; v0 < L.near_tot[l]; v0 += NEAR_ITEM, o += 4) {
+                while (r + 1 < L.near_off[l + 1] && L.near_vst[r + 1] <= v0) r++;
+                o[0] = l;
+                o[1] = r;
+                o[2] = v0;
+                o[3] = std::min(v0 + NEAR_ITEM, L.near_tot[l]);
+            }
         }
-        L.leaf_item_off.push_back((int)L.near_item.size() / 4);
-    }
+    });
     return L;
 }
 
