@@ -175,7 +175,8 @@ def test_free_wake_mixed(wing):
 
 def test_free_wake_tree(wing):
     """The native engine with the wake blobs' velocities by the treecode (theta 0.5, degree 6) against direct sums
-    over 30 steps: the same rows and particle counts, mu_w within 1e-7 of its largest value"""
+    over 30 steps: the same rows and particle counts, mu_w within 1e-7 of its largest value in FP64, within 1e-6 in
+    mixed precision (FP32 kernels in both the treecode and the direct sums)"""
     print("free wake 30 steps tree vs direct", flush=True)
     g, dm = wing
     dt = wk.te_panel_size(g) * 2
@@ -190,6 +191,15 @@ def test_free_wake_tree(wing):
         assert np.abs(a["mu_w"] - b["mu_w"]).max() < 1e-7 * np.abs(b["mu_w"]).max()
     with pytest.raises(ValueError):
         FW.FreeWakeSim(g, dm, V, o, tree=(1.5, 6))
+    # mixed precision: the treecode's FP32 kernels against the mixed direct sums
+    mixed = {}
+    for t in (None, (0.5, 6)):
+        s = FW.FreeWakeSim(g, dm, V, o, precision="mixed", tree=t)
+        mixed[t] = s.run(30)
+        s.close()
+    for a, b in zip(mixed[(0.5, 6)], mixed[None]):
+        assert a["particles"] == b["particles"] and a["rows"] == b["rows"]
+        assert np.abs(a["mu_w"] - b["mu_w"]).max() < 1e-6 * np.abs(b["mu_w"]).max()
 
 
 def test_free_wake_full_size():
