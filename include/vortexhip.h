@@ -302,6 +302,12 @@ int vp_factor_mode(vp_factor *f, int *mode, int *fallbacks);
  * restarted GMRES on M, preconditioned by the FP32 factors, finished without an FP64 refactorization, and the GMRES
  * iterations taken (environment VP_GMRES=0 turns it off). */
 int vp_factor_gmres(vp_factor *f, long long *solves, long long *iterations);
+/* The explicit inverse of the factors (no counterpart in the reference): whether it is in use, and the narrow
+ * applications counted before it was made.  A factorization forms it at the start of a narrow solve (one to four
+ * right-hand sides) once it has served 24 narrow applications (a refinement step each) and its padded size is at most
+ * 16,384; narrow solves then apply it by one matrix-vector product instead of the triangular sweeps.  Environment
+ * VP_LU_INVERSE=0 never forms it, 1 forms it at the first narrow solve. */
+int vp_factor_inverse(vp_factor *f, int *in_use, long long *narrow_applies);
 void vp_factor_destroy(vp_factor *f);
 /* The LU trailing update's GEMM tile shapes chosen by the autotuner (no counterpart in the reference), as JSON:
  * per precision the chosen BMxBNxBK (null before the first factorization large enough to time), the update size timed

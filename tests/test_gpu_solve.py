@@ -425,3 +425,48 @@ def test_gmres_ir_gives_up_to_the_fp64_refactorization():
     assert F.gmres_info == (0, 240) and F.precision_info == ("fp64", 1)
     assert np.abs(out["mu"] - ref["mu"]).max() < 1e-10 * np.abs(ref["mu"]).max()
     F.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_explicit_inverse_of_the_factors(mode):
+    """Narrow solves through the explicit inverse of the factors (VP_LU_INVERSE=1: formed at the first narrow
+    application) give the solves through the triangular sweeps (VP_LU_INVERSE=0) to the refinement's accuracy, for M
+    and Mᵀ, one to four right-hand sides, on the wing with its wake; by default the inverse appears at the first solve
+    after 24 narrow applications"""
+    import os
+    g = mesh.naca_wing(mesh.WingSpec(code="0012", n_around=24, n_span=8, aspect=6.0))
+    dm = dofmap.build_dofmap(g)
+    V = np.array([np.cos(np.radians(5.0)), 0.0, np.sin(np.radians(5.0))])
+    w = wk.build_wake(g, dm, V, wk.WakeOptions(length=2.0))
+    els = api.build_elements(g)
+    A, b, _ = api.assemble(g, dm, els, api.source_strengths(els, V), V, wake=w)
+    W = dofmap.wake_rows(dm, w.te)
+    rng = np.random.default_rng(5)
+    out = {}
+    try:
+        for pol in ("0", "1"):
+            os.environ["VP_LU_INVERSE"] = pol
+            F = solve.factorize_galerkin(dm, A, W=W, mode=mode)
+            R = rng.standard_normal((F.n, 3)) if pol == "0" else out["R"]
+            out["R"] = R
+            out[pol] = [F.solve_bordered(R[:, 0]), F.solve_bordered(R), F.solve_bordered_t(R[:, 1])]
+            print(f"VP_LU_INVERSE={pol} {mode}: inverse {F.inverse_info}", flush=True)
+            assert F.inverse_info[0] == (pol == "1")
+            F.close()
+        for a, c in zip(out["1"], out["0"]):
+            assert np.abs(a - c).max() < 1e-11 * np.abs(c).max()
+        os.environ.pop("VP_LU_INVERSE")
+        F = solve.factorize_galerkin(dm, A, W=W, mode=mode)
+        k = 0
+        while F.inverse_info[1] < 24:
+            F.solve_bordered(out["R"][:, 1 + k % 2])
+            k += 1
+        assert not F.inverse_info[0] and k <= 24
+        x = F.solve_bordered(out["R"][:, 0])  # the first solve after 24 applications forms it
+        used, applies = F.inverse_info
+        print(f"  default: inverse {used} after {applies} narrow applications", flush=True)
+        assert used and 24 <= applies <= 40
+        assert np.abs(x - out["0"][0]).max() < 1e-11 * np.abs(out["0"][0]).max()
+        F.close()
+    finally:
+        os.environ.pop("VP_LU_INVERSE", None)

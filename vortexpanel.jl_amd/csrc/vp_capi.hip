@@ -476,6 +476,14 @@ int vp_factor_gmres(vp_factor *f, long long *solves, long long *iterations)
     return VP_OK;
 }
 
+int vp_factor_inverse(vp_factor *f, int *in_use, long long *narrow_applies)
+{
+    if (!f) return VP_E_ARGUMENT;
+    if (in_use) *in_use = f->f.inv_ready ? 1 : 0;
+    if (narrow_applies) *narrow_applies = f->f.narrow_applies;
+    return VP_OK;
+}
+
 void vp_factor_destroy(vp_factor *f)
 {
     if (!f) return;

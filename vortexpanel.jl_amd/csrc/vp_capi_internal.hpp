@@ -58,6 +58,14 @@ inline bool gmres_only()
     return e && e[0] == '2';
 }
 constexpr int GMRES_M = 30, GMRES_RESTARTS = 8;
+// VP_LU_INVERSE: 0 never forms the explicit inverse of the factors, 1 forms it at the first narrow apply (tests), unset
+// after LU_INVERSE_AFTER narrow applies
+inline int lu_inverse_policy()
+{
+    const char *e = std::getenv("VP_LU_INVERSE");
+    return e ? (e[0] == '0' ? 0 : 1) : -1;
+}
+constexpr int LU_INVERSE_AFTER = 24, LU_INVERSE_MAX = 16384;
 
 inline bool graphs_enabled()
 {
@@ -165,6 +173,15 @@ struct Factor {
     double *gV = nullptr, *gx = nullptr, *gw = nullptr, *gb = nullptr, *gh = nullptr;
     long long gmres_solves = 0, gmres_iters = 0;
     bool gmres_column(int c, int nrhs, bool trans, double tolf, int max_restarts);
+    // The explicit inverse of the LU factors, G^-1 (npad x npad row-major, the factors' precision), made at the start of
+    // a narrow solve once the factorization has served LU_INVERSE_AFTER narrow applications (and npad <=
+    // LU_INVERSE_MAX): a narrow apply is then one GEMV between the butterflies instead of the panel sweeps, whose
+    // launches bound it.  Zbuf: its output.
+    void *ginv = nullptr;
+    double *Zbuf = nullptr;
+    bool inv_ready = false;
+    long long narrow_applies = 0, inverse_builds = 0;
+    void build_inverse();
     double anorm1 = 0.0, anorm_inf = -1.0, cond = -1.0;
     std::vector<double> se;
     std::vector<void *> owned;

@@ -703,6 +703,95 @@ __global__ void gemv_t_reduce(const double *__restrict__ part, int nsplit, long 
     Y[i] -= s;
 }
 
+// Products with the explicit inverse of the factors (TM = the factors' precision, FP64 sums): Y = G^-1 X with one
+// wavefront per row, and Y = G^-T X in two passes (slices of KSPLIT rows, then the slices added in order)
+template <typename TM, int NR>
+__global__ __launch_bounds__(256) void gemv_set(const TM *__restrict__ M, int n, const double *__restrict__ X,
+                                                double *__restrict__ Y)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const TM *m = M + (long long)row * n;
+    double acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; c++) acc[c] = 0.0;
+#pragma unroll 4
+    for (int k = lane; k < n; k += 64) {
+        const double a = (double)m[k];
+#pragma unroll
+        for (int c = 0; c < NR; c++) acc[c] = fma(a, X[(long long)k * NR + c], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NR; c++)
+        for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o);
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < NR; c++) Y[(long long)row * NR + c] = acc[c];
+}
+
+template <typename TM, int NR>
+__global__ __launch_bounds__(256) void gemv_t_part_inv(const TM *__restrict__ M, int n, const double *__restrict__ X,
+                                                       int kchunk, double *__restrict__ part)
+{
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    if (col >= n) return;
+    const int k0 = blockIdx.y * kchunk, k1 = min(n, k0 + kchunk);
+    double acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; c++) acc[c] = 0.0;
+#pragma unroll 4
+    for (int k = k0; k < k1; k++) {
+        const double a = (double)M[(long long)k * n + col];
+#pragma unroll
+        for (int c = 0; c < NR; c++) acc[c] = fma(a, X[(long long)k * NR + c], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < NR; c++) part[((long long)blockIdx.y * n + col) * NR + c] = acc[c];
+}
+
+__global__ void gemv_t_set(const double *__restrict__ part, int nsplit, long long len, double *__restrict__ Y)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    double s = 0.0;
+    for (int k = 0; k < nsplit; k++) s += part[(long long)k * len + i];
+    Y[i] = s;
+}
+
+template <typename TM, int NR>
+void launch_inverse_product(hipStream_t s, bool trans, const TM *Gi, int npad, const double *X, double *Y, double *part)
+{
+    if (!trans) {
+        gemv_set<TM, NR><<<(npad + 3) / 4, 256, 0, s>>>(Gi, npad, X, Y);
+    } else {
+        const int kchunk = (npad + KSPLIT - 1) / KSPLIT;
+        gemv_t_part_inv<TM, NR><<<dim3((npad + 255) / 256, KSPLIT), 256, 0, s>>>(Gi, npad, X, kchunk, part);
+        HIPCHECK2(hipGetLastError());
+        const long long len = (long long)npad * NR;
+        gemv_t_set<<<(unsigned)((len + 255) / 256), 256, 0, s>>>(part, KSPLIT, len, Y);
+    }
+    HIPCHECK2(hipGetLastError());
+}
+
+template <typename TM>
+void inverse_product(hipStream_t s, bool trans, const TM *Gi, int npad, int nrhs, const double *X, double *Y,
+                     double *part)
+{
+    switch (nrhs) {
+    case 1: launch_inverse_product<TM, 1>(s, trans, Gi, npad, X, Y, part); break;
+    case 2: launch_inverse_product<TM, 2>(s, trans, Gi, npad, X, Y, part); break;
+    case 3: launch_inverse_product<TM, 3>(s, trans, Gi, npad, X, Y, part); break;
+    default: launch_inverse_product<TM, 4>(s, trans, Gi, npad, X, Y, part); break;
+    }
+}
+
+template <typename T> __global__ void set_identity(T *__restrict__ X, int n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)n * n) return;
+    X[i] = (i / n == i % n) ? T(1) : T(0);
+}
+
 template <int NR>
 void launch_residual(hipStream_t s, bool trans, const double *M, long long ld, int npad, const double *X, double *Y,
                      double *part)
@@ -1183,6 +1272,15 @@ void Factor::apply_inverse(int nrhs, bool trans)
     long long h = npad / 4;
     rbt_rows<<<blocks(h * nrhs), 256, 0, s>>>(Y, npad, nrhs, trans ? d_rv : d_ru, R);
     HIPCHECK2(hipGetLastError());
+    if (inv_ready && nrhs <= GEMV_MAX) {  // G^-1 R (G^-T R) by the explicit inverse
+        if (mode == 0)
+            inverse_product<double>(s, trans, (const double *)ginv, npad, nrhs, R, Zbuf, part);
+        else
+            inverse_product<float>(s, trans, (const float *)ginv, npad, nrhs, R, Zbuf, part);
+        rbt_apply_v<<<blocks(h * nrhs), 256, 0, s>>>(Zbuf, npad, nrhs, trans ? d_ru : d_rv, Y);
+        HIPCHECK2(hipGetLastError());
+        return;
+    }
     if (mode == 0) {
         lu_solve_any<double>(s, G64, ld, npad, linv64, uinv64, R, nrhs, trans);
     } else {
@@ -1200,6 +1298,7 @@ void Factor::apply_inverse(int nrhs, bool trans)
 void Factor::apply_inverse_replay(int nrhs, bool trans)
 {
     hipStream_t s = ctx->stream;
+    if (nrhs <= GEMV_MAX && !inv_ready) narrow_applies++;
     if (!graphs_enabled()) {
         apply_inverse(nrhs, trans);
         return;
@@ -1247,6 +1346,10 @@ int Factor::solve(const double *Bin, int nrhs, double *Xout, bool device_io, int
 {
     hipStream_t s = ctx->stream;
     ensure_ws(nrhs);
+    if (nrhs <= GEMV_MAX && !inv_ready) {  // at a solve's start only: a GMRES cycle keeps one preconditioner
+        const int pol = lu_inverse_policy();
+        if (pol != 0 && npad <= LU_INVERSE_MAX && (pol == 1 || narrow_applies >= LU_INVERSE_AFTER)) build_inverse();
+    }
     size_t nn = (size_t)n * nrhs, sz = (size_t)npad * nrhs;
     HIPCHECK2(hipMemsetAsync(B, 0, sizeof(double) * sz, s));
     HIPCHECK2(hipMemcpyAsync(B, Bin, sizeof(double) * nn, device_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
@@ -1475,6 +1578,40 @@ bool Factor::gmres_column(int c, int nrhs, bool trans, double tolf, int max_rest
     return converged;
 }
 
+// G^-1 from the factors: the identity solved as npad right hand sides (the GEMM-based sweeps, 2 npad^3 flops), kept in
+// the factors' precision.  The graphs captured the panel sweeps, so they are captured again.
+void Factor::build_inverse()
+{
+    hipStream_t s = ctx->stream;
+    const long long nn = (long long)npad * npad;
+    if (mode == 0) {
+        double *g = dmalloc<double>((size_t)nn);
+        owned.push_back(g);
+        set_identity<double><<<blocks(nn), 256, 0, s>>>(g, npad);
+        HIPCHECK2(hipGetLastError());
+        lu_solve_any<double>(s, G64, ld, npad, linv64, uinv64, g, npad, false);
+        ginv = g;
+    } else {
+        float *g = dmalloc<float>((size_t)nn);
+        owned.push_back(g);
+        set_identity<float><<<blocks(nn), 256, 0, s>>>(g, npad);
+        HIPCHECK2(hipGetLastError());
+        lu_solve_any<float>(s, G32, ld, npad, linv32, uinv32, g, npad, false);
+        ginv = g;
+    }
+    if (!Zbuf) {
+        Zbuf = dmalloc<double>((size_t)npad * GEMV_MAX);
+        owned.push_back(Zbuf);
+    }
+    HIPCHECK2(hipStreamSynchronize(s));
+    for (InvGraph &g : inv_graph) {
+        if (g.exec) HIPCHECK2(hipGraphExecDestroy(g.exec));
+        g = InvGraph();
+    }
+    inv_ready = true;
+    inverse_builds++;
+}
+
 // dsgesv's fallback: the FP32 factors did not give a converged solve, so M (kept in FP64) is factored again in FP64
 // through the same butterflies, and the factorization stays FP64 from then on.
 void Factor::refactor_fp64()
@@ -1509,6 +1646,13 @@ void Factor::refactor_fp64()
     }
     G32 = nullptr;
     linv32 = uinv32 = nullptr;
+    if (ginv) {  // the FP32 factors' inverse goes with them
+        owned.erase(std::find(owned.begin(), owned.end(), ginv));
+        (void)hipFree(ginv);
+        ginv = nullptr;
+    }
+    inv_ready = false;
+    narrow_applies = 0;
     mode = 0;
     fallbacks++;
 }

@@ -89,7 +89,7 @@ EXPORTS = ["vp_create", "vp_destroy", "vp_last_error", "vp_sync", "vp_assemble",
            "vp_consistency", "vp_blob_velocities_device", "vp_device_alloc", "vp_device_free", "vp_copy_to_device",
            "vp_copy_from_device", "vp_explicit_create", "vp_explicit_rhs", "vp_explicit_info", "vp_explicit_destroy",
            "vp_diag_lu_probe", "vp_freewake_create", "vp_freewake_step", "vp_freewake_counts", "vp_freewake_read", "vp_freewake_destroy",
-           "vp_blob_velocities_tree", "vp_tree_coverage"]
+           "vp_blob_velocities_tree", "vp_tree_coverage", "vp_factor_inverse"]
 
 
 def load():

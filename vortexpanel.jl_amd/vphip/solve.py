@@ -87,6 +87,14 @@ class GalerkinFactorization:
         _check(self.ctx.h, self.ctx.lib.vp_factor_gmres(self.h, C.byref(sv), C.byref(it)))
         return sv.value, it.value
 
+    @property
+    def inverse_info(self) -> tuple[bool, int]:
+        """(explicit inverse of the factors in use, narrow applications counted before it was made): narrow solves
+        apply the inverse by one matrix-vector product once a factorization has served 24 of them"""
+        u, a = C.c_int(), C.c_longlong()
+        _check(self.ctx.h, self.ctx.lib.vp_factor_inverse(self.h, C.byref(u), C.byref(a)))
+        return bool(u.value), a.value
+
     def solve_bordered(self, R):
         """X = M^-1 R for R (n,) or (n, nrhs)"""
         R = np.asarray(R, dtype=np.float64)
