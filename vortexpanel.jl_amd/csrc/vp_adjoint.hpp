@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void adj_far_pairs(AsmParams P, RuleTab rt, co
         if (t < 0) break;
         const TestFar &F = tf[t];
         bool far = false;
-        if (valid && s != t) far = cls_norm(sc - ld3(F.c)) > P.far_ratio * fmax(F.diam, sdiam);
+        if (valid && s != t) far = cls_far(sc - ld3(F.c), P.far_ratio * fmax(F.diam, sdiam));
         // pairs with a closed-form point go to adj_far_exact; here only all-multipole pairs
         bool multi = far;
         if (far) {

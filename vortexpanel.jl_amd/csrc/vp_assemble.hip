@@ -352,7 +352,7 @@ __device__ __forceinline__ void far_tile(FAR_ARGS, int tt, int st, double *R, do
         if (valid && s != t) {
             // pair_rule's far test, src/assembly/assemble.jl:269 (bit-exact: FP64, no contraction)
             V3<double> d = f.c - ld3(F.c);
-            far = cls_norm(d) > P.far_ratio * fmax(F.diam, f.diam);
+            far = cls_far(d, P.far_ratio * fmax(F.diam, f.diam));
         }
         append_pair(valid && !far, t, s, near_list, near_count, near_cap);
         if (enum_only) continue;  // the far blocks are in A already
@@ -485,7 +485,7 @@ __global__ void seg_pairs(AsmParams P, const TestFar *__restrict__ tf, const int
         double L = cls_norm(fb - fa);
         V3<double> mid = fa + fb;
         mid = mk3(mid.x / 2, mid.y / 2, mid.z / 2);
-        far = cls_norm(mid - ld3(F.c)) > P.far_ratio * fmax(F.diam, L);
+        far = cls_far(mid - ld3(F.c), P.far_ratio * fmax(F.diam, L));
     }
     append_pair(in && !far, t, P.nsrc + f, near_list, near_count, near_cap);
     if (enum_only) return;

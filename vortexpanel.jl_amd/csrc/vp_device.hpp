@@ -455,6 +455,23 @@ __device__ inline double cls_norm(V3<double> a)
     return __builtin_sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
 }
 
+// The reference's far test norm(d) > T (assemble.jl:269, explicit.jl:58) without the square root wherever the squared
+// comparison already decides it: s2 = d.d is formed exactly as cls_norm forms it, and with q = T*T,
+// s2 > q(1 + 1e-12) puts the correctly rounded sqrt(s2) above T(1 + 4e-13), s2 < q(1 - 1e-12) puts it below
+// T(1 - 4e-13) (a few units of rounding in s2, q and the bound are 1e-16 each).  In the band between, and when q is
+// not a normal number, the test is the reference's own.  So the result is cls_norm(d) > T, bit for bit.
+__device__ inline bool cls_far(V3<double> d, double T)
+{
+#pragma clang fp contract(off)
+    const double s2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    const double q = T * T;
+    if (q >= 1e-290 && q <= 1e290) {
+        if (s2 > q * (1.0 + 1e-12)) return true;
+        if (s2 < q * (1.0 - 1e-12)) return false;
+    }
+    return __builtin_sqrt(s2) > T;
+}
+
 __device__ inline double cls_dot(V3<double> a, V3<double> b)
 {
 #pragma clang fp contract(off)

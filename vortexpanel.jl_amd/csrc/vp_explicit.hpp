@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void blob_rhs_far(AsmParams P, const TestFar *
         for (int k = 0; k < nk; k++) {
             const V3<double> xb = mk3(Bt[k][0], Bt[k][1], Bt[k][2]);
             const double del = Bt[k][6];
-            if (cls_norm(cls_sub(xb, ld3(F.c))) > P.far_ratio * fmax(F.diam, del)) {
+            if (cls_far(cls_sub(xb, ld3(F.c)), P.far_ratio * fmax(F.diam, del))) {
                 const V3<double> al = mk3(Bt[k][3], Bt[k][4], Bt[k][5]);
                 for (int q = 0; q < 3; q++) u[q] = u[q] + blob_vel(ld3(F.x[q]) - xb, al, del * del);
             } else {

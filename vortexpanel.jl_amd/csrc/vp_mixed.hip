@@ -1276,7 +1276,7 @@ __device__ __forceinline__ bool mx_far_test(const MxArgs &P, int e, int s)
         sc = mk3(m.x / 2.0, m.y / 2.0, m.z / 2.0);
         sd = cls_norm(cls_sub(fb, fa));
     }
-    return cls_norm(cls_sub(sc, tc)) > P.far_ratio * fmax(td, sd);
+    return cls_far(cls_sub(sc, tc), P.far_ratio * fmax(td, sd));
 }
 
 // ------------------------------------------------------------------------------------------------ kernels ---
