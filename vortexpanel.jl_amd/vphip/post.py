@@ -214,6 +214,63 @@ def kutta_joukowski_force(wte, mu_w, vinf):
     return 2 / V.dot(V) * np.cross(V, G)
 
 
+def _te_structure(wte):
+    """trefftz_drag's index structure of the trailing edge lines, built once for many calls: the segments' ends
+    (sa, sb), the vertices (vp), the segments' mean circulation as a matrix on mu_w (S, nseg x nte) and the vertices'
+    shed circulation as a matrix on it (D, nv x nseg)"""
+    sa, sb, vp, S_rows, D_rows = [], [], [], [], []
+    nte = len(wte.x)
+    seg0 = 0
+    for line, closed in zip(wte.lines, wte.closed):
+        line = list(line)
+        m = len(line)
+        nseg = m if closed else m - 1
+        for i in range(nseg):
+            row = np.zeros(nte)
+            row[line[i]] += 0.5
+            row[line[(i + 1) % m]] += 0.5
+            S_rows.append(row)
+            sa.append(line[i])
+            sb.append(line[(i + 1) % m])
+        for i in range(m):
+            vp.append(line[i])
+            D_rows.append((seg0 + i if i < nseg else -1,
+                           seg0 + i - 1 if i > 0 else (seg0 + nseg - 1 if closed else -1)))
+        seg0 += nseg
+    nseg = len(sa)
+    D = np.zeros((len(vp), nseg))
+    for v, (after, before) in enumerate(D_rows):
+        if after >= 0:
+            D[v, after] += 1.0
+        if before >= 0:
+            D[v, before] -= 1.0
+    S = np.array(S_rows).reshape(nseg, nte)
+    return np.array(sa, dtype=np.int64), np.array(sb, dtype=np.int64), np.array(vp, dtype=np.int64), S, D
+
+
+def trefftz_drag_batch(wte, MW, V):
+    """trefftz_drag for many freestreams at once: MW (nte x na) trailing edge doublets, V (na x 3); the same
+    formulas (forces.jl:53-81) vectorised over the angles"""
+    V = np.asarray(V, dtype=np.float64)
+    sa, sb, vp, S, Dm = _te_structure(wte)
+    if len(sa) == 0:
+        return np.zeros(len(V))
+    x = np.asarray(wte.x, dtype=np.float64)
+    G = S @ np.asarray(MW, dtype=np.float64)                      # (nseg, na) mean circulation per segment
+    VG = Dm @ G                                                   # (nv, na) shed circulation per vertex
+    d = V / np.linalg.norm(V, axis=1)[:, None]                    # (na, 3)
+    P = x[None] - np.einsum("ti,ai->at", x, d)[:, :, None] * d[:, None, :]   # (na, nte, 3)
+    A, B = P[:, sa], P[:, sb]
+    c = (A + B) / 2
+    ell = np.linalg.norm(B - A, axis=2)                           # (na, nseg)
+    n = np.cross(d[:, None, :], (B - A) / ell[:, :, None])
+    r = c[:, :, None, :] - P[:, vp][:, None, :, :]                # (na, nseg, nv, 3)
+    w = np.cross(d[:, None, None, :], r) / np.sum(r * r, axis=3)[..., None]
+    v = np.einsum("ak,askj->asj", VG.T / (4 * np.pi), w)
+    Dd = np.sum(G.T * np.einsum("asj,asj->as", n, v) * ell, axis=1)
+    return 2 * Dd / np.einsum("aj,aj->a", V, V)
+
+
 def trefftz_drag(wte, mu_w, vinf):
     """Induced drag in the Trefftz plane from the strips' mean circulation and the trailing vortices at the trailing
     edge nodes (midpoint normalwash on the lifting line), in units of q∞, forces.jl:53-81.  Vectorised over strips."""

@@ -39,23 +39,36 @@ def wing_loads(els, dm, mu, q, vinf, e_lift, area_ref):
 
 
 def wing_loads_batch(els, dm, MU, Q, alphas, area_ref):
-    """wing_loads for many angles at once (MU nslot x na, Q nq x na): the per-element pressure route as one batched
-    contraction over the angles, the trailing-edge loads per angle"""
+    """wing_loads for many angles at once (MU nslot x na, Q nq x na).  The pressure route, restated for all angles:
+    the surface gradient g = sum_k mu_k gradl_k (Post.jl:72-73) is one sparse (3 nt x nslot) matrix times MU, and with
+    u = V - (V.n) n + g, |u|^2 = |V|^2 - 2 (V.n)^2 + (V.n)^2 |n|^2 + 2 (V.g - (V.n)(n.g)) + |g|^2 (Post.jl:82-84) and
+    the force -sum_t cp_t A_t n_t (forces.jl:11-19) are dense products over (triangles x angles); the
+    Kutta-Joukowski circulation is a linear map of mu_w, and the Trefftz drag is vectorised over the angles."""
+    import scipy.sparse as sp
     a = np.radians(np.asarray(alphas, dtype=np.float64))
     V = np.stack([np.cos(a), np.zeros_like(a), np.sin(a)], axis=1)          # (na, 3)
     eL = np.stack([-np.sin(a), np.zeros_like(a), np.cos(a)], axis=1)
-    n = els.n
-    M = np.asarray(MU).T[:, dm.corner_slot]                                  # (na, nt, 3) corner doublets
-    u = (V[:, None, :] - np.einsum("aj,tj->at", V, n)[:, :, None] * n[None]  # gradient_velocity, Post.jl:72-73
-         + np.einsum("atk,tkj->atj", M, els.gradl))
-    cp = 1 - np.einsum("atj,atj->at", u, u) / np.einsum("aj,aj->a", V, V)[:, None]  # pressure_coefficient, :82-84
-    F = -np.einsum("at,tj->aj", cp, els.area[:, None] * n)                 # pressure_force per angle, forces.jl:11-19
+    n, nt, na = els.n, len(els.area), len(a)
+    rows = (3 * np.arange(nt)[:, None, None] + np.arange(3)[None, None, :]).repeat(3, axis=1)   # (nt, 3 corners, 3)
+    cols = np.broadcast_to(np.asarray(dm.corner_slot)[:, :, None], (nt, 3, 3))
+    Gm = sp.csr_matrix((np.asarray(els.gradl).reshape(-1), (rows.reshape(-1), cols.reshape(-1))), shape=(3 * nt, dm.nslot))
+    G = np.asarray(Gm @ np.asarray(MU, dtype=np.float64)).reshape(nt, 3, na)
+    Vn = n @ V.T                                                             # (nt, na)
+    VG = np.einsum("aj,tja->ta", V, G)
+    nG = np.einsum("tj,tja->ta", n, G)
+    VV = np.einsum("aj,aj->a", V, V)
+    uu = VV[None] - 2 * Vn ** 2 + Vn ** 2 * np.einsum("tj,tj->t", n, n)[:, None] + 2 * (VG - Vn * nG) + np.einsum("tja,tja->ta", G, G)
+    cp = 1 - uu / VV[None]
+    F = -(cp.T @ (els.area[:, None] * n))                                    # (na, 3)
     MW = dm.K @ np.asarray(Q)
-    CL_KJ, CDi_T = [], []
-    for k in range(len(a)):
-        CL_KJ.append(float(post.kutta_joukowski_force(dm.wake_te, MW[:, k], V[k]) @ eL[k]) / area_ref)
-        CDi_T.append(post.trefftz_drag(dm.wake_te, MW[:, k], V[k]) / area_ref)
-    return dict(CL=np.einsum("aj,aj->a", F, eL) / area_ref, CL_KJ=np.array(CL_KJ), CDi_T=np.array(CDi_T),
+    x = np.asarray(dm.wake_te.x, dtype=np.float64)
+    W = np.zeros((len(x), 3))
+    for sa_, sb_ in post.te_segments(dm.wake_te):                            # forces.jl:34-39, linear in mu_w
+        W[sa_] += (x[sb_] - x[sa_]) / 2
+        W[sb_] += (x[sb_] - x[sa_]) / 2
+    Fkj = (2 / VV)[:, None] * np.cross(V, MW.T @ W)
+    return dict(CL=np.einsum("aj,aj->a", F, eL) / area_ref, CL_KJ=np.einsum("aj,aj->a", Fkj, eL) / area_ref,
+                CDi_T=post.trefftz_drag_batch(dm.wake_te, MW, V) / area_ref,
                 CDi_p=np.einsum("aj,aj->a", F, V) / area_ref, CY=F[:, 1] / area_ref)
 
 
