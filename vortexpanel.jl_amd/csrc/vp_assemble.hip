@@ -228,6 +228,23 @@ template <typename T> __device__ __forceinline__ FarSrc<T> load_far_src(const El
 // the multipole beyond kernel_far diameters in T, the closed form in FP64 otherwise.  (The FP32 closed form,
 // sv_exact_f32, measured no faster here on the swept wing, 208 against 210 ms: the far pass is not bound by its
 // arithmetic there; so the mixed path keeps these points' FP64 accuracy.)
+// The closed form of source_velocity at x for the far tier's points inside kernel_far diameters (FP64), c the source's
+// centroid
+__device__ __forceinline__ V3<double> far_exact(const Elem *__restrict__ E, V3<double> c, V3<double> x)
+{
+    SrcT<double> s;
+    for (int k = 0; k < 3; k++) {
+        s.v[k] = ld3(E->v[k]) - c;
+        s.nu[k] = ld3(E->nu[k]);
+        s.len[k] = E->len[k];
+    }
+    s.n = ld3(E->n);
+    s.diam = E->diam;
+    s.area = E->area;
+    V3<double> ra = x - ld3(E->v[0]), rb = x - ld3(E->v[1]), rc = x - ld3(E->v[2]);
+    return sv_exact_abs(s, ra, rb, rc, dot(x - c, s.n), 0, nullptr, x);
+}
+
 template <typename T>
 __device__ __forceinline__ V3<double> far_eval(const FarSrc<T> &f, const Elem *__restrict__ E, V3<double> x, double kfar2,
                                                unsigned int &nexact)
@@ -237,30 +254,22 @@ __device__ __forceinline__ V3<double> far_eval(const FarSrc<T> &f, const Elem *_
         V3<T> w = sv_far_mom(f.m, mk3(T(r.x), T(r.y), T(r.z)));
         return mk3((double)w.x, (double)w.y, (double)w.z);
     }
-    SrcT<double> s;
-    for (int k = 0; k < 3; k++) {
-        s.v[k] = ld3(E->v[k]) - f.c;
-        s.nu[k] = ld3(E->nu[k]);
-        s.len[k] = E->len[k];
-    }
-    s.n = ld3(E->n);
-    s.diam = E->diam;
-    s.area = E->area;
-    V3<double> ra = x - ld3(E->v[0]), rb = x - ld3(E->v[1]), rc = x - ld3(E->v[2]);
     nexact++;
-    return sv_exact_abs(s, ra, rb, rc, dot(r, s.n), 0, nullptr, x);
+    return far_exact(E, f.c, x);
 }
 
 // The pair_block of a far pair (assemble.jl:245-258) from its three far-rule points: blk[a][b] += psi_a (w x n).gamma_b,
-// rb[a] -= sigma_S psi_a (w.n), with the weights W psi folded into F.wpsi; EXACT admits closed-form points
+// rb[a] -= sigma_S psi_a (w.n), with the weights W psi folded into F.wpsi; EXACT admits closed-form points, and
+// without it the points in the bit mask skip are left out (they are summed elsewhere)
 template <typename T, bool EXACT>
 __device__ __forceinline__ void far_block(const FarSrc<T> &f, const Elem *__restrict__ E, const TestFar &F, double kfar2,
-                                          double (*blk)[3], double *rb, unsigned int &nexact)
+                                          double (*blk)[3], double *rb, unsigned int &nexact, unsigned int skip = 0)
 {
     const V3<double> n = ld3(F.n);
     // unrolled: the three points are independent (C2 far kernel 17.1 -> 16.3 ms, no spills at two waves per SIMD)
 #pragma unroll
     for (int q = 0; q < 3; q++) {
+        if (!EXACT && (skip >> q & 1u)) continue;
         V3<double> w;
         if constexpr (EXACT) {
             w = far_eval<T>(f, E, ld3(F.x[q]), kfar2, nexact);
@@ -322,17 +331,57 @@ __device__ __forceinline__ void far_flush(const AsmParams &P, const double *R, c
     P, tf, el, sigma, tt_elems, tt_loc, tt_rows, tt_nrow, ts_elems, ts_loc, ts_cols, ts_ncol, A, bvec, near_list,      \
         near_count, near_cap, far_count, tiers, st0, enum_only
 
-// One tile (test tile tt, source tile st) of the far tier, by the whole workgroup, with its LDS sums R and rbacc
+// The closed-form points of a tile's far pairs (DEFER), after the tile's multipole points, one point per thread:
+// entries (i << 8) | (q << 6) | lane of the workgroup's queue dq (test triangle i of the tile, far-rule point q, source
+// lane), each point's terms of far_block summed into the tile's LDS sums.  Only the source's element record is read
+// here (no multipole moments), so that the closed form's registers are not live beside the multipole pass's: the FP32
+// far kernel runs at three waves per SIMD instead of two.
 template <typename T>
+__device__ __forceinline__ void far_deferred(const TestFar *__restrict__ tf, const Elem *__restrict__ el,
+                                             const double *__restrict__ sigma, int nt, const int *__restrict__ tt_elems,
+                                             const uint8_t *__restrict__ tt_loc, const int *__restrict__ ts_elems,
+                                             const uint8_t *__restrict__ ts_loc, int tt, int st,
+                                             const uint16_t *__restrict__ dq, unsigned int nq, double *R, double *rbacc)
+{
+    for (unsigned int j = threadIdx.x; j < nq; j += blockDim.x) {
+        const unsigned int e = dq[j];
+        const int i = (int)(e >> 8), q = (int)(e >> 6 & 3u), ln = (int)(e & 63u);
+        const int t = tt_elems[tt * TT_MAX + i], s = ts_elems[st * TS + ln];
+        const Elem *E = el + s;
+        const TestFar &F = tf[t];
+        const V3<double> n = ld3(F.n);
+        // the same conversions as load_far_src and far_block
+        const V3<double> w = far_exact(E, ld3(E->c), ld3(F.x[q])), vv = cross(w, n);
+        double c[3];
+        for (int b = 0; b < 3; b++)
+            c[b] = dot(vv, mk3((double)T(E->g[b][0]), (double)T(E->g[b][1]), (double)T(E->g[b][2])));
+        const double wn = (double)T(s < nt ? sigma[s] : 0.0) * dot(w, n);
+        int uloc[3], vl[3];
+        for (int k = 0; k < 3; k++) {
+            uloc[k] = ts_loc[(st * TS + ln) * 3 + k];
+            vl[k] = tt_loc[(tt * TT_MAX + i) * 3 + k];
+        }
+        for (int a = 0; a < 3; a++) {
+            const double psi = F.wpsi[q][a];
+            for (int b = 0; b < 3; b++) atomicAdd(&R[uloc[b] * RST + vl[a]], psi * c[b]);
+            atomicAdd(&rbacc[vl[a]], -(wn * psi));
+        }
+    }
+}
+
+// One tile (test tile tt, source tile st) of the far tier, by the whole workgroup, with its LDS sums R and rbacc.
+// DEFER: the far pairs' closed-form points go to the workgroup's queue dq (3 TT_MAX TS entries, count nq in LDS) and
+// are evaluated after the tile's multipole points; otherwise every far point is evaluated in place.
+template <typename T, bool DEFER>
 __device__ __forceinline__ void far_tile(FAR_ARGS, int tt, int st, double *R, double *rbacc, unsigned int &nfar,
-                                         unsigned int &nexw)
+                                         unsigned int &nexw, unsigned int &nq, uint16_t *__restrict__ dq)
 {
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
     if (!enum_only) {
         for (int i = threadIdx.x; i < UMAX * RST; i += blockDim.x) R[i] = 0.0;
         if (threadIdx.x < VMAX) rbacc[threadIdx.x] = 0.0;
     }
-    if (threadIdx.x == 0) nfar = nexw = 0;
+    if (threadIdx.x == 0) nfar = nexw = nq = 0;
     const int s = ts_elems[st * TS + lane];
     const bool valid = s >= 0;
     FarSrc<T> f;
@@ -359,8 +408,20 @@ __device__ __forceinline__ void far_tile(FAR_ARGS, int tt, int st, double *R, do
         if (tiers && valid && far) tiers[(size_t)t * P.nsrc + s] = T_FAR;
         if (far) {
             myfar++;
+            unsigned int skip = 0;  // DEFER: the pair's closed-form points, queued
+            if constexpr (DEFER) {
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    const V3<double> r = ld3(F.x[q]) - f.c;
+                    if (!(cls_dot(r, r) > kfar2)) {
+                        skip |= 1u << q;
+                        dq[atomicAdd(&nq, 1u)] = (uint16_t)((i << 8) | (q << 6) | lane);
+                    }
+                }
+                myex += __builtin_popcount(skip);
+            }
             double blk[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, rb[3] = {0, 0, 0};
-            far_block<T, true>(f, el + s, F, kfar2, blk, rb, myex);
+            far_block<T, !DEFER>(f, el + s, F, kfar2, blk, rb, myex, skip);
             const int vl[3] = {tt_loc[(tt * TT_MAX + i) * 3], tt_loc[(tt * TT_MAX + i) * 3 + 1],
                                tt_loc[(tt * TT_MAX + i) * 3 + 2]};
 #pragma unroll
@@ -372,6 +433,10 @@ __device__ __forceinline__ void far_tile(FAR_ARGS, int tt, int st, double *R, do
         }
     }
     if (enum_only) return;
+    if constexpr (DEFER) {
+        __syncthreads();  // the queue's entries and count
+        far_deferred<T>(tf, el, sigma, P.nt, tt_elems, tt_loc, ts_elems, ts_loc, tt, st, dq, nq, R, rbacc);
+    }
     if (myfar) atomicAdd(&nfar, myfar);
     if (myex) atomicAdd(&nexw, myex);
     __syncthreads();
@@ -386,27 +451,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 {
     __shared__ double R[UMAX * RST];
     __shared__ double rbacc[VMAX];
-    __shared__ unsigned int nfar, nexw;
-    far_tile<T>(FAR_PASS, blockIdx.x, blockIdx.y + st0, R, rbacc, nfar, nexw);
+    __shared__ unsigned int nfar, nexw, nq;
+    far_tile<T, false>(FAR_PASS, blockIdx.x, blockIdx.y + st0, R, rbacc, nfar, nexw, nq, nullptr);
 }
 
+// far_pairs_queue<T>: whether it defers the closed-form points, and its waves per SIMD (and workgroups per CU).  FP32
+// multipoles defer and run at three (C2 mixed assembly 32.3 -> 31.5 ms); FP64 evaluates in place at two (deferred, at
+// three it spills and took 22.8 against 14.3 ms, at two 15.3)
+#define FAR_Q_DEFER(T) (sizeof(T) == 4)
+#define FAR_Q_WAVES(T) (FAR_Q_DEFER(T) ? 3 : 2)
 // The far tier as one persistent work queue: a grid sized to the chip's resident workgroups takes tiles in order
 // (test tile fastest, as the grid form) from one counter until the ntt x nst tiles are done; every workgroup leaves
-// when the counter passes the last tile.  Tiles of bodies of any size share the queue.
+// when the counter passes the last tile.  Tiles of bodies of any size share the queue.  Each workgroup defers a
+// FP32 tile's closed-form points to its own slice of fq (3 TT_MAX TS entries), which lets that kernel run at three waves
+// per SIMD (three workgroups per CU).
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void far_pairs_queue(
-    FAR_ARGS, int ntt, int nst, unsigned int *__restrict__ queue)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FAR_Q_WAVES(T), FAR_Q_WAVES(T)))) void far_pairs_queue(
+    FAR_ARGS, int ntt, int nst, unsigned int *__restrict__ queue, uint16_t *__restrict__ fq)
 {
     __shared__ double R[UMAX * RST];
     __shared__ double rbacc[VMAX];
-    __shared__ unsigned int nfar, nexw, tile;
+    __shared__ unsigned int nfar, nexw, nq, tile;
+    uint16_t *const dq = fq + (size_t)blockIdx.x * (3 * TT_MAX * TS);
     const unsigned int ntiles = (unsigned int)ntt * (unsigned int)nst;
     for (;;) {
         if (threadIdx.x == 0) tile = atomicAdd(queue, 1u);
         __syncthreads();
         const unsigned int k = tile;
         if (k >= ntiles) break;
-        far_tile<T>(FAR_PASS, (int)(k % (unsigned int)ntt), (int)(k / (unsigned int)ntt) + st0, R, rbacc, nfar, nexw);
+        far_tile<T, FAR_Q_DEFER(T)>(FAR_PASS, (int)(k % (unsigned int)ntt), (int)(k / (unsigned int)ntt) + st0, R, rbacc,
+                                    nfar, nexw, nq, dq);
         __syncthreads();  // the flush has read R and the counters before the next tile clears them
     }
 }
@@ -1972,7 +2046,7 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
         int ncu = 0;
         if (!(q && q[0] == '0') &&
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess)
-            far_queue_wgs = 2LL * std::max(ncu, 1);
+            far_queue_wgs = std::max(ncu, 1);  // x FAR_Q_WAVES(T) at launch
     }
     if (o.test < 0 || o.test > 2) throw std::invalid_argument("test must be hat (0), onering (1) or dual (2)");
     if (o.precision != 0 && o.precision != 1) throw std::invalid_argument("precision must be 0 (fp64) or 1 (mixed)");
@@ -2151,6 +2225,11 @@ void Plan::build(Ctx *ctx_, const vp_surface &g, const vp_wake *wk, const vp_ass
     d_leaves = dalloc<Leaf>(leaf_cap);
     d_groups = dalloc<int4>(group_cap);
     owned.insert(owned.end(), {d_nonfar, d_touchp, d_near, d_nearm, d_simple, d_leaves, d_groups});
+    if (far_queue_wgs > 0 && opts.precision == 1) {  // the FP32 far queue's deferred closed-form points, a slice per workgroup
+        static_assert(FAR_Q_DEFER(float) && !FAR_Q_DEFER(double), "d_fq is made for the FP32 far queue only");
+        d_fq = dalloc<uint16_t>((size_t)far_queue_wgs * FAR_Q_WAVES(float) * 3 * TT_MAX * TS);
+        owned.push_back(d_fq);
+    }
     for (int k = 0; k < 6; k++) HIPCHECK(hipEventCreate(&ev[k]));
     for (int k = 0; k < TK_COUNT; k++)
         for (int j = 0; j < 2; j++) HIPCHECK(hipEventCreate(&tev[k][j]));
@@ -2210,17 +2289,18 @@ void Plan::assemble(const double *sigma, const double *vinf, bool host_sigma)
     auto far_pass = [&](bool enum_only) {
         if (nts - st0 > 0 && far_queue_wgs > 0) {  // one persistent work queue over every tile
             HIPCHECK(hipMemsetAsync(d_count + 7, 0, sizeof(unsigned int), st));
-            const unsigned int nwg = (unsigned int)std::min<long long>((long long)ntt * (nts - st0), far_queue_wgs);
+            const long long per_cu = opts.precision == 1 ? FAR_Q_WAVES(float) : FAR_Q_WAVES(double);
+            const unsigned int nwg = (unsigned int)std::min<long long>((long long)ntt * (nts - st0), per_cu * far_queue_wgs);
             if (opts.precision == 1)
                 far_pairs_queue<float><<<nwg, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse,
                                                             d_tsl, d_tsc, d_tsn, A, d_b, d_nonfar, d_count, nonfar_cap,
                                                             d_stats + 5, d_tiers, st0, enum_only, ntt, nts - st0,
-                                                            d_count + 7);
+                                                            d_count + 7, d_fq);
             else
                 far_pairs_queue<double><<<nwg, 256, 0, st>>>(P, d_tf, d_el, d_sigma, d_tte, d_ttl, d_ttr, d_ttn, d_tse,
                                                              d_tsl, d_tsc, d_tsn, A, d_b, d_nonfar, d_count, nonfar_cap,
                                                              d_stats + 5, d_tiers, st0, enum_only, ntt, nts - st0,
-                                                             d_count + 7);
+                                                             d_count + 7, d_fq);
             HIPCHECK(hipGetLastError());
         } else if (nts - st0 > 0) {
             if (opts.precision == 1)

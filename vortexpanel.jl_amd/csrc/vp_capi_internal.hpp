@@ -114,7 +114,8 @@ struct Plan {
     bool lists_sized = false;        // the non-far lists sized by an upper bound (the first assembly)
     bool nonfar_cap_forced = false;  // a first budget set by VP_NONFAR_CAP (no sizing pass)
     bool wake_only = false;  // assemble only the body test triangles' pairs with the wake's sources
-    long long far_queue_wgs = 0;  // > 0: the far tier as a persistent work queue of this many workgroups (0: a grid)
+    long long far_queue_wgs = 0;  // > 0: the far tier as a persistent work queue, this many CUs' workgroups (0: a grid)
+    uint16_t *d_fq = nullptr;     // the far queue's deferred closed-form points, 3 TT_MAX TS per workgroup
     int nts_body = 0;        // source tiles of the body (the wake's follow)
     int2 *d_bnear = nullptr; // near (test triangle, particle) pairs of blob_rhs
     unsigned int bnear_cap = 0;
