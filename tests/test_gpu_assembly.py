@@ -56,6 +56,18 @@ def test_abused_ellipsoid_contact_tiers():
     _compare(g, dm, np.array([np.cos(np.radians(5)), 0, np.sin(np.radians(5))]))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_every_far_point_closed_form(precision):
+    """kernel_far = 1e300 puts every far-rule point on source_velocity's closed form: the mixed path's far kernel then
+    queues every far point of every tile (its deferral queue full on the all-far tiles) and evaluates them after its
+    multipole pass.  Against the oracle with the same option: the FP64 bar, or the mixed path's (its FP32 near
+    kernels, 4e-7 of the largest entry on the full-size ellipsoid)."""
+    g = mesh.abused_ellipsoid(nth=24, nph=48, seed=7)
+    dm = dofmap.build_dofmap(g)
+    _compare(g, dm, VG, api.AssemblyOptions(precision=precision, kernel_far=1e300),
+             tol=1e-12 if precision == "fp64" else 2e-6)
+
+
 def test_wing_with_wake():
     g = mesh.naca_wing(mesh.WingSpec(code="0012", n_around=16, n_span=6, aspect=4.0))
     dm = dofmap.build_dofmap(g)
